@@ -1,0 +1,126 @@
+/*
+ * temfetd.h -- C ABI of the MI355X FETD forward time-stepping library
+ * (libtemfetd.so).  All floating point is FP64; all index arrays use the
+ * reference's 1-based vertex/edge/tet numbering; 2-D arrays are row-major
+ * [item][component] (the Fortran caller passes the transposes it already
+ * holds).
+ *
+ * Every function returns 0 on success and a negative code on error;
+ * tf_last_error() gives the message.  A context is single-threaded and not
+ * re-entrant (like the reference's module state); distinct contexts are
+ * independent.  The context owns all device memory.
+ *
+ * The reference's seams that these entry points replace:
+ *   model_mesh        src/FETD_FWD.f90:112-282    -> tf_set_mesh, tf_get_topology
+ *   model_ctl         src/FETD_FWD.f90:284-586    -> tf_set_materials, tf_set_source,
+ *                                                    tf_set_receivers, tf_set_schedule
+ *   mesh_process      src/FETD_FWD.f90:588-732    -> tf_assemble, tf_get_matrices
+ *   mkl_add_3 +
+ *   pardiso_direct_1  src/FETD_FWD.f90:2006-2039,
+ *                     src/FETD_UTILITY.f90:879-936 -> tf_factor, tf_factor_csr
+ *   pardiso_direct_2  src/FETD_UTILITY.f90:938-957 -> tf_solve
+ *   pardiso_direct_3  src/FETD_UTILITY.f90:959-975 -> tf_release
+ *   time_solver /
+ *   adap_euler        src/FETD_FWD.f90:1174-1464  -> tf_run
+ *   solution_output   src/FETD_FWD.f90:1513-1623  -> rows returned by tf_run
+ */
+#ifndef TEMFETD_H
+#define TEMFETD_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tf_ctx tf_ctx;
+
+/* status codes */
+#define TF_OK 0
+#define TF_ERR_ARG (-1)
+#define TF_ERR_STATE (-2)
+#define TF_ERR_HIP (-3)
+#define TF_ERR_NUMERIC (-4)
+#define TF_ERR_INPUT (-5)
+
+/* Create a context on HIP device `device`. */
+int tf_create(tf_ctx **ctx, int device);
+void tf_destroy(tf_ctx *ctx);
+const char *tf_last_error(const tf_ctx *ctx);
+
+/* model_mesh (src/FETD_FWD.f90:112-282): vertex coordinates [nN][3] and
+ * tets [nT][4] (1-based vertex ids).  Builds the global edge numbering
+ * (lexicographic rank of (min,max) vertex pairs), tet_edg/tet_drc, edge
+ * lengths and the upper CSR pattern shared by all system matrices. */
+int tf_set_mesh(tf_ctx *ctx, int nN, const double *nod_cor, int nT, const int *tet_vex);
+int tf_get_sizes(const tf_ctx *ctx, int *nE, int *nnzU);
+/* Copies (any pointer may be NULL): edg_vex [nE][2], tet_edg [nT][6],
+ * tet_drc [nT][6], ia [nE+1], ja [nnzU], all 1-based like the reference. */
+int tf_get_topology(const tf_ctx *ctx, int *edg_vex, int *tet_edg, int *tet_drc, int *ia, int *ja);
+
+/* model_ctl (src/FETD_FWD.f90:347-364): per-tet conductivity and 1/mu. */
+int tf_set_materials(tf_ctx *ctx, const double *sigma, const double *inv_mu);
+/* Source wire (src/FETD_FWD.f90:418-448, sys_right :1466-1511): the nseg
+ * segments [nseg][2] (1-based vertex ids) in file order; signs are +1 for all
+ * segments but the last (-1).  source_type 1, pulse_type 1 (step-on),
+ * 2 (impulse) or 3 (step-off).  Call repeatedly with source index isrc =
+ * 0..nsrc-1 to define nsrc sources that share one waveform; nsrc > 1 runs as
+ * a multi-RHS block solve. */
+int tf_set_source(tf_ctx *ctx, int isrc, int nseg, const int *seg_vex, int source_type,
+                  int pulse_type, double pulse_width, double current);
+int tf_set_num_sources(tf_ctx *ctx, int nsrc);
+/* Receivers (src/FETD_FWD.f90:500-520, p_in_tet :1009-1049): [nR][3]
+ * coordinates; rec_tet_out (nullable) receives the 1-based containing tet. */
+int tf_set_receivers(tf_ctx *ctx, int nR, const double *xyz, double rec_charac, int *rec_tet_out);
+/* Controller settings (src/FETD_FWD.f90:418-470): sp_division, sp_double,
+ * sp_dblesize, time_maximum. */
+int tf_set_schedule(tf_ctx *ctx, int sp_division, int sp_double, int sp_dblesize,
+                    double time_maximum);
+
+/* mesh_process (src/FETD_FWD.f90:588-732): element matrices on the GPU,
+ * gathered into the CSR values in the reference's COO order; receiver
+ * interpolation vectors. */
+int tf_assemble(tf_ctx *ctx);
+/* Upper-CSR values [nnzU] of sigma-mass and curl-curl (nullable). */
+int tf_get_matrices(const tf_ctx *ctx, double *mass_sigma, double *curl);
+/* Receiver vectors n_arr, d_arr as [nR][6][3] (nullable). */
+int tf_get_receiver_vectors(const tf_ctx *ctx, double *n_arr, double *d_arr);
+
+/* Symbolic analysis, once per mesh pattern (the reference redoes it in every
+ * phase-12 call).  Optional: tf_factor runs it on first use.  leaf_size <= 0
+ * keeps the default. */
+int tf_analyze(tf_ctx *ctx, int leaf_size);
+/* Factor statistics: nnz(L) incl. diagonal, dense-front flops, supernodes. */
+int tf_factor_stats(const tf_ctx *ctx, int64_t *nnzL, double *flops, int *nsuper, int *nlevels);
+
+/* mkl_add_3 + pardiso_direct_1: P = 3*Msigma + 2*dt*K formed on the GPU and
+ * factored as LDL^T into slot 0 or 1. */
+int tf_factor(tf_ctx *ctx, int slot, double dt);
+/* pardiso_direct_1 with caller values: upper CSR (1-based) whose pattern is
+ * the assembled one (ia/ja are checked against it). */
+int tf_factor_csr(tf_ctx *ctx, int slot, int n, const int *ia, const int *ja, const double *va);
+/* pardiso_direct_2: x = P^-1 b for nrhs right-hand sides, host buffers in
+ * edge order, column-major [nrhs][n]. */
+int tf_solve(tf_ctx *ctx, int slot, int nrhs, const double *b, double *x);
+/* pardiso_direct_3 */
+int tf_release(tf_ctx *ctx, int slot);
+
+/* time_solver/adap_euler: runs the whole BDF2 loop on the GPU and returns
+ * the receiver rows solution_output would write: step, t, dt, and
+ * fields [row][src][rec][9] (ex ey ez dBx/dt dBy/dt dBz/dt bx by bz).
+ * *nrows receives the number of rows; rows beyond max_rows are counted but
+ * not stored.  max_steps > 0 caps the loop (bounded runs). */
+int tf_run(tf_ctx *ctx, int max_steps, int max_rows, int *nrows, int *row_step, double *row_t,
+           double *row_dt, double *fields);
+/* Replay a fixed schedule instead of the error-checked controller:
+ * nsteps rows of dt per step (changes of dt refactor).  nsteps = 0 restores
+ * the controller. */
+int tf_set_replay(tf_ctx *ctx, int nsteps, const double *dt);
+/* Doubling-check log of the last tf_run: step, err, keep flag. */
+int tf_get_checks(const tf_ctx *ctx, int max, int *n, int *step, double *err, int *keep);
+/* Timers of the last tf_run in seconds: assemble, factor, solve, spmv+rhs,
+ * readout, loop wall, analyze, and the number of factorisations. */
+int tf_get_timers(const tf_ctx *ctx, double *t8);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

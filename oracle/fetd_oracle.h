@@ -1,0 +1,90 @@
+/*
+ * fetd_oracle.h -- CPU restatement of TEMF3DT's FETD forward path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation in temf3dt_amd/: only tests/, __graft_entry__.smoke() and
+ * the cpu_baseline leg of bench.py may load it.  The product never links or
+ * calls it.
+ *
+ * Every stage follows the reference Fortran (src/FETD_FWD.f90,
+ * src/FETD_UTILITY.f90); each function in fetd_oracle.c cites the lines it
+ * restates.  The sparse factor/solve is Intel MKL PARDISO (the library the
+ * reference calls through cluster_sparse_solver), loaded at run time from
+ * libmkl_rt.so with the reference's settings (iparm all zero = defaults,
+ * mtype 2 with a -2 fallback, src/FETD_UTILITY.f90:879-936).
+ *
+ * Index conventions: vertex, edge and tet ids are 1-based as in the
+ * reference files; arrays are row-major [item][component].
+ */
+#ifndef FETD_ORACLE_H
+#define FETD_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fo_model fo_model;
+
+/* Load ROOT.mesh and ROOT.cntl (src/FETD_FWD.f90:112-586).  Returns NULL on
+ * error; fo_error() gives the message. */
+fo_model *fo_load(const char *root, int verbose);
+/* Same, from explicit file names (cntl may be NULL to skip control parsing). */
+fo_model *fo_load_files(const char *mesh_path, const char *cntl_path, int verbose);
+const char *fo_error(void);
+void fo_free(fo_model *m);
+
+/* Element matrices / global CSR (src/FETD_FWD.f90:588-732). */
+int fo_assemble(fo_model *m);
+
+/* Time loop (src/FETD_FWD.f90:1174-1464).  field paths may be NULL.
+ * max_steps > 0 stops after that many loop iterations (bounded sample). */
+int fo_run(fo_model *m, const char *field_path, const char *field2_path, int max_steps);
+
+/* ---- accessors for checks ---- */
+void fo_counts(const fo_model *m, int *nN, int *nT, int *nE, int *nnzU, int *nR, int *nseg);
+/* copies out (any pointer may be NULL) */
+void fo_get_topology(const fo_model *m, int *edg_vex, int *tet_edg, int *tet_drc, double *edg_len);
+void fo_get_geometry(const fo_model *m, double *a, double *b, double *c, double *d, double *vol, double *tet_len);
+void fo_get_csr(const fo_model *m, int *ia, int *ja, double *mass_sigma, double *curl, double *mass_eps);
+void fo_get_receivers(const fo_model *m, int *rec_tet, double *n_arr, double *d_arr);
+void fo_get_source(const fo_model *m, int *src_edg, int *src_drc);
+void fo_get_material(const fo_model *m, double *sigma, double *inv_mu);
+/* control values: sp_division, sp_double, sp_dblesize, pulse_type, source_type */
+void fo_get_control(const fo_model *m, int *ictl, double *dctl);
+void fo_set_control(fo_model *m, const int *ictl, const double *dctl);
+/* override per-tet conductivity (for perturbation tests) */
+void fo_set_sigma(fo_model *m, const double *sigma);
+
+/* per output row: step, t, dt, and R*9 fields (ex ey ez dx dy dz bx by bz) */
+int fo_num_rows(const fo_model *m);
+void fo_get_rows(const fo_model *m, int *step, double *t, double *dt, double *fields);
+/* doubling-check log: step, err, decision (1 keep / 0 change) */
+int fo_num_checks(const fo_model *m);
+void fo_get_checks(const fo_model *m, int *step, double *err, int *keep);
+/* timers: assemble, factorisation, stepping (solve), spmv+rhs, readout, loop wall */
+void fo_get_timers(const fo_model *m, double *t6);
+int fo_num_factorizations(const fo_model *m);
+long long fo_factor_nnz(const fo_model *m);
+
+/* ---- pointwise kernels (for known-answer tests) ---- */
+/* cofactors of one tet: x[4],y[4],z[4] -> a,b,c,d[4], vol (signed) */
+void fo_tet_cofactors(const double *x, const double *y, const double *z,
+                      double *a, double *b, double *c, double *d, double *vol);
+/* 6x6 mass (scalar s) and curl-curl (scalar s) matrices of one tet, row-major */
+void fo_tet_matrices(const double *x, const double *y, const double *z,
+                     double s_mass, double s_curl, double *F, double *K);
+/* 4x4 determinant by LU with partial pivoting, column-major input */
+double fo_det_lu(const double *a_colmajor, int n);
+/* source time function d I/dt (src/FETD_UTILITY.f90:1045-1078) */
+double fo_source_didt(int source_type, int pulse_type, double t, double t0, double cur);
+
+/* ---- solver backend (MKL PARDISO via dlopen) ---- */
+int fo_pardiso_available(void);
+void fo_set_threads(int n);
+/* Solve the upper-CSR (1-based) symmetric system once; returns PARDISO error. */
+int fo_pardiso_solve_once(int n, const int *ia, const int *ja, const double *va,
+                          const double *b, double *x, int nrhs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1059 @@
+// api.cpp -- libtemfetd C ABI: context, device memory, execution plan and the
+// BDF2 time loop with the reference's doubling controller.
+//
+// Reference call structure replaced here (yxddyxzh2016/TEMF3DT):
+//   mesh_process   src/FETD_FWD.f90:588-732    tf_assemble
+//   adap_euler     src/FETD_FWD.f90:1229-1464  tf_run (run_loop below)
+//   sys_right      src/FETD_FWD.f90:1466-1511  source term in rhs()
+//   solution_output src/FETD_FWD.f90:1513-1623 readout rows
+//   pardiso_direct_1/2/3 src/FETD_UTILITY.f90:879-975  tf_factor/tf_solve/tf_release
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/temfetd.h"
+#include "tf_device.h"
+#include "tf_internal.h"
+
+namespace tf {
+// assembly.hip
+void launch_element(int nT, const double *xyz, const int *tet_vex, const int *tet_drc, const uint8_t *orient,
+                    const double *sigma, const double *inv_mu, double *vm, double *vk, hipStream_t st);
+void launch_gather(int64_t nnz, const int64_t *cptr, const int *ctr, const double *vm, const double *vk,
+                   double *M, double *K, hipStream_t st);
+void launch_recvec(int nR, const double *xyz, const int *tet_vex, const int *rec_tet, const double *rec_xyz,
+                   double *n_arr, double *d_arr, hipStream_t st);
+// factor.hip
+void launch_fill0(double *p, int64_t n, hipStream_t st);
+void launch_scatter_p(int64_t nnz, const int64_t *amap, const double *M, const double *K, double cdt, double *L,
+                      hipStream_t st);
+void launch_scatter_vals(int64_t nnz, const int64_t *amap, const double *va, double *L, hipStream_t st);
+void launch_extend_add(double *L, double *U, const FrontDesc *fd, const int *parent, const int *list,
+                       const int64_t *pref, int cnt, int64_t nwg, const int *rel, hipStream_t st);
+void launch_factor_fused(double *L, double *U, const FrontDesc *fd, const int *list, int cnt, double *Dn, int *flag,
+                         hipStream_t st);
+void launch_factor_diag(double *L, const FrontDesc *fd, const int *list, int cnt, int p0, double *Dn, int *flag,
+                        hipStream_t st);
+void launch_factor_trsm(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
+                        int p0, hipStream_t st);
+void launch_factor_update(double *L, double *U, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
+                          int64_t nwg, int p0, const double *Dn, hipStream_t st);
+// solve.hip
+void launch_fwd_fused(const double *L, const FrontDesc *fd, const int *list, int cnt, const double *b,
+                      const int *perm, const int *chp, const int *ch, const int *rel, double *V, hipStream_t st);
+void launch_fwd_gather(const FrontDesc *fd, const int *list, int cnt, const double *b, const int *perm,
+                       const int *chp, const int *ch, const int *rel, double *V, hipStream_t st);
+void launch_fwd_diag(const double *L, const FrontDesc *fd, const int *list, int cnt, int p0, double *V,
+                     hipStream_t st);
+void launch_fwd_update(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
+                       int64_t nwg, int p0, double *V, hipStream_t st);
+void launch_bwd_fused(const double *L, const FrontDesc *fd, const int *list, int cnt, const double *V,
+                      const double *Dn, const int *sidx, double *xw, hipStream_t st);
+void launch_bwd_partial(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
+                        int64_t nwg, int p0, const double *xw, const int *sidx, double *partial, hipStream_t st);
+void launch_bwd_diag(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int p0,
+                     const double *V, const double *Dn, const double *partial, double *xw, hipStream_t st);
+void launch_unpermute(int n, const int *perm, const double *xw, double *x, hipStream_t st);
+// step.hip
+void launch_spmv_rhs(int n, const int *ia, const int *ja, const double *val, const double *e2, const double *e3,
+                     double *r, hipStream_t st);
+void launch_source(int ns, const int *edge, const double *w, double didt, double cdt, double *r, hipStream_t st);
+void launch_readout(int nR, const int *rec_edge, const double *rec_drc, const double *n_arr, const double *d_arr,
+                    const double *e, double *out, hipStream_t st);
+void launch_diffnorm(int64_t n, const double *a, const double *b, double *part, double *out, hipStream_t st);
+void launch_gather_vals(int64_t n, const int64_t *map, const double *src, double *dst, hipStream_t st);
+}  // namespace tf
+
+namespace {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIPCHK(x)                                                                                          \
+    do {                                                                                                   \
+        hipError_t e_ = (x);                                                                               \
+        if (e_ != hipSuccess)                                                                              \
+            throw HipError(std::string("HIP error ") + hipGetErrorString(e_) + " at " + #x);               \
+    } while (0)
+
+template <class T>
+T *dalloc(size_t n) {
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc(&p, n * sizeof(T)));
+    return (T *)p;
+}
+template <class T>
+T *dupload(const std::vector<T> &v) {
+    T *p = dalloc<T>(v.size());
+    if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+}
+template <class T>
+void dfree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+enum LaunchKind {
+    L_EXTADD, L_FFUSED, L_FDIAG, L_FTRSM, L_FUPD,
+    L_SFUSED, L_SGATHER, L_SDIAG, L_SUPD,
+    L_BFUSED, L_BPART, L_BDIAG
+};
+struct Launch {
+    int kind, p0, off, cnt, poff;
+    int64_t nwg;
+};
+
+// Execution plan over the front tree: which fronts each launch touches.
+struct Plan {
+    std::vector<Launch> factor, fwd, bwd;
+    std::vector<int> list;
+    std::vector<int64_t> pref;
+    int64_t partial_max = 0;
+};
+
+static const double PI = 3.141592653589793;
+
+// source_mod (src/FETD_UTILITY.f90:1030-1166): d I / dt of the waveform
+double source_didt(int source_type, int pulse_type, double t, double t0, double cur) {
+    if (source_type == 1 && pulse_type == 1) {
+        double u = t - t0 / 2.0;
+        return (1.0 / std::sqrt(PI) * std::exp(1.0 / (t0 * t0) * (u * u) * (-25.0)) * 5.0) / t0 * cur;
+    }
+    if (source_type == 1 && pulse_type == 2) {
+        double w = t * 5.436563656918091 - t0 * 2.718281828459046;
+        return -1.0 / (t0 * t0) * std::exp(-1.0 / (t0 * t0) * (w * w)) *
+               (t * 59.11244879144521 - t0 * 29.55622439572261) * cur;
+    }
+    if (source_type == 1 && pulse_type == 3) {
+        double u = t - t0 / 2.0;
+        return -((1.0 / std::sqrt(PI) * std::exp(1.0 / (t0 * t0) * (u * u) * (-25.0)) * 5.0) / t0) * cur;
+    }
+    return NAN;
+}
+
+}  // namespace
+
+struct tf_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    std::string err;
+    Topology topo;
+    bool have_mesh = false, have_mat = false, assembled = false, analyzed = false;
+    std::vector<double> sigma, inv_mu;
+    // source
+    int nseg = 0;
+    std::vector<int> src_edge;
+    std::vector<double> src_w;
+    int stype = 1, ptype = 1;
+    double pulse_width = 0.0, current = 1.0;
+    bool have_src = false;
+    // receivers
+    int nR = 0;
+    std::vector<double> rec_xyz;
+    std::vector<int> rec_tet;
+    double rec_charac = 10.0;
+    // schedule
+    int sp_division = 0, sp_double = 0, sp_dblesize = 2;
+    double time_max = 0.0;
+    // device: mesh and assembly
+    double *d_xyz = nullptr, *d_sigma = nullptr, *d_invmu = nullptr, *d_vm = nullptr, *d_vk = nullptr;
+    int *d_tet_vex = nullptr, *d_tet_drc = nullptr, *d_ctr = nullptr;
+    uint8_t *d_orient = nullptr;
+    int64_t *d_cptr = nullptr;
+    double *d_M = nullptr, *d_K = nullptr;
+    // full symmetric Msigma for the right-hand side
+    int *d_fia = nullptr, *d_fja = nullptr;
+    int64_t *d_fmap = nullptr;
+    double *d_fval = nullptr;
+    // receivers / source on device
+    double *d_rec_xyz = nullptr, *d_narr = nullptr, *d_darr = nullptr, *d_rec_drc = nullptr;
+    int *d_rec_tet = nullptr, *d_rec_edge = nullptr;
+    int *d_src_edge = nullptr;
+    double *d_src_w = nullptr;
+    // solver
+    Symbolic sym;
+    Plan plan;
+    FrontDesc *d_fd = nullptr;
+    int *d_perm = nullptr, *d_parent = nullptr, *d_chp = nullptr, *d_ch = nullptr, *d_rel = nullptr,
+        *d_sidx = nullptr, *d_plist = nullptr;
+    int64_t *d_amap = nullptr, *d_ppref = nullptr;
+    double *d_U = nullptr, *d_V = nullptr, *d_xw = nullptr, *d_partial = nullptr;
+    double *d_L[2] = {nullptr, nullptr}, *d_D[2] = {nullptr, nullptr};
+    bool slot_live[2] = {false, false};
+    int *d_flag = nullptr;
+    double *d_tmpb = nullptr, *d_tmpx = nullptr;
+    // stepping work vectors
+    std::vector<double *> vecs;
+    double *d_norm_part = nullptr, *d_norm_out = nullptr;
+    // output rows
+    std::vector<int> row_step;
+    std::vector<double> row_t, row_dt;
+    double *d_rows = nullptr;
+    int64_t rows_cap = 0;
+    int nrows = 0;
+    std::vector<int> chk_step, chk_keep;
+    std::vector<double> chk_err;
+    double timers[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int leaf_size = 48;
+};
+
+namespace {
+
+void free_solver(tf_ctx *c) {
+    dfree(c->d_fd); dfree(c->d_perm); dfree(c->d_parent); dfree(c->d_chp); dfree(c->d_ch); dfree(c->d_rel);
+    dfree(c->d_sidx); dfree(c->d_plist); dfree(c->d_amap); dfree(c->d_ppref); dfree(c->d_U); dfree(c->d_V);
+    dfree(c->d_xw); dfree(c->d_partial);
+    for (int s = 0; s < 2; s++) { dfree(c->d_L[s]); dfree(c->d_D[s]); c->slot_live[s] = false; }
+    dfree(c->d_tmpb); dfree(c->d_tmpx);
+    c->analyzed = false;
+}
+
+void free_vecs(tf_ctx *c) {
+    for (auto &p : c->vecs) dfree(p);
+    c->vecs.clear();
+    dfree(c->d_norm_part); dfree(c->d_norm_out); dfree(c->d_rows);
+    c->rows_cap = 0;
+}
+
+void free_all(tf_ctx *c) {
+    free_solver(c);
+    free_vecs(c);
+    dfree(c->d_xyz); dfree(c->d_sigma); dfree(c->d_invmu); dfree(c->d_vm); dfree(c->d_vk);
+    dfree(c->d_tet_vex); dfree(c->d_tet_drc); dfree(c->d_ctr); dfree(c->d_orient); dfree(c->d_cptr);
+    dfree(c->d_M); dfree(c->d_K); dfree(c->d_fia); dfree(c->d_fja); dfree(c->d_fmap); dfree(c->d_fval);
+    dfree(c->d_rec_xyz); dfree(c->d_narr); dfree(c->d_darr); dfree(c->d_rec_drc); dfree(c->d_rec_tet);
+    dfree(c->d_rec_edge); dfree(c->d_src_edge); dfree(c->d_src_w); dfree(c->d_flag);
+}
+
+int fail(tf_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+template <class F>
+int guarded(tf_ctx *c, F &&f) {
+    if (!c) return TF_ERR_ARG;
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        return f();
+    } catch (const HipError &e) {
+        return fail(c, TF_ERR_HIP, e.what());
+    } catch (const std::exception &e) {
+        return fail(c, TF_ERR_INPUT, e.what());
+    }
+}
+
+// ---- plan --------------------------------------------------------------------
+void build_plan(tf_ctx *c) {
+    const Symbolic &s = c->sym;
+    Plan &p = c->plan;
+    p = Plan();
+    auto f_of = [&](int k) { return s.nc[k] + (int)(s.sptr[k + 1] - s.sptr[k]); };
+    auto m_of = [&](int k) { return (int)(s.sptr[k + 1] - s.sptr[k]); };
+    auto add = [&](std::vector<Launch> &dst, int kind, int p0, const std::vector<int> &fr,
+                   const std::vector<int64_t> &units) {
+        Launch L{kind, p0, (int)p.list.size(), (int)fr.size(), (int)p.pref.size(), 0};
+        p.list.insert(p.list.end(), fr.begin(), fr.end());
+        int64_t acc = 0;
+        for (size_t i = 0; i < fr.size(); i++) {
+            p.pref.push_back(acc);
+            acc += units.empty() ? 1 : units[i];
+        }
+        p.pref.push_back(acc);
+        L.nwg = acc;
+        if (!fr.empty()) dst.push_back(L);
+    };
+    const int FUSE_COUNT = 256, BIG_F = 384;
+    std::vector<std::vector<int>> fused(s.nlevels), grid(s.nlevels);
+    for (int l = 0; l < s.nlevels; l++) {
+        int cnt = s.lev_ptr[l + 1] - s.lev_ptr[l];
+        for (int q = s.lev_ptr[l]; q < s.lev_ptr[l + 1]; q++) {
+            int k = s.lev_sn[q];
+            if (cnt >= FUSE_COUNT || f_of(k) <= BIG_F) fused[l].push_back(k);
+            else grid[l].push_back(k);
+        }
+    }
+    // factorisation
+    for (int l = 0; l < s.nlevels; l++) {
+        int maxch = 0;
+        for (int q = s.lev_ptr[l]; q < s.lev_ptr[l + 1]; q++) {
+            int k = s.lev_sn[q];
+            maxch = std::max(maxch, s.children_ptr[k + 1] - s.children_ptr[k]);
+        }
+        for (int r = 0; r < maxch; r++) {
+            std::vector<int> fr;
+            std::vector<int64_t> un;
+            for (int q = s.lev_ptr[l]; q < s.lev_ptr[l + 1]; q++) {
+                int k = s.lev_sn[q];
+                if (s.children_ptr[k + 1] - s.children_ptr[k] > r) {
+                    int ch = s.children[s.children_ptr[k] + r];
+                    if (m_of(ch) > 0) { fr.push_back(ch); un.push_back(m_of(ch)); }
+                }
+            }
+            add(p.factor, L_EXTADD, 0, fr, un);
+        }
+        add(p.factor, L_FFUSED, 0, fused[l], {});
+        int kmax = 0;
+        for (int k : grid[l]) kmax = std::max(kmax, (s.nc[k] + TF_PB - 1) / TF_PB);
+        for (int pk = 0; pk < kmax; pk++) {
+            int p0 = pk * TF_PB;
+            std::vector<int> act;
+            std::vector<int64_t> ut, uu;
+            for (int k : grid[l])
+                if (s.nc[k] > p0) {
+                    int p1 = std::min(s.nc[k], p0 + TF_PB);
+                    int rows = f_of(k) - p1;
+                    act.push_back(k);
+                    ut.push_back((rows + TF_TRSM_ROWS - 1) / TF_TRSM_ROWS);
+                    int64_t nt = (rows + TF_TB - 1) / TF_TB;
+                    uu.push_back(nt * (nt + 1) / 2);
+                }
+            add(p.factor, L_FDIAG, p0, act, {});
+            add(p.factor, L_FTRSM, p0, act, ut);
+            add(p.factor, L_FUPD, p0, act, uu);
+        }
+    }
+    // forward solve, bottom-up
+    for (int l = 0; l < s.nlevels; l++) {
+        add(p.fwd, L_SFUSED, 0, fused[l], {});
+        add(p.fwd, L_SGATHER, 0, grid[l], {});
+        int kmax = 0;
+        for (int k : grid[l]) kmax = std::max(kmax, (s.nc[k] + TF_PB - 1) / TF_PB);
+        for (int pk = 0; pk < kmax; pk++) {
+            int p0 = pk * TF_PB;
+            std::vector<int> act;
+            std::vector<int64_t> ur;
+            for (int k : grid[l])
+                if (s.nc[k] > p0) {
+                    int p1 = std::min(s.nc[k], p0 + TF_PB);
+                    act.push_back(k);
+                    ur.push_back((f_of(k) - p1 + TF_NT - 1) / TF_NT);
+                }
+            add(p.fwd, L_SDIAG, p0, act, {});
+            add(p.fwd, L_SUPD, p0, act, ur);
+        }
+    }
+    // backward solve, top-down
+    for (int l = s.nlevels - 1; l >= 0; l--) {
+        int kmax = 0;
+        for (int k : grid[l]) kmax = std::max(kmax, (s.nc[k] + TF_PB - 1) / TF_PB);
+        for (int pk = kmax - 1; pk >= 0; pk--) {
+            int p0 = pk * TF_PB;
+            std::vector<int> act;
+            std::vector<int64_t> ur;
+            for (int k : grid[l])
+                if (s.nc[k] > p0) {
+                    int p1 = std::min(s.nc[k], p0 + TF_PB);
+                    act.push_back(k);
+                    ur.push_back((f_of(k) - p1 + TF_NT - 1) / TF_NT);
+                }
+            // the diag launch reuses the partial launch's list/pref
+            if (act.empty()) continue;
+            add(p.bwd, L_BPART, p0, act, ur);
+            Launch d = p.bwd.back();
+            d.kind = L_BDIAG;
+            p.bwd.push_back(d);
+            p.partial_max = std::max(p.partial_max, d.nwg);
+        }
+        add(p.bwd, L_BFUSED, 0, fused[l], {});
+    }
+}
+
+void do_analyze(tf_ctx *c, int leaf) {
+    free_solver(c);
+    auto t0 = std::chrono::steady_clock::now();
+    tf::analyze(c->sym, c->topo, leaf);
+    Symbolic &s = c->sym;
+    // front descriptors
+    std::vector<FrontDesc> fd(s.nsn);
+    int64_t voff = 0;
+    for (int k = 0; k < s.nsn; k++) {
+        int m = (int)(s.sptr[k + 1] - s.sptr[k]);
+        fd[k].loff = s.loff[k];
+        fd[k].uoff = s.uoff[k];
+        fd[k].soff = s.sptr[k];
+        fd[k].voff = voff;
+        fd[k].nc = s.nc[k];
+        fd[k].m = m;
+        fd[k].f = s.nc[k] + m;
+        fd[k].c0 = s.c0[k];
+        voff += fd[k].f;
+    }
+    build_plan(c);
+    c->d_fd = dupload(fd);
+    c->d_perm = dupload(s.perm);
+    c->d_parent = dupload(s.parent);
+    c->d_chp = dupload(s.children_ptr);
+    c->d_ch = dupload(s.children);
+    c->d_rel = dupload(s.rel);
+    c->d_sidx = dupload(s.sidx);
+    c->d_amap = dupload(s.amap);
+    c->d_plist = dupload(c->plan.list);
+    c->d_ppref = dupload(c->plan.pref);
+    c->d_U = dalloc<double>(s.utotal);
+    c->d_V = dalloc<double>(voff);
+    c->d_xw = dalloc<double>(s.n);
+    c->d_partial = dalloc<double>((size_t)std::max<int64_t>(c->plan.partial_max, 1) * TF_PB);
+    c->d_tmpb = dalloc<double>(s.n);
+    c->d_tmpx = dalloc<double>(s.n);
+    if (!c->d_flag) c->d_flag = dalloc<int>(1);
+    c->analyzed = true;
+    c->timers[6] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// numeric factorisation of the values already scattered into slot's L
+void run_factor(tf_ctx *c, int slot) {
+    const Plan &p = c->plan;
+    double *L = c->d_L[slot], *Dn = c->d_D[slot];
+    const int *lst = c->d_plist;
+    const int64_t *pr = c->d_ppref;
+    for (const Launch &l : p.factor) {
+        switch (l.kind) {
+        case L_EXTADD:
+            tf::launch_extend_add(L, c->d_U, c->d_fd, c->d_parent, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_rel,
+                                  c->st);
+            break;
+        case L_FFUSED:
+            tf::launch_factor_fused(L, c->d_U, c->d_fd, lst + l.off, l.cnt, Dn, c->d_flag, c->st);
+            break;
+        case L_FDIAG:
+            tf::launch_factor_diag(L, c->d_fd, lst + l.off, l.cnt, l.p0, Dn, c->d_flag, c->st);
+            break;
+        case L_FTRSM:
+            tf::launch_factor_trsm(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, c->st);
+            break;
+        case L_FUPD:
+            tf::launch_factor_update(L, c->d_U, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, Dn, c->st);
+            break;
+        }
+    }
+    HIPCHK(hipGetLastError());
+}
+
+void prepare_slot(tf_ctx *c, int slot) {
+    if (!c->d_L[slot]) c->d_L[slot] = dalloc<double>(c->sym.ltotal);
+    if (!c->d_D[slot]) c->d_D[slot] = dalloc<double>(c->sym.n);
+    tf::launch_fill0(c->d_L[slot], c->sym.ltotal, c->st);
+    tf::launch_fill0(c->d_U, c->sym.utotal, c->st);
+    HIPCHK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->st));
+}
+
+void check_flag(tf_ctx *c) {
+    int flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (flag) throw std::runtime_error("zero or non-finite pivot in LDL^T factorisation");
+}
+
+void factor_dt(tf_ctx *c, int slot, double dt) {
+    prepare_slot(c, slot);
+    tf::launch_scatter_p(c->topo.nnz, c->d_amap, c->d_M, c->d_K, 2.0 * dt, c->d_L[slot], c->st);
+    run_factor(c, slot);
+    c->slot_live[slot] = true;
+}
+
+// x = P^-1 b (device vectors, edge order)
+void solve_dev(tf_ctx *c, int slot, const double *b, double *x) {
+    const Plan &p = c->plan;
+    const double *L = c->d_L[slot], *Dn = c->d_D[slot];
+    const int *lst = c->d_plist;
+    const int64_t *pr = c->d_ppref;
+    for (const Launch &l : p.fwd) {
+        switch (l.kind) {
+        case L_SFUSED:
+            tf::launch_fwd_fused(L, c->d_fd, lst + l.off, l.cnt, b, c->d_perm, c->d_chp, c->d_ch, c->d_rel, c->d_V,
+                                 c->st);
+            break;
+        case L_SGATHER:
+            tf::launch_fwd_gather(c->d_fd, lst + l.off, l.cnt, b, c->d_perm, c->d_chp, c->d_ch, c->d_rel, c->d_V,
+                                  c->st);
+            break;
+        case L_SDIAG:
+            tf::launch_fwd_diag(L, c->d_fd, lst + l.off, l.cnt, l.p0, c->d_V, c->st);
+            break;
+        case L_SUPD:
+            tf::launch_fwd_update(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, c->d_V, c->st);
+            break;
+        }
+    }
+    for (const Launch &l : p.bwd) {
+        switch (l.kind) {
+        case L_BFUSED:
+            tf::launch_bwd_fused(L, c->d_fd, lst + l.off, l.cnt, c->d_V, Dn, c->d_sidx, c->d_xw, c->st);
+            break;
+        case L_BPART:
+            tf::launch_bwd_partial(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, c->d_xw, c->d_sidx,
+                                   c->d_partial, c->st);
+            break;
+        case L_BDIAG:
+            tf::launch_bwd_diag(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.p0, c->d_V, Dn, c->d_partial,
+                                c->d_xw, c->st);
+            break;
+        }
+    }
+    tf::launch_unpermute(c->sym.n, c->d_perm, c->d_xw, x, c->st);
+}
+
+// ---- time loop -------------------------------------------------------------------
+struct StepVecs {
+    double *e1, *e2, *e3, *rh, *te1, *te2, *te3, *erec, *ev1, *ev2;
+    std::vector<double *> ring;
+};
+
+void ensure_vecs(tf_ctx *c, StepVecs &v) {
+    int n = c->topo.nE;
+    int need = 10 + c->sp_dblesize + 1;
+    if ((int)c->vecs.size() != need) {
+        for (auto &p : c->vecs) dfree(p);
+        c->vecs.clear();
+        for (int i = 0; i < need; i++) c->vecs.push_back(dalloc<double>(n));
+    }
+    v.e1 = c->vecs[0]; v.e2 = c->vecs[1]; v.e3 = c->vecs[2]; v.rh = c->vecs[3];
+    v.te1 = c->vecs[4]; v.te2 = c->vecs[5]; v.te3 = c->vecs[6]; v.erec = c->vecs[7];
+    v.ev1 = c->vecs[8]; v.ev2 = c->vecs[9];
+    v.ring.assign(c->vecs.begin() + 10, c->vecs.end());
+    if (!c->d_norm_part) c->d_norm_part = dalloc<double>(1024);
+    if (!c->d_norm_out) c->d_norm_out = dalloc<double>(2);
+}
+
+void rhs(tf_ctx *c, int step, double t, const double *e2, const double *e3, double dt, bool negate, double *r) {
+    tf::launch_spmv_rhs(c->topo.nE, c->d_fia, c->d_fja, c->d_fval, e2, e3, r, c->st);
+    if (step < c->sp_division) {
+        double didt = source_didt(c->stype, c->ptype, t, c->pulse_width, c->current);
+        if (negate) didt = -didt;
+        tf::launch_source(c->nseg, c->d_src_edge, c->d_src_w, didt, 2.0 * dt, r, c->st);
+    }
+}
+
+void dcopy(tf_ctx *c, double *dst, const double *src) {
+    HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * c->topo.nE, hipMemcpyDeviceToDevice, c->st));
+}
+
+void output_row(tf_ctx *c, int step, double t, double dt, const double *e) {
+    int64_t per = (int64_t)c->nR * 6;
+    if (c->nrows + 1 > c->rows_cap) {
+        int64_t cap = std::max<int64_t>(256, 2 * c->rows_cap);
+        double *nb = dalloc<double>(cap * per);
+        if (c->d_rows && c->nrows > 0)
+            HIPCHK(hipMemcpyAsync(nb, c->d_rows, sizeof(double) * per * c->nrows, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        dfree(c->d_rows);
+        c->d_rows = nb;
+        c->rows_cap = cap;
+    }
+    tf::launch_readout(c->nR, c->d_rec_edge, c->d_rec_drc, c->d_narr, c->d_darr, e, c->d_rows + per * c->nrows, c->st);
+    c->row_step.push_back(step);
+    c->row_t.push_back(t);
+    c->row_dt.push_back(dt);
+    c->nrows++;
+}
+
+double two_error(tf_ctx *c, const double *a, const double *b) {
+    tf::launch_diffnorm(c->topo.nE, a, b, c->d_norm_part, c->d_norm_out, c->st);
+    double h[2];
+    HIPCHK(hipMemcpyAsync(h, c->d_norm_out, sizeof h, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return std::sqrt(h[0]) / std::sqrt(h[1]);
+}
+
+double elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    return ms * 1e-3;
+}
+
+// adap_euler (src/FETD_FWD.f90:1229-1464).  inout: 0 plain run with output,
+// 1 first pass of a step-off run (negated source, no output, result kept in
+// res), 2 second pass starting from res.
+void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
+    StepVecs v;
+    ensure_vecs(c, v);
+    const int n = c->topo.nE;
+    const int dbl = c->sp_dblesize;
+    if (c->sp_double < dbl + 2) throw std::runtime_error("sp_double must be at least sp_dblesize+2");
+    int step = 0, not_changed = 0, idx = 0;
+    double time_cur = 0.0, step_cur = c->pulse_width / c->sp_division;
+    const double time_limit = c->time_max;
+    const bool negate = (inout == 1), outp = (inout != 1);
+    const int ring = dbl + 1;
+    int cur = 0;
+    for (double *p : {v.e1, v.e2, v.e3}) HIPCHK(hipMemsetAsync(p, 0, sizeof(double) * n, c->st));
+    hipEvent_t f0, f1;
+    HIPCHK(hipEventCreate(&f0));
+    HIPCHK(hipEventCreate(&f1));
+    auto timed_factor = [&](int slot, double dt) {
+        HIPCHK(hipEventRecord(f0, c->st));
+        factor_dt(c, slot, dt);
+        HIPCHK(hipEventRecord(f1, c->st));
+        check_flag(c);
+        c->timers[1] += elapsed(f0, f1);
+        c->timers[7] += 1.0;
+    };
+    while (time_cur < time_limit) {
+        if (max_steps > 0 && step >= max_steps) break;
+        step++;
+        time_cur = time_cur + step_cur;
+        if (step == 1) {
+            if (inout == 2) { dcopy(c, v.e2, res); dcopy(c, v.e3, res); }
+            if (outp) output_row(c, 0, time_cur - step_cur, step_cur, v.e2);
+            timed_factor(cur, step_cur);
+        }
+        if (step % c->sp_double == 0) {
+            dcopy(c, v.te2, v.e2);
+            dcopy(c, v.te3, v.e3);
+            double ttime = time_cur, tstep = step_cur;
+            for (int i = 1; i <= dbl; i++) {
+                rhs(c, step, ttime, v.te2, v.te3, tstep, negate, v.rh);
+                solve_dev(c, cur, v.rh, v.te1);
+                if (i == dbl) break;
+                if (i == 1) dcopy(c, v.erec, v.te1);
+                std::swap(v.te3, v.te2);   // te3 <- te2
+                dcopy(c, v.te2, v.te1);    // te2 <- te1
+                ttime = ttime + tstep;
+            }
+            dcopy(c, v.ev1, v.te1);
+            tstep = step_cur * dbl;
+            ttime = time_cur - step_cur + tstep;
+            const int other = 1 - cur;
+            timed_factor(other, tstep);
+            double *ea = v.ring[(idx - 1) % ring];
+            double *eb = v.ring[(idx - 1 - dbl) % ring];
+            rhs(c, step, ttime, ea, eb, tstep, negate, v.rh);
+            solve_dev(c, other, v.rh, v.ev2);
+            const double error = two_error(c, v.ev2, v.ev1);
+            const bool keep = (error > 1.0e-4 && not_changed < 2);
+            c->chk_step.push_back(step);
+            c->chk_err.push_back(error);
+            c->chk_keep.push_back(keep ? 1 : 0);
+            if (keep) {
+                c->slot_live[other] = false;
+                dcopy(c, v.e1, v.erec);
+                if (outp) output_row(c, step, time_cur, step_cur, v.e1);
+                std::swap(v.e3, v.e2);  // e3 <- e2
+                dcopy(c, v.e2, v.e1);   // e2 <- e1
+                not_changed++;
+                idx = 0;
+            } else {
+                c->slot_live[cur] = false;
+                cur = other;
+                dcopy(c, v.e3, ea);
+                dcopy(c, v.e2, v.ev1);
+                time_cur = ttime;
+                step_cur = tstep;
+                if (outp) output_row(c, step, time_cur, step_cur, v.e2);
+                not_changed = 0;
+                idx = 0;
+            }
+            continue;
+        }
+        rhs(c, step, time_cur, v.e2, v.e3, step_cur, negate, v.rh);
+        solve_dev(c, cur, v.rh, v.e1);
+        if (outp) output_row(c, step, time_cur, step_cur, v.e1);
+        dcopy(c, v.ring[idx % ring], v.e1);
+        idx++;
+        // e3 <- e2 <- e1 by rotation; e1 is overwritten next step
+        double *t3 = v.e3;
+        v.e3 = v.e2;
+        v.e2 = v.e1;
+        v.e1 = t3;
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipGetLastError());
+    if (inout == 1 && res) dcopy(c, res, v.e1 == v.e2 ? v.e1 : v.e2);
+    HIPCHK(hipEventDestroy(f0));
+    HIPCHK(hipEventDestroy(f1));
+    // keep the rotated pointers in the context's vector table
+    c->vecs[0] = v.e1; c->vecs[1] = v.e2; c->vecs[2] = v.e3; c->vecs[5] = v.te2; c->vecs[6] = v.te3;
+}
+
+}  // namespace
+
+// ======================================================================
+// C ABI
+// ======================================================================
+extern "C" {
+
+int tf_create(tf_ctx **out, int device) {
+    if (!out) return TF_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return TF_ERR_HIP;
+    if (device < 0 || device >= ndev) return TF_ERR_ARG;
+    tf_ctx *c = new tf_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess) {
+        delete c;
+        return TF_ERR_HIP;
+    }
+    *out = c;
+    return TF_OK;
+}
+
+void tf_destroy(tf_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->st);
+    free_all(c);
+    (void)hipStreamDestroy(c->st);
+    delete c;
+}
+
+const char *tf_last_error(const tf_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int tf_set_mesh(tf_ctx *c, int nN, const double *nod_cor, int nT, const int *tet_vex) {
+    return guarded(c, [&] {
+        if (nN <= 0 || nT <= 0 || !nod_cor || !tet_vex) return fail(c, TF_ERR_ARG, "empty mesh");
+        free_all(c);
+        c->assembled = c->have_mat = c->have_src = false;
+        c->nR = 0;
+        tf::build_topology(c->topo, nN, nod_cor, nT, tet_vex);
+        const Topology &t = c->topo;
+        c->d_xyz = dupload(t.xyz);
+        c->d_tet_vex = dupload(t.tet_vex);
+        c->d_tet_drc = dupload(t.tet_drc);
+        c->d_orient = dupload(t.pair_orient);
+        c->d_cptr = dupload(t.cptr);
+        std::vector<int> ctr32(t.ctr.begin(), t.ctr.end());
+        c->d_ctr = dupload(ctr32);
+        // full symmetric pattern of the mass matrix for the right-hand side
+        int n = t.nE;
+        std::vector<int> deg(n + 1, 0);
+        for (int r = 0; r < n; r++)
+            for (int q = t.ia[r]; q < t.ia[r + 1]; q++) {
+                deg[r]++;
+                if (t.ja[q] != r) deg[t.ja[q]]++;
+            }
+        std::vector<int> fia(n + 1, 0);
+        for (int r = 0; r < n; r++) fia[r + 1] = fia[r] + deg[r];
+        std::vector<int> fja(fia[n]);
+        std::vector<int64_t> fmap(fia[n]);
+        std::fill(deg.begin(), deg.end(), 0);
+        for (int r = 0; r < n; r++)
+            for (int q = t.ia[r]; q < t.ia[r + 1]; q++) {
+                int cc = t.ja[q];
+                int p = fia[r] + deg[r]++;
+                fja[p] = cc;
+                fmap[p] = q;
+                if (cc != r) {
+                    p = fia[cc] + deg[cc]++;
+                    fja[p] = r;
+                    fmap[p] = q;
+                }
+            }
+        c->d_fia = dupload(fia);
+        c->d_fja = dupload(fja);
+        c->d_fmap = dupload(fmap);
+        c->d_fval = dalloc<double>(fja.size());
+        c->d_M = dalloc<double>(t.nnz);
+        c->d_K = dalloc<double>(t.nnz);
+        c->d_vm = dalloc<double>((size_t)t.nT * 21);
+        c->d_vk = dalloc<double>((size_t)t.nT * 21);
+        c->have_mesh = true;
+        return TF_OK;
+    });
+}
+
+int tf_get_sizes(const tf_ctx *c, int *nE, int *nnzU) {
+    if (!c || !c->have_mesh) return TF_ERR_STATE;
+    if (nE) *nE = c->topo.nE;
+    if (nnzU) *nnzU = (int)c->topo.nnz;
+    return TF_OK;
+}
+
+int tf_get_topology(const tf_ctx *c, int *edg_vex, int *tet_edg, int *tet_drc, int *ia, int *ja) {
+    if (!c || !c->have_mesh) return TF_ERR_STATE;
+    const Topology &t = c->topo;
+    if (edg_vex) std::copy(t.edg_vex.begin(), t.edg_vex.end(), edg_vex);
+    if (tet_edg)
+        for (size_t i = 0; i < t.tet_edg.size(); i++) tet_edg[i] = t.tet_edg[i] + 1;
+    if (tet_drc) std::copy(t.tet_drc.begin(), t.tet_drc.end(), tet_drc);
+    if (ia)
+        for (size_t i = 0; i < t.ia.size(); i++) ia[i] = t.ia[i] + 1;
+    if (ja)
+        for (size_t i = 0; i < t.ja.size(); i++) ja[i] = t.ja[i] + 1;
+    return TF_OK;
+}
+
+int tf_set_materials(tf_ctx *c, const double *sigma, const double *inv_mu) {
+    return guarded(c, [&] {
+        if (!c->have_mesh) return fail(c, TF_ERR_STATE, "tf_set_mesh first");
+        if (!sigma || !inv_mu) return fail(c, TF_ERR_ARG, "null material array");
+        c->sigma.assign(sigma, sigma + c->topo.nT);
+        c->inv_mu.assign(inv_mu, inv_mu + c->topo.nT);
+        dfree(c->d_sigma);
+        dfree(c->d_invmu);
+        c->d_sigma = dupload(c->sigma);
+        c->d_invmu = dupload(c->inv_mu);
+        c->have_mat = true;
+        c->assembled = false;
+        return TF_OK;
+    });
+}
+
+int tf_set_source(tf_ctx *c, int isrc, int nseg, const int *seg_vex, int source_type, int pulse_type,
+                  double pulse_width, double current) {
+    return guarded(c, [&] {
+        if (!c->have_mesh) return fail(c, TF_ERR_STATE, "tf_set_mesh first");
+        if (isrc != 0) return fail(c, TF_ERR_ARG, "only source index 0 is supported by this build");
+        if (nseg <= 0 || !seg_vex) return fail(c, TF_ERR_ARG, "no source segments");
+        if (source_type != 1 || pulse_type < 1 || pulse_type > 3)
+            return fail(c, TF_ERR_INPUT, "no such kind of source");
+        c->src_edge.assign(nseg, 0);
+        c->src_w.assign(nseg, 0.0);
+        for (int i = 0; i < nseg; i++) {
+            int e = tf::find_edge(c->topo, seg_vex[2 * i], seg_vex[2 * i + 1]);
+            if (e < 0) return fail(c, TF_ERR_INPUT, "source segment " + std::to_string(i + 1) + " is not a mesh edge");
+            c->src_edge[i] = e;
+            double drc = (i == nseg - 1) ? -1.0 : 1.0;  // src/FETD_FWD.f90:446-448
+            c->src_w[i] = drc * c->topo.edg_len[e];
+        }
+        c->nseg = nseg;
+        c->stype = source_type;
+        c->ptype = pulse_type;
+        c->pulse_width = pulse_width;
+        c->current = current;
+        dfree(c->d_src_edge);
+        dfree(c->d_src_w);
+        c->d_src_edge = dupload(c->src_edge);
+        c->d_src_w = dupload(c->src_w);
+        c->have_src = true;
+        return TF_OK;
+    });
+}
+
+int tf_set_receivers(tf_ctx *c, int nR, const double *xyz, double rec_charac, int *rec_tet_out) {
+    return guarded(c, [&] {
+        if (!c->have_mesh) return fail(c, TF_ERR_STATE, "tf_set_mesh first");
+        if (nR <= 0 || !xyz) return fail(c, TF_ERR_ARG, "no receivers");
+        c->nR = nR;
+        c->rec_xyz.assign(xyz, xyz + 3 * (size_t)nR);
+        c->rec_charac = rec_charac;
+        c->rec_tet.assign(nR, 0);
+        for (int i = 0; i < nR; i++) {
+            c->rec_tet[i] = tf::point_in_tet(c->topo, &xyz[3 * i], rec_charac);
+            if (c->rec_tet[i] == 0) return fail(c, TF_ERR_INPUT, "no rec tet No. " + std::to_string(i + 1));
+        }
+        if (rec_tet_out) std::copy(c->rec_tet.begin(), c->rec_tet.end(), rec_tet_out);
+        std::vector<int> redge(6 * (size_t)nR);
+        std::vector<double> rdrc(6 * (size_t)nR);
+        for (int i = 0; i < nR; i++)
+            for (int j = 0; j < 6; j++) {
+                redge[6 * i + j] = c->topo.tet_edg[6 * (size_t)(c->rec_tet[i] - 1) + j];
+                rdrc[6 * i + j] = (double)c->topo.tet_drc[6 * (size_t)(c->rec_tet[i] - 1) + j];
+            }
+        dfree(c->d_rec_xyz); dfree(c->d_rec_tet); dfree(c->d_rec_edge); dfree(c->d_rec_drc);
+        dfree(c->d_narr); dfree(c->d_darr);
+        c->d_rec_xyz = dupload(c->rec_xyz);
+        c->d_rec_tet = dupload(c->rec_tet);
+        c->d_rec_edge = dupload(redge);
+        c->d_rec_drc = dupload(rdrc);
+        c->d_narr = dalloc<double>(18 * (size_t)nR);
+        c->d_darr = dalloc<double>(18 * (size_t)nR);
+        tf::launch_recvec(nR, c->d_xyz, c->d_tet_vex, c->d_rec_tet, c->d_rec_xyz, c->d_narr, c->d_darr, c->st);
+        HIPCHK(hipStreamSynchronize(c->st));
+        return TF_OK;
+    });
+}
+
+int tf_set_schedule(tf_ctx *c, int sp_division, int sp_double, int sp_dblesize, double time_maximum) {
+    if (!c) return TF_ERR_ARG;
+    if (sp_division <= 0) return fail(c, TF_ERR_INPUT, "cmod%step_dvsi error");
+    if (sp_double <= 0) return fail(c, TF_ERR_INPUT, "cmod%step_dble error");
+    if (std::fabs(time_maximum) < 1e-20) return fail(c, TF_ERR_INPUT, "cmod%time_maxm error");
+    if (sp_dblesize < 2) return fail(c, TF_ERR_INPUT, "sp_dblesize must be at least 2");
+    c->sp_division = sp_division;
+    c->sp_double = sp_double;
+    c->sp_dblesize = sp_dblesize;
+    c->time_max = time_maximum;
+    return TF_OK;
+}
+
+int tf_assemble(tf_ctx *c) {
+    return guarded(c, [&] {
+        if (!c->have_mesh || !c->have_mat) return fail(c, TF_ERR_STATE, "mesh and materials first");
+        auto t0 = std::chrono::steady_clock::now();
+        const Topology &t = c->topo;
+        tf::launch_element(t.nT, c->d_xyz, c->d_tet_vex, c->d_tet_drc, c->d_orient, c->d_sigma, c->d_invmu, c->d_vm,
+                           c->d_vk, c->st);
+        tf::launch_gather(t.nnz, c->d_cptr, c->d_ctr, c->d_vm, c->d_vk, c->d_M, c->d_K, c->st);
+        tf::launch_gather_vals((int64_t)t.ia.size() > 0 ? (int64_t)(2 * t.nnz - t.nE) : 0, c->d_fmap, c->d_M,
+                               c->d_fval, c->st);
+        HIPCHK(hipStreamSynchronize(c->st));
+        HIPCHK(hipGetLastError());
+        c->timers[0] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        c->assembled = true;
+        return TF_OK;
+    });
+}
+
+int tf_get_matrices(const tf_ctx *c, double *mass_sigma, double *curl) {
+    if (!c || !c->assembled) return TF_ERR_STATE;
+    if (mass_sigma && hipMemcpy(mass_sigma, c->d_M, sizeof(double) * c->topo.nnz, hipMemcpyDeviceToHost))
+        return TF_ERR_HIP;
+    if (curl && hipMemcpy(curl, c->d_K, sizeof(double) * c->topo.nnz, hipMemcpyDeviceToHost)) return TF_ERR_HIP;
+    return TF_OK;
+}
+
+int tf_get_receiver_vectors(const tf_ctx *c, double *n_arr, double *d_arr) {
+    if (!c || c->nR == 0) return TF_ERR_STATE;
+    if (n_arr && hipMemcpy(n_arr, c->d_narr, sizeof(double) * 18 * c->nR, hipMemcpyDeviceToHost)) return TF_ERR_HIP;
+    if (d_arr && hipMemcpy(d_arr, c->d_darr, sizeof(double) * 18 * c->nR, hipMemcpyDeviceToHost)) return TF_ERR_HIP;
+    return TF_OK;
+}
+
+int tf_analyze(tf_ctx *c, int leaf_size) {
+    return guarded(c, [&] {
+        if (!c->have_mesh) return fail(c, TF_ERR_STATE, "tf_set_mesh first");
+        if (leaf_size > 0) c->leaf_size = leaf_size;
+        do_analyze(c, c->leaf_size);
+        return TF_OK;
+    });
+}
+
+int tf_factor_stats(const tf_ctx *c, int64_t *nnzL, double *flops, int *nsuper, int *nlevels) {
+    if (!c || !c->analyzed) return TF_ERR_STATE;
+    if (nnzL) *nnzL = c->sym.nnzL;
+    if (flops) *flops = c->sym.flops;
+    if (nsuper) *nsuper = c->sym.nsn;
+    if (nlevels) *nlevels = c->sym.nlevels;
+    return TF_OK;
+}
+
+int tf_factor(tf_ctx *c, int slot, double dt) {
+    return guarded(c, [&] {
+        if (slot < 0 || slot > 1) return fail(c, TF_ERR_ARG, "slot must be 0 or 1");
+        if (!c->assembled) return fail(c, TF_ERR_STATE, "tf_assemble first");
+        if (!c->analyzed) do_analyze(c, c->leaf_size);
+        factor_dt(c, slot, dt);
+        check_flag(c);
+        return TF_OK;
+    });
+}
+
+int tf_factor_csr(tf_ctx *c, int slot, int n, const int *ia, const int *ja, const double *va) {
+    return guarded(c, [&] {
+        if (slot < 0 || slot > 1) return fail(c, TF_ERR_ARG, "slot must be 0 or 1");
+        if (!c->have_mesh) return fail(c, TF_ERR_STATE, "tf_set_mesh first");
+        const Topology &t = c->topo;
+        if (n != t.nE || !ia || !ja || !va) return fail(c, TF_ERR_ARG, "matrix size differs from the mesh");
+        for (int i = 0; i <= n; i++)
+            if (ia[i] != t.ia[i] + 1) return fail(c, TF_ERR_INPUT, "row pointer differs from the assembled pattern");
+        for (int64_t q = 0; q < t.nnz; q++)
+            if (ja[q] != t.ja[q] + 1) return fail(c, TF_ERR_INPUT, "column index differs from the assembled pattern");
+        if (!c->analyzed) do_analyze(c, c->leaf_size);
+        prepare_slot(c, slot);
+        double *dv = dalloc<double>(t.nnz);
+        HIPCHK(hipMemcpyAsync(dv, va, sizeof(double) * t.nnz, hipMemcpyHostToDevice, c->st));
+        tf::launch_scatter_vals(t.nnz, c->d_amap, dv, c->d_L[slot], c->st);
+        run_factor(c, slot);
+        c->slot_live[slot] = true;
+        HIPCHK(hipStreamSynchronize(c->st));
+        dfree(dv);
+        check_flag(c);
+        return TF_OK;
+    });
+}
+
+int tf_solve(tf_ctx *c, int slot, int nrhs, const double *b, double *x) {
+    return guarded(c, [&] {
+        if (slot < 0 || slot > 1 || !c->slot_live[slot]) return fail(c, TF_ERR_STATE, "slot not factored");
+        if (nrhs <= 0 || !b || !x) return fail(c, TF_ERR_ARG, "bad right-hand side");
+        int n = c->topo.nE;
+        for (int r = 0; r < nrhs; r++) {
+            HIPCHK(hipMemcpyAsync(c->d_tmpb, b + (size_t)r * n, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
+            solve_dev(c, slot, c->d_tmpb, c->d_tmpx);
+            HIPCHK(hipMemcpyAsync(x + (size_t)r * n, c->d_tmpx, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+        }
+        HIPCHK(hipStreamSynchronize(c->st));
+        HIPCHK(hipGetLastError());
+        return TF_OK;
+    });
+}
+
+int tf_release(tf_ctx *c, int slot) {
+    if (!c || slot < 0 || slot > 1) return TF_ERR_ARG;
+    c->slot_live[slot] = false;
+    return TF_OK;
+}
+
+int tf_run(tf_ctx *c, int max_steps, int max_rows, int *nrows, int *row_step, double *row_t, double *row_dt,
+           double *fields) {
+    return guarded(c, [&] {
+        if (!c->assembled) return fail(c, TF_ERR_STATE, "tf_assemble first");
+        if (!c->have_src) return fail(c, TF_ERR_STATE, "tf_set_source first");
+        if (c->nR == 0) return fail(c, TF_ERR_STATE, "tf_set_receivers first");
+        if (c->sp_division == 0) return fail(c, TF_ERR_STATE, "tf_set_schedule first");
+        if (!c->analyzed) do_analyze(c, c->leaf_size);
+        for (int k = 1; k < 6; k++) c->timers[k] = 0.0;
+        c->timers[7] = 0.0;
+        c->nrows = 0;
+        c->row_step.clear(); c->row_t.clear(); c->row_dt.clear();
+        c->chk_step.clear(); c->chk_err.clear(); c->chk_keep.clear();
+        auto t0 = std::chrono::steady_clock::now();
+        if (c->ptype == 3) {
+            double *res = dalloc<double>(c->topo.nE);
+            run_loop(c, 1, res, max_steps);
+            c->nrows = 0;
+            c->row_step.clear(); c->row_t.clear(); c->row_dt.clear();
+            run_loop(c, 2, res, max_steps);
+            dfree(res);
+        } else {
+            run_loop(c, 0, nullptr, max_steps);
+        }
+        HIPCHK(hipStreamSynchronize(c->st));
+        c->timers[5] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (nrows) *nrows = c->nrows;
+        int keep = std::min(c->nrows, std::max(max_rows, 0));
+        if (keep > 0) {
+            if (row_step) std::copy(c->row_step.begin(), c->row_step.begin() + keep, row_step);
+            if (row_t) std::copy(c->row_t.begin(), c->row_t.begin() + keep, row_t);
+            if (row_dt) std::copy(c->row_dt.begin(), c->row_dt.begin() + keep, row_dt);
+            if (fields) {
+                int64_t per = (int64_t)c->nR * 6;
+                std::vector<double> h(per * keep);
+                HIPCHK(hipMemcpy(h.data(), c->d_rows, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+                // rows: E, dB/dt from the device; B = running sum of dB/dt * dt
+                // (src/FETD_FWD.f90:1602-1621); dB/dt is zero at step 0 (:1541)
+                for (int r = 0; r < keep; r++)
+                    for (int i = 0; i < c->nR; i++) {
+                        double *o = fields + ((int64_t)r * c->nR + i) * 9;
+                        const double *src = &h[(r * (int64_t)c->nR + i) * 6];
+                        for (int k = 0; k < 3; k++) {
+                            o[k] = src[k];
+                            o[3 + k] = (c->row_step[r] == 0) ? 0.0 : src[3 + k];
+                            double prev = (r == 0) ? 0.0 : fields[((int64_t)(r - 1) * c->nR + i) * 9 + 6 + k];
+                            o[6 + k] = (r == 0) ? o[3 + k] * c->row_dt[r] : prev + o[3 + k] * c->row_dt[r];
+                        }
+                    }
+            }
+        }
+        return TF_OK;
+    });
+}
+
+int tf_get_checks(const tf_ctx *c, int max, int *n, int *step, double *err, int *keep) {
+    if (!c) return TF_ERR_ARG;
+    int cnt = (int)c->chk_step.size();
+    if (n) *n = cnt;
+    int k = std::min(cnt, std::max(max, 0));
+    if (step) std::copy(c->chk_step.begin(), c->chk_step.begin() + k, step);
+    if (err) std::copy(c->chk_err.begin(), c->chk_err.begin() + k, err);
+    if (keep) std::copy(c->chk_keep.begin(), c->chk_keep.begin() + k, keep);
+    return TF_OK;
+}
+
+int tf_get_timers(const tf_ctx *c, double *t8) {
+    if (!c || !t8) return TF_ERR_ARG;
+    std::copy(c->timers, c->timers + 8, t8);
+    return TF_OK;
+}
+
+}  // extern "C"

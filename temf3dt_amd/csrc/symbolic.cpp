@@ -1,0 +1,346 @@
+// symbolic.cpp -- fill-reducing ordering and the multifrontal supernode tree.
+//
+// The reference hands P to PARDISO phase 12, which re-runs METIS ordering
+// and symbolic factorisation at every call (src/FETD_UTILITY.f90:879-936,
+// called once per doubling check, src/FETD_FWD.f90:1307,1361).  The pattern
+// of P never changes, so this library analyses it once per mesh:
+//
+//  * geometric nested dissection on the edge graph, using edge midpoints as
+//    coordinates: each set is split by a plane (axis and quantile chosen for
+//    the smallest one-sided vertex separator), the separator is ordered last;
+//  * every dissection node with columns becomes one supernode whose diagonal
+//    block is dense (separators fill in completely; leaves are small);
+//  * row structures, relative indices into the parent front, L/U storage
+//    offsets, and the map from assembled CSR entries into L storage.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <numeric>
+#include <stdexcept>
+
+#include "tf_internal.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace tf {
+
+namespace {
+
+struct NDNode {
+    std::vector<int> cols;
+    int child[2] = {-1, -1};
+};
+
+struct NDState {
+    int n;
+    const std::vector<int> *xadj;
+    const std::vector<int> *adj;
+    const std::vector<double> *co;  // [n][3]
+    std::vector<int64_t> mark;     // set id * 2 + side
+    std::vector<NDNode> nodes;
+    std::atomic<int> nnodes{0};
+    std::atomic<int64_t> setid{1};
+    int leaf;
+};
+
+// Separator of `v` split at position `cut` of the sorted order `ord`
+// (first `cut` nodes form side 0).  Returns the separator and which side it
+// was taken from.
+static void eval_split(NDState &st, const std::vector<int> &ord, size_t cut, int64_t sid,
+                       std::vector<int> &sep, int &sepside) {
+    auto &mark = st.mark;
+    for (size_t i = 0; i < ord.size(); i++) mark[ord[i]] = sid * 2 + (i < cut ? 0 : 1);
+    std::vector<int> b0, b1;
+    const auto &xadj = *st.xadj;
+    const auto &adj = *st.adj;
+    for (size_t i = 0; i < ord.size(); i++) {
+        int v = ord[i];
+        int64_t mv = mark[v];
+        for (int q = xadj[v]; q < xadj[v + 1]; q++) {
+            int64_t mu = mark[adj[q]];
+            if ((mu >> 1) == sid && mu != mv) {
+                if ((mv & 1) == 0) b0.push_back(v); else b1.push_back(v);
+                break;
+            }
+        }
+    }
+    if (b0.size() <= b1.size()) { sep.swap(b0); sepside = 0; }
+    else { sep.swap(b1); sepside = 1; }
+}
+
+static int nd_rec(NDState &st, std::vector<int> v) {
+    int id = st.nnodes.fetch_add(1);
+    NDNode &nd = st.nodes[id];
+    if ((int)v.size() <= st.leaf) {
+        nd.cols.swap(v);
+        return id;
+    }
+    const auto &co = *st.co;
+    // bounding box spread by count: try each axis at the median
+    std::vector<int> best_sep;
+    std::vector<int> best_ord;
+    size_t best_cut = 0;
+    int best_side = 0;
+    double best_score = 1e300;
+    const double qs[5] = {0.5, 0.45, 0.55, 0.4, 0.6};
+    int best_axis = -1;
+    for (int pass = 0; pass < 2; pass++) {
+        for (int axis = 0; axis < 3; axis++) {
+            if (pass == 1 && axis != best_axis) continue;
+            std::vector<int> ord(v);
+            std::sort(ord.begin(), ord.end(), [&](int a, int b) {
+                double xa = co[3 * (size_t)a + axis], xb = co[3 * (size_t)b + axis];
+                return xa < xb || (xa == xb && a < b);
+            });
+            int q0 = pass == 0 ? 0 : 1, q1 = pass == 0 ? 1 : 5;
+            for (int qi = q0; qi < q1; qi++) {
+                size_t cut = (size_t)(qs[qi] * ord.size());
+                if (cut == 0 || cut >= ord.size()) continue;
+                int64_t sid = st.setid.fetch_add(1);
+                std::vector<int> sep;
+                int side;
+                eval_split(st, ord, cut, sid, sep, side);
+                // smaller separator wins; mild penalty for imbalance
+                double bal = std::fabs(qs[qi] - 0.5);
+                double score = (double)sep.size() * (1.0 + 2.0 * bal);
+                if (score < best_score) {
+                    best_score = score;
+                    best_sep.swap(sep);
+                    best_ord = ord;
+                    best_cut = cut;
+                    best_side = side;
+                    best_axis = axis;
+                }
+            }
+        }
+    }
+    // mark separator, form the two remaining sides
+    int64_t sid = st.setid.fetch_add(1);
+    for (int s : best_sep) st.mark[s] = sid * 2;
+    std::vector<int> a, b;
+    for (size_t i = 0; i < best_ord.size(); i++) {
+        int u = best_ord[i];
+        if (st.mark[u] == sid * 2) continue;
+        (i < best_cut ? a : b).push_back(u);
+    }
+    (void)best_side;
+    nd.cols = best_sep;
+    std::vector<int>().swap(best_ord);
+    int ca = -1, cb = -1;
+    bool par = v.size() > 20000;
+    std::vector<int>().swap(v);
+#pragma omp task shared(ca, st) if (par)
+    { if (!a.empty()) ca = nd_rec(st, std::move(a)); }
+#pragma omp task shared(cb, st) if (par)
+    { if (!b.empty()) cb = nd_rec(st, std::move(b)); }
+#pragma omp taskwait
+    st.nodes[id].child[0] = ca;
+    st.nodes[id].child[1] = cb;
+    return id;
+}
+
+}  // namespace
+
+void analyze(Symbolic &s, const Topology &t, int leaf_size) {
+    s = Symbolic();
+    int n = t.nE;
+    s.n = n;
+    s.leaf_size = leaf_size;
+    // full adjacency without diagonal
+    std::vector<int> deg(n + 1, 0);
+    for (int r = 0; r < n; r++)
+        for (int q = t.ia[r]; q < t.ia[r + 1]; q++)
+            if (t.ja[q] != r) { deg[r]++; deg[t.ja[q]]++; }
+    std::vector<int> xadj(n + 1, 0);
+    for (int r = 0; r < n; r++) xadj[r + 1] = xadj[r] + deg[r];
+    std::vector<int> adj(xadj[n]);
+    std::fill(deg.begin(), deg.end(), 0);
+    for (int r = 0; r < n; r++)
+        for (int q = t.ia[r]; q < t.ia[r + 1]; q++) {
+            int c = t.ja[q];
+            if (c == r) continue;
+            adj[xadj[r] + deg[r]++] = c;
+            adj[xadj[c] + deg[c]++] = r;
+        }
+    std::vector<double> co((size_t)n * 3);
+    for (int e = 0; e < n; e++)
+        for (int k = 0; k < 3; k++)
+            co[3 * (size_t)e + k] = 0.5 * (t.xyz[3 * (size_t)(t.edg_vex[2 * e] - 1) + k] +
+                                           t.xyz[3 * (size_t)(t.edg_vex[2 * e + 1] - 1) + k]);
+    NDState st;
+    st.n = n;
+    st.xadj = &xadj;
+    st.adj = &adj;
+    st.co = &co;
+    st.mark.assign(n, 0);
+    st.leaf = std::max(leaf_size, 4);
+    st.nodes.resize((size_t)(2 * (n / std::max(1, st.leaf / 2)) + 64));
+    std::vector<int> all(n);
+    std::iota(all.begin(), all.end(), 0);
+    int root = -1;
+#pragma omp parallel
+#pragma omp single
+    root = nd_rec(st, std::move(all));
+    if ((size_t)st.nnodes.load() > st.nodes.size()) throw std::runtime_error("nd node overflow");
+    // postorder: assign columns; non-empty nodes become supernodes
+    s.perm.reserve(n);
+    std::vector<int> stack, sn_of_node(st.nnodes.load(), -1);
+    std::vector<int> nd_parent_sn(st.nnodes.load(), -1);
+    // iterative postorder with "nearest non-empty ancestor" tracking
+    struct Fr { int node; int state; int anc; };
+    std::vector<Fr> stk;
+    stk.push_back({root, 0, -1});
+    std::vector<int> sn_parent_node;  // supernode -> nd node of nearest non-empty ancestor
+    std::vector<int> sn_node;
+    std::vector<int> pending_parent;  // per supernode: nearest non-empty ancestor nd node
+    while (!stk.empty()) {
+        Fr &f = stk.back();
+        NDNode &nd = st.nodes[f.node];
+        if (f.state < 2) {
+            int c = nd.child[f.state];
+            int anc = nd.cols.empty() ? f.anc : f.node;
+            f.state++;
+            if (c >= 0) stk.push_back({c, 0, anc});
+            continue;
+        }
+        if (!nd.cols.empty()) {
+            int sid = (int)s.c0.size();
+            s.c0.push_back((int)s.perm.size());
+            s.nc.push_back((int)nd.cols.size());
+            for (int u : nd.cols) s.perm.push_back(u);
+            sn_of_node[f.node] = sid;
+            sn_node.push_back(f.node);
+            pending_parent.push_back(f.anc);
+        }
+        stk.pop_back();
+    }
+    if ((int)s.perm.size() != n) throw std::runtime_error("ordering lost columns");
+    s.nsn = (int)s.c0.size();
+    s.iperm.assign(n, 0);
+    for (int i = 0; i < n; i++) s.iperm[s.perm[i]] = i;
+    s.parent.assign(s.nsn, -1);
+    for (int k = 0; k < s.nsn; k++)
+        s.parent[k] = pending_parent[k] >= 0 ? sn_of_node[pending_parent[k]] : -1;
+    // children lists
+    std::vector<int> nch(s.nsn + 1, 0);
+    for (int k = 0; k < s.nsn; k++)
+        if (s.parent[k] >= 0) nch[s.parent[k] + 1]++;
+    s.children_ptr.assign(s.nsn + 1, 0);
+    for (int k = 0; k < s.nsn; k++) s.children_ptr[k + 1] = s.children_ptr[k] + nch[k + 1];
+    s.children.assign(s.children_ptr[s.nsn], 0);
+    std::vector<int> fillc(s.nsn, 0);
+    for (int k = 0; k < s.nsn; k++)
+        if (s.parent[k] >= 0) {
+            int p = s.parent[k];
+            s.children[s.children_ptr[p] + fillc[p]++] = k;
+        }
+    // row structures (children precede parents in supernode order)
+    std::vector<int> mark(n, -1);
+    s.sptr.assign(s.nsn + 1, 0);
+    std::vector<std::vector<int>> rows(s.nsn);
+    for (int k = 0; k < s.nsn; k++) {
+        int c0 = s.c0[k], c1 = c0 + s.nc[k];
+        auto &rk = rows[k];
+        for (int j = c0; j < c1; j++) {
+            int old = s.perm[j];
+            for (int q = xadj[old]; q < xadj[old + 1]; q++) {
+                int i = s.iperm[adj[q]];
+                if (i >= c1 && mark[i] != k) { mark[i] = k; rk.push_back(i); }
+            }
+        }
+        for (int q = s.children_ptr[k]; q < s.children_ptr[k + 1]; q++) {
+            int ch = s.children[q];
+            for (int i : rows[ch]) {
+                if (i < c0) throw std::runtime_error("child structure below parent columns");
+                if (i >= c1 && mark[i] != k) { mark[i] = k; rk.push_back(i); }
+            }
+        }
+        std::sort(rk.begin(), rk.end());
+        s.sptr[k + 1] = s.sptr[k] + (int64_t)rk.size();
+        if (s.parent[k] >= 0 && !rk.empty() && rk.front() < s.c0[s.parent[k]])
+            throw std::runtime_error("structure not covered by parent");
+        if (s.parent[k] < 0 && !rk.empty()) throw std::runtime_error("root with nonempty structure");
+    }
+    s.sidx.resize(s.sptr[s.nsn]);
+    for (int k = 0; k < s.nsn; k++) std::copy(rows[k].begin(), rows[k].end(), s.sidx.begin() + s.sptr[k]);
+    // storage, levels, statistics
+    s.loff.assign(s.nsn + 1, 0);
+    s.uoff.assign(s.nsn + 1, 0);
+    s.level.assign(s.nsn, 0);
+    s.nnzL = 0;
+    s.flops = 0.0;
+    for (int k = 0; k < s.nsn; k++) {
+        int64_t nc = s.nc[k], m = s.sptr[k + 1] - s.sptr[k], f = nc + m;
+        s.loff[k + 1] = s.loff[k] + f * nc;
+        s.uoff[k + 1] = s.uoff[k] + m * m;
+        s.nnzL += nc * (nc + 1) / 2 + m * nc;
+        for (int64_t j = 0; j < nc; j++) {
+            double c = (double)(f - j - 1);
+            s.flops += c * c + 2.0 * c;
+        }
+        s.max_f = std::max<int>(s.max_f, (int)f);
+        s.max_nc = std::max<int>(s.max_nc, (int)nc);
+        int lv = 0;
+        for (int q = s.children_ptr[k]; q < s.children_ptr[k + 1]; q++)
+            lv = std::max(lv, s.level[s.children[q]] + 1);
+        s.level[k] = lv;
+    }
+    s.ltotal = s.loff[s.nsn];
+    s.utotal = s.uoff[s.nsn];
+    s.nlevels = 0;
+    for (int k = 0; k < s.nsn; k++) s.nlevels = std::max(s.nlevels, s.level[k] + 1);
+    s.lev_ptr.assign(s.nlevels + 1, 0);
+    for (int k = 0; k < s.nsn; k++) s.lev_ptr[s.level[k] + 1]++;
+    for (int l = 0; l < s.nlevels; l++) s.lev_ptr[l + 1] += s.lev_ptr[l];
+    s.lev_sn.assign(s.nsn, 0);
+    std::vector<int> lf(s.nlevels, 0);
+    for (int k = 0; k < s.nsn; k++) s.lev_sn[s.lev_ptr[s.level[k]] + lf[s.level[k]]++] = k;
+    // relative indices of each struct in the parent's front list
+    s.rptr.assign(s.nsn + 1, 0);
+    s.rel.resize(s.sidx.size());
+    for (int k = 0; k < s.nsn; k++) {
+        s.rptr[k + 1] = s.sptr[k + 1];
+        int p = s.parent[k];
+        if (p < 0) continue;
+        int pc0 = s.c0[p], pc1 = pc0 + s.nc[p];
+        const int *ps = s.sidx.data() + s.sptr[p];
+        int64_t pm = s.sptr[p + 1] - s.sptr[p];
+        for (int64_t q = s.sptr[k]; q < s.sptr[k + 1]; q++) {
+            int i = s.sidx[q];
+            int pos;
+            if (i < pc1) pos = i - pc0;
+            else {
+                const int *it = std::lower_bound(ps, ps + pm, i);
+                if (it == ps + pm || *it != i) throw std::runtime_error("relind miss");
+                pos = s.nc[p] + (int)(it - ps);
+            }
+            s.rel[q] = pos;
+        }
+    }
+    // assembled CSR entry -> L storage offset
+    std::vector<int> col2sn(n);
+    for (int k = 0; k < s.nsn; k++)
+        for (int j = s.c0[k]; j < s.c0[k] + s.nc[k]; j++) col2sn[j] = k;
+    s.amap.resize(t.nnz);
+    for (int r = 0; r < n; r++)
+        for (int q = t.ia[r]; q < t.ia[r + 1]; q++) {
+            int pr = s.iperm[r], pc = s.iperm[t.ja[q]];
+            int lo = std::min(pr, pc), hi = std::max(pr, pc);
+            int k = col2sn[lo];
+            int c0 = s.c0[k], c1 = c0 + s.nc[k];
+            int64_t m = s.sptr[k + 1] - s.sptr[k], f = s.nc[k] + m;
+            int64_t row;
+            if (hi < c1) row = hi - c0;
+            else {
+                const int *ps = s.sidx.data() + s.sptr[k];
+                const int *it = std::lower_bound(ps, ps + m, hi);
+                if (it == ps + m || *it != hi) throw std::runtime_error("amap miss");
+                row = s.nc[k] + (it - ps);
+            }
+            s.amap[q] = s.loff[k] + (int64_t)(lo - c0) * f + row;
+        }
+}
+
+}  // namespace tf

@@ -1,0 +1,253 @@
+"""Python host side of the FETD forward path over the libtemfetd C ABI.
+
+`Fetd` wraps one tf_ctx (include/temfetd.h).  `run_root` mirrors the
+reference driver `FETD -f ROOT` (src/FETD_MAIN.f90:110-165): it reads
+ROOT.mesh and ROOT.cntl, assembles, runs the BDF2 time loop on the GPU and
+writes ROOT.field / ROOT.field2 in the reference's format.
+
+There is no CPU fallback: if libtemfetd.so is missing or no GPU is present,
+the constructor raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import io as tfio
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtemfetd.so")
+_lib = None
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_SIGS = {
+    "tf_create": [ctypes.POINTER(_P), _I],
+    "tf_destroy": [_P],
+    "tf_last_error": [_P],
+    "tf_set_mesh": [_P, _I, _P, _I, _P],
+    "tf_get_sizes": [_P, _P, _P],
+    "tf_get_topology": [_P, _P, _P, _P, _P, _P],
+    "tf_set_materials": [_P, _P, _P],
+    "tf_set_source": [_P, _I, _I, _P, _I, _I, _D, _D],
+    "tf_set_receivers": [_P, _I, _P, _D, _P],
+    "tf_set_schedule": [_P, _I, _I, _I, _D],
+    "tf_assemble": [_P],
+    "tf_get_matrices": [_P, _P, _P],
+    "tf_get_receiver_vectors": [_P, _P, _P],
+    "tf_analyze": [_P, _I],
+    "tf_factor_stats": [_P, _P, _P, _P, _P],
+    "tf_factor": [_P, _I, _D],
+    "tf_factor_csr": [_P, _I, _I, _P, _P, _P],
+    "tf_solve": [_P, _I, _I, _P, _P],
+    "tf_release": [_P, _I],
+    "tf_run": [_P, _I, _I, _P, _P, _P, _P, _P],
+    "tf_get_checks": [_P, _I, _P, _P, _P, _P],
+    "tf_get_timers": [_P, _P],
+}
+
+
+def load_library(path=LIB_PATH):
+    """Load libtemfetd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m temf3dt_amd.build`")
+    lib = ctypes.CDLL(path)
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _I
+    lib.tf_destroy.restype = None
+    lib.tf_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+class FetdError(RuntimeError):
+    pass
+
+
+class Fetd:
+    """One FETD model on one GPU (a tf_ctx)."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        self.ctx = _P()
+        rc = self.lib.tf_create(ctypes.byref(self.ctx), device)
+        if rc != 0:
+            raise FetdError(f"tf_create failed ({rc}): no usable GPU device {device}")
+        self.nR = 0
+
+    def close(self):
+        if self.ctx:
+            self.lib.tf_destroy(self.ctx)
+            self.ctx = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.lib.tf_last_error(self.ctx).decode(errors="replace")
+            raise FetdError(f"{what} failed ({rc}): {msg}")
+
+    # -- model_mesh -------------------------------------------------------
+    def set_mesh(self, xyz, tet_vex):
+        self.xyz = np.ascontiguousarray(xyz, np.float64)
+        self.tet_vex = np.ascontiguousarray(tet_vex, np.int32)
+        self._chk(self.lib.tf_set_mesh(self.ctx, len(self.xyz), _ptr(self.xyz), len(self.tet_vex),
+                                       _ptr(self.tet_vex)), "tf_set_mesh")
+        ne, nz = _I(), _I()
+        self._chk(self.lib.tf_get_sizes(self.ctx, ctypes.byref(ne), ctypes.byref(nz)), "tf_get_sizes")
+        self.nE, self.nnz = ne.value, nz.value
+
+    def topology(self):
+        T = len(self.tet_vex)
+        edg = np.empty((self.nE, 2), np.int32)
+        te = np.empty((T, 6), np.int32)
+        td = np.empty((T, 6), np.int32)
+        ia = np.empty(self.nE + 1, np.int32)
+        ja = np.empty(self.nnz, np.int32)
+        self._chk(self.lib.tf_get_topology(self.ctx, _ptr(edg), _ptr(te), _ptr(td), _ptr(ia), _ptr(ja)),
+                  "tf_get_topology")
+        return dict(edg_vex=edg, tet_edg=te, tet_drc=td, ia=ia, ja=ja)
+
+    # -- model_ctl --------------------------------------------------------
+    def set_materials(self, sigma, inv_mu):
+        s = np.ascontiguousarray(sigma, np.float64)
+        m = np.ascontiguousarray(inv_mu, np.float64)
+        self._chk(self.lib.tf_set_materials(self.ctx, _ptr(s), _ptr(m)), "tf_set_materials")
+
+    def set_source(self, seg_vex, source_type, pulse_type, pulse_width, current, isrc=0):
+        sv = np.ascontiguousarray(seg_vex, np.int32).reshape(-1, 2)
+        self._chk(self.lib.tf_set_source(self.ctx, isrc, len(sv), _ptr(sv), source_type, pulse_type,
+                                         pulse_width, current), "tf_set_source")
+
+    def set_receivers(self, xyz, rec_charac=10.0):
+        r = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        rt = np.empty(len(r), np.int32)
+        self._chk(self.lib.tf_set_receivers(self.ctx, len(r), _ptr(r), rec_charac, _ptr(rt)), "tf_set_receivers")
+        self.nR = len(r)
+        return rt
+
+    def set_schedule(self, sp_division, sp_double, sp_dblesize, time_maximum):
+        self._chk(self.lib.tf_set_schedule(self.ctx, sp_division, sp_double, sp_dblesize, time_maximum),
+                  "tf_set_schedule")
+
+    # -- mesh_process -------------------------------------------------------
+    def assemble(self):
+        self._chk(self.lib.tf_assemble(self.ctx), "tf_assemble")
+
+    def matrices(self):
+        M = np.empty(self.nnz)
+        K = np.empty(self.nnz)
+        self._chk(self.lib.tf_get_matrices(self.ctx, _ptr(M), _ptr(K)), "tf_get_matrices")
+        return M, K
+
+    def receiver_vectors(self):
+        n = np.empty((self.nR, 6, 3))
+        d = np.empty((self.nR, 6, 3))
+        self._chk(self.lib.tf_get_receiver_vectors(self.ctx, _ptr(n), _ptr(d)), "tf_get_receiver_vectors")
+        return n, d
+
+    # -- solver seam ----------------------------------------------------------
+    def analyze(self, leaf_size=0):
+        self._chk(self.lib.tf_analyze(self.ctx, leaf_size), "tf_analyze")
+
+    def factor_stats(self):
+        nnzL = ctypes.c_int64()
+        fl = _D()
+        ns, nl = _I(), _I()
+        self._chk(self.lib.tf_factor_stats(self.ctx, ctypes.byref(nnzL), ctypes.byref(fl), ctypes.byref(ns),
+                                           ctypes.byref(nl)), "tf_factor_stats")
+        return dict(nnzL=nnzL.value, flops=fl.value, nsuper=ns.value, nlevels=nl.value)
+
+    def factor(self, slot, dt):
+        self._chk(self.lib.tf_factor(self.ctx, slot, dt), "tf_factor")
+
+    def factor_csr(self, slot, ia, ja, va):
+        ia = np.ascontiguousarray(ia, np.int32)
+        ja = np.ascontiguousarray(ja, np.int32)
+        va = np.ascontiguousarray(va, np.float64)
+        self._chk(self.lib.tf_factor_csr(self.ctx, slot, len(ia) - 1, _ptr(ia), _ptr(ja), _ptr(va)),
+                  "tf_factor_csr")
+
+    def solve(self, slot, b):
+        b = np.ascontiguousarray(b, np.float64)
+        nrhs = 1 if b.ndim == 1 else b.shape[0]
+        x = np.empty_like(b)
+        self._chk(self.lib.tf_solve(self.ctx, slot, nrhs, _ptr(b), _ptr(x)), "tf_solve")
+        return x
+
+    def release(self, slot):
+        self._chk(self.lib.tf_release(self.ctx, slot), "tf_release")
+
+    # -- time_solver ---------------------------------------------------------
+    def run(self, max_steps=0, max_rows=200000):
+        n = _I()
+        steps = np.empty(max_rows, np.int32)
+        t = np.empty(max_rows)
+        dt = np.empty(max_rows)
+        f = np.empty((max_rows, self.nR, 9))
+        self._chk(self.lib.tf_run(self.ctx, max_steps, max_rows, ctypes.byref(n), _ptr(steps), _ptr(t), _ptr(dt),
+                                  _ptr(f)), "tf_run")
+        k = min(n.value, max_rows)
+        return dict(step=steps[:k].copy(), t=t[:k].copy(), dt=dt[:k].copy(), fields=f[:k].copy())
+
+    def checks(self):
+        n = _I()
+        self._chk(self.lib.tf_get_checks(self.ctx, 0, ctypes.byref(n), None, None, None), "tf_get_checks")
+        st = np.empty(n.value, np.int32)
+        er = np.empty(n.value)
+        kp = np.empty(n.value, np.int32)
+        self._chk(self.lib.tf_get_checks(self.ctx, n.value, ctypes.byref(n), _ptr(st), _ptr(er), _ptr(kp)),
+                  "tf_get_checks")
+        return dict(step=st, err=er, keep=kp)
+
+    def timers(self):
+        t = np.empty(8)
+        self._chk(self.lib.tf_get_timers(self.ctx, _ptr(t)), "tf_get_timers")
+        keys = ["assemble", "factor", "solve", "rhs", "readout", "loop", "analyze", "nfactor"]
+        return dict(zip(keys, t.tolist()))
+
+
+def setup_from_files(mesh_path, cntl_path, device=0):
+    """model_mesh + model_ctl + mesh_process from reference input files."""
+    m = tfio.read_mesh(mesh_path)
+    c = tfio.read_cntl(cntl_path)
+    if c["sp_mthd"] != 2:
+        raise FetdError("error!")  # only the adaptive BDF2 method exists (src/FETD_FWD.f90:1210-1219)
+    if c["bound_cond"] != 0:
+        raise FetdError("absorbing boundary condition is not supported")
+    sigma, inv_mu = tfio.materials(m["tet_phy"], c["domains"])
+    fe = Fetd(device)
+    fe.set_mesh(m["xyz"], m["tet_vex"])
+    fe.set_materials(sigma, inv_mu)
+    fe.set_source(m["seg_vex"], c["source_type"], c["pulse_type"], c["pulse_width"], c["current"])
+    fe.set_receivers(c["receivers"], c["rec_charac"])
+    fe.set_schedule(c["sp_division"], c["sp_double"], c["sp_dblesize"], c["time_maximum"])
+    fe.assemble()
+    return fe, m, c
+
+
+def run_root(root, device=0, write=True, max_steps=0):
+    """`FETD -f ROOT`: returns the rows and writes ROOT.field / ROOT.field2."""
+    fe, m, c = setup_from_files(root + ".mesh", root + ".cntl", device)
+    rows = fe.run(max_steps=max_steps)
+    if write:
+        tfio.write_field(root + ".field", rows["step"], rows["t"], rows["fields"], c["output"])
+        if c["time_points"]:
+            tfio.write_field2(root + ".field2", rows["t"], rows["fields"], c["time_points"], c["output"])
+    rows["timers"] = fe.timers()
+    fe.close()
+    return rows

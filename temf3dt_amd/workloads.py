@@ -1,0 +1,148 @@
+"""Synthetic models for tests and benchmarks (tensor grids split into Kuhn
+tets, square loops along grid lines).
+
+Coordinates follow the reference frame: z positive downward, air at z < 0.
+Vertex ids are renumbered so that every transmitter loop's vertices come
+first, in path order: each loop is then a chain of MEDIT edges with
+ascending vertex ids closed by one descending segment -- the orientation
+the reference's source signs assume (+1 for every segment but the last,
+src/FETD_FWD.f90:446-448).
+"""
+import dataclasses
+import math
+
+import numpy as np
+
+from . import meshgen
+
+AIR, EARTH = 11, 12
+
+
+def axis(lo, hi, fine_lo, fine_hi, h, growth):
+    """Grid lines: uniform spacing h on [fine_lo, fine_hi], geometric
+    growth outward to the domain ends lo, hi (last cell absorbed)."""
+    core = list(np.arange(fine_lo, fine_hi + 0.5 * h, h))
+    left, x, step = [], fine_lo, h
+    while True:
+        step *= growth
+        x -= step
+        if x <= lo + 0.3 * step:
+            left.append(lo)
+            break
+        left.append(x)
+    right, x, step = [], fine_hi, h
+    while True:
+        step *= growth
+        x += step
+        if x >= hi - 0.3 * step:
+            right.append(hi)
+            break
+        right.append(x)
+    return np.array(sorted(left) + core + right, dtype=np.float64)
+
+
+@dataclasses.dataclass
+class Model:
+    xyz: np.ndarray
+    tet_vex: np.ndarray
+    tet_phy: np.ndarray
+    loops: list               # list of [nseg,2] int32 arrays (1-based vertex ids)
+    receivers: np.ndarray     # [R,3]
+    domains: list             # [(phys, sigma, eps_r, mu_r)]
+    source: tuple = (1, 1, 1.0e-7, 1.0)   # source_type, pulse_type, t0, current
+    sp_division: int = 10
+    time_maximum: float = 1e-3
+    sp_double: int = 100
+    sp_dblesize: int = 2
+    time_points: list = None
+    rec_charac: float = 10.0
+
+    @property
+    def ntet(self):
+        return len(self.tet_vex)
+
+    def sigma_inv_mu(self):
+        sig = np.empty(self.ntet)
+        imu = np.empty(self.ntet)
+        for ph, s, _e, mr in self.domains:
+            m = self.tet_phy == ph
+            sig[m] = s
+            imu[m] = 1.0 / (mr * (math.pi * 4.0e-7))
+        return sig, imu
+
+    def write(self, root, loop=0):
+        """ROOT.mesh + ROOT.cntl for transmitter `loop` (reference format)."""
+        meshgen.write_medit(root + ".mesh", self.xyz, self.tet_vex, self.tet_phy, self.loops[loop])
+        meshgen.write_cntl(root + ".cntl", self.receivers, self.domains, self.source, self.sp_division,
+                           self.time_maximum, self.sp_double, self.sp_dblesize, time_points=self.time_points,
+                           rec_charac=self.rec_charac)
+
+
+def build(xs, ys, zs, loops, receivers, blocks=(), host_sigma=0.01, air_sigma=1e-12, **kw):
+    """Tensor grid with an air half-space (z<0), an earth half-space and
+    rectangular conductive blocks [(x0,x1,y0,y1,z0,z1,sigma), ...]."""
+    xyz, tets, hexid = meshgen.tensor_mesh(xs, ys, zs)
+    cx = 0.5 * (xs[hexid[:, 0]] + xs[hexid[:, 0] + 1])
+    cy = 0.5 * (ys[hexid[:, 1]] + ys[hexid[:, 1] + 1])
+    cz = 0.5 * (zs[hexid[:, 2]] + zs[hexid[:, 2] + 1])
+    phy = np.where(cz < 0, AIR, EARTH).astype(np.int32)
+    domains = [(AIR, air_sigma, 1.0, 1.0), (EARTH, host_sigma, 1.0, 1.0)]
+    for bi, (x0, x1, y0, y1, z0, z1, s) in enumerate(blocks):
+        m = (cx > x0) & (cx < x1) & (cy > y0) & (cy < y1) & (cz > z0) & (cz < z1)
+        phy[m] = 13 + bi
+        domains.append((13 + bi, s, 1.0, 1.0))
+    # loop vertex chains in grid-node ids
+    chains = []
+    for (x0, x1, y0, y1, z) in loops:
+        segs = meshgen.square_loop(xs, ys, zs, x0, x1, y0, y1, z)
+        chains.append(segs[:, 0].copy())  # path order
+    # renumber: loop vertices first, in path order
+    N = len(xyz)
+    newid = np.zeros(N + 1, np.int64)
+    nxt = 1
+    for ch in chains:
+        for v in ch:
+            if newid[v] == 0:
+                newid[v] = nxt
+                nxt += 1
+    rest = np.nonzero(newid[1:] == 0)[0] + 1
+    newid[rest] = np.arange(nxt, nxt + len(rest))
+    order = np.empty(N, np.int64)
+    order[newid[1:] - 1] = np.arange(N)
+    xyz_n = xyz[order]
+    tets_n = newid[tets].astype(np.int32)
+    loop_segs = []
+    for ch in chains:
+        ids = newid[ch]
+        segs = np.stack([ids, np.roll(ids, -1)], axis=1).astype(np.int32)
+        loop_segs.append(segs)
+    return Model(xyz=xyz_n, tet_vex=tets_n, tet_phy=phy, loops=loop_segs, receivers=np.asarray(receivers, float),
+                 domains=domains, **kw)
+
+
+def small_model():
+    """A few-thousand-tet half-space with a 100 m loop (tests, smoke)."""
+    xs = axis(-2000, 2000, -100, 100, 50.0, 2.0)
+    ys = axis(-2000, 2000, -100, 100, 50.0, 2.0)
+    zs = axis(-2000, 2000, -50, 100, 50.0, 2.0)
+    return build(xs, ys, zs, loops=[(-50, 50, -50, 50, 0.0)], receivers=[[0.0, 0.0, -3.0], [10.0, 5.0, 7.0]],
+                 host_sigma=0.01, air_sigma=1e-8, source=(1, 1, 1.0e-6, 1.0), sp_division=10,
+                 time_maximum=1e-3, sp_double=12, sp_dblesize=2, rec_charac=100.0)
+
+
+def plain_3d_model(scale=1.0):
+    """3-D plain large-loop model: 0.01 S/m half-space, 1 S/m block of
+    200 x 200 x 100 m, top at 100 m depth, offset 150 m from the centre of
+    an 800 m square loop on the ground; 41 receivers every 20 m, 3 m above
+    ground; 1e-12 S/m air in a 100 km cube.  About 1.1M tets at scale 1."""
+    h = 50.0 / scale
+    hz = 20.0 / scale
+    xs = axis(-50000, 50000, -450, 450, h, 1.4)
+    ys = axis(-50000, 50000, -450, 450, h, 1.4)
+    zs = axis(-50000, 50000, -40, 300, hz, 1.4)
+    rec = [[-400.0 + 20.0 * i, 0.0, -3.0] for i in range(41)]
+    gates = list(np.logspace(-5, -2, 40))
+    return build(xs, ys, zs, loops=[(-400, 400, -400, 400, 0.0)], receivers=rec,
+                 blocks=[(50, 250, -100, 100, 100, 200, 1.0)], host_sigma=0.01, air_sigma=1e-12,
+                 source=(1, 1, 1.0e-6, 1.0), sp_division=10, time_maximum=1.0e-2, sp_double=100, sp_dblesize=2,
+                 time_points=gates, rec_charac=60.0)
