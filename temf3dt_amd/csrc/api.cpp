@@ -40,24 +40,33 @@ void launch_factor_fused(double *L, double *U, const FrontDesc *fd, const int *l
 void launch_factor_diag(double *L, const FrontDesc *fd, const int *list, int cnt, int p0, double *Dn, int *flag,
                         hipStream_t st);
 void launch_factor_trsm(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
-                        int p0, hipStream_t st);
+                        int p0, const double *Dn, hipStream_t st);
 void launch_factor_update(double *L, double *U, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
                           int64_t nwg, int p0, const double *Dn, hipStream_t st);
+// factor.hip (inversion)
+void launch_inv_w(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
+                  int p0, double *W, const int64_t *woff, hipStream_t st);
+void launch_inv_apply(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
+                      int p0, const double *W, const int64_t *woff, hipStream_t st);
+void launch_inv_diag(double *L, const FrontDesc *fd, const int *list, int cnt, int p0, hipStream_t st);
 // solve.hip
-void launch_fwd_fused(const double *L, const FrontDesc *fd, const int *list, int cnt, const double *b,
-                      const int *perm, const int *chp, const int *ch, const int *rel, double *V, hipStream_t st);
+void launch_fwd_small(const double *L, const FrontDesc *fd, const int *list, int cnt, const double *b,
+                      const int *perm, const int *chp, const int *ch, const int *rel, double *V, double *Y,
+                      hipStream_t st);
+void launch_bwd_small(const double *L, const FrontDesc *fd, const int *list, int cnt, const double *Y,
+                      const double *Dn, const int *sidx, double *xw, hipStream_t st);
 void launch_fwd_gather(const FrontDesc *fd, const int *list, int cnt, const double *b, const int *perm,
                        const int *chp, const int *ch, const int *rel, double *V, hipStream_t st);
-void launch_fwd_diag(const double *L, const FrontDesc *fd, const int *list, int cnt, int p0, double *V,
-                     hipStream_t st);
-void launch_fwd_update(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
-                       int64_t nwg, int p0, double *V, hipStream_t st);
-void launch_bwd_fused(const double *L, const FrontDesc *fd, const int *list, int cnt, const double *V,
-                      const double *Dn, const int *sidx, double *xw, hipStream_t st);
-void launch_bwd_partial(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
-                        int64_t nwg, int p0, const double *xw, const int *sidx, double *partial, hipStream_t st);
-void launch_bwd_diag(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int p0,
-                     const double *V, const double *Dn, const double *partial, double *xw, hipStream_t st);
+void launch_fwd_own(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
+                    int64_t nwg, const double *V, double *Y, hipStream_t st);
+void launch_fwd_struct(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
+                       int64_t nwg, double *V, const double *Y, hipStream_t st);
+void launch_bwd_gather(const FrontDesc *fd, const int *list, int cnt, const int *sidx, const double *xw, double *V,
+                       hipStream_t st);
+void launch_bwd_t(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
+                  const double *Y, const double *Dn, const double *V, double *T, hipStream_t st);
+void launch_bwd_x(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
+                  const double *T, double *xw, hipStream_t st);
 void launch_unpermute(int n, const int *perm, const double *xw, double *x, hipStream_t st);
 // step.hip
 void launch_spmv_rhs(int n, const int *ia, const int *ja, const double *val, const double *e2, const double *e3,
@@ -102,9 +111,9 @@ void dfree(T *&p) {
 }
 
 enum LaunchKind {
-    L_EXTADD, L_FFUSED, L_FDIAG, L_FTRSM, L_FUPD,
-    L_SFUSED, L_SGATHER, L_SDIAG, L_SUPD,
-    L_BFUSED, L_BPART, L_BDIAG
+    L_EXTADD, L_FFUSED, L_FDIAG, L_FTRSM, L_FUPD, L_IW, L_IAPPLY, L_IDIAG,
+    L_SSMALL, L_SGATHER, L_SOWN, L_SSTRUCT,
+    L_BSMALL, L_BGATHER, L_BT, L_BX
 };
 struct Launch {
     int kind, p0, off, cnt, poff;
@@ -116,7 +125,8 @@ struct Plan {
     std::vector<Launch> factor, fwd, bwd;
     std::vector<int> list;
     std::vector<int64_t> pref;
-    int64_t partial_max = 0;
+    std::vector<int64_t> woff;  // per supernode: offset in the inversion scratch
+    int64_t wsize = 0;
 };
 
 static const double PI = 3.141592653589793;
@@ -185,7 +195,8 @@ struct tf_ctx {
     int *d_perm = nullptr, *d_parent = nullptr, *d_chp = nullptr, *d_ch = nullptr, *d_rel = nullptr,
         *d_sidx = nullptr, *d_plist = nullptr;
     int64_t *d_amap = nullptr, *d_ppref = nullptr;
-    double *d_U = nullptr, *d_V = nullptr, *d_xw = nullptr, *d_partial = nullptr;
+    double *d_U = nullptr, *d_V = nullptr, *d_xw = nullptr, *d_Y = nullptr, *d_T = nullptr, *d_W = nullptr;
+    int64_t *d_woff = nullptr;
     double *d_L[2] = {nullptr, nullptr}, *d_D[2] = {nullptr, nullptr};
     bool slot_live[2] = {false, false};
     int *d_flag = nullptr;
@@ -210,7 +221,7 @@ namespace {
 void free_solver(tf_ctx *c) {
     dfree(c->d_fd); dfree(c->d_perm); dfree(c->d_parent); dfree(c->d_chp); dfree(c->d_ch); dfree(c->d_rel);
     dfree(c->d_sidx); dfree(c->d_plist); dfree(c->d_amap); dfree(c->d_ppref); dfree(c->d_U); dfree(c->d_V);
-    dfree(c->d_xw); dfree(c->d_partial);
+    dfree(c->d_xw); dfree(c->d_Y); dfree(c->d_T); dfree(c->d_W); dfree(c->d_woff);
     for (int s = 0; s < 2; s++) { dfree(c->d_L[s]); dfree(c->d_D[s]); c->slot_live[s] = false; }
     dfree(c->d_tmpb); dfree(c->d_tmpx);
     c->analyzed = false;
@@ -269,18 +280,17 @@ void build_plan(tf_ctx *c) {
         }
         p.pref.push_back(acc);
         L.nwg = acc;
-        if (!fr.empty()) dst.push_back(L);
+        if (!fr.empty() && acc > 0) dst.push_back(L);
     };
-    const int FUSE_COUNT = 256, BIG_F = 384;
-    std::vector<std::vector<int>> fused(s.nlevels), grid(s.nlevels);
-    for (int l = 0; l < s.nlevels; l++) {
-        int cnt = s.lev_ptr[l + 1] - s.lev_ptr[l];
+    // small fronts (one workgroup each) vs large fronts (grid launches)
+    std::vector<std::vector<int>> small(s.nlevels), grid(s.nlevels);
+    for (int l = 0; l < s.nlevels; l++)
         for (int q = s.lev_ptr[l]; q < s.lev_ptr[l + 1]; q++) {
             int k = s.lev_sn[q];
-            if (cnt >= FUSE_COUNT || f_of(k) <= BIG_F) fused[l].push_back(k);
+            if (s.nc[k] <= TF_PB && f_of(k) <= 512) small[l].push_back(k);
             else grid[l].push_back(k);
         }
-    }
+    p.woff.assign(s.nsn, 0);
     // factorisation
     for (int l = 0; l < s.nlevels; l++) {
         int maxch = 0;
@@ -300,9 +310,15 @@ void build_plan(tf_ctx *c) {
             }
             add(p.factor, L_EXTADD, 0, fr, un);
         }
-        add(p.factor, L_FFUSED, 0, fused[l], {});
+        add(p.factor, L_FFUSED, 0, small[l], {});
         int kmax = 0;
-        for (int k : grid[l]) kmax = std::max(kmax, (s.nc[k] + TF_PB - 1) / TF_PB);
+        int64_t wlev = 0;
+        for (int k : grid[l]) {
+            kmax = std::max(kmax, (s.nc[k] + TF_PB - 1) / TF_PB);
+            p.woff[k] = wlev;
+            wlev += (int64_t)s.nc[k] * TF_PB;
+        }
+        p.wsize = std::max(p.wsize, wlev);
         for (int pk = 0; pk < kmax; pk++) {
             int p0 = pk * TF_PB;
             std::vector<int> act;
@@ -313,38 +329,14 @@ void build_plan(tf_ctx *c) {
                     int rows = f_of(k) - p1;
                     act.push_back(k);
                     ut.push_back((rows + TF_TRSM_ROWS - 1) / TF_TRSM_ROWS);
-                    int64_t nt = (rows + TF_TB - 1) / TF_TB;
+                    int64_t nt = (rows + 127) / 128;  // 128 x 128 matrix-core update tiles
                     uu.push_back(nt * (nt + 1) / 2);
                 }
             add(p.factor, L_FDIAG, p0, act, {});
             add(p.factor, L_FTRSM, p0, act, ut);
             add(p.factor, L_FUPD, p0, act, uu);
         }
-    }
-    // forward solve, bottom-up
-    for (int l = 0; l < s.nlevels; l++) {
-        add(p.fwd, L_SFUSED, 0, fused[l], {});
-        add(p.fwd, L_SGATHER, 0, grid[l], {});
-        int kmax = 0;
-        for (int k : grid[l]) kmax = std::max(kmax, (s.nc[k] + TF_PB - 1) / TF_PB);
-        for (int pk = 0; pk < kmax; pk++) {
-            int p0 = pk * TF_PB;
-            std::vector<int> act;
-            std::vector<int64_t> ur;
-            for (int k : grid[l])
-                if (s.nc[k] > p0) {
-                    int p1 = std::min(s.nc[k], p0 + TF_PB);
-                    act.push_back(k);
-                    ur.push_back((f_of(k) - p1 + TF_NT - 1) / TF_NT);
-                }
-            add(p.fwd, L_SDIAG, p0, act, {});
-            add(p.fwd, L_SUPD, p0, act, ur);
-        }
-    }
-    // backward solve, top-down
-    for (int l = s.nlevels - 1; l >= 0; l--) {
-        int kmax = 0;
-        for (int k : grid[l]) kmax = std::max(kmax, (s.nc[k] + TF_PB - 1) / TF_PB);
+        // in-place inversion of every large front's L11, right to left
         for (int pk = kmax - 1; pk >= 0; pk--) {
             int p0 = pk * TF_PB;
             std::vector<int> act;
@@ -353,17 +345,33 @@ void build_plan(tf_ctx *c) {
                 if (s.nc[k] > p0) {
                     int p1 = std::min(s.nc[k], p0 + TF_PB);
                     act.push_back(k);
-                    ur.push_back((f_of(k) - p1 + TF_NT - 1) / TF_NT);
+                    ur.push_back((s.nc[k] - p1 + TF_TB - 1) / TF_TB);
                 }
-            // the diag launch reuses the partial launch's list/pref
-            if (act.empty()) continue;
-            add(p.bwd, L_BPART, p0, act, ur);
-            Launch d = p.bwd.back();
-            d.kind = L_BDIAG;
-            p.bwd.push_back(d);
-            p.partial_max = std::max(p.partial_max, d.nwg);
+            add(p.factor, L_IW, p0, act, ur);
+            add(p.factor, L_IAPPLY, p0, act, ur);
+            add(p.factor, L_IDIAG, p0, act, {});
         }
-        add(p.bwd, L_BFUSED, 0, fused[l], {});
+    }
+    // forward solve, bottom-up
+    for (int l = 0; l < s.nlevels; l++) {
+        add(p.fwd, L_SSMALL, 0, small[l], {});
+        std::vector<int64_t> uo, us;
+        for (int k : grid[l]) {
+            uo.push_back((s.nc[k] + 63) / 64);
+            us.push_back((m_of(k) + 63) / 64);
+        }
+        add(p.fwd, L_SGATHER, 0, grid[l], {});
+        add(p.fwd, L_SOWN, 0, grid[l], uo);
+        add(p.fwd, L_SSTRUCT, 0, grid[l], us);
+    }
+    // backward solve, top-down
+    for (int l = s.nlevels - 1; l >= 0; l--) {
+        std::vector<int64_t> uc;
+        for (int k : grid[l]) uc.push_back((s.nc[k] + 15) / 16);
+        add(p.bwd, L_BGATHER, 0, grid[l], {});
+        add(p.bwd, L_BT, 0, grid[l], uc);
+        add(p.bwd, L_BX, 0, grid[l], uc);
+        add(p.bwd, L_BSMALL, 0, small[l], {});
     }
 }
 
@@ -401,7 +409,10 @@ void do_analyze(tf_ctx *c, int leaf) {
     c->d_U = dalloc<double>(s.utotal);
     c->d_V = dalloc<double>(voff);
     c->d_xw = dalloc<double>(s.n);
-    c->d_partial = dalloc<double>((size_t)std::max<int64_t>(c->plan.partial_max, 1) * TF_PB);
+    c->d_Y = dalloc<double>(s.n);
+    c->d_T = dalloc<double>(s.n);
+    c->d_W = dalloc<double>((size_t)std::max<int64_t>(c->plan.wsize, 1));
+    c->d_woff = dupload(c->plan.woff);
     c->d_tmpb = dalloc<double>(s.n);
     c->d_tmpx = dalloc<double>(s.n);
     if (!c->d_flag) c->d_flag = dalloc<int>(1);
@@ -428,10 +439,19 @@ void run_factor(tf_ctx *c, int slot) {
             tf::launch_factor_diag(L, c->d_fd, lst + l.off, l.cnt, l.p0, Dn, c->d_flag, c->st);
             break;
         case L_FTRSM:
-            tf::launch_factor_trsm(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, c->st);
+            tf::launch_factor_trsm(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, Dn, c->st);
             break;
         case L_FUPD:
             tf::launch_factor_update(L, c->d_U, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, Dn, c->st);
+            break;
+        case L_IW:
+            tf::launch_inv_w(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, c->d_W, c->d_woff, c->st);
+            break;
+        case L_IAPPLY:
+            tf::launch_inv_apply(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, c->d_W, c->d_woff, c->st);
+            break;
+        case L_IDIAG:
+            tf::launch_inv_diag(L, c->d_fd, lst + l.off, l.cnt, l.p0, c->st);
             break;
         }
     }
@@ -468,34 +488,35 @@ void solve_dev(tf_ctx *c, int slot, const double *b, double *x) {
     const int64_t *pr = c->d_ppref;
     for (const Launch &l : p.fwd) {
         switch (l.kind) {
-        case L_SFUSED:
-            tf::launch_fwd_fused(L, c->d_fd, lst + l.off, l.cnt, b, c->d_perm, c->d_chp, c->d_ch, c->d_rel, c->d_V,
-                                 c->st);
+        case L_SSMALL:
+            tf::launch_fwd_small(L, c->d_fd, lst + l.off, l.cnt, b, c->d_perm, c->d_chp, c->d_ch, c->d_rel, c->d_V,
+                                 c->d_Y, c->st);
             break;
         case L_SGATHER:
             tf::launch_fwd_gather(c->d_fd, lst + l.off, l.cnt, b, c->d_perm, c->d_chp, c->d_ch, c->d_rel, c->d_V,
                                   c->st);
             break;
-        case L_SDIAG:
-            tf::launch_fwd_diag(L, c->d_fd, lst + l.off, l.cnt, l.p0, c->d_V, c->st);
+        case L_SOWN:
+            tf::launch_fwd_own(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_V, c->d_Y, c->st);
             break;
-        case L_SUPD:
-            tf::launch_fwd_update(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, c->d_V, c->st);
+        case L_SSTRUCT:
+            tf::launch_fwd_struct(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_V, c->d_Y, c->st);
             break;
         }
     }
     for (const Launch &l : p.bwd) {
         switch (l.kind) {
-        case L_BFUSED:
-            tf::launch_bwd_fused(L, c->d_fd, lst + l.off, l.cnt, c->d_V, Dn, c->d_sidx, c->d_xw, c->st);
+        case L_BSMALL:
+            tf::launch_bwd_small(L, c->d_fd, lst + l.off, l.cnt, c->d_Y, Dn, c->d_sidx, c->d_xw, c->st);
             break;
-        case L_BPART:
-            tf::launch_bwd_partial(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, c->d_xw, c->d_sidx,
-                                   c->d_partial, c->st);
+        case L_BGATHER:
+            tf::launch_bwd_gather(c->d_fd, lst + l.off, l.cnt, c->d_sidx, c->d_xw, c->d_V, c->st);
             break;
-        case L_BDIAG:
-            tf::launch_bwd_diag(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.p0, c->d_V, Dn, c->d_partial,
-                                c->d_xw, c->st);
+        case L_BT:
+            tf::launch_bwd_t(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_Y, Dn, c->d_V, c->d_T, c->st);
+            break;
+        case L_BX:
+            tf::launch_bwd_x(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_T, c->d_xw, c->st);
             break;
         }
     }

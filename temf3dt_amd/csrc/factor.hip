@@ -14,6 +14,37 @@
 
 #define LDS_LD 65
 
+typedef double tf_d4 __attribute__((ext_vector_type(4)));
+#define MLD 80  // LDS leading dimension of 64-wide operand panels (bank-conflict padding)
+
+// 64 x 64 tile product on FP64 matrix cores, four waves of 32 x 32:
+//   acc(row, col) += sum_k Rop[k*ldr + row] * Cop[k*ldc + col],  k < kend
+// (kend a multiple of 4, operands zero-padded).  v_mfma_f64_16x16x4f64
+// lane layout (measured): lane l supplies A[l%16][l/16], B[l/16][l%16] and
+// receives D[4r + l/16][l%16] in result register r.  The column operand is
+// passed as A so that a result register's fast index is the row:
+//   acc[bi][bj][r] -> row 32*wr + 16*bi + (l & 15), col 32*wc + 16*bj + 4r + (l >> 4).
+__device__ __forceinline__ void mfma64_acc(const double *Rop, int ldr, const double *Cop, int ldc, int kend,
+                                           tf_d4 acc[2][2]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1;
+    const int l16 = lane & 15, lk = lane >> 4;
+    for (int k0 = 0; k0 < kend; k0 += 4) {
+        const int kk = k0 + lk;
+        const double x0 = Cop[kk * ldc + 32 * wc + l16], x1 = Cop[kk * ldc + 32 * wc + 16 + l16];
+        const double y0 = Rop[kk * ldr + 32 * wr + l16], y1 = Rop[kk * ldr + 32 * wr + 16 + l16];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, y0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, y0, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, y1, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, y1, acc[1][1], 0, 0, 0);
+    }
+}
+__device__ __forceinline__ void acc_zero(tf_d4 acc[2][2]) {
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) acc[i][j] = (tf_d4){0.0, 0.0, 0.0, 0.0};
+}
+#define ACC_ROW(bi) (32 * ((threadIdx.x >> 6) >> 1) + 16 * (bi) + (threadIdx.x & 15))
+#define ACC_COL(bj, r) (32 * ((threadIdx.x >> 6) & 1) + 16 * (bj) + 4 * (r) + ((threadIdx.x & 63) >> 4))
+
 // ---- workgroup-level building blocks ------------------------------------
 
 // Factor the pw x pw diagonal block of panel p0 in LDS: A = L D L^T with
@@ -51,33 +82,48 @@ __device__ void blk_diag_factor(double *L, const FrontDesc &d, int p0, double *D
     __syncthreads();
 }
 
-// Panel solve for rows [r0, r0+TF_TRSM_ROWS): L21 = A21 L11^-T D^-1, one
-// thread per row with the row held in registers.
-__device__ void blk_trsm(double *L, const FrontDesc &d, int p0, int r0, double *lds) {
+// Panel solve for rows [r0, r0+TF_TRSM_ROWS): L21 = A21 L11^-T D^-1,
+// computed as (A21 Xpp^T) D^-1 with Xpp = L11^-1 inverted in LDS, 64 rows
+// at a time with 4x4 register tiles.
+__device__ void blk_invert_to_lds(const double *L, const FrontDesc &d, int p0, double *lds);
+
+__device__ void blk_trsm(double *L, const FrontDesc &d, int p0, int r0, const double *Dn, double *lds) {
     const int pw = min(TF_PB, d.nc - p0);
     double *base = L + d.loff;
-    for (int idx = threadIdx.x; idx < pw * pw; idx += blockDim.x) {
-        int i = idx % pw, j = idx / pw;
-        lds[j * LDS_LD + i] = (i >= j) ? base[(int64_t)(p0 + j) * d.f + p0 + i] : 0.0;
-    }
+    double *X = lds;                     // Xpp [c*65 + r]; unit diagonal set below
+    double *At = lds + TF_PB * LDS_LD;   // [k][MLD rows]
+    blk_invert_to_lds(L, d, p0, X);
+    for (int j = threadIdx.x; j < TF_PB; j += blockDim.x) X[j * LDS_LD + j] = (j < pw) ? 1.0 : 0.0;
     __syncthreads();
-    const int r = r0 + threadIdx.x;
-    const bool act = r < d.f;
-    double a[TF_PB];
+    const int kend = (pw + 3) & ~3;
+    for (int sb = 0; sb < TF_TRSM_ROWS; sb += TF_TB) {
+        const int R0 = r0 + sb;
+        if (R0 >= d.f) break;
+#pragma unroll 8
+        for (int idx = threadIdx.x; idx < TF_PB * TF_TB; idx += TF_NT) {
+            int k = idx / TF_TB, r = idx % TF_TB;
+            At[k * MLD + r] = (k < pw && R0 + r < d.f) ? base[(int64_t)(p0 + k) * d.f + R0 + r] : 0.0;
+        }
+        __syncthreads();
+        tf_d4 acc[2][2];
+        acc_zero(acc);
+        // out[row][c] = sum_k A21[row][k] Xpp[c][k];  Xpp[c][k] = X[k*65 + c]
+        mfma64_acc(At, MLD, X, LDS_LD, kend, acc);
 #pragma unroll
-    for (int j = 0; j < TF_PB; j++) a[j] = (act && j < pw) ? base[(int64_t)(p0 + j) * d.f + r] : 0.0;
+        for (int bj = 0; bj < 2; bj++)
 #pragma unroll
-    for (int j = 0; j < TF_PB; j++) {
-        const double yj = a[j];
+            for (int r = 0; r < 4; r++) {
+                const int c = ACC_COL(bj, r);
+                if (c >= pw) continue;
+                const double dc = Dn[d.c0 + p0 + c];
 #pragma unroll
-        for (int i = j + 1; i < TF_PB; i++) a[i] -= lds[j * LDS_LD + i] * yj;
+                for (int bi = 0; bi < 2; bi++) {
+                    const int row = R0 + ACC_ROW(bi);
+                    if (row < d.f) base[(int64_t)(p0 + c) * d.f + row] = acc[bi][bj][r] / dc;
+                }
+            }
+        __syncthreads();
     }
-    if (act) {
-#pragma unroll
-        for (int j = 0; j < TF_PB; j++)
-            if (j < pw) base[(int64_t)(p0 + j) * d.f + r] = a[j] / lds[j * LDS_LD + j];
-    }
-    __syncthreads();
 }
 
 // Trailing update of tile (ti, tj), ti >= tj, of the rows/columns after the
@@ -88,9 +134,10 @@ __device__ void blk_update_tile(double *L, double *U, const FrontDesc &d, int p0
     const int p1 = p0 + pw;
     const int rb = p1 + ti * TF_TB, cb = p1 + tj * TF_TB;
     const double *base = L + d.loff;
-    double *A = lds;               // [k][64]
-    double *B = lds + TF_PB * TF_TB;  // [k][64], scaled by d_k
-    for (int idx = threadIdx.x; idx < TF_PB * TF_TB; idx += blockDim.x) {
+    double *A = lds;               // [k][MLD]: panel rows of the tile rows
+    double *B = lds + TF_PB * MLD; // [k][MLD]: panel rows of the tile columns, times d_k
+#pragma unroll 8
+    for (int idx = threadIdx.x; idx < TF_PB * TF_TB; idx += TF_NT) {
         int k = idx / TF_TB, i = idx % TF_TB;
         double va = 0.0, vb = 0.0;
         if (k < pw) {
@@ -98,38 +145,243 @@ __device__ void blk_update_tile(double *L, double *U, const FrontDesc &d, int p0
             if (rb + i < d.f) va = col[rb + i];
             if (cb + i < d.f) vb = col[cb + i] * Dn[d.c0 + p0 + k];
         }
-        A[idx] = va;
-        B[idx] = vb;
+        A[k * MLD + i] = va;
+        B[k * MLD + i] = vb;
     }
     __syncthreads();
-    const int tr = threadIdx.x & 15, tc = threadIdx.x >> 4;
-    double acc[4][4];
+    tf_d4 acc[2][2];
+    acc_zero(acc);
+    mfma64_acc(A, MLD, B, MLD, (pw + 3) & ~3, acc);
 #pragma unroll
-    for (int u = 0; u < 4; u++)
+    for (int bj = 0; bj < 2; bj++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) acc[u][v] = 0.0;
-    for (int k = 0; k < pw; k++) {
-        double a4[4], b4[4];
+        for (int r = 0; r < 4; r++) {
+            const int c = cb + ACC_COL(bj, r);
+            if (c >= d.f) continue;
+            double *col = front_col(L, U, d, c);
 #pragma unroll
-        for (int u = 0; u < 4; u++) a4[u] = A[k * TF_TB + tr + 16 * u];
+            for (int bi = 0; bi < 2; bi++) {
+                const int row = rb + ACC_ROW(bi);
+                if (row < d.f && row >= c) col[row] -= acc[bi][bj][r];
+            }
+        }
+    __syncthreads();
+}
+
+// Trailing update with FP64 matrix cores: C tile of TF_UT x TF_UT (rows rb..,
+// columns cb.. of the trailing block) -= L_rows D L_cols^T over the panel's
+// pw columns.  Four waves, each a 64 x 64 sub-tile of 4 x 4 blocks of
+// v_mfma_f64_16x16x4f64.  Lane layout of that instruction (measured): lane l
+// supplies A[l%16][l/16] and B[l/16][l%16] and receives D[4r + l/16][l%16]
+// in result register r.  Operand roles are chosen so that the result's
+// fast index is the tile row: lane l holds row (l & 15) of columns
+// 4r + (l >> 4), so every store covers four 128-byte row segments.
+#define TF_UT 128
+#define TF_ULD (TF_UT + 16)
+
+__device__ void blk_update_tile_mfma(double *L, double *U, const FrontDesc &d, int p0, const double *Dn,
+                                     int ti, int tj, double *lds) {
+    const int pw = min(TF_PB, d.nc - p0);
+    const int p1 = p0 + pw;
+    const int rb = p1 + ti * TF_UT, cb = p1 + tj * TF_UT;
+    const double *base = L + d.loff;
+    const int KH = TF_PB / 2;            // panel columns staged per half
+    double *Ap = lds;                    // [k][TF_ULD]: panel values of the tile rows
+    double *Bp = lds + KH * TF_ULD;      // [k][TF_ULD]: panel values of the tile columns, times d_k
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    const int l16 = lane & 15, lk = lane >> 4;
+    tf_d4 acc[4][4];
 #pragma unroll
-        for (int v = 0; v < 4; v++) b4[v] = B[k * TF_TB + tc + 16 * v];
+    for (int bi = 0; bi < 4; bi++)
 #pragma unroll
-        for (int u = 0; u < 4; u++)
+        for (int bj = 0; bj < 4; bj++) acc[bi][bj] = (tf_d4){0.0, 0.0, 0.0, 0.0};
+    for (int kh = 0; kh < TF_PB; kh += KH) {
+        // stage KH panel columns in two batches of 8 loads per operand per thread
+        constexpr int PER = KH * TF_UT / TF_NT;  // 16
 #pragma unroll
-            for (int v = 0; v < 4; v++) acc[u][v] += a4[u] * b4[v];
+        for (int half = 0; half < PER; half += 8) {
+            double va[8], vb[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int idx = (half + q) * TF_NT + threadIdx.x;
+                const int k = kh + idx / TF_UT, i = idx % TF_UT;
+                va[q] = 0.0;
+                vb[q] = 0.0;
+                if (k < pw) {
+                    const double *col = base + (int64_t)(p0 + k) * d.f;
+                    if (rb + i < d.f) va[q] = col[rb + i];
+                    if (cb + i < d.f) vb[q] = col[cb + i];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int idx = (half + q) * TF_NT + threadIdx.x;
+                const int kk = idx / TF_UT, i = idx % TF_UT;
+                const int k = kh + kk;
+                Ap[kk * TF_ULD + i] = va[q];
+                Bp[kk * TF_ULD + i] = (k < pw) ? vb[q] * Dn[d.c0 + p0 + k] : 0.0;
+            }
+        }
+        __syncthreads();
+        const int kend = min(KH, ((pw - kh) + 3) & ~3);
+        for (int k0 = 0; k0 < kend; k0 += 4) {
+            const int kk = k0 + lk;
+            double x[4], y[4];
+#pragma unroll
+            for (int bj = 0; bj < 4; bj++) x[bj] = Bp[kk * TF_ULD + 64 * wc + 16 * bj + l16];
+#pragma unroll
+            for (int bi = 0; bi < 4; bi++) y[bi] = Ap[kk * TF_ULD + 64 * wr + 16 * bi + l16];
+#pragma unroll
+            for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+                for (int bj = 0; bj < 4; bj++)
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[bj], y[bi], acc[bi][bj], 0, 0, 0);
+        }
+        __syncthreads();
     }
+    // epilogue: per result column block, load all C values first, then store
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        const int c = cb + tc + 16 * v;
-        if (c >= d.f) continue;
-        double *col = front_col(const_cast<double *>(L), U, d, c);
+    for (int bj = 0; bj < 4; bj++) {
+        double cv[4][4];
+        double *cp[4];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int r = rb + tr + 16 * u;
-            if (r < d.f && r >= c) col[r] -= acc[u][v];
+        for (int r = 0; r < 4; r++) {
+            const int c = cb + 64 * wc + 16 * bj + 4 * r + lk;
+            cp[r] = (c < d.f) ? front_col(L, U, d, c) : nullptr;
+#pragma unroll
+            for (int bi = 0; bi < 4; bi++) {
+                const int row = rb + 64 * wr + 16 * bi + l16;
+                cv[r][bi] = (cp[r] && row < d.f && row >= c) ? cp[r][row] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int c = cb + 64 * wc + 16 * bj + 4 * r + lk;
+#pragma unroll
+            for (int bi = 0; bi < 4; bi++) {
+                const int row = rb + 64 * wr + 16 * bi + l16;
+                if (cp[r] && row < d.f && row >= c) cp[r][row] = cv[r][bi] - acc[bi][bj][r];
+            }
         }
     }
+}
+
+// Invert the unit-lower pw x pw diagonal block of panel p0 into LDS
+// (column-major, ld 65, strict lower part; the diagonal is implicit).
+// Column j of X = L^-1 follows from columns > j:
+//   X[i][j] = -L[i][j] - sum_{j<k<i} X[i][k] L[k][j].
+__device__ void blk_invert_to_lds(const double *L, const FrontDesc &d, int p0, double *lds) {
+    const int pw = min(TF_PB, d.nc - p0);
+    const double *base = L + d.loff;
+    for (int idx = threadIdx.x; idx < TF_PB * TF_PB; idx += blockDim.x) {
+        int i = idx % TF_PB, j = idx / TF_PB;
+        lds[j * LDS_LD + i] = (i > j && i < pw && j < pw) ? base[(int64_t)(p0 + j) * d.f + p0 + i] : 0.0;
+    }
+    __syncthreads();
+    for (int j = pw - 2; j >= 0; j--) {
+        const int i = j + 1 + threadIdx.x;
+        double v = 0.0;
+        if (i < pw) {
+            double s = lds[j * LDS_LD + i];
+            for (int k = j + 1; k < i; k++) s += lds[k * LDS_LD + i] * lds[j * LDS_LD + k];
+            v = -s;
+        }
+        __syncthreads();
+        if (i < pw) lds[j * LDS_LD + i] = v;
+        __syncthreads();
+    }
+}
+
+__device__ void blk_invert_diag(double *L, const FrontDesc &d, int p0, double *lds) {
+    const int pw = min(TF_PB, d.nc - p0);
+    blk_invert_to_lds(L, d, p0, lds);
+    double *base = L + d.loff;
+    for (int idx = threadIdx.x; idx < pw * pw; idx += blockDim.x) {
+        int i = idx % pw, j = idx / pw;
+        if (i > j) base[(int64_t)(p0 + j) * d.f + p0 + i] = lds[j * LDS_LD + i];
+    }
+    __syncthreads();
+}
+
+// Blocked in-place inversion of L11, right to left by panels.  For panel
+// p (columns [p0,p1)) with the trailing block already inverted (X22):
+//   W   = X22 L21p        (rows [p1,nc))            k_inv_w
+//   X21 = -W Xpp, Xpp = Lpp^-1                        k_inv_apply
+//   Lpp <- Xpp                                        k_inv_diag
+__device__ void blk_inv_w(const double *L, const FrontDesc &d, int p0, int R0, double *W, double *lds) {
+    const int pw = min(TF_PB, d.nc - p0);
+    const int p1 = p0 + pw;
+    const int nq = d.nc - p1;
+    const double *base = L + d.loff;
+    double *Xt = lds;                // [k][MLD rows]: X22[R0+r][k0+k]
+    double *Lt = lds + TF_PB * MLD;  // [k][MLD cols]: L21p[k0+k][c]
+    tf_d4 acc[2][2];
+    acc_zero(acc);
+    for (int k0 = p1; k0 <= R0; k0 += TF_PB) {
+#pragma unroll 8
+        for (int idx = threadIdx.x; idx < TF_PB * TF_TB; idx += TF_NT) {
+            int kk = idx / TF_TB, r = idx % TF_TB;
+            int k = k0 + kk, row = R0 + r;
+            double xv = 0.0, lv = 0.0;
+            if (k < d.nc && row < d.nc && k < row) xv = base[(int64_t)k * d.f + row];
+            if (k < d.nc && r < pw) lv = base[(int64_t)(p0 + r) * d.f + k];
+            Xt[kk * MLD + r] = xv;
+            Lt[kk * MLD + r] = lv;
+        }
+        __syncthreads();
+        mfma64_acc(Xt, MLD, Lt, MLD, TF_PB, acc);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int c = ACC_COL(bj, r);
+            if (c >= pw) continue;
+#pragma unroll
+            for (int bi = 0; bi < 2; bi++) {
+                const int row = R0 + ACC_ROW(bi);
+                if (row < d.nc)
+                    W[(int64_t)c * nq + (row - p1)] = base[(int64_t)(p0 + c) * d.f + row] + acc[bi][bj][r];
+            }
+        }
+}
+
+__device__ void blk_inv_apply(double *L, const FrontDesc &d, int p0, int R0, const double *W, double *lds) {
+    const int pw = min(TF_PB, d.nc - p0);
+    const int p1 = p0 + pw;
+    const int nq = d.nc - p1;
+    double *X = lds;                              // Xpp [c*65 + r]
+    double *XT = lds + TF_PB * LDS_LD;            // XT[k*MLD + c] = Xpp[k][c], unit diagonal
+    double *Wt = XT + TF_PB * MLD;                // [k][MLD rows]
+    blk_invert_to_lds(L, d, p0, X);
+    for (int idx = threadIdx.x; idx < TF_PB * TF_PB; idx += blockDim.x) {
+        int k = idx / TF_PB, c = idx % TF_PB;
+        XT[k * MLD + c] = (k == c) ? (k < pw ? 1.0 : 0.0) : X[c * LDS_LD + k];
+    }
+    for (int idx = threadIdx.x; idx < TF_PB * TF_TB; idx += blockDim.x) {
+        int k = idx / TF_TB, r = idx % TF_TB;
+        int row = R0 + r;
+        Wt[k * MLD + r] = (k < pw && row < d.nc) ? W[(int64_t)k * nq + (row - p1)] : 0.0;
+    }
+    __syncthreads();
+    tf_d4 acc[2][2];
+    acc_zero(acc);
+    mfma64_acc(Wt, MLD, XT, MLD, (pw + 3) & ~3, acc);
+    double *base = L + d.loff;
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int c = ACC_COL(bj, r);
+            if (c >= pw) continue;
+#pragma unroll
+            for (int bi = 0; bi < 2; bi++) {
+                const int row = R0 + ACC_ROW(bi);
+                if (row < d.nc) base[(int64_t)(p0 + c) * d.f + row] = -acc[bi][bj][r];
+            }
+        }
     __syncthreads();
 }
 
@@ -170,14 +422,15 @@ __global__ void __launch_bounds__(TF_NT) k_extend_add(double *L, double *U, cons
     for (int a = b + threadIdx.x; a < dc.m; a += blockDim.x) dst[rc[a]] += uc[a];
 }
 
+// One workgroup per small front (nc <= 64): factor, update, invert L11.
 __global__ void __launch_bounds__(TF_NT) k_factor_fused(double *L, double *U, const FrontDesc *fd,
                                                         const int *list, double *Dn, int *flag) {
-    __shared__ double lds[2 * TF_PB * TF_TB + 64];
+    __shared__ double lds[2 * TF_PB * MLD + 64];
     const FrontDesc d = fd[list[blockIdx.x]];
     for (int p0 = 0; p0 < d.nc; p0 += TF_PB) {
         blk_diag_factor(L, d, p0, Dn, flag, lds);
         const int p1 = min(d.nc, p0 + TF_PB);
-        for (int r0 = p1; r0 < d.f; r0 += TF_TRSM_ROWS) blk_trsm(L, d, p0, r0, lds);
+        for (int r0 = p1; r0 < d.f; r0 += TF_TRSM_ROWS) blk_trsm(L, d, p0, r0, Dn, lds);
         const int nt = (d.f - p1 + TF_TB - 1) / TF_TB;
         const int ntile = nt * (nt + 1) / 2;
         for (int t = 0; t < ntile; t++) {
@@ -186,6 +439,7 @@ __global__ void __launch_bounds__(TF_NT) k_factor_fused(double *L, double *U, co
             blk_update_tile(L, U, d, p0, Dn, ti, tj, lds);
         }
     }
+    blk_invert_diag(L, d, 0, lds);
 }
 
 __global__ void __launch_bounds__(TF_NT) k_factor_diag(double *L, const FrontDesc *fd, const int *list,
@@ -196,24 +450,52 @@ __global__ void __launch_bounds__(TF_NT) k_factor_diag(double *L, const FrontDes
 }
 
 __global__ void __launch_bounds__(TF_NT) k_factor_trsm(double *L, const FrontDesc *fd, const int *list,
-                                                       const int64_t *pref, int cnt, int p0) {
-    __shared__ double lds[TF_PB * LDS_LD];
+                                                       const int64_t *pref, int cnt, int p0, const double *Dn) {
+    __shared__ double lds[TF_PB * LDS_LD + TF_PB * MLD];
     const int i = seg_find(pref, cnt, blockIdx.x);
     const FrontDesc d = fd[list[i]];
     const int p1 = min(d.nc, p0 + TF_PB);
     const int r0 = p1 + (int)(blockIdx.x - pref[i]) * TF_TRSM_ROWS;
-    blk_trsm(L, d, p0, r0, lds);
+    blk_trsm(L, d, p0, r0, Dn, lds);
 }
 
 __global__ void __launch_bounds__(TF_NT) k_factor_update(double *L, double *U, const FrontDesc *fd,
                                                          const int *list, const int64_t *pref,
                                                          int cnt, int p0, const double *Dn) {
-    __shared__ double lds[2 * TF_PB * TF_TB];
+    __shared__ double lds[TF_PB * TF_ULD];
     const int i = seg_find(pref, cnt, blockIdx.x);
     const FrontDesc d = fd[list[i]];
     int ti, tj;
     tri_index((int64_t)blockIdx.x - pref[i], ti, tj);
-    blk_update_tile(L, U, d, p0, Dn, ti, tj, lds);
+    blk_update_tile_mfma(L, U, d, p0, Dn, ti, tj, lds);
+}
+
+__global__ void __launch_bounds__(TF_NT) k_inv_w(const double *L, const FrontDesc *fd, const int *list,
+                                                 const int64_t *pref, int cnt, int p0, double *W,
+                                                 const int64_t *woff) {
+    __shared__ double lds[2 * TF_PB * MLD];
+    const int i = seg_find(pref, cnt, blockIdx.x);
+    const int s = list[i];
+    const FrontDesc d = fd[s];
+    const int p1 = min(d.nc, p0 + TF_PB);
+    blk_inv_w(L, d, p0, p1 + (int)(blockIdx.x - pref[i]) * TF_TB, W + woff[s], lds);
+}
+
+__global__ void __launch_bounds__(TF_NT) k_inv_apply(double *L, const FrontDesc *fd, const int *list,
+                                                     const int64_t *pref, int cnt, int p0, const double *W,
+                                                     const int64_t *woff) {
+    __shared__ double lds[TF_PB * LDS_LD + 2 * TF_PB * MLD];
+    const int i = seg_find(pref, cnt, blockIdx.x);
+    const int s = list[i];
+    const FrontDesc d = fd[s];
+    const int p1 = min(d.nc, p0 + TF_PB);
+    blk_inv_apply(L, d, p0, p1 + (int)(blockIdx.x - pref[i]) * TF_TB, W + woff[s], lds);
+}
+
+__global__ void __launch_bounds__(TF_NT) k_inv_diag(double *L, const FrontDesc *fd, const int *list, int p0) {
+    __shared__ double lds[TF_PB * LDS_LD];
+    const FrontDesc d = fd[list[blockIdx.x]];
+    blk_invert_diag(L, d, p0, lds);
 }
 
 // ---- host launchers --------------------------------------------------------
@@ -251,10 +533,10 @@ void launch_factor_diag(double *L, const FrontDesc *fd, const int *list, int cnt
     hipLaunchKernelGGL(k_factor_diag, dim3(cnt), dim3(TF_NT), 0, st, L, fd, list, p0, Dn, flag);
 }
 void launch_factor_trsm(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
-                        int64_t nwg, int p0, hipStream_t st) {
+                        int64_t nwg, int p0, const double *Dn, hipStream_t st) {
     if (nwg <= 0) return;
     hipLaunchKernelGGL(k_factor_trsm, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt,
-                       p0);
+                       p0, Dn);
 }
 void launch_factor_update(double *L, double *U, const FrontDesc *fd, const int *list,
                           const int64_t *pref, int cnt, int64_t nwg, int p0, const double *Dn,
@@ -262,6 +544,21 @@ void launch_factor_update(double *L, double *U, const FrontDesc *fd, const int *
     if (nwg <= 0) return;
     hipLaunchKernelGGL(k_factor_update, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, U, fd, list, pref,
                        cnt, p0, Dn);
+}
+
+void launch_inv_w(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
+                  int64_t nwg, int p0, double *W, const int64_t *woff, hipStream_t st) {
+    if (nwg > 0)
+        hipLaunchKernelGGL(k_inv_w, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt, p0, W, woff);
+}
+void launch_inv_apply(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
+                      int64_t nwg, int p0, const double *W, const int64_t *woff, hipStream_t st) {
+    if (nwg > 0)
+        hipLaunchKernelGGL(k_inv_apply, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt, p0, W,
+                           woff);
+}
+void launch_inv_diag(double *L, const FrontDesc *fd, const int *list, int cnt, int p0, hipStream_t st) {
+    if (cnt > 0) hipLaunchKernelGGL(k_inv_diag, dim3(cnt), dim3(TF_NT), 0, st, L, fd, list, p0);
 }
 
 }  // namespace tf

@@ -2,21 +2,25 @@
 // multifrontal LDL^T factors (FP64), one right-hand side.
 //
 // Replaces PARDISO phase 33 (pardiso_direct_2, src/FETD_UTILITY.f90:938-957;
-// called per time step at src/FETD_FWD.f90:1334,1371,1431).  The forward
-// pass runs bottom-up over the front tree and hands every front's update
-// vector to its parent (extend-add, no atomics, so results do not depend on
-// scheduling); the backward pass runs top-down and reads the already solved
-// rows of ancestor fronts.  Input b and output x are in edge order; the
-// permutation is applied while gathering/scattering.
+// called per time step at src/FETD_FWD.f90:1334,1371,1431).
+//
+// Every front's unit-lower diagonal block L11 is stored inverted (X11 =
+// L11^-1, computed once per factorisation), so each front's solve is a pair
+// of matrix-vector products instead of a sequential substitution:
+//   forward  (bottom-up):  v = [b_own; 0] + children's update vectors
+//                          y_own = X11 v_own,  v_struct -= L21 y_own
+//   backward (top-down):   t = D^-1 y_own - L21^T x_struct,  x_own = X11^T t
+// The forward pass hands every front's update vector to its parent
+// (extend-add without atomics, so results do not depend on scheduling).
+// Vectors Y (forward result) and T are kept in the factor's column order;
+// b and x are in edge order.
 #include "tf_device.h"
-
-#define LDS_LD 65
 
 // ---- forward ------------------------------------------------------------------
 
 // vf = [b(own columns); 0] + extend-add of the children's update vectors
-__device__ void blk_fwd_gather(const FrontDesc *fd, int s, const double *b, const int *perm,
-                               const int *chp, const int *ch, const int *rel, double *V) {
+__device__ void blk_fwd_gather(const FrontDesc *fd, int s, const double *b, const int *perm, const int *chp,
+                               const int *ch, const int *rel, double *V) {
     const FrontDesc d = fd[s];
     double *vf = V + d.voff;
     for (int j = threadIdx.x; j < d.f; j += blockDim.x) vf[j] = (j < d.nc) ? b[perm[d.c0 + j]] : 0.0;
@@ -30,115 +34,116 @@ __device__ void blk_fwd_gather(const FrontDesc *fd, int s, const double *b, cons
     }
 }
 
-// unit-lower solve of the panel's diagonal block
-__device__ void blk_fwd_diag(const double *L, const FrontDesc &d, int p0, double *V, double *xs) {
-    const int pw = min(TF_PB, d.nc - p0);
-    const double *base = L + d.loff;
-    double *vf = V + d.voff;
-    for (int i = threadIdx.x; i < pw; i += blockDim.x) xs[i] = vf[p0 + i];
-    __syncthreads();
-    for (int j = 0; j < pw; j++) {
-        const double xj = xs[j];
-        const double *col = base + (int64_t)(p0 + j) * d.f + p0;
-        for (int i = j + 1 + threadIdx.x; i < pw; i += blockDim.x) xs[i] -= col[i] * xj;
-        __syncthreads();
+// y[r] (rows r0..r0+63 of the front) = sum_j A[r][j] x[j] over columns
+// [0, jend(r)), A column-major with leading dimension f: 64 rows x 4 column
+// phases per workgroup, phases reduced through LDS.
+__device__ double blk_gemv_rows(const double *A, int f, int r, int jend, const double *x, double *red) {
+    const int ph = threadIdx.x >> 6;
+    double acc = 0.0;
+    int j = ph;
+    for (; j + 12 < jend; j += 16) {
+        double a0 = A[(int64_t)j * f + r], a1 = A[(int64_t)(j + 4) * f + r];
+        double a2 = A[(int64_t)(j + 8) * f + r], a3 = A[(int64_t)(j + 12) * f + r];
+        acc += a0 * x[j] + a1 * x[j + 4] + a2 * x[j + 8] + a3 * x[j + 12];
     }
-    for (int i = threadIdx.x; i < pw; i += blockDim.x) vf[p0 + i] = xs[i];
+    for (; j < jend; j += 4) acc += A[(int64_t)j * f + r] * x[j];
+    red[threadIdx.x] = acc;
     __syncthreads();
+    double s = 0.0;
+    if (ph == 0) s = red[threadIdx.x] + red[threadIdx.x + 64] + red[threadIdx.x + 128] + red[threadIdx.x + 192];
+    __syncthreads();
+    return s;
 }
 
-// vf[r] -= L[r, panel] * vf[panel] for rows [r0, r0+NT)
-__device__ void blk_fwd_update(const double *L, const FrontDesc &d, int p0, int r0, double *V,
-                               double *xs) {
-    const int pw = min(TF_PB, d.nc - p0);
-    const double *base = L + d.loff;
-    double *vf = V + d.voff;
-    for (int i = threadIdx.x; i < pw; i += blockDim.x) xs[i] = vf[p0 + i];
-    __syncthreads();
-    const int r = r0 + threadIdx.x;
-    if (r < d.f) {
-        double acc = 0.0;
-        for (int j = 0; j < pw; j++) acc += base[(int64_t)(p0 + j) * d.f + r] * xs[j];
-        vf[r] -= acc;
-    }
-    __syncthreads();
-}
+// ---- kernels: small fronts (nc <= 64), one workgroup per front ------------------
 
-// ---- backward ----------------------------------------------------------------
-
-__device__ __forceinline__ double xrow(const FrontDesc &d, int r, const double *xw, const int *sidx) {
-    return (r < d.nc) ? xw[d.c0 + r] : xw[sidx[d.soff + r - d.nc]];
-}
-
-// partial[j] = sum over rows [r0, r0+NT) of L[r, p0+j] * x(r)
-__device__ void blk_bwd_partial(const double *L, const FrontDesc &d, int p0, int r0, const double *xw,
-                                const int *sidx, double *out, double *lds) {
-    const int pw = min(TF_PB, d.nc - p0);
-    const int p1 = p0 + pw;
-    const double *base = L + d.loff;
-    double *xr = lds;  // [TF_NT]
-    const int r = r0 + threadIdx.x;
-    xr[threadIdx.x] = (r < d.f && r >= p1) ? xrow(d, r, xw, sidx) : 0.0;
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nw = blockDim.x >> 6;
-    for (int j = wave; j < pw; j += nw) {
-        const double *col = base + (int64_t)(p0 + j) * d.f;
-        double acc = 0.0;
-        for (int t = lane; t < TF_NT; t += 64) {
-            int rr = r0 + t;
-            if (rr < d.f && rr >= p1) acc += col[rr] * xr[t];
-        }
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-        if (lane == 0) out[j] = acc;
-    }
-    __syncthreads();
-}
-
-// x[panel] = L11^-T (D^-1 y[panel] - sum of partials)
-__device__ void blk_bwd_diag(const double *L, const FrontDesc &d, int p0, const double *V,
-                             const double *Dn, const double *partial, int nparts, double *xw,
-                             double *lds) {
-    const int pw = min(TF_PB, d.nc - p0);
-    const double *base = L + d.loff;
-    const double *vf = V + d.voff;
-    double *L11 = lds;                  // [64][65]
-    double *t = lds + TF_PB * LDS_LD;   // [64]
-    for (int idx = threadIdx.x; idx < pw * pw; idx += blockDim.x) {
-        int i = idx % pw, j = idx / pw;
-        L11[j * LDS_LD + i] = (i > j) ? base[(int64_t)(p0 + j) * d.f + p0 + i] : 0.0;
-    }
-    for (int j = threadIdx.x; j < pw; j += blockDim.x) {
-        double s = vf[p0 + j] / Dn[d.c0 + p0 + j];
-        for (int q = 0; q < nparts; q++) s -= partial[(int64_t)q * TF_PB + j];
-        t[j] = s;
-    }
-    __syncthreads();
-    // L11^T x = t, reverse column sweep: x_j final, then t_i -= L[j][i] x_j
-    for (int j = pw - 1; j >= 0; j--) {
-        const double xj = t[j];
-        for (int i = threadIdx.x; i < j; i += blockDim.x) t[i] -= L11[i * LDS_LD + j] * xj;
-        __syncthreads();
-    }
-    for (int j = threadIdx.x; j < pw; j += blockDim.x) xw[d.c0 + p0 + j] = t[j];
-    __syncthreads();
-}
-
-// ---- kernels -------------------------------------------------------------------
-
-__global__ void __launch_bounds__(TF_NT) k_fwd_fused(const double *L, const FrontDesc *fd, const int *list,
+__global__ void __launch_bounds__(TF_NT) k_fwd_small(const double *L, const FrontDesc *fd, const int *list,
                                                      const double *b, const int *perm, const int *chp,
-                                                     const int *ch, const int *rel, double *V) {
+                                                     const int *ch, const int *rel, double *V, double *Y) {
     __shared__ double xs[TF_PB];
+    __shared__ double red[TF_NT];
     const int s = list[blockIdx.x];
     blk_fwd_gather(fd, s, b, perm, chp, ch, rel, V);
     const FrontDesc d = fd[s];
-    for (int p0 = 0; p0 < d.nc; p0 += TF_PB) {
-        blk_fwd_diag(L, d, p0, V, xs);
-        const int p1 = min(d.nc, p0 + TF_PB);
-        for (int r0 = p1; r0 < d.f; r0 += TF_NT) blk_fwd_update(L, d, p0, r0, V, xs);
+    const double *A = L + d.loff;
+    double *vf = V + d.voff;
+    for (int i = threadIdx.x; i < d.nc; i += blockDim.x) xs[i] = vf[i];
+    __syncthreads();
+    // y_own = X11 v_own (unit lower): rows < nc
+    {
+        const int r = threadIdx.x & 63;
+        double acc = blk_gemv_rows(A, d.f, r, r < d.nc ? r : 0, xs, red);
+        if (threadIdx.x < d.nc) {
+            double y = xs[r] + acc;
+            vf[r] = y;
+            Y[d.c0 + r] = y;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < d.nc; i += blockDim.x) xs[i] = vf[i];
+    __syncthreads();
+    // v_struct -= L21 y_own
+    for (int r0 = d.nc; r0 < d.f; r0 += 64) {
+        const int r = r0 + (threadIdx.x & 63);
+        const int rr = r < d.f ? r : d.f - 1;
+        double acc = blk_gemv_rows(A, d.f, rr, d.nc, xs, red);
+        if (threadIdx.x < 64 && r < d.f) vf[r] -= acc;
     }
 }
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// t = D^-1 y_own - L21^T x_struct ; x_own = X11^T t, for small fronts
+__global__ void __launch_bounds__(TF_NT) k_bwd_small(const double *L, const FrontDesc *fd, const int *list,
+                                                     const double *Y, const double *Dn, const int *sidx,
+                                                     double *xw) {
+    __shared__ double ws[TF_NT];
+    __shared__ double ts[TF_PB];
+    const FrontDesc d = fd[list[blockIdx.x]];
+    const double *A = L + d.loff;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double acc[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) acc[k] = 0.0;
+    for (int r0 = d.nc; r0 < d.f; r0 += TF_NT) {
+        const int r = r0 + threadIdx.x;
+        ws[threadIdx.x] = (r < d.f) ? xw[sidx[d.soff + r - d.nc]] : 0.0;
+        __syncthreads();
+        for (int c = 0; c < 4; c++) {
+            const int rr = r0 + 64 * c + lane;
+            if (rr < d.f) {
+                const double w = ws[64 * c + lane];
+#pragma unroll
+                for (int k = 0; k < 16; k++) {
+                    const int j = wave * 16 + k;
+                    if (j < d.nc) acc[k] += A[(int64_t)j * d.f + rr] * w;
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        double s = wave_sum(acc[k]);
+        const int j = wave * 16 + k;
+        if (lane == 0 && j < d.nc) ts[j] = Y[d.c0 + j] / Dn[d.c0 + j] - s;
+    }
+    __syncthreads();
+    // x_j = t_j + sum_{i>j} X[i][j] t_i
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const int j = wave * 16 + k;
+        double v = 0.0;
+        if (j < d.nc && lane > j && lane < d.nc) v = A[(int64_t)j * d.f + lane] * ts[lane];
+        v = wave_sum(v);
+        if (lane == 0 && j < d.nc) xw[d.c0 + j] = ts[j] + v;
+    }
+}
+
+// ---- kernels: large fronts, several workgroups per front --------------------------
 
 __global__ void __launch_bounds__(TF_NT) k_fwd_gather(const FrontDesc *fd, const int *list, const double *b,
                                                       const int *perm, const int *chp, const int *ch,
@@ -146,77 +151,85 @@ __global__ void __launch_bounds__(TF_NT) k_fwd_gather(const FrontDesc *fd, const
     blk_fwd_gather(fd, list[blockIdx.x], b, perm, chp, ch, rel, V);
 }
 
-__global__ void __launch_bounds__(TF_NT) k_fwd_diag(const double *L, const FrontDesc *fd, const int *list,
-                                                    int p0, double *V) {
-    __shared__ double xs[TF_PB];
-    const FrontDesc d = fd[list[blockIdx.x]];
-    blk_fwd_diag(L, d, p0, V, xs);
-}
-
-__global__ void __launch_bounds__(TF_NT) k_fwd_update(const double *L, const FrontDesc *fd, const int *list,
-                                                      const int64_t *pref, int cnt, int p0, double *V) {
-    __shared__ double xs[TF_PB];
+// y_own rows [64w, 64w+64): y = v + sum_{j<r} X[r][j] v[j]
+__global__ void __launch_bounds__(TF_NT) k_fwd_own(const double *L, const FrontDesc *fd, const int *list,
+                                                   const int64_t *pref, int cnt, const double *V, double *Y) {
+    __shared__ double red[TF_NT];
     const int i = seg_find(pref, cnt, blockIdx.x);
     const FrontDesc d = fd[list[i]];
-    const int p1 = min(d.nc, p0 + TF_PB);
-    blk_fwd_update(L, d, p0, p1 + (int)(blockIdx.x - pref[i]) * TF_NT, V, xs);
+    const int r = (int)(blockIdx.x - pref[i]) * 64 + (threadIdx.x & 63);
+    const int rr = r < d.nc ? r : d.nc - 1;
+    const double *vf = V + d.voff;
+    double acc = blk_gemv_rows(L + d.loff, d.f, rr, rr, vf, red);
+    if (threadIdx.x < 64 && r < d.nc) Y[d.c0 + r] = vf[r] + acc;
 }
 
-__global__ void __launch_bounds__(TF_NT) k_bwd_fused(const double *L, const FrontDesc *fd, const int *list,
-                                                     const double *V, const double *Dn, const int *sidx,
-                                                     double *xw) {
-    __shared__ double lds[TF_PB * LDS_LD + TF_PB];
-    __shared__ double xr[TF_NT];
-    __shared__ double part[TF_PB];
+// v_struct rows: v[nc + r] -= sum_j L21[r][j] y_own[j]
+__global__ void __launch_bounds__(TF_NT) k_fwd_struct(const double *L, const FrontDesc *fd, const int *list,
+                                                      const int64_t *pref, int cnt, double *V, const double *Y) {
+    __shared__ double red[TF_NT];
+    const int i = seg_find(pref, cnt, blockIdx.x);
+    const FrontDesc d = fd[list[i]];
+    const int r = d.nc + (int)(blockIdx.x - pref[i]) * 64 + (threadIdx.x & 63);
+    const int rr = r < d.f ? r : d.f - 1;
+    double acc = blk_gemv_rows(L + d.loff, d.f, rr, d.nc, Y + d.c0, red);
+    if (threadIdx.x < 64 && r < d.f) V[d.voff + r] -= acc;
+}
+
+// x_struct of every large front gathered into its (now free) update-vector
+// slot of V, so the column sweeps below read it contiguously.
+__global__ void __launch_bounds__(TF_NT) k_bwd_gather(const FrontDesc *fd, const int *list, const int *sidx,
+                                                      const double *xw, double *V) {
     const FrontDesc d = fd[list[blockIdx.x]];
-    const int npan = (d.nc + TF_PB - 1) / TF_PB;
-    for (int pk = npan - 1; pk >= 0; pk--) {
-        const int p0 = pk * TF_PB;
-        const int pw = min(TF_PB, d.nc - p0);
-        const int p1 = p0 + pw;
-        for (int j = threadIdx.x; j < TF_PB; j += blockDim.x) part[j] = 0.0;
-        __syncthreads();
-        for (int r0 = p1; r0 < d.f; r0 += TF_NT) {
-            // partial sums for this row chunk, accumulated in `part`
-            const int r = r0 + threadIdx.x;
-            xr[threadIdx.x] = (r < d.f) ? xrow(d, r, xw, sidx) : 0.0;
-            __syncthreads();
-            const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-            const int nw = blockDim.x >> 6;
-            for (int j = wave; j < pw; j += nw) {
-                const double *col = L + d.loff + (int64_t)(p0 + j) * d.f;
-                double acc = 0.0;
-                for (int t = lane; t < TF_NT; t += 64) {
-                    int rr = r0 + t;
-                    if (rr < d.f) acc += col[rr] * xr[t];
-                }
-                for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-                if (lane == 0) part[j] += acc;
-            }
-            __syncthreads();
-        }
-        blk_bwd_diag(L, d, p0, V, Dn, part, 1, xw, lds);
+    for (int r = threadIdx.x; r < d.m; r += blockDim.x) V[d.voff + d.nc + r] = xw[sidx[d.soff + r]];
+}
+
+// dot(col[lo:hi], x[lo:hi]) by one wave, four independent accumulators
+__device__ __forceinline__ double wave_dot(const double *col, const double *x, int lo, int hi, int lane) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int r = lo + lane;
+    for (; r + 192 < hi; r += 256) {
+        a0 += col[r] * x[r];
+        a1 += col[r + 64] * x[r + 64];
+        a2 += col[r + 128] * x[r + 128];
+        a3 += col[r + 192] * x[r + 192];
+    }
+    for (; r < hi; r += 64) a0 += col[r] * x[r];
+    return wave_sum((a0 + a1) + (a2 + a3));
+}
+
+// t[j] = y[j]/d_j - L21[:,j] . x_struct, one wave per column, 16 columns
+// per workgroup
+__global__ void __launch_bounds__(TF_NT) k_bwd_t(const double *L, const FrontDesc *fd, const int *list,
+                                                 const int64_t *pref, int cnt, const double *Y, const double *Dn,
+                                                 const double *V, double *T) {
+    const int i = seg_find(pref, cnt, blockIdx.x);
+    const FrontDesc d = fd[list[i]];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const double *xs = V + d.voff;  // rows >= nc hold x_struct
+    for (int q = 0; q < 4; q++) {
+        const int j = (int)(blockIdx.x - pref[i]) * 16 + wave * 4 + q;
+        if (j >= d.nc) break;
+        const double *col = L + d.loff + (int64_t)j * d.f;
+        double s = wave_dot(col, xs, d.nc, d.f, lane);
+        if (lane == 0) T[d.c0 + j] = Y[d.c0 + j] / Dn[d.c0 + j] - s;
     }
 }
 
-__global__ void __launch_bounds__(TF_NT) k_bwd_partial(const double *L, const FrontDesc *fd, const int *list,
-                                                       const int64_t *pref, int cnt, int p0, const double *xw,
-                                                       const int *sidx, double *partial) {
-    __shared__ double xr[TF_NT];
+// x[j] = t[j] + X11[j+1:nc, j] . t[j+1:nc], one wave per column
+__global__ void __launch_bounds__(TF_NT) k_bwd_x(const double *L, const FrontDesc *fd, const int *list,
+                                                 const int64_t *pref, int cnt, const double *T, double *xw) {
     const int i = seg_find(pref, cnt, blockIdx.x);
     const FrontDesc d = fd[list[i]];
-    const int p1 = min(d.nc, p0 + TF_PB);
-    blk_bwd_partial(L, d, p0, p1 + (int)(blockIdx.x - pref[i]) * TF_NT, xw, sidx,
-                    partial + (int64_t)blockIdx.x * TF_PB, xr);
-}
-
-__global__ void __launch_bounds__(TF_NT) k_bwd_diag(const double *L, const FrontDesc *fd, const int *list,
-                                                    const int64_t *pref, int p0, const double *V,
-                                                    const double *Dn, const double *partial, double *xw) {
-    __shared__ double lds[TF_PB * LDS_LD + TF_PB];
-    const FrontDesc d = fd[list[blockIdx.x]];
-    const int64_t q0 = pref[blockIdx.x], q1 = pref[blockIdx.x + 1];
-    blk_bwd_diag(L, d, p0, V, Dn, partial + q0 * TF_PB, (int)(q1 - q0), xw, lds);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const double *t = T + d.c0;
+    for (int q = 0; q < 4; q++) {
+        const int j = (int)(blockIdx.x - pref[i]) * 16 + wave * 4 + q;
+        if (j >= d.nc) break;
+        const double *col = L + d.loff + (int64_t)j * d.f;
+        double s = wave_dot(col, t, j + 1, d.nc, lane);
+        if (lane == 0) xw[d.c0 + j] = t[j] + s;
+    }
 }
 
 __global__ void k_unpermute(int n, const int *perm, const double *xw, double *x) {
@@ -227,43 +240,44 @@ __global__ void k_unpermute(int n, const int *perm, const double *xw, double *x)
 // ---- host launchers ----------------------------------------------------------
 namespace tf {
 
-void launch_fwd_fused(const double *L, const FrontDesc *fd, const int *list, int cnt, const double *b,
-                      const int *perm, const int *chp, const int *ch, const int *rel, double *V,
+void launch_fwd_small(const double *L, const FrontDesc *fd, const int *list, int cnt, const double *b,
+                      const int *perm, const int *chp, const int *ch, const int *rel, double *V, double *Y,
                       hipStream_t st) {
     if (cnt > 0)
-        hipLaunchKernelGGL(k_fwd_fused, dim3(cnt), dim3(TF_NT), 0, st, L, fd, list, b, perm, chp, ch, rel, V);
+        hipLaunchKernelGGL(k_fwd_small, dim3(cnt), dim3(TF_NT), 0, st, L, fd, list, b, perm, chp, ch, rel, V, Y);
+}
+void launch_bwd_small(const double *L, const FrontDesc *fd, const int *list, int cnt, const double *Y,
+                      const double *Dn, const int *sidx, double *xw, hipStream_t st) {
+    if (cnt > 0) hipLaunchKernelGGL(k_bwd_small, dim3(cnt), dim3(TF_NT), 0, st, L, fd, list, Y, Dn, sidx, xw);
 }
 void launch_fwd_gather(const FrontDesc *fd, const int *list, int cnt, const double *b, const int *perm,
                        const int *chp, const int *ch, const int *rel, double *V, hipStream_t st) {
     if (cnt > 0)
         hipLaunchKernelGGL(k_fwd_gather, dim3(cnt), dim3(TF_NT), 0, st, fd, list, b, perm, chp, ch, rel, V);
 }
-void launch_fwd_diag(const double *L, const FrontDesc *fd, const int *list, int cnt, int p0, double *V,
-                     hipStream_t st) {
-    if (cnt > 0) hipLaunchKernelGGL(k_fwd_diag, dim3(cnt), dim3(TF_NT), 0, st, L, fd, list, p0, V);
-}
-void launch_fwd_update(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
-                       int64_t nwg, int p0, double *V, hipStream_t st) {
+void launch_fwd_own(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
+                    int64_t nwg, const double *V, double *Y, hipStream_t st) {
     if (nwg > 0)
-        hipLaunchKernelGGL(k_fwd_update, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt, p0, V);
+        hipLaunchKernelGGL(k_fwd_own, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt, V, Y);
 }
-void launch_bwd_fused(const double *L, const FrontDesc *fd, const int *list, int cnt, const double *V,
-                      const double *Dn, const int *sidx, double *xw, hipStream_t st) {
-    if (cnt > 0)
-        hipLaunchKernelGGL(k_bwd_fused, dim3(cnt), dim3(TF_NT), 0, st, L, fd, list, V, Dn, sidx, xw);
-}
-void launch_bwd_partial(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
-                        int64_t nwg, int p0, const double *xw, const int *sidx, double *partial,
-                        hipStream_t st) {
+void launch_fwd_struct(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
+                       int64_t nwg, double *V, const double *Y, hipStream_t st) {
     if (nwg > 0)
-        hipLaunchKernelGGL(k_bwd_partial, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt, p0,
-                           xw, sidx, partial);
+        hipLaunchKernelGGL(k_fwd_struct, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt, V, Y);
 }
-void launch_bwd_diag(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
-                     int p0, const double *V, const double *Dn, const double *partial, double *xw,
-                     hipStream_t st) {
-    if (cnt > 0)
-        hipLaunchKernelGGL(k_bwd_diag, dim3(cnt), dim3(TF_NT), 0, st, L, fd, list, pref, p0, V, Dn, partial, xw);
+void launch_bwd_gather(const FrontDesc *fd, const int *list, int cnt, const int *sidx, const double *xw, double *V,
+                       hipStream_t st) {
+    if (cnt > 0) hipLaunchKernelGGL(k_bwd_gather, dim3(cnt), dim3(TF_NT), 0, st, fd, list, sidx, xw, V);
+}
+void launch_bwd_t(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
+                  const double *Y, const double *Dn, const double *V, double *T, hipStream_t st) {
+    if (nwg > 0)
+        hipLaunchKernelGGL(k_bwd_t, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt, Y, Dn, V, T);
+}
+void launch_bwd_x(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
+                  const double *T, double *xw, hipStream_t st) {
+    if (nwg > 0)
+        hipLaunchKernelGGL(k_bwd_x, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt, T, xw);
 }
 void launch_unpermute(int n, const int *perm, const double *xw, double *x, hipStream_t st) {
     hipLaunchKernelGGL(k_unpermute, dim3((n + 255) / 256), dim3(256), 0, st, n, perm, xw, x);
