@@ -1,29 +1,30 @@
 """Benchmark: FP64 source-timesteps/s of the FETD forward path on one GPU.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--model plain3d|small]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--model batched3d|plain3d|small]
 
 A "step" is one BDF2 time step of the reference's adaptive loop
-(src/FETD_FWD.f90:1281-1443) for one source: right-hand side (SpMV with
-Msigma + loop source), forward/diagonal/backward substitution with the
+(src/FETD_FWD.f90:1281-1443) for every source: right-hand side (SpMV with
+Msigma + loop sources), forward/diagonal/backward substitution with the
 LDL^T factors, receiver readout, and -- every sp_double steps -- the
 doubling check (two fine solves, a trial factorisation, one coarse solve).
 The timed region is K steps of the loop from t = 0, including every
-factorisation that falls inside it; the mesh, assembly and symbolic analysis
+factorisation that falls inside it; mesh, assembly and symbolic analysis
 are set up before it (inputs resident in HBM).
 
-Workload (default): the 3-D plain large-loop model, ~0.92M tets / 1.1M
-edges (0.01 S/m half-space, 1 S/m block, 800 m loop, 41 receivers, 1e-12 S/m
-air, 100 km cube), synthetic tensor-grid mesh.
+Default workload (batched3d): 64 transmitters (20 m loops on an 8 x 8 grid,
+50 m apart) over the 3-D earth (0.01 S/m host, 1 S/m block, 1e-12 S/m air,
+100 km cube), ~1.35M tets / ~1.6M edges, one receiver per transmitter.
+`value` is the batched rate (64 sources per block solve, one factorisation
+per dt); the plain single-source path is timed on the same mesh over the
+same steps and reported beside it.
 
-Prints one JSON line with roofline and cpu_baseline objects.
+Prints one JSON line (with roofline and cpu_baseline objects).
 """
 import argparse
 import json
 import os
 import sys
 import time
-
-import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -34,20 +35,20 @@ FP64_PEAK_TFLOPS = 78.6
 
 def build_model(name):
     from temf3dt_amd import workloads
-    if name == "plain3d":
-        return workloads.plain_3d_model()
-    if name == "small":
-        return workloads.small_model()
-    raise SystemExit(f"unknown model {name}")
+    return {"batched3d": workloads.batched_3d_model, "plain3d": workloads.plain_3d_model,
+            "small": workloads.small_model, "multismall": workloads.multi_source_small_model}[name]()
 
 
-def setup_gpu(m):
+def setup_gpu(m, loops):
     from temf3dt_amd import fetd
     fe = fetd.Fetd(0)
     fe.set_mesh(m.xyz, m.tet_vex)
     sig, imu = m.sigma_inv_mu()
     fe.set_materials(sig, imu)
-    fe.set_source(m.loops[0], *m.source)
+    if len(loops) == 1:
+        fe.set_source(m.loops[loops[0]], *m.source)
+    else:
+        fe.set_sources([m.loops[q] for q in loops], *m.source)
     fe.set_receivers(m.receivers, m.rec_charac)
     fe.set_schedule(m.sp_division, m.sp_double, m.sp_dblesize, m.time_maximum)
     fe.assemble()
@@ -56,21 +57,31 @@ def setup_gpu(m):
     return fe, time.time() - t
 
 
-def cpu_baseline(m, gpu_counts, sample_steps):
-    """Oracle (C restatement + MKL PARDISO, the reference's solver) timed on
-    the host cores for the first `sample_steps` steps of the same model; the
-    per-factorisation and per-solve times are then weighted by the GPU run's
-    factorisation and solve counts to express the rate in source-timesteps/s."""
+def timed_run(fe, steps, warmup):
+    if warmup > 0:
+        fe.run(max_steps=warmup)
+    t0 = time.time()
+    rows = fe.run(max_steps=steps)
+    wall = time.time() - t0
+    return rows, wall, fe.timers()
+
+
+def cpu_baseline(m, counts, sample_steps):
+    """Oracle (C restatement + MKL PARDISO, the reference's solver) on the
+    host cores for the first `sample_steps` steps of one transmitter of the
+    same model.  Per-factorisation and per-step times are weighted by the
+    GPU run's counts to give source-timesteps/s; the reference runs one
+    transmitter per job, so 64 transmitters are 64 such runs at this rate."""
     import tempfile
 
     from oracle import pyoracle
     if not pyoracle.pardiso_available():
         return None
-    threads = int(os.environ.get("TF_CPU_THREADS", os.cpu_count() or 1))
+    threads = int(os.environ.get("TF_CPU_THREADS", min(16, os.cpu_count() or 1)))
     pyoracle.set_threads(threads)
     with tempfile.TemporaryDirectory() as tmp:
         root = os.path.join(tmp, "cpu")
-        m.write(root)
+        m.write(root, loop=0)
         o = pyoracle.Oracle(root + ".mesh", root + ".cntl")
         o.assemble()
         t0 = time.time()
@@ -78,68 +89,80 @@ def cpu_baseline(m, gpu_counts, sample_steps):
         wall = time.time() - t0
     tm = r["timers"]
     nfact = max(1, r["nfact"])
-    nsol = max(1, len(r["step"]) - 1)
+    nstep = max(1, int(r["step"][-1]))
     t_fact = tm[1] / nfact
-    t_step = (wall - tm[1]) / nsol
-    steps, nf = gpu_counts
+    t_step = (wall - tm[1]) / nstep
+    steps, nf = counts
     est = steps / (nf * t_fact + steps * t_step)
     return {"value": est, "unit": "source-timesteps/s", "cores": threads, "kind": "port",
-            "sample": (f"first {sample_steps} steps of the same model on the host with MKL PARDISO "
-                       f"({nfact} factorisations, {t_fact:.2f} s each; {t_step * 1e3:.1f} ms per step), "
-                       f"weighted by the GPU run's {nf} factorisations over {steps} steps"),
-            "factor_nnz_pardiso": int(r["factor_nnz"])}
+            "sample": (f"first {nstep} steps of transmitter 0 on the host with MKL PARDISO "
+                       f"({nfact} factorisation(s), {t_fact:.1f} s each; {t_step * 1e3:.0f} ms per step); "
+                       f"rate = GPU run's {steps} steps / ({nf} factorisations + {steps} steps) at those costs"),
+            "pardiso_factor_nnz": int(r["factor_nnz"]), "wall_s": wall}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--model", default="plain3d")
-    ap.add_argument("--cpu-sample", type=int, default=12)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--model", default="batched3d")
+    ap.add_argument("--cpu-sample", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
-              "and not launched", file=sys.stderr)
+        print("bench.py: this build runs on one GPU; the multi-GPU sharding of the path is a design in "
+              "DESIGN.md and is not launched", file=sys.stderr)
         sys.exit(2)
     m = build_model(a.model)
-    fe, t_an = setup_gpu(m)
+    nsrc = len(m.loops)
+    fe, t_an = setup_gpu(m, list(range(nsrc)))
     st = fe.factor_stats()
-    print(f"[bench] model {a.model}: {m.ntet} tets, {fe.nE} edges, nnzU {fe.nnz}, nnzL {st['nnzL']:.3e}, "
-          f"factor flops {st['flops']:.3e}, analyze {t_an:.2f} s", file=sys.stderr, flush=True)
-    if a.warmup > 0:
-        fe.run(max_steps=a.warmup)
-    t0 = time.time()
-    rows = fe.run(max_steps=a.steps)
-    wall = time.time() - t0
-    tm = fe.timers()
+    n = fe.nE
+    print(f"[bench] {a.model}: {m.ntet} tets, {n} edges, nnzU {fe.nnz}, nnzL {st['nnzL']:.3e}, "
+          f"factor flops {st['flops']:.3e}, {nsrc} sources, analyze {t_an:.2f} s", file=sys.stderr, flush=True)
+    rows, wall, tm = timed_run(fe, a.steps, a.warmup)
     steps = int(rows["step"][-1])
     nf = int(tm["nfactor"])
-    value = steps / wall
-    print(f"[bench] {steps} steps in {wall:.3f} s, {nf} factorisations ({tm['factor']:.3f} s), "
-          f"t_end {rows['t'][-1]:.3e}", file=sys.stderr, flush=True)
-    # dominant-kernel roofline: the factorisation (FP64 flops) and the solve
-    # (bytes of L streamed per step); both reported, factor is the headline
-    fact_tflops = st["flops"] / (tm["factor"] / max(nf, 1)) / 1e12 if nf else 0.0
-    roof = {"bound": "mfma", "achieved": fact_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": fact_tflops / FP64_PEAK_TFLOPS, "traffic": None,
-            "kernel": "multifrontal LDL^T (all factor kernels, event-timed per factorisation)"}
-    cpu = None
-    if not a.no_cpu:
-        cpu = cpu_baseline(m, (steps, nf), a.cpu_sample)
+    value = nsrc * steps / wall
+    print(f"[bench] batched: {steps} steps x {nsrc} sources in {wall:.3f} s, {nf} factorisations "
+          f"({tm['factor']:.3f} s), t_end {rows['t'][-1]:.3e}", file=sys.stderr, flush=True)
+    # block-solve roofline: per step the solves stream L once for all K
+    # sources; flops_k = K (4 nnzL + 2n + 4 nnzU + 6n) per solve
+    nsolves = steps + 3 * len(fe.checks()["step"])
+    t_solve = max(wall - tm["factor"], 1e-9)
+    flops_k = nsrc * (4.0 * st["nnzL"] + 8.0 * n + 4.0 * fe.nnz)
+    achieved = flops_k * nsolves / t_solve / 1e12
+    roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+            "kernel": (f"block solve step (RHS + forward/backward tile products for {nsrc} sources), "
+                       f"{nsolves} solves in the non-factorisation time")}
+    fe.close()
+    plain = None
+    if not a.no_plain:
+        fp, _ = setup_gpu(m, [0])
+        prow, pwall, ptm = timed_run(fp, a.steps, a.warmup)
+        psteps = int(prow["step"][-1])
+        plain = {"value": psteps / pwall, "steps": psteps, "wall_s": pwall, "factor_s": ptm["factor"],
+                 "factorisations": int(ptm["nfactor"])}
+        fp.close()
+        print(f"[bench] plain: {psteps} steps in {pwall:.3f} s ({ptm['factor']:.3f} s factorising)",
+              file=sys.stderr, flush=True)
+    cpu = None if a.no_cpu else cpu_baseline(m, (steps, nf), a.cpu_sample)
     out = {
         "metric": "FP64 source-timesteps/sec at 1M tets (batched vs plain); adjoint/forward time",
         "value": value, "unit": "source-timesteps/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
         "ms_per_step": 1e3 * wall / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{a.model}: plain single-source BDF2 loop, {m.ntet} tets, {fe.nE} edges, "
-                               f"{len(m.receivers)} receivers", "mode": "plain", "sources": 1,
-                   "nnzL": st["nnzL"], "factorisations": nf},
+        "config": {"workload": f"{a.model}: {nsrc}-transmitter BDF2 loop, block solve of {nsrc} sources per "
+                               f"factorisation, {m.ntet} tets, {n} edges", "mode": "batched", "sources": nsrc,
+                   "nnzL": st["nnzL"], "factorisations": nf,
+                   "plain_single_source": plain,
+                   "batched_over_plain": (value / plain["value"]) if plain else None},
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
-    fe.close()
 
 
 if __name__ == "__main__":

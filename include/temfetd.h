@@ -66,6 +66,9 @@ int tf_set_materials(tf_ctx *ctx, const double *sigma, const double *inv_mu);
  * a multi-RHS block solve. */
 int tf_set_source(tf_ctx *ctx, int isrc, int nseg, const int *seg_vex, int source_type,
                   int pulse_type, double pulse_width, double current);
+/* Number of sources (1..64) time-stepped together on one factorisation; the
+ * doubling check then uses the largest per-source error.  Call before
+ * tf_set_source.  No reference counterpart (one source per reference run). */
 int tf_set_num_sources(tf_ctx *ctx, int nsrc);
 /* Receivers (src/FETD_FWD.f90:500-520, p_in_tet :1009-1049): [nR][3]
  * coordinates; rec_tet_out (nullable) receives the 1-based containing tet. */
@@ -98,7 +101,7 @@ int tf_factor(tf_ctx *ctx, int slot, double dt);
  * the assembled one (ia/ja are checked against it). */
 int tf_factor_csr(tf_ctx *ctx, int slot, int n, const int *ia, const int *ja, const double *va);
 /* pardiso_direct_2: x = P^-1 b for nrhs right-hand sides, host buffers in
- * edge order, column-major [nrhs][n]. */
+ * edge order, [nrhs][n]; nrhs > 1 runs the block solve, 64 at a time. */
 int tf_solve(tf_ctx *ctx, int slot, int nrhs, const double *b, double *x);
 /* pardiso_direct_3 */
 int tf_release(tf_ctx *ctx, int slot);
@@ -110,10 +113,13 @@ int tf_release(tf_ctx *ctx, int slot);
  * not stored.  max_steps > 0 caps the loop (bounded runs). */
 int tf_run(tf_ctx *ctx, int max_steps, int max_rows, int *nrows, int *row_step, double *row_t,
            double *row_dt, double *fields);
-/* Replay a fixed schedule instead of the error-checked controller:
- * nsteps rows of dt per step (changes of dt refactor).  nsteps = 0 restores
- * the controller. */
-int tf_set_replay(tf_ctx *ctx, int nsteps, const double *dt);
+/* Replay the controller's decisions instead of computing them: keep[i] is
+ * the keep (1) / change (0) outcome of the i-th doubling check (as logged by
+ * tf_get_checks of an earlier run); the checks' solves and factorisations
+ * still run.  nchecks = 0 restores the error-checked controller
+ * (src/FETD_FWD.f90:1380-1417).  No reference counterpart: it makes runs with
+ * different source sets follow one schedule. */
+int tf_set_replay(tf_ctx *ctx, int nchecks, const int *keep);
 /* Doubling-check log of the last tf_run: step, err, keep flag. */
 int tf_get_checks(const tf_ctx *ctx, int max, int *n, int *step, double *err, int *keep);
 /* Timers of the last tf_run in seconds: assemble, factor, solve, spmv+rhs,

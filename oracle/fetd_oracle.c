@@ -208,6 +208,8 @@ struct fo_model {
     double timers[6];
     int nfact;
     long long factor_nnz;
+    int nreplay;
+    int *replay; /* forced keep/change decisions of the doubling checks */
 };
 
 static void *xcalloc(size_t n, size_t s) {
@@ -226,7 +228,7 @@ void fo_free(fo_model *m) {
                     m->tet_len, m->rec_xyz, m->rec_tet, m->n_arr, m->d_arr, m->sig, m->per,
                     m->mag, m->src_edg, m->src_drc, m->gates, m->ia, m->ja, m->av, m->bv,
                     m->cv, m->ia0, m->ja0, m->bfull, m->row_step, m->row_t, m->row_dt,
-                    m->row_f, m->chk_step, m->chk_keep, m->chk_err};
+                    m->row_f, m->chk_step, m->chk_keep, m->chk_err, m->replay};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free(ptrs[i]);
     free(m);
 }
@@ -1154,6 +1156,10 @@ static int adap_euler(fo_model *m, outctx *o, int inout, double *res_e, int max_
             m->timers[2] += now_s() - t0;
             double error = two_error(ev2, ev1, n);
             int keep = (error > 1.0e-4 && not_changed < 2);
+            if (m->nreplay > 0) {
+                if (m->nchk >= m->nreplay) { set_err("replayed schedule is too short"); rc = -1; break; }
+                keep = m->replay[m->nchk] != 0;
+            }
             log_check(m, step, error, keep);
             if (m->verbose)
                 printf(" error: %.6e %s\n", error, keep ? "keep step" : "change step");
@@ -1293,6 +1299,16 @@ void fo_set_control(fo_model *m, const int *ictl, const double *dctl) {
         m->pulse_type = ictl[3]; m->src_type = ictl[4];
     }
     if (dctl) { m->pulse_width = dctl[0]; m->current = dctl[1]; m->time_maxm = dctl[2]; }
+}
+void fo_set_replay(fo_model *m, int n, const int *keep) {
+    free(m->replay);
+    m->replay = NULL;
+    m->nreplay = 0;
+    if (n > 0) {
+        m->replay = xcalloc((size_t)n, sizeof(int));
+        memcpy(m->replay, keep, sizeof(int) * (size_t)n);
+        m->nreplay = n;
+    }
 }
 void fo_set_sigma(fo_model *m, const double *sigma) {
     memcpy(m->sig, sigma, sizeof(double) * (size_t)m->nT);

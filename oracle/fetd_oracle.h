@@ -51,6 +51,8 @@ void fo_get_material(const fo_model *m, double *sigma, double *inv_mu);
 /* control values: sp_division, sp_double, sp_dblesize, pulse_type, source_type */
 void fo_get_control(const fo_model *m, int *ictl, double *dctl);
 void fo_set_control(fo_model *m, const int *ictl, const double *dctl);
+/* force the keep (1) / change (0) decisions of the doubling checks */
+void fo_set_replay(fo_model *m, int n, const int *keep);
 /* override per-tet conductivity (for perturbation tests) */
 void fo_set_sigma(fo_model *m, const double *sigma);
 

@@ -125,6 +125,10 @@ class Oracle:
         self.L.fo_get_material(self.m, _p(s), _p(im))
         return s, im
 
+    def set_replay(self, keep):
+        k = np.ascontiguousarray(keep, np.int32)
+        self.L.fo_set_replay(self.m, len(k), _p(k))
+
     def set_sigma(self, sigma):
         s = np.ascontiguousarray(sigma, np.float64)
         self.L.fo_set_sigma(self.m, _p(s))

@@ -68,6 +68,28 @@ void launch_bwd_t(const double *L, const FrontDesc *fd, const int *list, const i
 void launch_bwd_x(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
                   const double *T, double *xw, hipStream_t st);
 void launch_unpermute(int n, const int *perm, const double *xw, double *x, hipStream_t st);
+// msolve.hip
+void launch_mfwd_gather(const FrontDesc *fd, const int *list, int cnt, const double *b, const int *perm,
+                        const int *chp, const int *ch, const int *rel, double *V, int K, hipStream_t st);
+void launch_mbwd_gather(const FrontDesc *fd, const int *list, int cnt, const int *sidx, const double *xw, double *V,
+                        int K, hipStream_t st);
+void launch_mfwd_own(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
+                     int64_t nwg, const double *V, double *Y, int K, hipStream_t st);
+void launch_mfwd_struct(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
+                        int64_t nwg, double *V, const double *Y, int K, hipStream_t st);
+void launch_mbwd_t(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
+                   const double *Y, const double *Dn, const double *V, double *T, int K, hipStream_t st);
+void launch_mbwd_x(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
+                   const double *T, double *xw, int K, hipStream_t st);
+void launch_munpermute(int n, int K, const int *perm, const double *xw, double *x, hipStream_t st);
+void launch_mspmv_rhs(int n, int K, const int *ia, const int *ja, const double *val, const double *e2,
+                      const double *e3, double *r, hipStream_t st);
+void launch_msource(int ns, int K, const int *edge, const int *src, const double *w, double didt, double cdt,
+                    double *r, hipStream_t st);
+void launch_mreadout(int nR, int K, const int *rec_edge, const double *rec_drc, const double *n_arr,
+                     const double *d_arr, const double *e, double *out, hipStream_t st);
+void launch_mdiffnorm(int64_t n, int K, const double *a, const double *b, double *part, double *out,
+                      hipStream_t st);
 // step.hip
 void launch_spmv_rhs(int n, const int *ia, const int *ja, const double *val, const double *e2, const double *e3,
                      double *r, hipStream_t st);
@@ -113,7 +135,8 @@ void dfree(T *&p) {
 enum LaunchKind {
     L_EXTADD, L_FFUSED, L_FDIAG, L_FTRSM, L_FUPD, L_IW, L_IAPPLY, L_IDIAG,
     L_SSMALL, L_SGATHER, L_SOWN, L_SSTRUCT,
-    L_BSMALL, L_BGATHER, L_BT, L_BX
+    L_BSMALL, L_BGATHER, L_BT, L_BX,
+    L_MFGATHER, L_MFOWN, L_MFSTRUCT, L_MBGATHER, L_MBT, L_MBX
 };
 struct Launch {
     int kind, p0, off, cnt, poff;
@@ -122,7 +145,7 @@ struct Launch {
 
 // Execution plan over the front tree: which fronts each launch touches.
 struct Plan {
-    std::vector<Launch> factor, fwd, bwd;
+    std::vector<Launch> factor, fwd, bwd, mfwd, mbwd;
     std::vector<int> list;
     std::vector<int64_t> pref;
     std::vector<int64_t> woff;  // per supernode: offset in the inversion scratch
@@ -158,10 +181,14 @@ struct tf_ctx {
     Topology topo;
     bool have_mesh = false, have_mat = false, assembled = false, analyzed = false;
     std::vector<double> sigma, inv_mu;
-    // source
+    // sources: nsrc loops sharing one waveform; entries (edge, source, weight)
+    int nsrc = 1;
     int nseg = 0;
-    std::vector<int> src_edge;
+    std::vector<std::vector<int>> src_edges;
+    std::vector<std::vector<double>> src_ws;
+    std::vector<int> src_edge, src_id;
     std::vector<double> src_w;
+    std::vector<int> replay;  // forced keep/change decisions of the doubling checks
     int stype = 1, ptype = 1;
     double pulse_width = 0.0, current = 1.0;
     bool have_src = false;
@@ -186,8 +213,12 @@ struct tf_ctx {
     // receivers / source on device
     double *d_rec_xyz = nullptr, *d_narr = nullptr, *d_darr = nullptr, *d_rec_drc = nullptr;
     int *d_rec_tet = nullptr, *d_rec_edge = nullptr;
-    int *d_src_edge = nullptr;
+    int *d_src_edge = nullptr, *d_src_id = nullptr;
     double *d_src_w = nullptr;
+    // block-solve work (K right-hand sides)
+    double *d_mV = nullptr, *d_mY = nullptr, *d_mT = nullptr, *d_mxw = nullptr, *d_mb = nullptr, *d_mx = nullptr;
+    int mK = 0;
+    int64_t vrows = 0;
     // solver
     Symbolic sym;
     Plan plan;
@@ -203,6 +234,7 @@ struct tf_ctx {
     double *d_tmpb = nullptr, *d_tmpx = nullptr;
     // stepping work vectors
     std::vector<double *> vecs;
+    int64_t vec_len = 0;
     double *d_norm_part = nullptr, *d_norm_out = nullptr;
     // output rows
     std::vector<int> row_step;
@@ -224,6 +256,8 @@ void free_solver(tf_ctx *c) {
     dfree(c->d_xw); dfree(c->d_Y); dfree(c->d_T); dfree(c->d_W); dfree(c->d_woff);
     for (int s = 0; s < 2; s++) { dfree(c->d_L[s]); dfree(c->d_D[s]); c->slot_live[s] = false; }
     dfree(c->d_tmpb); dfree(c->d_tmpx);
+    dfree(c->d_mV); dfree(c->d_mY); dfree(c->d_mT); dfree(c->d_mxw); dfree(c->d_mb); dfree(c->d_mx);
+    c->mK = 0;
     c->analyzed = false;
 }
 
@@ -241,7 +275,7 @@ void free_all(tf_ctx *c) {
     dfree(c->d_tet_vex); dfree(c->d_tet_drc); dfree(c->d_ctr); dfree(c->d_orient); dfree(c->d_cptr);
     dfree(c->d_M); dfree(c->d_K); dfree(c->d_fia); dfree(c->d_fja); dfree(c->d_fmap); dfree(c->d_fval);
     dfree(c->d_rec_xyz); dfree(c->d_narr); dfree(c->d_darr); dfree(c->d_rec_drc); dfree(c->d_rec_tet);
-    dfree(c->d_rec_edge); dfree(c->d_src_edge); dfree(c->d_src_w); dfree(c->d_flag);
+    dfree(c->d_rec_edge); dfree(c->d_src_edge); dfree(c->d_src_id); dfree(c->d_src_w); dfree(c->d_flag);
 }
 
 int fail(tf_ctx *c, int code, const std::string &msg) {
@@ -373,6 +407,30 @@ void build_plan(tf_ctx *c) {
         add(p.bwd, L_BX, 0, grid[l], uc);
         add(p.bwd, L_BSMALL, 0, small[l], {});
     }
+    // block solve for K right-hand sides: every front through the tile kernels
+    for (int l = 0; l < s.nlevels; l++) {
+        std::vector<int> all(s.lev_sn.begin() + s.lev_ptr[l], s.lev_sn.begin() + s.lev_ptr[l + 1]);
+        std::vector<int64_t> uo, us;
+        for (int k : all) {
+            uo.push_back((s.nc[k] + 63) / 64);
+            us.push_back((m_of(k) + 63) / 64);
+        }
+        add(p.mfwd, L_MFGATHER, 0, all, {});
+        add(p.mfwd, L_MFOWN, 0, all, uo);
+        add(p.mfwd, L_MFSTRUCT, 0, all, us);
+    }
+    for (int l = s.nlevels - 1; l >= 0; l--) {
+        std::vector<int> all(s.lev_sn.begin() + s.lev_ptr[l], s.lev_sn.begin() + s.lev_ptr[l + 1]);
+        std::vector<int> withm;
+        std::vector<int64_t> uc;
+        for (int k : all) {
+            uc.push_back((s.nc[k] + 63) / 64);
+            if (m_of(k) > 0) withm.push_back(k);
+        }
+        add(p.mbwd, L_MBGATHER, 0, withm, {});
+        add(p.mbwd, L_MBT, 0, all, uc);
+        add(p.mbwd, L_MBX, 0, all, uc);
+    }
 }
 
 void do_analyze(tf_ctx *c, int leaf) {
@@ -395,6 +453,7 @@ void do_analyze(tf_ctx *c, int leaf) {
         fd[k].c0 = s.c0[k];
         voff += fd[k].f;
     }
+    c->vrows = voff;
     build_plan(c);
     c->d_fd = dupload(fd);
     c->d_perm = dupload(s.perm);
@@ -523,6 +582,57 @@ void solve_dev(tf_ctx *c, int slot, const double *b, double *x) {
     tf::launch_unpermute(c->sym.n, c->d_perm, c->d_xw, x, c->st);
 }
 
+void ensure_block(tf_ctx *c, int K) {
+    if (c->mK >= K) return;
+    dfree(c->d_mV); dfree(c->d_mY); dfree(c->d_mT); dfree(c->d_mxw); dfree(c->d_mb); dfree(c->d_mx);
+    const int64_t n = c->sym.n;
+    c->d_mV = dalloc<double>((size_t)c->vrows * K);
+    c->d_mY = dalloc<double>((size_t)n * K);
+    c->d_mT = dalloc<double>((size_t)n * K);
+    c->d_mxw = dalloc<double>((size_t)n * K);
+    c->d_mb = dalloc<double>((size_t)n * K);
+    c->d_mx = dalloc<double>((size_t)n * K);
+    c->mK = K;
+}
+
+// X = P^-1 B for K <= 64 right-hand sides, [n][K] device arrays in edge order
+void solve_block(tf_ctx *c, int slot, int K, const double *b, double *x) {
+    ensure_block(c, K);
+    const Plan &p = c->plan;
+    const double *L = c->d_L[slot], *Dn = c->d_D[slot];
+    const int *lst = c->d_plist;
+    const int64_t *pr = c->d_ppref;
+    for (const Launch &l : p.mfwd) {
+        switch (l.kind) {
+        case L_MFGATHER:
+            tf::launch_mfwd_gather(c->d_fd, lst + l.off, l.cnt, b, c->d_perm, c->d_chp, c->d_ch, c->d_rel, c->d_mV,
+                                   K, c->st);
+            break;
+        case L_MFOWN:
+            tf::launch_mfwd_own(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_mV, c->d_mY, K, c->st);
+            break;
+        case L_MFSTRUCT:
+            tf::launch_mfwd_struct(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_mV, c->d_mY, K, c->st);
+            break;
+        }
+    }
+    for (const Launch &l : p.mbwd) {
+        switch (l.kind) {
+        case L_MBGATHER:
+            tf::launch_mbwd_gather(c->d_fd, lst + l.off, l.cnt, c->d_sidx, c->d_mxw, c->d_mV, K, c->st);
+            break;
+        case L_MBT:
+            tf::launch_mbwd_t(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_mY, Dn, c->d_mV, c->d_mT, K,
+                              c->st);
+            break;
+        case L_MBX:
+            tf::launch_mbwd_x(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_mT, c->d_mxw, K, c->st);
+            break;
+        }
+    }
+    tf::launch_munpermute(c->sym.n, K, c->d_perm, c->d_mxw, x, c->st);
+}
+
 // ---- time loop -------------------------------------------------------------------
 struct StepVecs {
     double *e1, *e2, *e3, *rh, *te1, *te2, *te3, *erec, *ev1, *ev2;
@@ -530,36 +640,46 @@ struct StepVecs {
 };
 
 void ensure_vecs(tf_ctx *c, StepVecs &v) {
-    int n = c->topo.nE;
+    const int64_t len = (int64_t)c->topo.nE * c->nsrc;
     int need = 10 + c->sp_dblesize + 1;
-    if ((int)c->vecs.size() != need) {
+    if ((int)c->vecs.size() != need || c->vec_len != len) {
         for (auto &p : c->vecs) dfree(p);
         c->vecs.clear();
-        for (int i = 0; i < need; i++) c->vecs.push_back(dalloc<double>(n));
+        for (int i = 0; i < need; i++) c->vecs.push_back(dalloc<double>(len));
+        c->vec_len = len;
     }
     v.e1 = c->vecs[0]; v.e2 = c->vecs[1]; v.e3 = c->vecs[2]; v.rh = c->vecs[3];
     v.te1 = c->vecs[4]; v.te2 = c->vecs[5]; v.te3 = c->vecs[6]; v.erec = c->vecs[7];
     v.ev1 = c->vecs[8]; v.ev2 = c->vecs[9];
     v.ring.assign(c->vecs.begin() + 10, c->vecs.end());
-    if (!c->d_norm_part) c->d_norm_part = dalloc<double>(1024);
-    if (!c->d_norm_out) c->d_norm_out = dalloc<double>(2);
+    if (!c->d_norm_part) c->d_norm_part = dalloc<double>(256 * 64 * 2);
+    if (!c->d_norm_out) c->d_norm_out = dalloc<double>(2 * 64);
 }
 
+// r = Msigma (4 e2 - e3) - 2 dt s(t), K sources side by side
 void rhs(tf_ctx *c, int step, double t, const double *e2, const double *e3, double dt, bool negate, double *r) {
-    tf::launch_spmv_rhs(c->topo.nE, c->d_fia, c->d_fja, c->d_fval, e2, e3, r, c->st);
-    if (step < c->sp_division) {
+    const int K = c->nsrc;
+    if (K == 1) tf::launch_spmv_rhs(c->topo.nE, c->d_fia, c->d_fja, c->d_fval, e2, e3, r, c->st);
+    else tf::launch_mspmv_rhs(c->topo.nE, K, c->d_fia, c->d_fja, c->d_fval, e2, e3, r, c->st);
+    if (step < c->sp_division) {  // sys_right: source on while step < sp_division
         double didt = source_didt(c->stype, c->ptype, t, c->pulse_width, c->current);
         if (negate) didt = -didt;
-        tf::launch_source(c->nseg, c->d_src_edge, c->d_src_w, didt, 2.0 * dt, r, c->st);
+        if (K == 1) tf::launch_source(c->nseg, c->d_src_edge, c->d_src_w, didt, 2.0 * dt, r, c->st);
+        else tf::launch_msource(c->nseg, K, c->d_src_edge, c->d_src_id, c->d_src_w, didt, 2.0 * dt, r, c->st);
     }
 }
 
 void dcopy(tf_ctx *c, double *dst, const double *src) {
-    HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * c->topo.nE, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * c->topo.nE * c->nsrc, hipMemcpyDeviceToDevice, c->st));
+}
+
+void step_solve(tf_ctx *c, int slot, const double *b, double *x) {
+    if (c->nsrc == 1) solve_dev(c, slot, b, x);
+    else solve_block(c, slot, c->nsrc, b, x);
 }
 
 void output_row(tf_ctx *c, int step, double t, double dt, const double *e) {
-    int64_t per = (int64_t)c->nR * 6;
+    const int64_t per = (int64_t)c->nR * 6 * c->nsrc;
     if (c->nrows + 1 > c->rows_cap) {
         int64_t cap = std::max<int64_t>(256, 2 * c->rows_cap);
         double *nb = dalloc<double>(cap * per);
@@ -570,19 +690,27 @@ void output_row(tf_ctx *c, int step, double t, double dt, const double *e) {
         c->d_rows = nb;
         c->rows_cap = cap;
     }
-    tf::launch_readout(c->nR, c->d_rec_edge, c->d_rec_drc, c->d_narr, c->d_darr, e, c->d_rows + per * c->nrows, c->st);
+    double *o = c->d_rows + per * c->nrows;
+    if (c->nsrc == 1) tf::launch_readout(c->nR, c->d_rec_edge, c->d_rec_drc, c->d_narr, c->d_darr, e, o, c->st);
+    else tf::launch_mreadout(c->nR, c->nsrc, c->d_rec_edge, c->d_rec_drc, c->d_narr, c->d_darr, e, o, c->st);
     c->row_step.push_back(step);
     c->row_t.push_back(t);
     c->row_dt.push_back(dt);
     c->nrows++;
 }
 
+// two_error (src/FETD_UTILITY.f90:578-599): ||a-b|| / ||b||; for K sources
+// the largest of the K per-source values decides
 double two_error(tf_ctx *c, const double *a, const double *b) {
-    tf::launch_diffnorm(c->topo.nE, a, b, c->d_norm_part, c->d_norm_out, c->st);
-    double h[2];
-    HIPCHK(hipMemcpyAsync(h, c->d_norm_out, sizeof h, hipMemcpyDeviceToHost, c->st));
+    const int K = c->nsrc;
+    std::vector<double> h(2 * K);
+    if (K == 1) tf::launch_diffnorm(c->topo.nE, a, b, c->d_norm_part, c->d_norm_out, c->st);
+    else tf::launch_mdiffnorm(c->topo.nE, K, a, b, c->d_norm_part, c->d_norm_out, c->st);
+    HIPCHK(hipMemcpyAsync(h.data(), c->d_norm_out, sizeof(double) * 2 * K, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
-    return std::sqrt(h[0]) / std::sqrt(h[1]);
+    double e = 0.0;
+    for (int q = 0; q < K; q++) e = std::max(e, std::sqrt(h[2 * q]) / std::sqrt(h[2 * q + 1]));
+    return e;
 }
 
 double elapsed(hipEvent_t a, hipEvent_t b) {
@@ -606,7 +734,8 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
     const bool negate = (inout == 1), outp = (inout != 1);
     const int ring = dbl + 1;
     int cur = 0;
-    for (double *p : {v.e1, v.e2, v.e3}) HIPCHK(hipMemsetAsync(p, 0, sizeof(double) * n, c->st));
+    for (double *p : {v.e1, v.e2, v.e3}) HIPCHK(hipMemsetAsync(p, 0, sizeof(double) * n * c->nsrc, c->st));
+    int ncheck = 0;
     hipEvent_t f0, f1;
     HIPCHK(hipEventCreate(&f0));
     HIPCHK(hipEventCreate(&f1));
@@ -633,7 +762,7 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
             double ttime = time_cur, tstep = step_cur;
             for (int i = 1; i <= dbl; i++) {
                 rhs(c, step, ttime, v.te2, v.te3, tstep, negate, v.rh);
-                solve_dev(c, cur, v.rh, v.te1);
+                step_solve(c, cur, v.rh, v.te1);
                 if (i == dbl) break;
                 if (i == 1) dcopy(c, v.erec, v.te1);
                 std::swap(v.te3, v.te2);   // te3 <- te2
@@ -648,9 +777,16 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
             double *ea = v.ring[(idx - 1) % ring];
             double *eb = v.ring[(idx - 1 - dbl) % ring];
             rhs(c, step, ttime, ea, eb, tstep, negate, v.rh);
-            solve_dev(c, other, v.rh, v.ev2);
+            step_solve(c, other, v.rh, v.ev2);
             const double error = two_error(c, v.ev2, v.ev1);
-            const bool keep = (error > 1.0e-4 && not_changed < 2);
+            // keep the step iff err > 1e-4 and fewer than 2 kept in a row
+            // (src/FETD_FWD.f90:1381); a replayed schedule forces the decision
+            bool keep = (error > 1.0e-4 && not_changed < 2);
+            if (!c->replay.empty()) {
+                if (ncheck >= (int)c->replay.size()) throw std::runtime_error("replayed schedule is too short");
+                keep = c->replay[ncheck] != 0;
+            }
+            ncheck++;
             c->chk_step.push_back(step);
             c->chk_err.push_back(error);
             c->chk_keep.push_back(keep ? 1 : 0);
@@ -676,7 +812,7 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
             continue;
         }
         rhs(c, step, time_cur, v.e2, v.e3, step_cur, negate, v.rh);
-        solve_dev(c, cur, v.rh, v.e1);
+        step_solve(c, cur, v.rh, v.e1);
         if (outp) output_row(c, step, time_cur, step_cur, v.e1);
         dcopy(c, v.ring[idx % ring], v.e1);
         idx++;
@@ -819,35 +955,74 @@ int tf_set_materials(tf_ctx *c, const double *sigma, const double *inv_mu) {
     });
 }
 
+int tf_set_num_sources(tf_ctx *c, int nsrc) {
+    if (!c) return TF_ERR_ARG;
+    if (nsrc < 1 || nsrc > 64) return fail(c, TF_ERR_ARG, "number of sources must be 1..64");
+    c->nsrc = nsrc;
+    c->src_edges.assign(nsrc, {});
+    c->src_ws.assign(nsrc, {});
+    c->have_src = false;
+    return TF_OK;
+}
+
 int tf_set_source(tf_ctx *c, int isrc, int nseg, const int *seg_vex, int source_type, int pulse_type,
                   double pulse_width, double current) {
     return guarded(c, [&] {
         if (!c->have_mesh) return fail(c, TF_ERR_STATE, "tf_set_mesh first");
-        if (isrc != 0) return fail(c, TF_ERR_ARG, "only source index 0 is supported by this build");
+        if ((int)c->src_edges.size() != c->nsrc) {
+            c->src_edges.assign(c->nsrc, {});
+            c->src_ws.assign(c->nsrc, {});
+        }
+        if (isrc < 0 || isrc >= c->nsrc) return fail(c, TF_ERR_ARG, "source index out of range");
         if (nseg <= 0 || !seg_vex) return fail(c, TF_ERR_ARG, "no source segments");
         if (source_type != 1 || pulse_type < 1 || pulse_type > 3)
             return fail(c, TF_ERR_INPUT, "no such kind of source");
-        c->src_edge.assign(nseg, 0);
-        c->src_w.assign(nseg, 0.0);
+        bool any = false;
+        for (auto &e : c->src_edges) any |= !e.empty();
+        if (any && (source_type != c->stype || pulse_type != c->ptype || pulse_width != c->pulse_width ||
+                    current != c->current))
+            return fail(c, TF_ERR_INPUT, "all sources of one run share the waveform");
+        std::vector<int> edges(nseg);
+        std::vector<double> ws(nseg);
         for (int i = 0; i < nseg; i++) {
             int e = tf::find_edge(c->topo, seg_vex[2 * i], seg_vex[2 * i + 1]);
             if (e < 0) return fail(c, TF_ERR_INPUT, "source segment " + std::to_string(i + 1) + " is not a mesh edge");
-            c->src_edge[i] = e;
-            double drc = (i == nseg - 1) ? -1.0 : 1.0;  // src/FETD_FWD.f90:446-448
-            c->src_w[i] = drc * c->topo.edg_len[e];
+            edges[i] = e;
+            const double drc = (i == nseg - 1) ? -1.0 : 1.0;  // src/FETD_FWD.f90:446-448
+            ws[i] = drc * c->topo.edg_len[e];
         }
-        c->nseg = nseg;
+        c->src_edges[isrc] = edges;
+        c->src_ws[isrc] = ws;
         c->stype = source_type;
         c->ptype = pulse_type;
         c->pulse_width = pulse_width;
         c->current = current;
-        dfree(c->d_src_edge);
-        dfree(c->d_src_w);
-        c->d_src_edge = dupload(c->src_edge);
-        c->d_src_w = dupload(c->src_w);
-        c->have_src = true;
+        bool all = true;
+        for (auto &e : c->src_edges) all &= !e.empty();
+        c->have_src = all;
+        if (all) {
+            c->src_edge.clear(); c->src_id.clear(); c->src_w.clear();
+            for (int q = 0; q < c->nsrc; q++)
+                for (size_t i = 0; i < c->src_edges[q].size(); i++) {
+                    c->src_edge.push_back(c->src_edges[q][i]);
+                    c->src_id.push_back(q);
+                    c->src_w.push_back(c->src_ws[q][i]);
+                }
+            c->nseg = (int)c->src_edge.size();
+            dfree(c->d_src_edge); dfree(c->d_src_id); dfree(c->d_src_w);
+            c->d_src_edge = dupload(c->src_edge);
+            c->d_src_id = dupload(c->src_id);
+            c->d_src_w = dupload(c->src_w);
+        }
         return TF_OK;
     });
+}
+
+int tf_set_replay(tf_ctx *c, int nchecks, const int *keep) {
+    if (!c) return TF_ERR_ARG;
+    if (nchecks < 0 || (nchecks > 0 && !keep)) return fail(c, TF_ERR_ARG, "bad replay schedule");
+    c->replay.assign(keep, keep + nchecks);
+    return TF_OK;
 }
 
 int tf_set_receivers(tf_ctx *c, int nR, const double *xyz, double rec_charac, int *rec_tet_out) {
@@ -987,11 +1162,26 @@ int tf_solve(tf_ctx *c, int slot, int nrhs, const double *b, double *x) {
     return guarded(c, [&] {
         if (slot < 0 || slot > 1 || !c->slot_live[slot]) return fail(c, TF_ERR_STATE, "slot not factored");
         if (nrhs <= 0 || !b || !x) return fail(c, TF_ERR_ARG, "bad right-hand side");
-        int n = c->topo.nE;
-        for (int r = 0; r < nrhs; r++) {
-            HIPCHK(hipMemcpyAsync(c->d_tmpb, b + (size_t)r * n, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
+        const int n = c->topo.nE;
+        if (nrhs == 1) {
+            HIPCHK(hipMemcpyAsync(c->d_tmpb, b, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
             solve_dev(c, slot, c->d_tmpb, c->d_tmpx);
-            HIPCHK(hipMemcpyAsync(x + (size_t)r * n, c->d_tmpx, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipMemcpyAsync(x, c->d_tmpx, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+        } else {
+            // blocks of up to 64 right-hand sides through the block solve
+            for (int q0 = 0; q0 < nrhs; q0 += 64) {
+                const int K = std::min(64, nrhs - q0);
+                ensure_block(c, K);
+                std::vector<double> h((size_t)n * K);
+                for (int i = 0; i < n; i++)
+                    for (int q = 0; q < K; q++) h[(size_t)i * K + q] = b[(size_t)(q0 + q) * n + i];
+                HIPCHK(hipMemcpyAsync(c->d_mb, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->st));
+                solve_block(c, slot, K, c->d_mb, c->d_mx);
+                HIPCHK(hipMemcpyAsync(h.data(), c->d_mx, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
+                HIPCHK(hipStreamSynchronize(c->st));
+                for (int i = 0; i < n; i++)
+                    for (int q = 0; q < K; q++) x[(size_t)(q0 + q) * n + i] = h[(size_t)i * K + q];
+            }
         }
         HIPCHK(hipStreamSynchronize(c->st));
         HIPCHK(hipGetLastError());
@@ -1020,7 +1210,7 @@ int tf_run(tf_ctx *c, int max_steps, int max_rows, int *nrows, int *row_step, do
         c->chk_step.clear(); c->chk_err.clear(); c->chk_keep.clear();
         auto t0 = std::chrono::steady_clock::now();
         if (c->ptype == 3) {
-            double *res = dalloc<double>(c->topo.nE);
+            double *res = dalloc<double>((size_t)c->topo.nE * c->nsrc);
             run_loop(c, 1, res, max_steps);
             c->nrows = 0;
             c->row_step.clear(); c->row_t.clear(); c->row_dt.clear();
@@ -1038,22 +1228,26 @@ int tf_run(tf_ctx *c, int max_steps, int max_rows, int *nrows, int *row_step, do
             if (row_t) std::copy(c->row_t.begin(), c->row_t.begin() + keep, row_t);
             if (row_dt) std::copy(c->row_dt.begin(), c->row_dt.begin() + keep, row_dt);
             if (fields) {
-                int64_t per = (int64_t)c->nR * 6;
+                const int K = c->nsrc, R = c->nR;
+                const int64_t per = (int64_t)R * 6 * K;
                 std::vector<double> h(per * keep);
                 HIPCHK(hipMemcpy(h.data(), c->d_rows, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
-                // rows: E, dB/dt from the device; B = running sum of dB/dt * dt
-                // (src/FETD_FWD.f90:1602-1621); dB/dt is zero at step 0 (:1541)
+                // fields[row][src][rec][9]: E and dB/dt from the device; dB/dt is
+                // zero at step 0 (src/FETD_FWD.f90:1541); B is the running sum of
+                // dB/dt * dt (:1602-1621)
                 for (int r = 0; r < keep; r++)
-                    for (int i = 0; i < c->nR; i++) {
-                        double *o = fields + ((int64_t)r * c->nR + i) * 9;
-                        const double *src = &h[(r * (int64_t)c->nR + i) * 6];
-                        for (int k = 0; k < 3; k++) {
-                            o[k] = src[k];
-                            o[3 + k] = (c->row_step[r] == 0) ? 0.0 : src[3 + k];
-                            double prev = (r == 0) ? 0.0 : fields[((int64_t)(r - 1) * c->nR + i) * 9 + 6 + k];
-                            o[6 + k] = (r == 0) ? o[3 + k] * c->row_dt[r] : prev + o[3 + k] * c->row_dt[r];
+                    for (int q = 0; q < K; q++)
+                        for (int i = 0; i < R; i++) {
+                            double *o = fields + (((int64_t)r * K + q) * R + i) * 9;
+                            const double *src = &h[(r * (int64_t)K + q) * R * 6 + (int64_t)i * 6];
+                            for (int k = 0; k < 3; k++) {
+                                o[k] = src[k];
+                                o[3 + k] = (c->row_step[r] == 0) ? 0.0 : src[3 + k];
+                                const double prev =
+                                    (r == 0) ? 0.0 : fields[(((int64_t)(r - 1) * K + q) * R + i) * 9 + 6 + k];
+                                o[6 + k] = (r == 0) ? o[3 + k] * c->row_dt[r] : prev + o[3 + k] * c->row_dt[r];
+                            }
                         }
-                    }
             }
         }
         return TF_OK;

@@ -45,6 +45,8 @@ _SIGS = {
     "tf_run": [_P, _I, _I, _P, _P, _P, _P, _P],
     "tf_get_checks": [_P, _I, _P, _P, _P, _P],
     "tf_get_timers": [_P, _P],
+    "tf_set_num_sources": [_P, _I],
+    "tf_set_replay": [_P, _I, _P],
 }
 
 
@@ -84,6 +86,7 @@ class Fetd:
         if rc != 0:
             raise FetdError(f"tf_create failed ({rc}): no usable GPU device {device}")
         self.nR = 0
+        self.nsrc = 1
 
     def close(self):
         if self.ctx:
@@ -132,6 +135,17 @@ class Fetd:
         sv = np.ascontiguousarray(seg_vex, np.int32).reshape(-1, 2)
         self._chk(self.lib.tf_set_source(self.ctx, isrc, len(sv), _ptr(sv), source_type, pulse_type,
                                          pulse_width, current), "tf_set_source")
+
+    def set_sources(self, loops, source_type, pulse_type, pulse_width, current):
+        """Several transmitter loops sharing one waveform (block solve)."""
+        self._chk(self.lib.tf_set_num_sources(self.ctx, len(loops)), "tf_set_num_sources")
+        self.nsrc = len(loops)
+        for q, segs in enumerate(loops):
+            self.set_source(segs, source_type, pulse_type, pulse_width, current, isrc=q)
+
+    def set_replay(self, keep):
+        k = np.ascontiguousarray(keep if keep is not None else [], np.int32)
+        self._chk(self.lib.tf_set_replay(self.ctx, len(k), _ptr(k) if len(k) else None), "tf_set_replay")
 
     def set_receivers(self, xyz, rec_charac=10.0):
         r = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
@@ -193,16 +207,18 @@ class Fetd:
         self._chk(self.lib.tf_release(self.ctx, slot), "tf_release")
 
     # -- time_solver ---------------------------------------------------------
-    def run(self, max_steps=0, max_rows=200000):
+    def run(self, max_steps=0, max_rows=20000):
         n = _I()
         steps = np.empty(max_rows, np.int32)
         t = np.empty(max_rows)
         dt = np.empty(max_rows)
-        f = np.empty((max_rows, self.nR, 9))
+        f = np.empty((max_rows, self.nsrc, self.nR, 9))
         self._chk(self.lib.tf_run(self.ctx, max_steps, max_rows, ctypes.byref(n), _ptr(steps), _ptr(t), _ptr(dt),
                                   _ptr(f)), "tf_run")
         k = min(n.value, max_rows)
-        return dict(step=steps[:k].copy(), t=t[:k].copy(), dt=dt[:k].copy(), fields=f[:k].copy())
+        fields = f[:k].copy()
+        return dict(step=steps[:k].copy(), t=t[:k].copy(), dt=dt[:k].copy(),
+                    fields=fields[:, 0] if self.nsrc == 1 else fields)
 
     def checks(self):
         n = _I()

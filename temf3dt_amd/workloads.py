@@ -146,3 +146,36 @@ def plain_3d_model(scale=1.0):
                  blocks=[(50, 250, -100, 100, 100, 200, 1.0)], host_sigma=0.01, air_sigma=1e-12,
                  source=(1, 1, 1.0e-6, 1.0), sp_division=10, time_maximum=1.0e-2, sp_double=100, sp_dblesize=2,
                  time_points=gates, rec_charac=60.0)
+
+
+def multi_source_small_model(nside=2):
+    """nside x nside 20 m loops on a 50 m grid over a half-space with a
+    block, one coincident receiver 1 m above each loop centre (tests)."""
+    cs = [-25.0 * (nside - 1) + 50.0 * i for i in range(nside)]
+    xs = axis(-1500, 1500, cs[0] - 20, cs[-1] + 20, 10.0, 2.5)
+    ys = axis(-1500, 1500, cs[0] - 20, cs[-1] + 20, 10.0, 2.5)
+    zs = axis(-1500, 1500, -20, 40, 10.0, 2.5)
+    loops = [(cx - 10, cx + 10, cy - 10, cy + 10, 0.0) for cy in cs for cx in cs]
+    rec = [[cx, cy, -1.0] for cy in cs for cx in cs]
+    return build(xs, ys, zs, loops=loops, receivers=rec, blocks=[(-30, 30, -30, 30, 20, 50, 0.5)],
+                 host_sigma=0.01, air_sigma=1e-8, source=(1, 1, 1.0e-6, 1.0), sp_division=10,
+                 time_maximum=5e-5, sp_double=12, sp_dblesize=2, rec_charac=30.0)
+
+
+def batched_3d_model(nside=8):
+    """Multi-transmitter model: the 3-D earth of plain_3d_model (0.01 S/m
+    host, 1 S/m block, 1e-12 S/m air) with nside x nside 20 m square loops
+    (1 A) on a 50 m grid, each with a coincident receiver 1 m above its
+    centre; 10 m cells under every transmitter, ~1.5M tets at nside 8.
+    Gates 1e-6..1e-3 s."""
+    cs = [-25.0 * (nside - 1) + 50.0 * i for i in range(nside)]
+    lo, hi = cs[0] - 20.0, cs[-1] + 20.0
+    xs = axis(-50000, 50000, lo, hi, 10.0, 1.75)
+    ys = axis(-50000, 50000, lo, hi, 10.0, 1.75)
+    zs = axis(-50000, 50000, -20, 200, 10.0, 1.75)
+    loops = [(cx - 10, cx + 10, cy - 10, cy + 10, 0.0) for cy in cs for cx in cs]
+    rec = [[cx, cy, -1.0] for cy in cs for cx in cs]
+    gates = list(np.logspace(-6, -3, 40))
+    return build(xs, ys, zs, loops=loops, receivers=rec, blocks=[(50, 250, -100, 100, 100, 200, 1.0)],
+                 host_sigma=0.01, air_sigma=1e-12, source=(1, 1, 1.0e-7, 1.0), sp_division=10,
+                 time_maximum=1.0e-3, sp_double=100, sp_dblesize=2, time_points=gates, rec_charac=20.0)

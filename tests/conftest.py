@@ -36,3 +36,22 @@ def oracle_mod_big(src_mod_big):
     o = pyoracle.Oracle(src_mod_big["mesh"], src_mod_big["cntl"])
     o.assemble()
     return o
+
+
+def gpu_from_model(m, loops=None, device=0):
+    """Fetd context for a workloads.Model; `loops` selects transmitters
+    (default: all of them, run as one block)."""
+    from temf3dt_amd import fetd
+    loops = list(range(len(m.loops))) if loops is None else list(loops)
+    fe = fetd.Fetd(device)
+    fe.set_mesh(m.xyz, m.tet_vex)
+    sig, imu = m.sigma_inv_mu()
+    fe.set_materials(sig, imu)
+    if len(loops) == 1:
+        fe.set_source(m.loops[loops[0]], *m.source)
+    else:
+        fe.set_sources([m.loops[q] for q in loops], *m.source)
+    fe.set_receivers(m.receivers, m.rec_charac)
+    fe.set_schedule(m.sp_division, m.sp_double, m.sp_dblesize, m.time_maximum)
+    fe.assemble()
+    return fe
