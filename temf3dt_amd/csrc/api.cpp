@@ -55,41 +55,45 @@ void launch_fwd_small(const double *L, const FrontDesc *fd, const int *list, int
                       hipStream_t st);
 void launch_bwd_small(const double *L, const FrontDesc *fd, const int *list, int cnt, const double *Y,
                       const double *Dn, const int *sidx, double *xw, hipStream_t st);
-void launch_fwd_gather(const FrontDesc *fd, const int *list, int cnt, const double *b, const int *perm,
-                       const int *chp, const int *ch, const int *rel, double *V, hipStream_t st);
 void launch_fwd_own(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
                     int64_t nwg, const double *V, double *Y, hipStream_t st);
 void launch_fwd_struct(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
                        int64_t nwg, double *V, const double *Y, hipStream_t st);
-void launch_bwd_gather(const FrontDesc *fd, const int *list, int cnt, const int *sidx, const double *xw, double *V,
-                       hipStream_t st);
 void launch_bwd_t(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
                   const double *Y, const double *Dn, const double *V, double *T, hipStream_t st);
 void launch_bwd_x(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
                   const double *T, double *xw, hipStream_t st);
 void launch_unpermute(int n, const int *perm, const double *xw, double *x, hipStream_t st);
 // msolve.hip
-void launch_mfwd_gather(const FrontDesc *fd, const int *list, int cnt, const double *b, const int *perm,
-                        const int *chp, const int *ch, const int *rel, double *V, int K, hipStream_t st);
-void launch_mbwd_gather(const FrontDesc *fd, const int *list, int cnt, const int *sidx, const double *xw, double *V,
-                        int K, hipStream_t st);
 void launch_mfwd_own(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
                      int64_t nwg, const double *V, double *Y, int K, hipStream_t st);
-void launch_mfwd_struct(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
-                        int64_t nwg, double *V, const double *Y, int K, hipStream_t st);
-void launch_mbwd_t(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
-                   const double *Y, const double *Dn, const double *V, double *T, int K, hipStream_t st);
 void launch_mbwd_x(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
                    const double *T, double *xw, int K, hipStream_t st);
 void launch_munpermute(int n, int K, const int *perm, const double *xw, double *x, hipStream_t st);
 void launch_mspmv_rhs(int n, int K, const int *ia, const int *ja, const double *val, const double *e2,
-                      const double *e3, double *r, hipStream_t st);
+                      const double *e3, double *w, double *r, hipStream_t st);
 void launch_msource(int ns, int K, const int *edge, const int *src, const double *w, double didt, double cdt,
                     double *r, hipStream_t st);
 void launch_mreadout(int nR, int K, const int *rec_edge, const double *rec_drc, const double *n_arr,
                      const double *d_arr, const double *e, double *out, hipStream_t st);
 void launch_mdiffnorm(int64_t n, int K, const double *a, const double *b, double *part, double *out,
                       hipStream_t st);
+void launch_mg_init(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg, const double *b,
+                    const int *perm, double *V, int K, hipStream_t st);
+void launch_mg_child(const FrontDesc *fd, const int *parent, const int *list, const int64_t *pref, int cnt, int64_t nwg,
+                     const int *rel, double *V, int K, hipStream_t st);
+void launch_mg_xs(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg, const int *sidx,
+                  const double *xw, double *V, int K, hipStream_t st);
+void launch_mfwd_struct_part(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
+                             int64_t nwg, const double *Y, int K, double *part, double *V, hipStream_t st);
+void launch_mfwd_struct_red(const FrontDesc *fd, const int *list, const int64_t *pref, const int64_t *ppref, int cnt,
+                            int64_t nwg, const double *part, double *V, int K, hipStream_t st);
+void launch_mbwd_t_part(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
+                        const double *V, int K, double *part, const double *Y, const double *Dn, double *T,
+                        hipStream_t st);
+void launch_mbwd_t_red(const FrontDesc *fd, const int *list, const int64_t *pref, const int64_t *ppref, int cnt,
+                       int64_t nwg, const double *part, const double *Y, const double *Dn, double *T, int K,
+                       hipStream_t st);
 // step.hip
 void launch_spmv_rhs(int n, const int *ia, const int *ja, const double *val, const double *e2, const double *e3,
                      double *r, hipStream_t st);
@@ -134,13 +138,14 @@ void dfree(T *&p) {
 
 enum LaunchKind {
     L_EXTADD, L_FFUSED, L_FDIAG, L_FTRSM, L_FUPD, L_IW, L_IAPPLY, L_IDIAG,
-    L_SSMALL, L_SGATHER, L_SOWN, L_SSTRUCT,
-    L_BSMALL, L_BGATHER, L_BT, L_BX,
-    L_MFGATHER, L_MFOWN, L_MFSTRUCT, L_MBGATHER, L_MBT, L_MBX
+    L_SSMALL, L_SOWN, L_SSTRUCT,
+    L_BSMALL, L_BT, L_BX,
+    L_GINIT, L_GCHILD, L_GXS, L_MFOWN, L_MFSPART, L_MFSRED, L_MBTPART, L_MBTRED, L_MBX
 };
 struct Launch {
     int kind, p0, off, cnt, poff;
     int64_t nwg;
+    int aux;  // reductions: offset of the paired partial launch's prefix sums
 };
 
 // Execution plan over the front tree: which fronts each launch touches.
@@ -150,6 +155,7 @@ struct Plan {
     std::vector<int64_t> pref;
     std::vector<int64_t> woff;  // per supernode: offset in the inversion scratch
     int64_t wsize = 0;
+    int64_t part_max = 0;       // partial-sum tiles (4096 doubles each) of one launch
 };
 
 static const double PI = 3.141592653589793;
@@ -217,6 +223,7 @@ struct tf_ctx {
     double *d_src_w = nullptr;
     // block-solve work (K right-hand sides)
     double *d_mV = nullptr, *d_mY = nullptr, *d_mT = nullptr, *d_mxw = nullptr, *d_mb = nullptr, *d_mx = nullptr;
+    double *d_part = nullptr;
     int mK = 0;
     int64_t vrows = 0;
     // solver
@@ -257,6 +264,7 @@ void free_solver(tf_ctx *c) {
     for (int s = 0; s < 2; s++) { dfree(c->d_L[s]); dfree(c->d_D[s]); c->slot_live[s] = false; }
     dfree(c->d_tmpb); dfree(c->d_tmpx);
     dfree(c->d_mV); dfree(c->d_mY); dfree(c->d_mT); dfree(c->d_mxw); dfree(c->d_mb); dfree(c->d_mx);
+    dfree(c->d_part);
     c->mK = 0;
     c->analyzed = false;
 }
@@ -305,7 +313,7 @@ void build_plan(tf_ctx *c) {
     auto m_of = [&](int k) { return (int)(s.sptr[k + 1] - s.sptr[k]); };
     auto add = [&](std::vector<Launch> &dst, int kind, int p0, const std::vector<int> &fr,
                    const std::vector<int64_t> &units) {
-        Launch L{kind, p0, (int)p.list.size(), (int)fr.size(), (int)p.pref.size(), 0};
+        Launch L{kind, p0, (int)p.list.size(), (int)fr.size(), (int)p.pref.size(), 0, 0};
         p.list.insert(p.list.end(), fr.begin(), fr.end());
         int64_t acc = 0;
         for (size_t i = 0; i < fr.size(); i++) {
@@ -315,6 +323,42 @@ void build_plan(tf_ctx *c) {
         p.pref.push_back(acc);
         L.nwg = acc;
         if (!fr.empty() && acc > 0) dst.push_back(L);
+    };
+    // v = [b(own); 0] + extend-add of the children's update vectors, for the
+    // fronts `fr` of one level: one init launch, then one launch per child
+    // rank (children of one parent never share a launch)
+    auto gathers = [&](std::vector<Launch> &dst, const std::vector<int> &fr) {
+        std::vector<int64_t> ui;
+        int maxch = 0;
+        for (int k : fr) {
+            ui.push_back((f_of(k) + 31) / 32);
+            maxch = std::max(maxch, s.children_ptr[k + 1] - s.children_ptr[k]);
+        }
+        add(dst, L_GINIT, 0, fr, ui);
+        for (int r = 0; r < maxch; r++) {
+            std::vector<int> chl;
+            std::vector<int64_t> uc;
+            for (int k : fr)
+                if (s.children_ptr[k + 1] - s.children_ptr[k] > r) {
+                    int ch = s.children[s.children_ptr[k] + r];
+                    if (m_of(ch) > 0) { chl.push_back(ch); uc.push_back((m_of(ch) + 31) / 32); }
+                }
+            add(dst, L_GCHILD, 0, chl, uc);
+        }
+    };
+    // partial-sum launch followed by its reduction; the reduction reads the
+    // partial launch's prefix sums (aux) to find each front's tiles
+    auto add_part_red = [&](std::vector<Launch> &dst, int kpart, int kred, const std::vector<int> &fr,
+                            const std::vector<int64_t> &upart, const std::vector<int64_t> &ured) {
+        size_t before = dst.size();
+        add(dst, kpart, 0, fr, upart);
+        int aux = 0;
+        if (dst.size() > before) {
+            aux = dst.back().poff;
+            p.part_max = std::max(p.part_max, dst.back().nwg);
+        }
+        add(dst, kred, 0, fr, ured);
+        if (!dst.empty() && dst.back().kind == kred) dst.back().aux = aux;
     };
     // small fronts (one workgroup each) vs large fronts (grid launches)
     std::vector<std::vector<int>> small(s.nlevels), grid(s.nlevels);
@@ -394,7 +438,7 @@ void build_plan(tf_ctx *c) {
             uo.push_back((s.nc[k] + 63) / 64);
             us.push_back((m_of(k) + 63) / 64);
         }
-        add(p.fwd, L_SGATHER, 0, grid[l], {});
+        gathers(p.fwd, grid[l]);
         add(p.fwd, L_SOWN, 0, grid[l], uo);
         add(p.fwd, L_SSTRUCT, 0, grid[l], us);
     }
@@ -402,7 +446,9 @@ void build_plan(tf_ctx *c) {
     for (int l = s.nlevels - 1; l >= 0; l--) {
         std::vector<int64_t> uc;
         for (int k : grid[l]) uc.push_back((s.nc[k] + 15) / 16);
-        add(p.bwd, L_BGATHER, 0, grid[l], {});
+        std::vector<int64_t> ux;
+        for (int k : grid[l]) ux.push_back((m_of(k) + 31) / 32);
+        add(p.bwd, L_GXS, 0, grid[l], ux);
         add(p.bwd, L_BT, 0, grid[l], uc);
         add(p.bwd, L_BX, 0, grid[l], uc);
         add(p.bwd, L_BSMALL, 0, small[l], {});
@@ -410,25 +456,35 @@ void build_plan(tf_ctx *c) {
     // block solve for K right-hand sides: every front through the tile kernels
     for (int l = 0; l < s.nlevels; l++) {
         std::vector<int> all(s.lev_sn.begin() + s.lev_ptr[l], s.lev_sn.begin() + s.lev_ptr[l + 1]);
-        std::vector<int64_t> uo, us;
+        std::vector<int> withm;
+        std::vector<int64_t> uo, up, ur;
         for (int k : all) {
             uo.push_back((s.nc[k] + 63) / 64);
-            us.push_back((m_of(k) + 63) / 64);
+            if (m_of(k) > 0) {
+                withm.push_back(k);
+                const int nch = (s.nc[k] + 511) / 512;
+                up.push_back((int64_t)((m_of(k) + 63) / 64) * nch);
+                ur.push_back(nch > 1 ? (m_of(k) + 63) / 64 : 0);
+            }
         }
-        add(p.mfwd, L_MFGATHER, 0, all, {});
+        gathers(p.mfwd, all);
         add(p.mfwd, L_MFOWN, 0, all, uo);
-        add(p.mfwd, L_MFSTRUCT, 0, all, us);
+        if (!withm.empty()) add_part_red(p.mfwd, L_MFSPART, L_MFSRED, withm, up, ur);
     }
     for (int l = s.nlevels - 1; l >= 0; l--) {
         std::vector<int> all(s.lev_sn.begin() + s.lev_ptr[l], s.lev_sn.begin() + s.lev_ptr[l + 1]);
         std::vector<int> withm;
-        std::vector<int64_t> uc;
+        std::vector<int64_t> ux, up, uc;
+        std::vector<int64_t> ur;
         for (int k : all) {
+            const int nch = (m_of(k) + 511) / 512;
             uc.push_back((s.nc[k] + 63) / 64);
-            if (m_of(k) > 0) withm.push_back(k);
+            up.push_back((int64_t)((s.nc[k] + 63) / 64) * nch);
+            ur.push_back(nch == 1 ? 0 : (s.nc[k] + 63) / 64);  // nch 0 (roots): T = Y/d in the reduction
+            if (m_of(k) > 0) { withm.push_back(k); ux.push_back((m_of(k) + 31) / 32); }
         }
-        add(p.mbwd, L_MBGATHER, 0, withm, {});
-        add(p.mbwd, L_MBT, 0, all, uc);
+        add(p.mbwd, L_GXS, 0, withm, ux);
+        add_part_red(p.mbwd, L_MBTPART, L_MBTRED, all, up, ur);
         add(p.mbwd, L_MBX, 0, all, uc);
     }
 }
@@ -472,6 +528,7 @@ void do_analyze(tf_ctx *c, int leaf) {
     c->d_T = dalloc<double>(s.n);
     c->d_W = dalloc<double>((size_t)std::max<int64_t>(c->plan.wsize, 1));
     c->d_woff = dupload(c->plan.woff);
+    c->d_part = dalloc<double>((size_t)std::max<int64_t>(c->plan.part_max, 1) * 4096);
     c->d_tmpb = dalloc<double>(s.n);
     c->d_tmpx = dalloc<double>(s.n);
     if (!c->d_flag) c->d_flag = dalloc<int>(1);
@@ -551,9 +608,12 @@ void solve_dev(tf_ctx *c, int slot, const double *b, double *x) {
             tf::launch_fwd_small(L, c->d_fd, lst + l.off, l.cnt, b, c->d_perm, c->d_chp, c->d_ch, c->d_rel, c->d_V,
                                  c->d_Y, c->st);
             break;
-        case L_SGATHER:
-            tf::launch_fwd_gather(c->d_fd, lst + l.off, l.cnt, b, c->d_perm, c->d_chp, c->d_ch, c->d_rel, c->d_V,
-                                  c->st);
+        case L_GINIT:
+            tf::launch_mg_init(c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, b, c->d_perm, c->d_V, 1, c->st);
+            break;
+        case L_GCHILD:
+            tf::launch_mg_child(c->d_fd, c->d_parent, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_rel, c->d_V, 1,
+                                c->st);
             break;
         case L_SOWN:
             tf::launch_fwd_own(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_V, c->d_Y, c->st);
@@ -568,8 +628,8 @@ void solve_dev(tf_ctx *c, int slot, const double *b, double *x) {
         case L_BSMALL:
             tf::launch_bwd_small(L, c->d_fd, lst + l.off, l.cnt, c->d_Y, Dn, c->d_sidx, c->d_xw, c->st);
             break;
-        case L_BGATHER:
-            tf::launch_bwd_gather(c->d_fd, lst + l.off, l.cnt, c->d_sidx, c->d_xw, c->d_V, c->st);
+        case L_GXS:
+            tf::launch_mg_xs(c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_sidx, c->d_xw, c->d_V, 1, c->st);
             break;
         case L_BT:
             tf::launch_bwd_t(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_Y, Dn, c->d_V, c->d_T, c->st);
@@ -604,26 +664,38 @@ void solve_block(tf_ctx *c, int slot, int K, const double *b, double *x) {
     const int64_t *pr = c->d_ppref;
     for (const Launch &l : p.mfwd) {
         switch (l.kind) {
-        case L_MFGATHER:
-            tf::launch_mfwd_gather(c->d_fd, lst + l.off, l.cnt, b, c->d_perm, c->d_chp, c->d_ch, c->d_rel, c->d_mV,
-                                   K, c->st);
+        case L_GINIT:
+            tf::launch_mg_init(c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, b, c->d_perm, c->d_mV, K, c->st);
+            break;
+        case L_GCHILD:
+            tf::launch_mg_child(c->d_fd, c->d_parent, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_rel, c->d_mV, K,
+                                c->st);
             break;
         case L_MFOWN:
             tf::launch_mfwd_own(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_mV, c->d_mY, K, c->st);
             break;
-        case L_MFSTRUCT:
-            tf::launch_mfwd_struct(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_mV, c->d_mY, K, c->st);
+        case L_MFSPART:
+            tf::launch_mfwd_struct_part(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_mY, K, c->d_part,
+                                        c->d_mV, c->st);
+            break;
+        case L_MFSRED:
+            tf::launch_mfwd_struct_red(c->d_fd, lst + l.off, pr + l.poff, pr + l.aux, l.cnt, l.nwg, c->d_part,
+                                       c->d_mV, K, c->st);
             break;
         }
     }
     for (const Launch &l : p.mbwd) {
         switch (l.kind) {
-        case L_MBGATHER:
-            tf::launch_mbwd_gather(c->d_fd, lst + l.off, l.cnt, c->d_sidx, c->d_mxw, c->d_mV, K, c->st);
+        case L_GXS:
+            tf::launch_mg_xs(c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_sidx, c->d_mxw, c->d_mV, K, c->st);
             break;
-        case L_MBT:
-            tf::launch_mbwd_t(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_mY, Dn, c->d_mV, c->d_mT, K,
-                              c->st);
+        case L_MBTPART:
+            tf::launch_mbwd_t_part(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_mV, K, c->d_part, c->d_mY,
+                                   Dn, c->d_mT, c->st);
+            break;
+        case L_MBTRED:
+            tf::launch_mbwd_t_red(c->d_fd, lst + l.off, pr + l.poff, pr + l.aux, l.cnt, l.nwg, c->d_part, c->d_mY, Dn,
+                                  c->d_mT, K, c->st);
             break;
         case L_MBX:
             tf::launch_mbwd_x(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_mT, c->d_mxw, K, c->st);
@@ -660,7 +732,7 @@ void ensure_vecs(tf_ctx *c, StepVecs &v) {
 void rhs(tf_ctx *c, int step, double t, const double *e2, const double *e3, double dt, bool negate, double *r) {
     const int K = c->nsrc;
     if (K == 1) tf::launch_spmv_rhs(c->topo.nE, c->d_fia, c->d_fja, c->d_fval, e2, e3, r, c->st);
-    else tf::launch_mspmv_rhs(c->topo.nE, K, c->d_fia, c->d_fja, c->d_fval, e2, e3, r, c->st);
+    else tf::launch_mspmv_rhs(c->topo.nE, K, c->d_fia, c->d_fja, c->d_fval, e2, e3, c->d_mb, r, c->st);
     if (step < c->sp_division) {  // sys_right: source on while step < sp_division
         double didt = source_didt(c->stype, c->ptype, t, c->pulse_width, c->current);
         if (negate) didt = -didt;
@@ -725,6 +797,7 @@ double elapsed(hipEvent_t a, hipEvent_t b) {
 void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
     StepVecs v;
     ensure_vecs(c, v);
+    if (c->nsrc > 1) ensure_block(c, c->nsrc);
     const int n = c->topo.nE;
     const int dbl = c->sp_dblesize;
     if (c->sp_double < dbl + 2) throw std::runtime_error("sp_double must be at least sp_dblesize+2");

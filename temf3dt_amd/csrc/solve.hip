@@ -145,12 +145,6 @@ __global__ void __launch_bounds__(TF_NT) k_bwd_small(const double *L, const Fron
 
 // ---- kernels: large fronts, several workgroups per front --------------------------
 
-__global__ void __launch_bounds__(TF_NT) k_fwd_gather(const FrontDesc *fd, const int *list, const double *b,
-                                                      const int *perm, const int *chp, const int *ch,
-                                                      const int *rel, double *V) {
-    blk_fwd_gather(fd, list[blockIdx.x], b, perm, chp, ch, rel, V);
-}
-
 // y_own rows [64w, 64w+64): y = v + sum_{j<r} X[r][j] v[j]
 __global__ void __launch_bounds__(TF_NT) k_fwd_own(const double *L, const FrontDesc *fd, const int *list,
                                                    const int64_t *pref, int cnt, const double *V, double *Y) {
@@ -174,14 +168,6 @@ __global__ void __launch_bounds__(TF_NT) k_fwd_struct(const double *L, const Fro
     const int rr = r < d.f ? r : d.f - 1;
     double acc = blk_gemv_rows(L + d.loff, d.f, rr, d.nc, Y + d.c0, red);
     if (threadIdx.x < 64 && r < d.f) V[d.voff + r] -= acc;
-}
-
-// x_struct of every large front gathered into its (now free) update-vector
-// slot of V, so the column sweeps below read it contiguously.
-__global__ void __launch_bounds__(TF_NT) k_bwd_gather(const FrontDesc *fd, const int *list, const int *sidx,
-                                                      const double *xw, double *V) {
-    const FrontDesc d = fd[list[blockIdx.x]];
-    for (int r = threadIdx.x; r < d.m; r += blockDim.x) V[d.voff + d.nc + r] = xw[sidx[d.soff + r]];
 }
 
 // dot(col[lo:hi], x[lo:hi]) by one wave, four independent accumulators
@@ -250,11 +236,6 @@ void launch_bwd_small(const double *L, const FrontDesc *fd, const int *list, int
                       const double *Dn, const int *sidx, double *xw, hipStream_t st) {
     if (cnt > 0) hipLaunchKernelGGL(k_bwd_small, dim3(cnt), dim3(TF_NT), 0, st, L, fd, list, Y, Dn, sidx, xw);
 }
-void launch_fwd_gather(const FrontDesc *fd, const int *list, int cnt, const double *b, const int *perm,
-                       const int *chp, const int *ch, const int *rel, double *V, hipStream_t st) {
-    if (cnt > 0)
-        hipLaunchKernelGGL(k_fwd_gather, dim3(cnt), dim3(TF_NT), 0, st, fd, list, b, perm, chp, ch, rel, V);
-}
 void launch_fwd_own(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
                     int64_t nwg, const double *V, double *Y, hipStream_t st) {
     if (nwg > 0)
@@ -264,10 +245,6 @@ void launch_fwd_struct(const double *L, const FrontDesc *fd, const int *list, co
                        int64_t nwg, double *V, const double *Y, hipStream_t st) {
     if (nwg > 0)
         hipLaunchKernelGGL(k_fwd_struct, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt, V, Y);
-}
-void launch_bwd_gather(const FrontDesc *fd, const int *list, int cnt, const int *sidx, const double *xw, double *V,
-                       hipStream_t st) {
-    if (cnt > 0) hipLaunchKernelGGL(k_bwd_gather, dim3(cnt), dim3(TF_NT), 0, st, fd, list, sidx, xw, V);
 }
 void launch_bwd_t(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
                   const double *Y, const double *Dn, const double *V, double *T, hipStream_t st) {
