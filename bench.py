@@ -57,13 +57,18 @@ def setup_gpu(m, loops):
     return fe, time.time() - t
 
 
-def timed_run(fe, steps, warmup):
+def timed_run(fe, steps, warmup, profile=False):
     if warmup > 0:
         fe.run(max_steps=warmup)
+    if profile:
+        fe.set_profiling(True)
     t0 = time.time()
     rows = fe.run(max_steps=steps)
     wall = time.time() - t0
-    return rows, wall, fe.timers()
+    prof = fe.profile() if profile else None
+    if profile:
+        fe.set_profiling(False)
+    return rows, wall, fe.timers(), prof
 
 
 def cpu_baseline(m, counts, sample_steps):
@@ -110,6 +115,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the tile-kernel autotuner (use defaults)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU sharding of the path is a design in "
@@ -122,27 +128,42 @@ def main():
     n = fe.nE
     print(f"[bench] {a.model}: {m.ntet} tets, {n} edges, nnzU {fe.nnz}, nnzL {st['nnzL']:.3e}, "
           f"factor flops {st['flops']:.3e}, {nsrc} sources, analyze {t_an:.2f} s", file=sys.stderr, flush=True)
-    rows, wall, tm = timed_run(fe, a.steps, a.warmup)
+    tune = None
+    if not a.no_tune and nsrc > 1:
+        from temf3dt_amd import autotune
+        fe.factor(0, m.source[2] / m.sp_division)
+        tune = autotune.autotune(fe, slot=0, nrhs=nsrc, log=lambda x: print(x, file=sys.stderr, flush=True))
+        fe.release(0)
+        print(f"[bench] autotune: {tune['best']} ({tune['best_ms']:.2f} ms per solve, "
+              f"{tune['seconds']:.1f} s)", file=sys.stderr, flush=True)
+    rows, wall, tm, prof = timed_run(fe, a.steps, a.warmup, profile=True)
     steps = int(rows["step"][-1])
     nf = int(tm["nfactor"])
     value = nsrc * steps / wall
     print(f"[bench] batched: {steps} steps x {nsrc} sources in {wall:.3f} s, {nf} factorisations "
           f"({tm['factor']:.3f} s), t_end {rows['t'][-1]:.3e}", file=sys.stderr, flush=True)
-    # block-solve roofline: per step the solves stream L once for all K
-    # sources; flops_k = K (4 nnzL + 2n + 4 nnzU + 6n) per solve
+    print("[bench] kernel time by category (launches, ms): " +
+          ", ".join(f"{k} {v[0]} {v[1]:.1f}" for k, v in prof.items()), file=sys.stderr, flush=True)
+    # Dominant kernel: the block-solve tile products (k_mtile, four forms).
+    # Algorithmic work per solve: the factor's off-diagonal entries
+    # (L21 and the strict part of L11^-1) each enter one forward and one
+    # backward product with K right-hand sides: 4 K nnzL_off flops.
     nsolves = steps + 3 * len(fe.checks()["step"])
-    t_solve = max(wall - tm["factor"], 1e-9)
-    flops_k = nsrc * (4.0 * st["nnzL"] + 8.0 * n + 4.0 * fe.nnz)
-    achieved = flops_k * nsolves / t_solve / 1e12
+    nnzl_off = st["nnzL"] - n
+    flops_solve = 4.0 * nsrc * nnzl_off
+    tile_launches, tile_ms = prof["tile"]
+    achieved = flops_solve * nsolves / (tile_ms * 1e-3) / 1e12
     roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-            "kernel": (f"block solve step (RHS + forward/backward tile products for {nsrc} sources), "
-                       f"{nsolves} solves in the non-factorisation time")}
+            "kernel": (f"k_mtile (block-solve tile products, {nsrc} right-hand sides): {nsolves} solves x "
+                       f"4*K*nnzL_off = {flops_solve:.3e} flops each, {tile_launches} launches, "
+                       f"{tile_ms:.1f} ms (HIP events)"),
+            "step_breakdown_ms": {k: round(v[1], 2) for k, v in prof.items()}}
     fe.close()
     plain = None
     if not a.no_plain:
         fp, _ = setup_gpu(m, [0])
-        prow, pwall, ptm = timed_run(fp, a.steps, a.warmup)
+        prow, pwall, ptm, _ = timed_run(fp, a.steps, a.warmup)
         psteps = int(prow["step"][-1])
         plain = {"value": psteps / pwall, "steps": psteps, "wall_s": pwall, "factor_s": ptm["factor"],
                  "factorisations": int(ptm["nfactor"])}
@@ -158,7 +179,7 @@ def main():
         "config": {"workload": f"{a.model}: {nsrc}-transmitter BDF2 loop, block solve of {nsrc} sources per "
                                f"factorisation, {m.ntet} tets, {n} edges", "mode": "batched", "sources": nsrc,
                    "nnzL": st["nnzL"], "factorisations": nf,
-                   "plain_single_source": plain,
+                   "plain_single_source": plain, "autotune": tune and tune["best"],
                    "batched_over_plain": (value / plain["value"]) if plain else None},
         "roofline": roof, "cpu_baseline": cpu,
     }

@@ -44,6 +44,8 @@ typedef struct tf_ctx tf_ctx;
 /* Create a context on HIP device `device`. */
 int tf_create(tf_ctx **ctx, int device);
 void tf_destroy(tf_ctx *ctx);
+/* Message of the last failed call on ctx; with ctx == NULL, of the last
+ * failed tf_create on this thread. */
 const char *tf_last_error(const tf_ctx *ctx);
 
 /* model_mesh (src/FETD_FWD.f90:112-282): vertex coordinates [nN][3] and
@@ -91,6 +93,29 @@ int tf_get_receiver_vectors(const tf_ctx *ctx, double *n_arr, double *d_arr);
  * phase-12 call).  Optional: tf_factor runs it on first use.  leaf_size <= 0
  * keeps the default. */
 int tf_analyze(tf_ctx *ctx, int leaf_size);
+/* Tuning knobs of the block-solve tile kernels (no reference counterpart;
+ * set by the autotuner, temf3dt_amd/autotune.py).  Keys:
+ *   "mt_units"  work units a tile launch aims for (default 4096, >= 64)
+ *   "mt_chmin"  minimum inner chunk of a split tile (default 256, multiple of 16)
+ *   "mt_ks_n"   k-steps per stage of the N-form kernels (2 or 4, default 2)
+ *   "mt_ks_t"   k-steps per stage of the T-form kernels (2 or 4, default 4)
+ * Results do not depend on the knobs beyond floating-point summation order. */
+int tf_set_tuning(tf_ctx *ctx, const char *key, int value);
+/* Kernel timing with HIP events on the context's stream (no reference
+ * counterpart).  tf_set_profiling(ctx, 1) clears the totals and starts
+ * recording an event pair around every kernel launch of the factorisation,
+ * the solves and the time step; tf_get_profile returns the number of
+ * launches and their summed duration (ms) in one category. */
+#define TF_PROF_TILE 0           /* block-solve tile products (k_mtile) */
+#define TF_PROF_TILE_RED 1       /* their partial-sum reductions */
+#define TF_PROF_PULL 2           /* block-solve forward pull */
+#define TF_PROF_FACTOR_UPDATE 3  /* trailing updates of the factorisation */
+#define TF_PROF_FACTOR_OTHER 4   /* rest of the factorisation */
+#define TF_PROF_SOLVE1 5         /* single right-hand-side solve kernels */
+#define TF_PROF_STEP 6           /* right-hand side: SpMV, history, source */
+#define TF_PROF_NCAT 7
+int tf_set_profiling(tf_ctx *ctx, int on);
+int tf_get_profile(tf_ctx *ctx, int category, int64_t *launches, double *ms);
 /* Factor statistics: nnz(L) incl. diagonal, dense-front flops, supernodes. */
 int tf_factor_stats(const tf_ctx *ctx, int64_t *nnzL, double *flops, int *nsuper, int *nlevels);
 
@@ -103,6 +128,11 @@ int tf_factor_csr(tf_ctx *ctx, int slot, int n, const int *ia, const int *ja, co
 /* pardiso_direct_2: x = P^-1 b for nrhs right-hand sides, host buffers in
  * edge order, [nrhs][n]; nrhs > 1 runs the block solve, 64 at a time. */
 int tf_solve(tf_ctx *ctx, int slot, int nrhs, const double *b, double *x);
+/* Device-resident solve timing for the autotuner (no reference counterpart):
+ * nrhs (1..64) deterministic right-hand sides already in HBM, one warm-up
+ * solve, then reps solves timed with HIP events; *ms_per_solve receives the
+ * average. */
+int tf_time_solve(tf_ctx *ctx, int slot, int nrhs, int reps, double *ms_per_solve);
 /* pardiso_direct_3 */
 int tf_release(tf_ctx *ctx, int slot);
 

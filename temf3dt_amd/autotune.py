@@ -1,0 +1,53 @@
+"""Autotuner for the block-solve tile kernels.
+
+The K-source block solve (mtile.hip) has four knobs (tf_set_tuning):
+  mt_ks_n / mt_ks_t  k-steps per register stage of the N- and T-form tile
+                     kernels (2: two stages in flight, 4: one wider stage)
+  mt_units           work units a tile launch aims for (sets how far long
+                     inner ranges are split into partial tiles)
+  mt_chmin           smallest inner chunk of a split tile
+The best setting depends on the front-size profile of the mesh's elimination
+tree, so it is measured on the model itself: after a factorisation, every
+candidate setting times `reps` device-resident solves (tf_time_solve, HIP
+events) and the fastest is kept.  The right-hand-side block width is reported
+the same way (source-solves per second at each width), since the sources of
+a run share one block of at most 64.
+"""
+import itertools
+import time
+
+DEFAULT_SPACE = {
+    "mt_ks_n": (2, 4),
+    "mt_ks_t": (2, 4),
+    "mt_units": (2048, 4096, 8192),
+    "mt_chmin": (256, 512),
+}
+
+
+def autotune(fe, slot=0, nrhs=64, reps=3, space=None, widths=(16, 32, 64), log=None):
+    """Tune fe (a factored temf3dt_amd.fetd.Fetd) in place.
+
+    Returns dict(best=settings, best_ms=..., table=[(settings, ms)], widths=[(w, ms, solves_per_s)],
+    seconds=tuning wall time)."""
+    t0 = time.time()
+    space = space or DEFAULT_SPACE
+    keys = list(space)
+    table = []
+    for vals in itertools.product(*(space[k] for k in keys)):
+        cfg = dict(zip(keys, vals))
+        for k, v in cfg.items():
+            fe.set_tuning(k, v)
+        ms = fe.time_solve(slot, nrhs, reps)
+        table.append((cfg, ms))
+        if log:
+            log(f"[autotune] {cfg} -> {ms:.3f} ms per {nrhs}-RHS solve")
+    best, best_ms = min(table, key=lambda x: x[1])
+    for k, v in best.items():
+        fe.set_tuning(k, v)
+    wtab = []
+    for w in widths:
+        ms = fe.time_solve(slot, w, reps)
+        wtab.append((w, ms, w / ms * 1e3))
+        if log:
+            log(f"[autotune] width {w}: {ms:.3f} ms, {w / ms * 1e3:.0f} source-solves/s")
+    return dict(best=best, best_ms=best_ms, table=table, widths=wtab, seconds=time.time() - t0)

@@ -69,6 +69,7 @@ def build(jobs=8, verbose=False):
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
     build_host_probe()
+    build_fortran()
     return LIB
 
 
@@ -86,6 +87,39 @@ def build_host_probe():
         raise RuntimeError(f"host probe build failed:\n{r.stderr}")
     os.replace(tmp, HOST_LIB)
     return HOST_LIB
+
+
+FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
+FORTRAN = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(HERE, "fetd_amd")
+FIO_CHECK = os.path.join(HERE, "fetd_io_check")
+
+
+def build_fortran():
+    """Fortran driver `fetd_amd -f ROOT` over the C ABI (ISO_C_BINDING), and
+    `fetd_io_check`, the Fortran reader/writer alone (no GPU), for tests.
+    Skipped (returns None) when no Fortran compiler is installed."""
+    if not os.path.exists(FLANG):
+        return None
+    mods = os.path.join(BUILD, "fmod")
+    os.makedirs(mods, exist_ok=True)
+    objs = {}
+    for name in ("fetd_io", "temfetd", "fetd_main", "fetd_io_check"):
+        obj = os.path.join(BUILD, name + ".f.o")
+        cmd = [FLANG, "-O2", "-J", mods, "-c", os.path.join(FORTRAN, name + ".f90"), "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed for {name}:\n{r.stdout}\n{r.stderr}")
+        objs[name] = obj
+    for exe, parts, libs in ((DRIVER, ("fetd_main", "fetd_io", "temfetd"), ["-L", HERE, "-ltemfetd",
+                                                                          "-Wl,-rpath,$ORIGIN"]),
+                             (FIO_CHECK, ("fetd_io_check", "fetd_io"), [])):
+        cmd = [FLANG, "-o", exe + ".tmp"] + [objs[p] for p in parts] + libs
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link of {exe} failed:\n{r.stdout}\n{r.stderr}")
+        os.replace(exe + ".tmp", exe)
+    return DRIVER
 
 
 def main():

@@ -65,11 +65,17 @@ void launch_bwd_x(const double *L, const FrontDesc *fd, const int *list, const i
                   const double *T, double *xw, hipStream_t st);
 void launch_unpermute(int n, const int *perm, const double *xw, double *x, hipStream_t st);
 // msolve.hip
-void launch_mfwd_own(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
-                     int64_t nwg, const double *V, double *Y, int K, hipStream_t st);
-void launch_mbwd_x(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
-                   const double *T, double *xw, int K, hipStream_t st);
 void launch_munpermute(int n, int K, const int *perm, const double *xw, double *x, hipStream_t st);
+void launch_fill_pattern(int64_t len, double *b, hipStream_t st);
+// mtile.hip
+void launch_mtile(int mode, int ks, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits, int K,
+                  const double *in0, const double *in1, const double *Dn, const int *sidx, double *out, double *part,
+                  hipStream_t st);
+void launch_mpull(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg, const double *b,
+                  const int *perm, const int *gptr, const int *gidx, const double *Z, double *Yin, int K,
+                  hipStream_t st);
+void launch_mtile_red(int mode, const FrontDesc *fd, const TileUnit *units, int nunits, int K, const double *in0,
+                      const double *in1, const double *Dn, double *out, const double *part, hipStream_t st);
 void launch_mspmv_rhs(int n, int K, const int *ia, const int *ja, const double *val, const double *e2,
                       const double *e3, double *w, double *r, hipStream_t st);
 void launch_msource(int ns, int K, const int *edge, const int *src, const double *w, double didt, double cdt,
@@ -84,16 +90,6 @@ void launch_mg_child(const FrontDesc *fd, const int *parent, const int *list, co
                      const int *rel, double *V, int K, hipStream_t st);
 void launch_mg_xs(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg, const int *sidx,
                   const double *xw, double *V, int K, hipStream_t st);
-void launch_mfwd_struct_part(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
-                             int64_t nwg, const double *Y, int K, double *part, double *V, hipStream_t st);
-void launch_mfwd_struct_red(const FrontDesc *fd, const int *list, const int64_t *pref, const int64_t *ppref, int cnt,
-                            int64_t nwg, const double *part, double *V, int K, hipStream_t st);
-void launch_mbwd_t_part(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
-                        const double *V, int K, double *part, const double *Y, const double *Dn, double *T,
-                        hipStream_t st);
-void launch_mbwd_t_red(const FrontDesc *fd, const int *list, const int64_t *pref, const int64_t *ppref, int cnt,
-                       int64_t nwg, const double *part, const double *Y, const double *Dn, double *T, int K,
-                       hipStream_t st);
 // step.hip
 void launch_spmv_rhs(int n, const int *ia, const int *ja, const double *val, const double *e2, const double *e3,
                      double *r, hipStream_t st);
@@ -140,13 +136,25 @@ enum LaunchKind {
     L_EXTADD, L_FFUSED, L_FDIAG, L_FTRSM, L_FUPD, L_IW, L_IAPPLY, L_IDIAG,
     L_SSMALL, L_SOWN, L_SSTRUCT,
     L_BSMALL, L_BT, L_BX,
-    L_GINIT, L_GCHILD, L_GXS, L_MFOWN, L_MFSPART, L_MFSRED, L_MBTPART, L_MBTRED, L_MBX
+    L_GINIT, L_GCHILD, L_GXS, L_MT, L_MTRED, L_MPULL
 };
 struct Launch {
     int kind, p0, off, cnt, poff;
     int64_t nwg;
     int aux;  // reductions: offset of the paired partial launch's prefix sums
 };
+
+int prof_category(int kind) {
+    switch (kind) {
+    case L_MT: return TF_PROF_TILE;
+    case L_MTRED: return TF_PROF_TILE_RED;
+    case L_MPULL: return TF_PROF_PULL;
+    case L_FUPD: return TF_PROF_FACTOR_UPDATE;
+    case L_EXTADD: case L_FFUSED: case L_FDIAG: case L_FTRSM: case L_IW: case L_IAPPLY: case L_IDIAG:
+        return TF_PROF_FACTOR_OTHER;
+    default: return TF_PROF_SOLVE1;
+    }
+}
 
 // Execution plan over the front tree: which fronts each launch touches.
 struct Plan {
@@ -156,6 +164,7 @@ struct Plan {
     std::vector<int64_t> woff;  // per supernode: offset in the inversion scratch
     int64_t wsize = 0;
     int64_t part_max = 0;       // partial-sum tiles (4096 doubles each) of one launch
+    std::vector<TileUnit> units;  // block-solve tile units (L_MT / L_MTRED launches)
 };
 
 static const double PI = 3.141592653589793;
@@ -233,6 +242,11 @@ struct tf_ctx {
     int *d_perm = nullptr, *d_parent = nullptr, *d_chp = nullptr, *d_ch = nullptr, *d_rel = nullptr,
         *d_sidx = nullptr, *d_plist = nullptr;
     int64_t *d_amap = nullptr, *d_ppref = nullptr;
+    TileUnit *d_units = nullptr;
+    int *d_gptr = nullptr, *d_gidx = nullptr;
+    double *d_mZ = nullptr;
+    int mt_units = 4096, mt_chmin = 256;  // tile-launch sizing (units per launch, min inner chunk)
+    int mt_ks_n = 2, mt_ks_t = 4;         // k-steps per stage of the N- and T-form tile kernels
     double *d_U = nullptr, *d_V = nullptr, *d_xw = nullptr, *d_Y = nullptr, *d_T = nullptr, *d_W = nullptr;
     int64_t *d_woff = nullptr;
     double *d_L[2] = {nullptr, nullptr}, *d_D[2] = {nullptr, nullptr};
@@ -252,6 +266,13 @@ struct tf_ctx {
     std::vector<int> chk_step, chk_keep;
     std::vector<double> chk_err;
     double timers[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // optional per-category kernel timing with HIP events on the context's stream
+    bool prof_on = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<int> prof_rec;  // triples: category, start event, end event
+    double prof_ms[TF_PROF_NCAT] = {0};
+    int64_t prof_cnt[TF_PROF_NCAT] = {0};
     int leaf_size = 48;
 };
 
@@ -259,12 +280,12 @@ namespace {
 
 void free_solver(tf_ctx *c) {
     dfree(c->d_fd); dfree(c->d_perm); dfree(c->d_parent); dfree(c->d_chp); dfree(c->d_ch); dfree(c->d_rel);
-    dfree(c->d_sidx); dfree(c->d_plist); dfree(c->d_amap); dfree(c->d_ppref); dfree(c->d_U); dfree(c->d_V);
+    dfree(c->d_sidx); dfree(c->d_plist); dfree(c->d_amap); dfree(c->d_ppref); dfree(c->d_units); dfree(c->d_gptr); dfree(c->d_gidx); dfree(c->d_U); dfree(c->d_V);
     dfree(c->d_xw); dfree(c->d_Y); dfree(c->d_T); dfree(c->d_W); dfree(c->d_woff);
     for (int s = 0; s < 2; s++) { dfree(c->d_L[s]); dfree(c->d_D[s]); c->slot_live[s] = false; }
     dfree(c->d_tmpb); dfree(c->d_tmpx);
     dfree(c->d_mV); dfree(c->d_mY); dfree(c->d_mT); dfree(c->d_mxw); dfree(c->d_mb); dfree(c->d_mx);
-    dfree(c->d_part);
+    dfree(c->d_mZ); dfree(c->d_part);
     c->mK = 0;
     c->analyzed = false;
 }
@@ -289,6 +310,43 @@ void free_all(tf_ctx *c) {
 int fail(tf_ctx *c, int code, const std::string &msg) {
     if (c) c->err = msg;
     return code;
+}
+
+// Kernel timing: prof_mark records an event before a launch, prof_close one
+// after it and files the pair under a category.  Pairs are folded into the
+// per-category totals when the pool fills up and on tf_get_profile.
+void prof_fold(tf_ctx *c) {
+    if (c->prof_rec.empty()) return;
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (size_t i = 0; i + 3 <= c->prof_rec.size(); i += 3) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev_pool[c->prof_rec[i + 1]], c->ev_pool[c->prof_rec[i + 2]]));
+        c->prof_ms[c->prof_rec[i]] += ms;
+        c->prof_cnt[c->prof_rec[i]] += 1;
+    }
+    c->prof_rec.clear();
+    c->ev_used = 0;
+}
+int prof_event(tf_ctx *c) {
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        c->ev_pool.push_back(e);
+    }
+    HIPCHK(hipEventRecord(c->ev_pool[c->ev_used], c->st));
+    return (int)c->ev_used++;
+}
+inline int prof_mark(tf_ctx *c) {
+    if (!c->prof_on) return -1;
+    if (c->ev_used + 2 > 8192) prof_fold(c);
+    return prof_event(c);
+}
+inline void prof_close(tf_ctx *c, int cat, int mark) {
+    if (mark < 0) return;
+    const int e = prof_event(c);
+    c->prof_rec.push_back(cat);
+    c->prof_rec.push_back(mark);
+    c->prof_rec.push_back(e);
 }
 
 template <class F>
@@ -345,20 +403,6 @@ void build_plan(tf_ctx *c) {
                 }
             add(dst, L_GCHILD, 0, chl, uc);
         }
-    };
-    // partial-sum launch followed by its reduction; the reduction reads the
-    // partial launch's prefix sums (aux) to find each front's tiles
-    auto add_part_red = [&](std::vector<Launch> &dst, int kpart, int kred, const std::vector<int> &fr,
-                            const std::vector<int64_t> &upart, const std::vector<int64_t> &ured) {
-        size_t before = dst.size();
-        add(dst, kpart, 0, fr, upart);
-        int aux = 0;
-        if (dst.size() > before) {
-            aux = dst.back().poff;
-            p.part_max = std::max(p.part_max, dst.back().nwg);
-        }
-        add(dst, kred, 0, fr, ured);
-        if (!dst.empty() && dst.back().kind == kred) dst.back().aux = aux;
     };
     // small fronts (one workgroup each) vs large fronts (grid launches)
     std::vector<std::vector<int>> small(s.nlevels), grid(s.nlevels);
@@ -454,38 +498,84 @@ void build_plan(tf_ctx *c) {
         add(p.bwd, L_BSMALL, 0, small[l], {});
     }
     // block solve for K right-hand sides: every front through the tile kernels
+    // mode 0 own forward, 1 struct forward, 2 backward T, 3 backward X
+    struct Tile { int k, t0, lo, hi; };
+    auto tiles_of = [&](int mode, const std::vector<int> &fr) {
+        std::vector<Tile> t;
+        for (int k : fr) {
+            const int nc = s.nc[k], m = m_of(k);
+            if (mode == 0)
+                for (int r = 0; r < nc; r += 64) t.push_back({k, r, 0, std::min(nc, r + 64)});
+            else if (mode == 1)
+                for (int r = 0; r < m; r += 64) t.push_back({k, r, 0, nc});
+            else if (mode == 2)
+                for (int j = 0; j < nc; j += 64) t.push_back({k, j, 0, m});
+            else
+                for (int j = 0; j < nc; j += 64) t.push_back({k, j, j, nc});
+        }
+        return t;
+    };
+    // one tile launch (+ its reduction): tiles longer than the chunk are split
+    // so that a launch has about mt_units units
+    auto add_tiles = [&](std::vector<Launch> &dst, int mode, const std::vector<Tile> &tl) {
+        if (tl.empty()) return;
+        int64_t work = 0;
+        for (const Tile &t : tl) work += t.hi - t.lo;
+        int64_t ch = (work + c->mt_units - 1) / c->mt_units;
+        ch = std::max<int64_t>(c->mt_chmin, (ch + 15) / 16 * 16);
+        // units of one front are emitted chunk-major (all tiles of inner chunk
+        // 0, then chunk 1, ...): neighbouring waves then share the right-hand
+        // side rows of their inner chunk in L1/L2
+        std::vector<TileUnit> tu, ru;
+        int slots = 0;
+        size_t t0 = 0;
+        while (t0 < tl.size()) {
+            size_t t1 = t0;
+            while (t1 < tl.size() && tl[t1].k == tl[t0].k) t1++;
+            std::vector<int> nk(t1 - t0), base(t1 - t0);
+            int maxnk = 0;
+            for (size_t j = t0; j < t1; j++) {
+                const int len = tl[j].hi - tl[j].lo;
+                nk[j - t0] = len <= ch ? 1 : (int)((len + ch - 1) / ch);
+                maxnk = std::max(maxnk, nk[j - t0]);
+                if (nk[j - t0] > 1) {
+                    base[j - t0] = slots;
+                    ru.push_back({tl[j].k, tl[j].t0, slots, nk[j - t0], 0, 0});
+                    slots += nk[j - t0];
+                }
+            }
+            for (int q = 0; q < maxnk; q++)
+                for (size_t j = t0; j < t1; j++) {
+                    const Tile &t = tl[j];
+                    const int n = nk[j - t0], len = t.hi - t.lo;
+                    if (q >= n) continue;
+                    if (n == 1) { tu.push_back({t.k, t.t0, t.lo, t.hi, -1, 0}); continue; }
+                    const int a = t.lo + (int)((int64_t)len * q / n / 16 * 16);
+                    const int b = (q + 1 == n) ? t.hi : t.lo + (int)((int64_t)len * (q + 1) / n / 16 * 16);
+                    tu.push_back({t.k, t.t0, a, b, base[j - t0] + q, 0});
+                }
+            t0 = t1;
+        }
+        dst.push_back(Launch{L_MT, mode, (int)p.units.size(), (int)tu.size(), 0, (int64_t)(tu.size() + 3) / 4, 0});
+        p.units.insert(p.units.end(), tu.begin(), tu.end());
+        if (!ru.empty()) {
+            dst.push_back(Launch{L_MTRED, mode, (int)p.units.size(), (int)ru.size(), 0, (int64_t)ru.size(), 0});
+            p.units.insert(p.units.end(), ru.begin(), ru.end());
+        }
+        p.part_max = std::max<int64_t>(p.part_max, slots);
+    };
     for (int l = 0; l < s.nlevels; l++) {
         std::vector<int> all(s.lev_sn.begin() + s.lev_ptr[l], s.lev_sn.begin() + s.lev_ptr[l + 1]);
-        std::vector<int> withm;
-        std::vector<int64_t> uo, up, ur;
-        for (int k : all) {
-            uo.push_back((s.nc[k] + 63) / 64);
-            if (m_of(k) > 0) {
-                withm.push_back(k);
-                const int nch = (s.nc[k] + 511) / 512;
-                up.push_back((int64_t)((m_of(k) + 63) / 64) * nch);
-                ur.push_back(nch > 1 ? (m_of(k) + 63) / 64 : 0);
-            }
-        }
-        gathers(p.mfwd, all);
-        add(p.mfwd, L_MFOWN, 0, all, uo);
-        if (!withm.empty()) add_part_red(p.mfwd, L_MFSPART, L_MFSRED, withm, up, ur);
+        std::vector<int64_t> ur;
+        for (int k : all) ur.push_back((s.nc[k] + 3) / 4);
+        add(p.mfwd, L_MPULL, 0, all, ur);
+        add_tiles(p.mfwd, 0, tiles_of(0, all));
+        add_tiles(p.mfwd, 1, tiles_of(1, all));
     }
     for (int l = s.nlevels - 1; l >= 0; l--) {
         std::vector<int> all(s.lev_sn.begin() + s.lev_ptr[l], s.lev_sn.begin() + s.lev_ptr[l + 1]);
-        std::vector<int> withm;
-        std::vector<int64_t> ux, up, uc;
-        std::vector<int64_t> ur;
-        for (int k : all) {
-            const int nch = (m_of(k) + 511) / 512;
-            uc.push_back((s.nc[k] + 63) / 64);
-            up.push_back((int64_t)((s.nc[k] + 63) / 64) * nch);
-            ur.push_back(nch == 1 ? 0 : (s.nc[k] + 63) / 64);  // nch 0 (roots): T = Y/d in the reduction
-            if (m_of(k) > 0) { withm.push_back(k); ux.push_back((m_of(k) + 31) / 32); }
-        }
-        add(p.mbwd, L_GXS, 0, withm, ux);
-        add_part_red(p.mbwd, L_MBTPART, L_MBTRED, all, up, ur);
-        add(p.mbwd, L_MBX, 0, all, uc);
+        add_tiles(p.mbwd, 2, tiles_of(2, all));
+        add_tiles(p.mbwd, 3, tiles_of(3, all));
     }
 }
 
@@ -521,6 +611,16 @@ void do_analyze(tf_ctx *c, int leaf) {
     c->d_amap = dupload(s.amap);
     c->d_plist = dupload(c->plan.list);
     c->d_ppref = dupload(c->plan.pref);
+    c->d_units = dupload(c->plan.units);
+    {  // rows receiving each struct entry's update: inverse of sidx, in entry order
+        std::vector<int> gptr(s.n + 1, 0), gidx(s.sidx.size());
+        for (int r : s.sidx) gptr[r + 1]++;
+        for (int i = 0; i < s.n; i++) gptr[i + 1] += gptr[i];
+        std::vector<int> fill(gptr.begin(), gptr.end() - 1);
+        for (size_t pz = 0; pz < s.sidx.size(); pz++) gidx[fill[s.sidx[pz]]++] = (int)pz;
+        c->d_gptr = dupload(gptr);
+        c->d_gidx = dupload(gidx);
+    }
     c->d_U = dalloc<double>(s.utotal);
     c->d_V = dalloc<double>(voff);
     c->d_xw = dalloc<double>(s.n);
@@ -543,6 +643,7 @@ void run_factor(tf_ctx *c, int slot) {
     const int *lst = c->d_plist;
     const int64_t *pr = c->d_ppref;
     for (const Launch &l : p.factor) {
+        const int pm = prof_mark(c);
         switch (l.kind) {
         case L_EXTADD:
             tf::launch_extend_add(L, c->d_U, c->d_fd, c->d_parent, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_rel,
@@ -570,6 +671,7 @@ void run_factor(tf_ctx *c, int slot) {
             tf::launch_inv_diag(L, c->d_fd, lst + l.off, l.cnt, l.p0, c->st);
             break;
         }
+        prof_close(c, prof_category(l.kind), pm);
     }
     HIPCHK(hipGetLastError());
 }
@@ -603,6 +705,7 @@ void solve_dev(tf_ctx *c, int slot, const double *b, double *x) {
     const int *lst = c->d_plist;
     const int64_t *pr = c->d_ppref;
     for (const Launch &l : p.fwd) {
+        const int pm = prof_mark(c);
         switch (l.kind) {
         case L_SSMALL:
             tf::launch_fwd_small(L, c->d_fd, lst + l.off, l.cnt, b, c->d_perm, c->d_chp, c->d_ch, c->d_rel, c->d_V,
@@ -622,8 +725,10 @@ void solve_dev(tf_ctx *c, int slot, const double *b, double *x) {
             tf::launch_fwd_struct(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_V, c->d_Y, c->st);
             break;
         }
+        prof_close(c, prof_category(l.kind), pm);
     }
     for (const Launch &l : p.bwd) {
+        const int pm = prof_mark(c);
         switch (l.kind) {
         case L_BSMALL:
             tf::launch_bwd_small(L, c->d_fd, lst + l.off, l.cnt, c->d_Y, Dn, c->d_sidx, c->d_xw, c->st);
@@ -638,15 +743,17 @@ void solve_dev(tf_ctx *c, int slot, const double *b, double *x) {
             tf::launch_bwd_x(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_T, c->d_xw, c->st);
             break;
         }
+        prof_close(c, prof_category(l.kind), pm);
     }
     tf::launch_unpermute(c->sym.n, c->d_perm, c->d_xw, x, c->st);
 }
 
 void ensure_block(tf_ctx *c, int K) {
     if (c->mK >= K) return;
-    dfree(c->d_mV); dfree(c->d_mY); dfree(c->d_mT); dfree(c->d_mxw); dfree(c->d_mb); dfree(c->d_mx);
+    dfree(c->d_mV); dfree(c->d_mY); dfree(c->d_mT); dfree(c->d_mxw); dfree(c->d_mb); dfree(c->d_mx); dfree(c->d_mZ);
     const int64_t n = c->sym.n;
-    c->d_mV = dalloc<double>((size_t)c->vrows * K);
+    c->d_mV = dalloc<double>((size_t)n * K);  // Yin: own rows after the descendants' updates
+    c->d_mZ = dalloc<double>(std::max<size_t>(c->sym.sidx.size(), 1) * K);  // per-front L21 Y
     c->d_mY = dalloc<double>((size_t)n * K);
     c->d_mT = dalloc<double>((size_t)n * K);
     c->d_mxw = dalloc<double>((size_t)n * K);
@@ -663,44 +770,37 @@ void solve_block(tf_ctx *c, int slot, int K, const double *b, double *x) {
     const int *lst = c->d_plist;
     const int64_t *pr = c->d_ppref;
     for (const Launch &l : p.mfwd) {
+        const int pm = prof_mark(c);
         switch (l.kind) {
-        case L_GINIT:
-            tf::launch_mg_init(c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, b, c->d_perm, c->d_mV, K, c->st);
+        case L_MPULL:
+            tf::launch_mpull(c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, b, c->d_perm, c->d_gptr, c->d_gidx,
+                             c->d_mZ, c->d_mV, K, c->st);
             break;
-        case L_GCHILD:
-            tf::launch_mg_child(c->d_fd, c->d_parent, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_rel, c->d_mV, K,
-                                c->st);
+        case L_MT:  // own forward (Y from Yin), struct forward (Z from Y)
+            tf::launch_mtile(l.p0, l.p0 >= 2 ? c->mt_ks_t : c->mt_ks_n, L, c->d_fd, c->d_units + l.off, l.cnt, K, l.p0 == 0 ? c->d_mV : c->d_mY, nullptr,
+                             Dn, nullptr, l.p0 == 0 ? c->d_mY : c->d_mZ, c->d_part, c->st);
             break;
-        case L_MFOWN:
-            tf::launch_mfwd_own(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_mV, c->d_mY, K, c->st);
-            break;
-        case L_MFSPART:
-            tf::launch_mfwd_struct_part(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_mY, K, c->d_part,
-                                        c->d_mV, c->st);
-            break;
-        case L_MFSRED:
-            tf::launch_mfwd_struct_red(c->d_fd, lst + l.off, pr + l.poff, pr + l.aux, l.cnt, l.nwg, c->d_part,
-                                       c->d_mV, K, c->st);
+        case L_MTRED:
+            tf::launch_mtile_red(l.p0, c->d_fd, c->d_units + l.off, l.cnt, K, l.p0 == 0 ? c->d_mV : c->d_mY, nullptr,
+                                 Dn, l.p0 == 0 ? c->d_mY : c->d_mZ, c->d_part, c->st);
             break;
         }
+        prof_close(c, prof_category(l.kind), pm);
     }
     for (const Launch &l : p.mbwd) {
+        const int pm = prof_mark(c);
         switch (l.kind) {
-        case L_GXS:
-            tf::launch_mg_xs(c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_sidx, c->d_mxw, c->d_mV, K, c->st);
+        case L_MT:  // backward T (from x at the struct rows, Y, D), backward X (from T)
+            tf::launch_mtile(l.p0, l.p0 >= 2 ? c->mt_ks_t : c->mt_ks_n, L, c->d_fd, c->d_units + l.off, l.cnt, K, l.p0 == 2 ? c->d_mxw : c->d_mT,
+                             l.p0 == 2 ? c->d_mY : c->d_mT, Dn, c->d_sidx, l.p0 == 2 ? c->d_mT : c->d_mxw, c->d_part,
+                             c->st);
             break;
-        case L_MBTPART:
-            tf::launch_mbwd_t_part(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_mV, K, c->d_part, c->d_mY,
-                                   Dn, c->d_mT, c->st);
-            break;
-        case L_MBTRED:
-            tf::launch_mbwd_t_red(c->d_fd, lst + l.off, pr + l.poff, pr + l.aux, l.cnt, l.nwg, c->d_part, c->d_mY, Dn,
-                                  c->d_mT, K, c->st);
-            break;
-        case L_MBX:
-            tf::launch_mbwd_x(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_mT, c->d_mxw, K, c->st);
+        case L_MTRED:
+            tf::launch_mtile_red(l.p0, c->d_fd, c->d_units + l.off, l.cnt, K, l.p0 == 2 ? c->d_mxw : c->d_mT,
+                                 l.p0 == 2 ? c->d_mY : c->d_mT, Dn, l.p0 == 2 ? c->d_mT : c->d_mxw, c->d_part, c->st);
             break;
         }
+        prof_close(c, prof_category(l.kind), pm);
     }
     tf::launch_munpermute(c->sym.n, K, c->d_perm, c->d_mxw, x, c->st);
 }
@@ -731,6 +831,7 @@ void ensure_vecs(tf_ctx *c, StepVecs &v) {
 // r = Msigma (4 e2 - e3) - 2 dt s(t), K sources side by side
 void rhs(tf_ctx *c, int step, double t, const double *e2, const double *e3, double dt, bool negate, double *r) {
     const int K = c->nsrc;
+    const int pm = prof_mark(c);
     if (K == 1) tf::launch_spmv_rhs(c->topo.nE, c->d_fia, c->d_fja, c->d_fval, e2, e3, r, c->st);
     else tf::launch_mspmv_rhs(c->topo.nE, K, c->d_fia, c->d_fja, c->d_fval, e2, e3, c->d_mb, r, c->st);
     if (step < c->sp_division) {  // sys_right: source on while step < sp_division
@@ -739,6 +840,7 @@ void rhs(tf_ctx *c, int step, double t, const double *e2, const double *e3, doub
         if (K == 1) tf::launch_source(c->nseg, c->d_src_edge, c->d_src_w, didt, 2.0 * dt, r, c->st);
         else tf::launch_msource(c->nseg, K, c->d_src_edge, c->d_src_id, c->d_src_w, didt, 2.0 * dt, r, c->st);
     }
+    prof_close(c, TF_PROF_STEP, pm);
 }
 
 void dcopy(tf_ctx *c, double *dst, const double *src) {
@@ -911,16 +1013,26 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
 // ======================================================================
 extern "C" {
 
+// message of the last tf_create failure (no context exists to hold it)
+static thread_local std::string g_create_err = "no context";
+
 int tf_create(tf_ctx **out, int device) {
     if (!out) return TF_ERR_ARG;
     *out = nullptr;
     int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return TF_ERR_HIP;
-    if (device < 0 || device >= ndev) return TF_ERR_ARG;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        g_create_err = "no HIP device available (libtemfetd runs on the GPU only)";
+        return TF_ERR_HIP;
+    }
+    if (device < 0 || device >= ndev) {
+        g_create_err = "device index out of range";
+        return TF_ERR_ARG;
+    }
     tf_ctx *c = new tf_ctx();
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess) {
         delete c;
+        g_create_err = "could not create a HIP stream on the device";
         return TF_ERR_HIP;
     }
     *out = c;
@@ -932,11 +1044,12 @@ void tf_destroy(tf_ctx *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->st);
     free_all(c);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->st);
     delete c;
 }
 
-const char *tf_last_error(const tf_ctx *c) { return c ? c->err.c_str() : "null context"; }
+const char *tf_last_error(const tf_ctx *c) { return c ? c->err.c_str() : g_create_err.c_str(); }
 
 int tf_set_mesh(tf_ctx *c, int nN, const double *nod_cor, int nT, const int *tet_vex) {
     return guarded(c, [&] {
@@ -1187,6 +1300,57 @@ int tf_analyze(tf_ctx *c, int leaf_size) {
     });
 }
 
+int tf_set_tuning(tf_ctx *c, const char *key, int value) {
+    return guarded(c, [&] {
+        if (!key) return fail(c, TF_ERR_ARG, "null key");
+        const std::string k(key);
+        bool replan = false;
+        if (k == "mt_units") {
+            if (value < 64) return fail(c, TF_ERR_ARG, "mt_units must be >= 64");
+            c->mt_units = value;
+            replan = true;
+        } else if (k == "mt_chmin") {
+            if (value < 16 || value % 16) return fail(c, TF_ERR_ARG, "mt_chmin must be a positive multiple of 16");
+            c->mt_chmin = value;
+            replan = true;
+        } else if (k == "mt_ks_n" || k == "mt_ks_t") {
+            if (value != 2 && value != 4) return fail(c, TF_ERR_ARG, "k-steps per stage must be 2 or 4");
+            (k == "mt_ks_n" ? c->mt_ks_n : c->mt_ks_t) = value;
+        } else {
+            return fail(c, TF_ERR_ARG, "unknown tuning key " + k);
+        }
+        if (replan && c->analyzed) {  // the tile units and partial buffer follow the plan
+            HIPCHK(hipStreamSynchronize(c->st));
+            build_plan(c);
+            dfree(c->d_plist); dfree(c->d_ppref); dfree(c->d_units); dfree(c->d_part);
+            c->d_plist = dupload(c->plan.list);
+            c->d_ppref = dupload(c->plan.pref);
+            c->d_units = dupload(c->plan.units);
+            c->d_part = dalloc<double>((size_t)std::max<int64_t>(c->plan.part_max, 1) * 4096);
+        }
+        return TF_OK;
+    });
+}
+
+int tf_set_profiling(tf_ctx *c, int on) {
+    return guarded(c, [&] {
+        prof_fold(c);
+        for (int k = 0; k < TF_PROF_NCAT; k++) { c->prof_ms[k] = 0.0; c->prof_cnt[k] = 0; }
+        c->prof_on = on != 0;
+        return TF_OK;
+    });
+}
+
+int tf_get_profile(tf_ctx *c, int category, int64_t *launches, double *ms) {
+    return guarded(c, [&] {
+        if (category < 0 || category >= TF_PROF_NCAT) return fail(c, TF_ERR_ARG, "bad profile category");
+        prof_fold(c);
+        if (launches) *launches = c->prof_cnt[category];
+        if (ms) *ms = c->prof_ms[category];
+        return TF_OK;
+    });
+}
+
 int tf_factor_stats(const tf_ctx *c, int64_t *nnzL, double *flops, int *nsuper, int *nlevels) {
     if (!c || !c->analyzed) return TF_ERR_STATE;
     if (nnzL) *nnzL = c->sym.nnzL;
@@ -1258,6 +1422,37 @@ int tf_solve(tf_ctx *c, int slot, int nrhs, const double *b, double *x) {
         }
         HIPCHK(hipStreamSynchronize(c->st));
         HIPCHK(hipGetLastError());
+        return TF_OK;
+    });
+}
+
+int tf_time_solve(tf_ctx *c, int slot, int nrhs, int reps, double *ms_per_solve) {
+    return guarded(c, [&] {
+        if (slot < 0 || slot > 1 || !c->slot_live[slot]) return fail(c, TF_ERR_STATE, "slot not factored");
+        if (nrhs < 1 || nrhs > 64 || reps < 1 || !ms_per_solve) return fail(c, TF_ERR_ARG, "bad arguments");
+        const int n = c->topo.nE;
+        double *b, *x;
+        if (nrhs == 1) { b = c->d_tmpb; x = c->d_tmpx; }
+        else { ensure_block(c, nrhs); b = c->d_mb; x = c->d_mx; }
+        tf::launch_fill_pattern((int64_t)n * nrhs, b, c->st);
+        auto one = [&] {
+            if (nrhs == 1) solve_dev(c, slot, b, x);
+            else solve_block(c, slot, nrhs, b, x);
+        };
+        one();  // warm-up
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, c->st));
+        for (int r = 0; r < reps; r++) one();
+        HIPCHK(hipEventRecord(e1, c->st));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        HIPCHK(hipGetLastError());
+        *ms_per_solve = ms / reps;
         return TF_OK;
     });
 }

@@ -1,116 +1,13 @@
-// msolve.hip -- block solve for K right-hand sides (K <= 64 sources sharing
-// one LDL^T factor), FP64 on matrix cores.
+// msolve.hip -- K right-hand-side (K <= 64 sources sharing one LDL^T factor)
+// vector kernels of the block solve and time step: gathers between fronts,
+// permutation, BDF2 history, SpMV, source, readout and the doubling-check
+// norms.  The tile products themselves are in mtile.hip.
 //
 // The reference solves one source per run (pardiso_direct_2 with nrhs=1,
 // src/FETD_UTILITY.f90:938-957).  Here K sources are time-stepped together:
-// every vector is [row][K] (the K values of a row contiguous) and each
-// front's solve becomes four tile products of 64 rows x 64 right-hand sides
-// (v_mfma_f64_16x16x4f64 via mfma64_acc):
-//   forward   Y_own = X11 V_own          V_struct -= L21 Y_own
-//   backward  T = D^-1 Y_own - L21^T Xs  X_own = X11^T T
-// with X11 = L11^-1 stored in place by the factorisation.  The factor is
-// read once per step for all K sources.
-//
-// Result layout of mfma64_acc here: the right-hand-side index plays the
-// "row" role (fast index of a result register), so every store writes
-// 16 consecutive right-hand sides (128 bytes) of 4 matrix rows.
+// every vector is [row][K] (the K values of a row contiguous), so the factor
+// is read once per step for all K sources.
 #include "tf_device.h"
-
-#define TLD 65  // odd leading dimension for operands staged transposed
-
-// ---- gathers ---------------------------------------------------------------------
-
-// ---- tile products ----------------------------------------------------------------
-
-// stage 64 rows x 64 rhs of a [row][K] block starting at row r0 (rows < rend)
-__device__ __forceinline__ void stage_rows(double *dst, const double *src, int r0, int rend, int K) {
-#pragma unroll 4
-    for (int idx = threadIdx.x; idx < 64 * 64; idx += TF_NT) {
-        const int r = idx >> 6, q = idx & 63;
-        dst[r * MLD + q] = (r0 + r < rend && q < K) ? src[(int64_t)(r0 + r) * K + q] : 0.0;
-    }
-}
-
-// forward own rows R0..R0+63 (< nc): Y = V + X11 V (strict lower part)
-__global__ void __launch_bounds__(TF_NT) k_mfwd_own(const double *L, const FrontDesc *fd, const int *list,
-                                                    const int64_t *pref, int cnt, const double *V, double *Y,
-                                                    int K) {
-    __shared__ double Bt[64 * MLD];  // [j][rhs]
-    __shared__ double Xt[64 * MLD];  // [j][row]
-    const int i = seg_find(pref, cnt, blockIdx.x);
-    const FrontDesc d = fd[list[i]];
-    const int R0 = (int)(blockIdx.x - pref[i]) * 64;
-    const double *A = L + d.loff;
-    const double *vf = V + d.voff * K;
-    tf_d4 acc[2][2];
-    acc_zero(acc);
-    for (int j0 = 0; j0 <= R0; j0 += 64) {
-        stage_rows(Bt, vf, j0, d.nc, K);
-#pragma unroll 4
-        for (int idx = threadIdx.x; idx < 64 * 64; idx += TF_NT) {
-            const int jj = idx >> 6, r = idx & 63;
-            const int j = j0 + jj, row = R0 + r;
-            Xt[jj * MLD + r] = (j < row && row < d.nc) ? A[(int64_t)j * d.f + row] : 0.0;
-        }
-        __syncthreads();
-        mfma64_acc(Bt, MLD, Xt, MLD, 64, acc);
-        __syncthreads();
-    }
-#pragma unroll
-    for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int row = R0 + ACC_COL(bj, r);
-#pragma unroll
-            for (int bi = 0; bi < 2; bi++) {
-                const int q = ACC_ROW(bi);
-                if (row < d.nc && q < K)
-                    Y[(int64_t)(d.c0 + row) * K + q] = vf[(int64_t)row * K + q] + acc[bi][bj][r];
-            }
-        }
-}
-
-// backward columns J0..J0+63: X = T + X11^T T (strict lower part of X11)
-__global__ void __launch_bounds__(TF_NT) k_mbwd_x(const double *L, const FrontDesc *fd, const int *list,
-                                                  const int64_t *pref, int cnt, const double *T, double *xw,
-                                                  int K) {
-    __shared__ double Bt[64 * MLD];  // [i][rhs]
-    __shared__ double Xt[64 * TLD];  // [i][col]
-    const int i = seg_find(pref, cnt, blockIdx.x);
-    const FrontDesc d = fd[list[i]];
-    const int J0 = (int)(blockIdx.x - pref[i]) * 64;
-    const double *A = L + d.loff;
-    const double *t = T + (int64_t)d.c0 * K;
-    tf_d4 acc[2][2];
-    acc_zero(acc);
-    for (int i0 = J0; i0 < d.nc; i0 += 64) {
-        stage_rows(Bt, t, i0, d.nc, K);
-#pragma unroll 4
-        for (int idx = threadIdx.x; idx < 64 * 64; idx += TF_NT) {
-            const int c = idx >> 6, ii = idx & 63;
-            const int j = J0 + c, row = i0 + ii;
-            Xt[ii * TLD + c] = (row > j && row < d.nc) ? A[(int64_t)j * d.f + row] : 0.0;
-        }
-        __syncthreads();
-        mfma64_acc(Bt, MLD, Xt, TLD, 64, acc);
-        __syncthreads();
-    }
-#pragma unroll
-    for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int j = J0 + ACC_COL(bj, r);
-            if (j >= d.nc) continue;
-#pragma unroll
-            for (int bi = 0; bi < 2; bi++) {
-                const int q = ACC_ROW(bi);
-                if (q < K) {
-                    const int64_t o = (int64_t)(d.c0 + j) * K + q;
-                    xw[o] = t[(int64_t)j * K + q] + acc[bi][bj][r];
-                }
-            }
-        }
-}
 
 __global__ void k_munpermute(int n, int K, const int *perm, const double *xw, double *x) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -256,170 +153,47 @@ __global__ void __launch_bounds__(TF_NT) k_mg_xs(const FrontDesc *fd, const int 
     }
 }
 
-// ---- split tile products with partial sums ----------------------------------------
-#define TF_SCH 512  // inner-dimension chunk per workgroup
-
-// partial (row tile, column chunk) of L21 Y_own: part[u][rhs*64 + row]
-__global__ void __launch_bounds__(TF_NT) k_mfwd_struct_part(const double *L, const FrontDesc *fd, const int *list,
-                                                            const int64_t *pref, int cnt, const double *Y, int K,
-                                                            double *part, double *V) {
-    __shared__ double Bt[64 * MLD];
-    __shared__ double Lt[64 * MLD];
+// ---- forward pull -----------------------------------------------------------------
+// Own rows of the fronts of one level: Yin[row] = b[perm[row]] - sum of the Z
+// rows (descendant updates) that land on row, in the fixed order of gidx.
+// Four rows x 64 right-hand sides per workgroup.
+__global__ void __launch_bounds__(256) k_mpull(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
+                                               const double *b, const int *perm, const int *gptr, const int *gidx,
+                                               const double *Z, double *Yin, int K) {
     const int i = seg_find(pref, cnt, blockIdx.x);
     const FrontDesc d = fd[list[i]];
-    const int u = (int)(blockIdx.x - pref[i]);
-    const int nch = (d.nc + TF_SCH - 1) / TF_SCH;
-    const int R0 = (u / nch) * 64, jc = (u % nch) * TF_SCH;
-    const double *A = L + d.loff;
-    const double *y = Y + (int64_t)d.c0 * K;
-    tf_d4 acc[2][2];
-    acc_zero(acc);
-    const int jend = min(d.nc, jc + TF_SCH);
-    for (int j0 = jc; j0 < jend; j0 += 64) {
-        stage_rows(Bt, y, j0, jend, K);
-#pragma unroll 4
-        for (int idx = threadIdx.x; idx < 64 * 64; idx += TF_NT) {
-            const int jj = idx >> 6, r = idx & 63;
-            const int j = j0 + jj, row = R0 + r;
-            Lt[jj * MLD + r] = (j < jend && row < d.m) ? A[(int64_t)j * d.f + d.nc + row] : 0.0;
-        }
-        __syncthreads();
-        mfma64_acc(Bt, MLD, Lt, MLD, 64, acc);
-        __syncthreads();
+    const int r = (int)(blockIdx.x - pref[i]) * 4 + (threadIdx.x >> 6), q = threadIdx.x & 63;
+    if (r >= d.nc || q >= K) return;
+    const int row = d.c0 + r;
+    double s = b[(int64_t)perm[row] * K + q];
+    const int p1 = gptr[row + 1];
+    int p = gptr[row];
+    for (; p + 4 <= p1; p += 4) {  // four independent loads in flight, summed in order
+        const double z0 = Z[(int64_t)gidx[p] * K + q], z1 = Z[(int64_t)gidx[p + 1] * K + q];
+        const double z2 = Z[(int64_t)gidx[p + 2] * K + q], z3 = Z[(int64_t)gidx[p + 3] * K + q];
+        s = (((s - z0) - z1) - z2) - z3;
     }
-    if (nch == 1) {  // single chunk: finish here, no partial tile
-        double *vs = V + (d.voff + d.nc) * K;
-#pragma unroll
-        for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int row = R0 + ACC_COL(bj, r);
-#pragma unroll
-                for (int bi = 0; bi < 2; bi++) {
-                    const int q = ACC_ROW(bi);
-                    if (row < d.m && q < K) vs[(int64_t)row * K + q] -= acc[bi][bj][r];
-                }
-            }
-        return;
-    }
-    double *pp = part + (int64_t)blockIdx.x * 4096;
-#pragma unroll
-    for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-#pragma unroll
-            for (int bi = 0; bi < 2; bi++) pp[ACC_ROW(bi) * 64 + ACC_COL(bj, r)] = acc[bi][bj][r];
-}
-
-// V_struct[row] -= sum over the row tile's chunks
-__global__ void __launch_bounds__(TF_NT) k_mfwd_struct_red(const FrontDesc *fd, const int *list, const int64_t *pref,
-                                                           const int64_t *ppref, int cnt, const double *part, double *V,
-                                                           int K) {
-    const int i = seg_find(pref, cnt, blockIdx.x);
-    const FrontDesc d = fd[list[i]];
-    const int rt = (int)(blockIdx.x - pref[i]);
-    const int nch = (d.nc + TF_SCH - 1) / TF_SCH;
-    if (nch == 1) return;
-    const double *p0 = part + (ppref[i] + (int64_t)rt * nch) * 4096;
-    double *vs = V + (d.voff + d.nc) * K;
-    for (int e = threadIdx.x; e < 64 * 64; e += TF_NT) {
-        const int q = e >> 6, rr = e & 63;
-        const int row = rt * 64 + rr;
-        if (row >= d.m || q >= K) continue;
-        double s = 0.0;
-        for (int c = 0; c < nch; c++) s += p0[(int64_t)c * 4096 + e];
-        vs[(int64_t)row * K + q] -= s;
-    }
-}
-
-// partial (column tile, row chunk) of L21^T Xs: part[u][rhs*64 + col]
-__global__ void __launch_bounds__(TF_NT) k_mbwd_t_part(const double *L, const FrontDesc *fd, const int *list,
-                                                       const int64_t *pref, int cnt, const double *V, int K,
-                                                       double *part, const double *Y, const double *Dn, double *T) {
-    __shared__ double Bt[64 * MLD];
-    __shared__ double Lt[64 * TLD];
-    const int i = seg_find(pref, cnt, blockIdx.x);
-    const FrontDesc d = fd[list[i]];
-    const int u = (int)(blockIdx.x - pref[i]);
-    const int nch = (d.m + TF_SCH - 1) / TF_SCH;
-    const int J0 = (u / nch) * 64, rc = (u % nch) * TF_SCH;
-    const double *A = L + d.loff;
-    const double *xs = V + (d.voff + d.nc) * K;
-    tf_d4 acc[2][2];
-    acc_zero(acc);
-    const int rend = min(d.m, rc + TF_SCH);
-    for (int r0 = rc; r0 < rend; r0 += 64) {
-        stage_rows(Bt, xs, r0, rend, K);
-#pragma unroll 4
-        for (int idx = threadIdx.x; idx < 64 * 64; idx += TF_NT) {
-            const int c = idx >> 6, rr = idx & 63;
-            const int j = J0 + c, row = r0 + rr;
-            Lt[rr * TLD + c] = (j < d.nc && row < rend) ? A[(int64_t)j * d.f + d.nc + row] : 0.0;
-        }
-        __syncthreads();
-        mfma64_acc(Bt, MLD, Lt, TLD, 64, acc);
-        __syncthreads();
-    }
-    if (nch == 1) {  // single chunk: finish here, no partial tile
-#pragma unroll
-        for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int j = J0 + ACC_COL(bj, r);
-                if (j >= d.nc) continue;
-                const double dj = Dn[d.c0 + j];
-#pragma unroll
-                for (int bi = 0; bi < 2; bi++) {
-                    const int q = ACC_ROW(bi);
-                    if (q < K) {
-                        const int64_t o = (int64_t)(d.c0 + j) * K + q;
-                        T[o] = Y[o] / dj - acc[bi][bj][r];
-                    }
-                }
-            }
-        return;
-    }
-    double *pp = part + (int64_t)blockIdx.x * 4096;
-#pragma unroll
-    for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-#pragma unroll
-            for (int bi = 0; bi < 2; bi++) pp[ACC_ROW(bi) * 64 + ACC_COL(bj, r)] = acc[bi][bj][r];
-}
-
-// T[col] = Y/d - sum over the column tile's chunks
-__global__ void __launch_bounds__(TF_NT) k_mbwd_t_red(const FrontDesc *fd, const int *list, const int64_t *pref,
-                                                      const int64_t *ppref, int cnt, const double *part, const double *Y,
-                                                      const double *Dn, double *T, int K) {
-    const int i = seg_find(pref, cnt, blockIdx.x);
-    const FrontDesc d = fd[list[i]];
-    const int jt = (int)(blockIdx.x - pref[i]);
-    const int nch = (d.m + TF_SCH - 1) / TF_SCH;
-    if (nch == 1) return;
-    const double *p0 = part + (ppref[i] + (int64_t)jt * nch) * 4096;
-    for (int e = threadIdx.x; e < 64 * 64; e += TF_NT) {
-        const int q = e >> 6, cc = e & 63;
-        const int j = jt * 64 + cc;
-        if (j >= d.nc || q >= K) continue;
-        double s = 0.0;
-        for (int c = 0; c < nch; c++) s += p0[(int64_t)c * 4096 + e];
-        const int64_t o = (int64_t)(d.c0 + j) * K + q;
-        T[o] = Y[o] / Dn[d.c0 + j] - s;
-    }
+    for (; p < p1; p++) s -= Z[(int64_t)gidx[p] * K + q];
+    Yin[(int64_t)row * K + q] = s;
 }
 
 // ---- host launchers --------------------------------------------------------------
-namespace tf {
-void launch_mfwd_own(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
-                     int64_t nwg, const double *V, double *Y, int K, hipStream_t st) {
-    if (nwg > 0)
-        hipLaunchKernelGGL(k_mfwd_own, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt, V, Y, K);
+// deterministic right-hand sides for solve timing
+__global__ void k_fill_pattern(int64_t len, double *b) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < len) b[i] = 1.0 + 0.125 * (double)(i % 7) - 0.0625 * (double)(i % 5);
 }
-void launch_mbwd_x(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
-                   const double *T, double *xw, int K, hipStream_t st) {
+
+namespace tf {
+void launch_fill_pattern(int64_t len, double *b, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_pattern, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, len, b);
+}
+void launch_mpull(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg, const double *b,
+                  const int *perm, const int *gptr, const int *gidx, const double *Z, double *Yin, int K,
+                  hipStream_t st) {
     if (nwg > 0)
-        hipLaunchKernelGGL(k_mbwd_x, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt, T, xw, K);
+        hipLaunchKernelGGL(k_mpull, dim3((unsigned)nwg), dim3(256), 0, st, fd, list, pref, cnt, b, perm, gptr, gidx, Z, Yin,
+                           K);
 }
 void launch_munpermute(int n, int K, const int *perm, const double *xw, double *x, hipStream_t st) {
     const int64_t tot = (int64_t)n * K;
@@ -460,31 +234,5 @@ void launch_mg_child(const FrontDesc *fd, const int *parent, const int *list, co
 void launch_mg_xs(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg, const int *sidx,
                   const double *xw, double *V, int K, hipStream_t st) {
     if (nwg > 0) hipLaunchKernelGGL(k_mg_xs, dim3((unsigned)nwg), dim3(TF_NT), 0, st, fd, list, pref, cnt, sidx, xw, V, K);
-}
-void launch_mfwd_struct_part(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
-                             int64_t nwg, const double *Y, int K, double *part, double *V, hipStream_t st) {
-    if (nwg > 0)
-        hipLaunchKernelGGL(k_mfwd_struct_part, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt, Y, K, part,
-                           V);
-}
-void launch_mfwd_struct_red(const FrontDesc *fd, const int *list, const int64_t *pref, const int64_t *ppref, int cnt,
-                            int64_t nwg, const double *part, double *V, int K, hipStream_t st) {
-    if (nwg > 0)
-        hipLaunchKernelGGL(k_mfwd_struct_red, dim3((unsigned)nwg), dim3(TF_NT), 0, st, fd, list, pref, ppref, cnt, part, V,
-                           K);
-}
-void launch_mbwd_t_part(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
-                        const double *V, int K, double *part, const double *Y, const double *Dn, double *T,
-                        hipStream_t st) {
-    if (nwg > 0)
-        hipLaunchKernelGGL(k_mbwd_t_part, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt, V, K, part, Y,
-                           Dn, T);
-}
-void launch_mbwd_t_red(const FrontDesc *fd, const int *list, const int64_t *pref, const int64_t *ppref, int cnt,
-                       int64_t nwg, const double *part, const double *Y, const double *Dn, double *T, int K,
-                       hipStream_t st) {
-    if (nwg > 0)
-        hipLaunchKernelGGL(k_mbwd_t_red, dim3((unsigned)nwg), dim3(TF_NT), 0, st, fd, list, pref, ppref, cnt, part, Y, Dn,
-                           T, K);
 }
 }  // namespace tf

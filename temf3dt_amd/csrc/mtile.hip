@@ -1,0 +1,274 @@
+// mtile.hip -- wave-level FP64 matrix-core tile products of the K-RHS block
+// solve (K <= 64 sources sharing one LDL^T factor).
+//
+// Each wave owns one work unit: a 64 x 64 output tile (64 matrix rows or
+// columns x 64 right-hand sides) accumulated over an inner range [k0, k1).
+// Operands go straight from memory into the MFMA lane layout -- no LDS, no
+// barriers -- so many waves per CU stream the factor at once.  The four
+// products of a front (X11 = L11^-1 stored in place by the factorisation):
+//   forward   Y_own = Yin + strict(X11) Yin           (N form, triangle)
+//             Z = L21 Y_own                          (N form)
+//   backward  T = Y_own / d - L21^T x(struct rows)   (T form, gathered B)
+//             X_own = T + strict(X11)^T T            (T form, triangle)
+// Yin (own rows of the right-hand side after the descendants' updates) is
+// formed by k_mpull from b and the Z rows of the descendants: the forward
+// update is written once per front and pulled once, not accumulated front
+// to front.  N form: output index runs down a column of L (16 consecutive
+// rows per lane group, 128-byte loads).  T form: output index runs across
+// columns; a lane reads two consecutive rows of one column per stage.
+//
+// v_mfma_f64_16x16x4f64 lane layout (measured): lane l supplies A[l%16][l/16]
+// and B[l/16][l%16] and receives D[4r + l/16][l%16] in result register r.
+// A carries the matrix (output index = l%16), B the right-hand sides, so
+// result register r of block (a, b) is output 16b + 4r + l/16, rhs 16a + l%16.
+//
+// Units whose tile needs several inner chunks write a 64 x 64 partial tile
+// (part[slot][out][rhs]); a reduction launch sums the chunks in a fixed
+// order, so the result does not depend on scheduling.
+#include "tf_device.h"
+
+enum { MT_OWN_FWD = 0, MT_STR_FWD = 1, MT_BWD_T = 2, MT_BWD_X = 3 };
+
+__device__ __forceinline__ void acc16_zero(tf_d4 acc[4][4]) {
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = (tf_d4){0.0, 0.0, 0.0, 0.0};
+}
+
+// Output-row/column count, matrix offset inside the L block, and whether the
+// (strict) triangle applies, per mode.
+template <int MODE>
+__device__ __forceinline__ void mt_geom(const FrontDesc &d, int &nout, int &roff) {
+    if (MODE == MT_OWN_FWD || MODE == MT_BWD_X) { nout = d.nc; roff = 0; }
+    else if (MODE == MT_STR_FWD) { nout = d.m; roff = d.nc; }
+    else { nout = d.nc; roff = d.nc; }
+}
+
+// Epilogue for one output element (o = output row/column of the front,
+// q = rhs); `s` is the accumulated product.
+template <int MODE>
+__device__ __forceinline__ void mt_store(const FrontDesc &d, int o, int q, double s, int K, const double *in0,
+                                         const double *in1, const double *Dn, double *out) {
+    if (MODE == MT_OWN_FWD) {  // Y[c0+o] = Yin[c0+o] + s
+        const int64_t oo = (int64_t)(d.c0 + o) * K + q;
+        out[oo] = in0[oo] + s;
+    } else if (MODE == MT_STR_FWD) {  // Z[soff+o] = s (subtracted where it is pulled)
+        out[(d.soff + o) * K + q] = s;
+    } else if (MODE == MT_BWD_T) {  // T = Y/d - s
+        const int64_t oo = (int64_t)(d.c0 + o) * K + q;
+        out[oo] = in1[oo] / Dn[d.c0 + o] - s;
+    } else {  // X = T + s
+        const int64_t oo = (int64_t)(d.c0 + o) * K + q;
+        out[oo] = in1[oo] + s;
+    }
+}
+
+// One wave: acc(out, rhs) = sum_{k0 <= i < k1} A(out, i) B(i, rhs) where
+//   N form: A(out, i) = Lf[i*f + roff + out]   (column i, row out)
+//   T form: A(out, i) = Lf[out*f + roff + i]   (column out, row i)
+// B(i, rhs) = Bm[row(i)*K + rhs] with row(i) = i, or bidx[i] when GATH.
+// TRI keeps only i < out (N) / i > out (T).  The inner range is walked in
+// stages of KS k-steps (4*KS inner values).  KS = 2: two register stages,
+// the next one's operands in flight while the current one is multiplied;
+// KS = 4: one stage at a time (a T-form lane then reads 32 contiguous bytes
+// of a column per stage, whole 128-byte lines per lane group).
+template <bool TFORM, bool TRI, bool GATH, int KS>
+__device__ __forceinline__ void mt_wave(const double *Lf, int64_t f, int roff, int nout, int o0, int k0, int k1,
+                                        const double *Bm, const int *bidx, int K, tf_d4 acc[4][4]) {
+    constexpr int SW = 4 * KS;  // inner values per stage
+    const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
+    int ob[4];
+    bool ov[4], qv[4];
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        ob[b] = o0 + 16 * b + l16;
+        ov[b] = ob[b] < nout;
+        qv[b] = 16 * b + l16 < K;
+    }
+    // inner index of k-step s of the stage starting at ib
+    auto inner = [&](int ib, int s) { return TFORM ? (ib + KS * lk + s) : (ib + 4 * s + lk); };
+    // gathered B rows of a stage: fetched one stage ahead of the operands
+    auto ldidx = [&](int ib, int r[KS]) {
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const int i = inner(ib, s);
+            r[s] = (GATH && i < k1) ? bidx[i] : 0;
+        }
+    };
+    auto load = [&](int ib, const int r[KS], double xa[KS][4], double yb[KS][4]) {
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const int i = inner(ib, s);
+            const bool iv = i < k1;
+            const int64_t brow = GATH ? (int64_t)r[s] : (int64_t)i;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                bool ok = iv && ov[b];
+                if (TRI) ok = ok && (TFORM ? (i > ob[b]) : (i < ob[b]));
+                const int64_t addr = TFORM ? ((int64_t)ob[b] * f + roff + i) : ((int64_t)i * f + roff + ob[b]);
+                xa[s][b] = ok ? Lf[addr] : 0.0;
+            }
+#pragma unroll
+            for (int a = 0; a < 4; a++) yb[s][a] = (iv && qv[a]) ? Bm[brow * K + 16 * a + l16] : 0.0;
+        }
+    };
+    auto mult = [&](const double xa[KS][4], const double yb[KS][4]) {
+#pragma unroll
+        for (int s = 0; s < KS; s++)
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s][b], yb[s][a], acc[a][b], 0, 0, 0);
+    };
+    if (k0 >= k1) return;
+    if (KS == 4) {
+        double xa[KS][4], yb[KS][4];
+        int rA[KS], rN[KS];
+        ldidx(k0, rA);
+        for (int ib = k0; ib < k1; ib += SW) {
+            if (GATH) ldidx(ib + SW, rN);
+            load(ib, rA, xa, yb);
+            mult(xa, yb);
+            if (GATH)
+#pragma unroll
+                for (int s = 0; s < KS; s++) rA[s] = rN[s];
+        }
+        return;
+    }
+    double xa0[KS][4], yb0[KS][4], xa1[KS][4], yb1[KS][4];
+    int rA[KS], rB[KS];
+    ldidx(k0, rA);
+    ldidx(k0 + SW, rB);
+    load(k0, rA, xa0, yb0);
+    int ib = k0;
+    for (;;) {
+        const bool more = ib + SW < k1;
+        if (more) {
+            if (GATH) ldidx(ib + 2 * SW, rA);
+            load(ib + SW, rB, xa1, yb1);
+        }
+        mult(xa0, yb0);
+        if (!more) break;
+        ib += SW;
+        const bool more2 = ib + SW < k1;
+        if (more2) {
+            if (GATH) ldidx(ib + 2 * SW, rB);
+            load(ib + SW, rA, xa0, yb0);
+        }
+        mult(xa1, yb1);
+        if (!more2) break;
+        ib += SW;
+    }
+}
+
+template <int MODE, int KS>
+__global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDesc *fd, const TileUnit *units,
+                                                  int nunits, int K, const double *in0, const double *in1,
+                                                  const double *Dn, const int *sidx, double *out, double *part) {
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int u = xcd_block(blockIdx.x, gridDim.x) * 4 + w;
+    if (u >= nunits) return;
+    const TileUnit tu = units[u];
+    const FrontDesc d = fd[tu.front];
+    int nout, roff;
+    mt_geom<MODE>(d, nout, roff);
+    tf_d4 acc[4][4];
+    acc16_zero(acc);
+    const bool tri = (MODE == MT_OWN_FWD) ? (tu.k1 > tu.t0) : (MODE == MT_BWD_X) ? (tu.k0 < tu.t0 + 64) : false;
+    const double *Lf = L + d.loff;
+    // B operand rows: own right-hand sides (Yin, Y_own, T) at c0, or the
+    // solution rows of the struct indices (backward T)
+    const double *Bm = in0 + (int64_t)d.c0 * K;
+    if (MODE == MT_OWN_FWD || MODE == MT_STR_FWD) {
+        if (tri) mt_wave<false, true, false, KS>(Lf, d.f, roff, nout, tu.t0, tu.k0, tu.k1, Bm, nullptr, K, acc);
+        else mt_wave<false, false, false, KS>(Lf, d.f, roff, nout, tu.t0, tu.k0, tu.k1, Bm, nullptr, K, acc);
+    } else if (MODE == MT_BWD_T) {
+        mt_wave<true, false, true, KS>(Lf, d.f, roff, nout, tu.t0, tu.k0, tu.k1, in0, sidx + d.soff, K, acc);
+    } else {
+        if (tri) mt_wave<true, true, false, KS>(Lf, d.f, roff, nout, tu.t0, tu.k0, tu.k1, Bm, nullptr, K, acc);
+        else mt_wave<true, false, false, KS>(Lf, d.f, roff, nout, tu.t0, tu.k0, tu.k1, Bm, nullptr, K, acc);
+    }
+    const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
+    if (tu.slot < 0) {
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int o = tu.t0 + 16 * b + 4 * r + lk;
+                if (o >= nout) continue;
+#pragma unroll
+                for (int a = 0; a < 4; a++) {
+                    const int q = 16 * a + l16;
+                    if (q < K) mt_store<MODE>(d, o, q, acc[a][b][r], K, in0, in1, Dn, out);
+                }
+            }
+    } else {
+        double *pp = part + (int64_t)tu.slot * 4096;
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int a = 0; a < 4; a++) pp[(16 * b + 4 * r + lk) * 64 + 16 * a + l16] = acc[a][b][r];
+    }
+}
+
+// Sum of a tile's chunk partials (slot0 .. slot0+nk-1, in order) + epilogue.
+// Reduction units reuse TileUnit: front, t0, k0 = slot0, k1 = nk.  Sixteen
+// workgroups per tile, one element per thread.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_mtile_red(const FrontDesc *fd, const TileUnit *units, int K,
+                                                   const double *in0, const double *in1, const double *Dn,
+                                                   double *out, const double *part) {
+    const TileUnit tu = units[blockIdx.x >> 4];
+    const FrontDesc d = fd[tu.front];
+    int nout, roff;
+    mt_geom<MODE>(d, nout, roff);
+    const int e = (blockIdx.x & 15) * 256 + threadIdx.x;
+    const int o = tu.t0 + (e >> 6), q = e & 63;
+    if (o >= nout || q >= K) return;
+    const double *p0 = part + (int64_t)tu.k0 * 4096 + e;
+    double s = 0.0;
+    for (int c = 0; c < tu.k1; c++) s += p0[(int64_t)c * 4096];
+    mt_store<MODE>(d, o, q, s, K, in0, in1, Dn, out);
+}
+
+namespace tf {
+// mode: 0 own forward, 1 struct forward, 2 backward T, 3 backward X
+void launch_mtile(int mode, int ks, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits, int K,
+                  const double *in0, const double *in1, const double *Dn, const int *sidx, double *out, double *part,
+                  hipStream_t st) {
+    if (nunits <= 0) return;
+    const dim3 g((nunits + 3) / 4), b(256);
+#define TF_MT(M, S) hipLaunchKernelGGL((k_mtile<M, S>), g, b, 0, st, L, fd, units, nunits, K, in0, in1, Dn, sidx, out, part)
+    if (ks == 4) {
+        switch (mode) {
+        case MT_OWN_FWD: TF_MT(MT_OWN_FWD, 4); break;
+        case MT_STR_FWD: TF_MT(MT_STR_FWD, 4); break;
+        case MT_BWD_T: TF_MT(MT_BWD_T, 4); break;
+        default: TF_MT(MT_BWD_X, 4); break;
+        }
+    } else {
+        switch (mode) {
+        case MT_OWN_FWD: TF_MT(MT_OWN_FWD, 2); break;
+        case MT_STR_FWD: TF_MT(MT_STR_FWD, 2); break;
+        case MT_BWD_T: TF_MT(MT_BWD_T, 2); break;
+        default: TF_MT(MT_BWD_X, 2); break;
+        }
+    }
+#undef TF_MT
+}
+void launch_mtile_red(int mode, const FrontDesc *fd, const TileUnit *units, int nunits, int K, const double *in0,
+                      const double *in1, const double *Dn, double *out, const double *part, hipStream_t st) {
+    if (nunits <= 0) return;
+    const dim3 g(nunits * 16), b(256);
+    switch (mode) {
+    case MT_OWN_FWD: hipLaunchKernelGGL(k_mtile_red<MT_OWN_FWD>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part); break;
+    case MT_STR_FWD: hipLaunchKernelGGL(k_mtile_red<MT_STR_FWD>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part); break;
+    case MT_BWD_T: hipLaunchKernelGGL(k_mtile_red<MT_BWD_T>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part); break;
+    default: hipLaunchKernelGGL(k_mtile_red<MT_BWD_X>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part); break;
+    }
+}
+}  // namespace tf

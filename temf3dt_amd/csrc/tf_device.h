@@ -21,6 +21,14 @@ struct FrontDesc {
     int f, nc, m, c0;
 };
 
+// Work unit of the block-solve tile kernels: a 64-wide output tile starting
+// at t0 of front `front`, accumulated over the inner range [k0, k1); slot >= 0
+// names its partial tile when the tile is split over several units.  A
+// reduction unit reuses the fields as (front, t0, first slot, #slots).
+struct TileUnit {
+    int front, t0, k0, k1, slot, pad;
+};
+
 // Column c of a front addressed uniformly: columns < nc live in the L block,
 // the rest in the update block.  Element (r, c), r >= c, is colptr(c)[r].
 __device__ __forceinline__ double *front_col(double *L, double *U, const FrontDesc &d, int c) {
@@ -38,6 +46,15 @@ __device__ __forceinline__ int seg_find(const int64_t *pref, int cnt, int64_t w)
         else hi = mid - 1;
     }
     return lo;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2): workgroup i runs on XCD i % 8.  This maps the hardware id to a
+// logical block so that every XCD processes one contiguous run of logical
+// blocks, keeping neighbouring work (which shares operands) in one L2.
+__device__ __forceinline__ int xcd_block(int i, int n) {
+    const int q = n >> 3, r = n & 7, x = i & 7;
+    return x * q + min(x, r) + (i >> 3);
 }
 
 __device__ __forceinline__ void tri_index(int64_t t, int &ti, int &tj) {

@@ -47,7 +47,14 @@ _SIGS = {
     "tf_get_timers": [_P, _P],
     "tf_set_num_sources": [_P, _I],
     "tf_set_replay": [_P, _I, _P],
+    "tf_set_tuning": [_P, ctypes.c_char_p, _I],
+    "tf_time_solve": [_P, _I, _I, _I, _P],
+    "tf_set_profiling": [_P, _I],
+    "tf_get_profile": [_P, _I, _P, _P],
 }
+
+# kernel-timing categories of tf_get_profile (include/temfetd.h)
+PROF_CATEGORIES = ["tile", "tile_red", "pull", "factor_update", "factor_other", "solve1", "step"]
 
 
 def load_library(path=LIB_PATH):
@@ -84,7 +91,8 @@ class Fetd:
         self.ctx = _P()
         rc = self.lib.tf_create(ctypes.byref(self.ctx), device)
         if rc != 0:
-            raise FetdError(f"tf_create failed ({rc}): no usable GPU device {device}")
+            msg = self.lib.tf_last_error(None).decode(errors="replace")
+            raise FetdError(f"tf_create failed ({rc}) for GPU device {device}: {msg}")
         self.nR = 0
         self.nsrc = 1
 
@@ -177,6 +185,28 @@ class Fetd:
     # -- solver seam ----------------------------------------------------------
     def analyze(self, leaf_size=0):
         self._chk(self.lib.tf_analyze(self.ctx, leaf_size), "tf_analyze")
+
+    def set_tuning(self, key, value):
+        """Block-solve tile-kernel knob (see tf_set_tuning)."""
+        self._chk(self.lib.tf_set_tuning(self.ctx, key.encode(), int(value)), f"tf_set_tuning({key})")
+
+    def time_solve(self, slot, nrhs, reps=3):
+        """Average ms of a device-resident solve with nrhs right-hand sides."""
+        ms = _D()
+        self._chk(self.lib.tf_time_solve(self.ctx, slot, nrhs, reps, ctypes.byref(ms)), "tf_time_solve")
+        return ms.value
+
+    def set_profiling(self, on=True):
+        self._chk(self.lib.tf_set_profiling(self.ctx, 1 if on else 0), "tf_set_profiling")
+
+    def profile(self):
+        """{category: (launches, ms)} of the kernels timed since set_profiling."""
+        out = {}
+        for k, name in enumerate(PROF_CATEGORIES):
+            n, ms = ctypes.c_int64(), _D()
+            self._chk(self.lib.tf_get_profile(self.ctx, k, ctypes.byref(n), ctypes.byref(ms)), "tf_get_profile")
+            out[name] = (n.value, ms.value)
+        return out
 
     def factor_stats(self):
         nnzL = ctypes.c_int64()

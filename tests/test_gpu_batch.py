@@ -89,3 +89,60 @@ def test_batched_run_matches_oracle_per_source(tmp_path):
         rel = np.max(np.abs(a[sel] - b[sel]) / np.abs(b[sel]))
         print(f"[gpu] batched source {q} vs oracle: dBz/dt max rel diff {rel:.3e}", flush=True)
         assert rel < 1e-6
+
+
+@pytest.mark.parametrize("knobs", [
+    {"mt_units": 64, "mt_chmin": 16},                        # every long tile split into partials
+    {"mt_units": 64, "mt_chmin": 16, "mt_ks_n": 4, "mt_ks_t": 2},
+    {"mt_units": 100000, "mt_chmin": 4096, "mt_ks_n": 2, "mt_ks_t": 4},  # no splits
+])
+def test_block_solve_tuning_knobs(gpu_mod_big_batch, knobs):
+    """The tile-kernel knobs change only the schedule and the summation
+    order: the block solve agrees with the default setting and keeps a
+    backward error at FP64 level, including the partial-sum reductions that
+    small meshes only reach with small chunks."""
+    fe, oc = gpu_mod_big_batch
+    rng = np.random.default_rng(7)
+    B = rng.standard_normal((24, fe.nE))
+    X0 = fe.solve(0, B)
+    print(f"[gpu] tuning knobs {knobs}", flush=True)
+    for k, v in knobs.items():
+        fe.set_tuning(k, v)
+    try:
+        X = fe.solve(0, B)
+        ms = fe.time_solve(0, 24, 2)
+    finally:
+        for k, v in {"mt_units": 4096, "mt_chmin": 256, "mt_ks_n": 2, "mt_ks_t": 4}.items():
+            fe.set_tuning(k, v)
+    assert ms > 0
+    rel = np.linalg.norm(X - X0) / np.linalg.norm(X0)
+    print(f"[gpu] knobs vs default: {rel:.3e}", flush=True)
+    assert rel < 1e-10
+    va = 2.0 * 1e-8 * oc["curl"] + 3.0 * oc["mass"]
+    R = _upper_matvec(oc["ia"] - 1, oc["ja"] - 1, va, X.T) - B.T
+    berr = np.abs(R).max() / (np.abs(va).max() * np.abs(X).max() + np.abs(B).max())
+    assert berr < 1e-12
+
+
+def test_tuning_rejects_bad_values(gpu_mod_big_batch):
+    from temf3dt_amd.fetd import FetdError
+    fe, _ = gpu_mod_big_batch
+    for k, v in [("mt_ks_n", 3), ("mt_chmin", 10), ("mt_units", 1), ("no_such_knob", 1)]:
+        with pytest.raises(FetdError):
+            fe.set_tuning(k, v)
+
+
+def test_kernel_profile_counts(gpu_mod_big_batch):
+    """HIP-event kernel timing: a block solve's tile launches are counted
+    and timed, a single-RHS solve lands in its own category."""
+    fe, _ = gpu_mod_big_batch
+    fe.set_profiling(True)
+    fe.time_solve(0, 8, 2)
+    fe.time_solve(0, 1, 1)
+    prof = fe.profile()
+    fe.set_profiling(False)
+    print(f"[gpu] profile {prof}", flush=True)
+    assert prof["tile"][0] > 0 and prof["tile"][1] > 0
+    assert prof["pull"][0] > 0
+    assert prof["solve1"][0] > 0
+    assert prof["factor_update"][0] == 0
