@@ -71,6 +71,28 @@ def timed_run(fe, steps, warmup, profile=False):
     return rows, wall, fe.timers(), prof
 
 
+def adjoint_ratio(fe, steps, forward_wall, ckpt):
+    """Gradient of a dB/dt misfit over the same bounded run: taped forward
+    (checkpoints every `ckpt` steps) + backward adjoint sweep, over the plain
+    forward's wall time, same mesh, source and schedule."""
+    import numpy as np
+    keep = fe.checks()["keep"]
+    fe.set_replay(keep)
+    fe.set_checkpointing(ckpt)
+    t0 = time.time()
+    rows = fe.run(max_steps=steps)
+    t1 = time.time()
+    w = np.zeros_like(rows["fields"])
+    w[..., 5] = 1.0  # J = sum of dBz/dt over rows and receivers
+    g = fe.adjoint(w)
+    t2 = time.time()
+    fe.set_checkpointing(0)
+    fe.set_replay([])
+    return {"steps": int(rows["step"][-1]), "checkpoint_every": ckpt, "taped_forward_s": t1 - t0,
+            "backward_s": t2 - t1, "forward_s": forward_wall, "ratio": (t2 - t0) / forward_wall,
+            "grad_finite": bool(np.all(np.isfinite(g)))}
+
+
 def cpu_baseline(m, counts, sample_steps):
     """Oracle (C restatement + MKL PARDISO, the reference's solver) on the
     host cores for the first `sample_steps` steps of one transmitter of the
@@ -116,6 +138,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile-kernel autotuner (use defaults)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
+    ap.add_argument("--ckpt", type=int, default=50, help="adjoint checkpoint interval (steps)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU sharding of the path is a design in "
@@ -161,15 +185,21 @@ def main():
             "step_breakdown_ms": {k: round(v[1], 2) for k, v in prof.items()}}
     fe.close()
     plain = None
+    adjoint = None
     if not a.no_plain:
         fp, _ = setup_gpu(m, [0])
         prow, pwall, ptm, _ = timed_run(fp, a.steps, a.warmup)
         psteps = int(prow["step"][-1])
         plain = {"value": psteps / pwall, "steps": psteps, "wall_s": pwall, "factor_s": ptm["factor"],
                  "factorisations": int(ptm["nfactor"])}
-        fp.close()
         print(f"[bench] plain: {psteps} steps in {pwall:.3f} s ({ptm['factor']:.3f} s factorising)",
               file=sys.stderr, flush=True)
+        if not a.no_adjoint:
+            adjoint = adjoint_ratio(fp, a.steps, pwall, a.ckpt)
+            print(f"[bench] adjoint: taped forward {adjoint['taped_forward_s']:.3f} s + backward "
+                  f"{adjoint['backward_s']:.3f} s = {adjoint['ratio']:.2f} x the plain forward", file=sys.stderr,
+                  flush=True)
+        fp.close()
     cpu = None if a.no_cpu else cpu_baseline(m, (steps, nf), a.cpu_sample)
     out = {
         "metric": "FP64 source-timesteps/sec at 1M tets (batched vs plain); adjoint/forward time",
@@ -180,7 +210,8 @@ def main():
                                f"factorisation, {m.ntet} tets, {n} edges", "mode": "batched", "sources": nsrc,
                    "nnzL": st["nnzL"], "factorisations": nf,
                    "plain_single_source": plain, "autotune": tune and tune["best"],
-                   "batched_over_plain": (value / plain["value"]) if plain else None},
+                   "batched_over_plain": (value / plain["value"]) if plain else None,
+                   "adjoint_over_forward": adjoint},
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

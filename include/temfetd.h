@@ -143,6 +143,20 @@ int tf_release(tf_ctx *ctx, int slot);
  * not stored.  max_steps > 0 caps the loop (bounded runs). */
 int tf_run(tf_ctx *ctx, int max_steps, int max_rows, int *nrows, int *row_step, double *row_t,
            double *row_dt, double *fields);
+/* Adjoint (no reference counterpart; the reference has no gradient).
+ * tf_set_checkpointing(ctx, k) with k > 0 makes the following tf_run record
+ * a tape of its solves and checkpoint the two BDF2 history states every k
+ * steps (k = 0 turns it off).  tf_adjoint then returns, for
+ *   J = sum over rows, sources, receivers, components of w * fields
+ * (w laid out like tf_run's fields: [nrows][src][rec][9], nrows = the rows of
+ * the taped run), grad_logsigma[nT] = dJ/d(log sigma_t) per tet: a
+ * backward-in-time sweep that recomputes each checkpoint segment, solves the
+ * adjoint systems with the same LDL^T factors (P is symmetric) and reduces
+ * -lambda^T M1_t (3 y - 4 u + v) per element.  The controller's keep/change
+ * decisions are those of the taped run (differentiate with them frozen; see
+ * tf_set_replay).  Step-on and impulse sources (pulse types 1, 2). */
+int tf_set_checkpointing(tf_ctx *ctx, int every);
+int tf_adjoint(tf_ctx *ctx, int nrows, const double *w, double *grad_logsigma);
 /* Replay the controller's decisions instead of computing them: keep[i] is
  * the keep (1) / change (0) outcome of the i-th doubling check (as logged by
  * tf_get_checks of an earlier run); the checks' solves and factorisations

@@ -67,6 +67,13 @@ void launch_unpermute(int n, const int *perm, const double *xw, double *x, hipSt
 // msolve.hip
 void launch_munpermute(int n, int K, const int *perm, const double *xw, double *x, hipStream_t st);
 void launch_fill_pattern(int64_t len, double *b, hipStream_t st);
+// adjoint.hip
+void launch_adj_seed(int nR, int K, const int *rec_edge, const double *rec_drc, const double *n_arr,
+                     const double *d_arr, const double *w, double *g, hipStream_t st);
+void launch_sens(int nT, int K, const int *tet_edg, const uint8_t *orient, const double *vm1, const double *lam,
+                 const double *y, const double *u, const double *v, double *grad, hipStream_t st);
+void launch_axpy(int64_t len, double a, const double *x, double *y, hipStream_t st);
+void launch_scale(int n, const double *s, double *g, hipStream_t st);
 // mtile.hip
 void launch_mtile(int mode, int ks, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits, int K,
                   const double *in0, const double *in1, const double *Dn, const int *sidx, double *out, double *part,
@@ -266,6 +273,24 @@ struct tf_ctx {
     std::vector<int> chk_step, chk_keep;
     std::vector<double> chk_err;
     double timers[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // adjoint: tape of the last tf_run (solve nodes, output rows, history
+    // checkpoints every ckpt_every steps)
+    struct TNode {
+        int u, v, step, row;  // history nodes stepped from; row written from this node or -1
+        double dt, t;
+    };
+    int ckpt_every = 0;
+    bool tape_valid = false;
+    std::vector<TNode> tape;
+    struct Ckpt {
+        int first, id2, id3;
+        double *s2, *s3;
+    };
+    std::vector<Ckpt> ckpts;
+    std::vector<int> row_node;
+    int *d_tet_edg = nullptr;
+    double *d_vm1 = nullptr;
+    double slot_dt[2] = {-1.0, -1.0};  // dt a slot was factored for (-1: none / caller matrix)
     // optional per-category kernel timing with HIP events on the context's stream
     bool prof_on = false;
     std::vector<hipEvent_t> ev_pool;
@@ -297,9 +322,19 @@ void free_vecs(tf_ctx *c) {
     c->rows_cap = 0;
 }
 
+void free_tape(tf_ctx *c) {
+    for (auto &k : c->ckpts) { dfree(k.s2); dfree(k.s3); }
+    c->ckpts.clear();
+    c->tape.clear();
+    c->row_node.clear();
+    c->tape_valid = false;
+}
+
 void free_all(tf_ctx *c) {
     free_solver(c);
     free_vecs(c);
+    free_tape(c);
+    dfree(c->d_tet_edg); dfree(c->d_vm1);
     dfree(c->d_xyz); dfree(c->d_sigma); dfree(c->d_invmu); dfree(c->d_vm); dfree(c->d_vk);
     dfree(c->d_tet_vex); dfree(c->d_tet_drc); dfree(c->d_ctr); dfree(c->d_orient); dfree(c->d_cptr);
     dfree(c->d_M); dfree(c->d_K); dfree(c->d_fia); dfree(c->d_fja); dfree(c->d_fmap); dfree(c->d_fval);
@@ -696,6 +731,7 @@ void factor_dt(tf_ctx *c, int slot, double dt) {
     tf::launch_scatter_p(c->topo.nnz, c->d_amap, c->d_M, c->d_K, 2.0 * dt, c->d_L[slot], c->st);
     run_factor(c, slot);
     c->slot_live[slot] = true;
+    c->slot_dt[slot] = dt;
 }
 
 // x = P^-1 b (device vectors, edge order)
@@ -911,6 +947,25 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
     int cur = 0;
     for (double *p : {v.e1, v.e2, v.e3}) HIPCHK(hipMemsetAsync(p, 0, sizeof(double) * n * c->nsrc, c->st));
     int ncheck = 0;
+    // adjoint tape: node ids of the history states e2, e3 and of the ring
+    const bool tape = c->ckpt_every > 0 && inout == 0;
+    int id2 = 0, id3 = 0;
+    std::vector<int> ring_id(ring, 0);
+    if (tape) {
+        for (auto &k : c->ckpts) { dfree(k.s2); dfree(k.s3); }
+        c->ckpts.clear();
+        c->tape.assign(1, tf_ctx::TNode{-1, -1, 0, 0, 0.0, 0.0});  // node 0: the zero initial state
+        c->row_node.assign(1, 0);
+    }
+    c->tape_valid = false;
+    auto node = [&](int u, int v, int st, double dt, double t) {
+        c->tape.push_back(tf_ctx::TNode{u, v, st, -1, dt, t});
+        return (int)c->tape.size() - 1;
+    };
+    auto row_of = [&](int k) {  // the output row just written comes from node k
+        c->tape[k].row = c->nrows - 1;
+        c->row_node.push_back(k);
+    };
     hipEvent_t f0, f1;
     HIPCHK(hipEventCreate(&f0));
     HIPCHK(hipEventCreate(&f1));
@@ -926,6 +981,13 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
         if (max_steps > 0 && step >= max_steps) break;
         step++;
         time_cur = time_cur + step_cur;
+        if (tape && (step - 1) % c->ckpt_every == 0) {  // history checkpoint before this step
+            const size_t bytes = sizeof(double) * n * c->nsrc;
+            tf_ctx::Ckpt k{(int)c->tape.size(), id2, id3, dalloc<double>(n * c->nsrc), dalloc<double>(n * c->nsrc)};
+            HIPCHK(hipMemcpyAsync(k.s2, v.e2, bytes, hipMemcpyDeviceToDevice, c->st));
+            HIPCHK(hipMemcpyAsync(k.s3, v.e3, bytes, hipMemcpyDeviceToDevice, c->st));
+            c->ckpts.push_back(k);
+        }
         if (step == 1) {
             if (inout == 2) { dcopy(c, v.e2, res); dcopy(c, v.e3, res); }
             if (outp) output_row(c, 0, time_cur - step_cur, step_cur, v.e2);
@@ -935,9 +997,17 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
             dcopy(c, v.te2, v.e2);
             dcopy(c, v.te3, v.e3);
             double ttime = time_cur, tstep = step_cur;
+            int fid[3] = {0, id2, id3};  // nodes of te1, te2, te3
+            int f1id = 0;
             for (int i = 1; i <= dbl; i++) {
                 rhs(c, step, ttime, v.te2, v.te3, tstep, negate, v.rh);
                 step_solve(c, cur, v.rh, v.te1);
+                if (tape) {
+                    fid[0] = node(fid[1], fid[2], step, tstep, ttime);
+                    if (i == 1) f1id = fid[0];
+                    fid[2] = fid[1];
+                    fid[1] = fid[0];
+                }
                 if (i == dbl) break;
                 if (i == 1) dcopy(c, v.erec, v.te1);
                 std::swap(v.te3, v.te2);   // te3 <- te2
@@ -969,6 +1039,7 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
                 c->slot_live[other] = false;
                 dcopy(c, v.e1, v.erec);
                 if (outp) output_row(c, step, time_cur, step_cur, v.e1);
+                if (tape) { row_of(f1id); id3 = id2; id2 = f1id; }
                 std::swap(v.e3, v.e2);  // e3 <- e2
                 dcopy(c, v.e2, v.e1);   // e2 <- e1
                 not_changed++;
@@ -981,6 +1052,7 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
                 time_cur = ttime;
                 step_cur = tstep;
                 if (outp) output_row(c, step, time_cur, step_cur, v.e2);
+                if (tape) { row_of(fid[0]); id3 = ring_id[(idx - 1) % ring]; id2 = fid[0]; }
                 not_changed = 0;
                 idx = 0;
             }
@@ -989,6 +1061,13 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
         rhs(c, step, time_cur, v.e2, v.e3, step_cur, negate, v.rh);
         step_solve(c, cur, v.rh, v.e1);
         if (outp) output_row(c, step, time_cur, step_cur, v.e1);
+        if (tape) {
+            const int k = node(id2, id3, step, step_cur, time_cur);
+            row_of(k);
+            id3 = id2;
+            id2 = k;
+            ring_id[idx % ring] = k;
+        }
         dcopy(c, v.ring[idx % ring], v.e1);
         idx++;
         // e3 <- e2 <- e1 by rotation; e1 is overwritten next step
@@ -999,6 +1078,7 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
     }
     HIPCHK(hipStreamSynchronize(c->st));
     HIPCHK(hipGetLastError());
+    if (tape) c->tape_valid = ((int)c->row_node.size() == c->nrows);
     if (inout == 1 && res) dcopy(c, res, v.e1 == v.e2 ? v.e1 : v.e2);
     HIPCHK(hipEventDestroy(f0));
     HIPCHK(hipEventDestroy(f1));
@@ -1012,6 +1092,149 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
 // C ABI
 // ======================================================================
 extern "C" {
+
+// ---- adjoint ------------------------------------------------------------------
+// Backward-in-time adjoint over the tape of the last tf_run.  Segment by
+// segment from the last checkpoint to the first: recompute the segment's
+// forward states from its checkpoint (same factors, same right-hand sides),
+// then walk its nodes backwards solving the adjoint systems with the same
+// symmetric factors and reducing the per-element sigma sensitivities.
+// w: [row][src][rec][9] weights of J = sum w * fields.
+void run_adjoint(tf_ctx *c, const double *w, double *grad_host) {
+    const int n = c->topo.nE, K = c->nsrc, R = c->nR, T = c->topo.nT;
+    const int64_t len = (int64_t)n * K;
+    const int nrows = c->nrows;
+    // effective per-row weights on E (n_arr) and dB/dt (d_arr): B_r is the
+    // running sum of dB/dt_i dt_i, so its weights fold into the dB/dt weights
+    // of every earlier row; dB/dt is zero at step 0 and row 0 is constant
+    std::vector<double> hw((size_t)nrows * K * R * 6, 0.0);
+    {
+        std::vector<double> bsum((size_t)K * R * 3, 0.0);
+        for (int r = nrows - 1; r >= 0; r--)
+            for (int q = 0; q < K; q++)
+                for (int i = 0; i < R; i++) {
+                    const double *wi = w + (((int64_t)r * K + q) * R + i) * 9;
+                    double *o = &hw[(((size_t)r * K + q) * R + i) * 6];
+                    for (int k = 0; k < 3; k++) {
+                        bsum[((size_t)q * R + i) * 3 + k] += wi[6 + k];
+                        o[k] = wi[k];
+                        o[3 + k] = (c->row_step[r] == 0) ? 0.0 : wi[3 + k] + c->row_dt[r] * bsum[((size_t)q * R + i) * 3 + k];
+                    }
+                }
+    }
+    double *d_w = dupload(hw);
+    if (!c->d_tet_edg) c->d_tet_edg = dupload(c->topo.tet_edg);
+    if (!c->d_vm1) {  // unit-conductivity element mass entries
+        std::vector<double> ones(T, 1.0);
+        double *d_ones = dupload(ones), *d_vk = dalloc<double>((size_t)T * 21);
+        c->d_vm1 = dalloc<double>((size_t)T * 21);
+        tf::launch_element(T, c->d_xyz, c->d_tet_vex, c->d_tet_drc, c->d_orient, d_ones, c->d_invmu, c->d_vm1, d_vk,
+                           c->st);
+        HIPCHK(hipStreamSynchronize(c->st));
+        dfree(d_ones);
+        dfree(d_vk);
+    }
+    // live nodes: those a row was written from, and their history
+    const int nn = (int)c->tape.size();
+    std::vector<char> live(nn, 0);
+    for (int r = 1; r < nrows; r++) live[c->row_node[r]] = 1;
+    for (int k = nn - 1; k > 0; k--)
+        if (live[k]) {
+            if (c->tape[k].u > 0) live[c->tape[k].u] = 1;
+            if (c->tape[k].v > 0) live[c->tape[k].v] = 1;
+        }
+    // vector pool
+    std::vector<double *> pool, owned;
+    auto get = [&]() {
+        double *p;
+        if (pool.empty()) { p = dalloc<double>(len); owned.push_back(p); }
+        else { p = pool.back(); pool.pop_back(); }
+        return p;
+    };
+    auto zeroed = [&]() {
+        double *p = get();
+        tf::launch_fill0(p, len, c->st);
+        return p;
+    };
+    double *zero = zeroed(), *rh = get(), *lam = get(), *grad = dalloc<double>(T);
+    tf::launch_fill0(grad, T, c->st);
+    std::vector<double *> state(nn, nullptr);
+    std::vector<double *> accA(nn, nullptr), accB(nn, nullptr);
+    state[0] = zero;
+    int last_used = 0;
+    auto slot_for = [&](double dt) {
+        for (int s = 0; s < 2; s++)
+            if (c->slot_live[s] && c->slot_dt[s] == dt) { last_used = s; return s; }
+        const int s = 1 - last_used;
+        factor_dt(c, s, dt);
+        check_flag(c);
+        c->timers[7] += 1.0;
+        last_used = s;
+        return s;
+    };
+    auto S = [&](int k) -> double * {
+        if (k == 0) return zero;
+        if (!state[k]) throw std::runtime_error("adjoint: history state missing from the tape");
+        return state[k];
+    };
+    const int nseg = (int)c->ckpts.size();
+    for (int g = nseg - 1; g >= 0; g--) {
+        const tf_ctx::Ckpt &ck = c->ckpts[g];
+        const int first = ck.first, end = (g + 1 < nseg) ? c->ckpts[g + 1].first : nn;
+        if (ck.id2 > 0) state[ck.id2] = ck.s2;
+        if (ck.id3 > 0) state[ck.id3] = ck.s3;
+        // recompute the segment's live states
+        for (int k = first; k < end; k++) {
+            if (!live[k]) continue;
+            const tf_ctx::TNode &nd = c->tape[k];
+            const int slot = slot_for(nd.dt);
+            rhs(c, nd.step, nd.t, S(nd.u), S(nd.v), nd.dt, false, rh);
+            state[k] = get();
+            step_solve(c, slot, rh, state[k]);
+        }
+        // adjoint sweep
+        for (int k = end - 1; k >= first; k--) {
+            if (!live[k]) continue;
+            const tf_ctx::TNode &nd = c->tape[k];
+            const int slot = slot_for(nd.dt);
+            if (accA[k] || accB[k]) {
+                const double *a = accA[k] ? accA[k] : zero, *b = accB[k] ? accB[k] : zero;
+                if (K == 1) tf::launch_spmv_rhs(n, c->d_fia, c->d_fja, c->d_fval, a, b, rh, c->st);
+                else tf::launch_mspmv_rhs(n, K, c->d_fia, c->d_fja, c->d_fval, a, b, c->d_mb, rh, c->st);
+            } else {
+                tf::launch_fill0(rh, len, c->st);
+            }
+            if (nd.row > 0)
+                tf::launch_adj_seed(R, K, c->d_rec_edge, c->d_rec_drc, c->d_narr, c->d_darr,
+                                    d_w + (int64_t)nd.row * K * R * 6, rh, c->st);
+            step_solve(c, slot, rh, lam);
+            tf::launch_sens(T, K, c->d_tet_edg, c->d_orient, c->d_vm1, lam, S(k), S(nd.u), S(nd.v), grad, c->st);
+            if (nd.u > 0) {
+                if (!accA[nd.u]) accA[nd.u] = zeroed();
+                tf::launch_axpy(len, 1.0, lam, accA[nd.u], c->st);
+            }
+            if (nd.v > 0) {
+                if (!accB[nd.v]) accB[nd.v] = zeroed();
+                tf::launch_axpy(len, 1.0, lam, accB[nd.v], c->st);
+            }
+            if (accA[k]) { pool.push_back(accA[k]); accA[k] = nullptr; }
+            if (accB[k]) { pool.push_back(accB[k]); accB[k] = nullptr; }
+        }
+        // the segment's states are no longer needed (earlier segments
+        // recompute their own; checkpoint buffers are not pooled)
+        for (int k = first; k < end; k++)
+            if (state[k]) { pool.push_back(state[k]); state[k] = nullptr; }
+        if (ck.id2 > 0) state[ck.id2] = nullptr;
+        if (ck.id3 > 0) state[ck.id3] = nullptr;
+    }
+    tf::launch_scale(T, c->d_sigma, grad, c->st);
+    HIPCHK(hipMemcpyAsync(grad_host, grad, sizeof(double) * T, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipGetLastError());
+    for (double *p : owned) dfree(p);
+    dfree(grad);
+    dfree(d_w);
+}
 
 // message of the last tf_create failure (no context exists to hold it)
 static thread_local std::string g_create_err = "no context";
@@ -1053,6 +1276,7 @@ const char *tf_last_error(const tf_ctx *c) { return c ? c->err.c_str() : g_creat
 
 int tf_set_mesh(tf_ctx *c, int nN, const double *nod_cor, int nT, const int *tet_vex) {
     return guarded(c, [&] {
+        c->tape_valid = false;
         if (nN <= 0 || nT <= 0 || !nod_cor || !tet_vex) return fail(c, TF_ERR_ARG, "empty mesh");
         free_all(c);
         c->assembled = c->have_mat = c->have_src = false;
@@ -1127,6 +1351,7 @@ int tf_get_topology(const tf_ctx *c, int *edg_vex, int *tet_edg, int *tet_drc, i
 
 int tf_set_materials(tf_ctx *c, const double *sigma, const double *inv_mu) {
     return guarded(c, [&] {
+        c->tape_valid = false;
         if (!c->have_mesh) return fail(c, TF_ERR_STATE, "tf_set_mesh first");
         if (!sigma || !inv_mu) return fail(c, TF_ERR_ARG, "null material array");
         c->sigma.assign(sigma, sigma + c->topo.nT);
@@ -1143,6 +1368,7 @@ int tf_set_materials(tf_ctx *c, const double *sigma, const double *inv_mu) {
 
 int tf_set_num_sources(tf_ctx *c, int nsrc) {
     if (!c) return TF_ERR_ARG;
+    c->tape_valid = false;
     if (nsrc < 1 || nsrc > 64) return fail(c, TF_ERR_ARG, "number of sources must be 1..64");
     c->nsrc = nsrc;
     c->src_edges.assign(nsrc, {});
@@ -1154,6 +1380,7 @@ int tf_set_num_sources(tf_ctx *c, int nsrc) {
 int tf_set_source(tf_ctx *c, int isrc, int nseg, const int *seg_vex, int source_type, int pulse_type,
                   double pulse_width, double current) {
     return guarded(c, [&] {
+        c->tape_valid = false;
         if (!c->have_mesh) return fail(c, TF_ERR_STATE, "tf_set_mesh first");
         if ((int)c->src_edges.size() != c->nsrc) {
             c->src_edges.assign(c->nsrc, {});
@@ -1213,6 +1440,7 @@ int tf_set_replay(tf_ctx *c, int nchecks, const int *keep) {
 
 int tf_set_receivers(tf_ctx *c, int nR, const double *xyz, double rec_charac, int *rec_tet_out) {
     return guarded(c, [&] {
+        c->tape_valid = false;
         if (!c->have_mesh) return fail(c, TF_ERR_STATE, "tf_set_mesh first");
         if (nR <= 0 || !xyz) return fail(c, TF_ERR_ARG, "no receivers");
         c->nR = nR;
@@ -1247,6 +1475,7 @@ int tf_set_receivers(tf_ctx *c, int nR, const double *xyz, double rec_charac, in
 
 int tf_set_schedule(tf_ctx *c, int sp_division, int sp_double, int sp_dblesize, double time_maximum) {
     if (!c) return TF_ERR_ARG;
+    c->tape_valid = false;
     if (sp_division <= 0) return fail(c, TF_ERR_INPUT, "cmod%step_dvsi error");
     if (sp_double <= 0) return fail(c, TF_ERR_INPUT, "cmod%step_dble error");
     if (std::fabs(time_maximum) < 1e-20) return fail(c, TF_ERR_INPUT, "cmod%time_maxm error");
@@ -1260,6 +1489,7 @@ int tf_set_schedule(tf_ctx *c, int sp_division, int sp_double, int sp_dblesize, 
 
 int tf_assemble(tf_ctx *c) {
     return guarded(c, [&] {
+        c->tape_valid = false;
         if (!c->have_mesh || !c->have_mat) return fail(c, TF_ERR_STATE, "mesh and materials first");
         auto t0 = std::chrono::steady_clock::now();
         const Topology &t = c->topo;
@@ -1332,6 +1562,28 @@ int tf_set_tuning(tf_ctx *c, const char *key, int value) {
     });
 }
 
+int tf_set_checkpointing(tf_ctx *c, int every) {
+    return guarded(c, [&] {
+        if (every < 0) return fail(c, TF_ERR_ARG, "checkpoint interval must be >= 0");
+        c->ckpt_every = every;
+        if (every == 0) free_tape(c);
+        return TF_OK;
+    });
+}
+
+int tf_adjoint(tf_ctx *c, int nrows, const double *w, double *grad_logsigma) {
+    return guarded(c, [&] {
+        if (c->ptype == 3) return fail(c, TF_ERR_STATE, "the adjoint covers step-on and impulse sources (pulse types 1, 2)");
+        if (!c->tape_valid) return fail(c, TF_ERR_STATE, "no taped run: tf_set_checkpointing(ctx, k > 0) then tf_run");
+        if (nrows != c->nrows) return fail(c, TF_ERR_ARG, "weights must cover every row of the taped run");
+        if (!w || !grad_logsigma) return fail(c, TF_ERR_ARG, "null buffer");
+        auto t0 = std::chrono::steady_clock::now();
+        run_adjoint(c, w, grad_logsigma);
+        c->timers[5] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return TF_OK;
+    });
+}
+
 int tf_set_profiling(tf_ctx *c, int on) {
     return guarded(c, [&] {
         prof_fold(c);
@@ -1388,6 +1640,7 @@ int tf_factor_csr(tf_ctx *c, int slot, int n, const int *ia, const int *ja, cons
         tf::launch_scatter_vals(t.nnz, c->d_amap, dv, c->d_L[slot], c->st);
         run_factor(c, slot);
         c->slot_live[slot] = true;
+        c->slot_dt[slot] = -1.0;
         HIPCHK(hipStreamSynchronize(c->st));
         dfree(dv);
         check_flag(c);

@@ -51,6 +51,8 @@ _SIGS = {
     "tf_time_solve": [_P, _I, _I, _I, _P],
     "tf_set_profiling": [_P, _I],
     "tf_get_profile": [_P, _I, _P, _P],
+    "tf_set_checkpointing": [_P, _I],
+    "tf_adjoint": [_P, _I, _P, _P],
 }
 
 # kernel-timing categories of tf_get_profile (include/temfetd.h)
@@ -249,6 +251,21 @@ class Fetd:
         fields = f[:k].copy()
         return dict(step=steps[:k].copy(), t=t[:k].copy(), dt=dt[:k].copy(),
                     fields=fields[:, 0] if self.nsrc == 1 else fields)
+
+    # -- adjoint ----------------------------------------------------------------
+    def set_checkpointing(self, every):
+        """Record the next run's tape with history checkpoints every `every`
+        steps (0: off), for adjoint()."""
+        self._chk(self.lib.tf_set_checkpointing(self.ctx, int(every)), "tf_set_checkpointing")
+
+    def adjoint(self, w):
+        """dJ/dlog(sigma) per tet for J = sum(w * fields) over the rows of the
+        last (taped) run; w shaped like run()'s fields."""
+        w = np.ascontiguousarray(w, np.float64)
+        nrows = w.shape[0]
+        g = np.empty(len(self.tet_vex))
+        self._chk(self.lib.tf_adjoint(self.ctx, nrows, _ptr(w), _ptr(g)), "tf_adjoint")
+        return g
 
     def checks(self):
         n = _I()
