@@ -71,6 +71,19 @@ def timed_run(fe, steps, warmup, profile=False):
     return rows, wall, fe.timers(), prof
 
 
+def measured_traffic():
+    """HBM bytes per k_mtile launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes of this workload, scaled by a 1 GiB device copy (the summary in
+    profiles/hbm_traffic_batched3d.txt gives the commands); None if absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_batched3d.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def adjoint_ratio(fe, steps, forward_wall, ckpt):
     """Gradient of a dB/dt misfit over the same bounded run: taped forward
     (checkpoints every `ckpt` steps) + backward adjoint sweep, over the plain
@@ -128,6 +141,19 @@ def cpu_baseline(m, counts, sample_steps):
             "pardiso_factor_nnz": int(r["factor_nnz"]), "wall_s": wall}
 
 
+def copy_probe():
+    """Calibration for HBM counters: three device copies of 1 GiB (read 1 GiB,
+    write 1 GiB each); FETCH_SIZE / WRITE_SIZE of these dispatches against the
+    known bytes gives the scale applied to the solver kernels' counters."""
+    import torch
+    a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": a.numel() * 8, "copies": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,7 +166,11 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the tile-kernel autotuner (use defaults)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
     ap.add_argument("--ckpt", type=int, default=50, help="adjoint checkpoint interval (steps)")
+    ap.add_argument("--copy-probe", action="store_true", help="1 GiB device copies only (counter calibration)")
     a = ap.parse_args()
+    if a.copy_probe:
+        copy_probe()
+        return
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU sharding of the path is a design in "
               "DESIGN.md and is not launched", file=sys.stderr)
@@ -177,8 +207,11 @@ def main():
     flops_solve = 4.0 * nsrc * nnzl_off
     tile_launches, tile_ms = prof["tile"]
     achieved = flops_solve * nsolves / (tile_ms * 1e-3) / 1e12
+    traffic = measured_traffic() if a.model == "batched3d" else None
     roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+            "traffic_note": ("HBM bytes per k_mtile launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on a "
+                             "1 GiB copy), profiles/hbm_traffic_batched3d.txt") if traffic else None,
             "kernel": (f"k_mtile (block-solve tile products, {nsrc} right-hand sides): {nsolves} solves x "
                        f"4*K*nnzL_off = {flops_solve:.3e} flops each, {tile_launches} launches, "
                        f"{tile_ms:.1f} ms (HIP events)"),

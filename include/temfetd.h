@@ -60,6 +60,13 @@ int tf_get_topology(const tf_ctx *ctx, int *edg_vex, int *tet_edg, int *tet_drc,
 
 /* model_ctl (src/FETD_FWD.f90:347-364): per-tet conductivity and 1/mu. */
 int tf_set_materials(tf_ctx *ctx, const double *sigma, const double *inv_mu);
+/* Anisotropic media (model_ctl 'anisotropic:' src/FETD_FWD.f90:368-410;
+ * f_kind_ani/e_kind_ani :853-912): per tet a row-major 3x3 conductivity
+ * tensor [nT][9] and the 3x3 curl weight [nT][9] the reference puts between
+ * the curls (the entrywise reciprocal of its permeability tensor, zero
+ * entries kept, src/FETD_FWD.f90:621-633).  Replaces tf_set_materials until
+ * that is called again. */
+int tf_set_materials_tensor(tf_ctx *ctx, const double *sigma9, const double *nu9);
 /* Source wire (src/FETD_FWD.f90:418-448, sys_right :1466-1511): the nseg
  * segments [nseg][2] (1-based vertex ids) in file order; signs are +1 for all
  * segments but the last (-1).  source_type 1, pulse_type 1 (step-on),

@@ -184,6 +184,8 @@ struct fo_model {
     double *n_arr, *d_arr; /* [R][6][3] */
     int ndom;
     double *sig, *per, *mag; /* per tet */
+    int aniso;               /* anisotropic media: per-tet 3x3 tensors (row-major) */
+    double *sig9, *nu9;      /* conductivity tensor; curl weight = elementwise 1/mu tensor */
     int bound_cond;
     int src_num, src_type, pulse_type;
     double pulse_width, current;
@@ -228,7 +230,7 @@ void fo_free(fo_model *m) {
                     m->tet_len, m->rec_xyz, m->rec_tet, m->n_arr, m->d_arr, m->sig, m->per,
                     m->mag, m->src_edg, m->src_drc, m->gates, m->ia, m->ja, m->av, m->bv,
                     m->cv, m->ia0, m->ja0, m->bfull, m->row_step, m->row_t, m->row_dt,
-                    m->row_f, m->chk_step, m->chk_keep, m->chk_err, m->replay};
+                    m->row_f, m->chk_step, m->chk_keep, m->chk_err, m->replay, m->sig9, m->nu9};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free(ptrs[i]);
     free(m);
 }
@@ -350,6 +352,93 @@ void fo_tet_matrices(const double *x, const double *y, const double *z, double s
         len[j] = dist3(P, Q);
     }
     elem_mats(a, b, c, d, vol, len, s_mass, s_curl, F, K);
+}
+
+/* f_kind_ani (src/FETD_FWD.f90:884-912) and e_kind_ani (:853-882): the
+ * same Whitney integrals with a 3x3 tensor A (row-major) between the
+ * gradients / curls.  Operation order follows the reference (scalar 1). */
+static void elem_mats_ani(const double *b, const double *c, const double *d, double vol,
+                          const double *len, const double *Am, const double *Ak, double *F,
+                          double *K) {
+    double g[6][3];
+    double v6 = 6.0 * vol;
+    for (int i = 0; i < 6; i++) {
+        int i1 = IND_TET[i][0], i2 = IND_TET[i][1];
+        double s = 2.0 * len[i] / (v6 * v6);
+        g[i][0] = (c[i1] * d[i2] - c[i2] * d[i1]) * s;
+        g[i][1] = (b[i2] * d[i1] - b[i1] * d[i2]) * s;
+        g[i][2] = (b[i1] * c[i2] - b[i2] * c[i1]) * s;
+    }
+    for (int i = 0; i < 6; i++) {
+        for (int j = 0; j < 6; j++) {
+            int im[2] = {IND_TET[i][0], IND_TET[i][1]};
+            int jn[2] = {IND_TET[j][0], IND_TET[j][1]};
+            double f[2][2];
+            for (int mm = 0; mm < 2; mm++)
+                for (int nn = 0; nn < 2; nn++) {
+                    double v1[3] = {b[im[mm]], c[im[mm]], d[im[mm]]};
+                    double v2[3] = {b[jn[nn]], c[jn[nn]], d[jn[nn]]};
+                    double w[3];
+                    for (int r = 0; r < 3; r++) w[r] = Am[3 * r] * v2[0] + Am[3 * r + 1] * v2[1] + Am[3 * r + 2] * v2[2];
+                    f[mm][nn] = v1[0] * w[0] + v1[1] * w[1] + v1[2] * w[2];
+                }
+            double v = f[1][1] * (1 + (im[0] == jn[0])) - f[0][1] * (1 + (im[1] == jn[0])) -
+                       f[1][0] * (1 + (im[0] == jn[1])) + f[0][0] * (1 + (im[1] == jn[1]));
+            F[i * 6 + j] = len[i] * len[j] / (vol * 720.0) * v;
+            double w[3];
+            for (int r = 0; r < 3; r++) w[r] = Ak[3 * r] * g[j][0] + Ak[3 * r + 1] * g[j][1] + Ak[3 * r + 2] * g[j][2];
+            K[i * 6 + j] = vol * (g[i][0] * w[0] + g[i][1] * w[1] + g[i][2] * w[2]);
+        }
+    }
+}
+
+void fo_tet_matrices_ani(const double *x, const double *y, const double *z, const double *A_mass,
+                         const double *A_curl, double *F, double *K) {
+    double a[4], b[4], c[4], d[4], vol, len[6];
+    fo_tet_cofactors(x, y, z, a, b, c, d, &vol);
+    for (int j = 0; j < 6; j++) {
+        int p = IND_TET[j][0], q = IND_TET[j][1];
+        double P[3] = {x[p], y[p], z[p]}, Q[3] = {x[q], y[q], z[q]};
+        len[j] = dist3(P, Q);
+    }
+    elem_mats_ani(b, c, d, vol, len, A_mass, A_curl, F, K);
+}
+
+/* model_ctl anisotropic tensors (src/FETD_FWD.f90:368-410): principal
+ * values v[3] rotated by the angles ang[3] (degrees) as R diag(v) R^T with
+ * R = Rx Ry Rz.  As in the reference, the (2,2) entry of Rz takes the cosine
+ * of the second angle. */
+void fo_aniso_tensor(const double *v, const double *ang_deg, double scale, double *A) {
+    const double d2r = 3.141592653589793238462643383279502884197 / 180.0;
+    double a1 = ang_deg[0] * d2r, a2 = ang_deg[1] * d2r, a3 = ang_deg[2] * d2r;
+    double dx[3][3] = {{1.0, 0.0, 0.0}, {0.0, cos(a1), -sin(a1)}, {0.0, sin(a1), cos(a1)}};
+    double dy[3][3] = {{cos(a2), 0.0, sin(a2)}, {0.0, 1.0, 0.0}, {-sin(a2), 0.0, cos(a2)}};
+    double dz[3][3] = {{cos(a3), -sin(a3), 0.0}, {sin(a3), cos(a2), 0.0}, {0.0, 0.0, 1.0}};
+    double t[3][3], dm[3][3], u[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double acc = 0.0;
+            for (int k = 0; k < 3; k++) acc += dx[i][k] * dy[k][j];
+            t[i][j] = acc;
+        }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double acc = 0.0;
+            for (int k = 0; k < 3; k++) acc += t[i][k] * dz[k][j];
+            dm[i][j] = acc;
+        }
+    for (int i = 0; i < 3; i++)  /* dm * diag(v) */
+        for (int j = 0; j < 3; j++) {
+            double acc = 0.0;
+            for (int k = 0; k < 3; k++) acc += dm[i][k] * (k == j ? v[j] : 0.0);
+            u[i][j] = acc;
+        }
+    for (int i = 0; i < 3; i++)  /* (dm diag(v)) dm^T, times scale */
+        for (int j = 0; j < 3; j++) {
+            double acc = 0.0;
+            for (int k = 0; k < 3; k++) acc += u[i][k] * dm[j][k];
+            A[3 * i + j] = acc * scale;
+        }
 }
 
 /* source_mod (src/FETD_UTILITY.f90:1030-1166) */
@@ -658,8 +747,19 @@ static int read_cntl(fo_model *m, const char *path) {
                 dom_ind[i] = (int)v[0]; dsig[i] = v[1]; dper[i] = v[2]; dmag[i] = v[3];
             }
         } else if (!strcmp(code, "anisotropic")) {
-            set_err("anisotropic media are not supported by this restatement");
-            goto bad2;
+            /* three records per domain: sigma, permittivity, permeability,
+             * each "domain v1 v2 v3 angle1 angle2 angle3" */
+            m->aniso = 1;
+            m->ndom = (int)fnum(value, NULL);
+            dom_ind = xcalloc((size_t)m->ndom, sizeof(int));
+            dsig = xcalloc((size_t)m->ndom * 18, sizeof(double)); /* sig v+ang, mag v+ang, per v+ang */
+            for (int i = 0; i < m->ndom; i++) {
+                for (int q = 0; q < 3; q++) {
+                    if (!read_nums(&r, v, 7)) goto bad;
+                    dom_ind[i] = (int)v[0];
+                    memcpy(&dsig[18 * i + 6 * q], &v[1], 6 * sizeof(double));
+                }
+            }
         } else if (!strcmp(code, "boundary condition")) {
             m->bound_cond = (int)fnum(value, NULL);
         } else if (!strcmp(code, "source")) {
@@ -698,6 +798,31 @@ static int read_cntl(fo_model *m, const char *path) {
     if (fabs(m->time_maxm) < 1e-20) { set_err("cmod%time_maxm error"); goto bad3; }
     if (m->step_dble == 0) { set_err("cmod%step_dble error"); goto bad3; }
     if (!any) { set_err("cmod%o_f error"); goto bad3; }
+    if (m->aniso) { /* tensors per tet (src/FETD_FWD.f90:381-408) */
+        m->sig9 = xcalloc((size_t)m->nT * 9, sizeof(double));
+        m->nu9 = xcalloc((size_t)m->nT * 9, sizeof(double));
+        m->sig = xcalloc((size_t)m->nT, sizeof(double));
+        m->per = xcalloc((size_t)m->nT, sizeof(double));
+        m->mag = xcalloc((size_t)m->nT, sizeof(double));
+        for (int i = 0; i < m->nT; i++) {
+            int found = 0;
+            for (int j = 0; j < m->ndom; j++)
+                if (m->tet_phy[i] == dom_ind[j]) {
+                    found = 1;
+                    double mg[9];
+                    fo_aniso_tensor(&dsig[18 * j], &dsig[18 * j + 3], 1.0, &m->sig9[9 * i]);
+                    fo_aniso_tensor(&dsig[18 * j + 12], &dsig[18 * j + 15], MU0, mg);
+                    /* the curl weight is the entrywise reciprocal, zero entries kept (:624-632) */
+                    for (int q = 0; q < 9; q++) m->nu9[9 * i + q] = (mg[q] == 0.0) ? 0.0 : 1.0 / mg[q];
+                    m->sig[i] = m->sig9[9 * i];
+                    m->mag[i] = mg[0];
+                }
+            if (!found) { set_err("error - physical domain not found!"); goto bad3; }
+        }
+        free(dom_ind); free(dsig);
+        dom_ind = NULL; dsig = NULL;
+        goto materials_done;
+    }
     /* materials per tet (src/FETD_FWD.f90:347-364) */
     m->sig = xcalloc((size_t)m->nT, sizeof(double));
     m->per = xcalloc((size_t)m->nT, sizeof(double));
@@ -715,6 +840,7 @@ static int read_cntl(fo_model *m, const char *path) {
     }
     free(dom_ind); free(dsig); free(dper); free(dmag);
     dom_ind = NULL;
+materials_done:
     /* source edges and signs (src/FETD_FWD.f90:436-448) */
     m->src_edg = xcalloc((size_t)m->nseg, sizeof(int));
     m->src_drc = xcalloc((size_t)m->nseg, sizeof(int));
@@ -746,7 +872,6 @@ static int read_cntl(fo_model *m, const char *path) {
     return 0;
 bad:
     set_err("truncated control file");
-bad2:
     fclose(r.f);
 bad3:
     free(dom_ind); free(dsig); free(dper); free(dmag);
@@ -793,9 +918,15 @@ int fo_assemble(fo_model *m) {
         double Fa[36], Fb[36], Kc[36], len[6];
         for (int j = 0; j < 6; j++) len[j] = m->tet_len[6 * i + j];
         double *a = &m->ca[4 * i], *b = &m->cb[4 * i], *c = &m->cc[4 * i], *d = &m->cd[4 * i];
-        elem_mats(a, b, c, d, m->vol[i], len, m->per[i], 1.0 / m->mag[i], Fa, Kc);
         double dummy[36];
-        elem_mats(a, b, c, d, m->vol[i], len, m->sig[i], 0.0, Fb, dummy);
+        if (m->aniso) {
+            /* the permittivity mass is multiplied by zero in P (:1288) */
+            for (int q = 0; q < 36; q++) Fa[q] = 0.0;
+            elem_mats_ani(b, c, d, m->vol[i], len, &m->sig9[9 * i], &m->nu9[9 * i], Fb, Kc);
+        } else {
+            elem_mats(a, b, c, d, m->vol[i], len, m->per[i], 1.0 / m->mag[i], Fa, Kc);
+            elem_mats(a, b, c, d, m->vol[i], len, m->sig[i], 0.0, Fb, dummy);
+        }
         const int *te = &m->tet_edg[6 * i];
         const int *td = &m->tet_drc[6 * i];
         for (int j = 0; j < 6; j++)
@@ -1279,6 +1410,13 @@ void fo_get_source(const fo_model *m, int *src_edg, int *src_drc) {
     if (src_edg) memcpy(src_edg, m->src_edg, sizeof(int) * (size_t)m->nseg);
     if (src_drc) memcpy(src_drc, m->src_drc, sizeof(int) * (size_t)m->nseg);
 }
+int fo_get_tensors(const fo_model *m, double *sig9, double *nu9) {
+    if (!m->aniso) return 0;
+    if (sig9) memcpy(sig9, m->sig9, sizeof(double) * 9 * (size_t)m->nT);
+    if (nu9) memcpy(nu9, m->nu9, sizeof(double) * 9 * (size_t)m->nT);
+    return 1;
+}
+
 void fo_get_material(const fo_model *m, double *sigma, double *inv_mu) {
     for (int i = 0; i < m->nT; i++) {
         if (sigma) sigma[i] = m->sig[i];

@@ -74,6 +74,16 @@ void fo_tet_cofactors(const double *x, const double *y, const double *z,
 /* 6x6 mass (scalar s) and curl-curl (scalar s) matrices of one tet, row-major */
 void fo_tet_matrices(const double *x, const double *y, const double *z,
                      double s_mass, double s_curl, double *F, double *K);
+/* anisotropic variants (f_kind_ani / e_kind_ani): 3x3 row-major tensors
+ * between the gradients (mass) and the curls (curl-curl) */
+void fo_tet_matrices_ani(const double *x, const double *y, const double *z, const double *A_mass,
+                         const double *A_curl, double *F, double *K);
+/* principal values v[3] rotated by three angles (degrees) as the reference
+ * builds its tensors, times scale; A row-major 3x3 */
+void fo_aniso_tensor(const double *v, const double *ang_deg, double scale, double *A);
+/* per-tet tensors of an anisotropic model ([nT][9], row-major; NULL-safe);
+ * returns 0 for an isotropic model */
+int fo_get_tensors(const fo_model *m, double *sig9, double *nu9);
 /* 4x4 determinant by LU with partial pivoting, column-major input */
 double fo_det_lu(const double *a_colmajor, int n);
 /* source time function d I/dt (src/FETD_UTILITY.f90:1045-1078) */

@@ -44,6 +44,7 @@ def lib():
         L.fo_run.argtypes = [_P, ctypes.c_char_p, ctypes.c_char_p, _I]
         L.fo_set_threads.argtypes = [_I]
         L.fo_pardiso_solve_once.argtypes = [_I, _P, _P, _P, _P, _P, _I]
+        L.fo_get_tensors.argtypes = [_P, _P, _P]
         _lib = L
     return _lib
 
@@ -125,6 +126,12 @@ class Oracle:
         self.L.fo_get_material(self.m, _p(s), _p(im))
         return s, im
 
+    def tensors(self):
+        """(sigma tensors, curl weights) [nT][3][3] of an anisotropic model, or None."""
+        s9 = np.empty((self.nT, 3, 3))
+        n9 = np.empty((self.nT, 3, 3))
+        return (s9, n9) if self.L.fo_get_tensors(self.m, _p(s9), _p(n9)) else None
+
     def set_replay(self, keep):
         k = np.ascontiguousarray(keep, np.int32)
         self.L.fo_set_replay(self.m, len(k), _p(k))
@@ -191,6 +198,24 @@ def tet_cofactors(x, y, z):
     vol = _D()
     lib().fo_tet_cofactors(_p(x), _p(y), _p(z), _p(a), _p(b), _p(c), _p(d), ctypes.byref(vol))
     return a, b, c, d, vol.value
+
+
+def tet_matrices_ani(x, y, z, A_mass, A_curl):
+    F = np.empty(36)
+    K = np.empty(36)
+    x, y, z = [np.ascontiguousarray(v, np.float64) for v in (x, y, z)]
+    Am = np.ascontiguousarray(A_mass, np.float64).reshape(9)
+    Ak = np.ascontiguousarray(A_curl, np.float64).reshape(9)
+    lib().fo_tet_matrices_ani(_p(x), _p(y), _p(z), _p(Am), _p(Ak), _p(F), _p(K))
+    return F.reshape(6, 6), K.reshape(6, 6)
+
+
+def aniso_tensor(values, angles_deg, scale=1.0):
+    v = np.ascontiguousarray(values, np.float64)
+    a = np.ascontiguousarray(angles_deg, np.float64)
+    A = np.empty(9)
+    lib().fo_aniso_tensor(_p(v), _p(a), _D(scale), _p(A))
+    return A.reshape(3, 3)
 
 
 def source_didt(source_type, pulse_type, t, t0, cur):

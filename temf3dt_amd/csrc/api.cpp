@@ -8,6 +8,8 @@
 //   solution_output src/FETD_FWD.f90:1513-1623 readout rows
 //   pardiso_direct_1/2/3 src/FETD_UTILITY.f90:879-975  tf_factor/tf_solve/tf_release
 #include <algorithm>
+#include <array>
+#include <map>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -24,6 +26,8 @@ namespace tf {
 // assembly.hip
 void launch_element(int nT, const double *xyz, const int *tet_vex, const int *tet_drc, const uint8_t *orient,
                     const double *sigma, const double *inv_mu, double *vm, double *vk, hipStream_t st);
+void launch_element_ani(int nT, const double *xyz, const int *tet_vex, const int *tet_drc, const uint8_t *orient,
+                        const double *As, const double *Ak, double *vm, double *vk, hipStream_t st);
 void launch_gather(int64_t nnz, const int64_t *cptr, const int *ctr, const double *vm, const double *vk,
                    double *M, double *K, hipStream_t st);
 void launch_recvec(int nR, const double *xyz, const int *tet_vex, const int *rec_tet, const double *rec_xyz,
@@ -67,6 +71,12 @@ void launch_unpermute(int n, const int *perm, const double *xw, double *x, hipSt
 // msolve.hip
 void launch_munpermute(int n, int K, const int *perm, const double *xw, double *x, hipStream_t st);
 void launch_fill_pattern(int64_t len, double *b, hipStream_t st);
+// step.hip: independent models
+void launch_source_models(int ns, const int *edge, const int *model, const double *w, const double *coef, double *r,
+                          hipStream_t st);
+void launch_scatter_p_models(int64_t nnz, const int64_t *amap, const int *qmodel, const double *M, const double *K,
+                             const double *cdt, const int64_t *slot_off, double *L, hipStream_t st);
+void launch_range_norms(int nm, const int *range, const double *a, const double *b, double *out, hipStream_t st);
 // adjoint.hip
 void launch_adj_seed(int nR, int K, const int *rec_edge, const double *rec_drc, const double *n_arr,
                      const double *d_arr, const double *w, double *g, hipStream_t st);
@@ -203,6 +213,8 @@ struct tf_ctx {
     Topology topo;
     bool have_mesh = false, have_mat = false, assembled = false, analyzed = false;
     std::vector<double> sigma, inv_mu;
+    bool aniso = false;  // tensors set by tf_set_materials_tensor
+    double *d_sig9 = nullptr, *d_nu9 = nullptr;
     // sources: nsrc loops sharing one waveform; entries (edge, source, weight)
     int nsrc = 1;
     int nseg = 0;
@@ -291,6 +303,23 @@ struct tf_ctx {
     int *d_tet_edg = nullptr;
     double *d_vm1 = nullptr;
     double slot_dt[2] = {-1.0, -1.0};  // dt a slot was factored for (-1: none / caller matrix)
+    // independent models in one context (tf_set_models): block-diagonal
+    // system, one tree of fronts per model, per-model controller state
+    std::vector<FrontDesc> fd_host;       // front descriptors, offsets of slot 0
+    int nmodels = 0;                      // 0: a single model
+    std::vector<int> m_tet0, m_ntet, m_rec0, m_nrec, m_e0, m_ne;
+    std::vector<int> front_model;
+    FrontDesc *d_fd_cur = nullptr, *d_fd_oth = nullptr;  // fronts in each model's current / other slot
+    double *d_Lmm = nullptr, *d_Dmm = nullptr;          // both slots of every model
+    int *d_qmodel = nullptr, *d_mrange = nullptr;
+    int64_t *d_slot_off = nullptr;
+    double *d_mcdt = nullptr, *d_mcoef = nullptr, *d_mnorm = nullptr, *d_mstage = nullptr;
+    bool use_graphs = true;
+    // rows of the last tf_run_models: all receivers per step, per-model metadata
+    std::vector<int> mm_step;                  // [steps]
+    std::vector<double> mm_t, mm_dt;           // [steps][M]
+    std::vector<char> mm_on;                   // [steps][M]
+    double mm_graph_steps = 0.0;
     // optional per-category kernel timing with HIP events on the context's stream
     bool prof_on = false;
     std::vector<hipEvent_t> ev_pool;
@@ -303,7 +332,10 @@ struct tf_ctx {
 
 namespace {
 
+void free_models(tf_ctx *c);
+
 void free_solver(tf_ctx *c) {
+    free_models(c);
     dfree(c->d_fd); dfree(c->d_perm); dfree(c->d_parent); dfree(c->d_chp); dfree(c->d_ch); dfree(c->d_rel);
     dfree(c->d_sidx); dfree(c->d_plist); dfree(c->d_amap); dfree(c->d_ppref); dfree(c->d_units); dfree(c->d_gptr); dfree(c->d_gidx); dfree(c->d_U); dfree(c->d_V);
     dfree(c->d_xw); dfree(c->d_Y); dfree(c->d_T); dfree(c->d_W); dfree(c->d_woff);
@@ -322,6 +354,12 @@ void free_vecs(tf_ctx *c) {
     c->rows_cap = 0;
 }
 
+void free_models(tf_ctx *c) {
+    dfree(c->d_fd_cur); dfree(c->d_fd_oth); dfree(c->d_Lmm); dfree(c->d_Dmm); dfree(c->d_qmodel);
+    dfree(c->d_mrange); dfree(c->d_slot_off); dfree(c->d_mcdt); dfree(c->d_mcoef); dfree(c->d_mnorm);
+    dfree(c->d_mstage);
+}
+
 void free_tape(tf_ctx *c) {
     for (auto &k : c->ckpts) { dfree(k.s2); dfree(k.s3); }
     c->ckpts.clear();
@@ -331,10 +369,11 @@ void free_tape(tf_ctx *c) {
 }
 
 void free_all(tf_ctx *c) {
+    free_models(c);
     free_solver(c);
     free_vecs(c);
     free_tape(c);
-    dfree(c->d_tet_edg); dfree(c->d_vm1);
+    dfree(c->d_tet_edg); dfree(c->d_vm1); dfree(c->d_sig9); dfree(c->d_nu9);
     dfree(c->d_xyz); dfree(c->d_sigma); dfree(c->d_invmu); dfree(c->d_vm); dfree(c->d_vk);
     dfree(c->d_tet_vex); dfree(c->d_tet_drc); dfree(c->d_ctr); dfree(c->d_orient); dfree(c->d_cptr);
     dfree(c->d_M); dfree(c->d_K); dfree(c->d_fia); dfree(c->d_fja); dfree(c->d_fmap); dfree(c->d_fval);
@@ -541,8 +580,8 @@ void build_plan(tf_ctx *c) {
             const int nc = s.nc[k], m = m_of(k);
             if (mode == 0)
                 for (int r = 0; r < nc; r += 64) t.push_back({k, r, 0, std::min(nc, r + 64)});
-            else if (mode == 1)
-                for (int r = 0; r < m; r += 64) t.push_back({k, r, 0, nc});
+            else if (mode == 1)  // row tiles start on 16-row lines of the column (rows nc + r)
+                for (int r = -(nc & 15); r < m; r += 64) t.push_back({k, r, 0, nc});
             else if (mode == 2)
                 for (int j = 0; j < nc; j += 64) t.push_back({k, j, 0, m});
             else
@@ -631,12 +670,15 @@ void do_analyze(tf_ctx *c, int leaf) {
         fd[k].nc = s.nc[k];
         fd[k].m = m;
         fd[k].f = s.nc[k] + m;
+        fd[k].ld = (int)tf::front_ld(fd[k].f);
+        fd[k].doff = s.c0[k];
         fd[k].c0 = s.c0[k];
         voff += fd[k].f;
     }
     c->vrows = voff;
     build_plan(c);
     c->d_fd = dupload(fd);
+    c->fd_host = fd;
     c->d_perm = dupload(s.perm);
     c->d_parent = dupload(s.parent);
     c->d_chp = dupload(s.children_ptr);
@@ -672,38 +714,38 @@ void do_analyze(tf_ctx *c, int leaf) {
 }
 
 // numeric factorisation of the values already scattered into slot's L
-void run_factor(tf_ctx *c, int slot) {
+// numeric factorisation into L / Dn, fronts addressed through fd
+void run_factor(tf_ctx *c, double *L, double *Dn, const FrontDesc *fd) {
     const Plan &p = c->plan;
-    double *L = c->d_L[slot], *Dn = c->d_D[slot];
     const int *lst = c->d_plist;
     const int64_t *pr = c->d_ppref;
     for (const Launch &l : p.factor) {
         const int pm = prof_mark(c);
         switch (l.kind) {
         case L_EXTADD:
-            tf::launch_extend_add(L, c->d_U, c->d_fd, c->d_parent, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_rel,
+            tf::launch_extend_add(L, c->d_U, fd, c->d_parent, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_rel,
                                   c->st);
             break;
         case L_FFUSED:
-            tf::launch_factor_fused(L, c->d_U, c->d_fd, lst + l.off, l.cnt, Dn, c->d_flag, c->st);
+            tf::launch_factor_fused(L, c->d_U, fd, lst + l.off, l.cnt, Dn, c->d_flag, c->st);
             break;
         case L_FDIAG:
-            tf::launch_factor_diag(L, c->d_fd, lst + l.off, l.cnt, l.p0, Dn, c->d_flag, c->st);
+            tf::launch_factor_diag(L, fd, lst + l.off, l.cnt, l.p0, Dn, c->d_flag, c->st);
             break;
         case L_FTRSM:
-            tf::launch_factor_trsm(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, Dn, c->st);
+            tf::launch_factor_trsm(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, Dn, c->st);
             break;
         case L_FUPD:
-            tf::launch_factor_update(L, c->d_U, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, Dn, c->st);
+            tf::launch_factor_update(L, c->d_U, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, Dn, c->st);
             break;
         case L_IW:
-            tf::launch_inv_w(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, c->d_W, c->d_woff, c->st);
+            tf::launch_inv_w(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, c->d_W, c->d_woff, c->st);
             break;
         case L_IAPPLY:
-            tf::launch_inv_apply(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, c->d_W, c->d_woff, c->st);
+            tf::launch_inv_apply(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, c->d_W, c->d_woff, c->st);
             break;
         case L_IDIAG:
-            tf::launch_inv_diag(L, c->d_fd, lst + l.off, l.cnt, l.p0, c->st);
+            tf::launch_inv_diag(L, fd, lst + l.off, l.cnt, l.p0, c->st);
             break;
         }
         prof_close(c, prof_category(l.kind), pm);
@@ -729,36 +771,35 @@ void check_flag(tf_ctx *c) {
 void factor_dt(tf_ctx *c, int slot, double dt) {
     prepare_slot(c, slot);
     tf::launch_scatter_p(c->topo.nnz, c->d_amap, c->d_M, c->d_K, 2.0 * dt, c->d_L[slot], c->st);
-    run_factor(c, slot);
+    run_factor(c, c->d_L[slot], c->d_D[slot], c->d_fd);
     c->slot_live[slot] = true;
     c->slot_dt[slot] = dt;
 }
 
 // x = P^-1 b (device vectors, edge order)
-void solve_dev(tf_ctx *c, int slot, const double *b, double *x) {
+void solve_dev(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd, const double *b, double *x) {
     const Plan &p = c->plan;
-    const double *L = c->d_L[slot], *Dn = c->d_D[slot];
     const int *lst = c->d_plist;
     const int64_t *pr = c->d_ppref;
     for (const Launch &l : p.fwd) {
         const int pm = prof_mark(c);
         switch (l.kind) {
         case L_SSMALL:
-            tf::launch_fwd_small(L, c->d_fd, lst + l.off, l.cnt, b, c->d_perm, c->d_chp, c->d_ch, c->d_rel, c->d_V,
+            tf::launch_fwd_small(L, fd, lst + l.off, l.cnt, b, c->d_perm, c->d_chp, c->d_ch, c->d_rel, c->d_V,
                                  c->d_Y, c->st);
             break;
         case L_GINIT:
-            tf::launch_mg_init(c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, b, c->d_perm, c->d_V, 1, c->st);
+            tf::launch_mg_init(fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, b, c->d_perm, c->d_V, 1, c->st);
             break;
         case L_GCHILD:
-            tf::launch_mg_child(c->d_fd, c->d_parent, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_rel, c->d_V, 1,
+            tf::launch_mg_child(fd, c->d_parent, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_rel, c->d_V, 1,
                                 c->st);
             break;
         case L_SOWN:
-            tf::launch_fwd_own(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_V, c->d_Y, c->st);
+            tf::launch_fwd_own(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_V, c->d_Y, c->st);
             break;
         case L_SSTRUCT:
-            tf::launch_fwd_struct(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_V, c->d_Y, c->st);
+            tf::launch_fwd_struct(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_V, c->d_Y, c->st);
             break;
         }
         prof_close(c, prof_category(l.kind), pm);
@@ -767,16 +808,16 @@ void solve_dev(tf_ctx *c, int slot, const double *b, double *x) {
         const int pm = prof_mark(c);
         switch (l.kind) {
         case L_BSMALL:
-            tf::launch_bwd_small(L, c->d_fd, lst + l.off, l.cnt, c->d_Y, Dn, c->d_sidx, c->d_xw, c->st);
+            tf::launch_bwd_small(L, fd, lst + l.off, l.cnt, c->d_Y, Dn, c->d_sidx, c->d_xw, c->st);
             break;
         case L_GXS:
-            tf::launch_mg_xs(c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_sidx, c->d_xw, c->d_V, 1, c->st);
+            tf::launch_mg_xs(fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_sidx, c->d_xw, c->d_V, 1, c->st);
             break;
         case L_BT:
-            tf::launch_bwd_t(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_Y, Dn, c->d_V, c->d_T, c->st);
+            tf::launch_bwd_t(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_Y, Dn, c->d_V, c->d_T, c->st);
             break;
         case L_BX:
-            tf::launch_bwd_x(L, c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_T, c->d_xw, c->st);
+            tf::launch_bwd_x(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_T, c->d_xw, c->st);
             break;
         }
         prof_close(c, prof_category(l.kind), pm);
@@ -799,25 +840,25 @@ void ensure_block(tf_ctx *c, int K) {
 }
 
 // X = P^-1 B for K <= 64 right-hand sides, [n][K] device arrays in edge order
-void solve_block(tf_ctx *c, int slot, int K, const double *b, double *x) {
+void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd, int K, const double *b,
+                 double *x) {
     ensure_block(c, K);
     const Plan &p = c->plan;
-    const double *L = c->d_L[slot], *Dn = c->d_D[slot];
     const int *lst = c->d_plist;
     const int64_t *pr = c->d_ppref;
     for (const Launch &l : p.mfwd) {
         const int pm = prof_mark(c);
         switch (l.kind) {
         case L_MPULL:
-            tf::launch_mpull(c->d_fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, b, c->d_perm, c->d_gptr, c->d_gidx,
+            tf::launch_mpull(fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, b, c->d_perm, c->d_gptr, c->d_gidx,
                              c->d_mZ, c->d_mV, K, c->st);
             break;
         case L_MT:  // own forward (Y from Yin), struct forward (Z from Y)
-            tf::launch_mtile(l.p0, l.p0 >= 2 ? c->mt_ks_t : c->mt_ks_n, L, c->d_fd, c->d_units + l.off, l.cnt, K, l.p0 == 0 ? c->d_mV : c->d_mY, nullptr,
+            tf::launch_mtile(l.p0, l.p0 >= 2 ? c->mt_ks_t : c->mt_ks_n, L, fd, c->d_units + l.off, l.cnt, K, l.p0 == 0 ? c->d_mV : c->d_mY, nullptr,
                              Dn, nullptr, l.p0 == 0 ? c->d_mY : c->d_mZ, c->d_part, c->st);
             break;
         case L_MTRED:
-            tf::launch_mtile_red(l.p0, c->d_fd, c->d_units + l.off, l.cnt, K, l.p0 == 0 ? c->d_mV : c->d_mY, nullptr,
+            tf::launch_mtile_red(l.p0, fd, c->d_units + l.off, l.cnt, K, l.p0 == 0 ? c->d_mV : c->d_mY, nullptr,
                                  Dn, l.p0 == 0 ? c->d_mY : c->d_mZ, c->d_part, c->st);
             break;
         }
@@ -827,12 +868,12 @@ void solve_block(tf_ctx *c, int slot, int K, const double *b, double *x) {
         const int pm = prof_mark(c);
         switch (l.kind) {
         case L_MT:  // backward T (from x at the struct rows, Y, D), backward X (from T)
-            tf::launch_mtile(l.p0, l.p0 >= 2 ? c->mt_ks_t : c->mt_ks_n, L, c->d_fd, c->d_units + l.off, l.cnt, K, l.p0 == 2 ? c->d_mxw : c->d_mT,
+            tf::launch_mtile(l.p0, l.p0 >= 2 ? c->mt_ks_t : c->mt_ks_n, L, fd, c->d_units + l.off, l.cnt, K, l.p0 == 2 ? c->d_mxw : c->d_mT,
                              l.p0 == 2 ? c->d_mY : c->d_mT, Dn, c->d_sidx, l.p0 == 2 ? c->d_mT : c->d_mxw, c->d_part,
                              c->st);
             break;
         case L_MTRED:
-            tf::launch_mtile_red(l.p0, c->d_fd, c->d_units + l.off, l.cnt, K, l.p0 == 2 ? c->d_mxw : c->d_mT,
+            tf::launch_mtile_red(l.p0, fd, c->d_units + l.off, l.cnt, K, l.p0 == 2 ? c->d_mxw : c->d_mT,
                                  l.p0 == 2 ? c->d_mY : c->d_mT, Dn, l.p0 == 2 ? c->d_mT : c->d_mxw, c->d_part, c->st);
             break;
         }
@@ -884,8 +925,33 @@ void dcopy(tf_ctx *c, double *dst, const double *src) {
 }
 
 void step_solve(tf_ctx *c, int slot, const double *b, double *x) {
-    if (c->nsrc == 1) solve_dev(c, slot, b, x);
-    else solve_block(c, slot, c->nsrc, b, x);
+    if (c->nsrc == 1) solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, b, x);
+    else solve_block(c, c->d_L[slot], c->d_D[slot], c->d_fd, c->nsrc, b, x);
+}
+
+void grow_rows(tf_ctx *c) {
+    const int64_t per = (int64_t)c->nR * 6 * c->nsrc;
+    if (c->nrows + 1 > c->rows_cap) {
+        int64_t cap = std::max<int64_t>(256, 2 * c->rows_cap);
+        double *nb = dalloc<double>(cap * per);
+        if (c->d_rows && c->nrows > 0)
+            HIPCHK(hipMemcpyAsync(nb, c->d_rows, sizeof(double) * per * c->nrows, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        dfree(c->d_rows);
+        c->d_rows = nb;
+        c->rows_cap = cap;
+    }
+}
+
+// a readout staged by a replayed step graph becomes the next row
+void output_row_from(tf_ctx *c, const double *stage) {
+    const int64_t per = (int64_t)c->nR * 6 * c->nsrc;
+    grow_rows(c);
+    HIPCHK(hipMemcpyAsync(c->d_rows + per * c->nrows, stage, sizeof(double) * per, hipMemcpyDeviceToDevice, c->st));
+    c->row_step.push_back(0);
+    c->row_t.push_back(0.0);
+    c->row_dt.push_back(0.0);
+    c->nrows++;
 }
 
 void output_row(tf_ctx *c, int step, double t, double dt, const double *e) {
@@ -946,7 +1012,6 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
     const int ring = dbl + 1;
     int cur = 0;
     for (double *p : {v.e1, v.e2, v.e3}) HIPCHK(hipMemsetAsync(p, 0, sizeof(double) * n * c->nsrc, c->st));
-    int ncheck = 0;
     // adjoint tape: node ids of the history states e2, e3 and of the ring
     const bool tape = c->ckpt_every > 0 && inout == 0;
     int id2 = 0, id3 = 0;
@@ -1027,11 +1092,11 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
             // keep the step iff err > 1e-4 and fewer than 2 kept in a row
             // (src/FETD_FWD.f90:1381); a replayed schedule forces the decision
             bool keep = (error > 1.0e-4 && not_changed < 2);
-            if (!c->replay.empty()) {
-                if (ncheck >= (int)c->replay.size()) throw std::runtime_error("replayed schedule is too short");
-                keep = c->replay[ncheck] != 0;
+            if (!c->replay.empty()) {  // indexed over all checks of the run (both passes of a step-off)
+                const size_t ic = c->chk_step.size();
+                if (ic >= c->replay.size()) throw std::runtime_error("replayed schedule is too short");
+                keep = c->replay[ic] != 0;
             }
-            ncheck++;
             c->chk_step.push_back(step);
             c->chk_err.push_back(error);
             c->chk_keep.push_back(keep ? 1 : 0);
@@ -1092,6 +1157,248 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
 // C ABI
 // ======================================================================
 extern "C" {
+
+// ---- independent models in one context ------------------------------------------
+// Device set-up after the symbolic analysis: model of every front and CSR
+// entry, both factor slots of every model in one buffer, the two front views.
+void setup_models(tf_ctx *c) {
+    const Symbolic &s = c->sym;
+    const int M = c->nmodels, n = s.n;
+    auto model_of_edge = [&](int e) {
+        int m = (int)(std::upper_bound(c->m_e0.begin(), c->m_e0.end(), e) - c->m_e0.begin()) - 1;
+        return m;
+    };
+    c->front_model.assign(s.nsn, 0);
+    for (int k = 0; k < s.nsn; k++) {
+        const int m = model_of_edge(s.perm[s.c0[k]]);
+        for (int j = 0; j < s.nc[k]; j++)
+            if (model_of_edge(s.perm[s.c0[k] + j]) != m) throw std::runtime_error("a front spans two models");
+        c->front_model[k] = m;
+    }
+    std::vector<int> qm(c->topo.nnz);
+    for (int r = 0; r < n; r++) {
+        const int m = model_of_edge(r);
+        for (int q = c->topo.ia[r]; q < c->topo.ia[r + 1]; q++) qm[q] = m;
+    }
+    std::vector<int> rg(2 * (size_t)M);
+    for (int m = 0; m < M; m++) { rg[2 * m] = c->m_e0[m]; rg[2 * m + 1] = c->m_e0[m] + c->m_ne[m]; }
+    free_models(c);
+    c->d_qmodel = dupload(qm);
+    c->d_mrange = dupload(rg);
+    c->d_Lmm = dalloc<double>(2 * (size_t)s.ltotal);
+    c->d_Dmm = dalloc<double>(2 * (size_t)n);
+    c->d_fd_cur = dalloc<FrontDesc>(s.nsn);
+    c->d_fd_oth = dalloc<FrontDesc>(s.nsn);
+    c->d_slot_off = dalloc<int64_t>(M);
+    c->d_mcdt = dalloc<double>(M);
+    c->d_mcoef = dalloc<double>(M);
+    c->d_mnorm = dalloc<double>(2 * (size_t)M);
+    c->d_mstage = dalloc<double>(6 * (size_t)c->nR);
+}
+
+// front views: every model's fronts in its slot cur[m] (which = 0) or the other one (which = 1)
+void upload_views(tf_ctx *c, const std::vector<int> &cur) {
+    const Symbolic &s = c->sym;
+    std::vector<FrontDesc> a(c->fd_host), b(c->fd_host);
+    for (int k = 0; k < s.nsn; k++) {
+        const int sa = cur[c->front_model[k]], sb = 1 - sa;
+        a[k].loff += (int64_t)sa * s.ltotal;
+        a[k].doff += sa * s.n;
+        b[k].loff += (int64_t)sb * s.ltotal;
+        b[k].doff += sb * s.n;
+    }
+    HIPCHK(hipMemcpyAsync(c->d_fd_cur, a.data(), sizeof(FrontDesc) * a.size(), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_fd_oth, b.data(), sizeof(FrontDesc) * b.size(), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+}
+
+// factorise every model at its own dt into the slot `target[m]`
+void factor_models(tf_ctx *c, const std::vector<double> &dt, const std::vector<int> &target, const FrontDesc *fd) {
+    const int M = c->nmodels;
+    std::vector<double> cdt(M);
+    std::vector<int64_t> off(M);
+    for (int m = 0; m < M; m++) { cdt[m] = 2.0 * dt[m]; off[m] = (int64_t)target[m] * c->sym.ltotal; }
+    HIPCHK(hipMemcpyAsync(c->d_mcdt, cdt.data(), sizeof(double) * M, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_slot_off, off.data(), sizeof(int64_t) * M, hipMemcpyHostToDevice, c->st));
+    tf::launch_fill0(c->d_Lmm, 2 * c->sym.ltotal, c->st);
+    tf::launch_fill0(c->d_U, c->sym.utotal, c->st);
+    HIPCHK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->st));
+    tf::launch_scatter_p_models(c->topo.nnz, c->d_amap, c->d_qmodel, c->d_M, c->d_K, c->d_mcdt, c->d_slot_off,
+                                c->d_Lmm, c->st);
+    run_factor(c, c->d_Lmm, c->d_Dmm, fd);
+    check_flag(c);
+    c->timers[7] += 1.0;
+}
+
+// right-hand side for every model: Msigma (4 e2 - e3) minus each model's
+// source at its own time and step (sys_right, src/FETD_FWD.f90:1466-1511)
+void rhs_models(tf_ctx *c, int step, const std::vector<double> &t, const std::vector<double> &dt, const double *e2,
+                const double *e3, double *r) {
+    const int M = c->nmodels;
+    tf::launch_spmv_rhs(c->topo.nE, c->d_fia, c->d_fja, c->d_fval, e2, e3, r, c->st);
+    if (step < c->sp_division) {
+        std::vector<double> coef(M);
+        for (int m = 0; m < M; m++)
+            coef[m] = 2.0 * dt[m] * source_didt(c->stype, c->ptype, t[m], c->pulse_width, c->current);
+        HIPCHK(hipMemcpyAsync(c->d_mcoef, coef.data(), sizeof(double) * M, hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));  // coef lives on the host stack
+        tf::launch_source_models(c->nseg, c->d_src_edge, c->d_src_id, c->d_src_w, c->d_mcoef, r, c->st);
+    }
+}
+
+// The BDF2 loop with the doubling controller of adap_euler
+// (src/FETD_FWD.f90:1229-1464) run for every model at once: one block-
+// diagonal solve per step, each model with its own time, dt, keep/change
+// decisions and factor slot.  Regular steps with the sources off replay a
+// captured hipGraph (SpMV, level-scheduled solve, readout).
+void run_models(tf_ctx *c, int max_steps) {
+    const int M = c->nmodels, n = c->topo.nE, R = c->nR;
+    StepVecs v;
+    ensure_vecs(c, v);
+    const int dbl = c->sp_dblesize, ring = dbl + 1;
+    if (c->sp_double < dbl + 2) throw std::runtime_error("sp_double must be at least sp_dblesize+2");
+    std::vector<double> tcur(M, 0.0), dt(M, c->pulse_width / c->sp_division);
+    std::vector<int> cur(M, 0), not_changed(M, 0);
+    std::vector<char> on(M, 1);
+    for (double *p : {v.e1, v.e2, v.e3}) HIPCHK(hipMemsetAsync(p, 0, sizeof(double) * n, c->st));
+    upload_views(c, cur);
+    c->mm_step.clear(); c->mm_t.clear(); c->mm_dt.clear(); c->mm_on.clear();
+    c->nrows = 0;
+    c->mm_graph_steps = 0.0;
+    std::map<std::array<const void *, 4>, hipGraphExec_t> graphs;
+    auto record = [&](int step, const std::vector<double> &tt, const double *e) {
+        // one row per step: every receiver, and per model its time, dt and whether it is still running
+        output_row(c, step, 0.0, 0.0, e);
+        c->mm_step.push_back(step);
+        c->mm_t.insert(c->mm_t.end(), tt.begin(), tt.end());
+        c->mm_dt.insert(c->mm_dt.end(), dt.begin(), dt.end());
+        c->mm_on.insert(c->mm_on.end(), on.begin(), on.end());
+    };
+    hipEvent_t f0, f1;
+    HIPCHK(hipEventCreate(&f0));
+    HIPCHK(hipEventCreate(&f1));
+    auto timed = [&](auto &&fn) {
+        HIPCHK(hipEventRecord(f0, c->st));
+        fn();
+        HIPCHK(hipEventRecord(f1, c->st));
+        HIPCHK(hipEventSynchronize(f1));
+        c->timers[1] += elapsed(f0, f1);
+    };
+    int step = 0, idx = 0;
+    std::vector<double> t0v(M, 0.0);
+    while (true) {
+        bool any = false;
+        for (int m = 0; m < M; m++) {
+            on[m] = on[m] && tcur[m] < c->time_max;
+            any = any || on[m];
+        }
+        if (!any || (max_steps > 0 && step >= max_steps)) break;
+        step++;
+        for (int m = 0; m < M; m++) tcur[m] = tcur[m] + dt[m];
+        if (step == 1) {
+            record(0, t0v, v.e2);
+            timed([&] { factor_models(c, dt, cur, c->d_fd_cur); });
+        }
+        if (step % c->sp_double == 0) {
+            dcopy(c, v.te2, v.e2);
+            dcopy(c, v.te3, v.e3);
+            std::vector<double> tt(tcur);
+            for (int i = 1; i <= dbl; i++) {
+                rhs_models(c, step, tt, dt, v.te2, v.te3, v.rh);
+                solve_dev(c, c->d_Lmm, c->d_Dmm, c->d_fd_cur, v.rh, v.te1);
+                if (i == dbl) break;
+                if (i == 1) dcopy(c, v.erec, v.te1);
+                std::swap(v.te3, v.te2);
+                dcopy(c, v.te2, v.te1);
+                for (int m = 0; m < M; m++) tt[m] = tt[m] + dt[m];
+            }
+            dcopy(c, v.ev1, v.te1);
+            std::vector<double> tstep(M), ttime(M);
+            std::vector<int> other(M);
+            for (int m = 0; m < M; m++) {
+                tstep[m] = dt[m] * dbl;
+                ttime[m] = tcur[m] - dt[m] + tstep[m];
+                other[m] = 1 - cur[m];
+            }
+            timed([&] { factor_models(c, tstep, other, c->d_fd_oth); });
+            double *ea = v.ring[(idx - 1) % ring];
+            double *eb = v.ring[(idx - 1 - dbl) % ring];
+            rhs_models(c, step, ttime, tstep, ea, eb, v.rh);
+            solve_dev(c, c->d_Lmm, c->d_Dmm, c->d_fd_oth, v.rh, v.ev2);
+            tf::launch_range_norms(M, c->d_mrange, v.ev2, v.ev1, c->d_mnorm, c->st);
+            std::vector<double> nrm(2 * (size_t)M);
+            HIPCHK(hipMemcpyAsync(nrm.data(), c->d_mnorm, sizeof(double) * nrm.size(), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+            // e3 <- e2 for every model; e2 <- first fine step (keep) or the
+            // two fine steps' end state (change), model by model
+            std::swap(v.e3, v.e2);
+            dcopy(c, v.e2, v.erec);
+            bool flip = false;
+            for (int m = 0; m < M; m++) {
+                const double err = std::sqrt(nrm[2 * m]) / std::sqrt(nrm[2 * m + 1]);
+                const bool keep = !on[m] || (err > 1.0e-4 && not_changed[m] < 2);
+                if (keep) {
+                    not_changed[m]++;
+                } else {
+                    HIPCHK(hipMemcpyAsync(v.e2 + c->m_e0[m], v.ev1 + c->m_e0[m], sizeof(double) * c->m_ne[m],
+                                          hipMemcpyDeviceToDevice, c->st));
+                    cur[m] = other[m];
+                    tcur[m] = ttime[m];
+                    dt[m] = tstep[m];
+                    not_changed[m] = 0;
+                    flip = true;
+                }
+            }
+            if (flip) upload_views(c, cur);
+            record(step, tcur, v.e2);
+            idx = 0;
+            continue;
+        }
+        if (step >= c->sp_division && c->use_graphs) {
+            // regular step, sources off: replay the captured step for this
+            // arrangement of the history vectors
+            const std::array<const void *, 4> key = {v.e1, v.e2, v.e3, v.ring[idx % ring]};
+            auto it = graphs.find(key);
+            if (it == graphs.end()) {
+                hipGraph_t g;
+                HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+                tf::launch_spmv_rhs(n, c->d_fia, c->d_fja, c->d_fval, v.e2, v.e3, v.rh, c->st);
+                solve_dev(c, c->d_Lmm, c->d_Dmm, c->d_fd_cur, v.rh, v.e1);
+                tf::launch_readout(R, c->d_rec_edge, c->d_rec_drc, c->d_narr, c->d_darr, v.e1, c->d_mstage, c->st);
+                HIPCHK(hipMemcpyAsync(v.ring[idx % ring], v.e1, sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
+                HIPCHK(hipStreamEndCapture(c->st, &g));
+                hipGraphExec_t ex;
+                HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+                HIPCHK(hipGraphDestroy(g));
+                it = graphs.emplace(key, ex).first;
+            }
+            HIPCHK(hipGraphLaunch(it->second, c->st));
+            // the row from the staging buffer
+            output_row_from(c, c->d_mstage);
+            c->mm_step.push_back(step);
+            c->mm_t.insert(c->mm_t.end(), tcur.begin(), tcur.end());
+            c->mm_dt.insert(c->mm_dt.end(), dt.begin(), dt.end());
+            c->mm_on.insert(c->mm_on.end(), on.begin(), on.end());
+            c->mm_graph_steps += 1.0;
+        } else {
+            rhs_models(c, step, tcur, dt, v.e2, v.e3, v.rh);
+            solve_dev(c, c->d_Lmm, c->d_Dmm, c->d_fd_cur, v.rh, v.e1);
+            record(step, tcur, v.e1);
+            dcopy(c, v.ring[idx % ring], v.e1);
+        }
+        idx++;
+        double *t3 = v.e3;
+        v.e3 = v.e2;
+        v.e2 = v.e1;
+        v.e1 = t3;
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipGetLastError());
+    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    HIPCHK(hipEventDestroy(f0));
+    HIPCHK(hipEventDestroy(f1));
+    c->vecs[0] = v.e1; c->vecs[1] = v.e2; c->vecs[2] = v.e3; c->vecs[5] = v.te2; c->vecs[6] = v.te3;
+}
 
 // ---- adjoint ------------------------------------------------------------------
 // Backward-in-time adjoint over the tape of the last tf_run.  Segment by
@@ -1362,6 +1669,30 @@ int tf_set_materials(tf_ctx *c, const double *sigma, const double *inv_mu) {
         c->d_invmu = dupload(c->inv_mu);
         c->have_mat = true;
         c->assembled = false;
+        c->aniso = false;
+        return TF_OK;
+    });
+}
+
+int tf_set_materials_tensor(tf_ctx *c, const double *sigma9, const double *nu9) {
+    return guarded(c, [&] {
+        c->tape_valid = false;
+        if (!c->have_mesh) return fail(c, TF_ERR_STATE, "tf_set_mesh first");
+        if (!sigma9 || !nu9) return fail(c, TF_ERR_ARG, "null material array");
+        const int T = c->topo.nT;
+        std::vector<double> s9(sigma9, sigma9 + 9 * (size_t)T), n9(nu9, nu9 + 9 * (size_t)T);
+        // scalar stand-ins (first diagonal entries) for what reads sigma per tet
+        c->sigma.resize(T);
+        c->inv_mu.resize(T);
+        for (int i = 0; i < T; i++) { c->sigma[i] = s9[9 * (size_t)i]; c->inv_mu[i] = n9[9 * (size_t)i]; }
+        dfree(c->d_sig9); dfree(c->d_nu9); dfree(c->d_sigma); dfree(c->d_invmu);
+        c->d_sig9 = dupload(s9);
+        c->d_nu9 = dupload(n9);
+        c->d_sigma = dupload(c->sigma);
+        c->d_invmu = dupload(c->inv_mu);
+        c->have_mat = true;
+        c->assembled = false;
+        c->aniso = true;
         return TF_OK;
     });
 }
@@ -1382,11 +1713,12 @@ int tf_set_source(tf_ctx *c, int isrc, int nseg, const int *seg_vex, int source_
     return guarded(c, [&] {
         c->tape_valid = false;
         if (!c->have_mesh) return fail(c, TF_ERR_STATE, "tf_set_mesh first");
-        if ((int)c->src_edges.size() != c->nsrc) {
-            c->src_edges.assign(c->nsrc, {});
-            c->src_ws.assign(c->nsrc, {});
+        const int nslots = c->nmodels > 0 ? c->nmodels : c->nsrc;  // one wire per model, or per source
+        if ((int)c->src_edges.size() != nslots) {
+            c->src_edges.assign(nslots, {});
+            c->src_ws.assign(nslots, {});
         }
-        if (isrc < 0 || isrc >= c->nsrc) return fail(c, TF_ERR_ARG, "source index out of range");
+        if (isrc < 0 || isrc >= nslots) return fail(c, TF_ERR_ARG, "source index out of range");
         if (nseg <= 0 || !seg_vex) return fail(c, TF_ERR_ARG, "no source segments");
         if (source_type != 1 || pulse_type < 1 || pulse_type > 3)
             return fail(c, TF_ERR_INPUT, "no such kind of source");
@@ -1415,7 +1747,7 @@ int tf_set_source(tf_ctx *c, int isrc, int nseg, const int *seg_vex, int source_
         c->have_src = all;
         if (all) {
             c->src_edge.clear(); c->src_id.clear(); c->src_w.clear();
-            for (int q = 0; q < c->nsrc; q++)
+            for (int q = 0; q < (int)c->src_edges.size(); q++)
                 for (size_t i = 0; i < c->src_edges[q].size(); i++) {
                     c->src_edge.push_back(c->src_edges[q][i]);
                     c->src_id.push_back(q);
@@ -1447,8 +1779,16 @@ int tf_set_receivers(tf_ctx *c, int nR, const double *xyz, double rec_charac, in
         c->rec_xyz.assign(xyz, xyz + 3 * (size_t)nR);
         c->rec_charac = rec_charac;
         c->rec_tet.assign(nR, 0);
+        if (c->nmodels > 0 && c->m_rec0.back() + c->m_nrec.back() != nR)
+            return fail(c, TF_ERR_ARG, "receiver count differs from the models' total");
         for (int i = 0; i < nR; i++) {
-            c->rec_tet[i] = tf::point_in_tet(c->topo, &xyz[3 * i], rec_charac);
+            int t0 = 0, t1 = -1;
+            if (c->nmodels > 0) {  // a receiver lies in its own model's tets
+                const int m = (int)(std::upper_bound(c->m_rec0.begin(), c->m_rec0.end(), i) - c->m_rec0.begin()) - 1;
+                t0 = c->m_tet0[m];
+                t1 = c->m_tet0[m] + c->m_ntet[m];
+            }
+            c->rec_tet[i] = tf::point_in_tet(c->topo, &xyz[3 * i], rec_charac, t0, t1);
             if (c->rec_tet[i] == 0) return fail(c, TF_ERR_INPUT, "no rec tet No. " + std::to_string(i + 1));
         }
         if (rec_tet_out) std::copy(c->rec_tet.begin(), c->rec_tet.end(), rec_tet_out);
@@ -1493,8 +1833,12 @@ int tf_assemble(tf_ctx *c) {
         if (!c->have_mesh || !c->have_mat) return fail(c, TF_ERR_STATE, "mesh and materials first");
         auto t0 = std::chrono::steady_clock::now();
         const Topology &t = c->topo;
-        tf::launch_element(t.nT, c->d_xyz, c->d_tet_vex, c->d_tet_drc, c->d_orient, c->d_sigma, c->d_invmu, c->d_vm,
-                           c->d_vk, c->st);
+        if (c->aniso)
+            tf::launch_element_ani(t.nT, c->d_xyz, c->d_tet_vex, c->d_tet_drc, c->d_orient, c->d_sig9, c->d_nu9,
+                                   c->d_vm, c->d_vk, c->st);
+        else
+            tf::launch_element(t.nT, c->d_xyz, c->d_tet_vex, c->d_tet_drc, c->d_orient, c->d_sigma, c->d_invmu,
+                               c->d_vm, c->d_vk, c->st);
         tf::launch_gather(t.nnz, c->d_cptr, c->d_ctr, c->d_vm, c->d_vk, c->d_M, c->d_K, c->st);
         tf::launch_gather_vals((int64_t)t.ia.size() > 0 ? (int64_t)(2 * t.nnz - t.nE) : 0, c->d_fmap, c->d_M,
                                c->d_fval, c->st);
@@ -1574,6 +1918,7 @@ int tf_set_checkpointing(tf_ctx *c, int every) {
 int tf_adjoint(tf_ctx *c, int nrows, const double *w, double *grad_logsigma) {
     return guarded(c, [&] {
         if (c->ptype == 3) return fail(c, TF_ERR_STATE, "the adjoint covers step-on and impulse sources (pulse types 1, 2)");
+        if (c->aniso) return fail(c, TF_ERR_STATE, "the adjoint is per-tet scalar log-conductivity (isotropic media)");
         if (!c->tape_valid) return fail(c, TF_ERR_STATE, "no taped run: tf_set_checkpointing(ctx, k > 0) then tf_run");
         if (nrows != c->nrows) return fail(c, TF_ERR_ARG, "weights must cover every row of the taped run");
         if (!w || !grad_logsigma) return fail(c, TF_ERR_ARG, "null buffer");
@@ -1638,7 +1983,7 @@ int tf_factor_csr(tf_ctx *c, int slot, int n, const int *ia, const int *ja, cons
         double *dv = dalloc<double>(t.nnz);
         HIPCHK(hipMemcpyAsync(dv, va, sizeof(double) * t.nnz, hipMemcpyHostToDevice, c->st));
         tf::launch_scatter_vals(t.nnz, c->d_amap, dv, c->d_L[slot], c->st);
-        run_factor(c, slot);
+        run_factor(c, c->d_L[slot], c->d_D[slot], c->d_fd);
         c->slot_live[slot] = true;
         c->slot_dt[slot] = -1.0;
         HIPCHK(hipStreamSynchronize(c->st));
@@ -1655,7 +2000,7 @@ int tf_solve(tf_ctx *c, int slot, int nrhs, const double *b, double *x) {
         const int n = c->topo.nE;
         if (nrhs == 1) {
             HIPCHK(hipMemcpyAsync(c->d_tmpb, b, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
-            solve_dev(c, slot, c->d_tmpb, c->d_tmpx);
+            solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, c->d_tmpb, c->d_tmpx);
             HIPCHK(hipMemcpyAsync(x, c->d_tmpx, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
         } else {
             // blocks of up to 64 right-hand sides through the block solve
@@ -1666,7 +2011,7 @@ int tf_solve(tf_ctx *c, int slot, int nrhs, const double *b, double *x) {
                 for (int i = 0; i < n; i++)
                     for (int q = 0; q < K; q++) h[(size_t)i * K + q] = b[(size_t)(q0 + q) * n + i];
                 HIPCHK(hipMemcpyAsync(c->d_mb, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->st));
-                solve_block(c, slot, K, c->d_mb, c->d_mx);
+                solve_block(c, c->d_L[slot], c->d_D[slot], c->d_fd, K, c->d_mb, c->d_mx);
                 HIPCHK(hipMemcpyAsync(h.data(), c->d_mx, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
                 HIPCHK(hipStreamSynchronize(c->st));
                 for (int i = 0; i < n; i++)
@@ -1689,8 +2034,8 @@ int tf_time_solve(tf_ctx *c, int slot, int nrhs, int reps, double *ms_per_solve)
         else { ensure_block(c, nrhs); b = c->d_mb; x = c->d_mx; }
         tf::launch_fill_pattern((int64_t)n * nrhs, b, c->st);
         auto one = [&] {
-            if (nrhs == 1) solve_dev(c, slot, b, x);
-            else solve_block(c, slot, nrhs, b, x);
+            if (nrhs == 1) solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, b, x);
+            else solve_block(c, c->d_L[slot], c->d_D[slot], c->d_fd, nrhs, b, x);
         };
         one();  // warm-up
         hipEvent_t e0, e1;
@@ -1716,9 +2061,119 @@ int tf_release(tf_ctx *c, int slot) {
     return TF_OK;
 }
 
+int tf_set_models(tf_ctx *c, int nmodels, const int *ntet, const int *nrec) {
+    return guarded(c, [&] {
+        c->tape_valid = false;
+        if (!c->have_mesh) return fail(c, TF_ERR_STATE, "tf_set_mesh first");
+        if (nmodels < 1 || !ntet || !nrec) return fail(c, TF_ERR_ARG, "bad model counts");
+        const Topology &t = c->topo;
+        std::vector<int> tet0(nmodels), rec0(nmodels), vlo(nmodels, INT32_MAX), vhi(nmodels, -1);
+        int64_t ts = 0, rs = 0;
+        for (int m = 0; m < nmodels; m++) {
+            if (ntet[m] <= 0 || nrec[m] < 0) return fail(c, TF_ERR_ARG, "bad model counts");
+            tet0[m] = (int)ts;
+            rec0[m] = (int)rs;
+            ts += ntet[m];
+            rs += nrec[m];
+        }
+        if (ts != t.nT) return fail(c, TF_ERR_ARG, "model tet counts do not add up to the mesh");
+        for (int m = 0; m < nmodels; m++)
+            for (int i = tet0[m]; i < tet0[m] + ntet[m]; i++)
+                for (int q = 0; q < 4; q++) {
+                    const int vx = t.tet_vex[4 * (size_t)i + q];
+                    vlo[m] = std::min(vlo[m], vx);
+                    vhi[m] = std::max(vhi[m], vx);
+                }
+        for (int m = 1; m < nmodels; m++)
+            if (vlo[m] <= vhi[m - 1]) return fail(c, TF_ERR_INPUT, "models must use disjoint, increasing vertex blocks");
+        // edges are numbered by (min, max) vertex: each model's edges form one range
+        std::vector<int> e0(nmodels, -1), ne(nmodels, 0);
+        for (int e = 0; e < t.nE; e++) {
+            const int v1 = t.edg_vex[2 * (size_t)e];
+            int m = 0;
+            while (m + 1 < nmodels && v1 >= vlo[m + 1]) m++;
+            if (e0[m] < 0) e0[m] = e;
+            ne[m]++;
+        }
+        for (int m = 0; m < nmodels; m++)
+            if (e0[m] < 0) return fail(c, TF_ERR_INPUT, "a model without edges");
+        c->nmodels = nmodels;
+        c->m_tet0 = tet0;
+        c->m_ntet.assign(ntet, ntet + nmodels);
+        c->m_rec0 = rec0;
+        c->m_nrec.assign(nrec, nrec + nmodels);
+        c->m_e0 = e0;
+        c->m_ne = ne;
+        c->nsrc = 1;
+        c->src_edges.assign(nmodels, {});
+        c->src_ws.assign(nmodels, {});
+        c->have_src = false;
+        return TF_OK;
+    });
+}
+
+int tf_run_models(tf_ctx *c, int max_steps, int use_graphs) {
+    return guarded(c, [&] {
+        if (c->nmodels <= 0) return fail(c, TF_ERR_STATE, "tf_set_models first");
+        if (!c->assembled) return fail(c, TF_ERR_STATE, "tf_assemble first");
+        if (!c->have_src) return fail(c, TF_ERR_STATE, "tf_set_source for every model first");
+        if (c->nR == 0) return fail(c, TF_ERR_STATE, "tf_set_receivers first");
+        if (c->sp_division == 0) return fail(c, TF_ERR_STATE, "tf_set_schedule first");
+        if (c->ptype == 3) return fail(c, TF_ERR_STATE, "independent models run step-on and impulse sources");
+        if (!c->analyzed) do_analyze(c, c->leaf_size);
+        if (!c->d_Lmm) setup_models(c);
+        for (int k = 1; k < 8; k++) c->timers[k] = 0.0;
+        c->row_step.clear(); c->row_t.clear(); c->row_dt.clear();
+        c->use_graphs = use_graphs != 0;
+        auto t0 = std::chrono::steady_clock::now();
+        run_models(c, max_steps);
+        c->timers[5] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return TF_OK;
+    });
+}
+
+int tf_get_model_rows(tf_ctx *c, int model, int max_rows, int *nrows, int *row_step, double *row_t, double *row_dt,
+                      double *fields) {
+    return guarded(c, [&] {
+        if (c->nmodels <= 0 || model < 0 || model >= c->nmodels) return fail(c, TF_ERR_ARG, "no such model");
+        const int M = c->nmodels, R = c->nR, r0 = c->m_rec0[model], nr = c->m_nrec[model];
+        const int64_t per = (int64_t)R * 6;
+        std::vector<double> h(per * c->nrows);
+        if (c->nrows > 0)
+            HIPCHK(hipMemcpy(h.data(), c->d_rows, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+        int k = 0;
+        std::vector<double> bsum(3 * (size_t)nr, 0.0);
+        for (int r = 0; r < c->nrows; r++) {
+            if (!c->mm_on[(size_t)r * M + model]) continue;
+            const double dtm = c->mm_dt[(size_t)r * M + model];
+            if (k < max_rows) {
+                if (row_step) row_step[k] = c->mm_step[r];
+                if (row_t) row_t[k] = c->mm_t[(size_t)r * M + model];
+                if (row_dt) row_dt[k] = dtm;
+                for (int i = 0; i < nr; i++)
+                    for (int q = 0; q < 3; q++) {
+                        const double *src = &h[(size_t)r * per + (size_t)(r0 + i) * 6];
+                        const double d = (c->mm_step[r] == 0) ? 0.0 : src[3 + q];
+                        bsum[3 * i + q] += d * dtm;
+                        if (fields) {
+                            double *o = fields + ((int64_t)k * nr + i) * 9;
+                            o[q] = src[q];
+                            o[3 + q] = d;
+                            o[6 + q] = bsum[3 * i + q];
+                        }
+                    }
+            }
+            k++;
+        }
+        if (nrows) *nrows = k;
+        return TF_OK;
+    });
+}
+
 int tf_run(tf_ctx *c, int max_steps, int max_rows, int *nrows, int *row_step, double *row_t, double *row_dt,
            double *fields) {
     return guarded(c, [&] {
+        if (c->nmodels > 0) return fail(c, TF_ERR_STATE, "independent models run with tf_run_models");
         if (!c->assembled) return fail(c, TF_ERR_STATE, "tf_assemble first");
         if (!c->have_src) return fail(c, TF_ERR_STATE, "tf_set_source first");
         if (c->nR == 0) return fail(c, TF_ERR_STATE, "tf_set_receivers first");

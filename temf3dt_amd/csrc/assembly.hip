@@ -122,6 +122,54 @@ __global__ void k_element(int nT, const double *xyz, const int *tet_vex, const i
     }
 }
 
+// Anisotropic media (f_kind_ani src/FETD_FWD.f90:884-912, e_kind_ani
+// :853-882): per tet a conductivity tensor As (between the gradients of the
+// Whitney mass) and a curl weight tensor Ak (between the curls), row-major
+// [nT][9].  Same 21 oriented, drc-signed entries as k_element.
+__global__ void k_element_ani(int nT, const double *xyz, const int *tet_vex, const int *tet_drc,
+                              const uint8_t *orient, const double *As, const double *Ak, double *vm, double *vk) {
+    int it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= nT) return;
+    double x[4], y[4], z[4], a[4], b[4], c[4], d[4], vol, len[6], g[6][3], S[9], N[9];
+    load_tet(xyz, tet_vex + 4 * (int64_t)it, x, y, z);
+    dev_cofactors(x, y, z, a, b, c, d, &vol);
+    for (int j = 0; j < 6; j++) len[j] = dev_len(x, y, z, j);
+    for (int q = 0; q < 9; q++) { S[q] = As[9 * (int64_t)it + q]; N[q] = Ak[9 * (int64_t)it + q]; }
+    const double v6 = 6.0 * vol;
+    for (int i = 0; i < 6; i++) {
+        int i1 = c_ind_tet[i][0], i2 = c_ind_tet[i][1];
+        const double sc = 2.0 * len[i] / (v6 * v6);
+        g[i][0] = (c[i1] * d[i2] - c[i2] * d[i1]) * sc;
+        g[i][1] = (b[i2] * d[i1] - b[i1] * d[i2]) * sc;
+        g[i][2] = (b[i1] * c[i2] - b[i2] * c[i1]) * sc;
+    }
+    const int *drc = tet_drc + 6 * (int64_t)it;
+    for (int p = 0; p < 21; p++) {
+        const int o = orient[21 * (int64_t)it + p];
+        const int ia = o & 15, ib = o >> 4;
+        const int im[2] = {c_ind_tet[ia][0], c_ind_tet[ia][1]};
+        const int jn[2] = {c_ind_tet[ib][0], c_ind_tet[ib][1]};
+        double f[2][2];
+        for (int mm = 0; mm < 2; mm++)
+            for (int nn = 0; nn < 2; nn++) {
+                const double v1[3] = {b[im[mm]], c[im[mm]], d[im[mm]]};
+                const double v2[3] = {b[jn[nn]], c[jn[nn]], d[jn[nn]]};
+                double w[3];
+                for (int r = 0; r < 3; r++) w[r] = S[3 * r] * v2[0] + S[3 * r + 1] * v2[1] + S[3 * r + 2] * v2[2];
+                f[mm][nn] = v1[0] * w[0] + v1[1] * w[1] + v1[2] * w[2];
+            }
+        double v = f[1][1] * (double)(1 + (im[0] == jn[0])) - f[0][1] * (double)(1 + (im[1] == jn[0])) -
+                   f[1][0] * (double)(1 + (im[0] == jn[1])) + f[0][0] * (double)(1 + (im[1] == jn[1]));
+        double F = len[ia] * len[ib] / (vol * 720.0) * v;
+        double w[3];
+        for (int r = 0; r < 3; r++) w[r] = N[3 * r] * g[ib][0] + N[3 * r + 1] * g[ib][1] + N[3 * r + 2] * g[ib][2];
+        double K = vol * (g[ia][0] * w[0] + g[ia][1] * w[1] + g[ia][2] * w[2]);
+        double sgn = (double)(drc[ia] * drc[ib]);
+        vm[21 * (int64_t)it + p] = F * sgn;
+        vk[21 * (int64_t)it + p] = K * sgn;
+    }
+}
+
 // CSR entry q = sum of its element contributions in COO (tet) order.
 __global__ void k_gather(int64_t nnz, const int64_t *cptr, const int *ctr, const double *vm, const double *vk,
                          double *M, double *K) {
@@ -168,6 +216,11 @@ void launch_element(int nT, const double *xyz, const int *tet_vex, const int *te
                     const double *sigma, const double *inv_mu, double *vm, double *vk, hipStream_t st) {
     hipLaunchKernelGGL(k_element, dim3((nT + 127) / 128), dim3(128), 0, st, nT, xyz, tet_vex, tet_drc, orient,
                        sigma, inv_mu, vm, vk);
+}
+void launch_element_ani(int nT, const double *xyz, const int *tet_vex, const int *tet_drc, const uint8_t *orient,
+                        const double *As, const double *Ak, double *vm, double *vk, hipStream_t st) {
+    hipLaunchKernelGGL(k_element_ani, dim3((nT + 127) / 128), dim3(128), 0, st, nT, xyz, tet_vex, tet_drc, orient,
+                       As, Ak, vm, vk);
 }
 void launch_gather(int64_t nnz, const int64_t *cptr, const int *ctr, const double *vm, const double *vk,
                    double *M, double *K, hipStream_t st) {

@@ -25,7 +25,7 @@ __device__ void blk_diag_factor(double *L, const FrontDesc &d, int p0, double *D
     double *base = L + d.loff;
     for (int idx = threadIdx.x; idx < pw * pw; idx += blockDim.x) {
         int i = idx % pw, j = idx / pw;
-        lds[j * LDS_LD + i] = (i >= j) ? base[(int64_t)(p0 + j) * d.f + p0 + i] : 0.0;
+        lds[j * LDS_LD + i] = (i >= j) ? base[(int64_t)(p0 + j) * d.ld + p0 + i] : 0.0;
     }
     __syncthreads();
     for (int j = 0; j < pw; j++) {
@@ -42,11 +42,11 @@ __device__ void blk_diag_factor(double *L, const FrontDesc &d, int p0, double *D
     }
     for (int idx = threadIdx.x; idx < pw * pw; idx += blockDim.x) {
         int i = idx % pw, j = idx / pw;
-        if (i >= j) base[(int64_t)(p0 + j) * d.f + p0 + i] = lds[j * LDS_LD + i];
+        if (i >= j) base[(int64_t)(p0 + j) * d.ld + p0 + i] = lds[j * LDS_LD + i];
     }
     for (int j = threadIdx.x; j < pw; j += blockDim.x) {
         double dj = lds[j * LDS_LD + j];
-        Dn[d.c0 + p0 + j] = dj;
+        Dn[d.doff + p0 + j] = dj;
         if (!(fabs(dj) > 1e-300) || !isfinite(dj)) atomicOr(flag, 1);
     }
     __syncthreads();
@@ -72,7 +72,7 @@ __device__ void blk_trsm(double *L, const FrontDesc &d, int p0, int r0, const do
 #pragma unroll 8
         for (int idx = threadIdx.x; idx < TF_PB * TF_TB; idx += TF_NT) {
             int k = idx / TF_TB, r = idx % TF_TB;
-            At[k * MLD + r] = (k < pw && R0 + r < d.f) ? base[(int64_t)(p0 + k) * d.f + R0 + r] : 0.0;
+            At[k * MLD + r] = (k < pw && R0 + r < d.f) ? base[(int64_t)(p0 + k) * d.ld + R0 + r] : 0.0;
         }
         __syncthreads();
         tf_d4 acc[2][2];
@@ -85,11 +85,11 @@ __device__ void blk_trsm(double *L, const FrontDesc &d, int p0, int r0, const do
             for (int r = 0; r < 4; r++) {
                 const int c = ACC_COL(bj, r);
                 if (c >= pw) continue;
-                const double dc = Dn[d.c0 + p0 + c];
+                const double dc = Dn[d.doff + p0 + c];
 #pragma unroll
                 for (int bi = 0; bi < 2; bi++) {
                     const int row = R0 + ACC_ROW(bi);
-                    if (row < d.f) base[(int64_t)(p0 + c) * d.f + row] = acc[bi][bj][r] / dc;
+                    if (row < d.f) base[(int64_t)(p0 + c) * d.ld + row] = acc[bi][bj][r] / dc;
                 }
             }
         __syncthreads();
@@ -111,9 +111,9 @@ __device__ void blk_update_tile(double *L, double *U, const FrontDesc &d, int p0
         int k = idx / TF_TB, i = idx % TF_TB;
         double va = 0.0, vb = 0.0;
         if (k < pw) {
-            const double *col = base + (int64_t)(p0 + k) * d.f;
+            const double *col = base + (int64_t)(p0 + k) * d.ld;
             if (rb + i < d.f) va = col[rb + i];
-            if (cb + i < d.f) vb = col[cb + i] * Dn[d.c0 + p0 + k];
+            if (cb + i < d.f) vb = col[cb + i] * Dn[d.doff + p0 + k];
         }
         A[k * MLD + i] = va;
         B[k * MLD + i] = vb;
@@ -179,7 +179,7 @@ __device__ void blk_update_tile_mfma(double *L, double *U, const FrontDesc &d, i
                 va[q] = 0.0;
                 vb[q] = 0.0;
                 if (k < pw) {
-                    const double *col = base + (int64_t)(p0 + k) * d.f;
+                    const double *col = base + (int64_t)(p0 + k) * d.ld;
                     if (rb + i < d.f) va[q] = col[rb + i];
                     if (cb + i < d.f) vb[q] = col[cb + i];
                 }
@@ -190,7 +190,7 @@ __device__ void blk_update_tile_mfma(double *L, double *U, const FrontDesc &d, i
                 const int kk = idx / TF_UT, i = idx % TF_UT;
                 const int k = kh + kk;
                 Ap[kk * TF_ULD + i] = va[q];
-                Bp[kk * TF_ULD + i] = (k < pw) ? vb[q] * Dn[d.c0 + p0 + k] : 0.0;
+                Bp[kk * TF_ULD + i] = (k < pw) ? vb[q] * Dn[d.doff + p0 + k] : 0.0;
             }
         }
         __syncthreads();
@@ -246,7 +246,7 @@ __device__ void blk_invert_to_lds(const double *L, const FrontDesc &d, int p0, d
     const double *base = L + d.loff;
     for (int idx = threadIdx.x; idx < TF_PB * TF_PB; idx += blockDim.x) {
         int i = idx % TF_PB, j = idx / TF_PB;
-        lds[j * LDS_LD + i] = (i > j && i < pw && j < pw) ? base[(int64_t)(p0 + j) * d.f + p0 + i] : 0.0;
+        lds[j * LDS_LD + i] = (i > j && i < pw && j < pw) ? base[(int64_t)(p0 + j) * d.ld + p0 + i] : 0.0;
     }
     __syncthreads();
     for (int j = pw - 2; j >= 0; j--) {
@@ -269,7 +269,7 @@ __device__ void blk_invert_diag(double *L, const FrontDesc &d, int p0, double *l
     double *base = L + d.loff;
     for (int idx = threadIdx.x; idx < pw * pw; idx += blockDim.x) {
         int i = idx % pw, j = idx / pw;
-        if (i > j) base[(int64_t)(p0 + j) * d.f + p0 + i] = lds[j * LDS_LD + i];
+        if (i > j) base[(int64_t)(p0 + j) * d.ld + p0 + i] = lds[j * LDS_LD + i];
     }
     __syncthreads();
 }
@@ -294,8 +294,8 @@ __device__ void blk_inv_w(const double *L, const FrontDesc &d, int p0, int R0, d
             int kk = idx / TF_TB, r = idx % TF_TB;
             int k = k0 + kk, row = R0 + r;
             double xv = 0.0, lv = 0.0;
-            if (k < d.nc && row < d.nc && k < row) xv = base[(int64_t)k * d.f + row];
-            if (k < d.nc && r < pw) lv = base[(int64_t)(p0 + r) * d.f + k];
+            if (k < d.nc && row < d.nc && k < row) xv = base[(int64_t)k * d.ld + row];
+            if (k < d.nc && r < pw) lv = base[(int64_t)(p0 + r) * d.ld + k];
             Xt[kk * MLD + r] = xv;
             Lt[kk * MLD + r] = lv;
         }
@@ -313,7 +313,7 @@ __device__ void blk_inv_w(const double *L, const FrontDesc &d, int p0, int R0, d
             for (int bi = 0; bi < 2; bi++) {
                 const int row = R0 + ACC_ROW(bi);
                 if (row < d.nc)
-                    W[(int64_t)c * nq + (row - p1)] = base[(int64_t)(p0 + c) * d.f + row] + acc[bi][bj][r];
+                    W[(int64_t)c * nq + (row - p1)] = base[(int64_t)(p0 + c) * d.ld + row] + acc[bi][bj][r];
             }
         }
 }
@@ -349,7 +349,7 @@ __device__ void blk_inv_apply(double *L, const FrontDesc &d, int p0, int R0, con
 #pragma unroll
             for (int bi = 0; bi < 2; bi++) {
                 const int row = R0 + ACC_ROW(bi);
-                if (row < d.nc) base[(int64_t)(p0 + c) * d.f + row] = -acc[bi][bj][r];
+                if (row < d.nc) base[(int64_t)(p0 + c) * d.ld + row] = -acc[bi][bj][r];
             }
         }
     __syncthreads();

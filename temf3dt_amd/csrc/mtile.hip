@@ -57,7 +57,7 @@ __device__ __forceinline__ void mt_store(const FrontDesc &d, int o, int q, doubl
         out[(d.soff + o) * K + q] = s;
     } else if (MODE == MT_BWD_T) {  // T = Y/d - s
         const int64_t oo = (int64_t)(d.c0 + o) * K + q;
-        out[oo] = in1[oo] / Dn[d.c0 + o] - s;
+        out[oo] = in1[oo] / Dn[d.doff + o] - s;
     } else {  // X = T + s
         const int64_t oo = (int64_t)(d.c0 + o) * K + q;
         out[oo] = in1[oo] + s;
@@ -65,8 +65,8 @@ __device__ __forceinline__ void mt_store(const FrontDesc &d, int o, int q, doubl
 }
 
 // One wave: acc(out, rhs) = sum_{k0 <= i < k1} A(out, i) B(i, rhs) where
-//   N form: A(out, i) = Lf[i*f + roff + out]   (column i, row out)
-//   T form: A(out, i) = Lf[out*f + roff + i]   (column out, row i)
+//   N form: A(out, i) = Lf[i*ld + roff + out]   (column i, row out)
+//   T form: A(out, i) = Lf[out*ld + roff + i]   (column out, row i)
 // B(i, rhs) = Bm[row(i)*K + rhs] with row(i) = i, or bidx[i] when GATH.
 // TRI keeps only i < out (N) / i > out (T).  The inner range is walked in
 // stages of KS k-steps (4*KS inner values).  KS = 2: two register stages,
@@ -83,9 +83,12 @@ __device__ __forceinline__ void mt_wave(const double *Lf, int64_t f, int roff, i
 #pragma unroll
     for (int b = 0; b < 4; b++) {
         ob[b] = o0 + 16 * b + l16;
-        ov[b] = ob[b] < nout;
+        ov[b] = ob[b] >= 0 && ob[b] < nout;
         qv[b] = 16 * b + l16 < K;
     }
+    // T form: stages start on 16-row boundaries of the column (columns are
+    // 128-byte aligned), so each stage reads whole cache lines
+    const int kb = TFORM ? (k0 - ((roff + k0) & 15)) : k0;
     // inner index of k-step s of the stage starting at ib
     auto inner = [&](int ib, int s) { return TFORM ? (ib + KS * lk + s) : (ib + 4 * s + lk); };
     // gathered B rows of a stage: fetched one stage ahead of the operands
@@ -93,14 +96,14 @@ __device__ __forceinline__ void mt_wave(const double *Lf, int64_t f, int roff, i
 #pragma unroll
         for (int s = 0; s < KS; s++) {
             const int i = inner(ib, s);
-            r[s] = (GATH && i < k1) ? bidx[i] : 0;
+            r[s] = (GATH && i >= k0 && i < k1) ? bidx[i] : 0;
         }
     };
     auto load = [&](int ib, const int r[KS], double xa[KS][4], double yb[KS][4]) {
 #pragma unroll
         for (int s = 0; s < KS; s++) {
             const int i = inner(ib, s);
-            const bool iv = i < k1;
+            const bool iv = i >= k0 && i < k1;
             const int64_t brow = GATH ? (int64_t)r[s] : (int64_t)i;
 #pragma unroll
             for (int b = 0; b < 4; b++) {
@@ -126,8 +129,8 @@ __device__ __forceinline__ void mt_wave(const double *Lf, int64_t f, int roff, i
     if (KS == 4) {
         double xa[KS][4], yb[KS][4];
         int rA[KS], rN[KS];
-        ldidx(k0, rA);
-        for (int ib = k0; ib < k1; ib += SW) {
+        ldidx(kb, rA);
+        for (int ib = kb; ib < k1; ib += SW) {
             if (GATH) ldidx(ib + SW, rN);
             load(ib, rA, xa, yb);
             mult(xa, yb);
@@ -139,10 +142,10 @@ __device__ __forceinline__ void mt_wave(const double *Lf, int64_t f, int roff, i
     }
     double xa0[KS][4], yb0[KS][4], xa1[KS][4], yb1[KS][4];
     int rA[KS], rB[KS];
-    ldidx(k0, rA);
-    ldidx(k0 + SW, rB);
-    load(k0, rA, xa0, yb0);
-    int ib = k0;
+    ldidx(kb, rA);
+    ldidx(kb + SW, rB);
+    load(kb, rA, xa0, yb0);
+    int ib = kb;
     for (;;) {
         const bool more = ib + SW < k1;
         if (more) {
@@ -182,13 +185,13 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
     // solution rows of the struct indices (backward T)
     const double *Bm = in0 + (int64_t)d.c0 * K;
     if (MODE == MT_OWN_FWD || MODE == MT_STR_FWD) {
-        if (tri) mt_wave<false, true, false, KS>(Lf, d.f, roff, nout, tu.t0, tu.k0, tu.k1, Bm, nullptr, K, acc);
-        else mt_wave<false, false, false, KS>(Lf, d.f, roff, nout, tu.t0, tu.k0, tu.k1, Bm, nullptr, K, acc);
+        if (tri) mt_wave<false, true, false, KS>(Lf, d.ld, roff, nout, tu.t0, tu.k0, tu.k1, Bm, nullptr, K, acc);
+        else mt_wave<false, false, false, KS>(Lf, d.ld, roff, nout, tu.t0, tu.k0, tu.k1, Bm, nullptr, K, acc);
     } else if (MODE == MT_BWD_T) {
-        mt_wave<true, false, true, KS>(Lf, d.f, roff, nout, tu.t0, tu.k0, tu.k1, in0, sidx + d.soff, K, acc);
+        mt_wave<true, false, true, KS>(Lf, d.ld, roff, nout, tu.t0, tu.k0, tu.k1, in0, sidx + d.soff, K, acc);
     } else {
-        if (tri) mt_wave<true, true, false, KS>(Lf, d.f, roff, nout, tu.t0, tu.k0, tu.k1, Bm, nullptr, K, acc);
-        else mt_wave<true, false, false, KS>(Lf, d.f, roff, nout, tu.t0, tu.k0, tu.k1, Bm, nullptr, K, acc);
+        if (tri) mt_wave<true, true, false, KS>(Lf, d.ld, roff, nout, tu.t0, tu.k0, tu.k1, Bm, nullptr, K, acc);
+        else mt_wave<true, false, false, KS>(Lf, d.ld, roff, nout, tu.t0, tu.k0, tu.k1, Bm, nullptr, K, acc);
     }
     const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
     if (tu.slot < 0) {
@@ -197,7 +200,7 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int o = tu.t0 + 16 * b + 4 * r + lk;
-                if (o >= nout) continue;
+                if (o < 0 || o >= nout) continue;
 #pragma unroll
                 for (int a = 0; a < 4; a++) {
                     const int q = 16 * a + l16;
@@ -228,7 +231,7 @@ __global__ void __launch_bounds__(256) k_mtile_red(const FrontDesc *fd, const Ti
     mt_geom<MODE>(d, nout, roff);
     const int e = (blockIdx.x & 15) * 256 + threadIdx.x;
     const int o = tu.t0 + (e >> 6), q = e & 63;
-    if (o >= nout || q >= K) return;
+    if (o < 0 || o >= nout || q >= K) return;
     const double *p0 = part + (int64_t)tu.k0 * 4096 + e;
     double s = 0.0;
     for (int c = 0; c < tu.k1; c++) s += p0[(int64_t)c * 4096];

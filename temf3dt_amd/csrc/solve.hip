@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(TF_NT) k_fwd_small(const double *L, const Fron
     // y_own = X11 v_own (unit lower): rows < nc
     {
         const int r = threadIdx.x & 63;
-        double acc = blk_gemv_rows(A, d.f, r, r < d.nc ? r : 0, xs, red);
+        double acc = blk_gemv_rows(A, d.ld, r, r < d.nc ? r : 0, xs, red);
         if (threadIdx.x < d.nc) {
             double y = xs[r] + acc;
             vf[r] = y;
@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(TF_NT) k_fwd_small(const double *L, const Fron
     for (int r0 = d.nc; r0 < d.f; r0 += 64) {
         const int r = r0 + (threadIdx.x & 63);
         const int rr = r < d.f ? r : d.f - 1;
-        double acc = blk_gemv_rows(A, d.f, rr, d.nc, xs, red);
+        double acc = blk_gemv_rows(A, d.ld, rr, d.nc, xs, red);
         if (threadIdx.x < 64 && r < d.f) vf[r] -= acc;
     }
 }
@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(TF_NT) k_bwd_small(const double *L, const Fron
 #pragma unroll
                 for (int k = 0; k < 16; k++) {
                     const int j = wave * 16 + k;
-                    if (j < d.nc) acc[k] += A[(int64_t)j * d.f + rr] * w;
+                    if (j < d.nc) acc[k] += A[(int64_t)j * d.ld + rr] * w;
                 }
             }
         }
@@ -129,7 +129,7 @@ __global__ void __launch_bounds__(TF_NT) k_bwd_small(const double *L, const Fron
     for (int k = 0; k < 16; k++) {
         double s = wave_sum(acc[k]);
         const int j = wave * 16 + k;
-        if (lane == 0 && j < d.nc) ts[j] = Y[d.c0 + j] / Dn[d.c0 + j] - s;
+        if (lane == 0 && j < d.nc) ts[j] = Y[d.c0 + j] / Dn[d.doff + j] - s;
     }
     __syncthreads();
     // x_j = t_j + sum_{i>j} X[i][j] t_i
@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(TF_NT) k_bwd_small(const double *L, const Fron
     for (int k = 0; k < 16; k++) {
         const int j = wave * 16 + k;
         double v = 0.0;
-        if (j < d.nc && lane > j && lane < d.nc) v = A[(int64_t)j * d.f + lane] * ts[lane];
+        if (j < d.nc && lane > j && lane < d.nc) v = A[(int64_t)j * d.ld + lane] * ts[lane];
         v = wave_sum(v);
         if (lane == 0 && j < d.nc) xw[d.c0 + j] = ts[j] + v;
     }
@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(TF_NT) k_fwd_own(const double *L, const FrontD
     const int r = (int)(blockIdx.x - pref[i]) * 64 + (threadIdx.x & 63);
     const int rr = r < d.nc ? r : d.nc - 1;
     const double *vf = V + d.voff;
-    double acc = blk_gemv_rows(L + d.loff, d.f, rr, rr, vf, red);
+    double acc = blk_gemv_rows(L + d.loff, d.ld, rr, rr, vf, red);
     if (threadIdx.x < 64 && r < d.nc) Y[d.c0 + r] = vf[r] + acc;
 }
 
@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(TF_NT) k_fwd_struct(const double *L, const Fro
     const FrontDesc d = fd[list[i]];
     const int r = d.nc + (int)(blockIdx.x - pref[i]) * 64 + (threadIdx.x & 63);
     const int rr = r < d.f ? r : d.f - 1;
-    double acc = blk_gemv_rows(L + d.loff, d.f, rr, d.nc, Y + d.c0, red);
+    double acc = blk_gemv_rows(L + d.loff, d.ld, rr, d.nc, Y + d.c0, red);
     if (threadIdx.x < 64 && r < d.f) V[d.voff + r] -= acc;
 }
 
@@ -196,9 +196,9 @@ __global__ void __launch_bounds__(TF_NT) k_bwd_t(const double *L, const FrontDes
     for (int q = 0; q < 4; q++) {
         const int j = (int)(blockIdx.x - pref[i]) * 16 + wave * 4 + q;
         if (j >= d.nc) break;
-        const double *col = L + d.loff + (int64_t)j * d.f;
+        const double *col = L + d.loff + (int64_t)j * d.ld;
         double s = wave_dot(col, xs, d.nc, d.f, lane);
-        if (lane == 0) T[d.c0 + j] = Y[d.c0 + j] / Dn[d.c0 + j] - s;
+        if (lane == 0) T[d.c0 + j] = Y[d.c0 + j] / Dn[d.doff + j] - s;
     }
 }
 
@@ -212,7 +212,7 @@ __global__ void __launch_bounds__(TF_NT) k_bwd_x(const double *L, const FrontDes
     for (int q = 0; q < 4; q++) {
         const int j = (int)(blockIdx.x - pref[i]) * 16 + wave * 4 + q;
         if (j >= d.nc) break;
-        const double *col = L + d.loff + (int64_t)j * d.f;
+        const double *col = L + d.loff + (int64_t)j * d.ld;
         double s = wave_dot(col, t, j + 1, d.nc, lane);
         if (lane == 0) xw[d.c0 + j] = t[j] + s;
     }

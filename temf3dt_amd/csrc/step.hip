@@ -107,7 +107,72 @@ __global__ void k_gather_vals(int64_t n, const int64_t *map, const double *src, 
     if (i < n) dst[i] = src[map[i]];
 }
 
+// ---- independent models in one context ---------------------------------------
+
+// sys_right for several models: r[edge] -= coef[model] * w for every wire
+// entry, coef = 2 dt_m * dI/dt(t_m) of the entry's model (zero once the
+// model's source is off)
+__global__ void k_source_models(int ns, const int *edge, const int *model, const double *w, const double *coef,
+                                double *r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ns) {
+        const int e = edge[i];
+        r[e] = r[e] - coef[model[i]] * w[i];
+    }
+}
+
+// P = 3 Msigma + 2 dt_m K into each model's target factor slot:
+// L[amap[q] + slot_off[model]] with the entry's model
+__global__ void k_scatter_p_models(int64_t nnz, const int64_t *amap, const int *qmodel, const double *M,
+                                   const double *K, const double *cdt, const int64_t *slot_off, double *L) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nnz) {
+        const int m = qmodel[q];
+        L[amap[q] + slot_off[m]] = cdt[m] * K[q] + 3.0 * M[q];
+    }
+}
+
+// per-model sums of (a-b)^2 and b^2 over the model's edge range: one
+// workgroup per model, fixed reduction order
+__global__ void __launch_bounds__(256) k_range_norms(const int *range, const double *a, const double *b, double *out) {
+    __shared__ double s1[256], s2[256];
+    const int m = blockIdx.x;
+    const int e0 = range[2 * m], e1 = range[2 * m + 1];
+    double x1 = 0.0, x2 = 0.0;
+    for (int e = e0 + threadIdx.x; e < e1; e += 256) {
+        const double d = a[e] - b[e];
+        x1 += d * d;
+        x2 += b[e] * b[e];
+    }
+    s1[threadIdx.x] = x1;
+    s2[threadIdx.x] = x2;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            s1[threadIdx.x] += s1[threadIdx.x + h];
+            s2[threadIdx.x] += s2[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * m] = s1[0];
+        out[2 * m + 1] = s2[0];
+    }
+}
+
 namespace tf {
+void launch_source_models(int ns, const int *edge, const int *model, const double *w, const double *coef, double *r,
+                          hipStream_t st) {
+    if (ns > 0) hipLaunchKernelGGL(k_source_models, dim3((ns + 63) / 64), dim3(64), 0, st, ns, edge, model, w, coef, r);
+}
+void launch_scatter_p_models(int64_t nnz, const int64_t *amap, const int *qmodel, const double *M, const double *K,
+                             const double *cdt, const int64_t *slot_off, double *L, hipStream_t st) {
+    hipLaunchKernelGGL(k_scatter_p_models, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, nnz, amap, qmodel, M,
+                       K, cdt, slot_off, L);
+}
+void launch_range_norms(int nm, const int *range, const double *a, const double *b, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_range_norms, dim3(nm), dim3(256), 0, st, range, a, b, out);
+}
 void launch_spmv_rhs(int n, const int *ia, const int *ja, const double *val, const double *e2, const double *e3,
                      double *r, hipStream_t st) {
     int64_t threads = (int64_t)n * 16;

@@ -141,6 +141,24 @@ static int nd_rec(NDState &st, std::vector<int> v) {
     return id;
 }
 
+// Disconnected graph (independent models in one context): one dissection
+// per connected component, joined under column-free nodes, so the fronts
+// form one tree per component and no front mixes components.
+static int nd_components(NDState &st, std::vector<std::vector<int>> &comps, size_t a, size_t b) {
+    if (b - a == 1) return nd_rec(st, std::move(comps[a]));
+    const int id = st.nnodes.fetch_add(1);
+    const size_t mid = (a + b) / 2;
+    int ca = -1, cb = -1;
+#pragma omp task shared(ca, st, comps)
+    { ca = nd_components(st, comps, a, mid); }
+#pragma omp task shared(cb, st, comps)
+    { cb = nd_components(st, comps, mid, b); }
+#pragma omp taskwait
+    st.nodes[id].child[0] = ca;
+    st.nodes[id].child[1] = cb;
+    return id;
+}
+
 }  // namespace
 
 void analyze(Symbolic &s, const Topology &t, int leaf_size) {
@@ -177,12 +195,28 @@ void analyze(Symbolic &s, const Topology &t, int leaf_size) {
     st.mark.assign(n, 0);
     st.leaf = std::max(leaf_size, 4);
     st.nodes.resize((size_t)(2 * (n / std::max(1, st.leaf / 2)) + 64));
-    std::vector<int> all(n);
-    std::iota(all.begin(), all.end(), 0);
+    // connected components (breadth-first, in order of the first vertex)
+    std::vector<int> comp(n, -1);
+    std::vector<std::vector<int>> comps;
+    for (int v0 = 0; v0 < n; v0++) {
+        if (comp[v0] >= 0) continue;
+        const int cid = (int)comps.size();
+        comps.emplace_back();
+        std::vector<int> &cv = comps.back();
+        comp[v0] = cid;
+        cv.push_back(v0);
+        for (size_t h = 0; h < cv.size(); h++) {
+            const int v = cv[h];
+            for (int q = xadj[v]; q < xadj[v + 1]; q++)
+                if (comp[adj[q]] < 0) { comp[adj[q]] = cid; cv.push_back(adj[q]); }
+        }
+        std::sort(cv.begin(), cv.end());
+    }
+    st.nodes.resize(st.nodes.size() + comps.size());
     int root = -1;
 #pragma omp parallel
 #pragma omp single
-    root = nd_rec(st, std::move(all));
+    root = nd_components(st, comps, 0, comps.size());
     if ((size_t)st.nnodes.load() > st.nodes.size()) throw std::runtime_error("nd node overflow");
     // postorder: assign columns; non-empty nodes become supernodes
     s.perm.reserve(n);
@@ -273,7 +307,7 @@ void analyze(Symbolic &s, const Topology &t, int leaf_size) {
     s.flops = 0.0;
     for (int k = 0; k < s.nsn; k++) {
         int64_t nc = s.nc[k], m = s.sptr[k + 1] - s.sptr[k], f = nc + m;
-        s.loff[k + 1] = s.loff[k] + f * nc;
+        s.loff[k + 1] = s.loff[k] + front_ld(f) * nc;
         s.uoff[k + 1] = s.uoff[k] + m * m;
         s.nnzL += nc * (nc + 1) / 2 + m * nc;
         for (int64_t j = 0; j < nc; j++) {
@@ -339,7 +373,7 @@ void analyze(Symbolic &s, const Topology &t, int leaf_size) {
                 if (it == ps + m || *it != hi) throw std::runtime_error("amap miss");
                 row = s.nc[k] + (it - ps);
             }
-            s.amap[q] = s.loff[k] + (int64_t)(lo - c0) * f + row;
+            s.amap[q] = s.loff[k] + (int64_t)(lo - c0) * front_ld(f) + row;
         }
 }
 

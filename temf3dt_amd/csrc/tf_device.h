@@ -14,11 +14,13 @@
 // rows.  L block: f x nc column-major (ld f) at L + loff; update block
 // (Schur complement passed to the parent): m x m column-major at U + uoff.
 struct FrontDesc {
-    int64_t loff;
+    int64_t loff;  // multiple of 16 doubles (128-byte aligned columns)
     int64_t uoff;
     int64_t soff;  // offset of struct rows / relative indices
     int64_t voff;  // offset of the front's slice in solve work vectors
     int f, nc, m, c0;
+    int ld;        // column stride of the L block: f rounded up to 16
+    int doff;      // offset of the front's pivots in the D array (c0 for one model)
 };
 
 // Work unit of the block-solve tile kernels: a 64-wide output tile starting
@@ -32,7 +34,7 @@ struct TileUnit {
 // Column c of a front addressed uniformly: columns < nc live in the L block,
 // the rest in the update block.  Element (r, c), r >= c, is colptr(c)[r].
 __device__ __forceinline__ double *front_col(double *L, double *U, const FrontDesc &d, int c) {
-    return (c < d.nc) ? (L + d.loff + (int64_t)c * d.f)
+    return (c < d.nc) ? (L + d.loff + (int64_t)c * d.ld)
                       : (U + d.uoff + (int64_t)(c - d.nc) * d.m - d.nc);
 }
 

@@ -56,10 +56,14 @@ struct Symbolic {
 };
 
 namespace tf {
+// column stride of a front's L block: rows rounded up to 16 doubles, so that
+// with 16-aligned offsets every column starts on a 128-byte line
+inline int64_t front_ld(int64_t f) { return (f + 15) / 16 * 16; }
 // topology.cpp
 void build_topology(Topology &t, int nN, const double *xyz, int nT, const int *tet_vex);
 int find_edge(const Topology &t, int v1, int v2);  // 0-based edge id or -1
-int point_in_tet(const Topology &t, const double *p, double rec_charac);  // 1-based or 0
+// first tet in [t0, t1) (all tets by default) holding p; 1-based or 0
+int point_in_tet(const Topology &t, const double *p, double rec_charac, int t0 = 0, int t1 = -1);
 double det_lu(const double *a_colmajor, int n);
 // symbolic.cpp
 void analyze(Symbolic &s, const Topology &t, int leaf_size);

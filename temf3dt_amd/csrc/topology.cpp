@@ -171,8 +171,9 @@ int find_edge(const Topology &t, int v1, int v2) {
 // p_in_tet (src/FETD_FWD.f90:1009-1049): first tet, in index order, whose
 // centroid lies within rec_charac of p and whose five determinants share a
 // sign (zero counts for both signs).
-int point_in_tet(const Topology &t, const double *p, double rec_charac) {
-    for (int i = 0; i < t.nT; i++) {
+int point_in_tet(const Topology &t, const double *p, double rec_charac, int t0, int t1) {
+    if (t1 < 0) t1 = t.nT;
+    for (int i = t0; i < t1; i++) {
         double x[4], y[4], z[4];
         for (int q = 0; q < 4; q++) {
             const double *v = &t.xyz[3 * (size_t)(t.tet_vex[4 * (size_t)i + q] - 1)];

@@ -53,6 +53,7 @@ _SIGS = {
     "tf_get_profile": [_P, _I, _P, _P],
     "tf_set_checkpointing": [_P, _I],
     "tf_adjoint": [_P, _I, _P, _P],
+    "tf_set_materials_tensor": [_P, _P, _P],
 }
 
 # kernel-timing categories of tf_get_profile (include/temfetd.h)
@@ -140,6 +141,13 @@ class Fetd:
         s = np.ascontiguousarray(sigma, np.float64)
         m = np.ascontiguousarray(inv_mu, np.float64)
         self._chk(self.lib.tf_set_materials(self.ctx, _ptr(s), _ptr(m)), "tf_set_materials")
+
+    def set_materials_tensor(self, sig9, nu9):
+        """Anisotropic media: [nT][3][3] conductivity tensors and curl weights."""
+        self.sig9 = np.ascontiguousarray(sig9, np.float64).reshape(-1, 9)
+        self.nu9 = np.ascontiguousarray(nu9, np.float64).reshape(-1, 9)
+        self._chk(self.lib.tf_set_materials_tensor(self.ctx, _ptr(self.sig9), _ptr(self.nu9)),
+                  "tf_set_materials_tensor")
 
     def set_source(self, seg_vex, source_type, pulse_type, pulse_width, current, isrc=0):
         sv = np.ascontiguousarray(seg_vex, np.int32).reshape(-1, 2)
@@ -292,10 +300,12 @@ def setup_from_files(mesh_path, cntl_path, device=0):
         raise FetdError("error!")  # only the adaptive BDF2 method exists (src/FETD_FWD.f90:1210-1219)
     if c["bound_cond"] != 0:
         raise FetdError("absorbing boundary condition is not supported")
-    sigma, inv_mu = tfio.materials(m["tet_phy"], c["domains"])
     fe = Fetd(device)
     fe.set_mesh(m["xyz"], m["tet_vex"])
-    fe.set_materials(sigma, inv_mu)
+    if c.get("aniso"):
+        fe.set_materials_tensor(*tfio.materials_tensor(m["tet_phy"], c["aniso"]))
+    else:
+        fe.set_materials(*tfio.materials(m["tet_phy"], c["domains"]))
     fe.set_source(m["seg_vex"], c["source_type"], c["pulse_type"], c["pulse_width"], c["current"])
     fe.set_receivers(c["receivers"], c["rec_charac"])
     fe.set_schedule(c["sp_division"], c["sp_double"], c["sp_dblesize"], c["time_maximum"])

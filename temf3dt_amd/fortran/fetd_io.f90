@@ -14,7 +14,8 @@ module fetd_io
   use iso_c_binding, only: c_int, c_double
   implicit none
   private
-  public :: cntl_t, read_medit, read_cntl, tet_materials, field_writer_t, fw_open, fw_row, fw_close
+  public :: cntl_t, read_medit, read_cntl, tet_materials, tet_tensors, aniso_tensor, field_writer_t, fw_open, &
+            fw_row, fw_close
 
   real(8), parameter :: mu0 = 3.14159265358979323846264338327950288419716939937510d0 * 4.0d-7
 
@@ -22,6 +23,8 @@ module fetd_io
     integer :: rec_num = 0, dom_num = 0, src_num = 0, bound_cond = 0
     real(8), allocatable :: rec(:, :)        ! (3, rec_num)
     real(8), allocatable :: dom(:, :)        ! (4, dom_num): id, sigma, permittivity, permeability
+    logical :: aniso = .false.
+    real(8), allocatable :: adom(:, :)       ! (19, dom_num): id, then (3 values, 3 angles) for sigma, eps, mu
     integer :: source_type = 1, pulse_type = 1
     real(8) :: pulse_width = 0.0d0, current = 1.0d0
     integer :: sp_mthd = 0, sp_division = 0, sp_double = 0, sp_dblesize = 0
@@ -177,7 +180,18 @@ contains
           read(u, *) c%dom(:, i)
         end do
       case ('anisotropic')
-        ierr = 3; msg = 'anisotropic media are not supported'; exit
+        ! three records per domain: "domain v1 v2 v3 angle1 angle2 angle3" for
+        ! conductivity, permittivity, permeability (src/FETD_FWD.f90:368-377)
+        read(val, *) n
+        c%dom_num = n
+        c%aniso = .true.
+        allocate(c%adom(19, n), c%dom(4, n))
+        do i = 1, n
+          read(u, *) c%adom(1, i), c%adom(2:7, i)
+          read(u, *) c%adom(1, i), c%adom(8:13, i)
+          read(u, *) c%adom(1, i), c%adom(14:19, i)
+          c%dom(:, i) = [c%adom(1, i), c%adom(2, i), c%adom(8, i), c%adom(14, i)]
+        end do
       case ('boundary condition')
         read(val, *) c%bound_cond
       case ('source')
@@ -258,6 +272,100 @@ contains
       end if
     end do
   end subroutine tet_materials
+
+  ! R diag(v) R^T with R = Rx Ry Rz from three angles in degrees, times
+  ! scale; as in the reference the (2,2) entry of Rz takes the cosine of the
+  ! second angle (src/FETD_FWD.f90:385-392).  A is row-major in a(1:9).
+  subroutine aniso_tensor(v, ang, scale, a)
+    real(8), intent(in) :: v(3), ang(3), scale
+    real(8), intent(out) :: a(9)
+    real(8), parameter :: d2r = 3.141592653589793238462643383279502884197d0 / 180.0d0
+    real(8) :: dx(3, 3), dy(3, 3), dz(3, 3), t(3, 3), dm(3, 3), w(3, 3), a1, a2, a3, acc
+    integer :: i, j, k
+    a1 = ang(1) * d2r; a2 = ang(2) * d2r; a3 = ang(3) * d2r
+    dx = reshape([1.0d0, 0.0d0, 0.0d0, 0.0d0, cos(a1), sin(a1), 0.0d0, -sin(a1), cos(a1)], [3, 3])
+    dy = reshape([cos(a2), 0.0d0, -sin(a2), 0.0d0, 1.0d0, 0.0d0, sin(a2), 0.0d0, cos(a2)], [3, 3])
+    dz = reshape([cos(a3), sin(a3), 0.0d0, -sin(a3), cos(a2), 0.0d0, 0.0d0, 0.0d0, 1.0d0], [3, 3])
+    do i = 1, 3
+      do j = 1, 3
+        acc = 0.0d0
+        do k = 1, 3
+          acc = acc + dx(i, k) * dy(k, j)
+        end do
+        t(i, j) = acc
+      end do
+    end do
+    do i = 1, 3
+      do j = 1, 3
+        acc = 0.0d0
+        do k = 1, 3
+          acc = acc + t(i, k) * dz(k, j)
+        end do
+        dm(i, j) = acc
+      end do
+    end do
+    do i = 1, 3
+      do j = 1, 3
+        acc = 0.0d0
+        do k = 1, 3
+          if (k == j) then
+            acc = acc + dm(i, k) * v(j)
+          else
+            acc = acc + dm(i, k) * 0.0d0
+          end if
+        end do
+        w(i, j) = acc
+      end do
+    end do
+    do i = 1, 3
+      do j = 1, 3
+        acc = 0.0d0
+        do k = 1, 3
+          acc = acc + w(i, k) * dm(j, k)
+        end do
+        a(3 * (i - 1) + j) = acc * scale
+      end do
+    end do
+  end subroutine aniso_tensor
+
+  ! per-tet tensors (9, nT): conductivity, and the curl weight = entrywise
+  ! reciprocal of the permeability tensor, zero entries kept (:621-633)
+  subroutine tet_tensors(c, nT, tet_phy, sig9, nu9, ierr)
+    type(cntl_t), intent(in) :: c
+    integer, intent(in) :: nT
+    integer(c_int), intent(in) :: tet_phy(nT)
+    real(c_double), intent(out) :: sig9(9, nT), nu9(9, nT)
+    integer, intent(out) :: ierr
+    real(8) :: s(9, c%dom_num), mg(9), nu(9, c%dom_num)
+    integer :: i, d, q
+    logical :: found
+    ierr = 0
+    do d = 1, c%dom_num
+      call aniso_tensor(c%adom(2:4, d), c%adom(5:7, d), 1.0d0, s(:, d))
+      call aniso_tensor(c%adom(14:16, d), c%adom(17:19, d), mu0, mg)
+      do q = 1, 9
+        if (mg(q) == 0.0d0) then
+          nu(q, d) = 0.0d0
+        else
+          nu(q, d) = 1.0d0 / mg(q)
+        end if
+      end do
+    end do
+    do i = 1, nT
+      found = .false.
+      do d = 1, c%dom_num
+        if (tet_phy(i) == nint(c%adom(1, d))) then
+          sig9(:, i) = s(:, d)
+          nu9(:, i) = nu(:, d)
+          found = .true.
+          exit
+        end if
+      end do
+      if (.not. found) then
+        ierr = 1; return
+      end if
+    end do
+  end subroutine tet_tensors
 
   ! ---- .field / .field2 ---------------------------------------------------
   subroutine fw_open(w, field_path, field2_path, nrec, o_f, gates)

@@ -10,7 +10,7 @@ program fetd_io_check
   implicit none
   character(512) :: mode, a1, a2, msg
   integer :: ierr, nN, nT, nseg, i, nrec, ng, nrows, step, u, o_f(9)
-  real(c_double), allocatable :: xyz(:, :), gates(:), vals(:, :)
+  real(c_double), allocatable :: xyz(:, :), gates(:), vals(:, :), s9(:, :), n9(:, :)
   integer(c_int), allocatable :: tet(:, :), tet_phy(:), seg(:, :)
   real(8) :: t
   type(cntl_t) :: c
@@ -49,6 +49,14 @@ program fetd_io_check
     write(*, '(a,4i8,es25.17)') 'schedule', c%sp_mthd, c%sp_division, c%sp_double, c%sp_dblesize, c%time_maximum
     write(*, '(a,9i2)') 'output', c%o_f
     write(*, '(a,es25.17)') 'rec_charac', c%rec_charac
+    if (c%aniso) then
+      allocate(s9(9, nT), n9(9, nT))
+      call tet_tensors(c, nT, tet_phy, s9, n9, ierr)
+      write(*, '(a,9es25.17)') 'tsig1', s9(:, 1)
+      write(*, '(a,9es25.17)') 'tnu1', n9(:, 1)
+      write(*, '(a,9es25.17)') 'tsigN', s9(:, nT)
+      write(*, '(a,9es25.17)') 'tnuN', n9(:, nT)
+    end if
     write(*, '(a,i6)') 'ngates', size(c%time_points)
   else if (trim(mode) == 'write') then
     call get_command_argument(3, a2)

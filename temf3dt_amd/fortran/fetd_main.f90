@@ -21,6 +21,7 @@ program fetd_amd
   logical :: loud
   integer :: narg, k, ierr, nN, nT, nseg, nrows, cap, i, rc
   real(c_double), allocatable :: xyz(:, :), sigma(:), inv_mu(:), row_t(:), row_dt(:), fields(:, :, :)
+  real(c_double), allocatable :: sig9(:, :), nu9(:, :)
   integer(c_int), allocatable :: tet(:, :), tet_phy(:), seg(:, :), rec_tet(:), row_step(:)
   type(cntl_t) :: c
   type(c_ptr) :: ctx
@@ -63,7 +64,12 @@ program fetd_amd
   call read_cntl(trim(root)//'.cntl', c, ierr, msg)
   if (ierr /= 0) call die(msg)
   allocate(sigma(nT), inv_mu(nT), rec_tet(c%rec_num))
-  call tet_materials(c, nT, tet_phy, sigma, inv_mu, ierr)
+  if (c%aniso) then
+    allocate(sig9(9, nT), nu9(9, nT))
+    call tet_tensors(c, nT, tet_phy, sig9, nu9, ierr)
+  else
+    call tet_materials(c, nT, tet_phy, sigma, inv_mu, ierr)
+  end if
   if (ierr /= 0) call die('error - physical domain not found!')
   if (loud) write(*, '(a,i0,a,i0,a,i0,a)') ' mesh: ', nN, ' vertices, ', nT, ' tets, ', nseg, ' source segments'
 
@@ -72,7 +78,11 @@ program fetd_amd
   call check(tf_set_mesh(ctx, int(nN, c_int), xyz, int(nT, c_int), tet), 'tf_set_mesh')
   call check(tf_get_sizes(ctx, nE, nnzU), 'tf_get_sizes')
   if (loud) write(*, '(a,i0,a,i0)') ' edges: ', nE, '  upper nonzeros: ', nnzU
-  call check(tf_set_materials(ctx, sigma, inv_mu), 'tf_set_materials')
+  if (c%aniso) then
+    call check(tf_set_materials_tensor(ctx, sig9, nu9), 'tf_set_materials_tensor')
+  else
+    call check(tf_set_materials(ctx, sigma, inv_mu), 'tf_set_materials')
+  end if
   call check(tf_set_source(ctx, 0_c_int, int(nseg, c_int), seg, int(c%source_type, c_int), &
                            int(c%pulse_type, c_int), c%pulse_width, c%current), 'tf_set_source')
   call check(tf_set_receivers(ctx, int(c%rec_num, c_int), c%rec, c%rec_charac, rec_tet), 'tf_set_receivers')

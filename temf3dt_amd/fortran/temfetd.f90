@@ -41,6 +41,11 @@ module temfetd
       type(c_ptr), value :: ctx
       real(c_double), intent(in) :: sigma(*), inv_mu(*)
     end function
+    integer(c_int) function tf_set_materials_tensor(ctx, sigma9, nu9) bind(C, name='tf_set_materials_tensor')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: sigma9(*), nu9(*)
+    end function
     integer(c_int) function tf_set_source(ctx, isrc, nseg, seg_vex, source_type, pulse_type, pulse_width, &
                                           current) bind(C, name='tf_set_source')
       import :: c_ptr, c_int, c_double

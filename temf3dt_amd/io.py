@@ -153,7 +153,15 @@ def read_cntl(path):
         elif key == "isotropic":
             c["domains"] = [rec.nums(4) for _ in range(int(_fnum(first)))]
         elif key == "anisotropic":
-            raise ValueError("anisotropic media are not supported")
+            # per domain three records (conductivity, permittivity,
+            # permeability): "domain v1 v2 v3 angle1 angle2 angle3"
+            doms = []
+            for _ in range(int(_fnum(first))):
+                recs = [rec.nums(7) for _ in range(3)]
+                doms.append((int(recs[0][0]), recs[0][1:4], recs[0][4:7], recs[1][1:4], recs[1][4:7],
+                             recs[2][1:4], recs[2][4:7]))
+            c["aniso"] = doms
+            c["domains"] = [(d[0], d[1][0], d[3][0], d[5][0]) for d in doms]
         elif key == "boundary condition":
             c["bound_cond"] = int(_fnum(first))
         elif key == "source":
@@ -202,6 +210,47 @@ def materials(tet_phy, domains):
     if not found.all():
         raise ValueError("error - physical domain not found!")
     return sigma, inv_mu
+
+
+def _mat3(a, b):
+    """3x3 product with sums in index order (matches the restated Fortran)."""
+    return [[a[i][0] * b[0][j] + a[i][1] * b[1][j] + a[i][2] * b[2][j] for j in range(3)] for i in range(3)]
+
+
+def aniso_tensor(values, angles_deg, scale=1.0):
+    """model_ctl's tensor (src/FETD_FWD.f90:381-408): R diag(v) R^T with
+    R = Rx Ry Rz from three angles in degrees; as in the reference, the (2,2)
+    entry of Rz takes the cosine of the second angle.  Row-major 3x3 list."""
+    d2r = math.pi / 180.0
+    a1, a2, a3 = (float(x) * d2r for x in angles_deg)
+    dx = [[1.0, 0.0, 0.0], [0.0, math.cos(a1), -math.sin(a1)], [0.0, math.sin(a1), math.cos(a1)]]
+    dy = [[math.cos(a2), 0.0, math.sin(a2)], [0.0, 1.0, 0.0], [-math.sin(a2), 0.0, math.cos(a2)]]
+    dz = [[math.cos(a3), -math.sin(a3), 0.0], [math.sin(a3), math.cos(a2), 0.0], [0.0, 0.0, 1.0]]
+    dm = _mat3(_mat3(dx, dy), dz)
+    pm = [[float(values[0]), 0.0, 0.0], [0.0, float(values[1]), 0.0], [0.0, 0.0, float(values[2])]]
+    dmt = [[dm[j][i] for j in range(3)] for i in range(3)]
+    a = _mat3(_mat3(dm, pm), dmt)
+    return [[a[i][j] * scale for j in range(3)] for i in range(3)]
+
+
+def materials_tensor(tet_phy, aniso):
+    """Per-tet conductivity tensors and curl weights [nT][3][3] from the
+    anisotropic domain table: the curl weight is the entrywise reciprocal of
+    the permeability tensor, zero entries kept (src/FETD_FWD.f90:621-633)."""
+    sig9 = np.empty((len(tet_phy), 3, 3))
+    nu9 = np.empty((len(tet_phy), 3, 3))
+    found = np.zeros(len(tet_phy), bool)
+    for dom, sv, sa, _pv, _pa, mv, ma in aniso:
+        S = np.array(aniso_tensor(sv, sa))
+        Mg = np.array(aniso_tensor(mv, ma, MU0))
+        Nu = np.where(Mg == 0.0, 0.0, 1.0 / np.where(Mg == 0.0, 1.0, Mg))
+        sel = tet_phy == int(dom)
+        sig9[sel] = S
+        nu9[sel] = Nu
+        found |= sel
+    if not found.all():
+        raise ValueError("error - physical domain not found!")
+    return sig9, nu9
 
 
 def fort_e(v, w=15, d=5):

@@ -92,17 +92,26 @@ def write_medit(path, xyz, tet_vex, tet_phy, seg_vex=None, seg_phy=51):
 
 
 def write_cntl(path, receivers, domains, source, sp_division, time_maximum, sp_double, sp_dblesize=2,
-               output=(0, 0, 0, 0, 0, 1, 0, 0, 0), time_points=None, rec_charac=None):
-    """.cntl in the reference grammar (src/FETD_FWD.f90:313-487)."""
+               output=(0, 0, 0, 0, 0, 1, 0, 0, 0), time_points=None, rec_charac=None, aniso=None):
+    """.cntl in the reference grammar (src/FETD_FWD.f90:313-487).  `aniso`:
+    [(domain, sig3, sig_ang3, per3, per_ang3, mag3, mag_ang3)] writes an
+    'anisotropic:' section instead of 'isotropic:'."""
     with open(path, "w") as f:
         f.write(f"receivers: {len(receivers)}\n")
         for r in receivers:
             f.write(f"{float(r[0])!r} {float(r[1])!r} {float(r[2])!r}\n")
         if rec_charac is not None:
             f.write(f"rec_charac: {float(rec_charac)!r}\n")
-        f.write(f"\nisotropic: {len(domains)}\n")
-        for d in domains:
-            f.write(f"{int(d[0])} {float(d[1])!r} {float(d[2])!r} {float(d[3])!r}\n")
+        if aniso:
+            f.write(f"\nanisotropic: {len(aniso)}\n")
+            for d in aniso:
+                for q in range(3):
+                    vals = list(d[1 + 2 * q]) + list(d[2 + 2 * q])
+                    f.write(f"{int(d[0])} " + " ".join(repr(float(v)) for v in vals) + "\n")
+        else:
+            f.write(f"\nisotropic: {len(domains)}\n")
+            for d in domains:
+                f.write(f"{int(d[0])} {float(d[1])!r} {float(d[2])!r} {float(d[3])!r}\n")
         f.write("\nboundary condition: 0\n\n")
         f.write("! TYPE 1 - STEPON / TYPE 2 - IMPULSE / TYPE 3 - STEPOFF\n")
         f.write(f"source: 1\n{int(source[0])} {int(source[1])} {float(source[2])!r} {float(source[3])!r}\n\n")

@@ -56,6 +56,7 @@ class Model:
     sp_dblesize: int = 2
     time_points: list = None
     rec_charac: float = 10.0
+    aniso: list = None        # [(phys, sig3, sig_ang3, per3, per_ang3, mag3, mag_ang3)] or None
 
     @property
     def ntet(self):
@@ -75,7 +76,7 @@ class Model:
         meshgen.write_medit(root + ".mesh", self.xyz, self.tet_vex, self.tet_phy, self.loops[loop])
         meshgen.write_cntl(root + ".cntl", self.receivers, self.domains, self.source, self.sp_division,
                            self.time_maximum, self.sp_double, self.sp_dblesize, time_points=self.time_points,
-                           rec_charac=self.rec_charac)
+                           rec_charac=self.rec_charac, aniso=self.aniso)
 
 
 def build(xs, ys, zs, loops, receivers, blocks=(), host_sigma=0.01, air_sigma=1e-12, **kw):
@@ -128,6 +129,24 @@ def small_model():
     return build(xs, ys, zs, loops=[(-50, 50, -50, 50, 0.0)], receivers=[[0.0, 0.0, -3.0], [10.0, 5.0, 7.0]],
                  host_sigma=0.01, air_sigma=1e-8, source=(1, 1, 1.0e-6, 1.0), sp_division=10,
                  time_maximum=1e-3, sp_double=12, sp_dblesize=2, rec_charac=100.0)
+
+
+def aniso_small_model():
+    """small_model with anisotropic media: a tilted uniaxial earth (0.01 S/m
+    across, 0.03 S/m along an axis rotated 30/20/10 degrees) and a weakly
+    anisotropic, axis-aligned permeability, air isotropic (tests).  The
+    permeability stays axis-aligned: the curl weight is its entrywise
+    reciprocal, which is its inverse only for a diagonal tensor."""
+    m = small_model()
+    aniso = []
+    for ph, sig, er, mr in m.domains:
+        if sig > 1e-6:  # earth
+            aniso.append((ph, (0.01, 0.01, 0.03), (30.0, 20.0, 10.0), (er, er, er), (0.0, 0.0, 0.0),
+                          (mr, mr * 1.05, mr), (0.0, 0.0, 0.0)))
+        else:
+            aniso.append((ph, (sig, sig, sig), (0.0, 0.0, 0.0), (er, er, er), (0.0, 0.0, 0.0),
+                          (mr, mr, mr), (0.0, 0.0, 0.0)))
+    return dataclasses.replace(m, aniso=aniso, time_maximum=2e-4)
 
 
 def plain_3d_model(scale=1.0):

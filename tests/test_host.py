@@ -69,13 +69,25 @@ def _analyze(H, n, nz, leaf):
                 rel=rel, amap=amap, loff=loff)
 
 
-@pytest.mark.parametrize("leaf", [4, 16])
-def test_supernodal_structure_covers_cholesky_fill(leaf):
+def _two_meshes():
+    """Two disjoint meshes of different sizes in one vertex numbering."""
+    from temf3dt_amd import meshgen
+    xa, ta, _ = meshgen.tensor_mesh(np.linspace(0, 5, 6), np.linspace(0, 4, 5), np.linspace(0, 3, 4))
+    xb, tb, _ = meshgen.tensor_mesh(np.linspace(0, 3, 4), np.linspace(0, 2, 3), np.linspace(0, 4, 5))
+    return np.vstack([xa, xb]), np.vstack([ta, tb + len(xa)]), len(xa)
+
+
+@pytest.mark.parametrize("leaf,kind", [(4, "one"), (16, "one"), (4, "two"), (16, "two")])
+def test_supernodal_structure_covers_cholesky_fill(leaf, kind):
     """Every nonzero of the dense Cholesky factor of the permuted matrix lies
     in the supernodal structure; relative indices and the assembly map
-    decode back to the right (row, column)."""
+    decode back to the right (row, column).  For two disjoint meshes the
+    fronts form one tree per mesh and no front mixes the two."""
     from temf3dt_amd import meshgen
-    xyz, tv, _ = meshgen.tensor_mesh(np.linspace(0, 5, 6), np.linspace(0, 4, 5), np.linspace(0, 6, 7))
+    if kind == "one":
+        xyz, tv, _ = meshgen.tensor_mesh(np.linspace(0, 5, 6), np.linspace(0, 4, 5), np.linspace(0, 6, 7))
+    else:
+        xyz, tv, nA = _two_meshes()
     H = _host()
     n, nz, ia, ja = _set_mesh(H, xyz, tv)
     s = _analyze(H, n, nz, leaf)
@@ -114,8 +126,18 @@ def test_supernodal_structure_covers_cholesky_fill(leaf):
             lo, hi = min(a, b), max(a, b)
             k = sn_of[lo]
             f = len(rows_of[k])
+            ld = (f + 15) // 16 * 16  # columns padded to 16 doubles
             off = s["amap"][q] - s["loff"][k]
-            assert off // f == lo - s["c0"][k] and rows_of[k][off % f] == hi
+            assert s["loff"][k] % 16 == 0
+            assert off // ld == lo - s["c0"][k] and off % ld < f and rows_of[k][off % ld] == hi
+    if kind == "two":
+        edg = np.zeros((n, 2), np.int32)
+        H.tfh_get_topology(_P(edg), None, None, None, None)
+        in_b = edg[:, 0] > nA  # 1-based vertex ids; mesh B's edges
+        assert (s["par"] < 0).sum() == 2
+        for k in range(s["nsn"]):
+            cols = perm[s["c0"][k]:s["c0"][k] + s["nc"][k]]
+            assert in_b[cols].all() or not in_b[cols].any()
 
 
 def test_generated_mesh_is_conforming_and_positive():
