@@ -141,6 +141,62 @@ def cpu_baseline(m, counts, sample_steps):
             "pardiso_factor_nnz": int(r["factor_nnz"]), "wall_s": wall}
 
 
+def uneven_bench(steps, n=48):
+    """Independent models of uneven size (20k-300k tets, seeded) stepped in
+    one context (block-diagonal solve, per-model controllers, replayed step
+    graphs) against the same models run one after another, each in its own
+    context, over the same number of steps."""
+    from temf3dt_amd import fetd, workloads
+    ms = workloads.uneven_models(n)
+    u = workloads.union(ms)
+    fe = fetd.Fetd(0)
+    fe.set_mesh(u["xyz"], u["tet_vex"])
+    fe.set_models(u["ntet"], u["nrec"])
+    fe.set_materials(u["sigma"], u["inv_mu"])
+    m0 = ms[0]
+    for i, lp in enumerate(u["loops"]):
+        fe.set_source(lp, *m0.source, isrc=i)
+    fe.set_receivers(u["receivers"], max(m.rec_charac for m in ms))
+    fe.set_schedule(m0.sp_division, m0.sp_double, m0.sp_dblesize, m0.time_maximum)
+    fe.assemble()
+    t = time.time()
+    fe.analyze()
+    t_an = time.time() - t
+    st = fe.factor_stats()
+    print(f"[bench] {n} models: {int(u['ntet'].sum())} tets ({int(u['ntet'].min())}-{int(u['ntet'].max())}), "
+          f"{fe.nE} edges, nnzL {st['nnzL']:.3e}, analyze {t_an:.2f} s", file=sys.stderr, flush=True)
+    out = {}
+    for graphs in (True, False):
+        fe.run_models(max_steps=3, graphs=graphs)  # warm-up
+        t = time.time()
+        fe.run_models(max_steps=steps, graphs=graphs)
+        wall = time.time() - t
+        done = sum(len(fe.model_rows(i)["t"]) - 1 for i in range(n))
+        out["graphs" if graphs else "eager"] = {"wall_s": wall, "model_steps": done, "value": done / wall,
+                                                "timers": fe.timers()}
+        print(f"[bench] one context, graphs={graphs}: {done} model-steps in {wall:.3f} s", file=sys.stderr,
+              flush=True)
+    fe.close()
+    seq_wall, seq_steps = 0.0, 0
+    for m in ms:
+        f1, _ = setup_gpu(m, [0])
+        f1.run(max_steps=3)
+        t = time.time()
+        r = f1.run(max_steps=steps)
+        seq_wall += time.time() - t
+        seq_steps += len(r["t"]) - 1
+        f1.close()
+    out["sequential"] = {"wall_s": seq_wall, "model_steps": seq_steps, "value": seq_steps / seq_wall}
+    print(f"[bench] one context per model, sequential: {seq_steps} model-steps in {seq_wall:.3f} s", file=sys.stderr,
+          flush=True)
+    res = {"metric": "FP64 source-timesteps/sec, independent models of uneven size in one context",
+           "value": out["graphs"]["value"], "unit": "source-timesteps/s", "n_gpus": 1, "steps": steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{n} independent models, {int(u['ntet'].sum())} tets in all", "modes": out,
+                      "over_sequential": out["graphs"]["value"] / out["sequential"]["value"]}}
+    print(json.dumps(res))
+
+
 def copy_probe():
     """Calibration for HBM counters: three device copies of 1 GiB (read 1 GiB,
     write 1 GiB each); FETCH_SIZE / WRITE_SIZE of these dispatches against the
@@ -170,6 +226,9 @@ def main():
     a = ap.parse_args()
     if a.copy_probe:
         copy_probe()
+        return
+    if a.model == "uneven48":
+        uneven_bench(a.steps)
         return
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU sharding of the path is a design in "

@@ -164,6 +164,23 @@ int tf_run(tf_ctx *ctx, int max_steps, int max_rows, int *nrows, int *row_step, 
  * tf_set_replay).  Step-on and impulse sources (pulse types 1, 2). */
 int tf_set_checkpointing(tf_ctx *ctx, int every);
 int tf_adjoint(tf_ctx *ctx, int nrows, const double *w, double *grad_logsigma);
+/* Independent models in one context (no reference counterpart: the
+ * reference runs one model per job).  Concatenate the models' meshes in
+ * tf_set_mesh (each model's vertices one block, blocks in model order; tets in
+ * model order), their per-tet materials in tf_set_materials and their
+ * receivers in tf_set_receivers; tf_set_models(ctx, M, ntet[M], nrec[M])
+ * (after tf_set_mesh, before tf_set_receivers) splits tets and receivers by
+ * model, and tf_set_source(ctx, m, ...) gives model m's wire.  The system is
+ * block diagonal: one tree of fronts per model, one solve per step for all of
+ * them.  tf_run_models steps every model with its own adaptive controller
+ * (time, dt, keep/change decisions and factor slot, src/FETD_FWD.f90:1229-1464)
+ * until each reaches time_maximum (max_steps > 0 caps the steps); with
+ * use_graphs regular steps with the sources off replay captured hipGraphs.
+ * tf_get_model_rows returns model m's rows like tf_run ([row][rec][9]). */
+int tf_set_models(tf_ctx *ctx, int nmodels, const int *ntet, const int *nrec);
+int tf_run_models(tf_ctx *ctx, int max_steps, int use_graphs);
+int tf_get_model_rows(tf_ctx *ctx, int model, int max_rows, int *nrows, int *row_step, double *row_t,
+                      double *row_dt, double *fields);
 /* Replay the controller's decisions instead of computing them: keep[i] is
  * the keep (1) / change (0) outcome of the i-th doubling check (as logged by
  * tf_get_checks of an earlier run); the checks' solves and factorisations

@@ -308,6 +308,7 @@ struct tf_ctx {
     std::vector<FrontDesc> fd_host;       // front descriptors, offsets of slot 0
     int nmodels = 0;                      // 0: a single model
     std::vector<int> m_tet0, m_ntet, m_rec0, m_nrec, m_e0, m_ne;
+    std::vector<int64_t> m_llo, m_lhi;    // each model's range of L storage (within a slot)
     std::vector<int> front_model;
     FrontDesc *d_fd_cur = nullptr, *d_fd_oth = nullptr;  // fronts in each model's current / other slot
     double *d_Lmm = nullptr, *d_Dmm = nullptr;          // both slots of every model
@@ -1175,6 +1176,20 @@ void setup_models(tf_ctx *c) {
             if (model_of_edge(s.perm[s.c0[k] + j]) != m) throw std::runtime_error("a front spans two models");
         c->front_model[k] = m;
     }
+    // a model's fronts are one run of the postorder (one tree per model), so
+    // its factor occupies one range of L storage in either slot
+    c->m_llo.assign(M, INT64_MAX);
+    c->m_lhi.assign(M, 0);
+    for (int k = 0; k < s.nsn; k++) {
+        const int m = c->front_model[k];
+        c->m_llo[m] = std::min<int64_t>(c->m_llo[m], s.loff[k]);
+        c->m_lhi[m] = std::max<int64_t>(c->m_lhi[m], s.loff[k + 1]);
+    }
+    {
+        int64_t span = 0;
+        for (int m = 0; m < M; m++) span += c->m_lhi[m] - c->m_llo[m];
+        if (span != s.ltotal) throw std::runtime_error("a model's fronts are not contiguous");
+    }
     std::vector<int> qm(c->topo.nnz);
     for (int r = 0; r < n; r++) {
         const int m = model_of_edge(r);
@@ -1191,7 +1206,7 @@ void setup_models(tf_ctx *c) {
     c->d_fd_oth = dalloc<FrontDesc>(s.nsn);
     c->d_slot_off = dalloc<int64_t>(M);
     c->d_mcdt = dalloc<double>(M);
-    c->d_mcoef = dalloc<double>(M);
+    c->d_mcoef = dalloc<double>(2 * (size_t)M);
     c->d_mnorm = dalloc<double>(2 * (size_t)M);
     c->d_mstage = dalloc<double>(6 * (size_t)c->nR);
 }
@@ -1220,7 +1235,9 @@ void factor_models(tf_ctx *c, const std::vector<double> &dt, const std::vector<i
     for (int m = 0; m < M; m++) { cdt[m] = 2.0 * dt[m]; off[m] = (int64_t)target[m] * c->sym.ltotal; }
     HIPCHK(hipMemcpyAsync(c->d_mcdt, cdt.data(), sizeof(double) * M, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->d_slot_off, off.data(), sizeof(int64_t) * M, hipMemcpyHostToDevice, c->st));
-    tf::launch_fill0(c->d_Lmm, 2 * c->sym.ltotal, c->st);
+    for (int m = 0; m < M; m++)  // clear the target slot of every model (not its current factor)
+        tf::launch_fill0(c->d_Lmm + (int64_t)target[m] * c->sym.ltotal + c->m_llo[m], c->m_lhi[m] - c->m_llo[m],
+                         c->st);
     tf::launch_fill0(c->d_U, c->sym.utotal, c->st);
     HIPCHK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->st));
     tf::launch_scatter_p_models(c->topo.nnz, c->d_amap, c->d_qmodel, c->d_M, c->d_K, c->d_mcdt, c->d_slot_off,
@@ -1237,10 +1254,12 @@ void rhs_models(tf_ctx *c, int step, const std::vector<double> &t, const std::ve
     const int M = c->nmodels;
     tf::launch_spmv_rhs(c->topo.nE, c->d_fia, c->d_fja, c->d_fval, e2, e3, r, c->st);
     if (step < c->sp_division) {
-        std::vector<double> coef(M);
-        for (int m = 0; m < M; m++)
-            coef[m] = 2.0 * dt[m] * source_didt(c->stype, c->ptype, t[m], c->pulse_width, c->current);
-        HIPCHK(hipMemcpyAsync(c->d_mcoef, coef.data(), sizeof(double) * M, hipMemcpyHostToDevice, c->st));
+        std::vector<double> coef(2 * (size_t)M);
+        for (int m = 0; m < M; m++) {
+            coef[2 * m] = source_didt(c->stype, c->ptype, t[m], c->pulse_width, c->current);
+            coef[2 * m + 1] = 2.0 * dt[m];
+        }
+        HIPCHK(hipMemcpyAsync(c->d_mcoef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, c->st));
         HIPCHK(hipStreamSynchronize(c->st));  // coef lives on the host stack
         tf::launch_source_models(c->nseg, c->d_src_edge, c->d_src_id, c->d_src_w, c->d_mcoef, r, c->st);
     }

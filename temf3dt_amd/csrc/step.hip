@@ -109,15 +109,16 @@ __global__ void k_gather_vals(int64_t n, const int64_t *map, const double *src, 
 
 // ---- independent models in one context ---------------------------------------
 
-// sys_right for several models: r[edge] -= coef[model] * w for every wire
-// entry, coef = 2 dt_m * dI/dt(t_m) of the entry's model (zero once the
-// model's source is off)
+// sys_right for several models: r[edge] -= (2 dt_m) (dI/dt(t_m) w) for
+// every wire entry, with dt_m and t_m of the entry's model; coef holds
+// (dI/dt, 2 dt) per model.  Same rounding as the single-model k_source.
 __global__ void k_source_models(int ns, const int *edge, const int *model, const double *w, const double *coef,
                                 double *r) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < ns) {
-        const int e = edge[i];
-        r[e] = r[e] - coef[model[i]] * w[i];
+        const int e = edge[i], m = model[i];
+        const double s = coef[2 * m] * w[i];
+        r[e] = r[e] - coef[2 * m + 1] * s;
     }
 }
 

@@ -54,6 +54,9 @@ _SIGS = {
     "tf_set_checkpointing": [_P, _I],
     "tf_adjoint": [_P, _I, _P, _P],
     "tf_set_materials_tensor": [_P, _P, _P],
+    "tf_set_models": [_P, _I, _P, _P],
+    "tf_run_models": [_P, _I, _I],
+    "tf_get_model_rows": [_P, _I, _I, _P, _P, _P, _P, _P],
 }
 
 # kernel-timing categories of tf_get_profile (include/temfetd.h)
@@ -259,6 +262,28 @@ class Fetd:
         fields = f[:k].copy()
         return dict(step=steps[:k].copy(), t=t[:k].copy(), dt=dt[:k].copy(),
                     fields=fields[:, 0] if self.nsrc == 1 else fields)
+
+    # -- independent models -------------------------------------------------------
+    def set_models(self, ntet, nrec):
+        ntet = np.ascontiguousarray(ntet, np.int32)
+        nrec = np.ascontiguousarray(nrec, np.int32)
+        self._chk(self.lib.tf_set_models(self.ctx, len(ntet), _ptr(ntet), _ptr(nrec)), "tf_set_models")
+        self.model_nrec = nrec.copy()
+
+    def run_models(self, max_steps=0, graphs=True):
+        self._chk(self.lib.tf_run_models(self.ctx, max_steps, 1 if graphs else 0), "tf_run_models")
+
+    def model_rows(self, m, max_rows=20000):
+        nr = int(self.model_nrec[m])
+        n = _I()
+        steps = np.empty(max_rows, np.int32)
+        t = np.empty(max_rows)
+        dt = np.empty(max_rows)
+        f = np.empty((max_rows, nr, 9))
+        self._chk(self.lib.tf_get_model_rows(self.ctx, m, max_rows, ctypes.byref(n), _ptr(steps), _ptr(t), _ptr(dt),
+                                             _ptr(f)), "tf_get_model_rows")
+        k = min(n.value, max_rows)
+        return dict(step=steps[:k].copy(), t=t[:k].copy(), dt=dt[:k].copy(), fields=f[:k].copy())
 
     # -- adjoint ----------------------------------------------------------------
     def set_checkpointing(self, every):

@@ -198,3 +198,66 @@ def batched_3d_model(nside=8):
     return build(xs, ys, zs, loops=loops, receivers=rec, blocks=[(50, 250, -100, 100, 100, 200, 1.0)],
                  host_sigma=0.01, air_sigma=1e-12, source=(1, 1, 1.0e-7, 1.0), sp_division=10,
                  time_maximum=1.0e-3, sp_double=100, sp_dblesize=2, time_points=gates, rec_charac=20.0)
+
+
+def uneven_model(target_tets, host_sigma, block, seed_name="", time_maximum=1e-2):
+    """One independent model: a half-space with one rectangular block and a
+    500 m square loop on the ground (corners at +-250 m), central receiver
+    1 m above it; the cell size across the loop and the outward growth
+    (1.3-3.0, out to +-50 km) are chosen so that the graded tensor grid has
+    about `target_tets` tets."""
+    best = None
+    for g in (1.3, 1.5, 2.0, 3.0):
+        for k in range(2, 80):
+            h = 500.0 / k
+            xs = axis(-50000, 50000, -250, 250, h, g)
+            zs = axis(-50000, 50000, -2 * h, 250, h, g)
+            nt = 6 * (len(xs) - 1) ** 2 * (len(zs) - 1)
+            if best is None or abs(math.log(nt / target_tets)) < abs(math.log(best[0] / target_tets)):
+                best = (nt, h, xs, zs)
+    _, h, xs, zs = best
+    gates = list(np.logspace(-5, -2, 40))
+    return build(xs, xs, zs, loops=[(-250.0, 250.0, -250.0, 250.0, 0.0)], receivers=[[0.0, 0.0, -1.0]],
+                 blocks=[block], host_sigma=host_sigma, air_sigma=1e-12, source=(1, 1, 1.0e-6, 1.0),
+                 sp_division=10, time_maximum=time_maximum, sp_double=100, sp_dblesize=2, time_points=gates,
+                 rec_charac=1.5 * h)
+
+
+def uneven_models(n=48, seed=3, tmin=2e4, tmax=3e5):
+    """Independent models of uneven size: host sigma log-uniform in
+    [1e-3, 1e-1] S/m, one block with sigma log-uniform in [1e-2, 10] S/m,
+    sides 50-300 m, top 20-300 m deep, tet counts log-uniform in [tmin, tmax]
+    (seeded)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        host = 10 ** rng.uniform(-3, -1)
+        bsig = 10 ** rng.uniform(-2, 1)
+        sx, sy, sz = rng.uniform(50, 300, 3)
+        top = rng.uniform(20, 300)
+        cx, cy = rng.uniform(-150, 150, 2)
+        nt = 10 ** rng.uniform(math.log10(tmin), math.log10(tmax))
+        out.append(uneven_model(nt, host, (cx - sx / 2, cx + sx / 2, cy - sy / 2, cy + sy / 2, top, top + sz, bsig)))
+    return out
+
+
+def union(models):
+    """The models' meshes in one vertex numbering (model blocks in order) for
+    one context: xyz, tets, per-tet sigma and 1/mu, per-model wire segments
+    (global ids), receivers, and the per-model tet and receiver counts."""
+    xyz, tets, sig, imu, loops, recs, ntet, nrec = [], [], [], [], [], [], [], []
+    off = 0
+    for m in models:
+        xyz.append(m.xyz)
+        tets.append(m.tet_vex + off)
+        s, i = m.sigma_inv_mu()
+        sig.append(s)
+        imu.append(i)
+        loops.append(m.loops[0] + off)
+        recs.append(m.receivers)
+        ntet.append(m.ntet)
+        nrec.append(len(m.receivers))
+        off += len(m.xyz)
+    return dict(xyz=np.vstack(xyz), tet_vex=np.ascontiguousarray(np.vstack(tets), np.int32), sigma=np.concatenate(sig),
+                inv_mu=np.concatenate(imu), loops=loops, receivers=np.vstack(recs), ntet=np.array(ntet),
+                nrec=np.array(nrec))
