@@ -46,13 +46,12 @@ void launch_factor_diag(double *L, const FrontDesc *fd, const int *list, int cnt
 void launch_factor_trsm(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
                         int p0, const double *Dn, hipStream_t st);
 void launch_factor_update(double *L, double *U, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
-                          int64_t nwg, int p0, const double *Dn, hipStream_t st);
+                          int64_t nwg, int p0, int mode, const double *Dn, hipStream_t st);
 // factor.hip (inversion)
-void launch_inv_w(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
-                  int p0, double *W, const int64_t *woff, hipStream_t st);
-void launch_inv_apply(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
-                      int p0, const double *W, const int64_t *woff, hipStream_t st);
-void launch_inv_diag(double *L, const FrontDesc *fd, const int *list, int cnt, int p0, hipStream_t st);
+void launch_inv_diag(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
+                     hipStream_t st);
+void launch_inv_merge(int step, double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
+                      int64_t nwg, int b, double *W, const int64_t *woff, hipStream_t st);
 // solve.hip
 void launch_fwd_small(const double *L, const FrontDesc *fd, const int *list, int cnt, const double *b,
                       const int *perm, const int *chp, const int *ch, const int *rel, double *V, double *Y,
@@ -150,7 +149,7 @@ void dfree(T *&p) {
 }
 
 enum LaunchKind {
-    L_EXTADD, L_FFUSED, L_FDIAG, L_FTRSM, L_FUPD, L_IW, L_IAPPLY, L_IDIAG,
+    L_EXTADD, L_FFUSED, L_FDIAG, L_FTRSM, L_FUPD, L_IMERGE_T, L_IMERGE_X, L_IDIAG,
     L_SSMALL, L_SOWN, L_SSTRUCT,
     L_BSMALL, L_BT, L_BX,
     L_GINIT, L_GCHILD, L_GXS, L_MT, L_MTRED, L_MPULL
@@ -167,7 +166,7 @@ int prof_category(int kind) {
     case L_MTRED: return TF_PROF_TILE_RED;
     case L_MPULL: return TF_PROF_PULL;
     case L_FUPD: return TF_PROF_FACTOR_UPDATE;
-    case L_EXTADD: case L_FFUSED: case L_FDIAG: case L_FTRSM: case L_IW: case L_IAPPLY: case L_IDIAG:
+    case L_EXTADD: case L_FFUSED: case L_FDIAG: case L_FTRSM: case L_IMERGE_T: case L_IMERGE_X: case L_IDIAG:
         return TF_PROF_FACTOR_OTHER;
     default: return TF_PROF_SOLVE1;
     }
@@ -512,41 +511,67 @@ void build_plan(tf_ctx *c) {
         int64_t wlev = 0;
         for (int k : grid[l]) {
             kmax = std::max(kmax, (s.nc[k] + TF_PB - 1) / TF_PB);
+            // inversion scratch: the largest merge level's T blocks
+            int64_t need = 0;
+            for (int b = TF_PB; b < s.nc[k]; b *= 2) {
+                int64_t npair = (s.nc[k] - b + 2 * b - 1) / (2 * b);
+                need = std::max(need, npair * inv_ldt(s.nc[k], b) * (int64_t)b);
+            }
             p.woff[k] = wlev;
-            wlev += (int64_t)s.nc[k] * TF_PB;
+            wlev += need;
         }
         p.wsize = std::max(p.wsize, wlev);
+        // panels of TF_PB columns grouped into super-panels of TF_SP columns:
+        // after each panel, its update of the rest of its super-panel
+        // (mode 0); after a super-panel's last panel, one rank-TF_SP update
+        // of everything to its right (mode 1)
         for (int pk = 0; pk < kmax; pk++) {
-            int p0 = pk * TF_PB;
-            std::vector<int> act;
-            std::vector<int64_t> ut, uu;
+            int p0 = pk * TF_PB, P0 = p0 - p0 % TF_SP;
+            std::vector<int> act, a0, a1;
+            std::vector<int64_t> ut, u0, u1;
             for (int k : grid[l])
                 if (s.nc[k] > p0) {
-                    int p1 = std::min(s.nc[k], p0 + TF_PB);
+                    int p1 = std::min(s.nc[k], p0 + TF_PB), pe = std::min(s.nc[k], P0 + TF_SP);
                     int rows = f_of(k) - p1;
                     act.push_back(k);
                     ut.push_back((rows + TF_TRSM_ROWS - 1) / TF_TRSM_ROWS);
-                    int64_t nt = (rows + 127) / 128;  // 128 x 128 matrix-core update tiles
-                    uu.push_back(nt * (nt + 1) / 2);
+                    if (p1 < pe) {
+                        int64_t nt = (rows + TF_UT - 1) / TF_UT, ncj = (pe - p1 + TF_UT - 1) / TF_UT;
+                        a0.push_back(k);
+                        u0.push_back(ncj * nt - ncj * (ncj - 1) / 2);
+                    } else if (f_of(k) > pe) {
+                        int64_t nt = (f_of(k) - pe + TF_UT - 1) / TF_UT;
+                        a1.push_back(k);
+                        u1.push_back(nt * (nt + 1) / 2);
+                    }
                 }
             add(p.factor, L_FDIAG, p0, act, {});
             add(p.factor, L_FTRSM, p0, act, ut);
-            add(p.factor, L_FUPD, p0, act, uu);
+            add(p.factor, L_FUPD, p0, a0, u0);
+            if (!a1.empty()) {
+                add(p.factor, L_FUPD, P0, a1, u1);
+                p.factor.back().aux = 1;
+            }
         }
-        // in-place inversion of every large front's L11, right to left
-        for (int pk = kmax - 1; pk >= 0; pk--) {
-            int p0 = pk * TF_PB;
+        // in-place inversion of every large front's L11 by doubling: all 64 x 64
+        // diagonal blocks, then merges of block pairs of size b = 64, 128, ...
+        {
+            std::vector<int64_t> ud;
+            for (int k : grid[l]) ud.push_back((s.nc[k] + TF_PB - 1) / TF_PB);
+            add(p.factor, L_IDIAG, 0, grid[l], ud);
+        }
+        for (int b = TF_PB;; b *= 2) {
             std::vector<int> act;
-            std::vector<int64_t> ur;
+            std::vector<int64_t> um;
+            const int64_t ntb = (b + TF_UT - 1) / TF_UT;
             for (int k : grid[l])
-                if (s.nc[k] > p0) {
-                    int p1 = std::min(s.nc[k], p0 + TF_PB);
+                if (s.nc[k] > b) {
                     act.push_back(k);
-                    ur.push_back((s.nc[k] - p1 + TF_TB - 1) / TF_TB);
+                    um.push_back((int64_t)((s.nc[k] - b + 2 * b - 1) / (2 * b)) * ntb * ntb);
                 }
-            add(p.factor, L_IW, p0, act, ur);
-            add(p.factor, L_IAPPLY, p0, act, ur);
-            add(p.factor, L_IDIAG, p0, act, {});
+            if (act.empty()) break;
+            add(p.factor, L_IMERGE_T, b, act, um);
+            add(p.factor, L_IMERGE_X, b, act, um);
         }
     }
     // forward solve, bottom-up
@@ -737,16 +762,16 @@ void run_factor(tf_ctx *c, double *L, double *Dn, const FrontDesc *fd) {
             tf::launch_factor_trsm(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, Dn, c->st);
             break;
         case L_FUPD:
-            tf::launch_factor_update(L, c->d_U, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, Dn, c->st);
+            tf::launch_factor_update(L, c->d_U, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, l.aux, Dn, c->st);
             break;
-        case L_IW:
-            tf::launch_inv_w(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, c->d_W, c->d_woff, c->st);
+        case L_IMERGE_T:
+            tf::launch_inv_merge(0, L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, c->d_W, c->d_woff, c->st);
             break;
-        case L_IAPPLY:
-            tf::launch_inv_apply(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, c->d_W, c->d_woff, c->st);
+        case L_IMERGE_X:
+            tf::launch_inv_merge(1, L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, c->d_W, c->d_woff, c->st);
             break;
         case L_IDIAG:
-            tf::launch_inv_diag(L, fd, lst + l.off, l.cnt, l.p0, c->st);
+            tf::launch_inv_diag(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->st);
             break;
         }
         prof_close(c, prof_category(l.kind), pm);

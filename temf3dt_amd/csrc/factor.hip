@@ -9,7 +9,8 @@
 // Work is organised by levels of the supernode tree (all fronts of a level
 // are independent).  Levels with many fronts run one workgroup per front
 // ("fused"); levels with few, large fronts split every 64-column panel into
-// three grid launches: diagonal-block factor, panel solve, trailing update.
+// grid launches (diagonal-block factor, panel solve, trailing updates
+// grouped by super-panels), then invert L11 by recursive doubling.
 #include "tf_device.h"
 
 #define LDS_LD 65
@@ -138,24 +139,23 @@ __device__ void blk_update_tile(double *L, double *U, const FrontDesc &d, int p0
     __syncthreads();
 }
 
-// Trailing update with FP64 matrix cores: C tile of TF_UT x TF_UT (rows rb..,
-// columns cb.. of the trailing block) -= L_rows D L_cols^T over the panel's
-// pw columns.  Four waves, each a 64 x 64 sub-tile of 4 x 4 blocks of
+#define TF_ULD (TF_UT + 16)
+
+// Trailing update on matrix cores, C -= L_rows D L_cols^T, over the panel
+// columns [kbeg, kend) (up to TF_SP of them) for the TF_UT x TF_UT tile
+// (ti, tj) of the region that starts at row/column r0; columns are limited
+// to [r0, cmax).  Four waves, each a 64 x 64 sub-tile of 4 x 4 blocks of
 // v_mfma_f64_16x16x4f64.  Lane layout of that instruction (measured): lane l
 // supplies A[l%16][l/16] and B[l/16][l%16] and receives D[4r + l/16][l%16]
 // in result register r.  Operand roles are chosen so that the result's
 // fast index is the tile row: lane l holds row (l & 15) of columns
 // 4r + (l >> 4), so every store covers four 128-byte row segments.
-#define TF_UT 128
-#define TF_ULD (TF_UT + 16)
-
-__device__ void blk_update_tile_mfma(double *L, double *U, const FrontDesc &d, int p0, const double *Dn,
-                                     int ti, int tj, double *lds) {
-    const int pw = min(TF_PB, d.nc - p0);
-    const int p1 = p0 + pw;
-    const int rb = p1 + ti * TF_UT, cb = p1 + tj * TF_UT;
+__device__ void blk_update_tile_mfma(double *L, double *U, const FrontDesc &d, int kbeg, int kend, int r0,
+                                     int cmax, const double *Dn, int ti, int tj, double *lds) {
+    const int kw = kend - kbeg;
+    const int rb = r0 + ti * TF_UT, cb = r0 + tj * TF_UT;
     const double *base = L + d.loff;
-    const int KH = TF_PB / 2;            // panel columns staged per half
+    const int KH = 32;                   // panel columns staged at a time
     double *Ap = lds;                    // [k][TF_ULD]: panel values of the tile rows
     double *Bp = lds + KH * TF_ULD;      // [k][TF_ULD]: panel values of the tile columns, times d_k
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -166,9 +166,9 @@ __device__ void blk_update_tile_mfma(double *L, double *U, const FrontDesc &d, i
     for (int bi = 0; bi < 4; bi++)
 #pragma unroll
         for (int bj = 0; bj < 4; bj++) acc[bi][bj] = (tf_d4){0.0, 0.0, 0.0, 0.0};
-    for (int kh = 0; kh < TF_PB; kh += KH) {
+    for (int kh = 0; kh < kw; kh += KH) {
         // stage KH panel columns in two batches of 8 loads per operand per thread
-        constexpr int PER = KH * TF_UT / TF_NT;  // 16
+        constexpr int PER = 32 * TF_UT / TF_NT;  // 16
 #pragma unroll
         for (int half = 0; half < PER; half += 8) {
             double va[8], vb[8];
@@ -178,10 +178,10 @@ __device__ void blk_update_tile_mfma(double *L, double *U, const FrontDesc &d, i
                 const int k = kh + idx / TF_UT, i = idx % TF_UT;
                 va[q] = 0.0;
                 vb[q] = 0.0;
-                if (k < pw) {
-                    const double *col = base + (int64_t)(p0 + k) * d.ld;
+                if (k < kw) {
+                    const double *col = base + (int64_t)(kbeg + k) * d.ld;
                     if (rb + i < d.f) va[q] = col[rb + i];
-                    if (cb + i < d.f) vb[q] = col[cb + i];
+                    if (cb + i < cmax) vb[q] = col[cb + i];
                 }
             }
 #pragma unroll
@@ -190,12 +190,12 @@ __device__ void blk_update_tile_mfma(double *L, double *U, const FrontDesc &d, i
                 const int kk = idx / TF_UT, i = idx % TF_UT;
                 const int k = kh + kk;
                 Ap[kk * TF_ULD + i] = va[q];
-                Bp[kk * TF_ULD + i] = (k < pw) ? vb[q] * Dn[d.doff + p0 + k] : 0.0;
+                Bp[kk * TF_ULD + i] = (k < kw) ? vb[q] * Dn[d.doff + kbeg + k] : 0.0;
             }
         }
         __syncthreads();
-        const int kend = min(KH, ((pw - kh) + 3) & ~3);
-        for (int k0 = 0; k0 < kend; k0 += 4) {
+        const int kend4 = min(KH, ((kw - kh) + 3) & ~3);
+        for (int k0 = 0; k0 < kend4; k0 += 4) {
             const int kk = k0 + lk;
             double x[4], y[4];
 #pragma unroll
@@ -218,7 +218,7 @@ __device__ void blk_update_tile_mfma(double *L, double *U, const FrontDesc &d, i
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int c = cb + 64 * wc + 16 * bj + 4 * r + lk;
-            cp[r] = (c < d.f) ? front_col(L, U, d, c) : nullptr;
+            cp[r] = (c < cmax) ? front_col(L, U, d, c) : nullptr;
 #pragma unroll
             for (int bi = 0; bi < 4; bi++) {
                 const int row = rb + 64 * wr + 16 * bi + l16;
@@ -274,85 +274,90 @@ __device__ void blk_invert_diag(double *L, const FrontDesc &d, int p0, double *l
     __syncthreads();
 }
 
-// Blocked in-place inversion of L11, right to left by panels.  For panel
-// p (columns [p0,p1)) with the trailing block already inverted (X22):
-//   W   = X22 L21p        (rows [p1,nc))            k_inv_w
-//   X21 = -W Xpp, Xpp = Lpp^-1                        k_inv_apply
-//   Lpp <- Xpp                                        k_inv_diag
-__device__ void blk_inv_w(const double *L, const FrontDesc &d, int p0, int R0, double *W, double *lds) {
-    const int pw = min(TF_PB, d.nc - p0);
-    const int p1 = p0 + pw;
-    const int nq = d.nc - p1;
-    const double *base = L + d.loff;
-    double *Xt = lds;                // [k][MLD rows]: X22[R0+r][k0+k]
-    double *Lt = lds + TF_PB * MLD;  // [k][MLD cols]: L21p[k0+k][c]
-    tf_d4 acc[2][2];
-    acc_zero(acc);
-    for (int k0 = p1; k0 <= R0; k0 += TF_PB) {
-#pragma unroll 8
-        for (int idx = threadIdx.x; idx < TF_PB * TF_TB; idx += TF_NT) {
-            int kk = idx / TF_TB, r = idx % TF_TB;
-            int k = k0 + kk, row = R0 + r;
-            double xv = 0.0, lv = 0.0;
-            if (k < d.nc && row < d.nc && k < row) xv = base[(int64_t)k * d.ld + row];
-            if (k < d.nc && r < pw) lv = base[(int64_t)(p0 + r) * d.ld + k];
-            Xt[kk * MLD + r] = xv;
-            Lt[kk * MLD + r] = lv;
-        }
-        __syncthreads();
-        mfma64_acc(Xt, MLD, Lt, MLD, TF_PB, acc);
-        __syncthreads();
-    }
-#pragma unroll
-    for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int c = ACC_COL(bj, r);
-            if (c >= pw) continue;
-#pragma unroll
-            for (int bi = 0; bi < 2; bi++) {
-                const int row = R0 + ACC_ROW(bi);
-                if (row < d.nc)
-                    W[(int64_t)c * nq + (row - p1)] = base[(int64_t)(p0 + c) * d.ld + row] + acc[bi][bj][r];
-            }
-        }
-}
+// In-place inversion of the unit-lower L11 of a large front by recursive
+// doubling (the diagonal positions hold D and are left alone; the inverse's
+// unit diagonal stays implicit).  Every 64 x 64 diagonal block is inverted
+// first; then blocks of size b = 64, 128, 256, ... are merged in pairs
+// I = [s, s+b), J = [s+b, min(s+2b, nc)), s = 2b*pair, with
+//   inv([L_II 0; L_JI L_JJ]) = [X_II 0; -X_JJ L_JI X_II  X_JJ]:
+//   step 0: T    = L_JI X_II    (scratch, |J| x b, column-major, ld ldT)
+//   step 1: X_JI = -X_JJ T      (in place over L_JI)
+// Each step is a grid of 128 x 128 matrix-core tiles, so the top merges of
+// a front with thousands of columns fill the whole chip in a handful of
+// launches.
 
-__device__ void blk_inv_apply(double *L, const FrontDesc &d, int p0, int R0, const double *W, double *lds) {
-    const int pw = min(TF_PB, d.nc - p0);
-    const int p1 = p0 + pw;
-    const int nq = d.nc - p1;
-    double *X = lds;                              // Xpp [c*65 + r]
-    double *XT = lds + TF_PB * LDS_LD;            // XT[k*MLD + c] = Xpp[k][c], unit diagonal
-    double *Wt = XT + TF_PB * MLD;                // [k][MLD rows]
-    blk_invert_to_lds(L, d, p0, X);
-    for (int idx = threadIdx.x; idx < TF_PB * TF_PB; idx += blockDim.x) {
-        int k = idx / TF_PB, c = idx % TF_PB;
-        XT[k * MLD + c] = (k == c) ? (k < pw ? 1.0 : 0.0) : X[c * LDS_LD + k];
-    }
-    for (int idx = threadIdx.x; idx < TF_PB * TF_TB; idx += blockDim.x) {
-        int k = idx / TF_TB, r = idx % TF_TB;
-        int row = R0 + r;
-        Wt[k * MLD + r] = (k < pw && row < d.nc) ? W[(int64_t)k * nq + (row - p1)] : 0.0;
-    }
-    __syncthreads();
-    tf_d4 acc[2][2];
-    acc_zero(acc);
-    mfma64_acc(Wt, MLD, XT, MLD, (pw + 3) & ~3, acc);
-    double *base = L + d.loff;
+// 128 x 128 tile (rows rb.., columns cb..) of op(A) op(B) over k in [kb, ke)
+// into acc (same fragment layout as the trailing update).  A(i, k) is read
+// at A[k*lda + i] and B(k, j) at B[j*ldb + k]; UA/UB make that operand unit
+// lower triangular (1 on the diagonal, 0 above it, stored values below).
+// B is staged with its k index fastest (coalesced) and an XOR swizzle on the
+// column index, so that neither the staging writes nor the fragment reads
+// pile onto a few LDS banks.
+template <bool UA, bool UB>
+__device__ void inv_tile(const double *A, int lda, const double *B, int ldb, int M, int N, int kb, int ke, int rb,
+                         int cb, double *lds, tf_d4 (&acc)[4][4]) {
+    const int KH = 32;
+    double *Ap = lds;                // [k][TF_ULD]: A(rb + i, k)
+    double *Bp = lds + KH * TF_ULD;  // [k][TF_ULD]: B(k, cb + (j ^ (k & 15)))
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    const int l16 = lane & 15, lk = lane >> 4;
 #pragma unroll
-    for (int bj = 0; bj < 2; bj++)
+    for (int bi = 0; bi < 4; bi++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int c = ACC_COL(bj, r);
-            if (c >= pw) continue;
+        for (int bj = 0; bj < 4; bj++) acc[bi][bj] = (tf_d4){0.0, 0.0, 0.0, 0.0};
+    for (int k0 = kb; k0 < ke; k0 += KH) {
+        constexpr int PER = 32 * TF_UT / TF_NT;  // 16
 #pragma unroll
-            for (int bi = 0; bi < 2; bi++) {
-                const int row = R0 + ACC_ROW(bi);
-                if (row < d.nc) base[(int64_t)(p0 + c) * d.ld + row] = -acc[bi][bj][r];
+        for (int half = 0; half < PER; half += 8) {
+            double va[8], vb[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int idx = (half + q) * TF_NT + threadIdx.x;
+                {   // A: row index fastest
+                    const int k = k0 + idx / TF_UT, i = rb + idx % TF_UT;
+                    double v = 0.0;
+                    if (k < ke && i < M) {
+                        if (!UA) v = A[(int64_t)k * lda + i];
+                        else v = (i > k) ? A[(int64_t)k * lda + i] : (i == k ? 1.0 : 0.0);
+                    }
+                    va[q] = v;
+                }
+                {   // B: k index fastest
+                    const int k = k0 + (idx & (KH - 1)), j = cb + idx / KH;
+                    double v = 0.0;
+                    if (k < ke && j < N) {
+                        if (!UB) v = B[(int64_t)j * ldb + k];
+                        else v = (k > j) ? B[(int64_t)j * ldb + k] : (k == j ? 1.0 : 0.0);
+                    }
+                    vb[q] = v;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int idx = (half + q) * TF_NT + threadIdx.x;
+                Ap[(idx / TF_UT) * TF_ULD + idx % TF_UT] = va[q];
+                const int kk = idx & (KH - 1), jj = idx / KH;
+                Bp[kk * TF_ULD + (jj ^ (kk & 15))] = vb[q];
             }
         }
-    __syncthreads();
+        __syncthreads();
+        const int kn = min(KH, ((ke - k0) + 3) & ~3);
+        for (int kq = 0; kq < kn; kq += 4) {
+            const int kk = kq + lk;
+            double x[4], y[4];
+#pragma unroll
+            for (int bj = 0; bj < 4; bj++) x[bj] = Bp[kk * TF_ULD + ((64 * wc + 16 * bj + l16) ^ (kk & 15))];
+#pragma unroll
+            for (int bi = 0; bi < 4; bi++) y[bi] = Ap[kk * TF_ULD + 64 * wr + 16 * bi + l16];
+#pragma unroll
+            for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+                for (int bj = 0; bj < 4; bj++)
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[bj], y[bi], acc[bi][bj], 0, 0, 0);
+        }
+        __syncthreads();
+    }
 }
 
 // ---- kernels ---------------------------------------------------------------
@@ -429,43 +434,88 @@ __global__ void __launch_bounds__(TF_NT) k_factor_trsm(double *L, const FrontDes
     blk_trsm(L, d, p0, r0, Dn, lds);
 }
 
+// Updates of a super-panel [P0, P0 + TF_SP) of TF_PB-column panels:
+//   mode 0 (after panel p0): the rest of the super-panel's own columns,
+//          rows from p1 down, with panel p0 only;
+//   mode 1 (p0 = P0, after its last panel): everything right of the
+//          super-panel with all of its columns at once.
+// The trailing block is read and written once per super-panel instead of
+// once per panel.
 __global__ void __launch_bounds__(TF_NT) k_factor_update(double *L, double *U, const FrontDesc *fd,
                                                          const int *list, const int64_t *pref,
-                                                         int cnt, int p0, const double *Dn) {
-    __shared__ double lds[TF_PB * TF_ULD];
+                                                         int cnt, int p0, int mode, const double *Dn) {
+    __shared__ double lds[2 * 32 * TF_ULD];
     const int i = seg_find(pref, cnt, blockIdx.x);
     const FrontDesc d = fd[list[i]];
-    int ti, tj;
-    tri_index((int64_t)blockIdx.x - pref[i], ti, tj);
-    blk_update_tile_mfma(L, U, d, p0, Dn, ti, tj, lds);
+    const int64_t u = (int64_t)blockIdx.x - pref[i];
+    if (mode == 0) {
+        const int P0 = p0 - p0 % TF_SP;
+        const int p1 = min(d.nc, p0 + TF_PB), pe = min(d.nc, P0 + TF_SP);
+        const int nt = (d.f - p1 + TF_UT - 1) / TF_UT;
+        int tj = 0, ti = (int)u;
+        while (ti >= nt - tj) { ti -= nt - tj; tj++; }
+        ti += tj;
+        blk_update_tile_mfma(L, U, d, p0, p1, p1, pe, Dn, ti, tj, lds);
+    } else {
+        const int pe = min(d.nc, p0 + TF_SP);
+        int ti, tj;
+        tri_index(u, ti, tj);
+        blk_update_tile_mfma(L, U, d, p0, pe, pe, d.f, Dn, ti, tj, lds);
+    }
 }
 
-__global__ void __launch_bounds__(TF_NT) k_inv_w(const double *L, const FrontDesc *fd, const int *list,
-                                                 const int64_t *pref, int cnt, int p0, double *W,
-                                                 const int64_t *woff) {
-    __shared__ double lds[2 * TF_PB * MLD];
-    const int i = seg_find(pref, cnt, blockIdx.x);
-    const int s = list[i];
-    const FrontDesc d = fd[s];
-    const int p1 = min(d.nc, p0 + TF_PB);
-    blk_inv_w(L, d, p0, p1 + (int)(blockIdx.x - pref[i]) * TF_TB, W + woff[s], lds);
-}
-
-__global__ void __launch_bounds__(TF_NT) k_inv_apply(double *L, const FrontDesc *fd, const int *list,
-                                                     const int64_t *pref, int cnt, int p0, const double *W,
-                                                     const int64_t *woff) {
-    __shared__ double lds[TF_PB * LDS_LD + 2 * TF_PB * MLD];
-    const int i = seg_find(pref, cnt, blockIdx.x);
-    const int s = list[i];
-    const FrontDesc d = fd[s];
-    const int p1 = min(d.nc, p0 + TF_PB);
-    blk_inv_apply(L, d, p0, p1 + (int)(blockIdx.x - pref[i]) * TF_TB, W + woff[s], lds);
-}
-
-__global__ void __launch_bounds__(TF_NT) k_inv_diag(double *L, const FrontDesc *fd, const int *list, int p0) {
+// All 64 x 64 diagonal blocks of the listed fronts' L11, one per workgroup.
+__global__ void __launch_bounds__(TF_NT) k_inv_diag(double *L, const FrontDesc *fd, const int *list,
+                                                    const int64_t *pref, int cnt) {
     __shared__ double lds[TF_PB * LDS_LD];
-    const FrontDesc d = fd[list[blockIdx.x]];
-    blk_invert_diag(L, d, p0, lds);
+    const int i = seg_find(pref, cnt, blockIdx.x);
+    const FrontDesc d = fd[list[i]];
+    blk_invert_diag(L, d, (int)(blockIdx.x - pref[i]) * TF_PB, lds);
+}
+
+// One merge step at block size b (see inv_tile); work unit u of a front is
+// (pair, ti, tj) with ntb x ntb tiles of 128 per pair, ntb = ceil(b / 128).
+template <int STEP>
+__global__ void __launch_bounds__(TF_NT) k_inv_merge(double *L, const FrontDesc *fd, const int *list,
+                                                     const int64_t *pref, int cnt, int b, double *W,
+                                                     const int64_t *woff) {
+    __shared__ double lds[2 * 32 * TF_ULD];
+    const int i = seg_find(pref, cnt, blockIdx.x);
+    const int sn = list[i];
+    const FrontDesc d = fd[sn];
+    const int64_t u = (int64_t)blockIdx.x - pref[i];
+    const int ntb = (b + TF_UT - 1) / TF_UT;
+    const int pair = (int)(u / (ntb * ntb)), t = (int)(u % (ntb * ntb));
+    const int ti = t / ntb, tj = t % ntb;
+    const int I0 = 2 * b * pair, J0 = I0 + b, Jn = min(b, d.nc - J0);
+    const int rb = ti * TF_UT, cb = tj * TF_UT;
+    if (rb >= Jn) return;  // whole workgroup: the last pair is short
+    const int ldT = inv_ldt(d.nc, b);
+    double *T = W + woff[sn] + (int64_t)pair * ldT * b;
+    double *base = L + d.loff;
+    tf_d4 acc[4][4];
+    if (STEP == 0)  // T = L_JI X_II; X_II(k, c) = 0 for k < c
+        inv_tile<false, true>(base + (int64_t)I0 * d.ld + J0, d.ld, base + (int64_t)I0 * d.ld + I0, d.ld, Jn, b,
+                              cb, b, rb, cb, lds, acc);
+    else  // X_JI = -X_JJ T; X_JJ(i, k) = 0 for k > i
+        inv_tile<true, false>(base + (int64_t)J0 * d.ld + J0, d.ld, T, ldT, Jn, b, 0, min(rb + TF_UT, Jn), rb,
+                              cb, lds, acc);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    const int l16 = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int c = cb + 64 * wc + 16 * bj + 4 * r + lk;
+            if (c >= b) continue;
+            double *col = (STEP == 0) ? T + (int64_t)c * ldT : base + (int64_t)(I0 + c) * d.ld + J0;
+#pragma unroll
+            for (int bi = 0; bi < 4; bi++) {
+                const int row = rb + 64 * wr + 16 * bi + l16;
+                if (row < Jn) col[row] = (STEP == 0) ? acc[bi][bj][r] : -acc[bi][bj][r];
+            }
+        }
 }
 
 // ---- host launchers --------------------------------------------------------
@@ -508,27 +558,24 @@ void launch_factor_trsm(double *L, const FrontDesc *fd, const int *list, const i
     hipLaunchKernelGGL(k_factor_trsm, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt,
                        p0, Dn);
 }
-void launch_factor_update(double *L, double *U, const FrontDesc *fd, const int *list,
-                          const int64_t *pref, int cnt, int64_t nwg, int p0, const double *Dn,
-                          hipStream_t st) {
-    if (nwg <= 0) return;
-    hipLaunchKernelGGL(k_factor_update, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, U, fd, list, pref,
-                       cnt, p0, Dn);
+void launch_factor_update(double *L, double *U, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
+                          int64_t nwg, int p0, int mode, const double *Dn, hipStream_t st) {
+    if (nwg > 0)
+        hipLaunchKernelGGL(k_factor_update, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, U, fd, list, pref, cnt, p0, mode,
+                           Dn);
 }
 
-void launch_inv_w(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
-                  int64_t nwg, int p0, double *W, const int64_t *woff, hipStream_t st) {
-    if (nwg > 0)
-        hipLaunchKernelGGL(k_inv_w, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt, p0, W, woff);
+void launch_inv_diag(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
+                     hipStream_t st) {
+    if (nwg > 0) hipLaunchKernelGGL(k_inv_diag, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt);
 }
-void launch_inv_apply(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
-                      int64_t nwg, int p0, const double *W, const int64_t *woff, hipStream_t st) {
-    if (nwg > 0)
-        hipLaunchKernelGGL(k_inv_apply, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt, p0, W,
-                           woff);
-}
-void launch_inv_diag(double *L, const FrontDesc *fd, const int *list, int cnt, int p0, hipStream_t st) {
-    if (cnt > 0) hipLaunchKernelGGL(k_inv_diag, dim3(cnt), dim3(TF_NT), 0, st, L, fd, list, p0);
+void launch_inv_merge(int step, double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
+                      int64_t nwg, int b, double *W, const int64_t *woff, hipStream_t st) {
+    if (nwg <= 0) return;
+    if (step == 0)
+        hipLaunchKernelGGL(k_inv_merge<0>, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt, b, W, woff);
+    else
+        hipLaunchKernelGGL(k_inv_merge<1>, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt, b, W, woff);
 }
 
 }  // namespace tf

@@ -8,6 +8,12 @@
 #define TF_TB 64       // tile edge of trailing-update tiles
 #define TF_NT 256      // threads per workgroup
 #define TF_TRSM_ROWS 256
+#define TF_UT 128      // tile edge of the matrix-core trailing update
+#define TF_SP 256      // super-panel: panel columns whose trailing update is applied at once
+
+// L11 inversion by doubling: leading dimension of the scratch block T (|J| x b)
+// of a merge at block size b in a front of nc columns (|J| <= min(b, nc - b)).
+__host__ __device__ inline int inv_ldt(int nc, int b) { return ((nc - b < b ? nc - b : b) + 15) & ~15; }
 
 // One multifrontal front (= supernode).  Its columns are the contiguous new
 // indices [c0, c0+nc); its rows are those columns followed by the m struct
