@@ -188,6 +188,19 @@ int tf_get_model_rows(tf_ctx *ctx, int model, int max_rows, int *nrows, int *row
  * (src/FETD_FWD.f90:1380-1417).  No reference counterpart: it makes runs with
  * different source sets follow one schedule. */
 int tf_set_replay(tf_ctx *ctx, int nchecks, const int *keep);
+/* Per-tet field slices: the .cntl key vtkout (src/FETD_FWD.f90:476-484) with
+ * vtk_field_save (:1659-1711) and solution_output's slice trigger
+ * (:1642-1656).  tf_set_slices gives the slice times before tf_run; during
+ * the run every output row, while a slice is pending, evaluates E and curl E
+ * at each tet's centroid and accumulates B += curl E * dt (curl E here is
+ * +curl, the opposite sign of the receivers' dB/dt; the reference writes
+ * absolute values).  A pending slice is taken at the first row with
+ * t >= its time.  tf_get_slice copies slice i as fields9[nT][9] (E, curl E,
+ * B) with the step and time of that row; it fails for a slice the run did
+ * not reach.  Single-source contexts (tf_run) only; nslice = 0 turns slices
+ * off. */
+int tf_set_slices(tf_ctx *ctx, int nslice, const double *times);
+int tf_get_slice(tf_ctx *ctx, int i, double *fields9, int *step, double *t);
 /* Doubling-check log of the last tf_run: step, err, keep flag. */
 int tf_get_checks(const tf_ctx *ctx, int max, int *n, int *step, double *err, int *keep);
 /* Timers of the last tf_run in seconds: assemble, factor, solve, spmv+rhs,

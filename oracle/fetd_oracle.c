@@ -212,6 +212,17 @@ struct fo_model {
     long long factor_nnz;
     int nreplay;
     int *replay; /* forced keep/change decisions of the doubling checks */
+    /* small-loop variant (src_small/FETD_FWD.f90:314-328,1526,1540) */
+    int sig_rescale;
+    double sig_parameter;
+    /* per-tet field slices (vtkout, src/FETD_FWD.f90:476-484,1642-1711) */
+    int nslice;
+    double *slice_t;      /* requested times */
+    int *slice_done, *slice_step;
+    double *slice_tt;     /* time of the output row that took the slice */
+    double *slice_f;      /* [slice][T][9]: E, curl E, running sum of curl E * dt */
+    double *tet_ed;       /* [T][6] current E and curl E */
+    double *tet_b;        /* [T][3] */
 };
 
 static void *xcalloc(size_t n, size_t s) {
@@ -230,7 +241,8 @@ void fo_free(fo_model *m) {
                     m->tet_len, m->rec_xyz, m->rec_tet, m->n_arr, m->d_arr, m->sig, m->per,
                     m->mag, m->src_edg, m->src_drc, m->gates, m->ia, m->ja, m->av, m->bv,
                     m->cv, m->ia0, m->ja0, m->bfull, m->row_step, m->row_t, m->row_dt,
-                    m->row_f, m->chk_step, m->chk_keep, m->chk_err, m->replay, m->sig9, m->nu9};
+                    m->row_f, m->chk_step, m->chk_keep, m->chk_err, m->replay, m->sig9, m->nu9,
+                    m->slice_t, m->slice_done, m->slice_step, m->slice_tt, m->slice_f, m->tet_ed, m->tet_b};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free(ptrs[i]);
     free(m);
 }
@@ -746,6 +758,16 @@ static int read_cntl(fo_model *m, const char *path) {
                 if (!read_nums(&r, v, 4)) goto bad;
                 dom_ind[i] = (int)v[0]; dsig[i] = v[1]; dper[i] = v[2]; dmag[i] = v[3];
             }
+            if (m->sig_rescale) {
+                /* src_small/FETD_FWD.f90:314-328: the parameter is the last
+                 * domain's sigma (the branch meant to take the minimum is
+                 * unreachable), inverted; domains 2..n are scaled by it */
+                double sp = 1.0;
+                for (int i = 1; i < m->ndom; i++) sp = dsig[i];
+                sp = 1.0 / sp;
+                for (int i = 1; i < m->ndom; i++) dsig[i] = dsig[i] * sp;
+                m->sig_parameter = sp;
+            }
         } else if (!strcmp(code, "anisotropic")) {
             /* three records per domain: sigma, permittivity, permeability,
              * each "domain v1 v2 v3 angle1 angle2 angle3" */
@@ -879,8 +901,14 @@ bad3:
 }
 
 fo_model *fo_load_files(const char *mesh_path, const char *cntl_path, int verbose) {
+    return fo_load_files_variant(mesh_path, cntl_path, verbose, 0);
+}
+
+fo_model *fo_load_files_variant(const char *mesh_path, const char *cntl_path, int verbose, int sig_rescale) {
     fo_model *m = xcalloc(1, sizeof(fo_model));
     m->verbose = verbose;
+    m->sig_rescale = sig_rescale;
+    m->sig_parameter = 1.0;
     if (read_mesh(m, mesh_path)) { fo_free(m); return NULL; }
     build_topology(m);
     if (cntl_path && read_cntl(m, cntl_path)) { fo_free(m); return NULL; }
@@ -1081,10 +1109,13 @@ static void write_row(FILE *f, const fo_model *m, int step, double t, const doub
     fprintf(f, "%6d", step);
     fort_e(s, t, 15, 5);
     fprintf(f, "%s  ", s);
+    /* small-loop variant: value / sp**2 * pi**log10(sp) (src_small/FETD_FWD.f90:1526,1540) */
+    const double sp2 = pow(m->sig_parameter, 2.0), spk = pow(PI, log10(m->sig_parameter));
     for (int r = 0; r < m->nR; r++) {
         for (int k = 0; k < 9; k++)
             if (m->o_f[k]) {
-                fort_e(s, vals9[9 * r + k], 15, 5);
+                const double val = m->sig_rescale ? vals9[9 * r + k] / sp2 * spk : vals9[9 * r + k];
+                fort_e(s, val, 15, 5);
                 fputs(s, f);
             }
         if (r < m->nR - 1) fputs("  ", f);
@@ -1098,6 +1129,65 @@ typedef struct {
     int *gate_ok;
     int out_enabled;
 } outctx;
+
+/* vtk_field_save (src/FETD_FWD.f90:1659-1711): E and curl E at every tet's
+ * centroid, B += curl E * dt; on every output row while a slice is pending
+ * (:1642-1656).  A slice is taken at the first row with t >= its time. */
+void fo_tet_fields(const fo_model *m, const double *e, double *ed) {
+#pragma omp parallel for schedule(static)
+    for (int it = 0; it < m->nT; it++) {
+        const int *tv = &m->tet_vex[4 * it];
+        double cp[3];
+        for (int k = 0; k < 3; k++)
+            cp[k] = (m->xyz[3 * (tv[0] - 1) + k] + m->xyz[3 * (tv[1] - 1) + k] + m->xyz[3 * (tv[2] - 1) + k] +
+                     m->xyz[3 * (tv[3] - 1) + k]) / 4.0;
+        const double *a = &m->ca[4 * it], *b = &m->cb[4 * it], *c = &m->cc[4 * it], *d = &m->cd[4 * it];
+        const double v6 = 6.0 * m->vol[it], v62 = v6 * v6;
+        double ef[3] = {0, 0, 0}, df[3] = {0, 0, 0};
+        for (int j = 0; j < 6; j++) {
+            const int j1 = IND_TET[j][0], j2 = IND_TET[j][1];
+            const double L1 = a[j1] + b[j1] * cp[0] + c[j1] * cp[1] + d[j1] * cp[2];
+            const double L2 = a[j2] + b[j2] * cp[0] + c[j2] * cp[1] + d[j2] * cp[2];
+            const double len = m->tet_len[6 * it + j];
+            const double nv[3] = {(L1 * b[j2] - L2 * b[j1]) * len / v62, (L1 * c[j2] - L2 * c[j1]) * len / v62,
+                                  (L1 * d[j2] - L2 * d[j1]) * len / v62};
+            const double dv[3] = {2.0 * (c[j1] * d[j2] - c[j2] * d[j1]) * len / v62,
+                                  2.0 * (b[j2] * d[j1] - b[j1] * d[j2]) * len / v62,
+                                  2.0 * (b[j1] * c[j2] - b[j2] * c[j1]) * len / v62};
+            const double ev = e[m->tet_edg[6 * it + j] - 1];
+            const double drc = (double)m->tet_drc[6 * it + j];
+            for (int k = 0; k < 3; k++) {
+                ef[k] += nv[k] * drc * ev;
+                df[k] += dv[k] * drc * ev;
+            }
+        }
+        for (int k = 0; k < 3; k++) {
+            ed[6 * (size_t)it + k] = ef[k];
+            ed[6 * (size_t)it + 3 + k] = df[k];
+        }
+    }
+}
+
+static void slice_output(fo_model *m, int step, double t, double dt, const double *e) {
+    int pending = 0;
+    for (int i = 0; i < m->nslice; i++) pending |= !m->slice_done[i];
+    if (!pending) return;
+    if (step == 0) memset(m->tet_b, 0, sizeof(double) * 3 * (size_t)m->nT);
+    fo_tet_fields(m, e, m->tet_ed);
+    for (size_t it = 0; it < (size_t)m->nT; it++)
+        for (int k = 0; k < 3; k++) m->tet_b[3 * it + k] = m->tet_b[3 * it + k] + m->tet_ed[6 * it + 3 + k] * dt;
+    for (int i = 0; i < m->nslice; i++)
+        if (!m->slice_done[i] && t >= m->slice_t[i]) {
+            double *o = &m->slice_f[(size_t)i * m->nT * 9];
+            for (size_t it = 0; it < (size_t)m->nT; it++) {
+                memcpy(&o[9 * it], &m->tet_ed[6 * it], 6 * sizeof(double));
+                memcpy(&o[9 * it + 6], &m->tet_b[3 * it], 3 * sizeof(double));
+            }
+            m->slice_done[i] = 1;
+            m->slice_step[i] = step;
+            m->slice_tt[i] = t;
+        }
+}
 
 /* solution_output (src/FETD_FWD.f90:1513-1657), receiver and file parts */
 static void solution_output(outctx *o, int step, double t, double dt, const double *e) {
@@ -1134,6 +1224,7 @@ static void solution_output(outctx *o, int step, double t, double dt, const doub
     m->row_t[r] = t;
     m->row_dt[r] = dt;
     if (o->f1) write_row(o->f1, m, step, t, f);
+    if (m->nslice > 0) slice_output(m, step, t, dt, e);
     for (int g = 0; g < m->n_gate; g++) {
         if (t >= m->gates[g] && !o->gate_ok[g]) {
             if (r == 0) break;
@@ -1347,6 +1438,7 @@ int fo_run(fo_model *m, const char *field_path, const char *field2_path, int max
     if (field2_path && m->n_gate > 0) o.f2 = fopen(field2_path, "w");
     m->nrows = 0;
     m->nchk = 0;
+    for (int i = 0; i < m->nslice; i++) m->slice_done[i] = 0;
     int rc;
     if (m->pulse_type == 3) {
         double *res = xcalloc((size_t)m->nE, sizeof(double));
@@ -1451,6 +1543,30 @@ void fo_set_replay(fo_model *m, int n, const int *keep) {
 void fo_set_sigma(fo_model *m, const double *sigma) {
     memcpy(m->sig, sigma, sizeof(double) * (size_t)m->nT);
 }
+void fo_set_slices(fo_model *m, int n, const double *times) {
+    free(m->slice_t); free(m->slice_done); free(m->slice_step); free(m->slice_tt); free(m->slice_f);
+    free(m->tet_ed); free(m->tet_b);
+    m->slice_t = NULL; m->slice_done = NULL; m->slice_step = NULL; m->slice_tt = NULL; m->slice_f = NULL;
+    m->tet_ed = NULL; m->tet_b = NULL;
+    m->nslice = n > 0 ? n : 0;
+    if (!m->nslice) return;
+    m->slice_t = xcalloc((size_t)n, sizeof(double));
+    memcpy(m->slice_t, times, sizeof(double) * (size_t)n);
+    m->slice_done = xcalloc((size_t)n, sizeof(int));
+    m->slice_step = xcalloc((size_t)n, sizeof(int));
+    m->slice_tt = xcalloc((size_t)n, sizeof(double));
+    m->slice_f = xcalloc((size_t)n * m->nT * 9, sizeof(double));
+    m->tet_ed = xcalloc((size_t)m->nT * 6, sizeof(double));
+    m->tet_b = xcalloc((size_t)m->nT * 3, sizeof(double));
+}
+int fo_get_slice(const fo_model *m, int i, double *f9, int *step, double *t) {
+    if (i < 0 || i >= m->nslice || !m->slice_done[i]) return -1;
+    if (f9) memcpy(f9, &m->slice_f[(size_t)i * m->nT * 9], sizeof(double) * 9 * (size_t)m->nT);
+    if (step) *step = m->slice_step[i];
+    if (t) *t = m->slice_tt[i];
+    return 0;
+}
+double fo_sig_parameter(const fo_model *m) { return m->sig_parameter; }
 int fo_num_rows(const fo_model *m) { return m->nrows; }
 void fo_get_rows(const fo_model *m, int *step, double *t, double *dt, double *fields) {
     if (step) memcpy(step, m->row_step, sizeof(int) * (size_t)m->nrows);

@@ -29,6 +29,11 @@ typedef struct fo_model fo_model;
 fo_model *fo_load(const char *root, int verbose);
 /* Same, from explicit file names (cntl may be NULL to skip control parsing). */
 fo_model *fo_load_files(const char *mesh_path, const char *cntl_path, int verbose);
+/* sig_rescale = 1: the small-loop variant (src_small/FETD_FWD.f90:314-328,
+ * 1526,1540): sigma of domains 2..n scaled by sp = 1 / sigma of the last
+ * domain, written values scaled by pi**log10(sp) / sp**2. */
+fo_model *fo_load_files_variant(const char *mesh_path, const char *cntl_path, int verbose, int sig_rescale);
+double fo_sig_parameter(const fo_model *m);
 const char *fo_error(void);
 void fo_free(fo_model *m);
 
@@ -38,6 +43,16 @@ int fo_assemble(fo_model *m);
 /* Time loop (src/FETD_FWD.f90:1174-1464).  field paths may be NULL.
  * max_steps > 0 stops after that many loop iterations (bounded sample). */
 int fo_run(fo_model *m, const char *field_path, const char *field2_path, int max_steps);
+
+/* Per-tet field slices (vtkout, src/FETD_FWD.f90:1642-1711): before fo_run,
+ * n slice times; after it, slice i as [T][9] (E, curl E at the centroid, the
+ * running sum of curl E * dt), with the step and time of the output row
+ * that took it.  fo_get_slice returns -1 for a slice never reached. */
+void fo_set_slices(fo_model *m, int n, const double *times);
+int fo_get_slice(const fo_model *m, int i, double *f9, int *step, double *t);
+/* vtk_field_save's per-tet evaluation for an edge vector e[nE]: ed[T][6] =
+ * E and curl E at the tet centroids (src/FETD_FWD.f90:1672-1705). */
+void fo_tet_fields(const fo_model *m, const double *e, double *ed);
 
 /* ---- accessors for checks ---- */
 void fo_counts(const fo_model *m, int *nN, int *nT, int *nE, int *nnzU, int *nR, int *nseg);

@@ -31,6 +31,13 @@ def lib():
         L.fo_load.argtypes = [ctypes.c_char_p, _I]
         L.fo_load_files.restype = _P
         L.fo_load_files.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _I]
+        L.fo_load_files_variant.restype = _P
+        L.fo_load_files_variant.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _I, _I]
+        L.fo_sig_parameter.restype = ctypes.c_double
+        L.fo_sig_parameter.argtypes = [_P]
+        L.fo_set_slices.argtypes = [_P, _I, _P]
+        L.fo_get_slice.argtypes = [_P, _I, _P, _P, _P]
+        L.fo_tet_fields.argtypes = [_P, _P, _P]
         L.fo_error.restype = ctypes.c_char_p
         L.fo_factor_nnz.restype = ctypes.c_longlong
         L.fo_factor_nnz.argtypes = [_P]
@@ -56,9 +63,10 @@ def _p(a):
 class Oracle:
     """One model in the CPU restatement."""
 
-    def __init__(self, mesh_path, cntl_path, verbose=0):
+    def __init__(self, mesh_path, cntl_path, verbose=0, sig_rescale=False):
         L = lib()
-        m = L.fo_load_files(mesh_path.encode(), cntl_path.encode() if cntl_path else None, verbose)
+        m = L.fo_load_files_variant(mesh_path.encode(), cntl_path.encode() if cntl_path else None, verbose,
+                                    1 if sig_rescale else 0)
         if not m:
             raise RuntimeError(L.fo_error().decode())
         self.L = L
@@ -139,6 +147,28 @@ class Oracle:
     def set_sigma(self, sigma):
         s = np.ascontiguousarray(sigma, np.float64)
         self.L.fo_set_sigma(self.m, _p(s))
+
+    def sig_parameter(self):
+        return self.L.fo_sig_parameter(self.m)
+
+    def set_slices(self, times):
+        t = np.ascontiguousarray(times, np.float64)
+        self.L.fo_set_slices(self.m, len(t), _p(t))
+
+    def tet_fields(self, e):
+        """Per-tet [T][6] E and curl E at the centroids for edge values e[nE]."""
+        e = np.ascontiguousarray(e, np.float64)
+        out = np.empty((self.nT, 6))
+        self.L.fo_tet_fields(self.m, _p(e), _p(out))
+        return out
+
+    def slice(self, i):
+        """Per-tet fields [T][9] of slice i and the (step, t) of the row that took it."""
+        f = np.empty((self.nT, 9))
+        st, t = ctypes.c_int(), ctypes.c_double()
+        if self.L.fo_get_slice(self.m, i, _p(f), ctypes.byref(st), ctypes.byref(t)) != 0:
+            raise RuntimeError(f"slice {i} was not reached")
+        return f, st.value, t.value
 
     def run(self, field=None, field2=None, max_steps=0):
         rc = self.L.fo_run(self.m, field.encode() if field else None, field2.encode() if field2 else None,

@@ -32,6 +32,8 @@ void launch_gather(int64_t nnz, const int64_t *cptr, const int *ctr, const doubl
                    double *M, double *K, hipStream_t st);
 void launch_recvec(int nR, const double *xyz, const int *tet_vex, const int *rec_tet, const double *rec_xyz,
                    double *n_arr, double *d_arr, hipStream_t st);
+void launch_slice(int nT, const double *xyz, const int *tet_vex, const int *tet_edg, const int *tet_drc,
+                  const double *e, double dt, int zero_b, double *B, double *out, hipStream_t st);
 // factor.hip
 void launch_fill0(double *p, int64_t n, hipStream_t st);
 void launch_scatter_p(int64_t nnz, const int64_t *amap, const double *M, const double *K, double cdt, double *L,
@@ -222,6 +224,11 @@ struct tf_ctx {
     std::vector<int> src_edge, src_id;
     std::vector<double> src_w;
     std::vector<int> replay;  // forced keep/change decisions of the doubling checks
+    // per-tet field slices (vtkout): requested times, which were taken, when
+    std::vector<double> slice_t, slice_tt;
+    std::vector<int> slice_done, slice_step;
+    double *d_slice = nullptr, *d_sliceB = nullptr;  // [slice][T][9], [T][3]
+    int slice_nT = 0;
     int stype = 1, ptype = 1;
     double pulse_width = 0.0, current = 1.0;
     bool have_src = false;
@@ -368,7 +375,14 @@ void free_tape(tf_ctx *c) {
     c->tape_valid = false;
 }
 
+void free_slices(tf_ctx *c) {
+    dfree(c->d_slice); dfree(c->d_sliceB);
+    c->d_slice = nullptr; c->d_sliceB = nullptr;
+    c->slice_nT = 0;
+}
+
 void free_all(tf_ctx *c) {
+    free_slices(c);
     free_models(c);
     free_solver(c);
     free_vecs(c);
@@ -980,6 +994,48 @@ void output_row_from(tf_ctx *c, const double *stage) {
     c->nrows++;
 }
 
+// Field slices on an output row (solution_output's vtk part,
+// src/FETD_FWD.f90:1642-1656): while a slice is pending, every row updates
+// the per-tet fields; a pending slice whose time has come takes them.
+void slice_row(tf_ctx *c, int step, double t, double dt, const double *e) {
+    const int n = (int)c->slice_t.size(), T = c->topo.nT;
+    std::vector<int> take;
+    bool pending = false;
+    for (int i = 0; i < n; i++) {
+        if (c->slice_done[i]) continue;
+        pending = true;
+        if (t >= c->slice_t[i]) take.push_back(i);
+    }
+    if (!pending) return;
+    double *out = take.empty() ? nullptr : c->d_slice + (int64_t)take[0] * T * 9;
+    tf::launch_slice(T, c->d_xyz, c->d_tet_vex, c->d_tet_edg, c->d_tet_drc, e, dt, step == 0 ? 1 : 0, c->d_sliceB,
+                     out, c->st);
+    for (size_t q = 0; q < take.size(); q++) {
+        const int i = take[q];
+        if (q > 0)
+            HIPCHK(hipMemcpyAsync(c->d_slice + (int64_t)i * T * 9, out, sizeof(double) * 9 * T,
+                                  hipMemcpyDeviceToDevice, c->st));
+        c->slice_done[i] = 1;
+        c->slice_step[i] = step;
+        c->slice_tt[i] = t;
+    }
+}
+
+// slice storage for a run that writes output rows
+void prepare_slices(tf_ctx *c) {
+    const int n = (int)c->slice_t.size(), T = c->topo.nT;
+    std::fill(c->slice_done.begin(), c->slice_done.end(), 0);
+    if (n == 0) return;
+    if (c->nsrc != 1) throw std::runtime_error("field slices need a single-source context");
+    if (c->slice_nT != T || !c->d_slice) {
+        free_slices(c);
+        c->d_slice = dalloc<double>((size_t)n * T * 9);
+        c->d_sliceB = dalloc<double>((size_t)T * 3);
+        c->slice_nT = T;
+    }
+    if (!c->d_tet_edg) c->d_tet_edg = dupload(c->topo.tet_edg);
+}
+
 void output_row(tf_ctx *c, int step, double t, double dt, const double *e) {
     const int64_t per = (int64_t)c->nR * 6 * c->nsrc;
     if (c->nrows + 1 > c->rows_cap) {
@@ -995,6 +1051,7 @@ void output_row(tf_ctx *c, int step, double t, double dt, const double *e) {
     double *o = c->d_rows + per * c->nrows;
     if (c->nsrc == 1) tf::launch_readout(c->nR, c->d_rec_edge, c->d_rec_drc, c->d_narr, c->d_darr, e, o, c->st);
     else tf::launch_mreadout(c->nR, c->nsrc, c->d_rec_edge, c->d_rec_drc, c->d_narr, c->d_darr, e, o, c->st);
+    if (!c->slice_t.empty()) slice_row(c, step, t, dt, e);
     c->row_step.push_back(step);
     c->row_t.push_back(t);
     c->row_dt.push_back(dt);
@@ -1038,6 +1095,7 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
     const int ring = dbl + 1;
     int cur = 0;
     for (double *p : {v.e1, v.e2, v.e3}) HIPCHK(hipMemsetAsync(p, 0, sizeof(double) * n * c->nsrc, c->st));
+    if (outp) prepare_slices(c);
     // adjoint tape: node ids of the history states e2, e3 and of the ring
     const bool tape = c->ckpt_every > 0 && inout == 0;
     int id2 = 0, id3 = 0;
@@ -1803,6 +1861,33 @@ int tf_set_source(tf_ctx *c, int isrc, int nseg, const int *seg_vex, int source_
             c->d_src_id = dupload(c->src_id);
             c->d_src_w = dupload(c->src_w);
         }
+        return TF_OK;
+    });
+}
+
+int tf_set_slices(tf_ctx *c, int nslice, const double *times) {
+    return guarded(c, [&] {
+        if (nslice < 0 || (nslice > 0 && !times)) return fail(c, TF_ERR_ARG, "bad slice times");
+        for (int i = 0; i < nslice; i++)
+            if (!std::isfinite(times[i])) return fail(c, TF_ERR_ARG, "slice times must be finite");
+        free_slices(c);
+        c->slice_t.assign(times, times + nslice);
+        c->slice_tt.assign(nslice, 0.0);
+        c->slice_done.assign(nslice, 0);
+        c->slice_step.assign(nslice, 0);
+        return TF_OK;
+    });
+}
+
+int tf_get_slice(tf_ctx *c, int i, double *fields9, int *step, double *t) {
+    return guarded(c, [&] {
+        if (i < 0 || i >= (int)c->slice_t.size()) return fail(c, TF_ERR_ARG, "no such slice");
+        if (!c->slice_done[i]) return fail(c, TF_ERR_STATE, "the run did not reach this slice's time");
+        if (fields9)
+            HIPCHK(hipMemcpy(fields9, c->d_slice + (int64_t)i * c->slice_nT * 9,
+                             sizeof(double) * 9 * c->slice_nT, hipMemcpyDeviceToHost));
+        if (step) *step = c->slice_step[i];
+        if (t) *t = c->slice_tt[i];
         return TF_OK;
     });
 }

@@ -211,7 +211,58 @@ __global__ void k_recvec(int nR, const double *xyz, const int *tet_vex, const in
     }
 }
 
+// Per-tet fields at the centroid for the field slices (vtk_field_save,
+// src/FETD_FWD.f90:1659-1711), one thread per tet: E = sum_j N_j drc_j e_j,
+// curl E = sum_j curl N_j drc_j e_j (+2 l_j grad L1 x grad L2 / (6V)^2, the
+// opposite sign of the receivers' d_arr) and the running sum B += curl E * dt
+// (B is cleared first on the step-0 row).  `out` ([T][9]: E, curl E, B) is
+// written only on a row that takes a slice.
+__global__ void k_slice(int nT, const double *xyz, const int *tet_vex, const int *tet_edg, const int *tet_drc,
+                        const double *e, double dt, int zero_b, double *B, double *out) {
+    int it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= nT) return;
+    double x[4], y[4], z[4], a[4], b[4], c[4], d[4], vol;
+    load_tet(xyz, tet_vex + 4 * (int64_t)it, x, y, z);
+    dev_cofactors(x, y, z, a, b, c, d, &vol);
+    const double cp[3] = {(x[0] + x[1] + x[2] + x[3]) / 4.0, (y[0] + y[1] + y[2] + y[3]) / 4.0,
+                          (z[0] + z[1] + z[2] + z[3]) / 4.0};
+    const double v6 = 6.0 * vol, v62 = v6 * v6;
+    double ef[3] = {0.0, 0.0, 0.0}, df[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < 6; j++) {
+        const int j1 = c_ind_tet[j][0], j2 = c_ind_tet[j][1];
+        const double L1 = a[j1] + b[j1] * cp[0] + c[j1] * cp[1] + d[j1] * cp[2];
+        const double L2 = a[j2] + b[j2] * cp[0] + c[j2] * cp[1] + d[j2] * cp[2];
+        const double len = dev_len(x, y, z, j);
+        const double nv[3] = {(L1 * b[j2] - L2 * b[j1]) * len / v62, (L1 * c[j2] - L2 * c[j1]) * len / v62,
+                              (L1 * d[j2] - L2 * d[j1]) * len / v62};
+        const double dv[3] = {2.0 * (c[j1] * d[j2] - c[j2] * d[j1]) * len / v62,
+                              2.0 * (b[j2] * d[j1] - b[j1] * d[j2]) * len / v62,
+                              2.0 * (b[j1] * c[j2] - b[j2] * c[j1]) * len / v62};
+        const double ev = e[tet_edg[6 * (int64_t)it + j]];
+        const double drc = (double)tet_drc[6 * (int64_t)it + j];
+        for (int k = 0; k < 3; k++) {
+            ef[k] += nv[k] * drc * ev;
+            df[k] += dv[k] * drc * ev;
+        }
+    }
+    double *bp = B + 3 * (int64_t)it;
+    for (int k = 0; k < 3; k++) bp[k] = (zero_b ? 0.0 : bp[k]) + df[k] * dt;
+    if (out) {
+        double *o = out + 9 * (int64_t)it;
+        for (int k = 0; k < 3; k++) {
+            o[k] = ef[k];
+            o[3 + k] = df[k];
+            o[6 + k] = bp[k];
+        }
+    }
+}
+
 namespace tf {
+void launch_slice(int nT, const double *xyz, const int *tet_vex, const int *tet_edg, const int *tet_drc,
+                  const double *e, double dt, int zero_b, double *B, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_slice, dim3((nT + 127) / 128), dim3(128), 0, st, nT, xyz, tet_vex, tet_edg, tet_drc, e, dt,
+                       zero_b, B, out);
+}
 void launch_element(int nT, const double *xyz, const int *tet_vex, const int *tet_drc, const uint8_t *orient,
                     const double *sigma, const double *inv_mu, double *vm, double *vk, hipStream_t st) {
     hipLaunchKernelGGL(k_element, dim3((nT + 127) / 128), dim3(128), 0, st, nT, xyz, tet_vex, tet_drc, orient,

@@ -47,6 +47,8 @@ _SIGS = {
     "tf_get_timers": [_P, _P],
     "tf_set_num_sources": [_P, _I],
     "tf_set_replay": [_P, _I, _P],
+    "tf_set_slices": [_P, _I, _P],
+    "tf_get_slice": [_P, _I, _P, _P, _P],
     "tf_set_tuning": [_P, ctypes.c_char_p, _I],
     "tf_time_solve": [_P, _I, _I, _I, _P],
     "tf_set_profiling": [_P, _I],
@@ -167,6 +169,20 @@ class Fetd:
     def set_replay(self, keep):
         k = np.ascontiguousarray(keep if keep is not None else [], np.int32)
         self._chk(self.lib.tf_set_replay(self.ctx, len(k), _ptr(k) if len(k) else None), "tf_set_replay")
+
+    def set_slices(self, times):
+        """Per-tet field slices at these times (the .cntl vtkout list)."""
+        t = np.ascontiguousarray(times if times is not None else [], np.float64)
+        self.nslice = len(t)
+        self._chk(self.lib.tf_set_slices(self.ctx, len(t), _ptr(t) if len(t) else None), "tf_set_slices")
+
+    def slice(self, i):
+        """Slice i after a run: per-tet [nT][9] (E, curl E, B at the centroid)
+        and the (step, t) of the output row that took it."""
+        f = np.empty((len(self.tet_vex), 9))
+        st, t = _I(), ctypes.c_double()
+        self._chk(self.lib.tf_get_slice(self.ctx, i, _ptr(f), ctypes.byref(st), ctypes.byref(t)), "tf_get_slice")
+        return f, st.value, t.value
 
     def set_receivers(self, xyz, rec_charac=10.0):
         r = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
@@ -317,35 +333,56 @@ class Fetd:
         return dict(zip(keys, t.tolist()))
 
 
-def setup_from_files(mesh_path, cntl_path, device=0):
-    """model_mesh + model_ctl + mesh_process from reference input files."""
+def setup_from_files(mesh_path, cntl_path, device=0, sig_rescale=False):
+    """model_mesh + model_ctl + mesh_process from reference input files.
+    sig_rescale: the small-loop variant (src_small/FETD_FWD.f90:314-328):
+    sigma of domains 2..n scaled by sp = 1 / sigma of the last domain (c["sp"])."""
     m = tfio.read_mesh(mesh_path)
     c = tfio.read_cntl(cntl_path)
     if c["sp_mthd"] != 2:
         raise FetdError("error!")  # only the adaptive BDF2 method exists (src/FETD_FWD.f90:1210-1219)
     if c["bound_cond"] != 0:
         raise FetdError("absorbing boundary condition is not supported")
+    c["sp"] = None
     fe = Fetd(device)
     fe.set_mesh(m["xyz"], m["tet_vex"])
     if c.get("aniso"):
+        if sig_rescale:
+            raise FetdError("the conductivity rescaling applies to isotropic domain tables")
         fe.set_materials_tensor(*tfio.materials_tensor(m["tet_phy"], c["aniso"]))
     else:
-        fe.set_materials(*tfio.materials(m["tet_phy"], c["domains"]))
+        if sig_rescale:
+            c["sp"] = tfio.sig_parameter(c["domains"])
+        fe.set_materials(*tfio.materials(m["tet_phy"], c["domains"], sig_rescale))
     fe.set_source(m["seg_vex"], c["source_type"], c["pulse_type"], c["pulse_width"], c["current"])
     fe.set_receivers(c["receivers"], c["rec_charac"])
     fe.set_schedule(c["sp_division"], c["sp_double"], c["sp_dblesize"], c["time_maximum"])
+    if c["vtk_times"]:
+        fe.set_slices(c["vtk_times"])
     fe.assemble()
     return fe, m, c
 
 
-def run_root(root, device=0, write=True, max_steps=0):
-    """`FETD -f ROOT`: returns the rows and writes ROOT.field / ROOT.field2."""
-    fe, m, c = setup_from_files(root + ".mesh", root + ".cntl", device)
+def run_root(root, device=0, write=True, max_steps=0, sig_rescale=False, replay=None):
+    """`FETD -f ROOT`: returns the rows and writes ROOT.field / ROOT.field2,
+    and ROOT_i.msh for every vtkout slice the run reached when ROOT.msh exists
+    (msh_output, src/FETD_FWD.f90:1642-1656,1713-1864).  replay: keep/change
+    outcomes of the doubling checks to follow instead of computing them."""
+    fe, m, c = setup_from_files(root + ".mesh", root + ".cntl", device, sig_rescale)
+    if replay is not None:
+        fe.set_replay(replay)
     rows = fe.run(max_steps=max_steps)
     if write:
-        tfio.write_field(root + ".field", rows["step"], rows["t"], rows["fields"], c["output"])
+        sp = c["sp"]
+        tfio.write_field(root + ".field", rows["step"], rows["t"], rows["fields"], c["output"], sp)
         if c["time_points"]:
-            tfio.write_field2(root + ".field2", rows["t"], rows["fields"], c["time_points"], c["output"])
+            tfio.write_field2(root + ".field2", rows["t"], rows["fields"], c["time_points"], c["output"], sp)
+        if c["vtk_times"] and os.path.exists(root + ".msh"):
+            for i in range(len(c["vtk_times"])):
+                if rows["t"][-1] < c["vtk_times"][i]:
+                    continue  # not reached
+                f9, _, _ = fe.slice(i)
+                tfio.write_msh_slice(root + ".msh", tfio.slice_path(root, i + 1), f9, c["output"])
     rows["timers"] = fe.timers()
     fe.close()
     return rows

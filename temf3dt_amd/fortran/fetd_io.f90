@@ -10,12 +10,16 @@
 !   field rows  -- solution_output, src/FETD_FWD.f90:1602-1641: format
 !                  (i6,e15.5,2x,R(m e15.5,2x)); a gate is filled from the first
 !                  row with t >= gate by linear interpolation with the row before
+!   msh slices  -- msh_output, src/FETD_FWD.f90:1713-1864: a copy of ROOT.msh
+!                  with one $ElementData section per selected component
+!   rescaling   -- the small-loop variant, src_small/FETD_FWD.f90:314-328,
+!                  1526,1540
 module fetd_io
   use iso_c_binding, only: c_int, c_double
   implicit none
   private
   public :: cntl_t, read_medit, read_cntl, tet_materials, tet_tensors, aniso_tensor, field_writer_t, fw_open, &
-            fw_row, fw_close
+            fw_row, fw_close, rescale_domains, msh_tet_count, write_msh_slice
 
   real(8), parameter :: mu0 = 3.14159265358979323846264338327950288419716939937510d0 * 4.0d-7
 
@@ -32,6 +36,8 @@ module fetd_io
     integer :: o_f(9) = 0
     real(8), allocatable :: time_points(:)
     integer :: vtk_num = 0
+    real(8), allocatable :: vtk_times(:)
+    real(8) :: sig_parameter = 1.0d0     ! small-loop variant's scale (1 = off)
   end type cntl_t
 
   type field_writer_t
@@ -42,7 +48,11 @@ module fetd_io
     logical, allocatable :: gate_ok(:)
     real(8), allocatable :: prev(:, :)       ! (9, nrec) previous row
     real(8) :: tprev = 0.0d0
+    logical :: rescale = .false.             ! small-loop variant: written v / sp**2 * pi**log10(sp)
+    real(8) :: sp2 = 1.0d0, spk = 1.0d0
   end type field_writer_t
+
+  real(8), parameter :: pi = 3.14159265358979323846264338327950288419716939937510d0
 
 contains
 
@@ -152,7 +162,6 @@ contains
     character(512) :: line
     character(128) :: key
     character(512) :: val
-    real(8) :: x
 
     ierr = 0; msg = ' '
     open(newunit=u, file=path, status='old', action='read', iostat=ios)
@@ -211,8 +220,9 @@ contains
         read(u, *) c%o_f
       case ('vtkout')
         read(val, *) c%vtk_num
+        allocate(c%vtk_times(max(c%vtk_num, 0)))
         do i = 1, c%vtk_num
-          read(u, *) x
+          read(u, *) c%vtk_times(i)
         end do
       case ('time_points')
         read(val, *) n
@@ -227,6 +237,7 @@ contains
     close(u)
     if (ierr /= 0) return
     if (.not. allocated(c%time_points)) allocate(c%time_points(0))
+    if (.not. allocated(c%vtk_times)) allocate(c%vtk_times(0))
     ! required keys (src/FETD_FWD.f90:490-497)
     if (c%rec_num == 0) then
       ierr = 4; msg = 'cmod%rec_num error'
@@ -246,6 +257,24 @@ contains
       ierr = 4; msg = 'cmod%o_f error'
     end if
   end subroutine read_cntl
+
+  ! The small-loop variant (src_small/FETD_FWD.f90:314-328): sp = 1 / sigma
+  ! of the last domain when there are several (the branch written to take
+  ! the smallest is unreachable); sigma of domains 2..n times sp.
+  subroutine rescale_domains(c)
+    type(cntl_t), intent(inout) :: c
+    integer :: i
+    real(8) :: sp
+    sp = 1.0d0
+    do i = 2, c%dom_num
+      sp = c%dom(2, i)
+    end do
+    sp = 1.0d0 / sp
+    do i = 2, c%dom_num
+      c%dom(2, i) = c%dom(2, i) * sp
+    end do
+    c%sig_parameter = sp
+  end subroutine rescale_domains
 
   ! per-tet sigma and 1/mu from the isotropic domain table (src/FETD_FWD.f90:347-364)
   subroutine tet_materials(c, nT, tet_phy, sigma, inv_mu, ierr)
@@ -368,13 +397,19 @@ contains
   end subroutine tet_tensors
 
   ! ---- .field / .field2 ---------------------------------------------------
-  subroutine fw_open(w, field_path, field2_path, nrec, o_f, gates)
+  subroutine fw_open(w, field_path, field2_path, nrec, o_f, gates, sig_parameter)
     type(field_writer_t), intent(out) :: w
     character(*), intent(in) :: field_path, field2_path
     integer, intent(in) :: nrec, o_f(9)
     real(8), intent(in) :: gates(:)
+    real(8), intent(in), optional :: sig_parameter
     integer :: k
     character(16) :: a, b
+    if (present(sig_parameter)) then
+      w%rescale = .true.
+      w%sp2 = sig_parameter**2.0d0
+      w%spk = pi**log10(sig_parameter)
+    end if
     w%nrec = nrec
     w%nsel = 0
     do k = 1, 9
@@ -402,7 +437,7 @@ contains
     real(8) :: g(9, w%nrec), a, b
     integer :: i, j, k
     w%nrow = w%nrow + 1
-    write(w%u1, w%fmt) step, t, ((vals(w%o_v(k), j), k = 1, w%nsel), j = 1, w%nrec)
+    write(w%u1, w%fmt) step, t, ((scaled(w, vals(w%o_v(k), j)), k = 1, w%nsel), j = 1, w%nrec)
     do i = 1, size(w%gates)
       if (t >= w%gates(i) .and. .not. w%gate_ok(i)) then
         if (w%nrow == 1) exit
@@ -413,13 +448,23 @@ contains
             g(k, j) = a * w%gates(i) + b
           end do
         end do
-        write(w%u2, w%fmt) i, w%gates(i), ((g(w%o_v(k), j), k = 1, w%nsel), j = 1, w%nrec)
+        write(w%u2, w%fmt) i, w%gates(i), ((scaled(w, g(w%o_v(k), j)), k = 1, w%nsel), j = 1, w%nrec)
         w%gate_ok(i) = .true.
       end if
     end do
     w%prev = vals
     w%tprev = t
   end subroutine fw_row
+
+  pure real(8) function scaled(w, v)
+    type(field_writer_t), intent(in) :: w
+    real(8), intent(in) :: v
+    if (w%rescale) then
+      scaled = v / w%sp2 * w%spk
+    else
+      scaled = v
+    end if
+  end function scaled
 
   subroutine fw_close(w)
     type(field_writer_t), intent(inout) :: w
@@ -428,4 +473,99 @@ contains
     w%u1 = -1
     w%u2 = -1
   end subroutine fw_close
+
+  ! ---- per-tet field slices (msh_output, src/FETD_FWD.f90:1713-1864) --------
+  ! Tetrahedra (element type 4) of a gmsh 4.x .msh and their tags in file
+  ! order: the $Elements walk of msh_output (:1744-1788), each block's
+  ! "dim tag type count" header, then the first number of every element line.
+  subroutine msh_tet_count(path, ntet, tags, ierr, msg)
+    character(*), intent(in) :: path
+    integer, intent(out) :: ntet, ierr
+    integer, allocatable, intent(out) :: tags(:)
+    character(*), intent(out) :: msg
+    integer :: u, ios, pass, nb, ib, d1, d2, et, cnt, j, tag
+    character(4096) :: line
+    ierr = 0; msg = ' '; ntet = 0
+    open(newunit=u, file=path, status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+      ierr = 1; msg = 'cannot open '//trim(path); return
+    end if
+    do pass = 1, 2
+      rewind(u)
+      do
+        read(u, '(a)', iostat=ios) line
+        if (ios /= 0) then
+          ierr = 2; msg = 'cell number not found'; close(u); return
+        end if
+        if (line(1:9) == '$Elements') exit
+      end do
+      read(u, *) nb
+      if (pass == 2) allocate(tags(ntet))
+      ntet = 0
+      do ib = 1, nb
+        read(u, *) d1, d2, et, cnt
+        do j = 1, cnt
+          read(u, *) tag
+          if (et == 4) then
+            ntet = ntet + 1
+            if (pass == 2) tags(ntet) = tag
+          end if
+        end do
+      end do
+    end do
+    close(u)
+  end subroutine msh_tet_count
+
+  ! dst = a byte copy of src followed by one $ElementData section per
+  ! selected component k (o_f(k) == 1): |f9(k, i)| per tetrahedron as
+  ! "(i10,e16.8)" rows after the gmsh header lines (:1789-1862).  f9(9, nT):
+  ! E, curl E, B in the mesh's tet order.
+  subroutine write_msh_slice(src, dst, nT, f9, o_f, ierr, msg)
+    character(*), intent(in) :: src, dst
+    integer, intent(in) :: nT, o_f(9)
+    real(8), intent(in) :: f9(9, nT)
+    integer, intent(out) :: ierr
+    character(*), intent(out) :: msg
+    character(2), parameter :: names(9) = ['Ex', 'Ey', 'Ez', 'Dx', 'Dy', 'Dz', 'Bx', 'By', 'Bz']
+    integer, allocatable :: tags(:)
+    character(len=:), allocatable :: buf
+    integer :: ntet, u, k, i, nbytes, ios
+    call msh_tet_count(src, ntet, tags, ierr, msg)
+    if (ierr /= 0) return
+    if (ntet /= nT) then
+      ierr = 3; write(msg, '(a,i0,a,i0)') 'the .msh has ', ntet, ' tetrahedra, the fields ', nT; return
+    end if
+    inquire(file=src, size=nbytes)
+    allocate(character(len=nbytes) :: buf)
+    open(newunit=u, file=src, access='stream', form='unformatted', status='old', action='read', iostat=ios)
+    if (ios == 0) read(u, iostat=ios) buf
+    close(u)
+    if (ios /= 0) then
+      ierr = 1; msg = 'cannot read '//trim(src); return
+    end if
+    open(newunit=u, file=dst, access='stream', form='unformatted', status='replace', action='write', iostat=ios)
+    if (ios /= 0) then
+      ierr = 1; msg = 'cannot write '//trim(dst); return
+    end if
+    write(u) buf
+    close(u)
+    open(newunit=u, file=dst, position='append', action='write', form='formatted')
+    do k = 1, 9
+      if (o_f(k) /= 1) cycle
+      write(u, '(a)') '$ElementData'
+      write(u, '(a)') '1'
+      write(u, '(a)') '"'//names(k)//' data"'
+      write(u, '(a)') '1'
+      write(u, '(a)') '0.0'
+      write(u, '(a)') '3'
+      write(u, '(a)') '0'
+      write(u, '(a)') '1'
+      write(u, '(i12)') nT
+      do i = 1, nT
+        write(u, '(i10,e16.8)') tags(i), abs(f9(k, i))
+      end do
+      write(u, '(a)') '$EndElementData'
+    end do
+    close(u)
+  end subroutine write_msh_slice
 end module fetd_io

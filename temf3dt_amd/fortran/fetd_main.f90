@@ -1,11 +1,16 @@
 ! fetd_main.f90 -- FETD forward driver on the MI355X library.
 !
-!   fetd_amd -f FILE_ROOT [-v full|none]
+!   fetd_amd -f FILE_ROOT [-v full|none] [-r]
 !
 ! Same command line, inputs and outputs as the reference program
 ! (src/FETD_MAIN.f90:23-95): reads FILE_ROOT.mesh and FILE_ROOT.cntl, runs the
 ! forward time-stepping loop, writes FILE_ROOT.field (one row per step) and
-! FILE_ROOT.field2 (one row per time_points gate).  The mesh and control
+! FILE_ROOT.field2 (one row per time_points gate), and for every vtkout time
+! the run reaches FILE_ROOT_i.msh, a copy of FILE_ROOT.msh with the per-tet
+! fields (msh_output, src/FETD_FWD.f90:1642-1656,1713-1864).  -r selects the
+! reference's small-loop build (src_small): conductivities of domains 2..n
+! scaled by 1 / sigma of the last domain, written values by
+! pi**log10(sp) / sp**2 (src_small/FETD_FWD.f90:314-328,1526,1540).  The mesh and control
 ! reading and the output writing stay in Fortran (module fetd_io); topology,
 ! assembly, factorisation, BDF2 stepping with the doubling controller and the
 ! receiver readout run on the GPU through the C ABI (module temfetd).
@@ -18,10 +23,13 @@ program fetd_amd
   character(512) :: arg, root, verbose
   character(256) :: msg
   character(1) :: job
-  logical :: loud
+  logical :: loud, rescale, have_msh
   integer :: narg, k, ierr, nN, nT, nseg, nrows, cap, i, rc
   real(c_double), allocatable :: xyz(:, :), sigma(:), inv_mu(:), row_t(:), row_dt(:), fields(:, :, :)
-  real(c_double), allocatable :: sig9(:, :), nu9(:, :)
+  real(c_double), allocatable :: sig9(:, :), nu9(:, :), f9(:, :)
+  real(c_double) :: slice_t
+  integer(c_int) :: slice_step
+  character(32) :: num
   integer(c_int), allocatable :: tet(:, :), tet_phy(:), seg(:, :), rec_tet(:), row_step(:)
   type(cntl_t) :: c
   type(c_ptr) :: ctx
@@ -33,10 +41,16 @@ program fetd_amd
   job = ' '
   root = ' '
   verbose = 'full'
+  rescale = .false.
   narg = command_argument_count()
   k = 1
   do while (k <= narg)
     call get_command_argument(k, arg)
+    if (trim(arg) == '-r') then
+      rescale = .true.
+      k = k + 1
+      cycle
+    end if
     if (arg(1:2) == '-v' .and. k < narg) then
       call get_command_argument(k + 1, verbose)
       k = k + 2
@@ -53,6 +67,7 @@ program fetd_amd
     write(*, '(a)') 'Usage: fetd_amd -[job] FILE_ROOT'
     write(*, '(a)') '[FORWARD]    -f    Calculates the predicted data and saves the EM solution'
     write(*, '(a)') 'Optional     -v    [full|none]'
+    write(*, '(a)') 'Optional     -r    small-loop variant (conductivity rescaling)'
     stop 1
   end if
   loud = trim(verbose) /= 'none'
@@ -63,6 +78,10 @@ program fetd_amd
   if (ierr /= 0) call die(msg)
   call read_cntl(trim(root)//'.cntl', c, ierr, msg)
   if (ierr /= 0) call die(msg)
+  if (rescale) then
+    if (c%aniso) call die('the conductivity rescaling applies to isotropic domain tables')
+    call rescale_domains(c)
+  end if
   allocate(sigma(nT), inv_mu(nT), rec_tet(c%rec_num))
   if (c%aniso) then
     allocate(sig9(9, nT), nu9(9, nT))
@@ -88,6 +107,7 @@ program fetd_amd
   call check(tf_set_receivers(ctx, int(c%rec_num, c_int), c%rec, c%rec_charac, rec_tet), 'tf_set_receivers')
   call check(tf_set_schedule(ctx, int(c%sp_division, c_int), int(c%sp_double, c_int), &
                              int(c%sp_dblesize, c_int), c%time_maximum), 'tf_set_schedule')
+  if (c%vtk_num > 0) call check(tf_set_slices(ctx, int(c%vtk_num, c_int), c%vtk_times), 'tf_set_slices')
   call check(tf_assemble(ctx), 'tf_assemble')
 
   ! rows are returned by the loop; a run longer than the buffer is repeated
@@ -102,11 +122,28 @@ program fetd_amd
   end do
 
   ! ---- solution_output: .field and .field2 ----
-  call fw_open(w, trim(root)//'.field', trim(root)//'.field2', c%rec_num, c%o_f, c%time_points)
+  if (rescale) then
+    call fw_open(w, trim(root)//'.field', trim(root)//'.field2', c%rec_num, c%o_f, c%time_points, c%sig_parameter)
+  else
+    call fw_open(w, trim(root)//'.field', trim(root)//'.field2', c%rec_num, c%o_f, c%time_points)
+  end if
   do i = 1, nrows
     call fw_row(w, int(row_step(i)), row_t(i), fields(:, :, i))
   end do
   call fw_close(w)
+
+  ! ---- field slices: ROOT_i.msh for every vtkout time the run reached ----
+  inquire(file=trim(root)//'.msh', exist=have_msh)
+  if (c%vtk_num > 0 .and. have_msh) then
+    allocate(f9(9, nT))
+    do i = 1, c%vtk_num
+      if (tf_get_slice(ctx, int(i - 1, c_int), f9, slice_step, slice_t) /= 0) cycle  ! not reached
+      write(num, '(i0)') i
+      call write_msh_slice(trim(root)//'.msh', trim(root)//'_'//trim(num)//'.msh', nT, f9, c%o_f, ierr, msg)
+      if (ierr /= 0) call die(msg)
+      if (loud) write(*, '(a,es12.5,a)') ' field slice at t = ', slice_t, ' -> '//trim(root)//'_'//trim(num)//'.msh'
+    end do
+  end if
 
   rc = tf_get_timers(ctx, tm)
   if (loud) then

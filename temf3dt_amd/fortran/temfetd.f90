@@ -112,6 +112,26 @@ module temfetd
       integer(c_int), intent(out) :: row_step(*)
       real(c_double), intent(out) :: row_t(*), row_dt(*), fields(*)
     end function
+    integer(c_int) function tf_set_replay(ctx, nchecks, keep) bind(C, name='tf_set_replay')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nchecks
+      integer(c_int), intent(in) :: keep(*)
+    end function
+    integer(c_int) function tf_set_slices(ctx, nslice, times) bind(C, name='tf_set_slices')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nslice
+      real(c_double), intent(in) :: times(*)
+    end function
+    integer(c_int) function tf_get_slice(ctx, i, fields9, step, t) bind(C, name='tf_get_slice')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: i
+      real(c_double), intent(out) :: fields9(*)
+      integer(c_int), intent(out) :: step
+      real(c_double), intent(out) :: t
+    end function
     integer(c_int) function tf_get_timers(ctx, t8) bind(C, name='tf_get_timers')
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx

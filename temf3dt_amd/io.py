@@ -6,9 +6,14 @@ Host-side mirror of the reference's input/output code:
                    src/FETD_UTILITY.f90:678-731; keys src/FETD_FWD.f90:313-497)
   * FieldWriter -- solution_output's .field/.field2 rows,
                    src/FETD_FWD.f90:1602-1641 (format "(i6,e15.5,2x,R(m e15.5,2x))")
+  * write_msh_slice -- msh_output's per-tet field slices appended to a copy
+                   of ROOT.msh, src/FETD_FWD.f90:1713-1864
+  * sig_parameter -- the small-loop variant's conductivity rescaling,
+                   src_small/FETD_FWD.f90:314-328,1526,1540
 """
 import gzip
 import math
+import shutil
 
 import numpy as np
 
@@ -196,9 +201,39 @@ def read_cntl(path):
     return c
 
 
-def materials(tet_phy, domains):
+def sig_parameter(domains):
+    """The small-loop variant's scale sp (src_small/FETD_FWD.f90:314-323):
+    1 / sigma of the last domain when there are several (the branch written
+    to take the smallest sigma is unreachable), else 1."""
+    sp = 1.0
+    for i, d in enumerate(domains):
+        if i > 0:
+            sp = float(d[1])
+    return 1.0 / sp
+
+
+def rescaled_domains(domains):
+    """Domain table of the small-loop variant: sigma of domains 2..n times sp
+    (src_small/FETD_FWD.f90:324-328); returns (domains, sp)."""
+    sp = sig_parameter(domains)
+    out = [tuple(d) if i == 0 else (d[0], float(d[1]) * sp) + tuple(d[2:]) for i, d in enumerate(domains)]
+    return out, sp
+
+
+def output_scale(values, sp):
+    """Written values of the small-loop variant: v / sp**2 * pi**log10(sp)
+    (src_small/FETD_FWD.f90:1526,1540); sp = None leaves them as they are."""
+    if sp is None:
+        return values
+    return np.asarray(values) / sp ** 2.0 * math.pi ** math.log10(sp)
+
+
+def materials(tet_phy, domains, sig_rescale=False):
     """Per-tet sigma and 1/mu from the isotropic domain table
-    (src/FETD_FWD.f90:347-364)."""
+    (src/FETD_FWD.f90:347-364); sig_rescale applies the small-loop variant's
+    scaling of domains 2..n."""
+    if sig_rescale:
+        domains, _ = rescaled_domains(domains)
     sigma = np.empty(len(tet_phy), np.float64)
     inv_mu = np.empty(len(tet_phy), np.float64)
     found = np.zeros(len(tet_phy), bool)
@@ -283,14 +318,14 @@ def format_row(step, t, vals, o_f):
     return "".join(parts)
 
 
-def write_field(path, steps, times, fields, o_f):
-    """fields: [rows][R][9]"""
+def write_field(path, steps, times, fields, o_f, sp=None):
+    """fields: [rows][R][9]; sp: the small-loop variant's scale (or None)"""
     with open(path, "w") as f:
         for s, t, v in zip(steps, times, fields):
-            f.write(format_row(int(s), float(t), v, o_f) + "\n")
+            f.write(format_row(int(s), float(t), output_scale(v, sp), o_f) + "\n")
 
 
-def write_field2(path, times, fields, gates, o_f):
+def write_field2(path, times, fields, gates, o_f, sp=None):
     """.field2 rows at the time_points gates: the first output row with
     t >= gate is interpolated linearly with the previous row
     (src/FETD_FWD.f90:1625-1641, interp_lin src/FETD_UTILITY.f90:21-31)."""
@@ -306,8 +341,62 @@ def write_field2(path, times, fields, gates, o_f):
                 y1, y2 = fields[r - 1], fields[r]
                 a = (y1 - y2) / (x1 - x2)
                 b = y1 - a * x1
-                lines.append(format_row(g + 1, tg, a * tg + b, o_f))
+                lines.append(format_row(g + 1, tg, output_scale(a * tg + b, sp), o_f))
                 ok[g] = True
     with open(path, "w") as f:
         for ln in lines:
             f.write(ln + "\n")
+
+
+# ---- per-tet field slices (msh_output, src/FETD_FWD.f90:1713-1864) -----------
+SLICE_NAMES = ["Ex", "Ey", "Ez", "Dx", "Dy", "Dz", "Bx", "By", "Bz"]
+
+
+def msh_tet_tags(path):
+    """Element tags of the tetrahedra (element type 4) of a gmsh 4.x .msh, in
+    file order: the $Elements walk of msh_output (src/FETD_FWD.f90:1744-1788),
+    which reads each block's "dim tag type count" header and the first number
+    of every element line."""
+    with open(path) as f:
+        for line in f:
+            if line[:9] == "$Elements":
+                break
+        else:
+            raise ValueError("cell number not found")
+        nblock = int(f.readline().split()[0])
+        tags = []
+        for _ in range(nblock):
+            _dim, _ent, etype, cnt = (int(x) for x in f.readline().split()[:4])
+            for _ in range(cnt):
+                tag = int(f.readline().split()[0])
+                if etype == 4:
+                    tags.append(tag)
+    return np.array(tags, np.int64)
+
+
+def slice_path(root, i):
+    """ROOT_i.msh for slice i (1-based), as msh_output names it."""
+    return f"{root}_{i}.msh"
+
+
+def write_msh_slice(src_msh, dst, fields9, o_f):
+    """A copy of src_msh with one $ElementData section per selected output
+    component: the component's absolute value per tetrahedron, "(i10,e16.8)"
+    rows after the gmsh header lines (src/FETD_FWD.f90:1789-1862).  fields9
+    is [nT][9] (E, curl E, B) in the mesh's tet order."""
+    tags = msh_tet_tags(src_msh)
+    f9 = np.asarray(fields9)
+    if len(tags) != len(f9):
+        raise ValueError(f"{src_msh} has {len(tags)} tetrahedra, the fields {len(f9)}")
+    shutil.copyfile(src_msh, dst)
+    with open(dst, "a") as f:
+        for k in range(9):
+            if not o_f[k]:
+                continue
+            f.write("$ElementData\n1\n")
+            f.write(f'"{SLICE_NAMES[k]} data"\n')
+            f.write("1\n0.0\n3\n0\n1\n")
+            f.write(f"{len(tags):12d}\n")
+            col = np.abs(f9[:, k])
+            f.write("".join(f"{int(t):10d}{fort_e(float(v), 16, 8)}\n" for t, v in zip(tags, col)))
+            f.write("$EndElementData\n")

@@ -110,3 +110,49 @@ def test_fortran_anisotropic_tensors_match_python(tmp_path):
     s9, n9 = tfio.materials_tensor(mesh["tet_phy"], c["aniso"])
     for key, arr in (("tsig1", s9[0]), ("tnu1", n9[0]), ("tsigN", s9[-1]), ("tnuN", n9[-1])):
         np.testing.assert_array_equal(np.array(kv[key], float), arr.reshape(9))
+
+
+def test_fortran_rescale_matches_python(src_small):
+    """The small-loop variant's domain table (src_small/FETD_FWD.f90:314-328)."""
+    out = _run("rescale", src_small["root"])
+    kv = {}
+    for ln in out.strip().split("\n"):
+        kv.setdefault(ln.split()[0], []).append(ln.split()[1:])
+    c = tfio.read_cntl(src_small["cntl"])
+    doms, sp = tfio.rescaled_domains(c["domains"])
+    assert float(kv["sp"][0][0]) == sp
+    np.testing.assert_array_equal(np.array(kv["dom"], dtype=float), np.array(doms, float))
+
+
+def test_fortran_writer_rescaled_matches_python(tmp_path):
+    """Rows written with the small-loop variant's scaling v / sp**2 *
+    pi**log10(sp): Fortran and Python writers agree byte for byte."""
+    rng = np.random.default_rng(5)
+    nrow, nrec = 120, 2
+    t = np.cumsum(np.full(nrow, 3.3e-9))
+    fields = rng.standard_normal((nrow, nrec, 9)) * 10.0 ** rng.integers(-14, 0, (nrow, nrec, 9))
+    gates = list(np.logspace(-8, np.log10(t[-1]) - 0.01, 10))
+    o_f = [0, 0, 0, 1, 1, 1, 0, 0, 1]
+    inp = tmp_path / "rows.txt"
+    _write_rows(inp, np.arange(nrow), t, fields, gates, o_f)
+    _run("write", str(inp), str(tmp_path / "f"), repr(1000.0))
+    tfio.write_field(str(tmp_path / "p.field"), np.arange(nrow), t, fields, o_f, 1000.0)
+    tfio.write_field2(str(tmp_path / "p.field2"), t, fields, gates, o_f, 1000.0)
+    for ext in ("field", "field2"):
+        assert open(tmp_path / f"f.{ext}").read() == open(tmp_path / f"p.{ext}").read(), ext
+
+
+def test_fortran_msh_slice_matches_python(src_small, tmp_path):
+    """ROOT_i.msh from the Fortran driver's writer equals the Python writer's."""
+    nT = len(tfio.msh_tet_tags(src_small["msh"]))
+    rng = np.random.default_rng(11)
+    f9 = rng.standard_normal((nT, 9)) * 10.0 ** rng.integers(-15, 2, (nT, 9))
+    o_f = [1, 1, 0, 0, 0, 1, 0, 1, 1]
+    inp = tmp_path / "vals.txt"
+    with open(inp, "w") as f:
+        f.write(f"{nT}\n{' '.join(map(str, o_f))}\n")
+        f.write("".join(" ".join(repr(float(x)) for x in row) + "\n" for row in f9))
+    _run("slice", src_small["msh"], str(inp), str(tmp_path / "f_1.msh"))
+    tfio.write_msh_slice(src_small["msh"], str(tmp_path / "p_1.msh"), f9, o_f)
+    with open(tmp_path / "f_1.msh", "rb") as a, open(tmp_path / "p_1.msh", "rb") as b:
+        assert a.read() == b.read()

@@ -42,3 +42,41 @@ def test_fortran_driver_reproduces_shipped_field(src_mod_big, tmp_path):
     print(f"[gpu] {len(a)} rows, {diff} differing in the last printed digit", flush=True)
     assert diff <= 2
     assert os.path.exists(str(root) + ".field2")
+
+
+def test_fortran_driver_small_variant_with_slices(src_small, tmp_path):
+    """`fetd_amd -f ROOT -r` on the reference's src_small model with two
+    vtkout times: the rows up to the first check whose decision sits near
+    the threshold (step 640) match the shipped 5-digit file, and each slice
+    is written as ROOT_i.msh with the oracle's |dBz/dt| per tet."""
+    import numpy as np
+    from oracle import pyoracle
+    from temf3dt_amd import io as tfio
+    root = str(tmp_path / "m")
+    shutil.copy(src_small["mesh"], root + ".mesh")
+    shutil.copy(src_small["msh"], root + ".msh")
+    times = [1e-6, 5e-5]
+    cntl = open(src_small["cntl"]).read().replace("vtkout: 0", "vtkout: 2\n1e-6\n5e-5")
+    open(root + ".cntl", "w").write(cntl)
+    print("[gpu] fetd_amd -f m -r (src_small, 2 slices)", flush=True)
+    r = subprocess.run([tfbuild.DRIVER, "-f", root, "-r"], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    a, b = _rows(open(root + ".field").read()), _rows(open(src_small["ref_field"]).read())
+    n = next(i for i, x in enumerate(b) if int(x[0]) == 640)
+    for x, y in zip(a[:n], b[:n]):
+        assert x[:2] == y[:2]
+        assert abs(float(x[2]) - float(y[2])) <= 1.01e-5 * abs(float(y[2])) + 1e-300, (x, y)
+    pyoracle.set_threads(min(16, os.cpu_count() or 8))
+    o = pyoracle.Oracle(src_small["mesh"], src_small["cntl"], sig_rescale=True)
+    o.assemble()
+    o.set_slices(times)
+    o.run(max_steps=n)
+    tags = tfio.msh_tet_tags(root + ".msh")
+    for i in range(2):
+        lines = open(f"{root}_{i + 1}.msh").read().splitlines()[-len(tags) - 1:-1]
+        vals = np.array([float(ln[10:]) for ln in lines])
+        f, _, _ = o.slice(i)
+        rel = np.abs(vals - np.abs(f[:, 5])).max() / np.abs(f[:, 5]).max()
+        print(f"[gpu] m_{i + 1}.msh |dBz/dt| max diff / max {rel:.2e}", flush=True)
+        assert rel < 1e-6
