@@ -271,7 +271,7 @@ struct tf_ctx {
     int *d_gptr = nullptr, *d_gidx = nullptr;
     double *d_mZ = nullptr;
     int mt_units = 4096, mt_chmin = 256;  // tile-launch sizing (units per launch, min inner chunk)
-    int mt_ks_n = 2, mt_ks_t = 4;         // k-steps per stage of the N- and T-form tile kernels
+    int mt_ks_n = 2, mt_ks_t = 2;         // k-steps per stage of the N- and T-form tile kernels
     double *d_U = nullptr, *d_V = nullptr, *d_xw = nullptr, *d_Y = nullptr, *d_T = nullptr, *d_W = nullptr;
     int64_t *d_woff = nullptr;
     double *d_L[2] = {nullptr, nullptr}, *d_D[2] = {nullptr, nullptr};
@@ -794,7 +794,7 @@ void run_factor(tf_ctx *c, double *L, double *Dn, const FrontDesc *fd) {
 }
 
 void prepare_slot(tf_ctx *c, int slot) {
-    if (!c->d_L[slot]) c->d_L[slot] = dalloc<double>(c->sym.ltotal);
+    if (!c->d_L[slot]) c->d_L[slot] = dalloc<double>(c->sym.ltotal + TF_READ_SLACK);
     if (!c->d_D[slot]) c->d_D[slot] = dalloc<double>(c->sym.n);
     tf::launch_fill0(c->d_L[slot], c->sym.ltotal, c->st);
     tf::launch_fill0(c->d_U, c->sym.utotal, c->st);
@@ -869,11 +869,13 @@ void ensure_block(tf_ctx *c, int K) {
     if (c->mK >= K) return;
     dfree(c->d_mV); dfree(c->d_mY); dfree(c->d_mT); dfree(c->d_mxw); dfree(c->d_mb); dfree(c->d_mx); dfree(c->d_mZ);
     const int64_t n = c->sym.n;
-    c->d_mV = dalloc<double>((size_t)n * K);  // Yin: own rows after the descendants' updates
+    // the tile kernels' unmasked reads of the last rows may run up to 63
+    // values past an array (see TF_READ_SLACK)
+    c->d_mV = dalloc<double>((size_t)n * K + TF_READ_SLACK);  // Yin: own rows after the descendants' updates
     c->d_mZ = dalloc<double>(std::max<size_t>(c->sym.sidx.size(), 1) * K);  // per-front L21 Y
-    c->d_mY = dalloc<double>((size_t)n * K);
-    c->d_mT = dalloc<double>((size_t)n * K);
-    c->d_mxw = dalloc<double>((size_t)n * K);
+    c->d_mY = dalloc<double>((size_t)n * K + TF_READ_SLACK);
+    c->d_mT = dalloc<double>((size_t)n * K + TF_READ_SLACK);
+    c->d_mxw = dalloc<double>((size_t)n * K + TF_READ_SLACK);
     c->d_mb = dalloc<double>((size_t)n * K);
     c->d_mx = dalloc<double>((size_t)n * K);
     c->mK = K;
@@ -1283,7 +1285,7 @@ void setup_models(tf_ctx *c) {
     free_models(c);
     c->d_qmodel = dupload(qm);
     c->d_mrange = dupload(rg);
-    c->d_Lmm = dalloc<double>(2 * (size_t)s.ltotal);
+    c->d_Lmm = dalloc<double>(2 * (size_t)s.ltotal + TF_READ_SLACK);
     c->d_Dmm = dalloc<double>(2 * (size_t)n);
     c->d_fd_cur = dalloc<FrontDesc>(s.nsn);
     c->d_fd_oth = dalloc<FrontDesc>(s.nsn);

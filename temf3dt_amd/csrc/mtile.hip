@@ -64,58 +64,76 @@ __device__ __forceinline__ void mt_store(const FrontDesc &d, int o, int q, doubl
     }
 }
 
+// Byte-offset load from a wave-uniform base: the base stays in scalar
+// registers and each lane adds a 32-bit offset (no 64-bit address math per
+// load).
+__device__ __forceinline__ double mt_ld(const double *base, uint32_t off) {
+    return *(const double *)((const char *)base + off);
+}
+
+// Raw buffer loads: resource (base, byte size) in scalar registers, a
+// per-lane 32-bit offset and a wave-uniform offset added by the load unit.
+// 0x00020000: the gfx9 buffer-resource word 3 (raw 32-bit data).
+typedef unsigned int mt_u2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t mt_rsrc(const double *base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ double mt_bld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+
 // One wave: acc(out, rhs) = sum_{k0 <= i < k1} A(out, i) B(i, rhs) where
 //   N form: A(out, i) = Lf[i*ld + roff + out]   (column i, row out)
 //   T form: A(out, i) = Lf[out*ld + roff + i]   (column out, row i)
 // B(i, rhs) = Bm[row(i)*K + rhs] with row(i) = i, or bidx[i] when GATH.
 // TRI keeps only i < out (N) / i > out (T).  The inner range is walked in
-// stages of KS k-steps (4*KS inner values).  KS = 2: two register stages,
-// the next one's operands in flight while the current one is multiplied;
-// KS = 4: one stage at a time (a T-form lane then reads 32 contiguous bytes
-// of a column per stage, whole 128-byte lines per lane group).
+// stages of KS k-steps (4*KS inner values; a T-form stage starts on a
+// 16-row line of the column).
+//
+// The factor block and the front's own right-hand-side rows are read with
+// raw buffer loads: per-lane offsets in vector registers, the stage offset
+// in a scalar register, and a read whose per-lane offset lies past the end
+// of the block returns zero.  Output rows/columns and right-hand-side
+// columns outside the tile's valid range are therefore read without masks
+// (they land in neighbouring rows or past the block) and their results are
+// never stored.  Stages inside [k0, k1) and off the
+// triangle form one contiguous, unmasked run, pipelined in pairs of stages:
+// the next stage's loads are issued (the last pair re-reads the final stage)
+// before the current one is multiplied, so each wait covers only the stage
+// being multiplied; gathered rows are fetched before the multiply that
+// precedes their stage's loads.  The
+// stages before and after that run clamp the inner index and zero what lies
+// outside.
 template <bool TFORM, bool TRI, bool GATH, int KS>
-__device__ __forceinline__ void mt_wave(const double *Lf, int64_t f, int roff, int nout, int o0, int k0, int k1,
-                                        const double *Bm, const int *bidx, int K, tf_d4 acc[4][4]) {
+__device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f, int roff, int o0, int k0, int k1,
+                                        const double *Bm, uint32_t bbytes, const int *bidx, int K, tf_d4 acc[4][4]) {
     constexpr int SW = 4 * KS;  // inner values per stage
+    if (k0 >= k1) return;
     const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
-    int ob[4];
-    bool ov[4], qv[4];
+    const __amdgpu_buffer_rsrc_t rL = mt_rsrc(Lf, lbytes), rR = mt_rsrc(Bm, bbytes);
+    // lane offsets (bytes) at the stage origin.  Only the per-lane offset and
+    // the immediate take part in a buffer load's range check, so whatever
+    // can leave the block stays there: the T-form output column (one offset
+    // per output block b), while an N-form row past the block's last column
+    // reads at most 62 values beyond it (the factor storage ends with slack).
+    uint32_t la[TFORM ? 4 : 1];
 #pragma unroll
-    for (int b = 0; b < 4; b++) {
-        ob[b] = o0 + 16 * b + l16;
-        ov[b] = ob[b] >= 0 && ob[b] < nout;
-        qv[b] = 16 * b + l16 < K;
-    }
-    // T form: stages start on 16-row boundaries of the column (columns are
-    // 128-byte aligned), so each stage reads whole cache lines
+    for (int b = 0; b < (TFORM ? 4 : 1); b++)
+        la[b] = TFORM ? (uint32_t)((int64_t)(o0 + 16 * b + l16) * f + roff + KS * lk) * 8u
+                      : (uint32_t)(lk * f + roff + o0 + l16) * 8u;
+    const uint32_t ba = (uint32_t)((TFORM ? KS * lk : lk) * K + l16) * 8u;
+    uint32_t bq[4];  // gathered rows: rhs column clamped into the row
+    if (GATH)
+#pragma unroll
+        for (int q = 0; q < 4; q++) bq[q] = (uint32_t)min(16 * q + l16, K - 1) * 8u;
     const int kb = TFORM ? (k0 - ((roff + k0) & 15)) : k0;
-    // inner index of k-step s of the stage starting at ib
+    const int nst = (k1 - kb + SW - 1) / SW;
+    // the unmasked run of stages [ja, jb)
+    const int lo = (TRI && TFORM) ? max(k0, o0 + 64) : k0;
+    const int hi = (TRI && !TFORM) ? min(k1, o0) : k1;
+    const int ja = min(nst, (max(lo - kb, 0) + SW - 1) / SW);
+    const int jb = max(ja, (hi - kb) / SW);
     auto inner = [&](int ib, int s) { return TFORM ? (ib + KS * lk + s) : (ib + 4 * s + lk); };
-    // gathered B rows of a stage: fetched one stage ahead of the operands
-    auto ldidx = [&](int ib, int r[KS]) {
-#pragma unroll
-        for (int s = 0; s < KS; s++) {
-            const int i = inner(ib, s);
-            r[s] = (GATH && i >= k0 && i < k1) ? bidx[i] : 0;
-        }
-    };
-    auto load = [&](int ib, const int r[KS], double xa[KS][4], double yb[KS][4]) {
-#pragma unroll
-        for (int s = 0; s < KS; s++) {
-            const int i = inner(ib, s);
-            const bool iv = i >= k0 && i < k1;
-            const int64_t brow = GATH ? (int64_t)r[s] : (int64_t)i;
-#pragma unroll
-            for (int b = 0; b < 4; b++) {
-                bool ok = iv && ov[b];
-                if (TRI) ok = ok && (TFORM ? (i > ob[b]) : (i < ob[b]));
-                const int64_t addr = TFORM ? ((int64_t)ob[b] * f + roff + i) : ((int64_t)i * f + roff + ob[b]);
-                xa[s][b] = ok ? Lf[addr] : 0.0;
-            }
-#pragma unroll
-            for (int a = 0; a < 4; a++) yb[s][a] = (iv && qv[a]) ? Bm[brow * K + 16 * a + l16] : 0.0;
-        }
-    };
     auto mult = [&](const double xa[KS][4], const double yb[KS][4]) {
 #pragma unroll
         for (int s = 0; s < KS; s++)
@@ -125,45 +143,92 @@ __device__ __forceinline__ void mt_wave(const double *Lf, int64_t f, int roff, i
                 for (int b = 0; b < 4; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s][b], yb[s][a], acc[a][b], 0, 0, 0);
     };
-    if (k0 >= k1) return;
-    if (KS == 4) {
+    // a masked stage, loaded and multiplied on its own
+    auto masked_stage = [&](int js) {
+        const int ib = kb + js * SW;
         double xa[KS][4], yb[KS][4];
-        int rA[KS], rN[KS];
-        ldidx(kb, rA);
-        for (int ib = kb; ib < k1; ib += SW) {
-            if (GATH) ldidx(ib + SW, rN);
-            load(ib, rA, xa, yb);
-            mult(xa, yb);
-            if (GATH)
 #pragma unroll
-                for (int s = 0; s < KS; s++) rA[s] = rN[s];
+        for (int s = 0; s < KS; s++) {
+            const int i = inner(ib, s);
+            const bool iv = i >= k0 && i < k1;
+            const int ic = min(max(i, k0), k1 - 1);
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int ob = o0 + 16 * b + l16;
+                bool ok = iv;
+                if (TRI) ok = ok && (TFORM ? (i > ob) : (i < ob));
+                const uint32_t off = TFORM ? (uint32_t)((int64_t)ob * f + roff + ic) * 8u
+                                           : (uint32_t)((int64_t)ic * f + roff + ob) * 8u;
+                const double x = mt_bld(rL, off, 0u);
+                xa[s][b] = ok ? x : 0.0;
+            }
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const double y = GATH ? mt_ld(Bm + (int64_t)bidx[ic] * K, bq[a])
+                                      : mt_bld(rR, (uint32_t)(ic * K + 16 * a + l16) * 8u, 0u);
+                yb[s][a] = iv ? y : 0.0;
+            }
         }
-        return;
+        mult(xa, yb);
+    };
+    if (TFORM)  // an N-form run starts at k0 itself (ja = 0)
+        for (int js = 0; js < ja; js++) masked_stage(js);
+    if (ja < jb) {
+        auto ldidx = [&](int js, int r[KS]) {
+            const int ib = kb + js * SW;
+#pragma unroll
+            for (int s = 0; s < KS; s++) r[s] = bidx[inner(ib, s)];
+        };
+        auto load = [&](int js, const int r[KS], double xa[KS][4], double yb[KS][4]) {
+            const int ib = kb + js * SW;
+#pragma unroll
+            for (int s = 0; s < KS; s++) {
+                const uint32_t sl = (uint32_t)(TFORM ? (ib + s) : (ib + 4 * s) * f) * 8u;
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    xa[s][b] = TFORM ? mt_bld(rL, la[b], sl) : mt_bld(rL, la[0] + 128u * b, sl);
+                if (GATH) {
+                    const double *Br = Bm + (int64_t)r[s] * K;
+#pragma unroll
+                    for (int a = 0; a < 4; a++) yb[s][a] = mt_ld(Br, bq[a]);
+                } else {
+                    const uint32_t sb = (uint32_t)((TFORM ? ib + s : ib + 4 * s) * K) * 8u;
+#pragma unroll
+                    for (int a = 0; a < 4; a++) yb[s][a] = mt_bld(rR, ba, sb + 128u * a);
+                }
+            }
+        };
+        double xa0[KS][4], yb0[KS][4], xa1[KS][4], yb1[KS][4];
+        int rA[KS], rB[KS];
+        const int last = jb - 1;
+        int js = ja;
+        if ((jb - ja) & 1) {  // odd run: its first stage on its own, the rest in pairs
+            if (GATH) ldidx(js, rA);
+            load(js, rA, xa0, yb0);
+            mult(xa0, yb0);
+            js++;
+        }
+        if (js < jb) {
+            if (GATH) {
+                ldidx(js, rA);
+                ldidx(js + 1, rB);
+            }
+            load(js, rA, xa0, yb0);
+            for (; js < jb; js += 2) {  // one exit per pair; js + 1 <= last
+                load(js + 1, rB, xa1, yb1);
+                if (GATH) ldidx(min(js + 2, last), rA);
+                __builtin_amdgcn_sched_barrier(0);
+                mult(xa0, yb0);
+                __builtin_amdgcn_sched_barrier(0);
+                load(min(js + 2, last), rA, xa0, yb0);
+                if (GATH) ldidx(min(js + 3, last), rB);
+                __builtin_amdgcn_sched_barrier(0);
+                mult(xa1, yb1);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
     }
-    double xa0[KS][4], yb0[KS][4], xa1[KS][4], yb1[KS][4];
-    int rA[KS], rB[KS];
-    ldidx(kb, rA);
-    ldidx(kb + SW, rB);
-    load(kb, rA, xa0, yb0);
-    int ib = kb;
-    for (;;) {
-        const bool more = ib + SW < k1;
-        if (more) {
-            if (GATH) ldidx(ib + 2 * SW, rA);
-            load(ib + SW, rB, xa1, yb1);
-        }
-        mult(xa0, yb0);
-        if (!more) break;
-        ib += SW;
-        const bool more2 = ib + SW < k1;
-        if (more2) {
-            if (GATH) ldidx(ib + 2 * SW, rB);
-            load(ib + SW, rA, xa0, yb0);
-        }
-        mult(xa1, yb1);
-        if (!more2) break;
-        ib += SW;
-    }
+    for (int js = jb; js < nst; js++) masked_stage(js);
 }
 
 template <int MODE, int KS>
@@ -181,17 +246,21 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
     acc16_zero(acc);
     const bool tri = (MODE == MT_OWN_FWD) ? (tu.k1 > tu.t0) : (MODE == MT_BWD_X) ? (tu.k0 < tu.t0 + 64) : false;
     const double *Lf = L + d.loff;
+    const uint32_t lbytes = (uint32_t)((int64_t)d.nc * d.ld * 8);
     // B operand rows: own right-hand sides (Yin, Y_own, T) at c0, or the
     // solution rows of the struct indices (backward T)
     const double *Bm = in0 + (int64_t)d.c0 * K;
-    if (MODE == MT_OWN_FWD || MODE == MT_STR_FWD) {
-        if (tri) mt_wave<false, true, false, KS>(Lf, d.ld, roff, nout, tu.t0, tu.k0, tu.k1, Bm, nullptr, K, acc);
-        else mt_wave<false, false, false, KS>(Lf, d.ld, roff, nout, tu.t0, tu.k0, tu.k1, Bm, nullptr, K, acc);
+    const uint32_t bbytes = (uint32_t)((int64_t)d.nc * K * 8);  // the front's own right-hand-side rows
+    if (MODE == MT_OWN_FWD) {
+        if (tri) mt_wave<false, true, false, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
+        else mt_wave<false, false, false, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
+    } else if (MODE == MT_STR_FWD) {
+        mt_wave<false, false, false, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
     } else if (MODE == MT_BWD_T) {
-        mt_wave<true, false, true, KS>(Lf, d.ld, roff, nout, tu.t0, tu.k0, tu.k1, in0, sidx + d.soff, K, acc);
+        mt_wave<true, false, true, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, in0, 0u, sidx + d.soff, K, acc);
     } else {
-        if (tri) mt_wave<true, true, false, KS>(Lf, d.ld, roff, nout, tu.t0, tu.k0, tu.k1, Bm, nullptr, K, acc);
-        else mt_wave<true, false, false, KS>(Lf, d.ld, roff, nout, tu.t0, tu.k0, tu.k1, Bm, nullptr, K, acc);
+        if (tri) mt_wave<true, true, false, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
+        else mt_wave<true, false, false, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
     }
     const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
     if (tu.slot < 0) {

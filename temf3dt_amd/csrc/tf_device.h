@@ -11,6 +11,11 @@
 #define TF_UT 128      // tile edge of the matrix-core trailing update
 #define TF_SP 256      // super-panel: panel columns whose trailing update is applied at once
 
+// Values allocated past the end of the factor storage and of the block-solve
+// vectors: the tile kernels' unmasked stage loads may read up to 63 values
+// beyond the last row of an array (results of those lanes are discarded).
+#define TF_READ_SLACK 4096
+
 // L11 inversion by doubling: leading dimension of the scratch block T (|J| x b)
 // of a merge at block size b in a front of nc columns (|J| <= min(b, nc - b)).
 __host__ __device__ inline int inv_ldt(int nc, int b) { return ((nc - b < b ? nc - b : b) + 15) & ~15; }

@@ -78,6 +78,8 @@ program fetd_amd
   if (ierr /= 0) call die(msg)
   call read_cntl(trim(root)//'.cntl', c, ierr, msg)
   if (ierr /= 0) call die(msg)
+  if (c%sp_mthd /= 2) call die('error!')  ! only the adaptive BDF2 method exists (src/FETD_FWD.f90:1210-1219)
+  if (c%bound_cond /= 0) call die('absorbing boundary condition is not supported')
   if (rescale) then
     if (c%aniso) call die('the conductivity rescaling applies to isotropic domain tables')
     call rescale_domains(c)
