@@ -88,7 +88,7 @@ void launch_scale(int n, const double *s, double *g, hipStream_t st);
 // mtile.hip
 void launch_mtile(int mode, int ks, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits, int K,
                   const double *in0, const double *in1, const double *Dn, const int *sidx, double *out, double *part,
-                  hipStream_t st);
+                  uint32_t gbytes, hipStream_t st);
 void launch_mpull(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg, const double *b,
                   const int *perm, const int *gptr, const int *gidx, const double *Z, double *Yin, int K,
                   hipStream_t st);
@@ -885,6 +885,10 @@ void ensure_block(tf_ctx *c, int K) {
 void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd, int K, const double *b,
                  double *x) {
     ensure_block(c, K);
+    // the tile kernels address the gathered solution vector with 32-bit byte offsets
+    const uint64_t gb = (uint64_t)c->sym.n * K * 8;
+    if (gb >= (1ull << 32)) throw std::runtime_error("block solve: n x K too large for the tile kernels (>= 4 GiB)");
+    const uint32_t gbytes = (uint32_t)gb;
     const Plan &p = c->plan;
     const int *lst = c->d_plist;
     const int64_t *pr = c->d_ppref;
@@ -897,7 +901,7 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
             break;
         case L_MT:  // own forward (Y from Yin), struct forward (Z from Y)
             tf::launch_mtile(l.p0, l.p0 >= 2 ? c->mt_ks_t : c->mt_ks_n, L, fd, c->d_units + l.off, l.cnt, K, l.p0 == 0 ? c->d_mV : c->d_mY, nullptr,
-                             Dn, nullptr, l.p0 == 0 ? c->d_mY : c->d_mZ, c->d_part, c->st);
+                             Dn, nullptr, l.p0 == 0 ? c->d_mY : c->d_mZ, c->d_part, 0u, c->st);
             break;
         case L_MTRED:
             tf::launch_mtile_red(l.p0, fd, c->d_units + l.off, l.cnt, K, l.p0 == 0 ? c->d_mV : c->d_mY, nullptr,
@@ -912,7 +916,7 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
         case L_MT:  // backward T (from x at the struct rows, Y, D), backward X (from T)
             tf::launch_mtile(l.p0, l.p0 >= 2 ? c->mt_ks_t : c->mt_ks_n, L, fd, c->d_units + l.off, l.cnt, K, l.p0 == 2 ? c->d_mxw : c->d_mT,
                              l.p0 == 2 ? c->d_mY : c->d_mT, Dn, c->d_sidx, l.p0 == 2 ? c->d_mT : c->d_mxw, c->d_part,
-                             c->st);
+                             gbytes, c->st);
             break;
         case L_MTRED:
             tf::launch_mtile_red(l.p0, fd, c->d_units + l.off, l.cnt, K, l.p0 == 2 ? c->d_mxw : c->d_mT,

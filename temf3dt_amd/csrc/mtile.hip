@@ -81,6 +81,13 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t mt_rsrc(const double *base, ui
 __device__ __forceinline__ double mt_bld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
 }
+typedef unsigned int mt_u4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void mt_bld2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, double &x0,
+                                        double &x1) {
+    const mt_u4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0);
+    x0 = __builtin_bit_cast(double, (mt_u2){v.x, v.y});
+    x1 = __builtin_bit_cast(double, (mt_u2){v.z, v.w});
+}
 
 // One wave: acc(out, rhs) = sum_{k0 <= i < k1} A(out, i) B(i, rhs) where
 //   N form: A(out, i) = Lf[i*ld + roff + out]   (column i, row out)
@@ -121,11 +128,7 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
     for (int b = 0; b < (TFORM ? 4 : 1); b++)
         la[b] = TFORM ? (uint32_t)((int64_t)(o0 + 16 * b + l16) * f + roff + KS * lk) * 8u
                       : (uint32_t)(lk * f + roff + o0 + l16) * 8u;
-    const uint32_t ba = (uint32_t)((TFORM ? KS * lk : lk) * K + l16) * 8u;
-    uint32_t bq[4];  // gathered rows: rhs column clamped into the row
-    if (GATH)
-#pragma unroll
-        for (int q = 0; q < 4; q++) bq[q] = (uint32_t)min(16 * q + l16, K - 1) * 8u;
+    const uint32_t ba = GATH ? (uint32_t)l16 * 8u : (uint32_t)((TFORM ? KS * lk : lk) * K + l16) * 8u;
     const int kb = TFORM ? (k0 - ((roff + k0) & 15)) : k0;
     const int nst = (k1 - kb + SW - 1) / SW;
     // the unmasked run of stages [ja, jb)
@@ -164,8 +167,8 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
             }
 #pragma unroll
             for (int a = 0; a < 4; a++) {
-                const double y = GATH ? mt_ld(Bm + (int64_t)bidx[ic] * K, bq[a])
-                                      : mt_bld(rR, (uint32_t)(ic * K + 16 * a + l16) * 8u, 0u);
+                const uint32_t row = (uint32_t)(GATH ? bidx[ic] : ic);
+                const double y = mt_bld(rR, (row * (uint32_t)K + 16u * a + l16) * 8u, 0u);
                 yb[s][a] = iv ? y : 0.0;
             }
         }
@@ -181,16 +184,23 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
         };
         auto load = [&](int js, const int r[KS], double xa[KS][4], double yb[KS][4]) {
             const int ib = kb + js * SW;
+            if (TFORM && KS % 2 == 0) {  // a lane's KS rows of a column are contiguous: 16-byte loads
+#pragma unroll
+                for (int s = 0; s < KS; s += 2)
+#pragma unroll
+                    for (int b = 0; b < 4; b++) mt_bld2(rL, la[b], (uint32_t)(ib + s) * 8u, xa[s][b], xa[s + 1][b]);
+            }
 #pragma unroll
             for (int s = 0; s < KS; s++) {
                 const uint32_t sl = (uint32_t)(TFORM ? (ib + s) : (ib + 4 * s) * f) * 8u;
+                if (!(TFORM && KS % 2 == 0))
 #pragma unroll
-                for (int b = 0; b < 4; b++)
-                    xa[s][b] = TFORM ? mt_bld(rL, la[b], sl) : mt_bld(rL, la[0] + 128u * b, sl);
+                    for (int b = 0; b < 4; b++)
+                        xa[s][b] = TFORM ? mt_bld(rL, la[b], sl) : mt_bld(rL, la[0] + 128u * b, sl);
                 if (GATH) {
-                    const double *Br = Bm + (int64_t)r[s] * K;
+                    const uint32_t vo = ((uint32_t)r[s] * (uint32_t)K) * 8u + ba;
 #pragma unroll
-                    for (int a = 0; a < 4; a++) yb[s][a] = mt_ld(Br, bq[a]);
+                    for (int a = 0; a < 4; a++) yb[s][a] = mt_bld(rR, vo + 128u * a, 0u);
                 } else {
                     const uint32_t sb = (uint32_t)((TFORM ? ib + s : ib + 4 * s) * K) * 8u;
 #pragma unroll
@@ -234,7 +244,8 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
 template <int MODE, int KS>
 __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDesc *fd, const TileUnit *units,
                                                   int nunits, int K, const double *in0, const double *in1,
-                                                  const double *Dn, const int *sidx, double *out, double *part) {
+                                                  const double *Dn, const int *sidx, double *out, double *part,
+                                                  uint32_t gbytes) {
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int u = xcd_block(blockIdx.x, gridDim.x) * 4 + w;
     if (u >= nunits) return;
@@ -257,7 +268,7 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
     } else if (MODE == MT_STR_FWD) {
         mt_wave<false, false, false, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
     } else if (MODE == MT_BWD_T) {
-        mt_wave<true, false, true, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, in0, 0u, sidx + d.soff, K, acc);
+        mt_wave<true, false, true, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, in0, gbytes, sidx + d.soff, K, acc);
     } else {
         if (tri) mt_wave<true, true, false, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
         else mt_wave<true, false, false, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
@@ -309,12 +320,14 @@ __global__ void __launch_bounds__(256) k_mtile_red(const FrontDesc *fd, const Ti
 
 namespace tf {
 // mode: 0 own forward, 1 struct forward, 2 backward T, 3 backward X
+// gbytes: size of the gathered vector (backward T form), < 4 GiB
 void launch_mtile(int mode, int ks, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits, int K,
                   const double *in0, const double *in1, const double *Dn, const int *sidx, double *out, double *part,
-                  hipStream_t st) {
+                  uint32_t gbytes, hipStream_t st) {
     if (nunits <= 0) return;
     const dim3 g((nunits + 3) / 4), b(256);
-#define TF_MT(M, S) hipLaunchKernelGGL((k_mtile<M, S>), g, b, 0, st, L, fd, units, nunits, K, in0, in1, Dn, sidx, out, part)
+#define TF_MT(M, S) \
+    hipLaunchKernelGGL((k_mtile<M, S>), g, b, 0, st, L, fd, units, nunits, K, in0, in1, Dn, sidx, out, part, gbytes)
     if (ks == 4) {
         switch (mode) {
         case MT_OWN_FWD: TF_MT(MT_OWN_FWD, 4); break;
