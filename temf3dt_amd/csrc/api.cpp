@@ -549,12 +549,12 @@ void build_plan(tf_ctx *c) {
                     int rows = f_of(k) - p1;
                     act.push_back(k);
                     ut.push_back((rows + TF_TRSM_ROWS - 1) / TF_TRSM_ROWS);
-                    if (p1 < pe) {
-                        int64_t nt = (rows + TF_UT - 1) / TF_UT, ncj = (pe - p1 + TF_UT - 1) / TF_UT;
+                    if (p1 < pe) {  // 64 x 64 update tiles, one per wave
+                        int64_t nt = (rows + 63) / 64, ncj = (pe - p1 + 63) / 64;
                         a0.push_back(k);
                         u0.push_back(ncj * nt - ncj * (ncj - 1) / 2);
                     } else if (f_of(k) > pe) {
-                        int64_t nt = (f_of(k) - pe + TF_UT - 1) / TF_UT;
+                        int64_t nt = (f_of(k) - pe + 63) / 64;
                         a1.push_back(k);
                         u1.push_back(nt * (nt + 1) / 2);
                     }

@@ -139,103 +139,7 @@ __device__ void blk_update_tile(double *L, double *U, const FrontDesc &d, int p0
     __syncthreads();
 }
 
-#define TF_ULD (TF_UT + 16)
-
-// Trailing update on matrix cores, C -= L_rows D L_cols^T, over the panel
-// columns [kbeg, kend) (up to TF_SP of them) for the TF_UT x TF_UT tile
-// (ti, tj) of the region that starts at row/column r0; columns are limited
-// to [r0, cmax).  Four waves, each a 64 x 64 sub-tile of 4 x 4 blocks of
-// v_mfma_f64_16x16x4f64.  Lane layout of that instruction (measured): lane l
-// supplies A[l%16][l/16] and B[l/16][l%16] and receives D[4r + l/16][l%16]
-// in result register r.  Operand roles are chosen so that the result's
-// fast index is the tile row: lane l holds row (l & 15) of columns
-// 4r + (l >> 4), so every store covers four 128-byte row segments.
-__device__ void blk_update_tile_mfma(double *L, double *U, const FrontDesc &d, int kbeg, int kend, int r0,
-                                     int cmax, const double *Dn, int ti, int tj, double *lds) {
-    const int kw = kend - kbeg;
-    const int rb = r0 + ti * TF_UT, cb = r0 + tj * TF_UT;
-    const double *base = L + d.loff;
-    const int KH = 32;                   // panel columns staged at a time
-    double *Ap = lds;                    // [k][TF_ULD]: panel values of the tile rows
-    double *Bp = lds + KH * TF_ULD;      // [k][TF_ULD]: panel values of the tile columns, times d_k
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int wr = w >> 1, wc = w & 1;
-    const int l16 = lane & 15, lk = lane >> 4;
-    tf_d4 acc[4][4];
-#pragma unroll
-    for (int bi = 0; bi < 4; bi++)
-#pragma unroll
-        for (int bj = 0; bj < 4; bj++) acc[bi][bj] = (tf_d4){0.0, 0.0, 0.0, 0.0};
-    for (int kh = 0; kh < kw; kh += KH) {
-        // stage KH panel columns in two batches of 8 loads per operand per thread
-        constexpr int PER = 32 * TF_UT / TF_NT;  // 16
-#pragma unroll
-        for (int half = 0; half < PER; half += 8) {
-            double va[8], vb[8];
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const int idx = (half + q) * TF_NT + threadIdx.x;
-                const int k = kh + idx / TF_UT, i = idx % TF_UT;
-                va[q] = 0.0;
-                vb[q] = 0.0;
-                if (k < kw) {
-                    const double *col = base + (int64_t)(kbeg + k) * d.ld;
-                    if (rb + i < d.f) va[q] = col[rb + i];
-                    if (cb + i < cmax) vb[q] = col[cb + i];
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const int idx = (half + q) * TF_NT + threadIdx.x;
-                const int kk = idx / TF_UT, i = idx % TF_UT;
-                const int k = kh + kk;
-                Ap[kk * TF_ULD + i] = va[q];
-                Bp[kk * TF_ULD + i] = (k < kw) ? vb[q] * Dn[d.doff + kbeg + k] : 0.0;
-            }
-        }
-        __syncthreads();
-        const int kend4 = min(KH, ((kw - kh) + 3) & ~3);
-        for (int k0 = 0; k0 < kend4; k0 += 4) {
-            const int kk = k0 + lk;
-            double x[4], y[4];
-#pragma unroll
-            for (int bj = 0; bj < 4; bj++) x[bj] = Bp[kk * TF_ULD + 64 * wc + 16 * bj + l16];
-#pragma unroll
-            for (int bi = 0; bi < 4; bi++) y[bi] = Ap[kk * TF_ULD + 64 * wr + 16 * bi + l16];
-#pragma unroll
-            for (int bi = 0; bi < 4; bi++)
-#pragma unroll
-                for (int bj = 0; bj < 4; bj++)
-                    acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[bj], y[bi], acc[bi][bj], 0, 0, 0);
-        }
-        __syncthreads();
-    }
-    // epilogue: per result column block, load all C values first, then store
-#pragma unroll
-    for (int bj = 0; bj < 4; bj++) {
-        double cv[4][4];
-        double *cp[4];
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int c = cb + 64 * wc + 16 * bj + 4 * r + lk;
-            cp[r] = (c < cmax) ? front_col(L, U, d, c) : nullptr;
-#pragma unroll
-            for (int bi = 0; bi < 4; bi++) {
-                const int row = rb + 64 * wr + 16 * bi + l16;
-                cv[r][bi] = (cp[r] && row < d.f && row >= c) ? cp[r][row] : 0.0;
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int c = cb + 64 * wc + 16 * bj + 4 * r + lk;
-#pragma unroll
-            for (int bi = 0; bi < 4; bi++) {
-                const int row = rb + 64 * wr + 16 * bi + l16;
-                if (cp[r] && row < d.f && row >= c) cp[r][row] = cv[r][bi] - acc[bi][bj][r];
-            }
-        }
-    }
-}
+#define TF_ULD (TF_UT + 16)  // LDS row stride of the inversion's staged operands
 
 // Invert the unit-lower pw x pw diagonal block of panel p0 into LDS
 // (column-major, ld 65, strict lower part; the diagonal is implicit).
@@ -360,6 +264,124 @@ __device__ void inv_tile(const double *A, int lda, const double *B, int ldb, int
     }
 }
 
+// Wave-level trailing update on matrix cores: one wave updates the 64 x 64
+// tile C(rb.., cb..) -= L_rows D L_cols^T over the panel columns
+// [kbeg, kend), on the lower triangle of the trailing block (rows < f,
+// columns < cmax).  Both operands are read straight from the panel columns
+// into the MFMA lane layout with raw buffer loads (stage offset in a scalar
+// register, one lane offset per operand, block offsets as immediates; rows
+// past the front read into the next column or the storage slack and are
+// never stored), four inner values per k-step, KS k-steps per stage, the
+// next stage in flight while the current one is multiplied; a last partial
+// stage zeroes the columns past kend.  The column operand is scaled by the
+// pivots and passed as the MFMA's A operand, so that result register r of
+// block (a, b) holds column cb + 16a + 4r + l/16, row rb + 16b + l%16: every
+// store covers four 128-byte row segments.
+template <int KS>
+__device__ __forceinline__ void upd_wave(double *L, double *U, const FrontDesc &d, const double *Dn, int kbeg,
+                                         int kend, int rb, int cb, int cmax) {
+    constexpr int SW = 4 * KS;
+    const double *base = L + d.loff;
+    const __amdgpu_buffer_rsrc_t rL = mt_rsrc(base, (uint32_t)((int64_t)d.nc * d.ld * 8));
+    const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
+    const uint32_t lr = (uint32_t)(lk * d.ld + rb + l16) * 8u, lc = (uint32_t)(lk * d.ld + cb + l16) * 8u;
+    const double *Dk = Dn + d.doff;
+    tf_d4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = (tf_d4){0.0, 0.0, 0.0, 0.0};
+    auto mult = [&](const double xr[KS][4], const double yc[KS][4]) {
+#pragma unroll
+        for (int s = 0; s < KS; s++)
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(yc[s][a], xr[s][b], acc[a][b], 0, 0, 0);
+    };
+    auto load = [&](int ib, double xr[KS][4], double yc[KS][4]) {
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const uint32_t sl = (uint32_t)((ib + 4 * s) * d.ld) * 8u;
+            const double dk = Dk[ib + 4 * s + lk];
+#pragma unroll
+            for (int b = 0; b < 4; b++) xr[s][b] = mt_bld(rL, lr + 128u * b, sl);
+#pragma unroll
+            for (int a = 0; a < 4; a++) yc[s][a] = mt_bld(rL, lc + 128u * a, sl) * dk;
+        }
+    };
+    const int nfull = (kend - kbeg) / SW;
+    int ib = kbeg;
+    if (nfull > 0) {
+        double xr0[KS][4], yc0[KS][4], xr1[KS][4], yc1[KS][4];
+        int j = 0;
+        if (nfull & 1) {  // odd count: the first stage on its own, the rest in pairs
+            load(ib, xr0, yc0);
+            mult(xr0, yc0);
+            ib += SW;
+            j = 1;
+        }
+        if (j < nfull) {
+            const int last = kbeg + (nfull - 1) * SW;
+            load(ib, xr0, yc0);
+            for (; j < nfull; j += 2) {  // ib + SW <= last
+                load(ib + SW, xr1, yc1);
+                __builtin_amdgcn_sched_barrier(0);
+                mult(xr0, yc0);
+                __builtin_amdgcn_sched_barrier(0);
+                load(min(ib + 2 * SW, last), xr0, yc0);
+                __builtin_amdgcn_sched_barrier(0);
+                mult(xr1, yc1);
+                __builtin_amdgcn_sched_barrier(0);
+                ib += 2 * SW;
+            }
+        }
+    }
+    if (ib < kend) {  // partial stage: columns past kend read as zero
+        double xr[KS][4], yc[KS][4];
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const int k = ib + 4 * s + lk;
+            const bool kv = k < kend;
+            const int kc = kv ? k : kend - 1;
+            const uint32_t sl = (uint32_t)(kc * d.ld) * 8u;
+            const uint32_t l0 = (uint32_t)(rb + l16) * 8u, c0 = (uint32_t)(cb + l16) * 8u;
+            const double dk = kv ? Dk[kc] : 0.0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) xr[s][b] = mt_bld(rL, sl + l0 + 128u * b, 0u);
+#pragma unroll
+            for (int a = 0; a < 4; a++) yc[s][a] = mt_bld(rL, sl + c0 + 128u * a, 0u) * dk;
+        }
+        mult(xr, yc);
+    }
+    // C -= acc on the stored lower triangle; all loads of a column block first
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        double cv[4][4];
+        double *cp[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int c = cb + 16 * a + 4 * r + lk;
+            cp[r] = (c < cmax) ? front_col(L, U, d, c) : nullptr;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int row = rb + 16 * b + l16;
+                cv[r][b] = (cp[r] && row < d.f && row >= c) ? cp[r][row] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int c = cb + 16 * a + 4 * r + lk;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int row = rb + 16 * b + l16;
+                if (cp[r] && row < d.f && row >= c) cp[r][row] = cv[r][b] - acc[a][b][r];
+            }
+        }
+    }
+}
+
 // ---- kernels ---------------------------------------------------------------
 
 __global__ void k_fill0(double *p, int64_t n) {
@@ -434,33 +456,36 @@ __global__ void __launch_bounds__(TF_NT) k_factor_trsm(double *L, const FrontDes
     blk_trsm(L, d, p0, r0, Dn, lds);
 }
 
-// Updates of a super-panel [P0, P0 + TF_SP) of TF_PB-column panels:
+// Updates of a super-panel [P0, P0 + TF_SP) of TF_PB-column panels, one
+// 64 x 64 tile per wave:
 //   mode 0 (after panel p0): the rest of the super-panel's own columns,
 //          rows from p1 down, with panel p0 only;
 //   mode 1 (p0 = P0, after its last panel): everything right of the
 //          super-panel with all of its columns at once.
 // The trailing block is read and written once per super-panel instead of
-// once per panel.
-__global__ void __launch_bounds__(TF_NT) k_factor_update(double *L, double *U, const FrontDesc *fd,
-                                                         const int *list, const int64_t *pref,
-                                                         int cnt, int p0, int mode, const double *Dn) {
-    __shared__ double lds[2 * 32 * TF_ULD];
-    const int i = seg_find(pref, cnt, blockIdx.x);
+// once per panel.  pref[] counts tiles per front.
+__global__ void __launch_bounds__(TF_NT, 2) k_factor_update(double *L, double *U, const FrontDesc *fd,
+                                                            const int *list, const int64_t *pref, int cnt,
+                                                            int p0, int mode, const double *Dn) {
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t ug = (int64_t)xcd_block(blockIdx.x, gridDim.x) * 4 + w;
+    if (ug >= pref[cnt]) return;
+    const int i = seg_find(pref, cnt, ug);
     const FrontDesc d = fd[list[i]];
-    const int64_t u = (int64_t)blockIdx.x - pref[i];
+    const int64_t u = ug - pref[i];
     if (mode == 0) {
         const int P0 = p0 - p0 % TF_SP;
         const int p1 = min(d.nc, p0 + TF_PB), pe = min(d.nc, P0 + TF_SP);
-        const int nt = (d.f - p1 + TF_UT - 1) / TF_UT;
+        const int nt = (d.f - p1 + 63) / 64;
         int tj = 0, ti = (int)u;
         while (ti >= nt - tj) { ti -= nt - tj; tj++; }
         ti += tj;
-        blk_update_tile_mfma(L, U, d, p0, p1, p1, pe, Dn, ti, tj, lds);
+        upd_wave<2>(L, U, d, Dn, p0, p1, p1 + 64 * ti, p1 + 64 * tj, pe);
     } else {
         const int pe = min(d.nc, p0 + TF_SP);
         int ti, tj;
         tri_index(u, ti, tj);
-        blk_update_tile_mfma(L, U, d, p0, pe, pe, d.f, Dn, ti, tj, lds);
+        upd_wave<2>(L, U, d, Dn, p0, pe, pe + 64 * ti, pe + 64 * tj, d.f);
     }
 }
 
@@ -558,11 +583,12 @@ void launch_factor_trsm(double *L, const FrontDesc *fd, const int *list, const i
     hipLaunchKernelGGL(k_factor_trsm, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt,
                        p0, Dn);
 }
+// ntiles: 64 x 64 tiles in all (pref[cnt]); four per workgroup
 void launch_factor_update(double *L, double *U, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
-                          int64_t nwg, int p0, int mode, const double *Dn, hipStream_t st) {
-    if (nwg > 0)
-        hipLaunchKernelGGL(k_factor_update, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, U, fd, list, pref, cnt, p0, mode,
-                           Dn);
+                          int64_t ntiles, int p0, int mode, const double *Dn, hipStream_t st) {
+    if (ntiles > 0)
+        hipLaunchKernelGGL(k_factor_update, dim3((unsigned)((ntiles + 3) / 4)), dim3(TF_NT), 0, st, L, U, fd, list,
+                           pref, cnt, p0, mode, Dn);
 }
 
 void launch_inv_diag(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,

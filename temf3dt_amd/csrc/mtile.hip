@@ -71,24 +71,6 @@ __device__ __forceinline__ double mt_ld(const double *base, uint32_t off) {
     return *(const double *)((const char *)base + off);
 }
 
-// Raw buffer loads: resource (base, byte size) in scalar registers, a
-// per-lane 32-bit offset and a wave-uniform offset added by the load unit.
-// 0x00020000: the gfx9 buffer-resource word 3 (raw 32-bit data).
-typedef unsigned int mt_u2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t mt_rsrc(const double *base, uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)bytes, 0x00020000);
-}
-__device__ __forceinline__ double mt_bld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
-}
-typedef unsigned int mt_u4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ void mt_bld2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, double &x0,
-                                        double &x1) {
-    const mt_u4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0);
-    x0 = __builtin_bit_cast(double, (mt_u2){v.x, v.y});
-    x1 = __builtin_bit_cast(double, (mt_u2){v.z, v.w});
-}
-
 // One wave: acc(out, rhs) = sum_{k0 <= i < k1} A(out, i) B(i, rhs) where
 //   N form: A(out, i) = Lf[i*ld + roff + out]   (column i, row out)
 //   T form: A(out, i) = Lf[out*ld + roff + i]   (column out, row i)
