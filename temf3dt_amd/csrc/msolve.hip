@@ -187,25 +187,34 @@ __global__ void __launch_bounds__(TF_NT) k_mg_xs(const FrontDesc *fd, const int 
 // ---- forward pull -----------------------------------------------------------------
 // Own rows of the fronts of one level: Yin[row] = b[perm[row]] - sum of the Z
 // rows (descendant updates) that land on row, in the fixed order of gidx.
-// Four rows x 64 right-hand sides per workgroup.
+// One wave per row (lane = right-hand side); the row's Z indices are loaded
+// once across the lanes and broadcast from scalar registers, so eight Z rows
+// are in flight at a time (summed in order).
 __global__ void __launch_bounds__(256) k_mpull(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
                                                const double *b, const int *perm, const int *gptr, const int *gidx,
                                                const double *Z, double *Yin, int K) {
     const int i = seg_find(pref, cnt, blockIdx.x);
     const FrontDesc d = fd[list[i]];
-    const int r = (int)(blockIdx.x - pref[i]) * 4 + (threadIdx.x >> 6), q = threadIdx.x & 63;
-    if (r >= d.nc || q >= K) return;
-    const int row = d.c0 + r;
+    const int lane = threadIdx.x & 63;
+    const int r = (int)(blockIdx.x - pref[i]) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (r >= d.nc) return;
+    const int row = d.c0 + r, q = min(lane, K - 1);
     double s = b[(int64_t)perm[row] * K + q];
-    const int p1 = gptr[row + 1];
-    int p = gptr[row];
-    for (; p + 4 <= p1; p += 4) {  // four independent loads in flight, summed in order
-        const double z0 = Z[(int64_t)gidx[p] * K + q], z1 = Z[(int64_t)gidx[p + 1] * K + q];
-        const double z2 = Z[(int64_t)gidx[p + 2] * K + q], z3 = Z[(int64_t)gidx[p + 3] * K + q];
-        s = (((s - z0) - z1) - z2) - z3;
+    const int p0 = gptr[row], p1 = gptr[row + 1];
+    for (int pb = p0; pb < p1; pb += 64) {
+        const int n = min(64, p1 - pb);
+        const int mine = lane < n ? gidx[pb + lane] : 0;
+        int t = 0;
+        for (; t + 8 <= n; t += 8) {
+            double z[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) z[u] = Z[(int64_t)__builtin_amdgcn_readlane(mine, t + u) * K + q];
+#pragma unroll
+            for (int u = 0; u < 8; u++) s -= z[u];
+        }
+        for (; t < n; t++) s -= Z[(int64_t)__builtin_amdgcn_readlane(mine, t) * K + q];
     }
-    for (; p < p1; p++) s -= Z[(int64_t)gidx[p] * K + q];
-    Yin[(int64_t)row * K + q] = s;
+    if (lane < K) Yin[(int64_t)row * K + lane] = s;
 }
 
 // ---- host launchers --------------------------------------------------------------
