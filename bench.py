@@ -39,7 +39,7 @@ def build_model(name):
             "small": workloads.small_model, "multismall": workloads.multi_source_small_model}[name]()
 
 
-def setup_gpu(m, loops):
+def setup_gpu(m, loops, leaf=0):
     from temf3dt_amd import fetd
     fe = fetd.Fetd(0)
     fe.set_mesh(m.xyz, m.tet_vex)
@@ -53,7 +53,7 @@ def setup_gpu(m, loops):
     fe.set_schedule(m.sp_division, m.sp_double, m.sp_dblesize, m.time_maximum)
     fe.assemble()
     t = time.time()
-    fe.analyze()
+    fe.analyze(leaf)
     return fe, time.time() - t
 
 
@@ -222,6 +222,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the tile-kernel autotuner (use defaults)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
     ap.add_argument("--ckpt", type=int, default=50, help="adjoint checkpoint interval (steps)")
+    ap.add_argument("--leaf", type=int, default=0, help="nested-dissection leaf size (0: the library default)")
     ap.add_argument("--copy-probe", action="store_true", help="1 GiB device copies only (counter calibration)")
     a = ap.parse_args()
     if a.copy_probe:
@@ -236,7 +237,7 @@ def main():
         sys.exit(2)
     m = build_model(a.model)
     nsrc = len(m.loops)
-    fe, t_an = setup_gpu(m, list(range(nsrc)))
+    fe, t_an = setup_gpu(m, list(range(nsrc)), a.leaf)
     st = fe.factor_stats()
     n = fe.nE
     print(f"[bench] {a.model}: {m.ntet} tets, {n} edges, nnzU {fe.nnz}, nnzL {st['nnzL']:.3e}, "
