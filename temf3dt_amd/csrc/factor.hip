@@ -167,6 +167,10 @@ __device__ void blk_invert_to_lds(const double *L, const FrontDesc &d, int p0, d
     }
 }
 
+// Diagonal block of panel p0 replaced by its inverse's strict lower part; the
+// diagonal positions, whose pivots are kept in D, are zeroed, so that the
+// whole upper part of L11's storage reads as zero and the solves can run the
+// strict-triangle products of X11 = L11^-1 as plain dense products.
 __device__ void blk_invert_diag(double *L, const FrontDesc &d, int p0, double *lds) {
     const int pw = min(TF_PB, d.nc - p0);
     blk_invert_to_lds(L, d, p0, lds);
@@ -174,6 +178,7 @@ __device__ void blk_invert_diag(double *L, const FrontDesc &d, int p0, double *l
     for (int idx = threadIdx.x; idx < pw * pw; idx += blockDim.x) {
         int i = idx % pw, j = idx / pw;
         if (i > j) base[(int64_t)(p0 + j) * d.ld + p0 + i] = lds[j * LDS_LD + i];
+        else if (i == j) base[(int64_t)(p0 + j) * d.ld + p0 + i] = 0.0;
     }
     __syncthreads();
 }

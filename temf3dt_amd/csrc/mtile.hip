@@ -75,9 +75,10 @@ __device__ __forceinline__ double mt_ld(const double *base, uint32_t off) {
 //   N form: A(out, i) = Lf[i*ld + roff + out]   (column i, row out)
 //   T form: A(out, i) = Lf[out*ld + roff + i]   (column out, row i)
 // B(i, rhs) = Bm[row(i)*K + rhs] with row(i) = i, or bidx[i] when GATH.
-// TRI keeps only i < out (N) / i > out (T).  The inner range is walked in
-// stages of KS k-steps (4*KS inner values; a T-form stage starts on a
-// 16-row line of the column).
+// The strict triangle of X11 needs no mask (the diagonal and upper part of
+// L11's storage are zero).  The inner range is walked in stages of KS
+// k-steps (4*KS inner values; a T-form stage starts on a 16-row line of the
+// column).
 //
 // The factor block and the front's own right-hand-side rows are read with
 // raw buffer loads: per-lane offsets in vector registers, the stage offset
@@ -85,15 +86,14 @@ __device__ __forceinline__ double mt_ld(const double *base, uint32_t off) {
 // of the block returns zero.  Output rows/columns and right-hand-side
 // columns outside the tile's valid range are therefore read without masks
 // (they land in neighbouring rows or past the block) and their results are
-// never stored.  Stages inside [k0, k1) and off the
-// triangle form one contiguous, unmasked run, pipelined in pairs of stages:
-// the next stage's loads are issued (the last pair re-reads the final stage)
-// before the current one is multiplied, so each wait covers only the stage
-// being multiplied; gathered rows are fetched before the multiply that
-// precedes their stage's loads.  The
-// stages before and after that run clamp the inner index and zero what lies
-// outside.
-template <bool TFORM, bool TRI, bool GATH, int KS>
+// never stored.  Stages inside [k0, k1) form one contiguous, unmasked run,
+// pipelined in pairs of stages: the next stage's loads are issued (the last
+// pair re-reads the final stage) before the current one is multiplied, so
+// each wait covers only the stage being multiplied; gathered rows are
+// fetched before the multiply that precedes their stage's loads.  The
+// stages before and after that run (a T-form stage aligned before k0, a
+// partial last stage) clamp the inner index and zero what lies outside.
+template <bool TFORM, bool GATH, int KS>
 __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f, int roff, int o0, int k0, int k1,
                                         const double *Bm, uint32_t bbytes, const int *bidx, int K, tf_d4 acc[4][4]) {
     constexpr int SW = 4 * KS;  // inner values per stage
@@ -114,10 +114,8 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
     const int kb = TFORM ? (k0 - ((roff + k0) & 15)) : k0;
     const int nst = (k1 - kb + SW - 1) / SW;
     // the unmasked run of stages [ja, jb)
-    const int lo = (TRI && TFORM) ? max(k0, o0 + 64) : k0;
-    const int hi = (TRI && !TFORM) ? min(k1, o0) : k1;
-    const int ja = min(nst, (max(lo - kb, 0) + SW - 1) / SW);
-    const int jb = max(ja, (hi - kb) / SW);
+    const int ja = min(nst, (k0 - kb + SW - 1) / SW);
+    const int jb = max(ja, (k1 - kb) / SW);
     auto inner = [&](int ib, int s) { return TFORM ? (ib + KS * lk + s) : (ib + 4 * s + lk); };
     auto mult = [&](const double xa[KS][4], const double yb[KS][4]) {
 #pragma unroll
@@ -140,12 +138,10 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
 #pragma unroll
             for (int b = 0; b < 4; b++) {
                 const int ob = o0 + 16 * b + l16;
-                bool ok = iv;
-                if (TRI) ok = ok && (TFORM ? (i > ob) : (i < ob));
                 const uint32_t off = TFORM ? (uint32_t)((int64_t)ob * f + roff + ic) * 8u
                                            : (uint32_t)((int64_t)ic * f + roff + ob) * 8u;
                 const double x = mt_bld(rL, off, 0u);
-                xa[s][b] = ok ? x : 0.0;
+                xa[s][b] = iv ? x : 0.0;
             }
 #pragma unroll
             for (int a = 0; a < 4; a++) {
@@ -237,24 +233,20 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
     mt_geom<MODE>(d, nout, roff);
     tf_d4 acc[4][4];
     acc16_zero(acc);
-    const bool tri = (MODE == MT_OWN_FWD) ? (tu.k1 > tu.t0) : (MODE == MT_BWD_X) ? (tu.k0 < tu.t0 + 64) : false;
     const double *Lf = L + d.loff;
     const uint32_t lbytes = (uint32_t)((int64_t)d.nc * d.ld * 8);
     // B operand rows: own right-hand sides (Yin, Y_own, T) at c0, or the
     // solution rows of the struct indices (backward T)
     const double *Bm = in0 + (int64_t)d.c0 * K;
     const uint32_t bbytes = (uint32_t)((int64_t)d.nc * K * 8);  // the front's own right-hand-side rows
-    if (MODE == MT_OWN_FWD) {
-        if (tri) mt_wave<false, true, false, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
-        else mt_wave<false, false, false, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
-    } else if (MODE == MT_STR_FWD) {
-        mt_wave<false, false, false, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
-    } else if (MODE == MT_BWD_T) {
-        mt_wave<true, false, true, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, in0, gbytes, sidx + d.soff, K, acc);
-    } else {
-        if (tri) mt_wave<true, true, false, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
-        else mt_wave<true, false, false, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
-    }
+    // the strict triangle of X11 needs no mask: the factorisation leaves the
+    // diagonal and upper part of L11's storage zero
+    if (MODE == MT_OWN_FWD || MODE == MT_STR_FWD)
+        mt_wave<false, false, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
+    else if (MODE == MT_BWD_T)
+        mt_wave<true, true, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, in0, gbytes, sidx + d.soff, K, acc);
+    else
+        mt_wave<true, false, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
     const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
     if (tu.slot < 0) {
 #pragma unroll
