@@ -44,9 +44,9 @@ void launch_extend_add(double *L, double *U, const FrontDesc *fd, const int *par
 void launch_factor_fused(double *L, double *U, const FrontDesc *fd, const int *list, int cnt, double *Dn, int *flag,
                          hipStream_t st);
 void launch_factor_diag(double *L, const FrontDesc *fd, const int *list, int cnt, int p0, double *Dn, int *flag,
-                        hipStream_t st);
+                        double *W, const int64_t *woff, hipStream_t st);
 void launch_factor_trsm(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
-                        int p0, const double *Dn, hipStream_t st);
+                        int p0, const double *Dn, const double *W, const int64_t *woff, hipStream_t st);
 void launch_factor_update(double *L, double *U, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
                           int64_t nwg, int p0, int mode, const double *Dn, hipStream_t st);
 // factor.hip (inversion)
@@ -525,8 +525,9 @@ void build_plan(tf_ctx *c) {
         int64_t wlev = 0;
         for (int k : grid[l]) {
             kmax = std::max(kmax, (s.nc[k] + TF_PB - 1) / TF_PB);
-            // inversion scratch: the largest merge level's T blocks
-            int64_t need = 0;
+            // scratch: the pivot block's inverse during the panels (64 x 64),
+            // then the largest merge level's T blocks of the inversion
+            int64_t need = (int64_t)TF_PB * TF_PB;
             for (int b = TF_PB; b < s.nc[k]; b *= 2) {
                 int64_t npair = (s.nc[k] - b + 2 * b - 1) / (2 * b);
                 need = std::max(need, npair * inv_ldt(s.nc[k], b) * (int64_t)b);
@@ -770,10 +771,10 @@ void run_factor(tf_ctx *c, double *L, double *Dn, const FrontDesc *fd) {
             tf::launch_factor_fused(L, c->d_U, fd, lst + l.off, l.cnt, Dn, c->d_flag, c->st);
             break;
         case L_FDIAG:
-            tf::launch_factor_diag(L, fd, lst + l.off, l.cnt, l.p0, Dn, c->d_flag, c->st);
+            tf::launch_factor_diag(L, fd, lst + l.off, l.cnt, l.p0, Dn, c->d_flag, c->d_W, c->d_woff, c->st);
             break;
         case L_FTRSM:
-            tf::launch_factor_trsm(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, Dn, c->st);
+            tf::launch_factor_trsm(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, Dn, c->d_W, c->d_woff, c->st);
             break;
         case L_FUPD:
             tf::launch_factor_update(L, c->d_U, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, l.aux, Dn, c->st);

@@ -58,12 +58,19 @@ __device__ void blk_diag_factor(double *L, const FrontDesc &d, int p0, double *D
 // at a time with 4x4 register tiles.
 __device__ void blk_invert_to_lds(const double *L, const FrontDesc &d, int p0, double *lds);
 
-__device__ void blk_trsm(double *L, const FrontDesc &d, int p0, int r0, const double *Dn, double *lds) {
+__device__ void blk_trsm(double *L, const FrontDesc &d, int p0, int r0, const double *Dn, double *lds,
+                         const double *Xg = nullptr) {
     const int pw = min(TF_PB, d.nc - p0);
     double *base = L + d.loff;
     double *X = lds;                     // Xpp [c*65 + r]; unit diagonal set below
     double *At = lds + TF_PB * LDS_LD;   // [k][MLD rows]
-    blk_invert_to_lds(L, d, p0, X);
+    if (Xg) {  // the inverse the pivot kernel left (column-major, ld 64)
+        for (int idx = threadIdx.x; idx < TF_PB * TF_PB; idx += blockDim.x)
+            X[(idx >> 6) * LDS_LD + (idx & 63)] = Xg[idx];
+        __syncthreads();
+    } else {
+        blk_invert_to_lds(L, d, p0, X);
+    }
     for (int j = threadIdx.x; j < TF_PB; j += blockDim.x) X[j * LDS_LD + j] = (j < pw) ? 1.0 : 0.0;
     __syncthreads();
     const int kend = (pw + 3) & ~3;
@@ -387,6 +394,62 @@ __device__ __forceinline__ void upd_wave(double *L, double *U, const FrontDesc &
     }
 }
 
+// Pivot block of panel p0 of a large front by one wave, in registers: lane i
+// holds row i of the pw x pw block; the entries of column j that the
+// elimination and the inversion need from other rows are read with
+// readlane.  LDL^T is written back with D; then X = L11^-1 of the block,
+// column by column from the right (X[i][j] = -L[i][j] - sum_{j<k<i}
+// X[i][k] L[k][j]), is stored column-major (ld 64, zero on and above the
+// diagonal) at Xg for the panel solve.
+__device__ __forceinline__ double lane_bcast(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ void wave_diag_factor(double *L, const FrontDesc &d, int p0, double *Dn, int *flag, double *Xg) {
+    const int pw = min(TF_PB, d.nc - p0);
+    const int i = threadIdx.x & 63;  // lane i holds row i of the pivot block
+    double *base = L + d.loff + (int64_t)p0 * d.ld + p0;
+    double a[TF_PB];
+#pragma unroll
+    for (int j = 0; j < TF_PB; j++) a[j] = (j <= i && i < pw) ? base[(int64_t)j * d.ld + i] : 0.0;
+    // right-looking LDL^T: a_ik -= (a_ij / d_j) a_kj for j < k <= i, then
+    // a_ij /= d_j; a_kj (row k, column j) is broadcast from lane k
+    double di = 0.0;
+#pragma unroll
+    for (int j = 0; j < TF_PB; j++) {  // columns j >= pw: all zero, no effect
+        double dj = lane_bcast(a[j], j);
+        if (i == j) di = dj;
+        if (!(fabs(dj) > 1e-300) || !isfinite(dj)) dj = (dj < 0.0) ? -1e-300 : 1e-300;
+        const double lij = (i > j) ? a[j] / dj : 0.0;
+#pragma unroll
+        for (int k = j + 1; k < TF_PB; k++) a[k] -= lij * lane_bcast(a[j], k);
+        if (i > j) a[j] = lij;
+    }
+    if (i < pw) {
+#pragma unroll
+        for (int j = 0; j < TF_PB; j++)
+            if (j <= i) base[(int64_t)j * d.ld + i] = a[j];
+        Dn[d.doff + p0 + i] = di;
+        if (!(fabs(di) > 1e-300) || !isfinite(di)) atomicOr(flag, 1);
+    }
+    // X = L11^-1 (unit lower), strict part, by columns from the right:
+    // x_ij = -(l_ij + sum_{j<k<i} x_ik l_kj); x_ij overwrites l_ij once
+    // column j of L has been broadcast
+#pragma unroll
+    for (int j = TF_PB - 2; j >= 0; j--) {
+        double s = a[j];
+#pragma unroll
+        for (int k = j + 1; k < TF_PB - 1; k++) s += a[k] * lane_bcast(a[j], k);  // a[k] = 0 for k >= i
+        a[j] = (i > j && i < pw) ? -s : 0.0;
+    }
+    a[TF_PB - 1] = 0.0;
+#pragma unroll
+    for (int j = 0; j < TF_PB; j++) Xg[j * TF_PB + i] = a[j];
+}
+
 // ---- kernels ---------------------------------------------------------------
 
 __global__ void k_fill0(double *p, int64_t n) {
@@ -444,21 +507,25 @@ __global__ void __launch_bounds__(TF_NT) k_factor_fused(double *L, double *U, co
     blk_invert_diag(L, d, 0, lds);
 }
 
-__global__ void __launch_bounds__(TF_NT) k_factor_diag(double *L, const FrontDesc *fd, const int *list,
-                                                       int p0, double *Dn, int *flag) {
-    __shared__ double lds[TF_PB * LDS_LD];
-    const FrontDesc d = fd[list[blockIdx.x]];
-    blk_diag_factor(L, d, p0, Dn, flag, lds);
+// One wave per front: pivot block of panel p0 and its inverse (for the panel
+// solve) at W + woff[front].
+__global__ void __launch_bounds__(64) k_factor_diag(double *L, const FrontDesc *fd, const int *list, int p0,
+                                                    double *Dn, int *flag, double *W, const int64_t *woff) {
+    const int sn = list[blockIdx.x];
+    const FrontDesc d = fd[sn];
+    wave_diag_factor(L, d, p0, Dn, flag, W + woff[sn]);
 }
 
 __global__ void __launch_bounds__(TF_NT) k_factor_trsm(double *L, const FrontDesc *fd, const int *list,
-                                                       const int64_t *pref, int cnt, int p0, const double *Dn) {
+                                                       const int64_t *pref, int cnt, int p0, const double *Dn,
+                                                       const double *W, const int64_t *woff) {
     __shared__ double lds[TF_PB * LDS_LD + TF_PB * MLD];
     const int i = seg_find(pref, cnt, blockIdx.x);
-    const FrontDesc d = fd[list[i]];
+    const int sn = list[i];
+    const FrontDesc d = fd[sn];
     const int p1 = min(d.nc, p0 + TF_PB);
     const int r0 = p1 + (int)(blockIdx.x - pref[i]) * TF_TRSM_ROWS;
-    blk_trsm(L, d, p0, r0, Dn, lds);
+    blk_trsm(L, d, p0, r0, Dn, lds, W + woff[sn]);
 }
 
 // Updates of a super-panel [P0, P0 + TF_SP) of TF_PB-column panels, one
@@ -577,18 +644,18 @@ void launch_factor_fused(double *L, double *U, const FrontDesc *fd, const int *l
     if (cnt <= 0) return;
     hipLaunchKernelGGL(k_factor_fused, dim3(cnt), dim3(TF_NT), 0, st, L, U, fd, list, Dn, flag);
 }
-void launch_factor_diag(double *L, const FrontDesc *fd, const int *list, int cnt, int p0, double *Dn,
-                        int *flag, hipStream_t st) {
+void launch_factor_diag(double *L, const FrontDesc *fd, const int *list, int cnt, int p0, double *Dn, int *flag,
+                        double *W, const int64_t *woff, hipStream_t st) {
     if (cnt <= 0) return;
-    hipLaunchKernelGGL(k_factor_diag, dim3(cnt), dim3(TF_NT), 0, st, L, fd, list, p0, Dn, flag);
+    hipLaunchKernelGGL(k_factor_diag, dim3(cnt), dim3(64), 0, st, L, fd, list, p0, Dn, flag, W, woff);
 }
 void launch_factor_trsm(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
-                        int64_t nwg, int p0, const double *Dn, hipStream_t st) {
+                        int64_t nwg, int p0, const double *Dn, const double *W, const int64_t *woff,
+                        hipStream_t st) {
     if (nwg <= 0) return;
-    hipLaunchKernelGGL(k_factor_trsm, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt,
-                       p0, Dn);
+    hipLaunchKernelGGL(k_factor_trsm, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt, p0, Dn, W,
+                       woff);
 }
-// ntiles: 64 x 64 tiles in all (pref[cnt]); four per workgroup
 void launch_factor_update(double *L, double *U, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
                           int64_t ntiles, int p0, int mode, const double *Dn, hipStream_t st) {
     if (ntiles > 0)
