@@ -250,7 +250,10 @@ def main():
         fe.release(0)
         print(f"[bench] autotune: {tune['best']} ({tune['best_ms']:.2f} ms per solve, "
               f"{tune['seconds']:.1f} s)", file=sys.stderr, flush=True)
-    rows, wall, tm, prof = timed_run(fe, a.steps, a.warmup, profile=True)
+    rows, wall, tm, _ = timed_run(fe, a.steps, a.warmup)
+    # the same run again with HIP events around every launch group (which
+    # also turns off the step graphs) for the per-kernel breakdown
+    _, _, _, prof = timed_run(fe, a.steps, 0, profile=True)
     steps = int(rows["step"][-1])
     nf = int(tm["nfactor"])
     value = nsrc * steps / wall

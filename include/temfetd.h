@@ -106,7 +106,10 @@ int tf_analyze(tf_ctx *ctx, int leaf_size);
  *   "mt_units"  work units a tile launch aims for (default 4096, >= 64)
  *   "mt_chmin"  minimum inner chunk of a split tile (default 256, multiple of 16)
  *   "mt_ks_n"   k-steps per stage of the N-form kernels (2 or 4, default 2)
- *   "mt_ks_t"   k-steps per stage of the T-form kernels (2 or 4, default 4)
+ *   "mt_ks_t"   k-steps per stage of the T-form kernels (2 or 4, default 2)
+ *   "graphs"    1 (default): regular steps of tf_run (sources off) replay a
+ *               captured hipGraph of right-hand side, solve and history copy;
+ *               0: every step is launched kernel by kernel.  Bit-identical.
  * Results do not depend on the knobs beyond floating-point summation order. */
 int tf_set_tuning(tf_ctx *ctx, const char *key, int value);
 /* Kernel timing with HIP events on the context's stream (no reference

@@ -930,6 +930,14 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
 }
 
 // ---- time loop -------------------------------------------------------------------
+// Executable graphs of a run, keyed by the buffers they were captured on.
+struct GraphCache {
+    std::map<std::array<const void *, 6>, hipGraphExec_t> m;
+    ~GraphCache() {
+        for (auto &kv : m) (void)hipGraphExecDestroy(kv.second);
+    }
+};
+
 struct StepVecs {
     double *e1, *e2, *e3, *rh, *te1, *te2, *te3, *erec, *ev1, *ev2;
     std::vector<double *> ring;
@@ -1122,6 +1130,7 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
         c->tape[k].row = c->nrows - 1;
         c->row_node.push_back(k);
     };
+    GraphCache graphs;
     hipEvent_t f0, f1;
     HIPCHK(hipEventCreate(&f0));
     HIPCHK(hipEventCreate(&f1));
@@ -1214,8 +1223,30 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
             }
             continue;
         }
-        rhs(c, step, time_cur, v.e2, v.e3, step_cur, negate, v.rh);
-        step_solve(c, cur, v.rh, v.e1);
+        if (step >= c->sp_division && c->use_graphs && !c->prof_on) {
+            // regular step, sources off: the right-hand side, the solve and
+            // the ring copy replay a graph captured for this factor slot and
+            // arrangement of the history vectors (at most 2 x 3 x ring of them)
+            const std::array<const void *, 6> key = {c->d_L[cur], v.e1, v.e2, v.e3, v.rh, v.ring[idx % ring]};
+            auto it = graphs.m.find(key);
+            if (it == graphs.m.end()) {
+                hipGraph_t g;
+                HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+                rhs(c, step, time_cur, v.e2, v.e3, step_cur, negate, v.rh);
+                step_solve(c, cur, v.rh, v.e1);
+                dcopy(c, v.ring[idx % ring], v.e1);
+                HIPCHK(hipStreamEndCapture(c->st, &g));
+                hipGraphExec_t ex;
+                HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+                HIPCHK(hipGraphDestroy(g));
+                it = graphs.m.emplace(key, ex).first;
+            }
+            HIPCHK(hipGraphLaunch(it->second, c->st));
+        } else {
+            rhs(c, step, time_cur, v.e2, v.e3, step_cur, negate, v.rh);
+            step_solve(c, cur, v.rh, v.e1);
+            dcopy(c, v.ring[idx % ring], v.e1);
+        }
         if (outp) output_row(c, step, time_cur, step_cur, v.e1);
         if (tape) {
             const int k = node(id2, id3, step, step_cur, time_cur);
@@ -1224,7 +1255,6 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
             id2 = k;
             ring_id[idx % ring] = k;
         }
-        dcopy(c, v.ring[idx % ring], v.e1);
         idx++;
         // e3 <- e2 <- e1 by rotation; e1 is overwritten next step
         double *t3 = v.e3;
@@ -2023,6 +2053,9 @@ int tf_set_tuning(tf_ctx *c, const char *key, int value) {
             if (value < 16 || value % 16) return fail(c, TF_ERR_ARG, "mt_chmin must be a positive multiple of 16");
             c->mt_chmin = value;
             replan = true;
+        } else if (k == "graphs") {
+            if (value != 0 && value != 1) return fail(c, TF_ERR_ARG, "graphs must be 0 or 1");
+            c->use_graphs = value != 0;
         } else if (k == "mt_ks_n" || k == "mt_ks_t") {
             if (value != 2 && value != 4) return fail(c, TF_ERR_ARG, "k-steps per stage must be 2 or 4");
             (k == "mt_ks_n" ? c->mt_ks_n : c->mt_ks_t) = value;
