@@ -7,7 +7,8 @@
 //
 //  * geometric nested dissection on the edge graph, using edge midpoints as
 //    coordinates: each set is split by a plane (axis and quantile chosen for
-//    the smallest one-sided vertex separator), the separator is ordered last;
+//    the smallest separator); the separator is the minimum vertex cover of
+//    the edges that cross the plane, and is ordered last;
 //  * every dissection node with columns becomes one supernode whose diagonal
 //    block is dense (separators fill in completely; leaves are small);
 //  * row structures, relative indices into the parent front, L/U storage
@@ -15,6 +16,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <limits>
 #include <numeric>
 #include <stdexcept>
 
@@ -39,15 +41,92 @@ struct NDState {
     const std::vector<int> *adj;
     const std::vector<double> *co;  // [n][3]
     std::vector<int64_t> mark;     // set id * 2 + side
+    std::vector<int> loc;          // scratch: local index of a boundary vertex (sets are disjoint)
     std::vector<NDNode> nodes;
     std::atomic<int> nnodes{0};
     std::atomic<int64_t> setid{1};
     int leaf;
 };
 
+// Minimum vertex cover of a bipartite graph (left vertices 0..nl-1 with
+// adjacency lptr/ladj into right vertices 0..nr-1): maximum matching by
+// Hopcroft-Karp, then Koenig's construction -- the left vertices not
+// reached and the right vertices reached by alternating paths from the
+// free left vertices.
+static void bip_min_cover(int nl, int nr, const std::vector<int> &lptr, const std::vector<int> &ladj,
+                          std::vector<char> &coverL, std::vector<char> &coverR) {
+    const int INF = std::numeric_limits<int>::max();
+    std::vector<int> mL(nl, -1), mR(nr, -1), dist(nl), it(nl), q(nl), stk;
+    for (int u = 0; u < nl; u++)  // greedy start
+        for (int e = lptr[u]; e < lptr[u + 1]; e++)
+            if (mR[ladj[e]] < 0) { mL[u] = ladj[e]; mR[ladj[e]] = u; break; }
+    while (true) {
+        int qh = 0, qt = 0;
+        bool found = false;
+        for (int u = 0; u < nl; u++) {
+            if (mL[u] < 0) { dist[u] = 0; q[qt++] = u; }
+            else dist[u] = INF;
+        }
+        while (qh < qt) {
+            const int u = q[qh++];
+            for (int e = lptr[u]; e < lptr[u + 1]; e++) {
+                const int w = mR[ladj[e]];
+                if (w < 0) found = true;
+                else if (dist[w] == INF) { dist[w] = dist[u] + 1; q[qt++] = w; }
+            }
+        }
+        if (!found) break;
+        // augmenting paths along the BFS layers, depth first without recursion
+        for (int u = 0; u < nl; u++) it[u] = lptr[u];
+        for (int s0 = 0; s0 < nl; s0++) {
+            if (mL[s0] >= 0) continue;
+            stk.assign(1, s0);
+            while (!stk.empty()) {
+                const int u = stk.back();
+                bool moved = false;
+                while (it[u] < lptr[u + 1]) {
+                    const int w = mR[ladj[it[u]]];
+                    if (w < 0) {  // free right vertex: flip the path held on the stack
+                        for (int x : stk) { const int r = ladj[it[x]]; mL[x] = r; mR[r] = x; }
+                        stk.clear();
+                        moved = true;
+                        break;
+                    }
+                    if (dist[w] == dist[u] + 1) { stk.push_back(w); moved = true; break; }
+                    it[u]++;
+                }
+                if (!moved) {  // dead end
+                    dist[u] = INF;
+                    stk.pop_back();
+                    if (!stk.empty()) it[stk.back()]++;
+                }
+            }
+        }
+    }
+    // alternating reachability from the free left vertices
+    std::vector<char> visL(nl, 0), visR(nr, 0);
+    int qh = 0, qt = 0;
+    for (int u = 0; u < nl; u++)
+        if (mL[u] < 0) { visL[u] = 1; q[qt++] = u; }
+    while (qh < qt) {
+        const int u = q[qh++];
+        for (int e = lptr[u]; e < lptr[u + 1]; e++) {
+            const int r = ladj[e];
+            if (visR[r] || mL[u] == r) continue;
+            visR[r] = 1;
+            const int w = mR[r];
+            if (w >= 0 && !visL[w]) { visL[w] = 1; q[qt++] = w; }
+        }
+    }
+    coverL.assign(nl, 0);
+    coverR.assign(nr, 0);
+    for (int u = 0; u < nl; u++) coverL[u] = !visL[u];
+    for (int r = 0; r < nr; r++) coverR[r] = visR[r];
+}
+
 // Separator of `v` split at position `cut` of the sorted order `ord`
-// (first `cut` nodes form side 0).  Returns the separator and which side it
-// was taken from.
+// (first `cut` nodes form side 0): the smallest set of boundary vertices
+// that covers every edge between the two sides.
 static void eval_split(NDState &st, const std::vector<int> &ord, size_t cut, int64_t sid,
                        std::vector<int> &sep, int &sepside) {
     auto &mark = st.mark;
@@ -66,8 +145,23 @@ static void eval_split(NDState &st, const std::vector<int> &ord, size_t cut, int
             }
         }
     }
-    if (b0.size() <= b1.size()) { sep.swap(b0); sepside = 0; }
-    else { sep.swap(b1); sepside = 1; }
+    sepside = b0.size() <= b1.size() ? 0 : 1;
+    const int nl = (int)b0.size(), nr = (int)b1.size();
+    for (int r = 0; r < nr; r++) st.loc[b1[r]] = r;
+    std::vector<int> lptr(nl + 1, 0), ladj;
+    for (int u = 0; u < nl; u++) {
+        const int v = b0[u];
+        for (int q = xadj[v]; q < xadj[v + 1]; q++)
+            if (mark[adj[q]] == sid * 2 + 1) ladj.push_back(st.loc[adj[q]]);
+        lptr[u + 1] = (int)ladj.size();
+    }
+    std::vector<char> cL, cR;
+    bip_min_cover(nl, nr, lptr, ladj, cL, cR);
+    sep.clear();
+    for (int u = 0; u < nl; u++)
+        if (cL[u]) sep.push_back(b0[u]);
+    for (int r = 0; r < nr; r++)
+        if (cR[r]) sep.push_back(b1[r]);
 }
 
 static int nd_rec(NDState &st, std::vector<int> v) {
@@ -78,13 +172,15 @@ static int nd_rec(NDState &st, std::vector<int> v) {
         return id;
     }
     const auto &co = *st.co;
-    // bounding box spread by count: try each axis at the median
+    // planes normal to each axis at count quantiles 0.4..0.6, then the best
+    // axis at the quantiles in between; the smallest separator wins, with a
+    // mild penalty for imbalance
     std::vector<int> best_sep;
     std::vector<int> best_ord;
     size_t best_cut = 0;
     int best_side = 0;
     double best_score = 1e300;
-    const double qs[5] = {0.5, 0.45, 0.55, 0.4, 0.6};
+    static const std::vector<double> qs0 = {0.4, 0.45, 0.5, 0.55, 0.6}, qs1 = {0.425, 0.475, 0.525, 0.575};
     int best_axis = -1;
     for (int pass = 0; pass < 2; pass++) {
         for (int axis = 0; axis < 3; axis++) {
@@ -94,17 +190,16 @@ static int nd_rec(NDState &st, std::vector<int> v) {
                 double xa = co[3 * (size_t)a + axis], xb = co[3 * (size_t)b + axis];
                 return xa < xb || (xa == xb && a < b);
             });
-            int q0 = pass == 0 ? 0 : 1, q1 = pass == 0 ? 1 : 5;
-            for (int qi = q0; qi < q1; qi++) {
+            const std::vector<double> &qs = pass == 0 ? qs0 : qs1;
+            for (size_t qi = 0; qi < qs.size(); qi++) {
                 size_t cut = (size_t)(qs[qi] * ord.size());
                 if (cut == 0 || cut >= ord.size()) continue;
                 int64_t sid = st.setid.fetch_add(1);
                 std::vector<int> sep;
                 int side;
                 eval_split(st, ord, cut, sid, sep, side);
-                // smaller separator wins; mild penalty for imbalance
                 double bal = std::fabs(qs[qi] - 0.5);
-                double score = (double)sep.size() * (1.0 + 2.0 * bal);
+                double score = (double)sep.size() * (1.0 + 0.5 * bal);
                 if (score < best_score) {
                     best_score = score;
                     best_sep.swap(sep);
@@ -193,6 +288,7 @@ void analyze(Symbolic &s, const Topology &t, int leaf_size) {
     st.adj = &adj;
     st.co = &co;
     st.mark.assign(n, 0);
+    st.loc.assign(n, 0);
     st.leaf = std::max(leaf_size, 4);
     st.nodes.resize((size_t)(2 * (n / std::max(1, st.leaf / 2)) + 64));
     // connected components (breadth-first, in order of the first vertex)
