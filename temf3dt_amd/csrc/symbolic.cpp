@@ -18,6 +18,8 @@
 #include <cmath>
 #include <limits>
 #include <numeric>
+#include <tuple>
+#include <queue>
 #include <stdexcept>
 
 #include "tf_internal.h"
@@ -42,6 +44,9 @@ struct NDState {
     const std::vector<double> *co;  // [n][3]
     std::vector<int64_t> mark;     // set id * 2 + side
     std::vector<int> loc;          // scratch: local index of a boundary vertex (sets are disjoint)
+    std::vector<int8_t> part;      // side 0 / 1, or 2 for the separator, of the set being split
+    std::vector<int> ver;          // version of a vertex's queued gains
+    std::vector<int64_t> stamp;    // refinement pass that moved the vertex
     std::vector<NDNode> nodes;
     std::atomic<int> nnodes{0};
     std::atomic<int64_t> setid{1};
@@ -164,6 +169,103 @@ static void eval_split(NDState &st, const std::vector<int> &ord, size_t cut, int
         if (cR[r]) sep.push_back(b1[r]);
 }
 
+// Fiduccia-Mattheyses refinement of the vertex separator of the set `vs`
+// (members have mark == code; part[] gives side 0 / 1 or 2 = separator).
+// Moving a separator vertex into side X pulls its neighbours on the other
+// side into the separator: gain = 1 - their number.  Each pass makes
+// best-first moves, every vertex at most once, accepting losses for up to
+// 64 moves past the best state, and rolls back to the smallest separator
+// seen that keeps both sides within maxfrac of the set.
+static void fm_refine(NDState &st, const std::vector<int> &vs, int64_t code, double maxfrac) {
+    auto &part = st.part;
+    auto &ver = st.ver;
+    auto &stamp = st.stamp;
+    const auto &xadj = *st.xadj;
+    const auto &adj = *st.adj;
+    const auto &mark = st.mark;
+    int64_t cnt[3] = {0, 0, 0};
+    for (int v : vs) cnt[part[v]]++;
+    const int64_t maxside = (int64_t)(maxfrac * (double)vs.size());
+    auto gain = [&](int v, int X) {
+        int g = 1;
+        for (int q = xadj[v]; q < xadj[v + 1]; q++) {
+            const int u = adj[q];
+            if (mark[u] == code && part[u] == 1 - X) g--;
+        }
+        return g;
+    };
+    using Entry = std::tuple<int, int, int, int>;  // gain, vertex, side, version
+    for (int pass = 0; pass < 8; pass++) {
+        const int64_t pst = st.setid.fetch_add(1);
+        std::priority_queue<Entry> pq;
+        auto queue = [&](int w) {
+            if (part[w] != 2 || stamp[w] == pst) return;
+            ver[w]++;
+            pq.emplace(gain(w, 0), w, 0, ver[w]);
+            pq.emplace(gain(w, 1), w, 1, ver[w]);
+        };
+        for (int v : vs)
+            if (part[v] == 2) queue(v);
+        std::vector<std::pair<int, int>> moves;
+        std::vector<int> pulled;
+        std::vector<size_t> pptr(1, 0);
+        int64_t best = cnt[2], best_max = std::max(cnt[0], cnt[1]);
+        size_t best_len = 0;
+        int since = 0;
+        while (!pq.empty()) {
+            const auto [g, v, X, vv] = pq.top();
+            pq.pop();
+            if (vv != ver[v] || part[v] != 2 || stamp[v] == pst) continue;
+            if (cnt[X] + 1 > maxside) continue;
+            stamp[v] = pst;
+            part[v] = (int8_t)X;
+            cnt[X]++;
+            cnt[2]--;
+            moves.emplace_back(v, X);
+            const size_t p0 = pulled.size();
+            for (int q = xadj[v]; q < xadj[v + 1]; q++) {
+                const int u = adj[q];
+                if (mark[u] == code && part[u] == 1 - X) {
+                    part[u] = 2;
+                    cnt[1 - X]--;
+                    cnt[2]++;
+                    pulled.push_back(u);
+                }
+            }
+            pptr.push_back(pulled.size());
+            for (int q = xadj[v]; q < xadj[v + 1]; q++)
+                if (mark[adj[q]] == code) queue(adj[q]);
+            for (size_t k = p0; k < pulled.size(); k++) {
+                const int u = pulled[k];
+                queue(u);
+                for (int q = xadj[u]; q < xadj[u + 1]; q++)
+                    if (mark[adj[q]] == code) queue(adj[q]);
+            }
+            const int64_t mx = std::max(cnt[0], cnt[1]);
+            if (cnt[2] < best || (cnt[2] == best && mx < best_max)) {
+                best = cnt[2];
+                best_max = mx;
+                best_len = moves.size();
+                since = 0;
+            } else if (++since > 64) {
+                break;
+            }
+        }
+        for (size_t i = moves.size(); i-- > best_len;) {  // roll back past the best state
+            const auto [v, X] = moves[i];
+            for (size_t k = pptr[i]; k < pptr[i + 1]; k++) {
+                part[pulled[k]] = (int8_t)(1 - X);
+                cnt[1 - X]++;
+                cnt[2]--;
+            }
+            part[v] = 2;
+            cnt[X]--;
+            cnt[2]++;
+        }
+        if (best_len == 0) break;
+    }
+}
+
 static int nd_rec(NDState &st, std::vector<int> v) {
     int id = st.nnodes.fetch_add(1);
     NDNode &nd = st.nodes[id];
@@ -211,15 +313,18 @@ static int nd_rec(NDState &st, std::vector<int> v) {
             }
         }
     }
-    // mark separator, form the two remaining sides
-    int64_t sid = st.setid.fetch_add(1);
-    for (int s : best_sep) st.mark[s] = sid * 2;
-    std::vector<int> a, b;
+    // refine the separator, then form the two remaining sides
+    const int64_t code = st.setid.fetch_add(1) * 2;
     for (size_t i = 0; i < best_ord.size(); i++) {
-        int u = best_ord[i];
-        if (st.mark[u] == sid * 2) continue;
-        (i < best_cut ? a : b).push_back(u);
+        const int u = best_ord[i];
+        st.mark[u] = code;
+        st.part[u] = i < best_cut ? 0 : 1;
     }
+    for (int u : best_sep) st.part[u] = 2;
+    fm_refine(st, best_ord, code, 0.6);
+    std::vector<int> a, b;
+    best_sep.clear();
+    for (int u : best_ord) (st.part[u] == 2 ? best_sep : st.part[u] == 0 ? a : b).push_back(u);
     (void)best_side;
     nd.cols = best_sep;
     std::vector<int>().swap(best_ord);
@@ -289,6 +394,9 @@ void analyze(Symbolic &s, const Topology &t, int leaf_size) {
     st.co = &co;
     st.mark.assign(n, 0);
     st.loc.assign(n, 0);
+    st.part.assign(n, 0);
+    st.ver.assign(n, 0);
+    st.stamp.assign(n, 0);
     st.leaf = std::max(leaf_size, 4);
     st.nodes.resize((size_t)(2 * (n / std::max(1, st.leaf / 2)) + 64));
     // connected components (breadth-first, in order of the first vertex)
