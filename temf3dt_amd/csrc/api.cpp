@@ -70,7 +70,6 @@ void launch_bwd_x(const double *L, const FrontDesc *fd, const int *list, const i
                   const double *T, double *xw, hipStream_t st);
 void launch_unpermute(int n, const int *perm, const double *xw, double *x, hipStream_t st);
 // msolve.hip
-void launch_munpermute(int n, int K, const int *perm, const double *xw, double *x, hipStream_t st);
 void launch_fill_pattern(int64_t len, double *b, hipStream_t st);
 // step.hip: independent models
 void launch_source_models(int ns, const int *edge, const int *model, const double *w, const double *coef, double *r,
@@ -88,12 +87,13 @@ void launch_scale(int n, const double *s, double *g, hipStream_t st);
 // mtile.hip
 void launch_mtile(int mode, int ks, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits, int K,
                   const double *in0, const double *in1, const double *Dn, const int *sidx, double *out, double *part,
-                  uint32_t gbytes, hipStream_t st);
+                  uint32_t gbytes, const int *perm, double *x1, double *x2, hipStream_t st);
 void launch_mpull(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg, const double *b,
                   const int *perm, const int *gptr, const int *gidx, const double *Z, double *Yin, int K,
                   hipStream_t st);
 void launch_mtile_red(int mode, const FrontDesc *fd, const TileUnit *units, int nunits, int K, const double *in0,
-                      const double *in1, const double *Dn, double *out, const double *part, hipStream_t st);
+                      const double *in1, const double *Dn, double *out, const double *part, const int *perm,
+                      double *x1, double *x2, hipStream_t st);
 void launch_mspmv_rhs(int n, int K, const int *ia, const int *ja, const double *val, const double *e2,
                       const double *e3, double *w, double *r, hipStream_t st);
 void launch_msource(int ns, int K, const int *edge, const int *src, const double *w, double didt, double cdt,
@@ -882,9 +882,10 @@ void ensure_block(tf_ctx *c, int K) {
     c->mK = K;
 }
 
-// X = P^-1 B for K <= 64 right-hand sides, [n][K] device arrays in edge order
+// X = P^-1 B for K <= 64 right-hand sides, [n][K] device arrays in edge order;
+// x2, if given, receives a second copy of X
 void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd, int K, const double *b,
-                 double *x) {
+                 double *x, double *x2 = nullptr) {
     ensure_block(c, K);
     // the tile kernels address the gathered solution vector with 32-bit byte offsets
     const uint64_t gb = (uint64_t)c->sym.n * K * 8;
@@ -901,32 +902,36 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
                              c->d_mZ, c->d_mV, K, c->st);
             break;
         case L_MT:  // own forward (Y from Yin), struct forward (Z from Y)
-            tf::launch_mtile(l.p0, l.p0 >= 2 ? c->mt_ks_t : c->mt_ks_n, L, fd, c->d_units + l.off, l.cnt, K, l.p0 == 0 ? c->d_mV : c->d_mY, nullptr,
-                             Dn, nullptr, l.p0 == 0 ? c->d_mY : c->d_mZ, c->d_part, 0u, c->st);
+            tf::launch_mtile(l.p0, c->mt_ks_n, L, fd, c->d_units + l.off, l.cnt, K, l.p0 == 0 ? c->d_mV : c->d_mY,
+                             nullptr, Dn, nullptr, l.p0 == 0 ? c->d_mY : c->d_mZ, c->d_part, 0u, nullptr, nullptr,
+                             nullptr, c->st);
             break;
         case L_MTRED:
             tf::launch_mtile_red(l.p0, fd, c->d_units + l.off, l.cnt, K, l.p0 == 0 ? c->d_mV : c->d_mY, nullptr,
-                                 Dn, l.p0 == 0 ? c->d_mY : c->d_mZ, c->d_part, c->st);
+                                 Dn, l.p0 == 0 ? c->d_mY : c->d_mZ, c->d_part, nullptr, nullptr, nullptr, c->st);
             break;
         }
         prof_close(c, prof_category(l.kind), pm);
     }
+    // backward T (from x at the struct rows, Y, D), backward X (from T); the
+    // backward X epilogues write the finished rows in edge order as well
     for (const Launch &l : p.mbwd) {
         const int pm = prof_mark(c);
+        const bool bx = l.p0 == 3;
         switch (l.kind) {
-        case L_MT:  // backward T (from x at the struct rows, Y, D), backward X (from T)
-            tf::launch_mtile(l.p0, l.p0 >= 2 ? c->mt_ks_t : c->mt_ks_n, L, fd, c->d_units + l.off, l.cnt, K, l.p0 == 2 ? c->d_mxw : c->d_mT,
-                             l.p0 == 2 ? c->d_mY : c->d_mT, Dn, c->d_sidx, l.p0 == 2 ? c->d_mT : c->d_mxw, c->d_part,
-                             gbytes, c->st);
+        case L_MT:
+            tf::launch_mtile(l.p0, c->mt_ks_t, L, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : c->d_mxw,
+                             bx ? c->d_mT : c->d_mY, Dn, c->d_sidx, bx ? c->d_mxw : c->d_mT, c->d_part, gbytes,
+                             c->d_perm, bx ? x : nullptr, bx ? x2 : nullptr, c->st);
             break;
         case L_MTRED:
-            tf::launch_mtile_red(l.p0, fd, c->d_units + l.off, l.cnt, K, l.p0 == 2 ? c->d_mxw : c->d_mT,
-                                 l.p0 == 2 ? c->d_mY : c->d_mT, Dn, l.p0 == 2 ? c->d_mT : c->d_mxw, c->d_part, c->st);
+            tf::launch_mtile_red(l.p0, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : c->d_mxw,
+                                 bx ? c->d_mT : c->d_mY, Dn, bx ? c->d_mxw : c->d_mT, c->d_part, c->d_perm,
+                                 bx ? x : nullptr, bx ? x2 : nullptr, c->st);
             break;
         }
         prof_close(c, prof_category(l.kind), pm);
     }
-    tf::launch_munpermute(c->sym.n, K, c->d_perm, c->d_mxw, x, c->st);
 }
 
 // ---- time loop -------------------------------------------------------------------
@@ -979,9 +984,14 @@ void dcopy(tf_ctx *c, double *dst, const double *src) {
     HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * c->topo.nE * c->nsrc, hipMemcpyDeviceToDevice, c->st));
 }
 
-void step_solve(tf_ctx *c, int slot, const double *b, double *x) {
-    if (c->nsrc == 1) solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, b, x);
-    else solve_block(c, c->d_L[slot], c->d_D[slot], c->d_fd, c->nsrc, b, x);
+// x2, if given, receives a copy of the solution
+void step_solve(tf_ctx *c, int slot, const double *b, double *x, double *x2 = nullptr) {
+    if (c->nsrc == 1) {
+        solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, b, x);
+        if (x2) dcopy(c, x2, x);
+    } else {
+        solve_block(c, c->d_L[slot], c->d_D[slot], c->d_fd, c->nsrc, b, x, x2);
+    }
 }
 
 void grow_rows(tf_ctx *c) {
@@ -1224,8 +1234,8 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
             continue;
         }
         if (step >= c->sp_division && c->use_graphs && !c->prof_on) {
-            // regular step, sources off: the right-hand side, the solve and
-            // the ring copy replay a graph captured for this factor slot and
+            // regular step, sources off: the right-hand side and the solve
+            // (with its ring copy) replay a graph captured for this factor slot and
             // arrangement of the history vectors (at most 2 x 3 x ring of them)
             const std::array<const void *, 6> key = {c->d_L[cur], v.e1, v.e2, v.e3, v.rh, v.ring[idx % ring]};
             auto it = graphs.m.find(key);
@@ -1233,8 +1243,7 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
                 hipGraph_t g;
                 HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
                 rhs(c, step, time_cur, v.e2, v.e3, step_cur, negate, v.rh);
-                step_solve(c, cur, v.rh, v.e1);
-                dcopy(c, v.ring[idx % ring], v.e1);
+                step_solve(c, cur, v.rh, v.e1, v.ring[idx % ring]);
                 HIPCHK(hipStreamEndCapture(c->st, &g));
                 hipGraphExec_t ex;
                 HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
@@ -1244,8 +1253,7 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
             HIPCHK(hipGraphLaunch(it->second, c->st));
         } else {
             rhs(c, step, time_cur, v.e2, v.e3, step_cur, negate, v.rh);
-            step_solve(c, cur, v.rh, v.e1);
-            dcopy(c, v.ring[idx % ring], v.e1);
+            step_solve(c, cur, v.rh, v.e1, v.ring[idx % ring]);
         }
         if (outp) output_row(c, step, time_cur, step_cur, v.e1);
         if (tape) {

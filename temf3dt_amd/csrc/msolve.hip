@@ -9,14 +9,6 @@
 // is read once per step for all K sources.
 #include "tf_device.h"
 
-__global__ void k_munpermute(int n, int K, const int *perm, const double *xw, double *x) {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < (int64_t)n * K) {
-        const int i = (int)(idx / K), q = (int)(idx % K);
-        x[(int64_t)perm[i] * K + q] = xw[idx];
-    }
-}
-
 // ---- K-wide step kernels ------------------------------------------------------------
 
 // w = 4 e2 - e3 (the BDF2 history combination), once per entry
@@ -234,10 +226,6 @@ void launch_mpull(const FrontDesc *fd, const int *list, const int64_t *pref, int
     if (nwg > 0)
         hipLaunchKernelGGL(k_mpull, dim3((unsigned)nwg), dim3(256), 0, st, fd, list, pref, cnt, b, perm, gptr, gidx, Z, Yin,
                            K);
-}
-void launch_munpermute(int n, int K, const int *perm, const double *xw, double *x, hipStream_t st) {
-    const int64_t tot = (int64_t)n * K;
-    hipLaunchKernelGGL(k_munpermute, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, n, K, perm, xw, x);
 }
 void launch_mspmv_rhs(int n, int K, const int *ia, const int *ja, const double *val, const double *e2,
                       const double *e3, double *w, double *r, hipStream_t st) {

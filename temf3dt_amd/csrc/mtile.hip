@@ -47,9 +47,16 @@ __device__ __forceinline__ void mt_geom(const FrontDesc &d, int &nout, int &roff
 
 // Epilogue for one output element (o = output row/column of the front,
 // q = rhs); `s` is the accumulated product.
+// Backward X also scatters the finished rows to edge order: x1 (and x2, if
+// given) at row perm[c0 + o].
+struct MtEdgeOut {
+    const int *perm;
+    double *x1, *x2;
+};
+
 template <int MODE>
 __device__ __forceinline__ void mt_store(const FrontDesc &d, int o, int q, double s, int K, const double *in0,
-                                         const double *in1, const double *Dn, double *out) {
+                                         const double *in1, const double *Dn, double *out, const MtEdgeOut &xo) {
     if (MODE == MT_OWN_FWD) {  // Y[c0+o] = Yin[c0+o] + s
         const int64_t oo = (int64_t)(d.c0 + o) * K + q;
         out[oo] = in0[oo] + s;
@@ -60,7 +67,13 @@ __device__ __forceinline__ void mt_store(const FrontDesc &d, int o, int q, doubl
         out[oo] = in1[oo] / Dn[d.doff + o] - s;
     } else {  // X = T + s
         const int64_t oo = (int64_t)(d.c0 + o) * K + q;
-        out[oo] = in1[oo] + s;
+        const double v = in1[oo] + s;
+        out[oo] = v;
+        if (xo.x1) {
+            const int64_t e = (int64_t)xo.perm[d.c0 + o] * K + q;
+            xo.x1[e] = v;
+            if (xo.x2) xo.x2[e] = v;
+        }
     }
 }
 
@@ -223,7 +236,7 @@ template <int MODE, int KS>
 __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDesc *fd, const TileUnit *units,
                                                   int nunits, int K, const double *in0, const double *in1,
                                                   const double *Dn, const int *sidx, double *out, double *part,
-                                                  uint32_t gbytes) {
+                                                  uint32_t gbytes, MtEdgeOut xo) {
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int u = xcd_block(blockIdx.x, gridDim.x) * 4 + w;
     if (u >= nunits) return;
@@ -258,7 +271,7 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
 #pragma unroll
                 for (int a = 0; a < 4; a++) {
                     const int q = 16 * a + l16;
-                    if (q < K) mt_store<MODE>(d, o, q, acc[a][b][r], K, in0, in1, Dn, out);
+                    if (q < K) mt_store<MODE>(d, o, q, acc[a][b][r], K, in0, in1, Dn, out, xo);
                 }
             }
     } else {
@@ -278,7 +291,7 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
 template <int MODE>
 __global__ void __launch_bounds__(256) k_mtile_red(const FrontDesc *fd, const TileUnit *units, int K,
                                                    const double *in0, const double *in1, const double *Dn,
-                                                   double *out, const double *part) {
+                                                   double *out, const double *part, MtEdgeOut xo) {
     const TileUnit tu = units[blockIdx.x >> 4];
     const FrontDesc d = fd[tu.front];
     int nout, roff;
@@ -289,19 +302,21 @@ __global__ void __launch_bounds__(256) k_mtile_red(const FrontDesc *fd, const Ti
     const double *p0 = part + (int64_t)tu.k0 * 4096 + e;
     double s = 0.0;
     for (int c = 0; c < tu.k1; c++) s += p0[(int64_t)c * 4096];
-    mt_store<MODE>(d, o, q, s, K, in0, in1, Dn, out);
+    mt_store<MODE>(d, o, q, s, K, in0, in1, Dn, out, xo);
 }
 
 namespace tf {
 // mode: 0 own forward, 1 struct forward, 2 backward T, 3 backward X
 // gbytes: size of the gathered vector (backward T form), < 4 GiB
+// perm, x1, x2: backward X also writes its rows to x1 (and x2) in edge order
 void launch_mtile(int mode, int ks, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits, int K,
                   const double *in0, const double *in1, const double *Dn, const int *sidx, double *out, double *part,
-                  uint32_t gbytes, hipStream_t st) {
+                  uint32_t gbytes, const int *perm, double *x1, double *x2, hipStream_t st) {
     if (nunits <= 0) return;
     const dim3 g((nunits + 3) / 4), b(256);
+    const MtEdgeOut xo{perm, x1, x2};
 #define TF_MT(M, S) \
-    hipLaunchKernelGGL((k_mtile<M, S>), g, b, 0, st, L, fd, units, nunits, K, in0, in1, Dn, sidx, out, part, gbytes)
+    hipLaunchKernelGGL((k_mtile<M, S>), g, b, 0, st, L, fd, units, nunits, K, in0, in1, Dn, sidx, out, part, gbytes, xo)
     if (ks == 4) {
         switch (mode) {
         case MT_OWN_FWD: TF_MT(MT_OWN_FWD, 4); break;
@@ -320,14 +335,16 @@ void launch_mtile(int mode, int ks, const double *L, const FrontDesc *fd, const 
 #undef TF_MT
 }
 void launch_mtile_red(int mode, const FrontDesc *fd, const TileUnit *units, int nunits, int K, const double *in0,
-                      const double *in1, const double *Dn, double *out, const double *part, hipStream_t st) {
+                      const double *in1, const double *Dn, double *out, const double *part, const int *perm,
+                      double *x1, double *x2, hipStream_t st) {
     if (nunits <= 0) return;
     const dim3 g(nunits * 16), b(256);
+    const MtEdgeOut xo{perm, x1, x2};
     switch (mode) {
-    case MT_OWN_FWD: hipLaunchKernelGGL(k_mtile_red<MT_OWN_FWD>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part); break;
-    case MT_STR_FWD: hipLaunchKernelGGL(k_mtile_red<MT_STR_FWD>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part); break;
-    case MT_BWD_T: hipLaunchKernelGGL(k_mtile_red<MT_BWD_T>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part); break;
-    default: hipLaunchKernelGGL(k_mtile_red<MT_BWD_X>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part); break;
+    case MT_OWN_FWD: hipLaunchKernelGGL(k_mtile_red<MT_OWN_FWD>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo); break;
+    case MT_STR_FWD: hipLaunchKernelGGL(k_mtile_red<MT_STR_FWD>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo); break;
+    case MT_BWD_T: hipLaunchKernelGGL(k_mtile_red<MT_BWD_T>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo); break;
+    default: hipLaunchKernelGGL(k_mtile_red<MT_BWD_X>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo); break;
     }
 }
 }  // namespace tf
