@@ -682,7 +682,7 @@ void build_plan(tf_ctx *c) {
     for (int l = 0; l < s.nlevels; l++) {
         std::vector<int> all(s.lev_sn.begin() + s.lev_ptr[l], s.lev_sn.begin() + s.lev_ptr[l + 1]);
         std::vector<int64_t> ur;
-        for (int k : all) ur.push_back((s.nc[k] + 3) / 4);
+        for (int k : all) ur.push_back((s.nc[k] + 4 * TF_RPW - 1) / (4 * TF_RPW));
         add(p.mfwd, L_MPULL, 0, all, ur);
         add_tiles(p.mfwd, 0, tiles_of(0, all));
         add_tiles(p.mfwd, 1, tiles_of(1, all));

@@ -17,46 +17,49 @@ __global__ void k_mhist(int64_t len, const double *e2, const double *e3, double 
     if (i < len) w[i] = 4.0 * e2[i] - e3[i];
 }
 
-// r[row][q] = sum_j M[row][j] w[j][q], one wave per row (lane = q, K <= 64).
-// The row's column indices and values are loaded once across the lanes and
-// broadcast one at a time from a scalar register, so the gathers of the
-// rows of w do not wait on each other and eight of them are in flight.
+// r[row][q] = sum_j M[row][j] w[j][q], one wave per row at a time (lane = q,
+// K <= 64).  The row's column indices and values are loaded once across the
+// lanes and broadcast one at a time from a scalar register, so the gathers
+// of the rows of w do not wait on each other and eight of them are in flight.
 __global__ void __launch_bounds__(256) k_mspmv_rhs(int n, int K, const int *ia, const int *ja, const double *val,
                                                    const double *w, double *r) {
     const int lane = threadIdx.x & 63;
-    const int row = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (row >= n) return;
-    const int p0 = ia[row], p1 = ia[row + 1];
-    const int q = min(lane, K - 1);
-    double acc = 0.0;
-    for (int pb = p0; pb < p1; pb += 64) {
-        const int cnt = min(64, p1 - pb);
-        const int myj = lane < cnt ? ja[pb + lane] : 0;
-        const double myv = lane < cnt ? val[pb + lane] : 0.0;
-        const int myjlo = myj;
-        const long long myvb = __builtin_bit_cast(long long, myv);
-        int t = 0;
-        for (; t + 8 <= cnt; t += 8) {
-            double x[8], v[8];
+    const int row0 = blockIdx.x * 4 * TF_RPW + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int rr = 0; rr < TF_RPW; rr++) {
+        const int row = row0 + 4 * rr;
+        if (row >= n) return;
+        const int p0 = ia[row], p1 = ia[row + 1];
+        const int q = min(lane, K - 1);
+        double acc = 0.0;
+        for (int pb = p0; pb < p1; pb += 64) {
+            const int cnt = min(64, p1 - pb);
+            const int myj = lane < cnt ? ja[pb + lane] : 0;
+            const double myv = lane < cnt ? val[pb + lane] : 0.0;
+            const int myjlo = myj;
+            const long long myvb = __builtin_bit_cast(long long, myv);
+            int t = 0;
+            for (; t + 8 <= cnt; t += 8) {
+                double x[8], v[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int j = __builtin_amdgcn_readlane(myjlo, t + u);
-                const int vlo = __builtin_amdgcn_readlane((int)myvb, t + u);
-                const int vhi = __builtin_amdgcn_readlane((int)(myvb >> 32), t + u);
-                v[u] = __builtin_bit_cast(double, ((long long)vhi << 32) | (unsigned)vlo);
-                x[u] = w[(int64_t)j * K + q];
+                for (int u = 0; u < 8; u++) {
+                    const int j = __builtin_amdgcn_readlane(myjlo, t + u);
+                    const int vlo = __builtin_amdgcn_readlane((int)myvb, t + u);
+                    const int vhi = __builtin_amdgcn_readlane((int)(myvb >> 32), t + u);
+                    v[u] = __builtin_bit_cast(double, ((long long)vhi << 32) | (unsigned)vlo);
+                    x[u] = w[(int64_t)j * K + q];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++) acc += v[u] * x[u];
             }
-#pragma unroll
-            for (int u = 0; u < 8; u++) acc += v[u] * x[u];
+            for (; t < cnt; t++) {
+                const int j = __builtin_amdgcn_readlane(myjlo, t);
+                const int vlo = __builtin_amdgcn_readlane((int)myvb, t);
+                const int vhi = __builtin_amdgcn_readlane((int)(myvb >> 32), t);
+                acc += __builtin_bit_cast(double, ((long long)vhi << 32) | (unsigned)vlo) * w[(int64_t)j * K + q];
+            }
         }
-        for (; t < cnt; t++) {
-            const int j = __builtin_amdgcn_readlane(myjlo, t);
-            const int vlo = __builtin_amdgcn_readlane((int)myvb, t);
-            const int vhi = __builtin_amdgcn_readlane((int)(myvb >> 32), t);
-            acc += __builtin_bit_cast(double, ((long long)vhi << 32) | (unsigned)vlo) * w[(int64_t)j * K + q];
-        }
+        if (lane < K) r[(int64_t)row * K + lane] = acc;
     }
-    if (lane < K) r[(int64_t)row * K + lane] = acc;
 }
 
 // r[edge][q] -= (2 dt) (didt w) for every source-wire entry (edge, q, w)
@@ -179,7 +182,8 @@ __global__ void __launch_bounds__(TF_NT) k_mg_xs(const FrontDesc *fd, const int 
 // ---- forward pull -----------------------------------------------------------------
 // Own rows of the fronts of one level: Yin[row] = b[perm[row]] - sum of the Z
 // rows (descendant updates) that land on row, in the fixed order of gidx.
-// One wave per row (lane = right-hand side); the row's Z indices are loaded
+// One wave per row at a time, TF_RPW rows per wave (lane = right-hand side);
+// the row's Z indices are loaded
 // once across the lanes and broadcast from scalar registers, so eight Z rows
 // are in flight at a time (summed in order).
 __global__ void __launch_bounds__(256) k_mpull(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
@@ -188,25 +192,28 @@ __global__ void __launch_bounds__(256) k_mpull(const FrontDesc *fd, const int *l
     const int i = seg_find(pref, cnt, blockIdx.x);
     const FrontDesc d = fd[list[i]];
     const int lane = threadIdx.x & 63;
-    const int r = (int)(blockIdx.x - pref[i]) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (r >= d.nc) return;
-    const int row = d.c0 + r, q = min(lane, K - 1);
-    double s = b[(int64_t)perm[row] * K + q];
-    const int p0 = gptr[row], p1 = gptr[row + 1];
-    for (int pb = p0; pb < p1; pb += 64) {
-        const int n = min(64, p1 - pb);
-        const int mine = lane < n ? gidx[pb + lane] : 0;
-        int t = 0;
-        for (; t + 8 <= n; t += 8) {
-            double z[8];
+    const int r0 = (int)(blockIdx.x - pref[i]) * 4 * TF_RPW + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int rr = 0; rr < TF_RPW; rr++) {
+        const int r = r0 + 4 * rr;
+        if (r >= d.nc) return;
+        const int row = d.c0 + r, q = min(lane, K - 1);
+        double s = b[(int64_t)perm[row] * K + q];
+        const int p0 = gptr[row], p1 = gptr[row + 1];
+        for (int pb = p0; pb < p1; pb += 64) {
+            const int n = min(64, p1 - pb);
+            const int mine = lane < n ? gidx[pb + lane] : 0;
+            int t = 0;
+            for (; t + 8 <= n; t += 8) {
+                double z[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) z[u] = Z[(int64_t)__builtin_amdgcn_readlane(mine, t + u) * K + q];
+                for (int u = 0; u < 8; u++) z[u] = Z[(int64_t)__builtin_amdgcn_readlane(mine, t + u) * K + q];
 #pragma unroll
-            for (int u = 0; u < 8; u++) s -= z[u];
+                for (int u = 0; u < 8; u++) s -= z[u];
+            }
+            for (; t < n; t++) s -= Z[(int64_t)__builtin_amdgcn_readlane(mine, t) * K + q];
         }
-        for (; t < n; t++) s -= Z[(int64_t)__builtin_amdgcn_readlane(mine, t) * K + q];
+        if (lane < K) Yin[(int64_t)row * K + lane] = s;
     }
-    if (lane < K) Yin[(int64_t)row * K + lane] = s;
 }
 
 // ---- host launchers --------------------------------------------------------------
@@ -231,7 +238,8 @@ void launch_mspmv_rhs(int n, int K, const int *ia, const int *ja, const double *
                       const double *e3, double *w, double *r, hipStream_t st) {
     const int64_t tot = (int64_t)n * K;
     hipLaunchKernelGGL(k_mhist, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, tot, e2, e3, w);
-    hipLaunchKernelGGL(k_mspmv_rhs, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, n, K, ia, ja, val, w, r);
+    hipLaunchKernelGGL(k_mspmv_rhs, dim3((unsigned)((n + 4 * TF_RPW - 1) / (4 * TF_RPW))), dim3(256), 0, st, n, K, ia,
+                       ja, val, w, r);
 }
 void launch_msource(int ns, int K, const int *edge, const int *src, const double *w, double didt, double cdt,
                     double *r, hipStream_t st) {

@@ -15,6 +15,7 @@
 // vectors: the tile kernels' unmasked stage loads may read up to 63 values
 // beyond the last row of an array (results of those lanes are discarded).
 #define TF_READ_SLACK 4096
+#define TF_RPW 4  // rows per wave of the row-per-wave kernels (SpMV, pull): 4 * TF_RPW rows per workgroup
 
 // L11 inversion by doubling: leading dimension of the scratch block T (|J| x b)
 // of a merge at block size b in a front of nc columns (|J| <= min(b, nc - b)).
