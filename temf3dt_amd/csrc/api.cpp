@@ -1121,6 +1121,8 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
     int cur = 0;
     for (double *p : {v.e1, v.e2, v.e3}) HIPCHK(hipMemsetAsync(p, 0, sizeof(double) * n * c->nsrc, c->st));
     if (outp) prepare_slices(c);
+    // factors from before this run may be of other materials: none is reused
+    c->slot_live[0] = c->slot_live[1] = false;
     // adjoint tape: node ids of the history states e2, e3 and of the ring
     const bool tape = c->ckpt_every > 0 && inout == 0;
     int id2 = 0, id3 = 0;
@@ -1193,7 +1195,9 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
             tstep = step_cur * dbl;
             ttime = time_cur - step_cur + tstep;
             const int other = 1 - cur;
-            timed_factor(other, tstep);
+            // the other slot still holds P(tstep) when the previous check kept
+            // dt: the same matrix, so the same factor (no refactorisation)
+            if (!(c->slot_live[other] && c->slot_dt[other] == tstep)) timed_factor(other, tstep);
             double *ea = v.ring[(idx - 1) % ring];
             double *eb = v.ring[(idx - 1 - dbl) % ring];
             rhs(c, step, ttime, ea, eb, tstep, negate, v.rh);
@@ -1211,7 +1215,6 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
             c->chk_err.push_back(error);
             c->chk_keep.push_back(keep ? 1 : 0);
             if (keep) {
-                c->slot_live[other] = false;
                 dcopy(c, v.e1, v.erec);
                 if (outp) output_row(c, step, time_cur, step_cur, v.e1);
                 if (tape) { row_of(f1id); id3 = id2; id2 = f1id; }

@@ -90,6 +90,11 @@ def test_run_matches_oracle_rows(gpu_mod_big, src_mod_big):
     np.testing.assert_array_equal(rows["t"], g["t"])
     ck = fe.checks()
     np.testing.assert_array_equal(ck["keep"], g["chk_keep"])
+    # a check right after a kept one reuses the other slot's factor of the
+    # same 2 dt; every other check and step 1 factorise
+    keep = np.asarray(ck["keep"])
+    assert keep[:-1].sum() > 0
+    assert fe.timers()["nfactor"] == 1 + len(keep) - keep[:-1].sum()
     # dB/dt_z at the receiver, 1e-6 relative over the rows past the ramp
     a = rows["fields"][:, 0, 5]
     b = g["fields"][:, 0, 5]
