@@ -110,6 +110,9 @@ int tf_analyze(tf_ctx *ctx, int leaf_size);
  *   "graphs"    1 (default): regular steps of tf_run (sources off) replay a
  *               captured hipGraph of right-hand side, solve and history copy;
  *               0: every step is launched kernel by kernel.  Bit-identical.
+ *   "overlap"   1 (default): tf_run factors the doubling check's P(2 dt) on a
+ *               second stream while the steps before the check run; 0: at
+ *               the check.  Bit-identical.
  * Results do not depend on the knobs beyond floating-point summation order. */
 int tf_set_tuning(tf_ctx *ctx, const char *key, int value);
 /* Kernel timing with HIP events on the context's stream (no reference

@@ -210,6 +210,7 @@ double source_didt(int source_type, int pulse_type, double t, double t0, double 
 struct tf_ctx {
     int device = 0;
     hipStream_t st = nullptr;
+    hipStream_t st2 = nullptr;  // the doubling check's factorisation, overlapped with the steps before it
     std::string err;
     Topology topo;
     bool have_mesh = false, have_mat = false, assembled = false, analyzed = false;
@@ -322,6 +323,7 @@ struct tf_ctx {
     int64_t *d_slot_off = nullptr;
     double *d_mcdt = nullptr, *d_mcoef = nullptr, *d_mnorm = nullptr, *d_mstage = nullptr;
     bool use_graphs = true;
+    bool overlap_factor = true;  // tf_run: factor P(2 dt) on st2 ahead of the check
     // rows of the last tf_run_models: all receivers per step, per-model metadata
     std::vector<int> mm_step;                  // [steps]
     std::vector<double> mm_t, mm_dt;           // [steps][M]
@@ -405,7 +407,8 @@ int fail(tf_ctx *c, int code, const std::string &msg) {
 // per-category totals when the pool fills up and on tf_get_profile.
 void prof_fold(tf_ctx *c) {
     if (c->prof_rec.empty()) return;
-    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipStreamSynchronize(c->st));  // events are recorded on both streams
+    HIPCHK(hipStreamSynchronize(c->st2));
     for (size_t i = 0; i + 3 <= c->prof_rec.size(); i += 3) {
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, c->ev_pool[c->prof_rec[i + 1]], c->ev_pool[c->prof_rec[i + 2]]));
@@ -1143,9 +1146,12 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
         c->row_node.push_back(k);
     };
     GraphCache graphs;
-    hipEvent_t f0, f1;
-    HIPCHK(hipEventCreate(&f0));
-    HIPCHK(hipEventCreate(&f1));
+    struct StreamDrain {  // no factorisation stays in flight past this run
+        hipStream_t s;
+        ~StreamDrain() { (void)hipStreamSynchronize(s); }
+    } drain{c->st2};
+    hipEvent_t f0, f1, a0, a1, a2;
+    for (hipEvent_t *e : {&f0, &f1, &a0, &a1, &a2}) HIPCHK(hipEventCreate(e));
     auto timed_factor = [&](int slot, double dt) {
         HIPCHK(hipEventRecord(f0, c->st));
         factor_dt(c, slot, dt);
@@ -1153,6 +1159,40 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
         check_flag(c);
         c->timers[1] += elapsed(f0, f1);
         c->timers[7] += 1.0;
+    };
+    // The check at step S needs P(2 dt) in the other slot.  It is factored on
+    // the second stream as soon as dt is known (step 1, or a check that
+    // changed dt), while the steps before S run on the first: the two touch
+    // disjoint buffers (the slot being factored, U and the inversion scratch
+    // against the current slot and the solve vectors).  The check waits for
+    // it.  Only started when the loop will reach the next check.
+    int pend = -1;
+    double pend_dt = 0.0;
+    auto reaches_next_check = [&]() {
+        const int S = (step / c->sp_double + 1) * c->sp_double;
+        if (max_steps > 0 && S > max_steps) return false;
+        double t = time_cur;
+        for (int k = step + 1; k < S; k++) t = t + step_cur;
+        return t < time_limit;
+    };
+    auto start_factor = [&](int slot, double dt) {
+        if (!c->overlap_factor || !reaches_next_check()) return;
+        HIPCHK(hipEventRecord(a0, c->st));  // the slot's last solve is enqueued before this point
+        HIPCHK(hipStreamWaitEvent(c->st2, a0, 0));
+        std::swap(c->st, c->st2);
+        HIPCHK(hipEventRecord(a1, c->st));
+        factor_dt(c, slot, dt);
+        HIPCHK(hipEventRecord(a2, c->st));
+        std::swap(c->st, c->st2);
+        pend = slot;
+        pend_dt = dt;
+    };
+    auto finish_factor = [&] {
+        HIPCHK(hipStreamWaitEvent(c->st, a2, 0));
+        check_flag(c);
+        c->timers[1] += elapsed(a1, a2);
+        c->timers[7] += 1.0;
+        pend = -1;
     };
     while (time_cur < time_limit) {
         if (max_steps > 0 && step >= max_steps) break;
@@ -1169,6 +1209,7 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
             if (inout == 2) { dcopy(c, v.e2, res); dcopy(c, v.e3, res); }
             if (outp) output_row(c, 0, time_cur - step_cur, step_cur, v.e2);
             timed_factor(cur, step_cur);
+            start_factor(1 - cur, step_cur * dbl);
         }
         if (step % c->sp_double == 0) {
             dcopy(c, v.te2, v.e2);
@@ -1197,7 +1238,12 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
             const int other = 1 - cur;
             // the other slot still holds P(tstep) when the previous check kept
             // dt: the same matrix, so the same factor (no refactorisation)
-            if (!(c->slot_live[other] && c->slot_dt[other] == tstep)) timed_factor(other, tstep);
+            if (pend >= 0) {
+                if (pend != other || pend_dt != tstep) throw std::runtime_error("factorisation in flight for another slot");
+                finish_factor();
+            } else if (!(c->slot_live[other] && c->slot_dt[other] == tstep)) {
+                timed_factor(other, tstep);
+            }
             double *ea = v.ring[(idx - 1) % ring];
             double *eb = v.ring[(idx - 1 - dbl) % ring];
             rhs(c, step, ttime, ea, eb, tstep, negate, v.rh);
@@ -1233,6 +1279,7 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
                 if (tape) { row_of(fid[0]); id3 = ring_id[(idx - 1) % ring]; id2 = fid[0]; }
                 not_changed = 0;
                 idx = 0;
+                start_factor(1 - cur, step_cur * dbl);
             }
             continue;
         }
@@ -1277,8 +1324,11 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
     HIPCHK(hipGetLastError());
     if (tape) c->tape_valid = ((int)c->row_node.size() == c->nrows);
     if (inout == 1 && res) dcopy(c, res, v.e1 == v.e2 ? v.e1 : v.e2);
-    HIPCHK(hipEventDestroy(f0));
-    HIPCHK(hipEventDestroy(f1));
+    if (pend >= 0) {  // not reached by construction; drained all the same
+        HIPCHK(hipStreamSynchronize(c->st2));
+        check_flag(c);
+    }
+    for (hipEvent_t e : {f0, f1, a0, a1, a2}) HIPCHK(hipEventDestroy(e));
     // keep the rotated pointers in the context's vector table
     c->vecs[0] = v.e1; c->vecs[1] = v.e2; c->vecs[2] = v.e3; c->vecs[5] = v.te2; c->vecs[6] = v.te3;
 }
@@ -1710,7 +1760,9 @@ int tf_create(tf_ctx **out, int device) {
     }
     tf_ctx *c = new tf_ctx();
     c->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess) {
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess ||
+        hipStreamCreate(&c->st2) != hipSuccess) {
+        if (c->st) (void)hipStreamDestroy(c->st);
         delete c;
         g_create_err = "could not create a HIP stream on the device";
         return TF_ERR_HIP;
@@ -1723,9 +1775,11 @@ void tf_destroy(tf_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->st);
+    (void)hipStreamSynchronize(c->st2);
     free_all(c);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->st);
+    (void)hipStreamDestroy(c->st2);
     delete c;
 }
 
@@ -2067,6 +2121,9 @@ int tf_set_tuning(tf_ctx *c, const char *key, int value) {
         } else if (k == "graphs") {
             if (value != 0 && value != 1) return fail(c, TF_ERR_ARG, "graphs must be 0 or 1");
             c->use_graphs = value != 0;
+        } else if (k == "overlap") {
+            if (value != 0 && value != 1) return fail(c, TF_ERR_ARG, "overlap must be 0 or 1");
+            c->overlap_factor = value != 0;
         } else if (k == "mt_ks_n" || k == "mt_ks_t") {
             if (value != 2 && value != 4) return fail(c, TF_ERR_ARG, "k-steps per stage must be 2 or 4");
             (k == "mt_ks_n" ? c->mt_ks_n : c->mt_ks_t) = value;

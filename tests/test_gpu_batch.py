@@ -112,6 +112,27 @@ def test_graph_replay_matches_eager(loops):
         np.testing.assert_array_equal(a[k], b[k])
 
 
+@pytest.mark.parametrize("loops", [[0, 1, 2, 3], [1]])
+def test_overlapped_factor_matches_serial(loops):
+    """The check's factorisation of P(2 dt), run on a second stream while the
+    steps before the check go on, gives the rows, decisions and number of
+    factorisations of the run that factors at the check itself."""
+    from temf3dt_amd import workloads
+    m = workloads.multi_source_small_model()
+    out = {}
+    for ov in (1, 0):
+        print(f"[gpu] overlapped factorisation: sources {loops}, overlap={ov}", flush=True)
+        fe = gpu_from_model(m, loops=loops)
+        fe.set_tuning("overlap", ov)
+        out[ov] = (fe.run(), fe.checks(), fe.timers()["nfactor"])
+        fe.close()
+    (a, ca, na), (b, cb, nb) = out[1], out[0]
+    assert len(ca["keep"]) >= 3 and na == nb
+    np.testing.assert_array_equal(ca["keep"], cb["keep"])
+    for k in ("step", "t", "dt", "fields"):
+        np.testing.assert_array_equal(a[k], b[k])
+
+
 @pytest.mark.parametrize("knobs", [
     {"mt_units": 64, "mt_chmin": 16},                        # every long tile split into partials
     {"mt_units": 64, "mt_chmin": 16, "mt_ks_n": 4, "mt_ks_t": 2},
@@ -148,7 +169,8 @@ def test_block_solve_tuning_knobs(gpu_mod_big_batch, knobs):
 def test_tuning_rejects_bad_values(gpu_mod_big_batch):
     from temf3dt_amd.fetd import FetdError
     fe, _ = gpu_mod_big_batch
-    for k, v in [("mt_ks_n", 3), ("mt_chmin", 10), ("mt_units", 1), ("graphs", 2), ("no_such_knob", 1)]:
+    for k, v in [("mt_ks_n", 3), ("mt_chmin", 10), ("mt_units", 1), ("graphs", 2), ("overlap", -1),
+                 ("no_such_knob", 1)]:
         with pytest.raises(FetdError):
             fe.set_tuning(k, v)
 
