@@ -686,8 +686,12 @@ void build_plan(tf_ctx *c) {
         std::vector<int> all(s.lev_sn.begin() + s.lev_ptr[l], s.lev_sn.begin() + s.lev_ptr[l + 1]);
         std::vector<int64_t> ur;
         for (int k : all) ur.push_back((s.nc[k] + 4 * TF_RPW - 1) / (4 * TF_RPW));
-        add(p.mfwd, L_MPULL, 0, all, ur);
-        add_tiles(p.mfwd, 0, tiles_of(0, all));
+        if (l == 0) {  // fronts without children: Yin is b itself, read through perm by the tiles
+            add_tiles(p.mfwd, 4, tiles_of(0, all));
+        } else {
+            add(p.mfwd, L_MPULL, 0, all, ur);
+            add_tiles(p.mfwd, 0, tiles_of(0, all));
+        }
         add_tiles(p.mfwd, 1, tiles_of(1, all));
     }
     for (int l = s.nlevels - 1; l >= 0; l--) {
@@ -904,14 +908,22 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
             tf::launch_mpull(fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, b, c->d_perm, c->d_gptr, c->d_gidx,
                              c->d_mZ, c->d_mV, K, c->st);
             break;
-        case L_MT:  // own forward (Y from Yin), struct forward (Z from Y)
-            tf::launch_mtile(l.p0, c->mt_ks_n, L, fd, c->d_units + l.off, l.cnt, K, l.p0 == 0 ? c->d_mV : c->d_mY,
-                             nullptr, Dn, nullptr, l.p0 == 0 ? c->d_mY : c->d_mZ, c->d_part, 0u, nullptr, nullptr,
-                             nullptr, c->st);
+        case L_MT:  // own forward (Y from Yin, or from b), struct forward (Z from Y)
+            if (l.p0 == 4)
+                tf::launch_mtile(4, c->mt_ks_n, L, fd, c->d_units + l.off, l.cnt, K, b, nullptr, Dn, nullptr,
+                                 c->d_mY, c->d_part, gbytes, c->d_perm, nullptr, nullptr, c->st);
+            else
+                tf::launch_mtile(l.p0, c->mt_ks_n, L, fd, c->d_units + l.off, l.cnt, K,
+                                 l.p0 == 0 ? c->d_mV : c->d_mY, nullptr, Dn, nullptr, l.p0 == 0 ? c->d_mY : c->d_mZ,
+                                 c->d_part, 0u, nullptr, nullptr, nullptr, c->st);
             break;
         case L_MTRED:
-            tf::launch_mtile_red(l.p0, fd, c->d_units + l.off, l.cnt, K, l.p0 == 0 ? c->d_mV : c->d_mY, nullptr,
-                                 Dn, l.p0 == 0 ? c->d_mY : c->d_mZ, c->d_part, nullptr, nullptr, nullptr, c->st);
+            if (l.p0 == 4)
+                tf::launch_mtile_red(4, fd, c->d_units + l.off, l.cnt, K, b, nullptr, Dn, c->d_mY, c->d_part,
+                                     c->d_perm, nullptr, nullptr, c->st);
+            else
+                tf::launch_mtile_red(l.p0, fd, c->d_units + l.off, l.cnt, K, l.p0 == 0 ? c->d_mV : c->d_mY, nullptr,
+                                     Dn, l.p0 == 0 ? c->d_mY : c->d_mZ, c->d_part, nullptr, nullptr, nullptr, c->st);
             break;
         }
         prof_close(c, prof_category(l.kind), pm);

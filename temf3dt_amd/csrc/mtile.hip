@@ -27,7 +27,10 @@
 // order, so the result does not depend on scheduling.
 #include "tf_device.h"
 
-enum { MT_OWN_FWD = 0, MT_STR_FWD = 1, MT_BWD_T = 2, MT_BWD_X = 3 };
+// MT_OWN_FWD_B: the own forward product of a front without children, whose
+// Yin is the right-hand side itself: B rows are read from b through perm
+// (no pull launch for the lowest level).
+enum { MT_OWN_FWD = 0, MT_STR_FWD = 1, MT_BWD_T = 2, MT_BWD_X = 3, MT_OWN_FWD_B = 4 };
 
 __device__ __forceinline__ void acc16_zero(tf_d4 acc[4][4]) {
 #pragma unroll
@@ -40,7 +43,7 @@ __device__ __forceinline__ void acc16_zero(tf_d4 acc[4][4]) {
 // (strict) triangle applies, per mode.
 template <int MODE>
 __device__ __forceinline__ void mt_geom(const FrontDesc &d, int &nout, int &roff) {
-    if (MODE == MT_OWN_FWD || MODE == MT_BWD_X) { nout = d.nc; roff = 0; }
+    if (MODE == MT_OWN_FWD || MODE == MT_OWN_FWD_B || MODE == MT_BWD_X) { nout = d.nc; roff = 0; }
     else if (MODE == MT_STR_FWD) { nout = d.m; roff = d.nc; }
     else { nout = d.nc; roff = d.nc; }
 }
@@ -60,6 +63,8 @@ __device__ __forceinline__ void mt_store(const FrontDesc &d, int o, int q, doubl
     if (MODE == MT_OWN_FWD) {  // Y[c0+o] = Yin[c0+o] + s
         const int64_t oo = (int64_t)(d.c0 + o) * K + q;
         out[oo] = in0[oo] + s;
+    } else if (MODE == MT_OWN_FWD_B) {  // Y[c0+o] = b[perm[c0+o]] + s
+        out[(int64_t)(d.c0 + o) * K + q] = in0[(int64_t)xo.perm[d.c0 + o] * K + q] + s;
     } else if (MODE == MT_STR_FWD) {  // Z[soff+o] = s (subtracted where it is pulled)
         out[(d.soff + o) * K + q] = s;
     } else if (MODE == MT_BWD_T) {  // T = Y/d - s
@@ -256,6 +261,8 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
     // diagonal and upper part of L11's storage zero
     if (MODE == MT_OWN_FWD || MODE == MT_STR_FWD)
         mt_wave<false, false, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
+    else if (MODE == MT_OWN_FWD_B)
+        mt_wave<false, true, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, in0, gbytes, xo.perm + d.c0, K, acc);
     else if (MODE == MT_BWD_T)
         mt_wave<true, true, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, in0, gbytes, sidx + d.soff, K, acc);
     else
@@ -306,7 +313,8 @@ __global__ void __launch_bounds__(256) k_mtile_red(const FrontDesc *fd, const Ti
 }
 
 namespace tf {
-// mode: 0 own forward, 1 struct forward, 2 backward T, 3 backward X
+// mode: 0 own forward, 1 struct forward, 2 backward T, 3 backward X,
+// 4 own forward of a childless front straight from b (through perm)
 // gbytes: size of the gathered vector (backward T form), < 4 GiB
 // perm, x1, x2: backward X also writes its rows to x1 (and x2) in edge order
 void launch_mtile(int mode, int ks, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits, int K,
@@ -320,6 +328,7 @@ void launch_mtile(int mode, int ks, const double *L, const FrontDesc *fd, const 
     if (ks == 4) {
         switch (mode) {
         case MT_OWN_FWD: TF_MT(MT_OWN_FWD, 4); break;
+        case MT_OWN_FWD_B: TF_MT(MT_OWN_FWD_B, 4); break;
         case MT_STR_FWD: TF_MT(MT_STR_FWD, 4); break;
         case MT_BWD_T: TF_MT(MT_BWD_T, 4); break;
         default: TF_MT(MT_BWD_X, 4); break;
@@ -327,6 +336,7 @@ void launch_mtile(int mode, int ks, const double *L, const FrontDesc *fd, const 
     } else {
         switch (mode) {
         case MT_OWN_FWD: TF_MT(MT_OWN_FWD, 2); break;
+        case MT_OWN_FWD_B: TF_MT(MT_OWN_FWD_B, 2); break;
         case MT_STR_FWD: TF_MT(MT_STR_FWD, 2); break;
         case MT_BWD_T: TF_MT(MT_BWD_T, 2); break;
         default: TF_MT(MT_BWD_X, 2); break;
@@ -342,6 +352,7 @@ void launch_mtile_red(int mode, const FrontDesc *fd, const TileUnit *units, int 
     const MtEdgeOut xo{perm, x1, x2};
     switch (mode) {
     case MT_OWN_FWD: hipLaunchKernelGGL(k_mtile_red<MT_OWN_FWD>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo); break;
+    case MT_OWN_FWD_B: hipLaunchKernelGGL(k_mtile_red<MT_OWN_FWD_B>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo); break;
     case MT_STR_FWD: hipLaunchKernelGGL(k_mtile_red<MT_STR_FWD>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo); break;
     case MT_BWD_T: hipLaunchKernelGGL(k_mtile_red<MT_BWD_T>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo); break;
     default: hipLaunchKernelGGL(k_mtile_red<MT_BWD_X>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo); break;
