@@ -159,7 +159,7 @@ enum LaunchKind {
 struct Launch {
     int kind, p0, off, cnt, poff;
     int64_t nwg;
-    int aux;  // reductions: offset of the paired partial launch's prefix sums
+    int aux;  // factor updates: mode; backward X tiles: 1 = no front-order copy of x (lowest level)
 };
 
 int prof_category(int kind) {
@@ -697,7 +697,10 @@ void build_plan(tf_ctx *c) {
     for (int l = s.nlevels - 1; l >= 0; l--) {
         std::vector<int> all(s.lev_sn.begin() + s.lev_ptr[l], s.lev_sn.begin() + s.lev_ptr[l + 1]);
         add_tiles(p.mbwd, 2, tiles_of(2, all));
+        const size_t nb = p.mbwd.size();
         add_tiles(p.mbwd, 3, tiles_of(3, all));
+        if (l == 0)  // no lower level gathers these rows: edge order only
+            for (size_t i = nb; i < p.mbwd.size(); i++) p.mbwd[i].aux = 1;
     }
 }
 
@@ -933,16 +936,17 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
     for (const Launch &l : p.mbwd) {
         const int pm = prof_mark(c);
         const bool bx = l.p0 == 3;
+        double *out = bx ? (l.aux == 1 ? nullptr : c->d_mxw) : c->d_mT;
         switch (l.kind) {
         case L_MT:
             tf::launch_mtile(l.p0, c->mt_ks_t, L, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : c->d_mxw,
-                             bx ? c->d_mT : c->d_mY, Dn, c->d_sidx, bx ? c->d_mxw : c->d_mT, c->d_part, gbytes,
-                             c->d_perm, bx ? x : nullptr, bx ? x2 : nullptr, c->st);
+                             bx ? c->d_mT : c->d_mY, Dn, c->d_sidx, out, c->d_part, gbytes, c->d_perm,
+                             bx ? x : nullptr, bx ? x2 : nullptr, c->st);
             break;
         case L_MTRED:
             tf::launch_mtile_red(l.p0, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : c->d_mxw,
-                                 bx ? c->d_mT : c->d_mY, Dn, bx ? c->d_mxw : c->d_mT, c->d_part, c->d_perm,
-                                 bx ? x : nullptr, bx ? x2 : nullptr, c->st);
+                                 bx ? c->d_mT : c->d_mY, Dn, out, c->d_part, c->d_perm, bx ? x : nullptr,
+                                 bx ? x2 : nullptr, c->st);
             break;
         }
         prof_close(c, prof_category(l.kind), pm);

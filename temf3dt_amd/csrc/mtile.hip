@@ -70,10 +70,10 @@ __device__ __forceinline__ void mt_store(const FrontDesc &d, int o, int q, doubl
     } else if (MODE == MT_BWD_T) {  // T = Y/d - s
         const int64_t oo = (int64_t)(d.c0 + o) * K + q;
         out[oo] = in1[oo] / Dn[d.doff + o] - s;
-    } else {  // X = T + s
+    } else {  // X = T + s (front order unless out is null: rows no lower level gathers)
         const int64_t oo = (int64_t)(d.c0 + o) * K + q;
         const double v = in1[oo] + s;
-        out[oo] = v;
+        if (out) out[oo] = v;
         if (xo.x1) {
             const int64_t e = (int64_t)xo.perm[d.c0 + o] * K + q;
             xo.x1[e] = v;
