@@ -99,7 +99,7 @@ int tf_get_receiver_vectors(const tf_ctx *ctx, double *n_arr, double *d_arr);
 /* Symbolic analysis, once per mesh pattern (the reference redoes it in every
  * phase-12 call).  Optional: tf_factor runs it on first use.  leaf_size is the
  * largest vertex set the nested dissection leaves undivided; <= 0 keeps the
- * default (256). */
+ * default (192). */
 int tf_analyze(tf_ctx *ctx, int leaf_size);
 /* Tuning knobs of the block-solve tile kernels (no reference counterpart;
  * set by the autotuner, temf3dt_amd/autotune.py).  Keys:

@@ -336,7 +336,7 @@ struct tf_ctx {
     std::vector<int> prof_rec;  // triples: category, start event, end event
     double prof_ms[TF_PROF_NCAT] = {0};
     int64_t prof_cnt[TF_PROF_NCAT] = {0};
-    int leaf_size = 256;  // nested-dissection leaf: larger leaves give fewer, larger fronts (fastest of 32-640 on a 1.35M-tet mesh with 64 right-hand sides)
+    int leaf_size = 192;  // nested-dissection leaf: larger leaves give fewer, larger fronts but more fill (fastest of 128-384 on a 1.35M-tet mesh with 64 right-hand sides)
 };
 
 namespace {
