@@ -7,7 +7,7 @@
 #define TF_PB 64       // panel width (columns per elimination panel)
 #define TF_TB 64       // tile edge of trailing-update tiles
 #define TF_NT 256      // threads per workgroup
-#define TF_TRSM_ROWS 256
+#define TF_TRSM_ROWS 128
 #define TF_UT 128      // tile edge of the matrix-core trailing update
 #define TF_SP 256      // super-panel: panel columns whose trailing update is applied at once
 
