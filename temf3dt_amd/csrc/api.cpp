@@ -622,14 +622,15 @@ void build_plan(tf_ctx *c) {
         std::vector<Tile> t;
         for (int k : fr) {
             const int nc = s.nc[k], m = m_of(k);
+            constexpr int TW = 16 * TF_MT_NB;  // output rows (columns) per tile
             if (mode == 0)
-                for (int r = 0; r < nc; r += 64) t.push_back({k, r, 0, std::min(nc, r + 64)});
+                for (int r = 0; r < nc; r += TW) t.push_back({k, r, 0, std::min(nc, r + TW)});
             else if (mode == 1)  // row tiles start on 16-row lines of the column (rows nc + r)
-                for (int r = -(nc & 15); r < m; r += 64) t.push_back({k, r, 0, nc});
+                for (int r = -(nc & 15); r < m; r += TW) t.push_back({k, r, 0, nc});
             else if (mode == 2)
-                for (int j = 0; j < nc; j += 64) t.push_back({k, j, 0, m});
+                for (int j = 0; j < nc; j += TW) t.push_back({k, j, 0, m});
             else
-                for (int j = 0; j < nc; j += 64) t.push_back({k, j, j, nc});
+                for (int j = 0; j < nc; j += TW) t.push_back({k, j, j, nc});
         }
         return t;
     };

@@ -1,10 +1,12 @@
 // mtile.hip -- wave-level FP64 matrix-core tile products of the K-RHS block
 // solve (K <= 64 sources sharing one LDL^T factor).
 //
-// Each wave owns one work unit: a 64 x 64 output tile (64 matrix rows or
-// columns x 64 right-hand sides) accumulated over an inner range [k0, k1).
-// Operands go straight from memory into the MFMA lane layout -- no LDS, no
-// barriers -- so many waves per CU stream the factor at once.  The four
+// Each wave owns one work unit: a (16 * TF_MT_NB) x 64 output tile (32 matrix
+// rows or columns x 64 right-hand sides at TF_MT_NB = 2) accumulated over an
+// inner range [k0, k1).  Operands go straight from memory into the MFMA lane
+// layout -- no LDS, no barriers -- so many waves per CU stream the factor at
+// once (about 120 VGPRs: four waves per SIMD; 64-row tiles took 200 VGPRs,
+// two waves per SIMD, and ran 3 % slower end to end).  The four
 // products of a front (X11 = L11^-1 stored in place by the factorisation):
 //   forward   Y_own = Yin + strict(X11) Yin           (N form, triangle)
 //             Z = L21 Y_own                          (N form)
@@ -22,7 +24,7 @@
 // A carries the matrix (output index = l%16), B the right-hand sides, so
 // result register r of block (a, b) is output 16b + 4r + l/16, rhs 16a + l%16.
 //
-// Units whose tile needs several inner chunks write a 64 x 64 partial tile
+// Units whose tile needs several inner chunks write a partial tile
 // (part[slot][out][rhs]); a reduction launch sums the chunks in a fixed
 // order, so the result does not depend on scheduling.
 #include "tf_device.h"
@@ -32,11 +34,11 @@
 // (no pull launch for the lowest level).
 enum { MT_OWN_FWD = 0, MT_STR_FWD = 1, MT_BWD_T = 2, MT_BWD_X = 3, MT_OWN_FWD_B = 4 };
 
-__device__ __forceinline__ void acc16_zero(tf_d4 acc[4][4]) {
+__device__ __forceinline__ void mt_acc_zero(tf_d4 acc[4][TF_MT_NB]) {
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = (tf_d4){0.0, 0.0, 0.0, 0.0};
+        for (int b = 0; b < TF_MT_NB; b++) acc[a][b] = (tf_d4){0.0, 0.0, 0.0, 0.0};
 }
 
 // Output-row/column count, matrix offset inside the L block, and whether the
@@ -113,7 +115,7 @@ __device__ __forceinline__ double mt_ld(const double *base, uint32_t off) {
 // partial last stage) clamp the inner index and zero what lies outside.
 template <bool TFORM, bool GATH, int KS>
 __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f, int roff, int o0, int k0, int k1,
-                                        const double *Bm, uint32_t bbytes, const int *bidx, int K, tf_d4 acc[4][4]) {
+                                        const double *Bm, uint32_t bbytes, const int *bidx, int K, tf_d4 acc[4][TF_MT_NB]) {
     constexpr int SW = 4 * KS;  // inner values per stage
     if (k0 >= k1) return;
     const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
@@ -123,9 +125,9 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
     // can leave the block stays there: the T-form output column (one offset
     // per output block b), while an N-form row past the block's last column
     // reads at most 62 values beyond it (the factor storage ends with slack).
-    uint32_t la[TFORM ? 4 : 1];
+    uint32_t la[TFORM ? TF_MT_NB : 1];
 #pragma unroll
-    for (int b = 0; b < (TFORM ? 4 : 1); b++)
+    for (int b = 0; b < (TFORM ? TF_MT_NB : 1); b++)
         la[b] = TFORM ? (uint32_t)((int64_t)(o0 + 16 * b + l16) * f + roff + KS * lk) * 8u
                       : (uint32_t)(lk * f + roff + o0 + l16) * 8u;
     const uint32_t ba = GATH ? (uint32_t)l16 * 8u : (uint32_t)((TFORM ? KS * lk : lk) * K + l16) * 8u;
@@ -135,26 +137,26 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
     const int ja = min(nst, (k0 - kb + SW - 1) / SW);
     const int jb = max(ja, (k1 - kb) / SW);
     auto inner = [&](int ib, int s) { return TFORM ? (ib + KS * lk + s) : (ib + 4 * s + lk); };
-    auto mult = [&](const double xa[KS][4], const double yb[KS][4]) {
+    auto mult = [&](const double xa[KS][TF_MT_NB], const double yb[KS][4]) {
 #pragma unroll
         for (int s = 0; s < KS; s++)
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
-                for (int b = 0; b < 4; b++)
+                for (int b = 0; b < TF_MT_NB; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s][b], yb[s][a], acc[a][b], 0, 0, 0);
     };
     // a masked stage, loaded and multiplied on its own
     auto masked_stage = [&](int js) {
         const int ib = kb + js * SW;
-        double xa[KS][4], yb[KS][4];
+        double xa[KS][TF_MT_NB], yb[KS][4];
 #pragma unroll
         for (int s = 0; s < KS; s++) {
             const int i = inner(ib, s);
             const bool iv = i >= k0 && i < k1;
             const int ic = min(max(i, k0), k1 - 1);
 #pragma unroll
-            for (int b = 0; b < 4; b++) {
+            for (int b = 0; b < TF_MT_NB; b++) {
                 const int ob = o0 + 16 * b + l16;
                 const uint32_t off = TFORM ? (uint32_t)((int64_t)ob * f + roff + ic) * 8u
                                            : (uint32_t)((int64_t)ic * f + roff + ob) * 8u;
@@ -178,20 +180,20 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
 #pragma unroll
             for (int s = 0; s < KS; s++) r[s] = bidx[inner(ib, s)];
         };
-        auto load = [&](int js, const int r[KS], double xa[KS][4], double yb[KS][4]) {
+        auto load = [&](int js, const int r[KS], double xa[KS][TF_MT_NB], double yb[KS][4]) {
             const int ib = kb + js * SW;
             if (TFORM && KS % 2 == 0) {  // a lane's KS rows of a column are contiguous: 16-byte loads
 #pragma unroll
                 for (int s = 0; s < KS; s += 2)
 #pragma unroll
-                    for (int b = 0; b < 4; b++) mt_bld2(rL, la[b], (uint32_t)(ib + s) * 8u, xa[s][b], xa[s + 1][b]);
+                    for (int b = 0; b < TF_MT_NB; b++) mt_bld2(rL, la[b], (uint32_t)(ib + s) * 8u, xa[s][b], xa[s + 1][b]);
             }
 #pragma unroll
             for (int s = 0; s < KS; s++) {
                 const uint32_t sl = (uint32_t)(TFORM ? (ib + s) : (ib + 4 * s) * f) * 8u;
                 if (!(TFORM && KS % 2 == 0))
 #pragma unroll
-                    for (int b = 0; b < 4; b++)
+                    for (int b = 0; b < TF_MT_NB; b++)
                         xa[s][b] = TFORM ? mt_bld(rL, la[b], sl) : mt_bld(rL, la[0] + 128u * b, sl);
                 if (GATH) {
                     const uint32_t vo = ((uint32_t)r[s] * (uint32_t)K) * 8u + ba;
@@ -204,7 +206,7 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
                 }
             }
         };
-        double xa0[KS][4], yb0[KS][4], xa1[KS][4], yb1[KS][4];
+        double xa0[KS][TF_MT_NB], yb0[KS][4], xa1[KS][TF_MT_NB], yb1[KS][4];
         int rA[KS], rB[KS];
         const int last = jb - 1;
         int js = ja;
@@ -249,8 +251,8 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
     const FrontDesc d = fd[tu.front];
     int nout, roff;
     mt_geom<MODE>(d, nout, roff);
-    tf_d4 acc[4][4];
-    acc16_zero(acc);
+    tf_d4 acc[4][TF_MT_NB];
+    mt_acc_zero(acc);
     const double *Lf = L + d.loff;
     const uint32_t lbytes = (uint32_t)((int64_t)d.nc * d.ld * 8);
     // B operand rows: own right-hand sides (Yin, Y_own, T) at c0, or the
@@ -270,7 +272,7 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
     const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
     if (tu.slot < 0) {
 #pragma unroll
-        for (int b = 0; b < 4; b++)
+        for (int b = 0; b < TF_MT_NB; b++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int o = tu.t0 + 16 * b + 4 * r + lk;
@@ -284,7 +286,7 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
     } else {
         double *pp = part + (int64_t)tu.slot * 4096;
 #pragma unroll
-        for (int b = 0; b < 4; b++)
+        for (int b = 0; b < TF_MT_NB; b++)
 #pragma unroll
             for (int r = 0; r < 4; r++)
 #pragma unroll
@@ -305,7 +307,7 @@ __global__ void __launch_bounds__(256) k_mtile_red(const FrontDesc *fd, const Ti
     mt_geom<MODE>(d, nout, roff);
     const int e = (blockIdx.x & 15) * 256 + threadIdx.x;
     const int o = tu.t0 + (e >> 6), q = e & 63;
-    if (o < 0 || o >= nout || q >= K) return;
+    if ((e >> 6) >= 16 * TF_MT_NB || o < 0 || o >= nout || q >= K) return;
     const double *p0 = part + (int64_t)tu.k0 * 4096 + e;
     double s = 0.0;
     for (int c = 0; c < tu.k1; c++) s += p0[(int64_t)c * 4096];
