@@ -224,6 +224,8 @@ def main():
     ap.add_argument("--ckpt", type=int, default=50, help="adjoint checkpoint interval (steps)")
     ap.add_argument("--leaf", type=int, default=0, help="nested-dissection leaf size (0: the library default)")
     ap.add_argument("--copy-probe", action="store_true", help="1 GiB device copies only (counter calibration)")
+    ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
+                    help="tf_set_tuning key for the batched run (repeatable), applied before the autotuner")
     a = ap.parse_args()
     if a.copy_probe:
         copy_probe()
@@ -238,6 +240,9 @@ def main():
     m = build_model(a.model)
     nsrc = len(m.loops)
     fe, t_an = setup_gpu(m, list(range(nsrc)), a.leaf)
+    for kv in a.set:
+        key, val = kv.split("=")
+        fe.set_tuning(key, int(val))
     st = fe.factor_stats()
     n = fe.nE
     print(f"[bench] {a.model}: {m.ntet} tets, {n} edges, nnzU {fe.nnz}, nnzL {st['nnzL']:.3e}, "

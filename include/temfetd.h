@@ -107,6 +107,8 @@ int tf_analyze(tf_ctx *ctx, int leaf_size);
  *   "mt_chmin"  minimum inner chunk of a split tile (default 256, multiple of 16)
  *   "mt_ks_n"   k-steps per stage of the N-form kernels (2 or 4, default 2)
  *   "mt_ks_t"   k-steps per stage of the T-form kernels (2 or 4, default 2)
+ *   "mt_wide"   levels of the elimination tree with at most this many fronts
+ *               use 64-row tiles, the others 32-row tiles (default 64, >= 0)
  *   "graphs"    1 (default): regular steps of tf_run (sources off) replay a
  *               captured hipGraph of right-hand side, solve and history copy;
  *               0: every step is launched kernel by kernel.  Bit-identical.

@@ -85,15 +85,15 @@ void launch_sens(int nT, int K, const int *tet_edg, const uint8_t *orient, const
 void launch_axpy(int64_t len, double a, const double *x, double *y, hipStream_t st);
 void launch_scale(int n, const double *s, double *g, hipStream_t st);
 // mtile.hip
-void launch_mtile(int mode, int ks, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits, int K,
-                  const double *in0, const double *in1, const double *Dn, const int *sidx, double *out, double *part,
-                  uint32_t gbytes, const int *perm, double *x1, double *x2, hipStream_t st);
+void launch_mtile(int mode, int ks, int nb, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits,
+                  int K, const double *in0, const double *in1, const double *Dn, const int *sidx, double *out,
+                  double *part, uint32_t gbytes, const int *perm, double *x1, double *x2, hipStream_t st);
 void launch_mpull(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg, const double *b,
                   const int *perm, const int *gptr, const int *gidx, const double *Z, double *Yin, int K,
                   hipStream_t st);
-void launch_mtile_red(int mode, const FrontDesc *fd, const TileUnit *units, int nunits, int K, const double *in0,
-                      const double *in1, const double *Dn, double *out, const double *part, const int *perm,
-                      double *x1, double *x2, hipStream_t st);
+void launch_mtile_red(int mode, int nb, const FrontDesc *fd, const TileUnit *units, int nunits, int K,
+                      const double *in0, const double *in1, const double *Dn, double *out, const double *part,
+                      const int *perm, double *x1, double *x2, hipStream_t st);
 void launch_mspmv_rhs(int n, int K, const int *ia, const int *ja, const double *val, const double *e2,
                       const double *e3, double *w, double *r, hipStream_t st);
 void launch_msource(int ns, int K, const int *edge, const int *src, const double *w, double didt, double cdt,
@@ -160,6 +160,7 @@ struct Launch {
     int kind, p0, off, cnt, poff;
     int64_t nwg;
     int aux;  // factor updates: mode; backward X tiles: 1 = no front-order copy of x (lowest level)
+    int nb = 4;  // tile launches: 16-row output blocks per wave (2 or 4)
 };
 
 int prof_category(int kind) {
@@ -272,6 +273,7 @@ struct tf_ctx {
     int *d_gptr = nullptr, *d_gidx = nullptr;
     double *d_mZ = nullptr;
     int mt_units = 4096, mt_chmin = 256;  // tile-launch sizing (units per launch, min inner chunk)
+    int mt_wide = 64;  // levels of at most this many fronts use 64-row tiles, the others 32-row tiles
     int mt_ks_n = 2, mt_ks_t = 2;         // k-steps per stage of the N- and T-form tile kernels
     double *d_U = nullptr, *d_V = nullptr, *d_xw = nullptr, *d_Y = nullptr, *d_T = nullptr, *d_W = nullptr;
     int64_t *d_woff = nullptr;
@@ -618,11 +620,14 @@ void build_plan(tf_ctx *c) {
     // block solve for K right-hand sides: every front through the tile kernels
     // mode 0 own forward, 1 struct forward, 2 backward T, 3 backward X
     struct Tile { int k, t0, lo, hi; };
+    // 64-row tiles on levels of at most mt_wide fronts (few large fronts),
+    // 32-row tiles below
+    auto nb_of = [&](const std::vector<int> &fr) { return (int)fr.size() <= c->mt_wide ? 4 : 2; };
     auto tiles_of = [&](int mode, const std::vector<int> &fr) {
         std::vector<Tile> t;
+        const int TW = 16 * nb_of(fr);  // output rows (columns) per tile
         for (int k : fr) {
             const int nc = s.nc[k], m = m_of(k);
-            constexpr int TW = 16 * TF_MT_NB;  // output rows (columns) per tile
             if (mode == 0)
                 for (int r = 0; r < nc; r += TW) t.push_back({k, r, 0, std::min(nc, r + TW)});
             else if (mode == 1)  // row tiles start on 16-row lines of the column (rows nc + r)
@@ -636,7 +641,7 @@ void build_plan(tf_ctx *c) {
     };
     // one tile launch (+ its reduction): tiles longer than the chunk are split
     // so that a launch has about mt_units units
-    auto add_tiles = [&](std::vector<Launch> &dst, int mode, const std::vector<Tile> &tl) {
+    auto add_tiles = [&](std::vector<Launch> &dst, int mode, const std::vector<Tile> &tl, int nb) {
         if (tl.empty()) return;
         int64_t work = 0;
         for (const Tile &t : tl) work += t.hi - t.lo;
@@ -675,10 +680,10 @@ void build_plan(tf_ctx *c) {
                 }
             t0 = t1;
         }
-        dst.push_back(Launch{L_MT, mode, (int)p.units.size(), (int)tu.size(), 0, (int64_t)(tu.size() + 3) / 4, 0});
+        dst.push_back(Launch{L_MT, mode, (int)p.units.size(), (int)tu.size(), 0, (int64_t)(tu.size() + 3) / 4, 0, nb});
         p.units.insert(p.units.end(), tu.begin(), tu.end());
         if (!ru.empty()) {
-            dst.push_back(Launch{L_MTRED, mode, (int)p.units.size(), (int)ru.size(), 0, (int64_t)ru.size(), 0});
+            dst.push_back(Launch{L_MTRED, mode, (int)p.units.size(), (int)ru.size(), 0, (int64_t)ru.size(), 0, nb});
             p.units.insert(p.units.end(), ru.begin(), ru.end());
         }
         p.part_max = std::max<int64_t>(p.part_max, slots);
@@ -688,18 +693,18 @@ void build_plan(tf_ctx *c) {
         std::vector<int64_t> ur;
         for (int k : all) ur.push_back((s.nc[k] + 4 * TF_RPW - 1) / (4 * TF_RPW));
         if (l == 0) {  // fronts without children: Yin is b itself, read through perm by the tiles
-            add_tiles(p.mfwd, 4, tiles_of(0, all));
+            add_tiles(p.mfwd, 4, tiles_of(0, all), nb_of(all));
         } else {
             add(p.mfwd, L_MPULL, 0, all, ur);
-            add_tiles(p.mfwd, 0, tiles_of(0, all));
+            add_tiles(p.mfwd, 0, tiles_of(0, all), nb_of(all));
         }
-        add_tiles(p.mfwd, 1, tiles_of(1, all));
+        add_tiles(p.mfwd, 1, tiles_of(1, all), nb_of(all));
     }
     for (int l = s.nlevels - 1; l >= 0; l--) {
         std::vector<int> all(s.lev_sn.begin() + s.lev_ptr[l], s.lev_sn.begin() + s.lev_ptr[l + 1]);
-        add_tiles(p.mbwd, 2, tiles_of(2, all));
+        add_tiles(p.mbwd, 2, tiles_of(2, all), nb_of(all));
         const size_t nb = p.mbwd.size();
-        add_tiles(p.mbwd, 3, tiles_of(3, all));
+        add_tiles(p.mbwd, 3, tiles_of(3, all), nb_of(all));
         if (l == 0)  // no lower level gathers these rows: edge order only
             for (size_t i = nb; i < p.mbwd.size(); i++) p.mbwd[i].aux = 1;
     }
@@ -914,19 +919,19 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
             break;
         case L_MT:  // own forward (Y from Yin, or from b), struct forward (Z from Y)
             if (l.p0 == 4)
-                tf::launch_mtile(4, c->mt_ks_n, L, fd, c->d_units + l.off, l.cnt, K, b, nullptr, Dn, nullptr,
+                tf::launch_mtile(4, c->mt_ks_n, l.nb, L, fd, c->d_units + l.off, l.cnt, K, b, nullptr, Dn, nullptr,
                                  c->d_mY, c->d_part, gbytes, c->d_perm, nullptr, nullptr, c->st);
             else
-                tf::launch_mtile(l.p0, c->mt_ks_n, L, fd, c->d_units + l.off, l.cnt, K,
+                tf::launch_mtile(l.p0, c->mt_ks_n, l.nb, L, fd, c->d_units + l.off, l.cnt, K,
                                  l.p0 == 0 ? c->d_mV : c->d_mY, nullptr, Dn, nullptr, l.p0 == 0 ? c->d_mY : c->d_mZ,
                                  c->d_part, 0u, nullptr, nullptr, nullptr, c->st);
             break;
         case L_MTRED:
             if (l.p0 == 4)
-                tf::launch_mtile_red(4, fd, c->d_units + l.off, l.cnt, K, b, nullptr, Dn, c->d_mY, c->d_part,
+                tf::launch_mtile_red(4, l.nb, fd, c->d_units + l.off, l.cnt, K, b, nullptr, Dn, c->d_mY, c->d_part,
                                      c->d_perm, nullptr, nullptr, c->st);
             else
-                tf::launch_mtile_red(l.p0, fd, c->d_units + l.off, l.cnt, K, l.p0 == 0 ? c->d_mV : c->d_mY, nullptr,
+                tf::launch_mtile_red(l.p0, l.nb, fd, c->d_units + l.off, l.cnt, K, l.p0 == 0 ? c->d_mV : c->d_mY, nullptr,
                                      Dn, l.p0 == 0 ? c->d_mY : c->d_mZ, c->d_part, nullptr, nullptr, nullptr, c->st);
             break;
         }
@@ -940,12 +945,12 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
         double *out = bx ? (l.aux == 1 ? nullptr : c->d_mxw) : c->d_mT;
         switch (l.kind) {
         case L_MT:
-            tf::launch_mtile(l.p0, c->mt_ks_t, L, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : c->d_mxw,
+            tf::launch_mtile(l.p0, c->mt_ks_t, l.nb, L, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : c->d_mxw,
                              bx ? c->d_mT : c->d_mY, Dn, c->d_sidx, out, c->d_part, gbytes, c->d_perm,
                              bx ? x : nullptr, bx ? x2 : nullptr, c->st);
             break;
         case L_MTRED:
-            tf::launch_mtile_red(l.p0, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : c->d_mxw,
+            tf::launch_mtile_red(l.p0, l.nb, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : c->d_mxw,
                                  bx ? c->d_mT : c->d_mY, Dn, out, c->d_part, c->d_perm, bx ? x : nullptr,
                                  bx ? x2 : nullptr, c->st);
             break;
@@ -2134,6 +2139,10 @@ int tf_set_tuning(tf_ctx *c, const char *key, int value) {
         } else if (k == "mt_chmin") {
             if (value < 16 || value % 16) return fail(c, TF_ERR_ARG, "mt_chmin must be a positive multiple of 16");
             c->mt_chmin = value;
+            replan = true;
+        } else if (k == "mt_wide") {
+            if (value < 0) return fail(c, TF_ERR_ARG, "mt_wide must be >= 0");
+            c->mt_wide = value;
             replan = true;
         } else if (k == "graphs") {
             if (value != 0 && value != 1) return fail(c, TF_ERR_ARG, "graphs must be 0 or 1");

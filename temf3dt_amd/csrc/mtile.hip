@@ -1,12 +1,14 @@
 // mtile.hip -- wave-level FP64 matrix-core tile products of the K-RHS block
 // solve (K <= 64 sources sharing one LDL^T factor).
 //
-// Each wave owns one work unit: a (16 * TF_MT_NB) x 64 output tile (32 matrix
-// rows or columns x 64 right-hand sides at TF_MT_NB = 2) accumulated over an
-// inner range [k0, k1).  Operands go straight from memory into the MFMA lane
+// Each wave owns one work unit: a (16 * NB) x 64 output tile (NB = 2: 32
+// matrix rows or columns x 64 right-hand sides; NB = 4: 64) accumulated over
+// an inner range [k0, k1).  Operands go straight from memory into the MFMA lane
 // layout -- no LDS, no barriers -- so many waves per CU stream the factor at
-// once (about 120 VGPRs: four waves per SIMD; 64-row tiles took 200 VGPRs,
-// two waves per SIMD, and ran 3 % slower end to end).  The four
+// once.  32-row tiles take about 120 VGPRs (four waves per SIMD) and suit
+// the many small fronts of the lower levels; 64-row tiles (about 200
+// VGPRs, two waves per SIMD) read each right-hand-side row for twice the
+// output and suit the few large fronts of the upper levels.  The four
 // products of a front (X11 = L11^-1 stored in place by the factorisation):
 //   forward   Y_own = Yin + strict(X11) Yin           (N form, triangle)
 //             Z = L21 Y_own                          (N form)
@@ -34,11 +36,12 @@
 // (no pull launch for the lowest level).
 enum { MT_OWN_FWD = 0, MT_STR_FWD = 1, MT_BWD_T = 2, MT_BWD_X = 3, MT_OWN_FWD_B = 4 };
 
-__device__ __forceinline__ void mt_acc_zero(tf_d4 acc[4][TF_MT_NB]) {
+template <int NB>
+__device__ __forceinline__ void mt_acc_zero(tf_d4 acc[4][NB]) {
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int b = 0; b < TF_MT_NB; b++) acc[a][b] = (tf_d4){0.0, 0.0, 0.0, 0.0};
+        for (int b = 0; b < NB; b++) acc[a][b] = (tf_d4){0.0, 0.0, 0.0, 0.0};
 }
 
 // Output-row/column count, matrix offset inside the L block, and whether the
@@ -113,9 +116,9 @@ __device__ __forceinline__ double mt_ld(const double *base, uint32_t off) {
 // fetched before the multiply that precedes their stage's loads.  The
 // stages before and after that run (a T-form stage aligned before k0, a
 // partial last stage) clamp the inner index and zero what lies outside.
-template <bool TFORM, bool GATH, int KS>
+template <bool TFORM, bool GATH, int KS, int NB>
 __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f, int roff, int o0, int k0, int k1,
-                                        const double *Bm, uint32_t bbytes, const int *bidx, int K, tf_d4 acc[4][TF_MT_NB]) {
+                                        const double *Bm, uint32_t bbytes, const int *bidx, int K, tf_d4 acc[4][NB]) {
     constexpr int SW = 4 * KS;  // inner values per stage
     if (k0 >= k1) return;
     const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
@@ -125,9 +128,9 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
     // can leave the block stays there: the T-form output column (one offset
     // per output block b), while an N-form row past the block's last column
     // reads at most 62 values beyond it (the factor storage ends with slack).
-    uint32_t la[TFORM ? TF_MT_NB : 1];
+    uint32_t la[TFORM ? NB : 1];
 #pragma unroll
-    for (int b = 0; b < (TFORM ? TF_MT_NB : 1); b++)
+    for (int b = 0; b < (TFORM ? NB : 1); b++)
         la[b] = TFORM ? (uint32_t)((int64_t)(o0 + 16 * b + l16) * f + roff + KS * lk) * 8u
                       : (uint32_t)(lk * f + roff + o0 + l16) * 8u;
     const uint32_t ba = GATH ? (uint32_t)l16 * 8u : (uint32_t)((TFORM ? KS * lk : lk) * K + l16) * 8u;
@@ -137,26 +140,26 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
     const int ja = min(nst, (k0 - kb + SW - 1) / SW);
     const int jb = max(ja, (k1 - kb) / SW);
     auto inner = [&](int ib, int s) { return TFORM ? (ib + KS * lk + s) : (ib + 4 * s + lk); };
-    auto mult = [&](const double xa[KS][TF_MT_NB], const double yb[KS][4]) {
+    auto mult = [&](const double xa[KS][NB], const double yb[KS][4]) {
 #pragma unroll
         for (int s = 0; s < KS; s++)
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
-                for (int b = 0; b < TF_MT_NB; b++)
+                for (int b = 0; b < NB; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s][b], yb[s][a], acc[a][b], 0, 0, 0);
     };
     // a masked stage, loaded and multiplied on its own
     auto masked_stage = [&](int js) {
         const int ib = kb + js * SW;
-        double xa[KS][TF_MT_NB], yb[KS][4];
+        double xa[KS][NB], yb[KS][4];
 #pragma unroll
         for (int s = 0; s < KS; s++) {
             const int i = inner(ib, s);
             const bool iv = i >= k0 && i < k1;
             const int ic = min(max(i, k0), k1 - 1);
 #pragma unroll
-            for (int b = 0; b < TF_MT_NB; b++) {
+            for (int b = 0; b < NB; b++) {
                 const int ob = o0 + 16 * b + l16;
                 const uint32_t off = TFORM ? (uint32_t)((int64_t)ob * f + roff + ic) * 8u
                                            : (uint32_t)((int64_t)ic * f + roff + ob) * 8u;
@@ -180,20 +183,20 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
 #pragma unroll
             for (int s = 0; s < KS; s++) r[s] = bidx[inner(ib, s)];
         };
-        auto load = [&](int js, const int r[KS], double xa[KS][TF_MT_NB], double yb[KS][4]) {
+        auto load = [&](int js, const int r[KS], double xa[KS][NB], double yb[KS][4]) {
             const int ib = kb + js * SW;
             if (TFORM && KS % 2 == 0) {  // a lane's KS rows of a column are contiguous: 16-byte loads
 #pragma unroll
                 for (int s = 0; s < KS; s += 2)
 #pragma unroll
-                    for (int b = 0; b < TF_MT_NB; b++) mt_bld2(rL, la[b], (uint32_t)(ib + s) * 8u, xa[s][b], xa[s + 1][b]);
+                    for (int b = 0; b < NB; b++) mt_bld2(rL, la[b], (uint32_t)(ib + s) * 8u, xa[s][b], xa[s + 1][b]);
             }
 #pragma unroll
             for (int s = 0; s < KS; s++) {
                 const uint32_t sl = (uint32_t)(TFORM ? (ib + s) : (ib + 4 * s) * f) * 8u;
                 if (!(TFORM && KS % 2 == 0))
 #pragma unroll
-                    for (int b = 0; b < TF_MT_NB; b++)
+                    for (int b = 0; b < NB; b++)
                         xa[s][b] = TFORM ? mt_bld(rL, la[b], sl) : mt_bld(rL, la[0] + 128u * b, sl);
                 if (GATH) {
                     const uint32_t vo = ((uint32_t)r[s] * (uint32_t)K) * 8u + ba;
@@ -206,7 +209,7 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
                 }
             }
         };
-        double xa0[KS][TF_MT_NB], yb0[KS][4], xa1[KS][TF_MT_NB], yb1[KS][4];
+        double xa0[KS][NB], yb0[KS][4], xa1[KS][NB], yb1[KS][4];
         int rA[KS], rB[KS];
         const int last = jb - 1;
         int js = ja;
@@ -239,7 +242,7 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
     for (int js = jb; js < nst; js++) masked_stage(js);
 }
 
-template <int MODE, int KS>
+template <int MODE, int KS, int NB>
 __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDesc *fd, const TileUnit *units,
                                                   int nunits, int K, const double *in0, const double *in1,
                                                   const double *Dn, const int *sidx, double *out, double *part,
@@ -251,8 +254,8 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
     const FrontDesc d = fd[tu.front];
     int nout, roff;
     mt_geom<MODE>(d, nout, roff);
-    tf_d4 acc[4][TF_MT_NB];
-    mt_acc_zero(acc);
+    tf_d4 acc[4][NB];
+    mt_acc_zero<NB>(acc);
     const double *Lf = L + d.loff;
     const uint32_t lbytes = (uint32_t)((int64_t)d.nc * d.ld * 8);
     // B operand rows: own right-hand sides (Yin, Y_own, T) at c0, or the
@@ -262,17 +265,17 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
     // the strict triangle of X11 needs no mask: the factorisation leaves the
     // diagonal and upper part of L11's storage zero
     if (MODE == MT_OWN_FWD || MODE == MT_STR_FWD)
-        mt_wave<false, false, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
+        mt_wave<false, false, KS, NB>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
     else if (MODE == MT_OWN_FWD_B)
-        mt_wave<false, true, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, in0, gbytes, xo.perm + d.c0, K, acc);
+        mt_wave<false, true, KS, NB>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, in0, gbytes, xo.perm + d.c0, K, acc);
     else if (MODE == MT_BWD_T)
-        mt_wave<true, true, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, in0, gbytes, sidx + d.soff, K, acc);
+        mt_wave<true, true, KS, NB>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, in0, gbytes, sidx + d.soff, K, acc);
     else
-        mt_wave<true, false, KS>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
+        mt_wave<true, false, KS, NB>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
     const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
     if (tu.slot < 0) {
 #pragma unroll
-        for (int b = 0; b < TF_MT_NB; b++)
+        for (int b = 0; b < NB; b++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int o = tu.t0 + 16 * b + 4 * r + lk;
@@ -286,7 +289,7 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
     } else {
         double *pp = part + (int64_t)tu.slot * 4096;
 #pragma unroll
-        for (int b = 0; b < TF_MT_NB; b++)
+        for (int b = 0; b < NB; b++)
 #pragma unroll
             for (int r = 0; r < 4; r++)
 #pragma unroll
@@ -300,14 +303,14 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
 template <int MODE>
 __global__ void __launch_bounds__(256) k_mtile_red(const FrontDesc *fd, const TileUnit *units, int K,
                                                    const double *in0, const double *in1, const double *Dn,
-                                                   double *out, const double *part, MtEdgeOut xo) {
+                                                   double *out, const double *part, MtEdgeOut xo, int nb) {
     const TileUnit tu = units[blockIdx.x >> 4];
     const FrontDesc d = fd[tu.front];
     int nout, roff;
     mt_geom<MODE>(d, nout, roff);
     const int e = (blockIdx.x & 15) * 256 + threadIdx.x;
     const int o = tu.t0 + (e >> 6), q = e & 63;
-    if ((e >> 6) >= 16 * TF_MT_NB || o < 0 || o >= nout || q >= K) return;
+    if ((e >> 6) >= 16 * nb || o < 0 || o >= nout || q >= K) return;
     const double *p0 = part + (int64_t)tu.k0 * 4096 + e;
     double s = 0.0;
     for (int c = 0; c < tu.k1; c++) s += p0[(int64_t)c * 4096];
@@ -319,45 +322,42 @@ namespace tf {
 // 4 own forward of a childless front straight from b (through perm)
 // gbytes: size of the gathered vector (backward T form), < 4 GiB
 // perm, x1, x2: backward X also writes its rows to x1 (and x2) in edge order
-void launch_mtile(int mode, int ks, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits, int K,
-                  const double *in0, const double *in1, const double *Dn, const int *sidx, double *out, double *part,
-                  uint32_t gbytes, const int *perm, double *x1, double *x2, hipStream_t st) {
+void launch_mtile(int mode, int ks, int nb, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits,
+                  int K, const double *in0, const double *in1, const double *Dn, const int *sidx, double *out,
+                  double *part, uint32_t gbytes, const int *perm, double *x1, double *x2, hipStream_t st) {
     if (nunits <= 0) return;
     const dim3 g((nunits + 3) / 4), b(256);
     const MtEdgeOut xo{perm, x1, x2};
-#define TF_MT(M, S) \
-    hipLaunchKernelGGL((k_mtile<M, S>), g, b, 0, st, L, fd, units, nunits, K, in0, in1, Dn, sidx, out, part, gbytes, xo)
-    if (ks == 4) {
-        switch (mode) {
-        case MT_OWN_FWD: TF_MT(MT_OWN_FWD, 4); break;
-        case MT_OWN_FWD_B: TF_MT(MT_OWN_FWD_B, 4); break;
-        case MT_STR_FWD: TF_MT(MT_STR_FWD, 4); break;
-        case MT_BWD_T: TF_MT(MT_BWD_T, 4); break;
-        default: TF_MT(MT_BWD_X, 4); break;
-        }
-    } else {
-        switch (mode) {
-        case MT_OWN_FWD: TF_MT(MT_OWN_FWD, 2); break;
-        case MT_OWN_FWD_B: TF_MT(MT_OWN_FWD_B, 2); break;
-        case MT_STR_FWD: TF_MT(MT_STR_FWD, 2); break;
-        case MT_BWD_T: TF_MT(MT_BWD_T, 2); break;
-        default: TF_MT(MT_BWD_X, 2); break;
-        }
+#define TF_MT(M, S, N) \
+    hipLaunchKernelGGL((k_mtile<M, S, N>), g, b, 0, st, L, fd, units, nunits, K, in0, in1, Dn, sidx, out, part, gbytes, xo)
+#define TF_MT_MODES(S, N)                                        \
+    switch (mode) {                                              \
+    case MT_OWN_FWD: TF_MT(MT_OWN_FWD, S, N); break;             \
+    case MT_OWN_FWD_B: TF_MT(MT_OWN_FWD_B, S, N); break;         \
+    case MT_STR_FWD: TF_MT(MT_STR_FWD, S, N); break;             \
+    case MT_BWD_T: TF_MT(MT_BWD_T, S, N); break;                 \
+    default: TF_MT(MT_BWD_X, S, N); break;                       \
     }
+    if (nb == 4) {
+        if (ks == 4) { TF_MT_MODES(4, 4) } else { TF_MT_MODES(2, 4) }
+    } else {
+        if (ks == 4) { TF_MT_MODES(4, 2) } else { TF_MT_MODES(2, 2) }
+    }
+#undef TF_MT_MODES
 #undef TF_MT
 }
-void launch_mtile_red(int mode, const FrontDesc *fd, const TileUnit *units, int nunits, int K, const double *in0,
-                      const double *in1, const double *Dn, double *out, const double *part, const int *perm,
-                      double *x1, double *x2, hipStream_t st) {
+void launch_mtile_red(int mode, int nb, const FrontDesc *fd, const TileUnit *units, int nunits, int K,
+                      const double *in0, const double *in1, const double *Dn, double *out, const double *part,
+                      const int *perm, double *x1, double *x2, hipStream_t st) {
     if (nunits <= 0) return;
     const dim3 g(nunits * 16), b(256);
     const MtEdgeOut xo{perm, x1, x2};
     switch (mode) {
-    case MT_OWN_FWD: hipLaunchKernelGGL(k_mtile_red<MT_OWN_FWD>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo); break;
-    case MT_OWN_FWD_B: hipLaunchKernelGGL(k_mtile_red<MT_OWN_FWD_B>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo); break;
-    case MT_STR_FWD: hipLaunchKernelGGL(k_mtile_red<MT_STR_FWD>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo); break;
-    case MT_BWD_T: hipLaunchKernelGGL(k_mtile_red<MT_BWD_T>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo); break;
-    default: hipLaunchKernelGGL(k_mtile_red<MT_BWD_X>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo); break;
+    case MT_OWN_FWD: hipLaunchKernelGGL(k_mtile_red<MT_OWN_FWD>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
+    case MT_OWN_FWD_B: hipLaunchKernelGGL(k_mtile_red<MT_OWN_FWD_B>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
+    case MT_STR_FWD: hipLaunchKernelGGL(k_mtile_red<MT_STR_FWD>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
+    case MT_BWD_T: hipLaunchKernelGGL(k_mtile_red<MT_BWD_T>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
+    default: hipLaunchKernelGGL(k_mtile_red<MT_BWD_X>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
     }
 }
 }  // namespace tf

@@ -15,7 +15,6 @@
 // vectors: the tile kernels' unmasked stage loads may read up to 63 values
 // beyond the last row of an array (results of those lanes are discarded).
 #define TF_READ_SLACK 4096
-#define TF_MT_NB 2  // 16-row output blocks per wave of the block-solve tile products (tile of 16*TF_MT_NB rows x 64 rhs)
 #define TF_RPW 4  // rows per wave of the row-per-wave kernels (SpMV, pull): 4 * TF_RPW rows per workgroup
 
 // L11 inversion by doubling: leading dimension of the scratch block T (|J| x b)

@@ -137,6 +137,8 @@ def test_overlapped_factor_matches_serial(loops):
     {"mt_units": 64, "mt_chmin": 16},                        # every long tile split into partials
     {"mt_units": 64, "mt_chmin": 16, "mt_ks_n": 4, "mt_ks_t": 2},
     {"mt_units": 100000, "mt_chmin": 4096, "mt_ks_n": 2, "mt_ks_t": 4},  # no splits
+    {"mt_wide": 0},        # 32-row tiles on every level
+    {"mt_wide": 100000, "mt_units": 64, "mt_chmin": 16},  # 64-row tiles on every level, split
 ])
 def test_block_solve_tuning_knobs(gpu_mod_big_batch, knobs):
     """The tile-kernel knobs change only the schedule and the summation
@@ -154,7 +156,7 @@ def test_block_solve_tuning_knobs(gpu_mod_big_batch, knobs):
         X = fe.solve(0, B)
         ms = fe.time_solve(0, 24, 2)
     finally:
-        for k, v in {"mt_units": 4096, "mt_chmin": 256, "mt_ks_n": 2, "mt_ks_t": 2}.items():
+        for k, v in {"mt_units": 4096, "mt_chmin": 256, "mt_ks_n": 2, "mt_ks_t": 2, "mt_wide": 64}.items():
             fe.set_tuning(k, v)
     assert ms > 0
     rel = np.linalg.norm(X - X0) / np.linalg.norm(X0)
@@ -169,7 +171,7 @@ def test_block_solve_tuning_knobs(gpu_mod_big_batch, knobs):
 def test_tuning_rejects_bad_values(gpu_mod_big_batch):
     from temf3dt_amd.fetd import FetdError
     fe, _ = gpu_mod_big_batch
-    for k, v in [("mt_ks_n", 3), ("mt_chmin", 10), ("mt_units", 1), ("graphs", 2), ("overlap", -1),
+    for k, v in [("mt_ks_n", 3), ("mt_chmin", 10), ("mt_units", 1), ("graphs", 2), ("overlap", -1), ("mt_wide", -1),
                  ("no_such_knob", 1)]:
         with pytest.raises(FetdError):
             fe.set_tuning(k, v)
