@@ -1,11 +1,13 @@
 """Autotuner for the block-solve tile kernels.
 
-The K-source block solve (mtile.hip) has four knobs (tf_set_tuning):
+The K-source block solve (mtile.hip) has five knobs (tf_set_tuning):
   mt_ks_n / mt_ks_t  k-steps per register stage of the N- and T-form tile
                      kernels (2: two stages in flight, 4: one wider stage)
   mt_units           work units a tile launch aims for (sets how far long
                      inner ranges are split into partial tiles)
   mt_chmin           smallest inner chunk of a split tile
+  mt_wide            levels of at most this many fronts use 64-row tiles,
+                     the others 32-row tiles
 The best setting depends on the front-size profile of the mesh's elimination
 tree, so it is measured on the model itself: after a factorisation, every
 candidate setting times `reps` device-resident solves (tf_time_solve, HIP
@@ -21,6 +23,7 @@ DEFAULT_SPACE = {
     "mt_ks_t": (2, 4),
     "mt_units": (2048, 4096, 8192),
     "mt_chmin": (256, 512),
+    "mt_wide": (16, 64, 256),
 }
 
 
