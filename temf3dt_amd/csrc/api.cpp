@@ -36,6 +36,8 @@ void launch_slice(int nT, const double *xyz, const int *tet_vex, const int *tet_
                   const double *e, double dt, int zero_b, double *B, double *out, hipStream_t st);
 // factor.hip
 void launch_fill0(double *p, int64_t n, hipStream_t st);
+void launch_transpose_l21(const double *L, double *LT, const FrontDesc *fd, const int *units, int64_t nunits,
+                          hipStream_t st);
 void launch_scatter_p(int64_t nnz, const int64_t *amap, const double *M, const double *K, double cdt, double *L,
                       hipStream_t st);
 void launch_scatter_vals(int64_t nnz, const int64_t *amap, const double *va, double *L, hipStream_t st);
@@ -87,7 +89,8 @@ void launch_scale(int n, const double *s, double *g, hipStream_t st);
 // mtile.hip
 void launch_mtile(int mode, int ks, int nb, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits,
                   int K, const double *in0, const double *in1, const double *Dn, const int *sidx, double *out,
-                  double *part, uint32_t gbytes, const int *perm, double *x1, double *x2, hipStream_t st);
+                  double *part, uint32_t gbytes, const int *perm, double *x1, double *x2, const double *LT,
+                  hipStream_t st);
 void launch_mpull(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg, const double *b,
                   const int *perm, const int *gptr, const int *gidx, const double *Z, double *Yin, int K,
                   hipStream_t st);
@@ -278,6 +281,12 @@ struct tf_ctx {
     double *d_U = nullptr, *d_V = nullptr, *d_xw = nullptr, *d_Y = nullptr, *d_T = nullptr, *d_W = nullptr;
     int64_t *d_woff = nullptr;
     double *d_L[2] = {nullptr, nullptr}, *d_D[2] = {nullptr, nullptr};
+    // row-major copies of every front's L21 per slot (block solves of K > 1
+    // sources read them in the backward product); lt_total doubles each
+    double *d_LT[2] = {nullptr, nullptr};
+    int64_t lt_total = 0;
+    int *d_tunits = nullptr;  // (front, first struct row, first own column) per 64 x 64 copy tile
+    int64_t n_tunits = 0;
     bool slot_live[2] = {false, false};
     int *d_flag = nullptr;
     double *d_tmpb = nullptr, *d_tmpx = nullptr;
@@ -350,7 +359,9 @@ void free_solver(tf_ctx *c) {
     dfree(c->d_fd); dfree(c->d_perm); dfree(c->d_parent); dfree(c->d_chp); dfree(c->d_ch); dfree(c->d_rel);
     dfree(c->d_sidx); dfree(c->d_plist); dfree(c->d_amap); dfree(c->d_ppref); dfree(c->d_units); dfree(c->d_gptr); dfree(c->d_gidx); dfree(c->d_U); dfree(c->d_V);
     dfree(c->d_xw); dfree(c->d_Y); dfree(c->d_T); dfree(c->d_W); dfree(c->d_woff);
-    for (int s = 0; s < 2; s++) { dfree(c->d_L[s]); dfree(c->d_D[s]); c->slot_live[s] = false; }
+    for (int s = 0; s < 2; s++) { dfree(c->d_L[s]); dfree(c->d_D[s]); dfree(c->d_LT[s]); c->slot_live[s] = false; }
+    dfree(c->d_tunits);
+    c->n_tunits = 0;
     dfree(c->d_tmpb); dfree(c->d_tmpx);
     dfree(c->d_mV); dfree(c->d_mY); dfree(c->d_mT); dfree(c->d_mxw); dfree(c->d_mb); dfree(c->d_mx);
     dfree(c->d_mZ); dfree(c->d_part);
@@ -730,9 +741,29 @@ void do_analyze(tf_ctx *c, int leaf) {
         fd[k].ld = (int)tf::front_ld(fd[k].f);
         fd[k].doff = s.c0[k];
         fd[k].c0 = s.c0[k];
+        fd[k].ldt = (s.nc[k] + 15) & ~15;
+        fd[k].toff = 0;
+        fd[k].pad = 0;
         voff += fd[k].f;
     }
     c->vrows = voff;
+    {  // row-major L21 store: offsets and the copy tiles
+        int64_t toff = 0;
+        std::vector<int> tu;
+        for (int k = 0; k < s.nsn; k++) {
+            fd[k].toff = toff;
+            toff += (int64_t)fd[k].m * fd[k].ldt;
+            for (int r0 = 0; r0 < fd[k].m; r0 += 64)
+                for (int c0 = 0; c0 < fd[k].nc; c0 += 64) {
+                    tu.push_back(k);
+                    tu.push_back(r0);
+                    tu.push_back(c0);
+                }
+        }
+        c->lt_total = toff + TF_READ_SLACK;
+        c->n_tunits = (int64_t)tu.size() / 3;
+        c->d_tunits = dupload(tu);
+    }
     build_plan(c);
     c->d_fd = dupload(fd);
     c->fd_host = fd;
@@ -810,9 +841,18 @@ void run_factor(tf_ctx *c, double *L, double *Dn, const FrontDesc *fd) {
     HIPCHK(hipGetLastError());
 }
 
+// the slot's row-major L21 copy, when block solves use one
+void copy_l21(tf_ctx *c, int slot) {
+    if (!c->d_LT[slot]) return;
+    const int pm = prof_mark(c);
+    tf::launch_transpose_l21(c->d_L[slot], c->d_LT[slot], c->d_fd, c->d_tunits, c->n_tunits, c->st);
+    prof_close(c, TF_PROF_FACTOR_OTHER, pm);
+}
+
 void prepare_slot(tf_ctx *c, int slot) {
     if (!c->d_L[slot]) c->d_L[slot] = dalloc<double>(c->sym.ltotal + TF_READ_SLACK);
     if (!c->d_D[slot]) c->d_D[slot] = dalloc<double>(c->sym.n);
+    if (c->nsrc > 1 && !c->d_LT[slot]) c->d_LT[slot] = dalloc<double>(c->lt_total);
     tf::launch_fill0(c->d_L[slot], c->sym.ltotal, c->st);
     tf::launch_fill0(c->d_U, c->sym.utotal, c->st);
     HIPCHK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->st));
@@ -829,6 +869,7 @@ void factor_dt(tf_ctx *c, int slot, double dt) {
     prepare_slot(c, slot);
     tf::launch_scatter_p(c->topo.nnz, c->d_amap, c->d_M, c->d_K, 2.0 * dt, c->d_L[slot], c->st);
     run_factor(c, c->d_L[slot], c->d_D[slot], c->d_fd);
+    copy_l21(c, slot);
     c->slot_live[slot] = true;
     c->slot_dt[slot] = dt;
 }
@@ -901,7 +942,7 @@ void ensure_block(tf_ctx *c, int K) {
 // X = P^-1 B for K <= 64 right-hand sides, [n][K] device arrays in edge order;
 // x2, if given, receives a second copy of X
 void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd, int K, const double *b,
-                 double *x, double *x2 = nullptr) {
+                 double *x, double *x2 = nullptr, const double *LT = nullptr) {
     ensure_block(c, K);
     // the tile kernels address the gathered solution vector with 32-bit byte offsets
     const uint64_t gb = (uint64_t)c->sym.n * K * 8;
@@ -920,11 +961,11 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
         case L_MT:  // own forward (Y from Yin, or from b), struct forward (Z from Y)
             if (l.p0 == 4)
                 tf::launch_mtile(4, c->mt_ks_n, l.nb, L, fd, c->d_units + l.off, l.cnt, K, b, nullptr, Dn, nullptr,
-                                 c->d_mY, c->d_part, gbytes, c->d_perm, nullptr, nullptr, c->st);
+                                 c->d_mY, c->d_part, gbytes, c->d_perm, nullptr, nullptr, nullptr, c->st);
             else
                 tf::launch_mtile(l.p0, c->mt_ks_n, l.nb, L, fd, c->d_units + l.off, l.cnt, K,
                                  l.p0 == 0 ? c->d_mV : c->d_mY, nullptr, Dn, nullptr, l.p0 == 0 ? c->d_mY : c->d_mZ,
-                                 c->d_part, 0u, nullptr, nullptr, nullptr, c->st);
+                                 c->d_part, 0u, nullptr, nullptr, nullptr, nullptr, c->st);
             break;
         case L_MTRED:
             if (l.p0 == 4)
@@ -947,7 +988,7 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
         case L_MT:
             tf::launch_mtile(l.p0, c->mt_ks_t, l.nb, L, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : c->d_mxw,
                              bx ? c->d_mT : c->d_mY, Dn, c->d_sidx, out, c->d_part, gbytes, c->d_perm,
-                             bx ? x : nullptr, bx ? x2 : nullptr, c->st);
+                             bx ? x : nullptr, bx ? x2 : nullptr, LT, c->st);
             break;
         case L_MTRED:
             tf::launch_mtile_red(l.p0, l.nb, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : c->d_mxw,
@@ -1015,7 +1056,7 @@ void step_solve(tf_ctx *c, int slot, const double *b, double *x, double *x2 = nu
         solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, b, x);
         if (x2) dcopy(c, x2, x);
     } else {
-        solve_block(c, c->d_L[slot], c->d_D[slot], c->d_fd, c->nsrc, b, x, x2);
+        solve_block(c, c->d_L[slot], c->d_D[slot], c->d_fd, c->nsrc, b, x, x2, c->d_LT[slot]);
     }
 }
 
@@ -2247,6 +2288,7 @@ int tf_factor_csr(tf_ctx *c, int slot, int n, const int *ia, const int *ja, cons
         HIPCHK(hipMemcpyAsync(dv, va, sizeof(double) * t.nnz, hipMemcpyHostToDevice, c->st));
         tf::launch_scatter_vals(t.nnz, c->d_amap, dv, c->d_L[slot], c->st);
         run_factor(c, c->d_L[slot], c->d_D[slot], c->d_fd);
+        copy_l21(c, slot);
         c->slot_live[slot] = true;
         c->slot_dt[slot] = -1.0;
         HIPCHK(hipStreamSynchronize(c->st));
@@ -2274,7 +2316,7 @@ int tf_solve(tf_ctx *c, int slot, int nrhs, const double *b, double *x) {
                 for (int i = 0; i < n; i++)
                     for (int q = 0; q < K; q++) h[(size_t)i * K + q] = b[(size_t)(q0 + q) * n + i];
                 HIPCHK(hipMemcpyAsync(c->d_mb, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->st));
-                solve_block(c, c->d_L[slot], c->d_D[slot], c->d_fd, K, c->d_mb, c->d_mx);
+                solve_block(c, c->d_L[slot], c->d_D[slot], c->d_fd, K, c->d_mb, c->d_mx, nullptr, c->d_LT[slot]);
                 HIPCHK(hipMemcpyAsync(h.data(), c->d_mx, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
                 HIPCHK(hipStreamSynchronize(c->st));
                 for (int i = 0; i < n; i++)
@@ -2298,7 +2340,7 @@ int tf_time_solve(tf_ctx *c, int slot, int nrhs, int reps, double *ms_per_solve)
         tf::launch_fill_pattern((int64_t)n * nrhs, b, c->st);
         auto one = [&] {
             if (nrhs == 1) solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, b, x);
-            else solve_block(c, c->d_L[slot], c->d_D[slot], c->d_fd, nrhs, b, x);
+            else solve_block(c, c->d_L[slot], c->d_D[slot], c->d_fd, nrhs, b, x, nullptr, c->d_LT[slot]);
         };
         one();  // warm-up
         hipEvent_t e0, e1;

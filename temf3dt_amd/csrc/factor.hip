@@ -452,6 +452,26 @@ __device__ void wave_diag_factor(double *L, const FrontDesc &d, int p0, double *
 
 // ---- kernels ---------------------------------------------------------------
 
+// Row-major copy of every front's L21 (the block solve's backward product
+// then reads it with the output index running along rows): one 64 x 64
+// tile per workgroup through LDS; units[u] = (front, first struct row,
+// first own column).
+__global__ void __launch_bounds__(256) k_transpose_l21(const double *L, double *LT, const FrontDesc *fd,
+                                                       const int *units) {
+    __shared__ double t[64][65];
+    const int *u = units + 3 * (int64_t)blockIdx.x;
+    const FrontDesc d = fd[u[0]];
+    const int r0 = u[1], c0 = u[2];
+    const int x = threadIdx.x & 63, y = threadIdx.x >> 6;
+    const double *src = L + d.loff + (int64_t)c0 * d.ld + d.nc + r0;
+    for (int c = y; c < 64; c += 4)  // column c0 + c, struct rows r0 + x
+        t[c][x] = (c0 + c < d.nc && r0 + x < d.m) ? src[(int64_t)c * d.ld + x] : 0.0;
+    __syncthreads();
+    double *dst = LT + d.toff + (int64_t)r0 * d.ldt + c0;
+    for (int r = y; r < 64; r += 4)  // struct row r0 + r, columns c0 + x
+        if (r0 + r < d.m && c0 + x < d.ldt) dst[(int64_t)r * d.ldt + x] = t[x][r];
+}
+
 __global__ void k_fill0(double *p, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -618,6 +638,10 @@ __global__ void __launch_bounds__(TF_NT) k_inv_merge(double *L, const FrontDesc 
 // ---- host launchers --------------------------------------------------------
 namespace tf {
 
+void launch_transpose_l21(const double *L, double *LT, const FrontDesc *fd, const int *units, int64_t nunits,
+                          hipStream_t st) {
+    if (nunits > 0) hipLaunchKernelGGL(k_transpose_l21, dim3((unsigned)nunits), dim3(256), 0, st, L, LT, fd, units);
+}
 void launch_fill0(double *p, int64_t n, hipStream_t st) {
     if (n <= 0) return;
     int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);

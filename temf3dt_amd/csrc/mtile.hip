@@ -246,7 +246,7 @@ template <int MODE, int KS, int NB>
 __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDesc *fd, const TileUnit *units,
                                                   int nunits, int K, const double *in0, const double *in1,
                                                   const double *Dn, const int *sidx, double *out, double *part,
-                                                  uint32_t gbytes, MtEdgeOut xo) {
+                                                  uint32_t gbytes, MtEdgeOut xo, const double *LT) {
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int u = xcd_block(blockIdx.x, gridDim.x) * 4 + w;
     if (u >= nunits) return;
@@ -268,6 +268,9 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
         mt_wave<false, false, KS, NB>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
     else if (MODE == MT_OWN_FWD_B)
         mt_wave<false, true, KS, NB>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, in0, gbytes, xo.perm + d.c0, K, acc);
+    else if (MODE == MT_BWD_T && LT)  // L21 read from its row-major copy: N form over struct rows
+        mt_wave<false, true, KS, NB>(LT + d.toff, (uint32_t)((int64_t)d.m * d.ldt * 8), d.ldt, 0, tu.t0, tu.k0,
+                                     tu.k1, in0, gbytes, sidx + d.soff, K, acc);
     else if (MODE == MT_BWD_T)
         mt_wave<true, true, KS, NB>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, in0, gbytes, sidx + d.soff, K, acc);
     else
@@ -322,14 +325,17 @@ namespace tf {
 // 4 own forward of a childless front straight from b (through perm)
 // gbytes: size of the gathered vector (backward T form), < 4 GiB
 // perm, x1, x2: backward X also writes its rows to x1 (and x2) in edge order
+// LT: row-major L21 copies for the backward T product (nullptr: read L21 itself)
 void launch_mtile(int mode, int ks, int nb, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits,
                   int K, const double *in0, const double *in1, const double *Dn, const int *sidx, double *out,
-                  double *part, uint32_t gbytes, const int *perm, double *x1, double *x2, hipStream_t st) {
+                  double *part, uint32_t gbytes, const int *perm, double *x1, double *x2, const double *LT,
+                  hipStream_t st) {
     if (nunits <= 0) return;
     const dim3 g((nunits + 3) / 4), b(256);
     const MtEdgeOut xo{perm, x1, x2};
-#define TF_MT(M, S, N) \
-    hipLaunchKernelGGL((k_mtile<M, S, N>), g, b, 0, st, L, fd, units, nunits, K, in0, in1, Dn, sidx, out, part, gbytes, xo)
+#define TF_MT(M, S, N)                                                                                            \
+    hipLaunchKernelGGL((k_mtile<M, S, N>), g, b, 0, st, L, fd, units, nunits, K, in0, in1, Dn, sidx, out, part, \
+                       gbytes, xo, LT)
 #define TF_MT_MODES(S, N)                                        \
     switch (mode) {                                              \
     case MT_OWN_FWD: TF_MT(MT_OWN_FWD, S, N); break;             \

@@ -30,9 +30,12 @@ struct FrontDesc {
     int64_t uoff;
     int64_t soff;  // offset of struct rows / relative indices
     int64_t voff;  // offset of the front's slice in solve work vectors
+    int64_t toff;  // offset of the row-major copy of L21 (m x nc, row stride ldt) in the transposed store
     int f, nc, m, c0;
     int ld;        // column stride of the L block: f rounded up to 16
     int doff;      // offset of the front's pivots in the D array (c0 for one model)
+    int ldt;       // row stride of the L21 copy: nc rounded up to 16
+    int pad;
 };
 
 // Work unit of the block-solve tile kernels: a 64-wide output tile starting
