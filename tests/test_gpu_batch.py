@@ -43,6 +43,26 @@ def test_block_solve_matches_single_rhs(gpu_mod_big_batch, nrhs):
     assert berr < 1e-12
 
 
+def test_block_solve_row_major_l21_matches_single_rhs():
+    """A multi-source context keeps a row-major copy of every front's L21 for
+    the block solve's backward product (N form over struct rows); its block
+    solves agree with the single-RHS path, which reads L21 itself."""
+    from temf3dt_amd import workloads
+    m = workloads.multi_source_small_model()
+    fe = gpu_from_model(m)  # four sources
+    fe.factor(0, 1e-8)
+    rng = np.random.default_rng(11)
+    for nrhs in (4, 64):
+        B = rng.standard_normal((nrhs, fe.nE))
+        print(f"[gpu] row-major L21 block solve nrhs={nrhs}", flush=True)
+        X = fe.solve(0, B)
+        for q in (0, nrhs - 1):
+            x1 = fe.solve(0, B[q])
+            rel = np.linalg.norm(X[q] - x1) / np.linalg.norm(x1)
+            assert rel < 1e-10, (nrhs, q, rel)
+    fe.close()
+
+
 @pytest.fixture(scope="module")
 def gpu_mod_big_batch(src_mod_big, oracle_mod_big):
     from temf3dt_amd import fetd
