@@ -1731,15 +1731,65 @@ void run_adjoint(tf_ctx *c, const double *w, double *grad_host) {
     std::vector<double *> state(nn, nullptr);
     std::vector<double *> accA(nn, nullptr), accB(nn, nullptr);
     state[0] = zero;
-    int last_used = 0;
+    // The sweep's factor slots: the dt of every live node in processing
+    // order (segments from the last; recompute forward, adjoint backward) is
+    // known from the tape, so the next dt to come is factored on the second
+    // stream while the current one is in use, as in tf_run.
+    std::vector<double> dts;
+    for (int g = (int)c->ckpts.size() - 1; g >= 0; g--) {
+        const int first = c->ckpts[g].first, end = (g + 1 < (int)c->ckpts.size()) ? c->ckpts[g + 1].first : nn;
+        for (int k = first; k < end; k++)
+            if (live[k]) dts.push_back(c->tape[k].dt);
+        for (int k = end - 1; k >= first; k--)
+            if (live[k]) dts.push_back(c->tape[k].dt);
+    }
+    size_t pos = 0;  // next entry of dts
+    int last_used = 0, pend = -1;
+    hipEvent_t a0, a2;
+    HIPCHK(hipEventCreate(&a0));
+    HIPCHK(hipEventCreate(&a2));
+    struct StreamDrain {
+        hipStream_t s;
+        ~StreamDrain() { (void)hipStreamSynchronize(s); }
+    } drain{c->st2};
     auto slot_for = [&](double dt) {
-        for (int s = 0; s < 2; s++)
-            if (c->slot_live[s] && c->slot_dt[s] == dt) { last_used = s; return s; }
-        const int s = 1 - last_used;
-        factor_dt(c, s, dt);
-        check_flag(c);
-        c->timers[7] += 1.0;
+        if (pos < dts.size() && dts[pos] == dt) pos++;
+        int s = -1;
+        for (int q = 0; q < 2; q++)
+            if (c->slot_live[q] && c->slot_dt[q] == dt) s = q;
+        if (s >= 0 && s == pend) {  // factored ahead: wait for it
+            HIPCHK(hipStreamWaitEvent(c->st, a2, 0));
+            check_flag(c);
+            pend = -1;
+        }
+        if (s < 0) {
+            if (pend >= 0) {  // not the dt it was factored for
+                HIPCHK(hipStreamSynchronize(c->st2));
+                check_flag(c);
+                pend = -1;
+            }
+            s = 1 - last_used;
+            factor_dt(c, s, dt);
+            check_flag(c);
+            c->timers[7] += 1.0;
+        }
         last_used = s;
+        // the next different dt, into the other slot, when nothing before it needs that slot
+        if (c->overlap_factor && pend < 0) {
+            size_t j = pos;
+            while (j < dts.size() && dts[j] == dt) j++;
+            const int o = 1 - s;
+            if (j < dts.size() && !(c->slot_live[o] && c->slot_dt[o] == dts[j])) {
+                HIPCHK(hipEventRecord(a0, c->st));
+                HIPCHK(hipStreamWaitEvent(c->st2, a0, 0));
+                std::swap(c->st, c->st2);
+                factor_dt(c, o, dts[j]);
+                HIPCHK(hipEventRecord(a2, c->st));
+                std::swap(c->st, c->st2);
+                c->timers[7] += 1.0;
+                pend = o;
+            }
+        }
         return s;
     };
     auto S = [&](int k) -> double * {
@@ -1800,7 +1850,11 @@ void run_adjoint(tf_ctx *c, const double *w, double *grad_host) {
     tf::launch_scale(T, c->d_sigma, grad, c->st);
     HIPCHK(hipMemcpyAsync(grad_host, grad, sizeof(double) * T, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipStreamSynchronize(c->st2));
+    if (pend >= 0) check_flag(c);  // factored ahead for nothing (the sweep ended)
     HIPCHK(hipGetLastError());
+    (void)hipEventDestroy(a0);
+    (void)hipEventDestroy(a2);
     for (double *p : owned) dfree(p);
     dfree(grad);
     dfree(d_w);
