@@ -113,8 +113,9 @@ int tf_analyze(tf_ctx *ctx, int leaf_size);
  *               captured hipGraph of right-hand side, solve and history copy;
  *               0: every step is launched kernel by kernel.  Bit-identical.
  *   "overlap"   1 (default): tf_run factors the doubling check's P(2 dt) on a
- *               second stream while the steps before the check run; 0: at
- *               the check.  Bit-identical.
+ *               second stream while the steps before the check run, and
+ *               tf_adjoint its sweep's next dt while the current one is in
+ *               use; 0: at the point of use.  Bit-identical.
  * Results do not depend on the knobs beyond floating-point summation order. */
 int tf_set_tuning(tf_ctx *ctx, const char *key, int value);
 /* Kernel timing with HIP events on the context's stream (no reference
