@@ -89,6 +89,21 @@ def test_adjoint_independent_of_checkpoint_interval(adj_setup):
     assert np.abs(gall - g50).max() <= 1e-10 * scale
 
 
+def test_adjoint_factoring_ahead_is_bit_identical(adj_setup):
+    """The sweep's next dt factored on the second stream while the current
+    one is in use gives the gradient of factoring at the point of use, bit
+    for bit (the same factors, the same order of operations)."""
+    s = adj_setup
+    fe = s["fe"]
+    _, g1 = _adjoint(s, 7)
+    fe.set_tuning("overlap", 0)
+    try:
+        _, g0 = _adjoint(s, 7)
+    finally:
+        fe.set_tuning("overlap", 1)
+    np.testing.assert_array_equal(g1, g0)
+
+
 def test_adjoint_matches_oracle_finite_differences(adj_setup, tmp_path):
     """Central differences through the oracle (the reference algorithm with
     MKL PARDISO on the CPU) on the same replayed schedule."""
