@@ -116,6 +116,11 @@ int tf_analyze(tf_ctx *ctx, int leaf_size);
  *               second stream while the steps before the check run, and
  *               tf_adjoint its sweep's next dt while the current one is in
  *               use; 0: at the point of use.  Bit-identical.
+ *   "indefinite" 0 (default): a factorisation with a negative pivot fails
+ *               with TF_ERR_NUMERIC (P is SPD in exact arithmetic; the
+ *               reference's SPD factorisation would fail there and be redone
+ *               as symmetric indefinite, src/FETD_UTILITY.f90:919-935);
+ *               1: the unpivoted LDL^T is accepted as it is.
  * Results do not depend on the knobs beyond floating-point summation order. */
 int tf_set_tuning(tf_ctx *ctx, const char *key, int value);
 /* Kernel timing with HIP events on the context's stream (no reference
@@ -135,6 +140,14 @@ int tf_set_profiling(tf_ctx *ctx, int on);
 int tf_get_profile(tf_ctx *ctx, int category, int64_t *launches, double *ms);
 /* Factor statistics: nnz(L) incl. diagonal, dense-front flops, supernodes. */
 int tf_factor_stats(const tf_ctx *ctx, int64_t *nnzL, double *flops, int *nsuper, int *nlevels);
+/* Pivots of the last factorisation (the counterpart of PARDISO's inertia
+ * report, iparm(22)/iparm(23), and of the mtype 2 failure test of
+ * pardiso_direct_1, src/FETD_UTILITY.f90:919-935): the number of negative
+ * pivots of D and the smallest and largest |d|.  Every factorisation scans
+ * its pivots; tf_factor, tf_factor_csr, tf_run, tf_run_models and tf_adjoint
+ * return TF_ERR_NUMERIC on a zero or non-finite pivot, and on a negative one
+ * unless tuning key "indefinite" is 1. */
+int tf_factor_pivots(const tf_ctx *ctx, int64_t *n_negative, double *min_abs, double *max_abs);
 
 /* mkl_add_3 + pardiso_direct_1: P = 3*Msigma + 2*dt*K formed on the GPU and
  * factored as LDL^T into slot 0 or 1. */

@@ -1189,6 +1189,32 @@ static void slice_output(fo_model *m, int step, double t, double dt, const doubl
         }
 }
 
+/* The .field2 gate rule of solution_output (src/FETD_FWD.f90:1624-1641):
+ * scanning the gates in order, the first one not yet filled whose time has
+ * come (t >= gate) is filled by interp_lin (src/FETD_UTILITY.f90:21-31)
+ * between the previous output row (tp, fp) and this one (t, f), marked and
+ * returned (its values in out[nq]); -1 when none is left for this row.  On
+ * the first output row (r == 0) the reference copies the row and leaves the
+ * gate loop without writing or marking anything (`exit`), so nothing is
+ * returned there. */
+static int gate_next(int ngate, const double *gates, int *ok, int r, double tp, const double *fp, double t,
+                     const double *f, int nq, double *out) {
+    for (int g = 0; g < ngate; g++) {
+        if (t >= gates[g] && !ok[g]) {
+            if (r == 0) return -1;
+            for (int q = 0; q < nq; q++) {
+                double x1 = tp, y1 = fp[q], x2 = t, y2 = f[q];
+                double a = (y1 - y2) / (x1 - x2);
+                double b = y1 - a * x1;
+                out[q] = a * gates[g] + b;
+            }
+            ok[g] = 1;
+            return g;
+        }
+    }
+    return -1;
+}
+
 /* solution_output (src/FETD_FWD.f90:1513-1657), receiver and file parts */
 static void solution_output(outctx *o, int step, double t, double dt, const double *e) {
     fo_model *m = o->m;
@@ -1225,24 +1251,34 @@ static void solution_output(outctx *o, int step, double t, double dt, const doub
     m->row_dt[r] = dt;
     if (o->f1) write_row(o->f1, m, step, t, f);
     if (m->nslice > 0) slice_output(m, step, t, dt, e);
-    for (int g = 0; g < m->n_gate; g++) {
-        if (t >= m->gates[g] && !o->gate_ok[g]) {
-            if (r == 0) break;
-            double ch[9 * 4096];
-            double *chp = (m->nR * 9 <= 9 * 4096) ? ch : malloc(sizeof(double) * m->nR * 9);
-            const double *fp = &m->row_f[(size_t)(r - 1) * m->nR * 9];
-            for (int q = 0; q < m->nR * 9; q++) {
-                double x1 = m->row_t[r - 1], y1 = fp[q], x2 = t, y2 = f[q];
-                double a = (y1 - y2) / (x1 - x2);
-                double b = y1 - a * x1;
-                chp[q] = a * m->gates[g] + b;
-            }
+    {
+        double ch[9 * 4096];
+        double *chp = (m->nR * 9 <= 9 * 4096) ? ch : malloc(sizeof(double) * m->nR * 9);
+        const double *fp = r > 0 ? &m->row_f[(size_t)(r - 1) * m->nR * 9] : NULL;
+        const double tp = r > 0 ? m->row_t[r - 1] : 0.0;
+        int g;
+        while ((g = gate_next(m->n_gate, m->gates, o->gate_ok, r, tp, fp, t, f, m->nR * 9, chp)) >= 0)
             if (o->f2) write_row(o->f2, m, g + 1, m->gates[g], chp);
-            if (chp != ch) free(chp);
-            o->gate_ok[g] = 1;
-        }
+        if (chp != ch) free(chp);
     }
     m->timers[4] += now_s() - t0;
+}
+
+void fo_gate_rows(int nrows, int nq, const double *t, const double *f, int ngate, const double *gates,
+                  double *out, int *row_of_gate) {
+    int *ok = xcalloc((size_t)ngate + 1, sizeof(int));
+    for (int g = 0; g < ngate; g++) row_of_gate[g] = -1;
+    for (int r = 0; r < nrows; r++) {
+        int g;
+        double *buf = xcalloc((size_t)nq, sizeof(double));
+        while ((g = gate_next(ngate, gates, ok, r, r > 0 ? t[r - 1] : 0.0, r > 0 ? f + (size_t)(r - 1) * nq : NULL,
+                              t[r], f + (size_t)r * nq, nq, buf)) >= 0) {
+            memcpy(out + (size_t)g * nq, buf, sizeof(double) * nq);
+            row_of_gate[g] = r;
+        }
+        free(buf);
+    }
+    free(ok);
 }
 
 static void sys_right(const fo_model *m, int step, double t, double *r, int negate) {

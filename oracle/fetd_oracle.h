@@ -82,6 +82,13 @@ void fo_get_timers(const fo_model *m, double *t6);
 int fo_num_factorizations(const fo_model *m);
 long long fo_factor_nnz(const fo_model *m);
 
+/* The .field2 gate rule (src/FETD_FWD.f90:1624-1641) applied to given output
+ * rows t[nrows], f[nrows][nq]: out[ngate][nq] receives each gate's
+ * interpolated values and row_of_gate[ngate] the row that filled it (-1:
+ * never filled). */
+void fo_gate_rows(int nrows, int nq, const double *t, const double *f, int ngate, const double *gates,
+                  double *out, int *row_of_gate);
+
 /* ---- pointwise kernels (for known-answer tests) ---- */
 /* cofactors of one tet: x[4],y[4],z[4] -> a,b,c,d[4], vol (signed) */
 void fo_tet_cofactors(const double *x, const double *y, const double *z,

@@ -52,6 +52,8 @@ def lib():
         L.fo_set_threads.argtypes = [_I]
         L.fo_pardiso_solve_once.argtypes = [_I, _P, _P, _P, _P, _P, _I]
         L.fo_get_tensors.argtypes = [_P, _P, _P]
+        L.fo_gate_rows.argtypes = [_I, _I, _P, _P, _I, _P, _P, _P]
+        L.fo_gate_rows.restype = None
         _lib = L
     return _lib
 
@@ -246,6 +248,18 @@ def aniso_tensor(values, angles_deg, scale=1.0):
     A = np.empty(9)
     lib().fo_aniso_tensor(_p(v), _p(a), _D(scale), _p(A))
     return A.reshape(3, 3)
+
+
+def gate_rows(t, f, gates):
+    """The .field2 gate rule on given rows: t[nrows], f[nrows][nq] ->
+    (values [ngate][nq], filling row per gate or -1)."""
+    t = np.ascontiguousarray(t, np.float64)
+    f = np.ascontiguousarray(f, np.float64).reshape(len(t), -1)
+    g = np.ascontiguousarray(gates, np.float64)
+    out = np.full((len(g), f.shape[1]), np.nan)
+    row = np.empty(len(g), np.int32)
+    lib().fo_gate_rows(len(t), f.shape[1], _p(t), _p(f), len(g), _p(g), _p(out), _p(row))
+    return out, row
 
 
 def source_didt(source_type, pulse_type, t, t0, cur):

@@ -36,6 +36,8 @@ void launch_slice(int nT, const double *xyz, const int *tet_vex, const int *tet_
                   const double *e, double dt, int zero_b, double *B, double *out, hipStream_t st);
 // factor.hip
 void launch_fill0(double *p, int64_t n, hipStream_t st);
+void launch_flags_init(FactorFlags *f, hipStream_t st);
+void launch_pivot_scan(const double *Dn, const FrontDesc *fd, int nsn, FactorFlags *f, hipStream_t st);
 void launch_transpose_l21(const double *L, double *LT, const FrontDesc *fd, const int *units, int64_t nunits,
                           hipStream_t st);
 void launch_scatter_p(int64_t nnz, const int64_t *amap, const double *M, const double *K, double cdt, double *L,
@@ -124,6 +126,10 @@ void launch_gather_vals(int64_t n, const int64_t *map, const double *src, double
 namespace {
 
 struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+// a factorisation whose pivots the caller did not accept (TF_ERR_NUMERIC)
+struct NumericError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
 
@@ -288,7 +294,12 @@ struct tf_ctx {
     int *d_tunits = nullptr;  // (front, first struct row, first own column) per 64 x 64 copy tile
     int64_t n_tunits = 0;
     bool slot_live[2] = {false, false};
-    int *d_flag = nullptr;
+    FactorFlags *d_flag = nullptr;
+    // pivots of the last factorisation: negative count, smallest and largest
+    // magnitude; allow_indefinite accepts negative pivots (tuning key)
+    long long last_nneg = 0;
+    double last_min_abs = 0.0, last_max_abs = 0.0;
+    bool allow_indefinite = false;
     double *d_tmpb = nullptr, *d_tmpx = nullptr;
     // stepping work vectors
     std::vector<double *> vecs;
@@ -461,6 +472,8 @@ int guarded(tf_ctx *c, F &&f) {
         return f();
     } catch (const HipError &e) {
         return fail(c, TF_ERR_HIP, e.what());
+    } catch (const NumericError &e) {
+        return fail(c, TF_ERR_NUMERIC, e.what());
     } catch (const std::exception &e) {
         return fail(c, TF_ERR_INPUT, e.what());
     }
@@ -796,7 +809,7 @@ void do_analyze(tf_ctx *c, int leaf) {
     c->d_part = dalloc<double>((size_t)std::max<int64_t>(c->plan.part_max, 1) * 4096);
     c->d_tmpb = dalloc<double>(s.n);
     c->d_tmpx = dalloc<double>(s.n);
-    if (!c->d_flag) c->d_flag = dalloc<int>(1);
+    if (!c->d_flag) c->d_flag = dalloc<FactorFlags>(1);
     c->analyzed = true;
     c->timers[6] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -815,10 +828,10 @@ void run_factor(tf_ctx *c, double *L, double *Dn, const FrontDesc *fd) {
                                   c->st);
             break;
         case L_FFUSED:
-            tf::launch_factor_fused(L, c->d_U, fd, lst + l.off, l.cnt, Dn, c->d_flag, c->st);
+            tf::launch_factor_fused(L, c->d_U, fd, lst + l.off, l.cnt, Dn, &c->d_flag->bad, c->st);
             break;
         case L_FDIAG:
-            tf::launch_factor_diag(L, fd, lst + l.off, l.cnt, l.p0, Dn, c->d_flag, c->d_W, c->d_woff, c->st);
+            tf::launch_factor_diag(L, fd, lst + l.off, l.cnt, l.p0, Dn, &c->d_flag->bad, c->d_W, c->d_woff, c->st);
             break;
         case L_FTRSM:
             tf::launch_factor_trsm(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, Dn, c->d_W, c->d_woff, c->st);
@@ -838,6 +851,7 @@ void run_factor(tf_ctx *c, double *L, double *Dn, const FrontDesc *fd) {
         }
         prof_close(c, prof_category(l.kind), pm);
     }
+    tf::launch_pivot_scan(Dn, fd, c->sym.nsn, c->d_flag, c->st);
     HIPCHK(hipGetLastError());
 }
 
@@ -855,14 +869,28 @@ void prepare_slot(tf_ctx *c, int slot) {
     if (c->nsrc > 1 && !c->d_LT[slot]) c->d_LT[slot] = dalloc<double>(c->lt_total);
     tf::launch_fill0(c->d_L[slot], c->sym.ltotal, c->st);
     tf::launch_fill0(c->d_U, c->sym.utotal, c->st);
-    HIPCHK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->st));
+    tf::launch_flags_init(c->d_flag, c->st);
 }
 
+// The pivot report of the factorisation just finished.  P = 3 Msigma +
+// 2 dt K is symmetric positive definite in exact arithmetic; the reference
+// factors it as SPD (PARDISO mtype 2) and, when that fails, again as
+// symmetric indefinite with pivoting (mtype -2, src/FETD_UTILITY.f90:919-935).
+// This factorisation has no pivoting: a negative pivot is reported as
+// TF_ERR_NUMERIC unless the caller accepts indefinite factors (tuning key
+// "indefinite"); a zero or non-finite pivot always is.
 void check_flag(tf_ctx *c) {
-    int flag = 0;
-    HIPCHK(hipMemcpyAsync(&flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    FactorFlags f;
+    HIPCHK(hipMemcpyAsync(&f, c->d_flag, sizeof(f), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
-    if (flag) throw std::runtime_error("zero or non-finite pivot in LDL^T factorisation");
+    c->last_nneg = (long long)f.nneg;
+    c->last_min_abs = __builtin_bit_cast(double, f.min_abs);
+    c->last_max_abs = __builtin_bit_cast(double, f.max_abs);
+    if (f.bad) throw NumericError("zero or non-finite pivot in LDL^T factorisation");
+    if (f.nneg && !c->allow_indefinite)
+        throw NumericError("P is not positive definite: " + std::to_string(f.nneg) +
+                           " negative pivots in LDL^T (the reference would refactor it as symmetric indefinite, "
+                           "PARDISO mtype -2; set tuning key \"indefinite\" to 1 to accept the unpivoted factor)");
 }
 
 void factor_dt(tf_ctx *c, int slot, double dt) {
@@ -1180,6 +1208,10 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
     const int dbl = c->sp_dblesize;
     if (c->sp_double < dbl + 2) throw std::runtime_error("sp_double must be at least sp_dblesize+2");
     int step = 0, not_changed = 0, idx = 0;
+    // the reference's e1 (handed to a step-off second pass) is the last
+    // regular or kept solution: after a check that changed dt it is the
+    // regular step before the check (now e3 here), otherwise e2
+    bool last_changed = false;
     double time_cur = 0.0, step_cur = c->pulse_width / c->sp_division;
     const double time_limit = c->time_max;
     const bool negate = (inout == 1), outp = (inout != 1);
@@ -1323,6 +1355,7 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
             c->chk_step.push_back(step);
             c->chk_err.push_back(error);
             c->chk_keep.push_back(keep ? 1 : 0);
+            last_changed = !keep;
             if (keep) {
                 dcopy(c, v.e1, v.erec);
                 if (outp) output_row(c, step, time_cur, step_cur, v.e1);
@@ -1377,6 +1410,7 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
             ring_id[idx % ring] = k;
         }
         idx++;
+        last_changed = false;
         // e3 <- e2 <- e1 by rotation; e1 is overwritten next step
         double *t3 = v.e3;
         v.e3 = v.e2;
@@ -1386,7 +1420,8 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
     HIPCHK(hipStreamSynchronize(c->st));
     HIPCHK(hipGetLastError());
     if (tape) c->tape_valid = ((int)c->row_node.size() == c->nrows);
-    if (inout == 1 && res) dcopy(c, res, v.e1 == v.e2 ? v.e1 : v.e2);
+    // adap_euler's res_e = e1 (src/FETD_FWD.f90:1459)
+    if (inout == 1 && res) dcopy(c, res, last_changed ? v.e3 : v.e2);
     if (pend >= 0) {  // not reached by construction; drained all the same
         HIPCHK(hipStreamSynchronize(c->st2));
         check_flag(c);
@@ -1483,7 +1518,7 @@ void factor_models(tf_ctx *c, const std::vector<double> &dt, const std::vector<i
         tf::launch_fill0(c->d_Lmm + (int64_t)target[m] * c->sym.ltotal + c->m_llo[m], c->m_lhi[m] - c->m_llo[m],
                          c->st);
     tf::launch_fill0(c->d_U, c->sym.utotal, c->st);
-    HIPCHK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->st));
+    tf::launch_flags_init(c->d_flag, c->st);
     tf::launch_scatter_p_models(c->topo.nnz, c->d_amap, c->d_qmodel, c->d_M, c->d_K, c->d_mcdt, c->d_slot_off,
                                 c->d_Lmm, c->st);
     run_factor(c, c->d_Lmm, c->d_Dmm, fd);
@@ -2245,6 +2280,9 @@ int tf_set_tuning(tf_ctx *c, const char *key, int value) {
         } else if (k == "overlap") {
             if (value != 0 && value != 1) return fail(c, TF_ERR_ARG, "overlap must be 0 or 1");
             c->overlap_factor = value != 0;
+        } else if (k == "indefinite") {
+            if (value != 0 && value != 1) return fail(c, TF_ERR_ARG, "indefinite must be 0 or 1");
+            c->allow_indefinite = value != 0;
         } else if (k == "mt_ks_n" || k == "mt_ks_t") {
             if (value != 2 && value != 4) return fail(c, TF_ERR_ARG, "k-steps per stage must be 2 or 4");
             (k == "mt_ks_n" ? c->mt_ks_n : c->mt_ks_t) = value;
@@ -2312,6 +2350,14 @@ int tf_factor_stats(const tf_ctx *c, int64_t *nnzL, double *flops, int *nsuper, 
     if (flops) *flops = c->sym.flops;
     if (nsuper) *nsuper = c->sym.nsn;
     if (nlevels) *nlevels = c->sym.nlevels;
+    return TF_OK;
+}
+
+int tf_factor_pivots(const tf_ctx *c, int64_t *n_negative, double *min_abs, double *max_abs) {
+    if (!c || !c->analyzed) return TF_ERR_STATE;
+    if (n_negative) *n_negative = c->last_nneg;
+    if (min_abs) *min_abs = c->last_min_abs;
+    if (max_abs) *max_abs = c->last_max_abs;
     return TF_OK;
 }
 

@@ -478,6 +478,53 @@ __global__ void k_fill0(double *p, int64_t n) {
     for (; i < n; i += stride) p[i] = 0.0;
 }
 
+__global__ void k_flags_init(FactorFlags *f) {
+    if (threadIdx.x == 0) {
+        f->bad = 0;
+        f->pad = 0;
+        f->nneg = 0ull;
+        f->min_abs = 0x7ff0000000000000ull;  // +inf
+        f->max_abs = 0ull;
+    }
+}
+
+// Pivot scan: one workgroup per front over its nc pivots D[doff .. doff+nc).
+// P is symmetric positive definite in exact arithmetic (PARDISO mtype 2);
+// a negative pivot is where the reference's SPD factorisation would fail and
+// fall back to the indefinite one (src/FETD_UTILITY.f90:919-935).
+__global__ void __launch_bounds__(256) k_pivot_scan(const double *Dn, const FrontDesc *fd, int nsn,
+                                                    FactorFlags *f) {
+    __shared__ unsigned long long s_neg[256], s_min[256], s_max[256];
+    unsigned long long neg = 0ull, mn = 0x7ff0000000000000ull, mx = 0ull;
+    for (int k = blockIdx.x; k < nsn; k += gridDim.x) {
+        const FrontDesc d = fd[k];
+        for (int j = threadIdx.x; j < d.nc; j += blockDim.x) {
+            const double v = Dn[d.doff + j];
+            if (v < 0.0) neg++;
+            const unsigned long long a = (unsigned long long)__double_as_longlong(fabs(v));
+            mn = a < mn ? a : mn;
+            mx = a > mx ? a : mx;
+        }
+    }
+    s_neg[threadIdx.x] = neg;
+    s_min[threadIdx.x] = mn;
+    s_max[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s_neg[threadIdx.x] += s_neg[threadIdx.x + w];
+            s_min[threadIdx.x] = min(s_min[threadIdx.x], s_min[threadIdx.x + w]);
+            s_max[threadIdx.x] = max(s_max[threadIdx.x], s_max[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (s_neg[0]) atomicAdd(&f->nneg, s_neg[0]);
+        atomicMin(&f->min_abs, s_min[0]);
+        atomicMax(&f->max_abs, s_max[0]);
+    }
+}
+
 // P = 3*M + 2*dt*K scattered into the L storage (src/FETD_FWD.f90:1288-1289).
 __global__ void k_scatter_p(int64_t nnz, const int64_t *amap, const double *M, const double *K,
                             double cdt, double *L) {
@@ -646,6 +693,14 @@ void launch_fill0(double *p, int64_t n, hipStream_t st) {
     if (n <= 0) return;
     int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
     hipLaunchKernelGGL(k_fill0, dim3((unsigned)blocks), dim3(256), 0, st, p, n);
+}
+void launch_flags_init(FactorFlags *f, hipStream_t st) {
+    hipLaunchKernelGGL(k_flags_init, dim3(1), dim3(64), 0, st, f);
+}
+void launch_pivot_scan(const double *Dn, const FrontDesc *fd, int nsn, FactorFlags *f, hipStream_t st) {
+    if (nsn <= 0) return;
+    const int blocks = std::min(nsn, 2048);
+    hipLaunchKernelGGL(k_pivot_scan, dim3(blocks), dim3(256), 0, st, Dn, fd, nsn, f);
 }
 void launch_scatter_p(int64_t nnz, const int64_t *amap, const double *M, const double *K, double cdt,
                       double *L, hipStream_t st) {

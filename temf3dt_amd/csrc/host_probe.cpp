@@ -22,6 +22,10 @@ extern "C" {
 
 const char *tfh_last_error(void) { return g.err.c_str(); }
 
+// 1 if a front of nc columns and m struct rows fits the kernels' 32-bit
+// buffer addressing (the check analyze applies to every front)
+int tfh_front_fits(long long nc, long long m) { return tf::front_fits_32(nc, m) ? 1 : 0; }
+
 int tfh_set_mesh(int nN, const double *xyz, int nT, const int *tet_vex, int *nE, long long *nnz) {
     try {
         tf::build_topology(g.t, nN, xyz, nT, tet_vex);

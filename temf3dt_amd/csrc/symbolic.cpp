@@ -518,6 +518,10 @@ void analyze(Symbolic &s, const Topology &t, int leaf_size) {
             double c = (double)(f - j - 1);
             s.flops += c * c + 2.0 * c;
         }
+        if (!front_fits_32(nc, m))
+            throw std::runtime_error("front of " + std::to_string(nc) + " columns and " + std::to_string(m) +
+                                     " struct rows exceeds the kernels' 4 GiB per-front buffer limit; "
+                                     "use a smaller nested-dissection leaf or a finer partition");
         s.max_f = std::max<int>(s.max_f, (int)f);
         s.max_nc = std::max<int>(s.max_nc, (int)nc);
         int lv = 0;

@@ -38,6 +38,19 @@ struct FrontDesc {
     int pad;
 };
 
+// Pivot report of one factorisation.  The factor kernels set `bad` for a
+// zero or non-finite pivot (bad must stay the first member: they take it as
+// an int *); the pivot scan after the factorisation counts the negative
+// pivots and records the smallest and largest pivot magnitudes as bit
+// patterns (non-negative doubles order like their bits).
+struct FactorFlags {
+    int bad;
+    int pad;
+    unsigned long long nneg;
+    unsigned long long min_abs;
+    unsigned long long max_abs;
+};
+
 // Work unit of the block-solve tile kernels: a 64-wide output tile starting
 // at t0 of front `front`, accumulated over the inner range [k0, k1); slot >= 0
 // names its partial tile when the tile is split over several units.  A

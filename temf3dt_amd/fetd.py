@@ -38,6 +38,7 @@ _SIGS = {
     "tf_get_receiver_vectors": [_P, _P, _P],
     "tf_analyze": [_P, _I],
     "tf_factor_stats": [_P, _P, _P, _P, _P],
+    "tf_factor_pivots": [_P, _P, _P, _P],
     "tf_factor": [_P, _I, _D],
     "tf_factor_csr": [_P, _I, _I, _P, _P, _P],
     "tf_solve": [_P, _I, _I, _P, _P],
@@ -88,7 +89,15 @@ def _ptr(a):
 
 
 class FetdError(RuntimeError):
-    pass
+    """A failed tf_* call; `code` is its status (TF_ERR_*)."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
+
+
+# status codes (include/temfetd.h)
+TF_ERR_ARG, TF_ERR_STATE, TF_ERR_HIP, TF_ERR_NUMERIC, TF_ERR_INPUT = -1, -2, -3, -4, -5
 
 
 class Fetd:
@@ -118,7 +127,7 @@ class Fetd:
     def _chk(self, rc, what):
         if rc != 0:
             msg = self.lib.tf_last_error(self.ctx).decode(errors="replace")
-            raise FetdError(f"{what} failed ({rc}): {msg}")
+            raise FetdError(f"{what} failed ({rc}): {msg}", rc)
 
     # -- model_mesh -------------------------------------------------------
     def set_mesh(self, xyz, tet_vex):
@@ -244,6 +253,14 @@ class Fetd:
         self._chk(self.lib.tf_factor_stats(self.ctx, ctypes.byref(nnzL), ctypes.byref(fl), ctypes.byref(ns),
                                            ctypes.byref(nl)), "tf_factor_stats")
         return dict(nnzL=nnzL.value, flops=fl.value, nsuper=ns.value, nlevels=nl.value)
+
+    def factor_pivots(self):
+        """Pivots of the last factorisation: negative count, min and max |d|."""
+        nn = ctypes.c_int64()
+        lo, hi = _D(), _D()
+        self._chk(self.lib.tf_factor_pivots(self.ctx, ctypes.byref(nn), ctypes.byref(lo), ctypes.byref(hi)),
+                  "tf_factor_pivots")
+        return dict(negative=nn.value, min_abs=lo.value, max_abs=hi.value)
 
     def factor(self, slot, dt):
         self._chk(self.lib.tf_factor(self.ctx, slot, dt), "tf_factor")

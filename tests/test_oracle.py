@@ -252,3 +252,151 @@ def test_anisotropic_cntl_parse(tmp_path):
     o.close()
     assert np.array_equal(s9, os9)
     assert np.array_equal(n9, on9)
+
+
+def test_impulse_waveform_is_derivative_of_its_gaussian():
+    """Pulse type 2 (deltafunc(1), src/FETD_UTILITY.f90:1135-1165) is the
+    time derivative of g(t) = exp(-((2e t - e t0) / t0)^2) (deltafunc(0)),
+    by central differences and by the chain rule written out:
+    g'(t) = -2 (2e t - e t0)(2e) / t0^2 g(t)."""
+    e = math.e
+    for t0 in (5e-9, 1e-7, 1e-6):
+        h = 1e-7 * t0
+        g = lambda s: math.exp(-(((2 * e) * s - e * t0) / t0) ** 2)  # noqa: E731
+        for t in np.linspace(0.0, t0, 21):
+            num = (g(t + h) - g(t - h)) / (2 * h)
+            chain = -2.0 * ((2 * e) * t - e * t0) * (2 * e) / t0 ** 2 * g(t)
+            got = pyoracle.source_didt(1, 2, t, t0, 1.0)
+            scale = 4 * e / t0  # |g'| <= ~2.4 e / t0
+            assert abs(got - chain) <= 1e-13 * scale, (t0, t)
+            assert abs(got - num) <= 1e-7 * scale, (t0, t)
+            # current scales the waveform
+            assert pyoracle.source_didt(1, 2, t, t0, 3.0) == pytest.approx(3.0 * got, rel=1e-15, abs=0)
+    # the pulse's integral over its support is g(t0) - g(0) = 0 (odd about t0/2)
+    t0 = 1e-7
+    ts = np.linspace(0.0, t0, 20001)
+    vals = np.array([pyoracle.source_didt(1, 2, t, t0, 1.0) for t in ts])
+    integ = np.sum(0.5 * (vals[1:] + vals[:-1]) * np.diff(ts))
+    assert abs(integ) < 1e-9 * np.abs(vals).max() * t0
+
+
+def test_field2_gate_rule_on_linear_rows():
+    """The .field2 gate rule (src/FETD_FWD.f90:1624-1641): every gate is
+    filled at the first row with t >= gate, by linear interpolation with the
+    row before.  Rows linear in t are reproduced at the gate times; a gate
+    already passed by the first row is skipped there (H12's `exit`) and
+    filled at the second row, by extrapolation of that line."""
+    rng = np.random.default_rng(11)
+    t = np.cumsum(rng.uniform(0.5e-6, 2e-6, 40))
+    t = np.concatenate([[0.0], t])
+    alpha, beta = rng.standard_normal(9), rng.standard_normal(9)
+    f = alpha[None, :] * t[:, None] + beta[None, :]
+    gates = np.concatenate([[-1e-6, 0.0], np.sort(rng.uniform(t[1], t[-1], 12)), [t[7], 2 * t[-1]]])
+    vals, row = pyoracle.gate_rows(t, f, gates)
+    for g, tg in enumerate(gates):
+        if tg > t[-1]:
+            assert row[g] == -1
+            continue
+        want_row = max(1, int(np.argmax(t >= tg)))  # never the first row
+        assert row[g] == want_row, (g, tg)
+        np.testing.assert_allclose(vals[g], alpha * tg + beta, rtol=1e-12, atol=1e-12 * np.abs(beta).max())
+    # a gate equal to a row time takes that row's values
+    g7 = len(gates) - 2
+    assert row[g7] == 7
+    np.testing.assert_allclose(vals[g7], f[7], rtol=1e-13)
+
+
+def test_field2_gate_rule_interpolates_between_bracketing_rows():
+    """For gates inside (t[r-1], t[r]] the value is the chord between the
+    bracketing rows (numpy's interp); several gates can fill on one row."""
+    t = np.array([0.0, 1.0, 2.0, 4.0, 8.0, 16.0])
+    f = np.stack([np.sin(t), t ** 2, np.exp(-t)], axis=1)
+    gates = np.array([0.5, 1.5, 1.9, 3.0, 3.5, 5.0, 12.0])
+    vals, row = pyoracle.gate_rows(t, f, gates)
+    np.testing.assert_array_equal(row, [1, 2, 2, 3, 3, 4, 5])
+    for q in range(3):
+        np.testing.assert_allclose(vals[:, q], np.interp(gates, t, f[:, q]), rtol=1e-14, atol=1e-15)
+
+
+def _stepoff_model(time_maximum):
+    import dataclasses
+
+    from temf3dt_amd import workloads
+    m = workloads.small_model()
+    return dataclasses.replace(m, time_maximum=time_maximum)
+
+
+def _oracle_run(m, pulse_type, tmp_path, name, replay=None):
+    import dataclasses
+    st, _, t0, cur = m.source
+    m = dataclasses.replace(m, source=(st, pulse_type, t0, cur))
+    root = str(tmp_path / name)
+    m.write(root)
+    o = pyoracle.Oracle(root + ".mesh", root + ".cntl")
+    o.assemble()
+    if replay is not None:
+        o.set_replay(replay)
+    r = o.run()
+    o.close()
+    return r
+
+
+@pytest.mark.parametrize("end", ["regular", "changed_check"])
+def test_stepoff_second_pass_starts_from_first_pass_e1(end, tmp_path):
+    """Two-pass step-off (src/FETD_FWD.f90:1212-1217,1295-1301,1453-1461):
+    pass 1 negates the type-3 source, so it is exactly the step-on run
+    (negation is exact in floating point); its final e1 starts pass 2 as
+    both history states.  e1 is the last regular or kept solution: when the
+    loop ends on a check that changed dt it is the regular step before that
+    check, not the check's e_veri_1.  Pass 2's row 0 (E at the receivers
+    from e1; dB/dt zeroed, H11) therefore equals a step-on run's E of the
+    corresponding row, bit for bit, in both cases (one solver thread, so
+    that PARDISO repeats its arithmetic exactly)."""
+    pyoracle.set_threads(1)
+    base = _stepoff_model(2e-4)
+    on = _oracle_run(base, 1, tmp_path, "probe")
+    chk_rows = [int(np.nonzero(on["step"] == s)[0][0]) for s in on["chk_step"]]
+    if end == "regular":
+        tmax = 0.5 * (on["t"][40] + on["t"][41])  # ends on a regular step
+        last_src = None
+    else:
+        changed = [r for r, k in zip(chk_rows, on["chk_keep"]) if k == 0 and r > 1]
+        assert changed, "the model must change dt at least once"
+        r = changed[0]
+        tmax = on["t"][r]  # the changed check's new time reaches time_maximum
+        last_src = r
+    m = _stepoff_model(tmax)
+    on = _oracle_run(m, 1, tmp_path, "on")
+    off = _oracle_run(m, 3, tmp_path, "off", replay=list(on["chk_keep"]) * 2)
+    if end == "regular":
+        assert on["step"][-1] % m.sp_double != 0
+        src = len(on["step"]) - 1
+    else:
+        assert on["step"][-1] % m.sp_double == 0 and on["chk_keep"][-1] == 0
+        src = len(on["step"]) - 2  # the regular step before the changed check
+        assert src == last_src - 1
+    assert off["step"][0] == 0
+    np.testing.assert_array_equal(off["fields"][0, :, 0:3], on["fields"][src, :, 0:3])
+    assert np.all(off["fields"][0, :, 3:6] == 0.0)
+    if end == "changed_check":  # the check's own e_veri_1 differs from e1
+        assert not np.array_equal(on["fields"][-1, :, 0:3], on["fields"][src, :, 0:3])
+
+
+def test_stepoff_is_negated_stepon_after_the_ramp(tmp_path):
+    """By linearity pass 2 = (free decay of e1) - (step-on response): with
+    the same schedule, off + on is the decay of the pass-1 end state, which
+    is small while t is small against time_maximum.  Tolerance: |dBz/dt of
+    off + on| <= 2 % of |on| for rows past the ramp up to time_maximum / 20,
+    and the sign flips (a dropped or doubled negation fails both)."""
+    pyoracle.set_threads(4)
+    m = _stepoff_model(2e-4)
+    on = _oracle_run(m, 1, tmp_path, "on")
+    off = _oracle_run(m, 3, tmp_path, "off", replay=list(on["chk_keep"]) * 2)
+    np.testing.assert_array_equal(off["step"], on["step"])
+    np.testing.assert_array_equal(off["t"], on["t"])
+    a, b = off["fields"][:, 0, 5], on["fields"][:, 0, 5]
+    sel = (on["step"] >= m.sp_division) & (on["t"] <= m.time_maximum / 20) & (np.abs(b) > 0)
+    assert sel.sum() > 50
+    rel = np.abs(a[sel] + b[sel]) / np.abs(b[sel])
+    assert rel.max() < 0.02, rel.max()
+    assert np.all(np.sign(a[sel]) == -np.sign(b[sel]))
