@@ -71,17 +71,31 @@ def timed_run(fe, steps, warmup, profile=False):
     return rows, wall, fe.timers(), prof
 
 
-def measured_traffic():
-    """HBM bytes per k_mtile launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE
-    passes of this workload, scaled by a 1 GiB device copy (the summary in
-    profiles/hbm_traffic_batched3d.txt gives the commands); None if absent."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_batched3d.txt")
+def measured_traffic(name, key="traffic_bytes_per_launch"):
+    """HBM bytes per launch (or per step) from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes summarised in profiles/<name>, scaled by a 1 GiB device
+    copy (the summary gives the commands); None if absent."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":")[1])
     return None
+
+
+def host_cpu():
+    """Host CPU description for the baseline: model name, logical CPUs of the
+    machine and of this process's affinity set."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
 
 
 def adjoint_ratio(fe, steps, forward_wall, ckpt):
@@ -106,39 +120,54 @@ def adjoint_ratio(fe, steps, forward_wall, ckpt):
             "grad_finite": bool(np.all(np.isfinite(g)))}
 
 
-def cpu_baseline(m, counts, sample_steps):
-    """Oracle (C restatement + MKL PARDISO, the reference's solver) on the
-    host cores for the first `sample_steps` steps of one transmitter of the
-    same model.  Per-factorisation and per-step times are weighted by the
-    GPU run's counts to give source-timesteps/s; the reference runs one
-    transmitter per job, so 64 transmitters are 64 such runs at this rate."""
+def cpu_baseline(m, counts, nsrc):
+    """The oracle (C restatement of the reference + MKL PARDISO, the
+    reference's solver) on the host cores: one factorisation of transmitter
+    0's system at the first dt, timed by phase (analysis 11, numeric 22),
+    one solve with 1 and with nsrc right-hand sides on it (phase 33), and one
+    right-hand side with its SpMV.  The GPU run's counts (steps, solves,
+    factorisations) weight them into source-timesteps/s:
+      reference    phase 12 (11 + 22) at every factorisation, one source per
+                   run as the reference does (src/FETD_FWD.f90:1307,1361), so
+                   nsrc transmitters are nsrc runs at this rate;
+      symbolic     the analysis once, phase 22 at every factorisation;
+      block        analysis once, and all nsrc sources through one phase-33
+                   call with nrhs = nsrc per solve (PARDISO's multi-RHS).
+    Threads: this process's CPUs, capped by OMP_NUM_THREADS (the host's
+    share per GPU)."""
     import tempfile
 
     from oracle import pyoracle
     if not pyoracle.pardiso_available():
         return None
-    threads = int(os.environ.get("TF_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    cpu = host_cpu()
+    threads = cpu["affinity"]
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    threads = int(os.environ.get("TF_CPU_THREADS", threads))
     pyoracle.set_threads(threads)
+    t0 = time.time()
     with tempfile.TemporaryDirectory() as tmp:
         root = os.path.join(tmp, "cpu")
         m.write(root, loop=0)
         o = pyoracle.Oracle(root + ".mesh", root + ".cntl")
         o.assemble()
-        t0 = time.time()
-        r = o.run(max_steps=sample_steps)
-        wall = time.time() - t0
-    tm = r["timers"]
-    nfact = max(1, r["nfact"])
-    nstep = max(1, int(r["step"][-1]))
-    t_fact = tm[1] / nfact
-    t_step = (wall - tm[1]) / nstep
-    steps, nf = counts
-    est = steps / (nf * t_fact + steps * t_step)
-    return {"value": est, "unit": "source-timesteps/s", "cores": threads, "kind": "port",
-            "sample": (f"first {nstep} steps of transmitter 0 on the host with MKL PARDISO "
-                       f"({nfact} factorisation(s), {t_fact:.1f} s each; {t_step * 1e3:.0f} ms per step); "
-                       f"rate = GPU run's {steps} steps / ({nf} factorisations + {steps} steps) at those costs"),
-            "pardiso_factor_nnz": int(r["factor_nnz"]), "wall_s": wall}
+        c = o.cpu_costs(m.source[2] / m.sp_division, nrhs_block=nsrc)
+        o.close()
+    wall = time.time() - t0
+    steps, nsolves, nf = counts
+    t12 = c["phase11_s"] + c["phase22_s"]
+    ref = steps / (nf * t12 + nsolves * (c["phase33_s"] + c["rhs_s"]))
+    sym = steps / (c["phase11_s"] + nf * c["phase22_s"] + nsolves * (c["phase33_s"] + c["rhs_s"]))
+    blk = nsrc * steps / (c["phase11_s"] + nf * c["phase22_s"] + nsolves * (c["phase33_block_s"] + nsrc * c["rhs_s"]))
+    return {"value": ref, "unit": "source-timesteps/s", "cores": threads, "kind": "port",
+            "sample": (f"transmitter 0 of the same model on the host with MKL PARDISO (mtype {int(c['mtype'])}): "
+                       f"phase 11 {c['phase11_s']:.2f} s, phase 22 {c['phase22_s']:.2f} s, phase 33 "
+                       f"{c['phase33_s'] * 1e3:.0f} ms (1 rhs) / {c['phase33_block_s'] * 1e3:.0f} ms ({nsrc} rhs), "
+                       f"rhs + SpMV {c['rhs_s'] * 1e3:.0f} ms; weighted by the GPU run's {steps} steps, {nsolves} "
+                       f"solves and {nf} factorisations"),
+            "variants": {"symbolic_once": sym, "pardiso_nrhs_block": blk},
+            "host": cpu, "pardiso_factor_nnz": int(c["factor_nnz"]), "costs": c, "wall_s": wall}
 
 
 def uneven_bench(steps, n=48):
@@ -197,6 +226,37 @@ def uneven_bench(steps, n=48):
     print(json.dumps(res))
 
 
+def single_source_rooflines(prof, nsolves, nfact, st, n, nnzu, lay):
+    """Roofline entries of a single-source context's profiled run: the step
+    (HBM) and the numeric factorisation (FP64 MFMA).
+    Step: bytes_1 = 16 nnzL_off (L read in both sweeps) + 8 n (D) + 8 sum of
+    struct rows (i32 row indices, both sweeps) + 12 nnzU + 4 (n + 1) (upper
+    CSR of Msigma) + 48 n (e2, e3, r, x and two work vectors), over the
+    summed duration of its SpMV/right-hand-side and solve kernels per solve.
+    Factorisation: F_fact = sum_j (c_j^2 + 2 c_j) flops over the summed
+    duration of all factorisation kernels (update, panel, inversion,
+    extend-add, scatter) per factorisation."""
+    out = {}
+    nnzl_off = st["nnzL"] - n
+    bytes1 = 16.0 * nnzl_off + 8.0 * n + 8.0 * lay["struct_rows"] + 12.0 * nnzu + 4.0 * (n + 1) + 48.0 * n
+    step_ms = (prof["solve1"][1] + prof["step"][1]) / nsolves
+    ach = bytes1 / (step_ms * 1e-3) / 1e9
+    out["plain_step"] = {
+        "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+        "traffic": measured_traffic("hbm_traffic_plain_step.txt", "traffic_bytes_per_step"),
+        "kernel": (f"single right-hand-side step (k_spmv_rhs + forward/backward solve kernels): bytes_1 = "
+                   f"{bytes1:.4e} B per step, {step_ms:.3f} ms per step ({nsolves} solves; HIP events)")}
+    fact_ms = (prof["factor_update"][1] + prof["factor_other"][1]) / max(nfact, 1)
+    fach = st["flops"] / (fact_ms * 1e-3) / 1e12
+    out["factorisation"] = {
+        "bound": "mfma", "achieved": fach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "frac": fach / FP64_PEAK_TFLOPS,
+        "traffic": measured_traffic("hbm_traffic_factor.txt", "traffic_bytes_per_factorisation"),
+        "kernel": (f"multifrontal LDL^T (k_factor_* + inversion + extend-add): F_fact = {st['flops']:.4e} flops "
+                   f"per factorisation, {fact_ms:.1f} ms each ({nfact} factorisations; HIP events)")}
+    return out
+
+
 def copy_probe():
     """Calibration for HBM counters: three device copies of 1 GiB (read 1 GiB,
     write 1 GiB each); FETCH_SIZE / WRITE_SIZE of these dispatches against the
@@ -216,8 +276,9 @@ def main():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--model", default="batched3d")
-    ap.add_argument("--cpu-sample", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sources", type=int, default=0,
+                    help="use only the first N transmitters (0: all); 1 makes the main run the plain path")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile-kernel autotuner (use defaults)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
@@ -238,6 +299,9 @@ def main():
               "DESIGN.md and is not launched", file=sys.stderr)
         sys.exit(2)
     m = build_model(a.model)
+    if a.sources > 0:
+        import dataclasses
+        m = dataclasses.replace(m, loops=m.loops[:a.sources])
     nsrc = len(m.loops)
     fe, t_an = setup_gpu(m, list(range(nsrc)), a.leaf)
     for kv in a.set:
@@ -245,6 +309,7 @@ def main():
         fe.set_tuning(key, int(val))
     st = fe.factor_stats()
     n = fe.nE
+    nnzu = fe.nnz
     print(f"[bench] {a.model}: {m.ntet} tets, {n} edges, nnzU {fe.nnz}, nnzL {st['nnzL']:.3e}, "
           f"factor flops {st['flops']:.3e}, {nsrc} sources, analyze {t_an:.2f} s", file=sys.stderr, flush=True)
     tune = None
@@ -256,31 +321,37 @@ def main():
         print(f"[bench] autotune: {tune['best']} ({tune['best_ms']:.2f} ms per solve, "
               f"{tune['seconds']:.1f} s)", file=sys.stderr, flush=True)
     rows, wall, tm, _ = timed_run(fe, a.steps, a.warmup)
+    steps = int(rows["step"][-1])
+    nf = int(tm["nfactor"])
+    nchk = len(fe.checks()["step"])
+    nsolves = steps + 2 * nchk  # a check step costs dblesize fine solves + 1 coarse solve
+    value = nsrc * steps / wall
+    print(f"[bench] batched: {steps} steps x {nsrc} sources in {wall:.3f} s, {nf} factorisations "
+          f"({tm['factor']:.3f} s), t_end {rows['t'][-1]:.3e}", file=sys.stderr, flush=True)
     # the same run again with HIP events around every launch group (which
     # also turns off the step graphs) for the per-kernel breakdown, with the
     # check's factorisation at the check, so that no kernel of the second
     # stream runs inside a timed launch group
     fe.set_tuning("overlap", 0)
-    _, _, _, prof = timed_run(fe, a.steps, 0, profile=True)
+    _, _, tm_prof, prof = timed_run(fe, a.steps, 0, profile=True)
     fe.set_tuning("overlap", 1)
-    steps = int(rows["step"][-1])
-    nf = int(tm["nfactor"])
-    value = nsrc * steps / wall
-    print(f"[bench] batched: {steps} steps x {nsrc} sources in {wall:.3f} s, {nf} factorisations "
-          f"({tm['factor']:.3f} s), t_end {rows['t'][-1]:.3e}", file=sys.stderr, flush=True)
     print("[bench] kernel time by category (launches, ms): " +
           ", ".join(f"{k} {v[0]} {v[1]:.1f}" for k, v in prof.items()), file=sys.stderr, flush=True)
-    # Dominant kernel: the block-solve tile products (k_mtile, four forms).
-    # Algorithmic work per solve: the factor's off-diagonal entries
-    # (L21 and the strict part of L11^-1) each enter one forward and one
-    # backward product with K right-hand sides: 4 K nnzL_off flops.
-    nsolves = steps + 3 * len(fe.checks()["step"])
+    lay = fe.factor_layout()
+    fe.close()
     nnzl_off = st["nnzL"] - n
-    flops_solve = 4.0 * nsrc * nnzl_off
-    tile_launches, tile_ms = prof["tile"]
-    achieved = flops_solve * nsolves / (tile_ms * 1e-3) / 1e12
-    traffic = measured_traffic() if a.model == "batched3d" else None
-    roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+    rooflines = {}
+    if nsrc > 1:
+        # block-solve tile products (k_mtile, four forms): the factor's
+        # off-diagonal entries (L21 and the strict part of L11^-1) each enter
+        # one forward and one backward product with K right-hand sides,
+        # 4 K nnzL_off flops per solve
+        flops_solve = 4.0 * nsrc * nnzl_off
+        tile_launches, tile_ms = prof["tile"]
+        achieved = flops_solve * nsolves / (tile_ms * 1e-3) / 1e12
+        traffic = measured_traffic("hbm_traffic_batched3d.txt") if a.model == "batched3d" else None
+        rooflines["block_solve_tiles"] = {
+            "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
             "traffic_note": ("HBM bytes per k_mtile launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on a "
                              "1 GiB copy), profiles/hbm_traffic_batched3d.txt") if traffic else None,
@@ -288,24 +359,36 @@ def main():
                        f"4*K*nnzL_off = {flops_solve:.3e} flops each, {tile_launches} launches, "
                        f"{tile_ms:.1f} ms (HIP events; breakdown run with the factorisations at the checks)"),
             "step_breakdown_ms": {k: round(v[1], 2) for k, v in prof.items()}}
-    fe.close()
+    if nsrc == 1:
+        rooflines.update(single_source_rooflines(prof, nsolves, int(tm_prof["nfactor"]), st, n, nnzu, lay))
     plain = None
     adjoint = None
     if not a.no_plain:
         fp, _ = setup_gpu(m, [0])
         prow, pwall, ptm, _ = timed_run(fp, a.steps, a.warmup)
         psteps = int(prow["step"][-1])
+        pchk = len(fp.checks()["step"])
         plain = {"value": psteps / pwall, "steps": psteps, "wall_s": pwall, "factor_s": ptm["factor"],
                  "factorisations": int(ptm["nfactor"])}
         print(f"[bench] plain: {psteps} steps in {pwall:.3f} s ({ptm['factor']:.3f} s factorising)",
               file=sys.stderr, flush=True)
+        # kernel breakdown of the plain run (factorisations at the checks)
+        fp.set_tuning("overlap", 0)
+        _, _, pptm, pprof = timed_run(fp, a.steps, 0, profile=True)
+        fp.set_tuning("overlap", 1)
+        print("[bench] plain kernel time by category (launches, ms): " +
+              ", ".join(f"{k} {v[0]} {v[1]:.1f}" for k, v in pprof.items()), file=sys.stderr, flush=True)
+        rooflines.update(single_source_rooflines(pprof, psteps + 2 * pchk, int(pptm["nfactor"]), st, n, nnzu, lay))
         if not a.no_adjoint:
             adjoint = adjoint_ratio(fp, a.steps, pwall, a.ckpt)
             print(f"[bench] adjoint: taped forward {adjoint['taped_forward_s']:.3f} s + backward "
                   f"{adjoint['backward_s']:.3f} s = {adjoint['ratio']:.2f} x the plain forward", file=sys.stderr,
                   flush=True)
         fp.close()
-    cpu = None if a.no_cpu else cpu_baseline(m, (steps, nf), a.cpu_sample)
+    roof = dict(rooflines.get("block_solve_tiles") or rooflines.get("plain_step") or {})
+    roof["all"] = {k: {q: v[q] for q in ("bound", "achieved", "peak", "unit", "frac", "traffic")}
+                   for k, v in rooflines.items()}
+    cpu = None if a.no_cpu else cpu_baseline(m, (steps, nsolves, nf), nsrc)
     out = {
         "metric": "FP64 source-timesteps/sec at 1M tets (batched vs plain); adjoint/forward time",
         "value": value, "unit": "source-timesteps/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
@@ -317,7 +400,7 @@ def main():
                    "plain_single_source": plain, "autotune": tune and tune["best"],
                    "batched_over_plain": (value / plain["value"]) if plain else None,
                    "adjoint_over_forward": adjoint},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "rooflines": rooflines, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 

@@ -140,6 +140,11 @@ int tf_set_profiling(tf_ctx *ctx, int on);
 int tf_get_profile(tf_ctx *ctx, int category, int64_t *launches, double *ms);
 /* Factor statistics: nnz(L) incl. diagonal, dense-front flops, supernodes. */
 int tf_factor_stats(const tf_ctx *ctx, int64_t *nnzL, double *flops, int *nsuper, int *nlevels);
+/* Storage of the supernodal factor (no reference counterpart; sizes for
+ * memory planning and the solve's byte count): the struct row indices of all
+ * fronts (sum of each front's rows below its own columns), the doubles of one
+ * factor slot's L storage and of the update stack. */
+int tf_factor_layout(const tf_ctx *ctx, int64_t *struct_rows, int64_t *l_doubles, int64_t *u_doubles);
 /* Pivots of the last factorisation (the counterpart of PARDISO's inertia
  * report, iparm(22)/iparm(23), and of the mtype 2 failure test of
  * pardiso_direct_1, src/FETD_UTILITY.f90:919-935): the number of negative

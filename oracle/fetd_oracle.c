@@ -1338,6 +1338,73 @@ static void log_check(fo_model *m, int step, double err, int keep) {
     m->nchk++;
 }
 
+/* Cost sample for the CPU baseline: the pieces of one time step and one
+ * factorisation of this model at time step dt, timed separately.  P is
+ * formed as in mkl_add_3 and analysed (phase 11) and factored (phase 22)
+ * with PARDISO's defaults, mtype 2, falling back to -2 like pardiso_direct_1
+ * (src/FETD_UTILITY.f90:879-936); a phase-12 call costs phase 11 + 22, the
+ * reference's path at every factorisation, while a caller that keeps the
+ * analysis for the unchanged pattern pays phase 22 only.  Then phase 33
+ * with one right-hand side (pardiso_direct_2) and with nrhs_block of them
+ * on the same factor, and the right-hand side sys_right + SpMV of one
+ * source (src/FETD_FWD.f90:1425-1428).  out: [0] phase 11 s, [1] phase 22 s,
+ * [2] phase 33 (1 rhs) s, [3] phase 33 (nrhs_block) s, [4] rhs + SpMV s,
+ * [5] factor nnz (iparm(18)), [6] mtype used.  Returns a PARDISO error. */
+int fo_cpu_costs(fo_model *m, double dt, int nrhs_block, double *out) {
+    if (!load_mkl()) return -9999;
+    const int n = m->nE;
+    double *pv = malloc(sizeof(double) * m->nnz);
+    form_p(m, dt, pv);
+    pslot s;
+    memset(&s, 0, sizeof s);
+    int maxfct = 1, mnum = 1, msglvl = 0, error = 0, idum = 0, nrhs = 1;
+    double ddum = 0;
+    s.n = n; s.ia = m->ia; s.ja = m->ja; s.va = pv;
+    double t11 = 0.0, t22 = 0.0;
+    for (int attempt = 0; attempt < 2; attempt++) {
+        memset(s.pt, 0, sizeof s.pt);
+        memset(s.iparm, 0, sizeof s.iparm);
+        s.mtype = attempt == 0 ? 2 : -2;
+        int ph = 11;
+        double t0 = now_s();
+        g_pardiso(s.pt, &maxfct, &mnum, &s.mtype, &ph, &s.n, pv, m->ia, m->ja, &idum, &nrhs, s.iparm, &msglvl,
+                  &ddum, &ddum, &error);
+        t11 = now_s() - t0;
+        if (error == 0) {
+            ph = 22;
+            t0 = now_s();
+            g_pardiso(s.pt, &maxfct, &mnum, &s.mtype, &ph, &s.n, pv, m->ia, m->ja, &idum, &nrhs, s.iparm, &msglvl,
+                      &ddum, &ddum, &error);
+            t22 = now_s() - t0;
+        }
+        s.live = 1;
+        if (error == 0) break;
+        pslot_release(&s);
+    }
+    if (error != 0) { free(pv); return error; }
+    s.nnzL = (long long)s.iparm[17];
+    const int K = nrhs_block > 1 ? nrhs_block : 1;
+    double *b = malloc(sizeof(double) * (size_t)n * K), *x = malloc(sizeof(double) * (size_t)n * K);
+    double *e2 = calloc((size_t)n, sizeof(double)), *e3 = calloc((size_t)n, sizeof(double));
+    for (size_t i = 0; i < (size_t)n * K; i++) b[i] = sin(0.37 * (double)i);
+    for (int i = 0; i < n; i++) { e2[i] = cos(0.11 * i); e3[i] = sin(0.23 * i); }
+    double t0 = now_s();
+    pslot_solve(&s, b, x, 1);
+    const double t33 = now_s() - t0;
+    t0 = now_s();
+    pslot_solve(&s, b, x, K);
+    const double t33k = now_s() - t0;
+    t0 = now_s();
+    sys_right(m, 1, 0.5 * m->pulse_width, x, 0);
+    build_rhs(m, e2, e3, dt, x);
+    const double trhs = now_s() - t0;
+    out[0] = t11; out[1] = t22; out[2] = t33; out[3] = t33k; out[4] = trhs;
+    out[5] = (double)s.nnzL; out[6] = (double)s.mtype;
+    pslot_release(&s);
+    free(b); free(x); free(e2); free(e3); free(pv);
+    return 0;
+}
+
 /* adap_euler (src/FETD_FWD.f90:1229-1464).  inout: 0 = plain run with
  * output, 1 = first pass of a step-off run (negated source, no output,
  * result in res_e), 2 = second pass starting from res_e. */

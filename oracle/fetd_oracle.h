@@ -112,6 +112,11 @@ double fo_det_lu(const double *a_colmajor, int n);
 double fo_source_didt(int source_type, int pulse_type, double t, double t0, double cur);
 
 /* ---- solver backend (MKL PARDISO via dlopen) ---- */
+/* Timed pieces of one step and one factorisation at time step dt (CPU
+ * baseline): out[0] phase 11, [1] phase 22, [2] phase 33 with 1 rhs,
+ * [3] phase 33 with nrhs_block, [4] sys_right + SpMV (seconds), [5] factor
+ * nnz, [6] mtype used.  Returns a PARDISO error code (0 on success). */
+int fo_cpu_costs(fo_model *m, double dt, int nrhs_block, double *out);
 int fo_pardiso_available(void);
 void fo_set_threads(int n);
 /* Solve the upper-CSR (1-based) symmetric system once; returns PARDISO error. */

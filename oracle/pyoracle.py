@@ -54,6 +54,7 @@ def lib():
         L.fo_get_tensors.argtypes = [_P, _P, _P]
         L.fo_gate_rows.argtypes = [_I, _I, _P, _P, _I, _P, _P, _P]
         L.fo_gate_rows.restype = None
+        L.fo_cpu_costs.argtypes = [_P, _D, _I, _P]
         _lib = L
     return _lib
 
@@ -141,6 +142,16 @@ class Oracle:
         s9 = np.empty((self.nT, 3, 3))
         n9 = np.empty((self.nT, 3, 3))
         return (s9, n9) if self.L.fo_get_tensors(self.m, _p(s9), _p(n9)) else None
+
+    def cpu_costs(self, dt, nrhs_block=64):
+        """Timed pieces at time step dt: phase 11, phase 22, phase 33 with
+        1 and nrhs_block right-hand sides, rhs + SpMV (s), factor nnz, mtype."""
+        out = np.zeros(8)
+        rc = self.L.fo_cpu_costs(self.m, dt, nrhs_block, _p(out))
+        if rc != 0:
+            raise RuntimeError(f"PARDISO error {rc}")
+        keys = ["phase11_s", "phase22_s", "phase33_s", "phase33_block_s", "rhs_s", "factor_nnz", "mtype"]
+        return dict(zip(keys, out[:7].tolist()))
 
     def set_replay(self, keep):
         k = np.ascontiguousarray(keep, np.int32)

@@ -2353,6 +2353,14 @@ int tf_factor_stats(const tf_ctx *c, int64_t *nnzL, double *flops, int *nsuper, 
     return TF_OK;
 }
 
+int tf_factor_layout(const tf_ctx *c, int64_t *struct_rows, int64_t *l_doubles, int64_t *u_doubles) {
+    if (!c || !c->analyzed) return TF_ERR_STATE;
+    if (struct_rows) *struct_rows = (int64_t)c->sym.sidx.size();
+    if (l_doubles) *l_doubles = c->sym.ltotal;
+    if (u_doubles) *u_doubles = c->sym.utotal;
+    return TF_OK;
+}
+
 int tf_factor_pivots(const tf_ctx *c, int64_t *n_negative, double *min_abs, double *max_abs) {
     if (!c || !c->analyzed) return TF_ERR_STATE;
     if (n_negative) *n_negative = c->last_nneg;
@@ -2367,7 +2375,12 @@ int tf_factor(tf_ctx *c, int slot, double dt) {
         if (!c->assembled) return fail(c, TF_ERR_STATE, "tf_assemble first");
         if (!c->analyzed) do_analyze(c, c->leaf_size);
         factor_dt(c, slot, dt);
-        check_flag(c);
+        try {
+            check_flag(c);
+        } catch (...) {
+            c->slot_live[slot] = false;
+            throw;
+        }
         return TF_OK;
     });
 }
@@ -2393,7 +2406,12 @@ int tf_factor_csr(tf_ctx *c, int slot, int n, const int *ia, const int *ja, cons
         c->slot_dt[slot] = -1.0;
         HIPCHK(hipStreamSynchronize(c->st));
         dfree(dv);
-        check_flag(c);
+        try {
+            check_flag(c);
+        } catch (...) {
+            c->slot_live[slot] = false;
+            throw;
+        }
         return TF_OK;
     });
 }

@@ -39,6 +39,7 @@ _SIGS = {
     "tf_analyze": [_P, _I],
     "tf_factor_stats": [_P, _P, _P, _P, _P],
     "tf_factor_pivots": [_P, _P, _P, _P],
+    "tf_factor_layout": [_P, _P, _P, _P],
     "tf_factor": [_P, _I, _D],
     "tf_factor_csr": [_P, _I, _I, _P, _P, _P],
     "tf_solve": [_P, _I, _I, _P, _P],
@@ -253,6 +254,12 @@ class Fetd:
         self._chk(self.lib.tf_factor_stats(self.ctx, ctypes.byref(nnzL), ctypes.byref(fl), ctypes.byref(ns),
                                            ctypes.byref(nl)), "tf_factor_stats")
         return dict(nnzL=nnzL.value, flops=fl.value, nsuper=ns.value, nlevels=nl.value)
+
+    def factor_layout(self):
+        """Struct row indices, L doubles per slot, update-stack doubles."""
+        v = [ctypes.c_int64() for _ in range(3)]
+        self._chk(self.lib.tf_factor_layout(self.ctx, *[ctypes.byref(x) for x in v]), "tf_factor_layout")
+        return dict(struct_rows=v[0].value, l_doubles=v[1].value, u_doubles=v[2].value)
 
     def factor_pivots(self):
         """Pivots of the last factorisation: negative count, min and max |d|."""

@@ -261,3 +261,30 @@ def union(models):
     return dict(xyz=np.vstack(xyz), tet_vex=np.ascontiguousarray(np.vstack(tets), np.int32), sigma=np.concatenate(sig),
                 inv_mu=np.concatenate(imu), loops=loops, receivers=np.vstack(recs), ntet=np.array(ntet),
                 nrec=np.array(nrec))
+
+
+def adjoint_blocks(seed=4):
+    """True model of the adjoint workload: three blocks of 1, 0.1 and
+    0.003 S/m with sides of 100-300 m (seeded) under the receiver line of
+    plain_3d_model, in a 0.01 S/m host."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s, x in zip((1.0, 0.1, 0.003), (-250.0, 0.0, 250.0)):
+        sx, sy, sz = rng.uniform(100, 300, 3)
+        top = rng.uniform(40, 150)
+        cx = x + rng.uniform(-50, 50)
+        cy = rng.uniform(-50, 50)
+        out.append((cx - sx / 2, cx + sx / 2, cy - sy / 2, cy + sy / 2, top, top + sz, s))
+    return out
+
+
+def tet_sigma(m, host_sigma, blocks=(), air_sigma=1e-12):
+    """Per-tet conductivity of model m's mesh for another earth: host_sigma
+    below z = 0, air above, and blocks [(x0,x1,y0,y1,z0,z1,sigma)] by tet
+    centroid (later blocks win)."""
+    c = m.xyz[m.tet_vex - 1].mean(axis=1)
+    sig = np.where(c[:, 2] < 0, air_sigma, host_sigma)
+    for (x0, x1, y0, y1, z0, z1, s) in blocks:
+        sel = (c[:, 0] > x0) & (c[:, 0] < x1) & (c[:, 1] > y0) & (c[:, 1] < y1) & (c[:, 2] > z0) & (c[:, 2] < z1)
+        sig[sel] = s
+    return sig

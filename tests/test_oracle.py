@@ -400,3 +400,16 @@ def test_stepoff_is_negated_stepon_after_the_ramp(tmp_path):
     rel = np.abs(a[sel] + b[sel]) / np.abs(b[sel])
     assert rel.max() < 0.02, rel.max()
     assert np.all(np.sign(a[sel]) == -np.sign(b[sel]))
+
+
+def test_cpu_cost_sample(oracle_mod_big):
+    """The CPU baseline's cost sample: phases 11/22/33 and the SpMV time on
+    the shipped model, with the same factor size as the full run's phase 12."""
+    pyoracle.set_threads(4)
+    c = oracle_mod_big.cpu_costs(1e-8, nrhs_block=8)
+    assert c["mtype"] == 2
+    assert all(c[k] > 0 for k in ("phase11_s", "phase22_s", "phase33_s", "phase33_block_s", "rhs_s"))
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "src_mod_big", "oracle_rows.npz"))
+    assert len(g["step"]) == 579
+    r = oracle_mod_big.run(max_steps=1)
+    assert int(c["factor_nnz"]) == int(r["factor_nnz"])
