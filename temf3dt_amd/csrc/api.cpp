@@ -902,8 +902,16 @@ void factor_dt(tf_ctx *c, int slot, double dt) {
     c->slot_dt[slot] = dt;
 }
 
-// x = P^-1 b (device vectors, edge order)
-void solve_dev(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd, const double *b, double *x) {
+// Work vectors of the single right-hand-side solve: V (every front's
+// f-row vector, vrows doubles), Y, T and the permuted solution xw (n each).
+struct SolveWork {
+    double *V, *Y, *T, *xw;
+};
+SolveWork ctx_work(tf_ctx *c) { return SolveWork{c->d_V, c->d_Y, c->d_T, c->d_xw}; }
+
+// x = P^-1 b (device vectors, edge order), work vectors w
+void solve_dev(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd, const double *b, double *x,
+               const SolveWork &w) {
     const Plan &p = c->plan;
     const int *lst = c->d_plist;
     const int64_t *pr = c->d_ppref;
@@ -911,21 +919,21 @@ void solve_dev(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd
         const int pm = prof_mark(c);
         switch (l.kind) {
         case L_SSMALL:
-            tf::launch_fwd_small(L, fd, lst + l.off, l.cnt, b, c->d_perm, c->d_chp, c->d_ch, c->d_rel, c->d_V,
-                                 c->d_Y, c->st);
+            tf::launch_fwd_small(L, fd, lst + l.off, l.cnt, b, c->d_perm, c->d_chp, c->d_ch, c->d_rel, w.V,
+                                 w.Y, c->st);
             break;
         case L_GINIT:
-            tf::launch_mg_init(fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, b, c->d_perm, c->d_V, 1, c->st);
+            tf::launch_mg_init(fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, b, c->d_perm, w.V, 1, c->st);
             break;
         case L_GCHILD:
-            tf::launch_mg_child(fd, c->d_parent, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_rel, c->d_V, 1,
+            tf::launch_mg_child(fd, c->d_parent, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_rel, w.V, 1,
                                 c->st);
             break;
         case L_SOWN:
-            tf::launch_fwd_own(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_V, c->d_Y, c->st);
+            tf::launch_fwd_own(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, w.V, w.Y, c->st);
             break;
         case L_SSTRUCT:
-            tf::launch_fwd_struct(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_V, c->d_Y, c->st);
+            tf::launch_fwd_struct(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, w.V, w.Y, c->st);
             break;
         }
         prof_close(c, prof_category(l.kind), pm);
@@ -934,21 +942,24 @@ void solve_dev(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd
         const int pm = prof_mark(c);
         switch (l.kind) {
         case L_BSMALL:
-            tf::launch_bwd_small(L, fd, lst + l.off, l.cnt, c->d_Y, Dn, c->d_sidx, c->d_xw, c->st);
+            tf::launch_bwd_small(L, fd, lst + l.off, l.cnt, w.Y, Dn, c->d_sidx, w.xw, c->st);
             break;
         case L_GXS:
-            tf::launch_mg_xs(fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_sidx, c->d_xw, c->d_V, 1, c->st);
+            tf::launch_mg_xs(fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_sidx, w.xw, w.V, 1, c->st);
             break;
         case L_BT:
-            tf::launch_bwd_t(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_Y, Dn, c->d_V, c->d_T, c->st);
+            tf::launch_bwd_t(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, w.Y, Dn, w.V, w.T, c->st);
             break;
         case L_BX:
-            tf::launch_bwd_x(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_T, c->d_xw, c->st);
+            tf::launch_bwd_x(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, w.T, w.xw, c->st);
             break;
         }
         prof_close(c, prof_category(l.kind), pm);
     }
-    tf::launch_unpermute(c->sym.n, c->d_perm, c->d_xw, x, c->st);
+    tf::launch_unpermute(c->sym.n, c->d_perm, w.xw, x, c->st);
+}
+void solve_dev(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd, const double *b, double *x) {
+    solve_dev(c, L, Dn, fd, b, x, ctx_work(c));
 }
 
 void ensure_block(tf_ctx *c, int K) {

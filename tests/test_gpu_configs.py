@@ -73,7 +73,7 @@ def test_cfg_plain3d_matches_oracle():
     np.testing.assert_array_equal(rows["t"], g["t"])
     np.testing.assert_array_equal(ck["keep"], g["chk_keep"])
     np.testing.assert_allclose(ck["err"], g["chk_err"], rtol=1e-3)
-    _compare_dbdt(rows["fields"][:, 0], g["fields"], "plain 3-D, 41 receivers")
+    _compare_dbdt(rows["fields"], g["fields"], "plain 3-D, 41 receivers")  # [row][receiver][9]
 
 
 def test_cfg_batched3d_transmitters_match_oracle():
@@ -169,7 +169,7 @@ def test_cfg_adjoint_gradient_on_plain3d_mesh():
     gw = _gates_and_weights(t, gates)
 
     def gate_vals(rows):
-        f = rows["fields"][:, 0, :, 5]  # dBz/dt [row][receiver]
+        f = rows["fields"][:, :, 5]  # dBz/dt [row][receiver] (one source)
         return np.array([w1 * f[r - 1] + w2 * f[r] for r, w1, w2 in gw])
 
     d_true = gate_vals(obs_rows)
@@ -191,10 +191,10 @@ def test_cfg_adjoint_gradient_on_plain3d_mesh():
     fe.set_checkpointing(ckpt)
     rows = fe.run(max_steps=steps)
     resid = (gate_vals(rows) - d_obs) / sd ** 2  # dJ/d(gate value)
-    w = np.zeros_like(rows["fields"])  # [row][src][rec][9]
+    w = np.zeros_like(rows["fields"])  # [row][rec][9]
     for (r, w1, w2), res in zip(gw, resid):
-        w[r - 1, 0, :, 5] += w1 * res
-        w[r, 0, :, 5] += w2 * res
+        w[r - 1, :, 5] += w1 * res
+        w[r, :, 5] += w2 * res
     grad = fe.adjoint(w)
     fe.set_checkpointing(0)
     assert np.all(np.isfinite(grad))
