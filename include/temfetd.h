@@ -163,6 +163,22 @@ int tf_factor_csr(tf_ctx *ctx, int slot, int n, const int *ia, const int *ja, co
 /* pardiso_direct_2: x = P^-1 b for nrhs right-hand sides, host buffers in
  * edge order, [nrhs][n]; nrhs > 1 runs the block solve, 64 at a time. */
 int tf_solve(tf_ctx *ctx, int slot, int nrhs, const double *b, double *x);
+/* No reference counterpart.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): tf_solve for one right-hand side with b and x as device
+ * pointers (n doubles each, edge order), which allocates nothing.  Its work
+ * vectors live in a file-scope device array of TF_NOALLOC_SCRATCH_BYTES per
+ * GPU (held until the process ends), so it serves systems whose work vectors
+ * fit: n + (n + struct rows of the factor) + 2 n doubles, each rounded up to
+ * 32, at most 8,388,608 doubles (tf_factor_layout gives the struct rows;
+ * the 1.1M-unknown plain 3-D mesh fits, the 1.6M-unknown batched mesh does
+ * not).  Larger systems return
+ * TF_ERR_ARG.  The slot must have been factored by tf_factor or
+ * tf_factor_csr.  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (from two threads or contexts) race.  Returns after the
+ * solve has finished. */
+#define TF_NOALLOC_SCRATCH_BYTES (64 * 1024 * 1024)
+int tf_solve_noalloc(tf_ctx *ctx, int slot, const double *b, double *x);
 /* Device-resident solve timing for the autotuner (no reference counterpart):
  * nrhs (1..64) deterministic right-hand sides already in HBM, one warm-up
  * solve, then reps solves timed with HIP events; *ms_per_solve receives the

@@ -73,6 +73,7 @@ void launch_bwd_t(const double *L, const FrontDesc *fd, const int *list, const i
 void launch_bwd_x(const double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
                   const double *T, double *xw, hipStream_t st);
 void launch_unpermute(int n, const int *perm, const double *xw, double *x, hipStream_t st);
+hipError_t noalloc_scratch(int device, double **ptr);
 // msolve.hip
 void launch_fill_pattern(int64_t len, double *b, hipStream_t st);
 // step.hip: independent models
@@ -301,6 +302,7 @@ struct tf_ctx {
     double last_min_abs = 0.0, last_max_abs = 0.0;
     bool allow_indefinite = false;
     double *d_tmpb = nullptr, *d_tmpx = nullptr;
+    double *d_scratch = nullptr;  // this device's scratch array of tf_solve_noalloc (not owned)
     // stepping work vectors
     std::vector<double *> vecs;
     int64_t vec_len = 0;
@@ -1930,6 +1932,14 @@ int tf_create(tf_ctx **out, int device) {
         g_create_err = "could not create a HIP stream on the device";
         return TF_ERR_HIP;
     }
+    // the allocation-free solve's scratch array on this device (resolved once per device)
+    if (tf::noalloc_scratch(device, &c->d_scratch) != hipSuccess) {
+        (void)hipStreamDestroy(c->st);
+        (void)hipStreamDestroy(c->st2);
+        delete c;
+        g_create_err = "could not resolve the device address of the solve scratch array";
+        return TF_ERR_HIP;
+    }
     *out = c;
     return TF_OK;
 }
@@ -2454,6 +2464,39 @@ int tf_solve(tf_ctx *c, int slot, int nrhs, const double *b, double *x) {
         }
         HIPCHK(hipStreamSynchronize(c->st));
         HIPCHK(hipGetLastError());
+        return TF_OK;
+    });
+}
+
+int tf_solve_noalloc(tf_ctx *c, int slot, const double *b, double *x) {
+    return guarded(c, [&] {
+        if (slot < 0 || slot > 1 || !c->slot_live[slot]) return fail(c, TF_ERR_STATE, "slot not factored");
+        if (c->nmodels > 0) return fail(c, TF_ERR_STATE, "independent-model contexts have no single-system solve");
+        if (!b || !x) return fail(c, TF_ERR_ARG, "bad right-hand side");
+        for (const void *p : {(const void *)b, (const void *)x}) {
+            hipPointerAttribute_t at;
+            if (hipPointerGetAttributes(&at, p) != hipSuccess || at.type == hipMemoryTypeHost ||
+                at.type == hipMemoryTypeUnregistered) {
+                (void)hipGetLastError();
+                return fail(c, TF_ERR_ARG, "tf_solve_noalloc takes device pointers for b and x");
+            }
+        }
+        const int64_t n = c->sym.n;
+        auto up = [](int64_t v) { return (v + 31) / 32 * 32; };  // 256-byte aligned vectors
+        const int64_t need = up(c->vrows) + 3 * up(n);
+        if (need > TF_NOALLOC_DOUBLES)
+            return fail(c, TF_ERR_ARG,
+                        "system too large for the allocation-free solve: its work vectors need " +
+                            std::to_string(need) + " doubles, the scratch array holds " +
+                            std::to_string((int64_t)TF_NOALLOC_DOUBLES) + " (64 MiB)");
+        double *p = c->d_scratch;
+        const SolveWork w{p, p + up(c->vrows), p + up(c->vrows) + up(n), p + up(c->vrows) + 2 * up(n)};
+        const bool prof = c->prof_on;  // no event pool growth inside this call
+        c->prof_on = false;
+        solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, b, x, w);
+        c->prof_on = prof;
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->st));
         return TF_OK;
     });
 }

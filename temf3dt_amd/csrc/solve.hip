@@ -16,6 +16,9 @@
 // b and x are in edge order.
 #include "tf_device.h"
 
+#include <mutex>
+#include <vector>
+
 // ---- forward ------------------------------------------------------------------
 
 // vf = [b(own columns); 0] + extend-add of the children's update vectors
@@ -223,8 +226,38 @@ __global__ void k_unpermute(int n, const int *perm, const double *xw, double *x)
     if (i < n) x[perm[i]] = xw[i];
 }
 
+// ---- workspace of the allocation-free solve ---------------------------------------
+// The work vectors of tf_solve_noalloc (V, Y, T and the permuted solution)
+// live in this file-scope array: TF_NOALLOC_DOUBLES doubles on every GPU the
+// library runs on, held until the process ends, shared by every call on that
+// GPU (calls that overlap on one GPU race; every vector is written before it
+// is read, so nothing needs clearing between calls).
+static __device__ __align__(256) double scratch[TF_NOALLOC_DOUBLES];
+
 // ---- host launchers ----------------------------------------------------------
 namespace tf {
+
+// Device address of this GPU's scratch array, resolved once per device with
+// that device current (the name `scratch` is only a host-side stand-in);
+// nullptr and the HIP error if the lookup fails.
+hipError_t noalloc_scratch(int device, double **ptr) {
+    static std::mutex mu;
+    static std::vector<double *> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (device < 0) return hipErrorInvalidDevice;
+    if ((int)addr.size() <= device) addr.resize(device + 1, nullptr);
+    if (!addr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            *ptr = nullptr;
+            return e;
+        }
+        addr[device] = static_cast<double *>(p);
+    }
+    *ptr = addr[device];
+    return hipSuccess;
+}
 
 void launch_fwd_small(const double *L, const FrontDesc *fd, const int *list, int cnt, const double *b,
                       const int *perm, const int *chp, const int *ch, const int *rel, double *V, double *Y,

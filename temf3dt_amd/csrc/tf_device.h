@@ -15,6 +15,9 @@
 // vectors: the tile kernels' unmasked stage loads may read up to 63 values
 // beyond the last row of an array (results of those lanes are discarded).
 #define TF_READ_SLACK 4096
+// Doubles of the file-scope scratch array of the allocation-free solve
+// (64 MiB per GPU; TF_NOALLOC_SCRATCH_BYTES in include/temfetd.h)
+#define TF_NOALLOC_DOUBLES (8 * 1024 * 1024)
 #define TF_RPW 4  // rows per wave of the row-per-wave kernels (SpMV, pull): 4 * TF_RPW rows per workgroup
 
 // L11 inversion by doubling: leading dimension of the scratch block T (|J| x b)

@@ -40,6 +40,7 @@ _SIGS = {
     "tf_factor_stats": [_P, _P, _P, _P, _P],
     "tf_factor_pivots": [_P, _P, _P, _P],
     "tf_factor_layout": [_P, _P, _P, _P],
+    "tf_solve_noalloc": [_P, _I, _P, _P],
     "tf_factor": [_P, _I, _D],
     "tf_factor_csr": [_P, _I, _I, _P, _P, _P],
     "tf_solve": [_P, _I, _I, _P, _P],
@@ -285,6 +286,11 @@ class Fetd:
         x = np.empty_like(b)
         self._chk(self.lib.tf_solve(self.ctx, slot, nrhs, _ptr(b), _ptr(x)), "tf_solve")
         return x
+
+    def solve_noalloc(self, slot, b_ptr, x_ptr):
+        """tf_solve_noalloc: x = P^-1 b with b and x device addresses (ints,
+        n doubles each, edge order); allocates nothing."""
+        self._chk(self.lib.tf_solve_noalloc(self.ctx, slot, _P(b_ptr), _P(x_ptr)), "tf_solve_noalloc")
 
     def release(self, slot):
         self._chk(self.lib.tf_release(self.ctx, slot), "tf_release")
