@@ -90,14 +90,22 @@ void launch_sens(int nT, int K, const int *tet_edg, const uint8_t *orient, const
 void launch_axpy(int64_t len, double a, const double *x, double *y, hipStream_t st);
 void launch_scale(int n, const double *s, double *g, hipStream_t st);
 // mtile.hip
-void launch_mtile(int mode, int ks, int nb, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits,
+void launch_mtile(int mode, int ks, int nb, int na, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits,
                   int K, const double *in0, const double *in1, const double *Dn, const int *sidx, double *out,
                   double *part, uint32_t gbytes, const int *perm, double *x1, double *x2, const double *LT,
                   hipStream_t st);
+void launch_spull(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg, const double *b,
+                  const int *perm, const int *gptr, const int *gidx, const double *Z, double *Yin, hipStream_t st);
 void launch_mpull(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg, const double *b,
                   const int *perm, const int *gptr, const int *gidx, const double *Z, double *Yin, int K,
                   hipStream_t st);
-void launch_mtile_red(int mode, int nb, const FrontDesc *fd, const TileUnit *units, int nunits, int K,
+void launch_stile(int mode, int nc, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits,
+                  const double *in0, const double *in1, const double *Dn, const int *sidx, double *out, double *part,
+                  const int *perm, double *x1, double *x2, hipStream_t st);
+void launch_stile_red(int mode, const FrontDesc *fd, const TileUnit *units, int nunits, const double *in0,
+                      const double *in1, const double *Dn, double *out, const double *part, const int *perm,
+                      double *x1, double *x2, hipStream_t st);
+void launch_mtile_red(int mode, int nb, int na, const FrontDesc *fd, const TileUnit *units, int nunits, int K,
                       const double *in0, const double *in1, const double *Dn, double *out, const double *part,
                       const int *perm, double *x1, double *x2, hipStream_t st);
 void launch_mspmv_rhs(int n, int K, const int *ia, const int *ja, const double *val, const double *e2,
@@ -164,7 +172,8 @@ enum LaunchKind {
     L_EXTADD, L_FFUSED, L_FDIAG, L_FTRSM, L_FUPD, L_IMERGE_T, L_IMERGE_X, L_IDIAG,
     L_SSMALL, L_SOWN, L_SSTRUCT,
     L_BSMALL, L_BT, L_BX,
-    L_GINIT, L_GCHILD, L_GXS, L_MT, L_MTRED, L_MPULL
+    L_GINIT, L_GCHILD, L_GXS, L_MT, L_MTRED, L_MPULL,
+    L_ST, L_STRED, L_SPULL  // single right-hand side through the tile kernels
 };
 struct Launch {
     int kind, p0, off, cnt, poff;
@@ -187,14 +196,20 @@ int prof_category(int kind) {
 
 // Execution plan over the front tree: which fronts each launch touches.
 struct Plan {
-    std::vector<Launch> factor, fwd, bwd, mfwd, mbwd;
+    std::vector<Launch> factor, fwd, bwd, mfwd, mbwd, sfwd, sbwd;
     std::vector<int> list;
     std::vector<int64_t> pref;
     std::vector<int64_t> woff;  // per supernode: offset in the inversion scratch
     int64_t wsize = 0;
     int64_t part_max = 0;       // partial-sum tiles (4096 doubles each) of one launch
+    int64_t spart_max = 0;      // the same for the single right-hand-side plan (1024 doubles each)
     std::vector<TileUnit> units;  // block-solve tile units (L_MT / L_MTRED launches)
 };
+
+// partial-sum storage for both plans' split tiles
+size_t part_doubles(const Plan &p) {
+    return (size_t)std::max<int64_t>({p.part_max * 4096, p.spart_max * 1024, 4096});
+}
 
 static const double PI = 3.141592653589793;
 
@@ -285,6 +300,12 @@ struct tf_ctx {
     int mt_units = 4096, mt_chmin = 256;  // tile-launch sizing (units per launch, min inner chunk)
     int mt_wide = 64;  // levels of at most this many fronts use 64-row tiles, the others 32-row tiles
     int mt_ks_n = 2, mt_ks_t = 2;         // k-steps per stage of the N- and T-form tile kernels
+    // single right-hand-side solve: 2 through the vector tile kernels, 1
+    // through the matrix-core tile kernels with one right-hand-side block, 0
+    // through the matrix-vector kernels of solve.hip; launch sizing of its tiles
+    int s_tiles = 2;
+    int s_units = 8192, s_chmin = 512;
+    int s_tcols = 32;  // output columns per pass of the vector T-form tiles (16 or 32)
     double *d_U = nullptr, *d_V = nullptr, *d_xw = nullptr, *d_Y = nullptr, *d_T = nullptr, *d_W = nullptr;
     int64_t *d_woff = nullptr;
     double *d_L[2] = {nullptr, nullptr}, *d_D[2] = {nullptr, nullptr};
@@ -649,15 +670,17 @@ void build_plan(tf_ctx *c) {
     // 64-row tiles on levels of at most mt_wide fronts (few large fronts),
     // 32-row tiles below
     auto nb_of = [&](const std::vector<int> &fr) { return (int)fr.size() <= c->mt_wide ? 4 : 2; };
-    auto tiles_of = [&](int mode, const std::vector<int> &fr) {
+    auto tiles_of = [&](int mode, const std::vector<int> &fr, int TW = 0) {
         std::vector<Tile> t;
-        const int TW = 16 * nb_of(fr);  // output rows (columns) per tile
+        if (TW == 0) TW = 16 * nb_of(fr);  // output rows (columns) per tile
         for (int k : fr) {
             const int nc = s.nc[k], m = m_of(k);
             if (mode == 0)
                 for (int r = 0; r < nc; r += TW) t.push_back({k, r, 0, std::min(nc, r + TW)});
-            else if (mode == 1)  // row tiles start on 16-row lines of the column (rows nc + r)
-                for (int r = -(nc & 15); r < m; r += TW) t.push_back({k, r, 0, nc});
+            else if (mode == 1) {  // row tiles start on 16-row lines of the column (rows nc + r)
+                if (m > 0)
+                    for (int r = -(nc & 15); r < m; r += TW) t.push_back({k, r, 0, nc});
+            }
             else if (mode == 2)
                 for (int j = 0; j < nc; j += TW) t.push_back({k, j, 0, m});
             else
@@ -667,12 +690,15 @@ void build_plan(tf_ctx *c) {
     };
     // one tile launch (+ its reduction): tiles longer than the chunk are split
     // so that a launch has about mt_units units
-    auto add_tiles = [&](std::vector<Launch> &dst, int mode, const std::vector<Tile> &tl, int nb) {
+    auto add_tiles = [&](std::vector<Launch> &dst, int mode, const std::vector<Tile> &tl, int nb, int64_t utarget = 0,
+                         int chmin = 0, int kt = L_MT, int kr = L_MTRED) {
         if (tl.empty()) return;
+        if (utarget == 0) utarget = c->mt_units;
+        if (chmin == 0) chmin = c->mt_chmin;
         int64_t work = 0;
         for (const Tile &t : tl) work += t.hi - t.lo;
-        int64_t ch = (work + c->mt_units - 1) / c->mt_units;
-        ch = std::max<int64_t>(c->mt_chmin, (ch + 15) / 16 * 16);
+        int64_t ch = (work + utarget - 1) / utarget;
+        ch = std::max<int64_t>(chmin, (ch + 15) / 16 * 16);
         // units of one front are emitted chunk-major (all tiles of inner chunk
         // 0, then chunk 1, ...): neighbouring waves then share the right-hand
         // side rows of their inner chunk in L1/L2
@@ -706,13 +732,14 @@ void build_plan(tf_ctx *c) {
                 }
             t0 = t1;
         }
-        dst.push_back(Launch{L_MT, mode, (int)p.units.size(), (int)tu.size(), 0, (int64_t)(tu.size() + 3) / 4, 0, nb});
+        dst.push_back(Launch{kt, mode, (int)p.units.size(), (int)tu.size(), 0, (int64_t)(tu.size() + 3) / 4, 0, nb});
         p.units.insert(p.units.end(), tu.begin(), tu.end());
         if (!ru.empty()) {
-            dst.push_back(Launch{L_MTRED, mode, (int)p.units.size(), (int)ru.size(), 0, (int64_t)ru.size(), 0, nb});
+            dst.push_back(Launch{kr, mode, (int)p.units.size(), (int)ru.size(), 0, (int64_t)ru.size(), 0, nb});
             p.units.insert(p.units.end(), ru.begin(), ru.end());
         }
-        p.part_max = std::max<int64_t>(p.part_max, slots);
+        int64_t &pmax = (kt == L_ST) ? p.spart_max : p.part_max;
+        pmax = std::max<int64_t>(pmax, slots);
     };
     for (int l = 0; l < s.nlevels; l++) {
         std::vector<int> all(s.lev_sn.begin() + s.lev_ptr[l], s.lev_sn.begin() + s.lev_ptr[l + 1]);
@@ -733,6 +760,31 @@ void build_plan(tf_ctx *c) {
         add_tiles(p.mbwd, 3, tiles_of(3, all), nb_of(all));
         if (l == 0)  // no lower level gathers these rows: edge order only
             for (size_t i = nb; i < p.mbwd.size(); i++) p.mbwd[i].aux = 1;
+    }
+    // One right-hand side through the same four products, with one 16-wide
+    // right-hand-side block per tile: 64-row tiles on every level (a tile's
+    // matrix-core work is a quarter of the K = 64 tile's, so the factor
+    // stream sets its pace), own launch sizing (s_units, s_chmin), and the
+    // pull as a thread per row.
+    for (int l = 0; l < s.nlevels; l++) {
+        std::vector<int> all(s.lev_sn.begin() + s.lev_ptr[l], s.lev_sn.begin() + s.lev_ptr[l + 1]);
+        if (l == 0) {
+            add_tiles(p.sfwd, 4, tiles_of(0, all, 64), 4, c->s_units, c->s_chmin, L_ST, L_STRED);
+        } else {
+            std::vector<int64_t> ur;
+            for (int k : all) ur.push_back((s.nc[k] + 63) / 64);
+            add(p.sfwd, L_SPULL, 0, all, ur);
+            add_tiles(p.sfwd, 0, tiles_of(0, all, 64), 4, c->s_units, c->s_chmin, L_ST, L_STRED);
+        }
+        add_tiles(p.sfwd, 1, tiles_of(1, all, 64), 4, c->s_units, c->s_chmin, L_ST, L_STRED);
+    }
+    for (int l = s.nlevels - 1; l >= 0; l--) {
+        std::vector<int> all(s.lev_sn.begin() + s.lev_ptr[l], s.lev_sn.begin() + s.lev_ptr[l + 1]);
+        add_tiles(p.sbwd, 2, tiles_of(2, all, 64), 4, c->s_units, c->s_chmin, L_ST, L_STRED);
+        const size_t nb = p.sbwd.size();
+        add_tiles(p.sbwd, 3, tiles_of(3, all, 64), 4, c->s_units, c->s_chmin, L_ST, L_STRED);
+        if (l == 0)
+            for (size_t i = nb; i < p.sbwd.size(); i++) p.sbwd[i].aux = 1;
     }
 }
 
@@ -802,13 +854,14 @@ void do_analyze(tf_ctx *c, int leaf) {
         c->d_gidx = dupload(gidx);
     }
     c->d_U = dalloc<double>(s.utotal);
-    c->d_V = dalloc<double>(voff);
-    c->d_xw = dalloc<double>(s.n);
-    c->d_Y = dalloc<double>(s.n);
-    c->d_T = dalloc<double>(s.n);
+    // (the tile kernels' unmasked reads may run up to 63 values past a vector)
+    c->d_V = dalloc<double>(voff + TF_READ_SLACK);
+    c->d_xw = dalloc<double>(s.n + TF_READ_SLACK);
+    c->d_Y = dalloc<double>(s.n + TF_READ_SLACK);
+    c->d_T = dalloc<double>(s.n + TF_READ_SLACK);
     c->d_W = dalloc<double>((size_t)std::max<int64_t>(c->plan.wsize, 1));
     c->d_woff = dupload(c->plan.woff);
-    c->d_part = dalloc<double>((size_t)std::max<int64_t>(c->plan.part_max, 1) * 4096);
+    c->d_part = dalloc<double>(part_doubles(c->plan));
     c->d_tmpb = dalloc<double>(s.n);
     c->d_tmpx = dalloc<double>(s.n);
     if (!c->d_flag) c->d_flag = dalloc<FactorFlags>(1);
@@ -906,14 +959,96 @@ void factor_dt(tf_ctx *c, int slot, double dt) {
 
 // Work vectors of the single right-hand-side solve: V (every front's
 // f-row vector, vrows doubles), Y, T and the permuted solution xw (n each).
+// With the tile kernels V holds Yin (n) followed by Z (one value per struct
+// row), and part the split tiles' partial sums.
 struct SolveWork {
-    double *V, *Y, *T, *xw;
+    double *V, *Y, *T, *xw, *part;
 };
-SolveWork ctx_work(tf_ctx *c) { return SolveWork{c->d_V, c->d_Y, c->d_T, c->d_xw}; }
+SolveWork ctx_work(tf_ctx *c) { return SolveWork{c->d_V, c->d_Y, c->d_T, c->d_xw, c->d_part}; }
 
-// x = P^-1 b (device vectors, edge order), work vectors w
+void solve_dev_gemv(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd, const double *b, double *x,
+                    const SolveWork &w);
+
+// x = P^-1 b (device vectors, edge order), work vectors w; x2, if given,
+// receives a second copy of x.  The four products of every front (own and
+// struct forward, backward T and X) run as tile launches per level with one
+// right-hand side; the forward pull forms each level's Yin from b and the
+// descendants' Z, the backward X epilogue writes x in edge order.
 void solve_dev(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd, const double *b, double *x,
-               const SolveWork &w) {
+               const SolveWork &w, double *x2 = nullptr) {
+    if (!c->s_tiles) {
+        solve_dev_gemv(c, L, Dn, fd, b, x, w);
+        if (x2) HIPCHK(hipMemcpyAsync(x2, x, sizeof(double) * c->sym.n, hipMemcpyDeviceToDevice, c->st));
+        return;
+    }
+    const Plan &p = c->plan;
+    const int *lst = c->d_plist;
+    const int64_t *pr = c->d_ppref;
+    const int64_t n = c->sym.n;
+    const uint32_t gbytes = (uint32_t)(n * 8);
+    double *Yin = w.V, *Z = w.V + n;
+    for (const Launch &l : p.sfwd) {
+        const int pm = prof_mark(c);
+        const bool fromb = l.p0 == 4;
+        const double *in0 = fromb ? b : (l.p0 == 0 ? Yin : w.Y);
+        double *out = l.p0 == 1 ? Z : w.Y;
+        switch (l.kind) {
+        case L_SPULL:
+            tf::launch_spull(fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, b, c->d_perm, c->d_gptr, c->d_gidx, Z, Yin,
+                             c->st);
+            break;
+        case L_ST:
+            if (c->s_tiles == 2)
+                tf::launch_stile(l.p0, c->s_tcols, L, fd, c->d_units + l.off, l.cnt, in0, nullptr, Dn, nullptr, out,
+                                 w.part, c->d_perm, nullptr, nullptr, c->st);
+            else
+                tf::launch_mtile(l.p0, c->mt_ks_n, 4, 1, L, fd, c->d_units + l.off, l.cnt, 1, in0, nullptr, Dn, nullptr,
+                                 out, w.part, fromb ? gbytes : 0u, fromb ? c->d_perm : nullptr, nullptr, nullptr,
+                                 nullptr, c->st);
+            break;
+        case L_STRED:
+            if (c->s_tiles == 2)
+                tf::launch_stile_red(l.p0, fd, c->d_units + l.off, l.cnt, in0, nullptr, Dn, out, w.part,
+                                     fromb ? c->d_perm : nullptr, nullptr, nullptr, c->st);
+            else
+                tf::launch_mtile_red(l.p0, 4, 1, fd, c->d_units + l.off, l.cnt, 1, in0, nullptr, Dn, out, w.part,
+                                     fromb ? c->d_perm : nullptr, nullptr, nullptr, c->st);
+            break;
+        }
+        prof_close(c, prof_category(l.kind), pm);
+    }
+    for (const Launch &l : p.sbwd) {
+        const int pm = prof_mark(c);
+        const bool bx = l.p0 == 3;
+        double *out = bx ? (l.aux == 1 ? nullptr : w.xw) : w.T;
+        switch (l.kind) {
+        case L_ST:
+            if (c->s_tiles == 2)
+                tf::launch_stile(l.p0, c->s_tcols, L, fd, c->d_units + l.off, l.cnt, bx ? w.T : w.xw, bx ? w.T : w.Y,
+                                 Dn, c->d_sidx, out, w.part, c->d_perm, bx ? x : nullptr, bx ? x2 : nullptr, c->st);
+            else
+                tf::launch_mtile(l.p0, c->mt_ks_t, 4, 1, L, fd, c->d_units + l.off, l.cnt, 1, bx ? w.T : w.xw,
+                                 bx ? w.T : w.Y, Dn, c->d_sidx, out, w.part, gbytes, c->d_perm, bx ? x : nullptr,
+                                 bx ? x2 : nullptr, nullptr, c->st);
+            break;
+        case L_STRED:
+            if (c->s_tiles == 2)
+                tf::launch_stile_red(l.p0, fd, c->d_units + l.off, l.cnt, bx ? w.T : w.xw, bx ? w.T : w.Y, Dn, out,
+                                     w.part, c->d_perm, bx ? x : nullptr, bx ? x2 : nullptr, c->st);
+            else
+                tf::launch_mtile_red(l.p0, 4, 1, fd, c->d_units + l.off, l.cnt, 1, bx ? w.T : w.xw, bx ? w.T : w.Y, Dn,
+                                     out, w.part, c->d_perm, bx ? x : nullptr, bx ? x2 : nullptr, c->st);
+            break;
+        }
+        prof_close(c, prof_category(l.kind), pm);
+    }
+}
+
+// The same solve through the matrix-vector kernels of solve.hip (tuning key
+// "s_tiles" = 0): every front's update vector is extend-added into its
+// parent's front vector in V.
+void solve_dev_gemv(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd, const double *b, double *x,
+                    const SolveWork &w) {
     const Plan &p = c->plan;
     const int *lst = c->d_plist;
     const int64_t *pr = c->d_ppref;
@@ -989,6 +1124,7 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
     const uint64_t gb = (uint64_t)c->sym.n * K * 8;
     if (gb >= (1ull << 32)) throw std::runtime_error("block solve: n x K too large for the tile kernels (>= 4 GiB)");
     const uint32_t gbytes = (uint32_t)gb;
+    const int na = K <= 16 ? 1 : 4;  // 16-wide right-hand-side blocks per tile
     const Plan &p = c->plan;
     const int *lst = c->d_plist;
     const int64_t *pr = c->d_ppref;
@@ -1001,19 +1137,19 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
             break;
         case L_MT:  // own forward (Y from Yin, or from b), struct forward (Z from Y)
             if (l.p0 == 4)
-                tf::launch_mtile(4, c->mt_ks_n, l.nb, L, fd, c->d_units + l.off, l.cnt, K, b, nullptr, Dn, nullptr,
+                tf::launch_mtile(4, c->mt_ks_n, l.nb, na, L, fd, c->d_units + l.off, l.cnt, K, b, nullptr, Dn, nullptr,
                                  c->d_mY, c->d_part, gbytes, c->d_perm, nullptr, nullptr, nullptr, c->st);
             else
-                tf::launch_mtile(l.p0, c->mt_ks_n, l.nb, L, fd, c->d_units + l.off, l.cnt, K,
+                tf::launch_mtile(l.p0, c->mt_ks_n, l.nb, na, L, fd, c->d_units + l.off, l.cnt, K,
                                  l.p0 == 0 ? c->d_mV : c->d_mY, nullptr, Dn, nullptr, l.p0 == 0 ? c->d_mY : c->d_mZ,
                                  c->d_part, 0u, nullptr, nullptr, nullptr, nullptr, c->st);
             break;
         case L_MTRED:
             if (l.p0 == 4)
-                tf::launch_mtile_red(4, l.nb, fd, c->d_units + l.off, l.cnt, K, b, nullptr, Dn, c->d_mY, c->d_part,
+                tf::launch_mtile_red(4, l.nb, na, fd, c->d_units + l.off, l.cnt, K, b, nullptr, Dn, c->d_mY, c->d_part,
                                      c->d_perm, nullptr, nullptr, c->st);
             else
-                tf::launch_mtile_red(l.p0, l.nb, fd, c->d_units + l.off, l.cnt, K, l.p0 == 0 ? c->d_mV : c->d_mY, nullptr,
+                tf::launch_mtile_red(l.p0, l.nb, na, fd, c->d_units + l.off, l.cnt, K, l.p0 == 0 ? c->d_mV : c->d_mY, nullptr,
                                      Dn, l.p0 == 0 ? c->d_mY : c->d_mZ, c->d_part, nullptr, nullptr, nullptr, c->st);
             break;
         }
@@ -1027,12 +1163,12 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
         double *out = bx ? (l.aux == 1 ? nullptr : c->d_mxw) : c->d_mT;
         switch (l.kind) {
         case L_MT:
-            tf::launch_mtile(l.p0, c->mt_ks_t, l.nb, L, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : c->d_mxw,
+            tf::launch_mtile(l.p0, c->mt_ks_t, l.nb, na, L, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : c->d_mxw,
                              bx ? c->d_mT : c->d_mY, Dn, c->d_sidx, out, c->d_part, gbytes, c->d_perm,
                              bx ? x : nullptr, bx ? x2 : nullptr, LT, c->st);
             break;
         case L_MTRED:
-            tf::launch_mtile_red(l.p0, l.nb, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : c->d_mxw,
+            tf::launch_mtile_red(l.p0, l.nb, na, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : c->d_mxw,
                                  bx ? c->d_mT : c->d_mY, Dn, out, c->d_part, c->d_perm, bx ? x : nullptr,
                                  bx ? x2 : nullptr, c->st);
             break;
@@ -1094,8 +1230,7 @@ void dcopy(tf_ctx *c, double *dst, const double *src) {
 // x2, if given, receives a copy of the solution
 void step_solve(tf_ctx *c, int slot, const double *b, double *x, double *x2 = nullptr) {
     if (c->nsrc == 1) {
-        solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, b, x);
-        if (x2) dcopy(c, x2, x);
+        solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, b, x, ctx_work(c), x2);
     } else {
         solve_block(c, c->d_L[slot], c->d_D[slot], c->d_fd, c->nsrc, b, x, x2, c->d_LT[slot]);
     }
@@ -2301,6 +2436,20 @@ int tf_set_tuning(tf_ctx *c, const char *key, int value) {
         } else if (k == "overlap") {
             if (value != 0 && value != 1) return fail(c, TF_ERR_ARG, "overlap must be 0 or 1");
             c->overlap_factor = value != 0;
+        } else if (k == "s_tiles") {
+            if (value < 0 || value > 2) return fail(c, TF_ERR_ARG, "s_tiles must be 0, 1 or 2");
+            c->s_tiles = value;
+        } else if (k == "s_tcols") {
+            if (value != 16 && value != 32) return fail(c, TF_ERR_ARG, "s_tcols must be 16 or 32");
+            c->s_tcols = value;
+        } else if (k == "s_units") {
+            if (value < 64) return fail(c, TF_ERR_ARG, "s_units must be >= 64");
+            c->s_units = value;
+            replan = true;
+        } else if (k == "s_chmin") {
+            if (value < 16 || value % 16) return fail(c, TF_ERR_ARG, "s_chmin must be a positive multiple of 16");
+            c->s_chmin = value;
+            replan = true;
         } else if (k == "indefinite") {
             if (value != 0 && value != 1) return fail(c, TF_ERR_ARG, "indefinite must be 0 or 1");
             c->allow_indefinite = value != 0;
@@ -2317,7 +2466,7 @@ int tf_set_tuning(tf_ctx *c, const char *key, int value) {
             c->d_plist = dupload(c->plan.list);
             c->d_ppref = dupload(c->plan.pref);
             c->d_units = dupload(c->plan.units);
-            c->d_part = dalloc<double>((size_t)std::max<int64_t>(c->plan.part_max, 1) * 4096);
+            c->d_part = dalloc<double>(part_doubles(c->plan));
         }
         return TF_OK;
     });
@@ -2483,14 +2632,16 @@ int tf_solve_noalloc(tf_ctx *c, int slot, const double *b, double *x) {
         }
         const int64_t n = c->sym.n;
         auto up = [](int64_t v) { return (v + 31) / 32 * 32; };  // 256-byte aligned vectors
-        const int64_t need = up(c->vrows) + 3 * up(n);
+        const int64_t nv = up(c->vrows + TF_READ_SLACK), nn = up(n + TF_READ_SLACK);
+        const int64_t npart = up(c->plan.spart_max * 1024);
+        const int64_t need = nv + 3 * nn + npart;
         if (need > TF_NOALLOC_DOUBLES)
             return fail(c, TF_ERR_ARG,
                         "system too large for the allocation-free solve: its work vectors need " +
                             std::to_string(need) + " doubles, the scratch array holds " +
                             std::to_string((int64_t)TF_NOALLOC_DOUBLES) + " (64 MiB)");
         double *p = c->d_scratch;
-        const SolveWork w{p, p + up(c->vrows), p + up(c->vrows) + up(n), p + up(c->vrows) + 2 * up(n)};
+        const SolveWork w{p, p + nv, p + nv + nn, p + nv + 2 * nn, p + nv + 3 * nn};
         const bool prof = c->prof_on;  // no event pool growth inside this call
         c->prof_on = false;
         solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, b, x, w);

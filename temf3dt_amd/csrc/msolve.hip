@@ -216,6 +216,34 @@ __global__ void __launch_bounds__(256) k_mpull(const FrontDesc *fd, const int *l
     }
 }
 
+// The same pull for one right-hand side: a thread per own row, 64 rows per
+// workgroup, the row's Z entries subtracted in gidx order (the order of
+// k_mpull, so one column of a block solve and this give the same bits).
+__global__ void __launch_bounds__(64) k_spull(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
+                                               const double *b, const int *perm, const int *gptr, const int *gidx,
+                                               const double *Z, double *Yin) {
+    const int i = seg_find(pref, cnt, blockIdx.x);
+    const FrontDesc d = fd[list[i]];
+    const int r = (int)(blockIdx.x - pref[i]) * 64 + threadIdx.x;
+    if (r >= d.nc) return;
+    const int row = d.c0 + r;
+    double s = b[perm[row]];
+    const int p1 = gptr[row + 1];
+    int p = gptr[row];
+    for (; p + 8 <= p1; p += 8) {  // eight index loads, then eight values, in flight
+        int g[8];
+        double z[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) g[u] = gidx[p + u];
+#pragma unroll
+        for (int u = 0; u < 8; u++) z[u] = Z[g[u]];
+#pragma unroll
+        for (int u = 0; u < 8; u++) s -= z[u];
+    }
+    for (; p < p1; p++) s -= Z[gidx[p]];
+    Yin[row] = s;
+}
+
 // ---- host launchers --------------------------------------------------------------
 // deterministic right-hand sides for solve timing
 __global__ void k_fill_pattern(int64_t len, double *b) {
@@ -233,6 +261,12 @@ void launch_mpull(const FrontDesc *fd, const int *list, const int64_t *pref, int
     if (nwg > 0)
         hipLaunchKernelGGL(k_mpull, dim3((unsigned)nwg), dim3(256), 0, st, fd, list, pref, cnt, b, perm, gptr, gidx, Z, Yin,
                            K);
+}
+void launch_spull(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg, const double *b,
+                  const int *perm, const int *gptr, const int *gidx, const double *Z, double *Yin, hipStream_t st) {
+    if (nwg > 0)
+        hipLaunchKernelGGL(k_spull, dim3((unsigned)nwg), dim3(64), 0, st, fd, list, pref, cnt, b, perm, gptr, gidx, Z,
+                           Yin);
 }
 void launch_mspmv_rhs(int n, int K, const int *ia, const int *ja, const double *val, const double *e2,
                       const double *e3, double *w, double *r, hipStream_t st) {

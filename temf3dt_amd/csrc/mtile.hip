@@ -36,10 +36,10 @@
 // (no pull launch for the lowest level).
 enum { MT_OWN_FWD = 0, MT_STR_FWD = 1, MT_BWD_T = 2, MT_BWD_X = 3, MT_OWN_FWD_B = 4 };
 
-template <int NB>
-__device__ __forceinline__ void mt_acc_zero(tf_d4 acc[4][NB]) {
+template <int NA, int NB>
+__device__ __forceinline__ void mt_acc_zero(tf_d4 acc[NA][NB]) {
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int a = 0; a < NA; a++)
 #pragma unroll
         for (int b = 0; b < NB; b++) acc[a][b] = (tf_d4){0.0, 0.0, 0.0, 0.0};
 }
@@ -116,9 +116,9 @@ __device__ __forceinline__ double mt_ld(const double *base, uint32_t off) {
 // fetched before the multiply that precedes their stage's loads.  The
 // stages before and after that run (a T-form stage aligned before k0, a
 // partial last stage) clamp the inner index and zero what lies outside.
-template <bool TFORM, bool GATH, int KS, int NB>
+template <bool TFORM, bool GATH, int KS, int NB, int NA>
 __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f, int roff, int o0, int k0, int k1,
-                                        const double *Bm, uint32_t bbytes, const int *bidx, int K, tf_d4 acc[4][NB]) {
+                                        const double *Bm, uint32_t bbytes, const int *bidx, int K, tf_d4 acc[NA][NB]) {
     constexpr int SW = 4 * KS;  // inner values per stage
     if (k0 >= k1) return;
     const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
@@ -140,11 +140,11 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
     const int ja = min(nst, (k0 - kb + SW - 1) / SW);
     const int jb = max(ja, (k1 - kb) / SW);
     auto inner = [&](int ib, int s) { return TFORM ? (ib + KS * lk + s) : (ib + 4 * s + lk); };
-    auto mult = [&](const double xa[KS][NB], const double yb[KS][4]) {
+    auto mult = [&](const double xa[KS][NB], const double yb[KS][NA]) {
 #pragma unroll
         for (int s = 0; s < KS; s++)
 #pragma unroll
-            for (int a = 0; a < 4; a++)
+            for (int a = 0; a < NA; a++)
 #pragma unroll
                 for (int b = 0; b < NB; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s][b], yb[s][a], acc[a][b], 0, 0, 0);
@@ -152,7 +152,7 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
     // a masked stage, loaded and multiplied on its own
     auto masked_stage = [&](int js) {
         const int ib = kb + js * SW;
-        double xa[KS][NB], yb[KS][4];
+        double xa[KS][NB], yb[KS][NA];
 #pragma unroll
         for (int s = 0; s < KS; s++) {
             const int i = inner(ib, s);
@@ -167,7 +167,7 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
                 xa[s][b] = iv ? x : 0.0;
             }
 #pragma unroll
-            for (int a = 0; a < 4; a++) {
+            for (int a = 0; a < NA; a++) {
                 const uint32_t row = (uint32_t)(GATH ? bidx[ic] : ic);
                 const double y = mt_bld(rR, (row * (uint32_t)K + 16u * a + l16) * 8u, 0u);
                 yb[s][a] = iv ? y : 0.0;
@@ -183,7 +183,7 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
 #pragma unroll
             for (int s = 0; s < KS; s++) r[s] = bidx[inner(ib, s)];
         };
-        auto load = [&](int js, const int r[KS], double xa[KS][NB], double yb[KS][4]) {
+        auto load = [&](int js, const int r[KS], double xa[KS][NB], double yb[KS][NA]) {
             const int ib = kb + js * SW;
             if (TFORM && KS % 2 == 0) {  // a lane's KS rows of a column are contiguous: 16-byte loads
 #pragma unroll
@@ -201,15 +201,15 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
                 if (GATH) {
                     const uint32_t vo = ((uint32_t)r[s] * (uint32_t)K) * 8u + ba;
 #pragma unroll
-                    for (int a = 0; a < 4; a++) yb[s][a] = mt_bld(rR, vo + 128u * a, 0u);
+                    for (int a = 0; a < NA; a++) yb[s][a] = mt_bld(rR, vo + 128u * a, 0u);
                 } else {
                     const uint32_t sb = (uint32_t)((TFORM ? ib + s : ib + 4 * s) * K) * 8u;
 #pragma unroll
-                    for (int a = 0; a < 4; a++) yb[s][a] = mt_bld(rR, ba, sb + 128u * a);
+                    for (int a = 0; a < NA; a++) yb[s][a] = mt_bld(rR, ba, sb + 128u * a);
                 }
             }
         };
-        double xa0[KS][NB], yb0[KS][4], xa1[KS][NB], yb1[KS][4];
+        double xa0[KS][NB], yb0[KS][NA], xa1[KS][NB], yb1[KS][NA];
         int rA[KS], rB[KS];
         const int last = jb - 1;
         int js = ja;
@@ -242,7 +242,7 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
     for (int js = jb; js < nst; js++) masked_stage(js);
 }
 
-template <int MODE, int KS, int NB>
+template <int MODE, int KS, int NB, int NA>
 __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDesc *fd, const TileUnit *units,
                                                   int nunits, int K, const double *in0, const double *in1,
                                                   const double *Dn, const int *sidx, double *out, double *part,
@@ -254,8 +254,8 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
     const FrontDesc d = fd[tu.front];
     int nout, roff;
     mt_geom<MODE>(d, nout, roff);
-    tf_d4 acc[4][NB];
-    mt_acc_zero<NB>(acc);
+    tf_d4 acc[NA][NB];
+    mt_acc_zero<NA, NB>(acc);
     const double *Lf = L + d.loff;
     const uint32_t lbytes = (uint32_t)((int64_t)d.nc * d.ld * 8);
     // B operand rows: own right-hand sides (Yin, Y_own, T) at c0, or the
@@ -265,16 +265,16 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
     // the strict triangle of X11 needs no mask: the factorisation leaves the
     // diagonal and upper part of L11's storage zero
     if (MODE == MT_OWN_FWD || MODE == MT_STR_FWD)
-        mt_wave<false, false, KS, NB>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
+        mt_wave<false, false, KS, NB, NA>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
     else if (MODE == MT_OWN_FWD_B)
-        mt_wave<false, true, KS, NB>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, in0, gbytes, xo.perm + d.c0, K, acc);
+        mt_wave<false, true, KS, NB, NA>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, in0, gbytes, xo.perm + d.c0, K, acc);
     else if (MODE == MT_BWD_T && LT)  // L21 read from its row-major copy: N form over struct rows
-        mt_wave<false, true, KS, NB>(LT + d.toff, (uint32_t)((int64_t)d.m * d.ldt * 8), d.ldt, 0, tu.t0, tu.k0,
+        mt_wave<false, true, KS, NB, NA>(LT + d.toff, (uint32_t)((int64_t)d.m * d.ldt * 8), d.ldt, 0, tu.t0, tu.k0,
                                      tu.k1, in0, gbytes, sidx + d.soff, K, acc);
     else if (MODE == MT_BWD_T)
-        mt_wave<true, true, KS, NB>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, in0, gbytes, sidx + d.soff, K, acc);
+        mt_wave<true, true, KS, NB, NA>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, in0, gbytes, sidx + d.soff, K, acc);
     else
-        mt_wave<true, false, KS, NB>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
+        mt_wave<true, false, KS, NB, NA>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
     const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
     if (tu.slot < 0) {
 #pragma unroll
@@ -284,40 +284,166 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
                 const int o = tu.t0 + 16 * b + 4 * r + lk;
                 if (o < 0 || o >= nout) continue;
 #pragma unroll
-                for (int a = 0; a < 4; a++) {
+                for (int a = 0; a < NA; a++) {
                     const int q = 16 * a + l16;
                     if (q < K) mt_store<MODE>(d, o, q, acc[a][b][r], K, in0, in1, Dn, out, xo);
                 }
             }
     } else {
-        double *pp = part + (int64_t)tu.slot * 4096;
+        // partial tile: 64 outputs x 16 NA right-hand sides
+        double *pp = part + (int64_t)tu.slot * (64 * 16 * NA);
 #pragma unroll
         for (int b = 0; b < NB; b++)
 #pragma unroll
             for (int r = 0; r < 4; r++)
 #pragma unroll
-                for (int a = 0; a < 4; a++) pp[(16 * b + 4 * r + lk) * 64 + 16 * a + l16] = acc[a][b][r];
+                for (int a = 0; a < NA; a++) pp[(16 * b + 4 * r + lk) * (16 * NA) + 16 * a + l16] = acc[a][b][r];
     }
 }
 
 // Sum of a tile's chunk partials (slot0 .. slot0+nk-1, in order) + epilogue.
-// Reduction units reuse TileUnit: front, t0, k0 = slot0, k1 = nk.  Sixteen
+// Reduction units reuse TileUnit: front, t0, k0 = slot0, k1 = nk.  A partial
+// tile holds 64 outputs x PS right-hand sides (PS = 16 NA); 64 PS / 256
 // workgroups per tile, one element per thread.
-template <int MODE>
+template <int MODE, int PS>
 __global__ void __launch_bounds__(256) k_mtile_red(const FrontDesc *fd, const TileUnit *units, int K,
                                                    const double *in0, const double *in1, const double *Dn,
                                                    double *out, const double *part, MtEdgeOut xo, int nb) {
-    const TileUnit tu = units[blockIdx.x >> 4];
+    constexpr int WPT = 64 * PS / 256;  // workgroups per tile
+    const TileUnit tu = units[blockIdx.x / WPT];
     const FrontDesc d = fd[tu.front];
     int nout, roff;
     mt_geom<MODE>(d, nout, roff);
-    const int e = (blockIdx.x & 15) * 256 + threadIdx.x;
-    const int o = tu.t0 + (e >> 6), q = e & 63;
-    if ((e >> 6) >= 16 * nb || o < 0 || o >= nout || q >= K) return;
-    const double *p0 = part + (int64_t)tu.k0 * 4096 + e;
+    const int e = (blockIdx.x % WPT) * 256 + threadIdx.x;
+    const int o = tu.t0 + e / PS, q = e % PS;
+    if (e / PS >= 16 * nb || o < 0 || o >= nout || q >= K) return;
+    const double *p0 = part + (int64_t)tu.k0 * (64 * PS) + e;
     double s = 0.0;
-    for (int c = 0; c < tu.k1; c++) s += p0[(int64_t)c * 4096];
+    for (int c = 0; c < tu.k1; c++) s += p0[(int64_t)c * (64 * PS)];
     mt_store<MODE>(d, o, q, s, K, in0, in1, Dn, out, xo);
+}
+
+// ---- one right-hand side on the vector ALUs ------------------------------------------
+// The same work units, products and epilogues as k_mtile with K = 1, as
+// matrix-vector tiles (one wave per unit, 16-byte loads, no LDS, no
+// barriers).  N form (own and struct forward): a tile of 64 output rows; a
+// lane reads two consecutive rows of one column per load, lanes 0-31 the even
+// columns of a step and lanes 32-63 the odd ones, which are folded together
+// at the end.  T form (backward T and X): a tile of 64 output columns, taken
+// 16 at a time; a lane reads two consecutive rows of each of the 16 columns
+// (128 rows per block), and the 16 column sums are reduced across the wave by
+// butterfly shuffles.  Every matrix value outside the unit's inner range is
+// zeroed before use; per-lane offsets past the L block read zero (buffer
+// range check), so columns past the front read nothing.
+template <int MODE>
+__device__ __forceinline__ double st_b(const FrontDesc &d, int k, const double *in0, const int *sidx, const int *perm) {
+    if (MODE == MT_OWN_FWD_B) return in0[perm[d.c0 + k]];
+    if (MODE == MT_BWD_T) return in0[sidx[d.soff + k]];
+    return in0[d.c0 + k];
+}
+
+template <int MODE>
+__device__ __forceinline__ void st_out(const FrontDesc &d, const TileUnit &tu, int o, double s, int nout,
+                                       const double *in0, const double *in1, const double *Dn, double *out,
+                                       double *part, const MtEdgeOut &xo) {
+    if (o < 0 || o >= nout) return;
+    if (tu.slot < 0) mt_store<MODE>(d, o, 0, s, 1, in0, in1, Dn, out, xo);
+    else part[(int64_t)tu.slot * 1024 + (o - tu.t0) * 16] = s;  // partial tile layout of NA = 1
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_stile_n(const double *L, const FrontDesc *fd, const TileUnit *units,
+                                                 int nunits, const double *in0, double *out, double *part,
+                                                 const int *perm) {
+    constexpr int U = 8;  // column pairs per step
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int u = xcd_block(blockIdx.x, gridDim.x) * 4 + w;
+    if (u >= nunits) return;
+    const TileUnit tu = units[u];
+    const FrontDesc d = fd[tu.front];
+    int nout, roff;
+    mt_geom<MODE>(d, nout, roff);
+    const int lane = threadIdx.x & 63, cp = lane >> 5;
+    const int o = tu.t0 + 2 * (lane & 31);  // rows o, o + 1 (of the block's rows from roff)
+    const __amdgpu_buffer_rsrc_t rL = mt_rsrc(L + d.loff, (uint32_t)((int64_t)d.nc * d.ld * 8));
+    const MtEdgeOut xo{perm, nullptr, nullptr};
+    double acc0 = 0.0, acc1 = 0.0;
+    for (int k = tu.k0; k < tu.k1; k += 2 * U) {
+        double x0[U], x1[U], bv[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            const int kk = k + 2 * q + cp;
+            mt_bld2(rL, (uint32_t)(kk * d.ld + roff + o) * 8u, 0u, x0[q], x1[q]);
+            bv[q] = st_b<MODE>(d, min(kk, tu.k1 - 1), in0, nullptr, perm);
+        }
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            const bool kv = k + 2 * q + cp < tu.k1;
+            const double b = kv ? bv[q] : 0.0;
+            acc0 += (kv ? x0[q] : 0.0) * b;
+            acc1 += (kv ? x1[q] : 0.0) * b;
+        }
+    }
+    acc0 += __shfl_xor(acc0, 32, 64);
+    acc1 += __shfl_xor(acc1, 32, 64);
+    if (cp == 0) {
+        st_out<MODE>(d, tu, o, acc0, nout, in0, nullptr, nullptr, out, part, xo);
+        st_out<MODE>(d, tu, o + 1, acc1, nout, in0, nullptr, nullptr, out, part, xo);
+    }
+}
+
+template <int MODE, int NC>
+__global__ void __launch_bounds__(256) k_stile_t(const double *L, const FrontDesc *fd, const TileUnit *units,
+                                                 int nunits, const double *in0, const double *in1, const double *Dn,
+                                                 const int *sidx, double *out, double *part, MtEdgeOut xo) {
+    // NC output columns per pass (16 or 32)
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int u = xcd_block(blockIdx.x, gridDim.x) * 4 + w;
+    if (u >= nunits) return;
+    const TileUnit tu = units[u];
+    const FrontDesc d = fd[tu.front];
+    int nout, roff;
+    mt_geom<MODE>(d, nout, roff);
+    const int lane = threadIdx.x & 63;
+    const __amdgpu_buffer_rsrc_t rL = mt_rsrc(L + d.loff, (uint32_t)((int64_t)d.nc * d.ld * 8));
+    // row blocks of 128 start where roff + row is even (16-byte loads)
+    const int kb0 = tu.k0 - ((roff + tu.k0) & 1);
+    for (int j0 = tu.t0; j0 < tu.t0 + 64 && j0 < nout; j0 += NC) {
+        double acc[NC];
+#pragma unroll
+        for (int j = 0; j < NC; j++) acc[j] = 0.0;
+        for (int kb = kb0; kb < tu.k1; kb += 128) {
+            const int r0 = kb + 2 * lane;
+            const bool v0 = r0 >= tu.k0 && r0 < tu.k1, v1 = r0 + 1 >= tu.k0 && r0 + 1 < tu.k1;
+            const double b0 = v0 ? st_b<MODE>(d, r0, in0, sidx, nullptr) : 0.0;
+            const double b1 = v1 ? st_b<MODE>(d, r0 + 1, in0, sidx, nullptr) : 0.0;
+            double x0[NC], x1[NC];
+#pragma unroll
+            for (int j = 0; j < NC; j++)
+                mt_bld2(rL, (uint32_t)((int64_t)(j0 + j) * d.ld + roff + r0) * 8u, 0u, x0[j], x1[j]);
+#pragma unroll
+            for (int j = 0; j < NC; j++) acc[j] += (v0 ? x0[j] : 0.0) * b0 + (v1 ? x1[j] : 0.0) * b1;
+        }
+        // sum over the 64 lanes: fold lanes 32 (and for NC = 16 also 16)
+        // apart, then halve the set of columns a lane keeps at each of the
+        // distances NC/2, ..., 2, 1; lane l ends with column j0 + (l % NC)
+#pragma unroll
+        for (int j = 0; j < NC; j++) {
+            acc[j] += __shfl_xor(acc[j], 32, 64);
+            if (NC == 16) acc[j] += __shfl_xor(acc[j], 16, 64);
+        }
+#pragma unroll
+        for (int dist = NC / 2; dist >= 1; dist >>= 1) {
+            const bool up = (lane & dist) != 0;
+#pragma unroll
+            for (int j = 0; j < dist; j++) {
+                const double keep = up ? acc[j + dist] : acc[j];
+                const double give = up ? acc[j] : acc[j + dist];
+                acc[j] = keep + __shfl_xor(give, dist, 64);
+            }
+        }
+        if (lane < NC) st_out<MODE>(d, tu, j0 + lane, acc[0], nout, in0, in1, Dn, out, part, xo);
+    }
 }
 
 namespace tf {
@@ -326,16 +452,20 @@ namespace tf {
 // gbytes: size of the gathered vector (backward T form), < 4 GiB
 // perm, x1, x2: backward X also writes its rows to x1 (and x2) in edge order
 // LT: row-major L21 copies for the backward T product (nullptr: read L21 itself)
-void launch_mtile(int mode, int ks, int nb, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits,
+void launch_mtile(int mode, int ks, int nb, int na, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits,
                   int K, const double *in0, const double *in1, const double *Dn, const int *sidx, double *out,
                   double *part, uint32_t gbytes, const int *perm, double *x1, double *x2, const double *LT,
                   hipStream_t st) {
     if (nunits <= 0) return;
     const dim3 g((nunits + 3) / 4), b(256);
     const MtEdgeOut xo{perm, x1, x2};
-#define TF_MT(M, S, N)                                                                                            \
-    hipLaunchKernelGGL((k_mtile<M, S, N>), g, b, 0, st, L, fd, units, nunits, K, in0, in1, Dn, sidx, out, part, \
-                       gbytes, xo, LT)
+#define TF_MT(M, S, N)                                                                                     \
+    if (na == 1)                                                                                           \
+        hipLaunchKernelGGL((k_mtile<M, S, N, 1>), g, b, 0, st, L, fd, units, nunits, K, in0, in1, Dn, sidx, \
+                           out, part, gbytes, xo, LT);                                                     \
+    else                                                                                                   \
+        hipLaunchKernelGGL((k_mtile<M, S, N, 4>), g, b, 0, st, L, fd, units, nunits, K, in0, in1, Dn, sidx, \
+                           out, part, gbytes, xo, LT)
 #define TF_MT_MODES(S, N)                                        \
     switch (mode) {                                              \
     case MT_OWN_FWD: TF_MT(MT_OWN_FWD, S, N); break;             \
@@ -352,18 +482,80 @@ void launch_mtile(int mode, int ks, int nb, const double *L, const FrontDesc *fd
 #undef TF_MT_MODES
 #undef TF_MT
 }
-void launch_mtile_red(int mode, int nb, const FrontDesc *fd, const TileUnit *units, int nunits, int K,
+template <int PS>
+void launch_red_ps(int mode, int nb, const FrontDesc *fd, const TileUnit *units, int nunits, int K, const double *in0,
+                   const double *in1, const double *Dn, double *out, const double *part, const MtEdgeOut &xo,
+                   hipStream_t st) {
+    const dim3 g(nunits * (64 * PS / 256)), b(256);
+    switch (mode) {
+    case MT_OWN_FWD: hipLaunchKernelGGL((k_mtile_red<MT_OWN_FWD, PS>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
+    case MT_OWN_FWD_B: hipLaunchKernelGGL((k_mtile_red<MT_OWN_FWD_B, PS>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
+    case MT_STR_FWD: hipLaunchKernelGGL((k_mtile_red<MT_STR_FWD, PS>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
+    case MT_BWD_T: hipLaunchKernelGGL((k_mtile_red<MT_BWD_T, PS>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
+    default: hipLaunchKernelGGL((k_mtile_red<MT_BWD_X, PS>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
+    }
+}
+void launch_stile(int mode, int nc, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits,
+                  const double *in0, const double *in1, const double *Dn, const int *sidx, double *out, double *part,
+                  const int *perm, double *x1, double *x2, hipStream_t st) {
+    if (nunits <= 0) return;
+    const dim3 g((nunits + 3) / 4), b(256);
+    const MtEdgeOut xo{perm, x1, x2};
+    switch (mode) {
+    case MT_OWN_FWD: hipLaunchKernelGGL(k_stile_n<MT_OWN_FWD>, g, b, 0, st, L, fd, units, nunits, in0, out, part, perm); break;
+    case MT_OWN_FWD_B: hipLaunchKernelGGL(k_stile_n<MT_OWN_FWD_B>, g, b, 0, st, L, fd, units, nunits, in0, out, part, perm); break;
+    case MT_STR_FWD: hipLaunchKernelGGL(k_stile_n<MT_STR_FWD>, g, b, 0, st, L, fd, units, nunits, in0, out, part, perm); break;
+    case MT_BWD_T:
+        if (nc == 32) hipLaunchKernelGGL((k_stile_t<MT_BWD_T, 32>), g, b, 0, st, L, fd, units, nunits, in0, in1, Dn, sidx, out, part, xo);
+        else hipLaunchKernelGGL((k_stile_t<MT_BWD_T, 16>), g, b, 0, st, L, fd, units, nunits, in0, in1, Dn, sidx, out, part, xo);
+        break;
+    default:
+        if (nc == 32) hipLaunchKernelGGL((k_stile_t<MT_BWD_X, 32>), g, b, 0, st, L, fd, units, nunits, in0, in1, Dn, sidx, out, part, xo);
+        else hipLaunchKernelGGL((k_stile_t<MT_BWD_X, 16>), g, b, 0, st, L, fd, units, nunits, in0, in1, Dn, sidx, out, part, xo);
+        break;
+    }
+}
+// one right-hand side: a thread per output of each reduced tile (64 per
+// tile, four tiles per workgroup), the partials summed in chunk order
+template <int MODE>
+__global__ void __launch_bounds__(256) k_stile_red(const FrontDesc *fd, const TileUnit *units, int nunits,
+                                                   const double *in0, const double *in1, const double *Dn, double *out,
+                                                   const double *part, MtEdgeOut xo) {
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= nunits) return;
+    const TileUnit tu = units[t];
+    const FrontDesc d = fd[tu.front];
+    int nout, roff;
+    mt_geom<MODE>(d, nout, roff);
+    const int e = threadIdx.x & 63, o = tu.t0 + e;
+    if (o < 0 || o >= nout) return;
+    const double *p0 = part + (int64_t)tu.k0 * 1024 + e * 16;
+    double s = 0.0;
+    for (int c = 0; c < tu.k1; c++) s += p0[(int64_t)c * 1024];
+    mt_store<MODE>(d, o, 0, s, 1, in0, in1, Dn, out, xo);
+}
+void launch_stile_red(int mode, const FrontDesc *fd, const TileUnit *units, int nunits, const double *in0,
+                      const double *in1, const double *Dn, double *out, const double *part, const int *perm,
+                      double *x1, double *x2, hipStream_t st) {
+    if (nunits <= 0) return;
+    const dim3 g((nunits + 3) / 4), b(256);
+    const MtEdgeOut xo{perm, x1, x2};
+    switch (mode) {
+    case MT_OWN_FWD: hipLaunchKernelGGL(k_stile_red<MT_OWN_FWD>, g, b, 0, st, fd, units, nunits, in0, in1, Dn, out, part, xo); break;
+    case MT_OWN_FWD_B: hipLaunchKernelGGL(k_stile_red<MT_OWN_FWD_B>, g, b, 0, st, fd, units, nunits, in0, in1, Dn, out, part, xo); break;
+    case MT_STR_FWD: hipLaunchKernelGGL(k_stile_red<MT_STR_FWD>, g, b, 0, st, fd, units, nunits, in0, in1, Dn, out, part, xo); break;
+    case MT_BWD_T: hipLaunchKernelGGL(k_stile_red<MT_BWD_T>, g, b, 0, st, fd, units, nunits, in0, in1, Dn, out, part, xo); break;
+    default: hipLaunchKernelGGL(k_stile_red<MT_BWD_X>, g, b, 0, st, fd, units, nunits, in0, in1, Dn, out, part, xo); break;
+    }
+}
+void launch_mtile_red(int mode, int nb, int na, const FrontDesc *fd, const TileUnit *units, int nunits, int K,
                       const double *in0, const double *in1, const double *Dn, double *out, const double *part,
                       const int *perm, double *x1, double *x2, hipStream_t st) {
     if (nunits <= 0) return;
-    const dim3 g(nunits * 16), b(256);
     const MtEdgeOut xo{perm, x1, x2};
-    switch (mode) {
-    case MT_OWN_FWD: hipLaunchKernelGGL(k_mtile_red<MT_OWN_FWD>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
-    case MT_OWN_FWD_B: hipLaunchKernelGGL(k_mtile_red<MT_OWN_FWD_B>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
-    case MT_STR_FWD: hipLaunchKernelGGL(k_mtile_red<MT_STR_FWD>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
-    case MT_BWD_T: hipLaunchKernelGGL(k_mtile_red<MT_BWD_T>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
-    default: hipLaunchKernelGGL(k_mtile_red<MT_BWD_X>, g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
-    }
+    if (na == 1)
+        launch_red_ps<16>(mode, nb, fd, units, nunits, K, in0, in1, Dn, out, part, xo, st);
+    else
+        launch_red_ps<64>(mode, nb, fd, units, nunits, K, in0, in1, Dn, out, part, xo, st);
 }
 }  // namespace tf

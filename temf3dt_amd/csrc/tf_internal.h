@@ -59,13 +59,14 @@ namespace tf {
 // column stride of a front's L block: rows rounded up to 16 doubles, so that
 // with 16-aligned offsets every column starts on a 128-byte line
 inline int64_t front_ld(int64_t f) { return (f + 15) / 16 * 16; }
-// The factor and block-solve kernels read a front's L block (nc columns of
-// stride ld) and the row-major copy of its L21 (m rows of stride nc rounded
-// up to 16) through buffer resources with 32-bit byte sizes and offsets:
-// both must stay below 4 GiB.
+// The factor and solve kernels read a front's L block (nc columns of stride
+// ld) and the row-major copy of its L21 (m rows of stride nc rounded up to
+// 16) through buffer resources with 32-bit byte sizes and offsets; a tile
+// may address up to 64 columns past the block (those reads return zero), so
+// (nc + 64) columns and the copy must stay below 4 GiB.
 inline bool front_fits_32(int64_t nc, int64_t m) {
     const int64_t lim = int64_t(1) << 32;
-    return nc * front_ld(nc + m) * 8 < lim && m * ((nc + 15) / 16 * 16) * 8 < lim;
+    return (nc + 64) * front_ld(nc + m) * 8 < lim && m * ((nc + 15) / 16 * 16) * 8 < lim;
 }
 // topology.cpp
 void build_topology(Topology &t, int nN, const double *xyz, int nT, const int *tet_vex);

@@ -109,3 +109,29 @@ def test_run_matches_oracle_rows(gpu_mod_big, src_mod_big):
     ck_rel = np.abs(ck["err"] - g["chk_err"]) / g["chk_err"]
     print(f"[gpu] check error max rel diff {ck_rel.max():.3e}", flush=True)
     assert ck_rel.max() < 1e-3
+
+
+def test_single_solve_tile_and_matrix_vector_paths_agree(gpu_mod_big):
+    """The single right-hand-side solve through the vector tile kernels
+    (default, tuning key s_tiles = 2), the matrix-core tile kernels (1) and
+    the matrix-vector kernels (0) on one factor: the same solution up to
+    summation order, for several launch sizings of the tiles."""
+    fe, _, _ = gpu_mod_big
+    fe.factor(0, 1e-7)
+    b = np.random.default_rng(17).standard_normal(fe.nE)
+    fe.set_tuning("s_tiles", 0)
+    x0 = fe.solve(0, b)
+    for kind in (1, 2):  # matrix-core tiles, vector tiles
+        fe.set_tuning("s_tiles", kind)
+        for units, chmin in [(8192, 256), (64, 16), (100000, 4096)]:
+            fe.set_tuning("s_units", units)
+            fe.set_tuning("s_chmin", chmin)
+            x1 = fe.solve(0, b)
+            rel = np.abs(x1 - x0).max() / np.abs(x0).max()
+            print(f"[gpu] single solve tiles {kind} (s_units {units}, s_chmin {chmin}) vs matrix-vector: {rel:.3e}",
+                  flush=True)
+            assert rel < 1e-12
+    fe.set_tuning("s_tiles", 2)
+    fe.set_tuning("s_units", 8192)
+    fe.set_tuning("s_chmin", 256)
+    fe.release(0)

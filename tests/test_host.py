@@ -166,19 +166,20 @@ def test_plain_3d_workload_size():
 
 
 def test_front_size_guard_of_32_bit_kernel_addressing():
-    """Fronts whose L block (nc columns of stride f rounded up to 16) or
-    row-major L21 copy reach 4 GiB are rejected by the analysis; the
-    kernels address both with 32-bit byte offsets."""
+    """Fronts whose L block (nc columns of stride f rounded up to 16, plus
+    the 64 columns a tile may address past it) or row-major L21 copy reach
+    4 GiB are rejected by the analysis; the kernels address both with 32-bit
+    byte offsets."""
     H = _host()
     H.tfh_front_fits.argtypes = [ctypes.c_longlong, ctypes.c_longlong]
     # the largest front of the 1.35M-tet batched mesh (10,167 columns) fits
     assert H.tfh_front_fits(10167, 3000) == 1
-    # 23,170 columns: 23,170 * 23,184 * 8 B = 4.297e9 B >= 2^32
+    # 23,170 columns: (23,170 + 64) * 23,184 * 8 B = 4.309e9 B >= 2^32
     assert H.tfh_front_fits(23170, 0) == 0
-    assert H.tfh_front_fits(23168, 0) == 1  # 23,168^2 * 8 = 4.294e9 < 2^32
+    assert H.tfh_front_fits(23104, 0) == 1  # (23,104 + 64) * 23,104 * 8 = 4.282e9 < 2^32
     # a narrow front with a long struct: the L block fits, the copy does not
     assert H.tfh_front_fits(16, 16 * 1024 * 1024 * 2) == 0
     ld = lambda f: (f + 15) // 16 * 16  # noqa: E731
     for nc, m in [(1, 0), (64, 64), (5000, 9000), (20000, 5000), (1000, 500000)]:
-        want = nc * ld(nc + m) * 8 < 2 ** 32 and m * ld(nc) * 8 < 2 ** 32
+        want = (nc + 64) * ld(nc + m) * 8 < 2 ** 32 and m * ld(nc) * 8 < 2 ** 32
         assert H.tfh_front_fits(nc, m) == int(want), (nc, m)
