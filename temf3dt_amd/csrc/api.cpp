@@ -109,7 +109,7 @@ void launch_mtile_red(int mode, int nb, int na, const FrontDesc *fd, const TileU
                       const double *in0, const double *in1, const double *Dn, double *out, const double *part,
                       const int *perm, double *x1, double *x2, hipStream_t st);
 void launch_mspmv_rhs(int n, int K, const int *ia, const int *ja, const double *val, const double *e2,
-                      const double *e3, double *w, double *r, hipStream_t st);
+                      const double *e3, double *w, double *r, int xcd, hipStream_t st);
 void launch_msource(int ns, int K, const int *edge, const int *src, const double *w, double didt, double cdt,
                     double *r, hipStream_t st);
 void launch_mreadout(int nR, int K, const int *rec_edge, const double *rec_drc, const double *n_arr,
@@ -205,6 +205,8 @@ struct Plan {
     int64_t spart_max = 0;      // the same for the single right-hand-side plan (1024 doubles each)
     std::vector<TileUnit> units;  // block-solve tile units (L_MT / L_MTRED launches)
 };
+
+void free_native(tf_ctx *c);
 
 // partial-sum storage for both plans' split tiles
 size_t part_doubles(const Plan &p) {
@@ -305,6 +307,31 @@ struct tf_ctx {
     // through the matrix-vector kernels of solve.hip; launch sizing of its tiles
     int s_tiles = 2;
     int s_units = 8192, s_chmin = 512;
+    int spmv_xcd = 1;  // block SpMV: contiguous row ranges per XCD
+    // The time loops (tf_run, tf_run_models, tf_adjoint) keep their vectors
+    // in the factor's column order ("native" order: row i is edge perm[i]),
+    // so the solves read b and write x in place instead of through perm, and
+    // the SpMV's neighbouring rows gather neighbouring rows (the nested
+    // dissection keeps subdomains contiguous).  The edge-indexed maps of the
+    // step (full CSR of Msigma, source and receiver edges, tet edges, model
+    // ranges) are translated once per analysis (tuning key "native").
+    bool native = true, native_ok = false;
+    int *d_ident = nullptr;
+    int *d_nfia = nullptr, *d_nfja = nullptr;
+    int64_t *d_nfmap = nullptr;
+    double *d_nfval = nullptr;
+    int64_t nfnnz = 0;
+    int *d_nsrc_edge = nullptr, *d_nrec_edge = nullptr, *d_ntet_edg = nullptr, *d_nmrange = nullptr;
+    std::vector<int> nm_e0;
+    // maps of the loop being run, and the permutation the solves map b and x
+    // through (nullptr: perm, edge-order vectors)
+    struct RunMaps {
+        const int *fia, *fja;
+        const double *fval;
+        const int *src_edge, *rec_edge, *tet_edg, *mrange, *perm;
+        const int *e0;  // host: first row of each model
+    } rm{};
+    const int *solve_perm = nullptr;
     int s_tcols = 32;  // output columns per pass of the vector T-form tiles (16 or 32)
     double *d_U = nullptr, *d_V = nullptr, *d_xw = nullptr, *d_Y = nullptr, *d_T = nullptr, *d_W = nullptr;
     int64_t *d_woff = nullptr;
@@ -430,7 +457,15 @@ void free_slices(tf_ctx *c) {
     c->slice_nT = 0;
 }
 
+void free_native(tf_ctx *c) {
+    dfree(c->d_ident); dfree(c->d_nfia); dfree(c->d_nfja); dfree(c->d_nfmap); dfree(c->d_nfval);
+    dfree(c->d_nsrc_edge); dfree(c->d_nrec_edge); dfree(c->d_ntet_edg); dfree(c->d_nmrange);
+    c->nm_e0.clear();
+    c->native_ok = false;
+}
+
 void free_all(tf_ctx *c) {
+    free_native(c);
     free_slices(c);
     free_models(c);
     free_solver(c);
@@ -790,6 +825,7 @@ void build_plan(tf_ctx *c) {
 
 void do_analyze(tf_ctx *c, int leaf) {
     free_solver(c);
+    free_native(c);
     auto t0 = std::chrono::steady_clock::now();
     tf::analyze(c->sym, c->topo, leaf);
     Symbolic &s = c->sym;
@@ -868,6 +904,109 @@ void do_analyze(tf_ctx *c, int leaf) {
     c->analyzed = true;
     c->timers[6] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
+
+// The step's edge-indexed maps in native order (see tf_ctx::native).
+void ensure_native(tf_ctx *c) {
+    if (c->native_ok) return;
+    const Topology &t = c->topo;
+    const Symbolic &s = c->sym;
+    const int n = t.nE;
+    const std::vector<int> &iperm = s.iperm;
+    std::vector<int> ident(n);
+    for (int i = 0; i < n; i++) ident[i] = i;
+    // full symmetric pattern of Msigma with rows and columns renumbered,
+    // columns sorted within each row; nfmap points at the upper-CSR value
+    std::vector<int> deg(n, 0);
+    for (int r = 0; r < n; r++)
+        for (int q = t.ia[r]; q < t.ia[r + 1]; q++) {
+            deg[iperm[r]]++;
+            if (t.ja[q] != r) deg[iperm[t.ja[q]]]++;
+        }
+    std::vector<int> fia(n + 1, 0);
+    for (int i = 0; i < n; i++) fia[i + 1] = fia[i] + deg[i];
+    std::vector<std::pair<int, int64_t>> ent(fia[n]);
+    std::fill(deg.begin(), deg.end(), 0);
+    for (int r = 0; r < n; r++)
+        for (int q = t.ia[r]; q < t.ia[r + 1]; q++) {
+            const int a = iperm[r], b = iperm[t.ja[q]];
+            ent[fia[a] + deg[a]++] = {b, q};
+            if (a != b) ent[fia[b] + deg[b]++] = {a, q};
+        }
+    std::vector<int> fja(fia[n]);
+    std::vector<int64_t> fmap(fia[n]);
+    for (int i = 0; i < n; i++) {
+        std::sort(ent.begin() + fia[i], ent.begin() + fia[i + 1]);
+        for (int p = fia[i]; p < fia[i + 1]; p++) { fja[p] = ent[p].first; fmap[p] = ent[p].second; }
+    }
+    free_native(c);
+    c->d_ident = dupload(ident);
+    c->d_nfia = dupload(fia);
+    c->d_nfja = dupload(fja);
+    c->d_nfmap = dupload(fmap);
+    c->nfnnz = fia[n];
+    c->d_nfval = dalloc<double>(std::max<int64_t>(c->nfnnz, 1));
+    if (c->assembled) tf::launch_gather_vals(c->nfnnz, c->d_nfmap, c->d_M, c->d_nfval, c->st);
+    if (!c->src_edge.empty()) {
+        std::vector<int> se(c->src_edge.size());
+        for (size_t i = 0; i < se.size(); i++) se[i] = iperm[c->src_edge[i]];
+        c->d_nsrc_edge = dupload(se);
+    }
+    if (c->nR > 0) {
+        std::vector<int> re(6 * (size_t)c->nR);
+        for (int i = 0; i < c->nR; i++)
+            for (int j = 0; j < 6; j++) re[6 * i + j] = iperm[t.tet_edg[6 * (size_t)(c->rec_tet[i] - 1) + j]];
+        c->d_nrec_edge = dupload(re);
+    }
+    {
+        std::vector<int> te(t.tet_edg.size());
+        for (size_t i = 0; i < te.size(); i++) te[i] = iperm[t.tet_edg[i]];
+        c->d_ntet_edg = dupload(te);
+    }
+    if (c->nmodels > 0) {  // each model's fronts are one run of the postorder: one range of rows
+        std::vector<int> rg(2 * (size_t)c->nmodels);
+        c->nm_e0.assign(c->nmodels, 0);
+        for (int m = 0; m < c->nmodels; m++) {
+            int lo = n, hi = -1;
+            for (int e = c->m_e0[m]; e < c->m_e0[m] + c->m_ne[m]; e++) {
+                lo = std::min(lo, iperm[e]);
+                hi = std::max(hi, iperm[e]);
+            }
+            if (hi - lo + 1 != c->m_ne[m]) throw std::runtime_error("model rows are not contiguous in factor order");
+            c->nm_e0[m] = lo;
+            rg[2 * m] = lo;
+            rg[2 * m + 1] = lo + c->m_ne[m];
+        }
+        c->d_nmrange = dupload(rg);
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->native_ok = true;
+}
+
+// the maps a time loop runs with, and its solves' permutation of b and x
+tf_ctx::RunMaps run_maps(tf_ctx *c) {
+    if (!c->d_tet_edg) c->d_tet_edg = dupload(c->topo.tet_edg);
+    if (c->native) {
+        ensure_native(c);
+        return tf_ctx::RunMaps{c->d_nfia, c->d_nfja, c->d_nfval, c->d_nsrc_edge, c->d_nrec_edge, c->d_ntet_edg,
+                               c->d_nmrange, c->d_ident, c->nm_e0.data()};
+    }
+    return tf_ctx::RunMaps{c->d_fia, c->d_fja, c->d_fval, c->d_src_edge, c->d_rec_edge, c->d_tet_edg,
+                           c->d_mrange, c->d_perm, c->m_e0.data()};
+}
+
+// Sets a time loop's maps for the lifetime of the guard: the step kernels
+// index through c->rm, the solves map b and x through c->solve_perm.
+struct RunScope {
+    tf_ctx *c;
+    explicit RunScope(tf_ctx *cc) : c(cc) {
+        c->rm = run_maps(c);
+        c->solve_perm = c->rm.perm;
+    }
+    ~RunScope() { c->solve_perm = nullptr; }
+};
+
+// the permutation the solves map b and x through
+inline const int *sperm(const tf_ctx *c) { return c->solve_perm ? c->solve_perm : c->d_perm; }
 
 // numeric factorisation of the values already scattered into slot's L
 // numeric factorisation into L / Dn, fronts addressed through fd
@@ -994,25 +1133,25 @@ void solve_dev(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd
         double *out = l.p0 == 1 ? Z : w.Y;
         switch (l.kind) {
         case L_SPULL:
-            tf::launch_spull(fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, b, c->d_perm, c->d_gptr, c->d_gidx, Z, Yin,
+            tf::launch_spull(fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, b, sperm(c), c->d_gptr, c->d_gidx, Z, Yin,
                              c->st);
             break;
         case L_ST:
             if (c->s_tiles == 2)
                 tf::launch_stile(l.p0, c->s_tcols, L, fd, c->d_units + l.off, l.cnt, in0, nullptr, Dn, nullptr, out,
-                                 w.part, c->d_perm, nullptr, nullptr, c->st);
+                                 w.part, sperm(c), nullptr, nullptr, c->st);
             else
                 tf::launch_mtile(l.p0, c->mt_ks_n, 4, 1, L, fd, c->d_units + l.off, l.cnt, 1, in0, nullptr, Dn, nullptr,
-                                 out, w.part, fromb ? gbytes : 0u, fromb ? c->d_perm : nullptr, nullptr, nullptr,
+                                 out, w.part, fromb ? gbytes : 0u, fromb ? sperm(c) : nullptr, nullptr, nullptr,
                                  nullptr, c->st);
             break;
         case L_STRED:
             if (c->s_tiles == 2)
                 tf::launch_stile_red(l.p0, fd, c->d_units + l.off, l.cnt, in0, nullptr, Dn, out, w.part,
-                                     fromb ? c->d_perm : nullptr, nullptr, nullptr, c->st);
+                                     fromb ? sperm(c) : nullptr, nullptr, nullptr, c->st);
             else
                 tf::launch_mtile_red(l.p0, 4, 1, fd, c->d_units + l.off, l.cnt, 1, in0, nullptr, Dn, out, w.part,
-                                     fromb ? c->d_perm : nullptr, nullptr, nullptr, c->st);
+                                     fromb ? sperm(c) : nullptr, nullptr, nullptr, c->st);
             break;
         }
         prof_close(c, prof_category(l.kind), pm);
@@ -1025,19 +1164,19 @@ void solve_dev(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd
         case L_ST:
             if (c->s_tiles == 2)
                 tf::launch_stile(l.p0, c->s_tcols, L, fd, c->d_units + l.off, l.cnt, bx ? w.T : w.xw, bx ? w.T : w.Y,
-                                 Dn, c->d_sidx, out, w.part, c->d_perm, bx ? x : nullptr, bx ? x2 : nullptr, c->st);
+                                 Dn, c->d_sidx, out, w.part, sperm(c), bx ? x : nullptr, bx ? x2 : nullptr, c->st);
             else
                 tf::launch_mtile(l.p0, c->mt_ks_t, 4, 1, L, fd, c->d_units + l.off, l.cnt, 1, bx ? w.T : w.xw,
-                                 bx ? w.T : w.Y, Dn, c->d_sidx, out, w.part, gbytes, c->d_perm, bx ? x : nullptr,
+                                 bx ? w.T : w.Y, Dn, c->d_sidx, out, w.part, gbytes, sperm(c), bx ? x : nullptr,
                                  bx ? x2 : nullptr, nullptr, c->st);
             break;
         case L_STRED:
             if (c->s_tiles == 2)
                 tf::launch_stile_red(l.p0, fd, c->d_units + l.off, l.cnt, bx ? w.T : w.xw, bx ? w.T : w.Y, Dn, out,
-                                     w.part, c->d_perm, bx ? x : nullptr, bx ? x2 : nullptr, c->st);
+                                     w.part, sperm(c), bx ? x : nullptr, bx ? x2 : nullptr, c->st);
             else
                 tf::launch_mtile_red(l.p0, 4, 1, fd, c->d_units + l.off, l.cnt, 1, bx ? w.T : w.xw, bx ? w.T : w.Y, Dn,
-                                     out, w.part, c->d_perm, bx ? x : nullptr, bx ? x2 : nullptr, c->st);
+                                     out, w.part, sperm(c), bx ? x : nullptr, bx ? x2 : nullptr, c->st);
             break;
         }
         prof_close(c, prof_category(l.kind), pm);
@@ -1056,11 +1195,11 @@ void solve_dev_gemv(tf_ctx *c, const double *L, const double *Dn, const FrontDes
         const int pm = prof_mark(c);
         switch (l.kind) {
         case L_SSMALL:
-            tf::launch_fwd_small(L, fd, lst + l.off, l.cnt, b, c->d_perm, c->d_chp, c->d_ch, c->d_rel, w.V,
+            tf::launch_fwd_small(L, fd, lst + l.off, l.cnt, b, sperm(c), c->d_chp, c->d_ch, c->d_rel, w.V,
                                  w.Y, c->st);
             break;
         case L_GINIT:
-            tf::launch_mg_init(fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, b, c->d_perm, w.V, 1, c->st);
+            tf::launch_mg_init(fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, b, sperm(c), w.V, 1, c->st);
             break;
         case L_GCHILD:
             tf::launch_mg_child(fd, c->d_parent, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_rel, w.V, 1,
@@ -1093,7 +1232,7 @@ void solve_dev_gemv(tf_ctx *c, const double *L, const double *Dn, const FrontDes
         }
         prof_close(c, prof_category(l.kind), pm);
     }
-    tf::launch_unpermute(c->sym.n, c->d_perm, w.xw, x, c->st);
+    tf::launch_unpermute(c->sym.n, sperm(c), w.xw, x, c->st);
 }
 void solve_dev(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd, const double *b, double *x) {
     solve_dev(c, L, Dn, fd, b, x, ctx_work(c));
@@ -1132,13 +1271,13 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
         const int pm = prof_mark(c);
         switch (l.kind) {
         case L_MPULL:
-            tf::launch_mpull(fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, b, c->d_perm, c->d_gptr, c->d_gidx,
+            tf::launch_mpull(fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, b, sperm(c), c->d_gptr, c->d_gidx,
                              c->d_mZ, c->d_mV, K, c->st);
             break;
         case L_MT:  // own forward (Y from Yin, or from b), struct forward (Z from Y)
             if (l.p0 == 4)
                 tf::launch_mtile(4, c->mt_ks_n, l.nb, na, L, fd, c->d_units + l.off, l.cnt, K, b, nullptr, Dn, nullptr,
-                                 c->d_mY, c->d_part, gbytes, c->d_perm, nullptr, nullptr, nullptr, c->st);
+                                 c->d_mY, c->d_part, gbytes, sperm(c), nullptr, nullptr, nullptr, c->st);
             else
                 tf::launch_mtile(l.p0, c->mt_ks_n, l.nb, na, L, fd, c->d_units + l.off, l.cnt, K,
                                  l.p0 == 0 ? c->d_mV : c->d_mY, nullptr, Dn, nullptr, l.p0 == 0 ? c->d_mY : c->d_mZ,
@@ -1147,7 +1286,7 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
         case L_MTRED:
             if (l.p0 == 4)
                 tf::launch_mtile_red(4, l.nb, na, fd, c->d_units + l.off, l.cnt, K, b, nullptr, Dn, c->d_mY, c->d_part,
-                                     c->d_perm, nullptr, nullptr, c->st);
+                                     sperm(c), nullptr, nullptr, c->st);
             else
                 tf::launch_mtile_red(l.p0, l.nb, na, fd, c->d_units + l.off, l.cnt, K, l.p0 == 0 ? c->d_mV : c->d_mY, nullptr,
                                      Dn, l.p0 == 0 ? c->d_mY : c->d_mZ, c->d_part, nullptr, nullptr, nullptr, c->st);
@@ -1164,12 +1303,12 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
         switch (l.kind) {
         case L_MT:
             tf::launch_mtile(l.p0, c->mt_ks_t, l.nb, na, L, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : c->d_mxw,
-                             bx ? c->d_mT : c->d_mY, Dn, c->d_sidx, out, c->d_part, gbytes, c->d_perm,
+                             bx ? c->d_mT : c->d_mY, Dn, c->d_sidx, out, c->d_part, gbytes, sperm(c),
                              bx ? x : nullptr, bx ? x2 : nullptr, LT, c->st);
             break;
         case L_MTRED:
             tf::launch_mtile_red(l.p0, l.nb, na, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : c->d_mxw,
-                                 bx ? c->d_mT : c->d_mY, Dn, out, c->d_part, c->d_perm, bx ? x : nullptr,
+                                 bx ? c->d_mT : c->d_mY, Dn, out, c->d_part, sperm(c), bx ? x : nullptr,
                                  bx ? x2 : nullptr, c->st);
             break;
         }
@@ -1212,13 +1351,14 @@ void ensure_vecs(tf_ctx *c, StepVecs &v) {
 void rhs(tf_ctx *c, int step, double t, const double *e2, const double *e3, double dt, bool negate, double *r) {
     const int K = c->nsrc;
     const int pm = prof_mark(c);
-    if (K == 1) tf::launch_spmv_rhs(c->topo.nE, c->d_fia, c->d_fja, c->d_fval, e2, e3, r, c->st);
-    else tf::launch_mspmv_rhs(c->topo.nE, K, c->d_fia, c->d_fja, c->d_fval, e2, e3, c->d_mb, r, c->st);
+    const tf_ctx::RunMaps &m = c->rm;
+    if (K == 1) tf::launch_spmv_rhs(c->topo.nE, m.fia, m.fja, m.fval, e2, e3, r, c->st);
+    else tf::launch_mspmv_rhs(c->topo.nE, K, m.fia, m.fja, m.fval, e2, e3, c->d_mb, r, c->spmv_xcd, c->st);
     if (step < c->sp_division) {  // sys_right: source on while step < sp_division
         double didt = source_didt(c->stype, c->ptype, t, c->pulse_width, c->current);
         if (negate) didt = -didt;
-        if (K == 1) tf::launch_source(c->nseg, c->d_src_edge, c->d_src_w, didt, 2.0 * dt, r, c->st);
-        else tf::launch_msource(c->nseg, K, c->d_src_edge, c->d_src_id, c->d_src_w, didt, 2.0 * dt, r, c->st);
+        if (K == 1) tf::launch_source(c->nseg, m.src_edge, c->d_src_w, didt, 2.0 * dt, r, c->st);
+        else tf::launch_msource(c->nseg, K, m.src_edge, c->d_src_id, c->d_src_w, didt, 2.0 * dt, r, c->st);
     }
     prof_close(c, TF_PROF_STEP, pm);
 }
@@ -1275,7 +1415,7 @@ void slice_row(tf_ctx *c, int step, double t, double dt, const double *e) {
     }
     if (!pending) return;
     double *out = take.empty() ? nullptr : c->d_slice + (int64_t)take[0] * T * 9;
-    tf::launch_slice(T, c->d_xyz, c->d_tet_vex, c->d_tet_edg, c->d_tet_drc, e, dt, step == 0 ? 1 : 0, c->d_sliceB,
+    tf::launch_slice(T, c->d_xyz, c->d_tet_vex, c->rm.tet_edg, c->d_tet_drc, e, dt, step == 0 ? 1 : 0, c->d_sliceB,
                      out, c->st);
     for (size_t q = 0; q < take.size(); q++) {
         const int i = take[q];
@@ -1316,8 +1456,8 @@ void output_row(tf_ctx *c, int step, double t, double dt, const double *e) {
         c->rows_cap = cap;
     }
     double *o = c->d_rows + per * c->nrows;
-    if (c->nsrc == 1) tf::launch_readout(c->nR, c->d_rec_edge, c->d_rec_drc, c->d_narr, c->d_darr, e, o, c->st);
-    else tf::launch_mreadout(c->nR, c->nsrc, c->d_rec_edge, c->d_rec_drc, c->d_narr, c->d_darr, e, o, c->st);
+    if (c->nsrc == 1) tf::launch_readout(c->nR, c->rm.rec_edge, c->d_rec_drc, c->d_narr, c->d_darr, e, o, c->st);
+    else tf::launch_mreadout(c->nR, c->nsrc, c->rm.rec_edge, c->d_rec_drc, c->d_narr, c->d_darr, e, o, c->st);
     if (!c->slice_t.empty()) slice_row(c, step, t, dt, e);
     c->row_step.push_back(step);
     c->row_t.push_back(t);
@@ -1679,7 +1819,7 @@ void factor_models(tf_ctx *c, const std::vector<double> &dt, const std::vector<i
 void rhs_models(tf_ctx *c, int step, const std::vector<double> &t, const std::vector<double> &dt, const double *e2,
                 const double *e3, double *r) {
     const int M = c->nmodels;
-    tf::launch_spmv_rhs(c->topo.nE, c->d_fia, c->d_fja, c->d_fval, e2, e3, r, c->st);
+    tf::launch_spmv_rhs(c->topo.nE, c->rm.fia, c->rm.fja, c->rm.fval, e2, e3, r, c->st);
     if (step < c->sp_division) {
         std::vector<double> coef(2 * (size_t)M);
         for (int m = 0; m < M; m++) {
@@ -1688,7 +1828,7 @@ void rhs_models(tf_ctx *c, int step, const std::vector<double> &t, const std::ve
         }
         HIPCHK(hipMemcpyAsync(c->d_mcoef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, c->st));
         HIPCHK(hipStreamSynchronize(c->st));  // coef lives on the host stack
-        tf::launch_source_models(c->nseg, c->d_src_edge, c->d_src_id, c->d_src_w, c->d_mcoef, r, c->st);
+        tf::launch_source_models(c->nseg, c->rm.src_edge, c->d_src_id, c->d_src_w, c->d_mcoef, r, c->st);
     }
 }
 
@@ -1771,7 +1911,7 @@ void run_models(tf_ctx *c, int max_steps) {
             double *eb = v.ring[(idx - 1 - dbl) % ring];
             rhs_models(c, step, ttime, tstep, ea, eb, v.rh);
             solve_dev(c, c->d_Lmm, c->d_Dmm, c->d_fd_oth, v.rh, v.ev2);
-            tf::launch_range_norms(M, c->d_mrange, v.ev2, v.ev1, c->d_mnorm, c->st);
+            tf::launch_range_norms(M, c->rm.mrange, v.ev2, v.ev1, c->d_mnorm, c->st);
             std::vector<double> nrm(2 * (size_t)M);
             HIPCHK(hipMemcpyAsync(nrm.data(), c->d_mnorm, sizeof(double) * nrm.size(), hipMemcpyDeviceToHost, c->st));
             HIPCHK(hipStreamSynchronize(c->st));
@@ -1786,7 +1926,7 @@ void run_models(tf_ctx *c, int max_steps) {
                 if (keep) {
                     not_changed[m]++;
                 } else {
-                    HIPCHK(hipMemcpyAsync(v.e2 + c->m_e0[m], v.ev1 + c->m_e0[m], sizeof(double) * c->m_ne[m],
+                    HIPCHK(hipMemcpyAsync(v.e2 + c->rm.e0[m], v.ev1 + c->rm.e0[m], sizeof(double) * c->m_ne[m],
                                           hipMemcpyDeviceToDevice, c->st));
                     cur[m] = other[m];
                     tcur[m] = ttime[m];
@@ -1808,9 +1948,9 @@ void run_models(tf_ctx *c, int max_steps) {
             if (it == graphs.end()) {
                 hipGraph_t g;
                 HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-                tf::launch_spmv_rhs(n, c->d_fia, c->d_fja, c->d_fval, v.e2, v.e3, v.rh, c->st);
+                tf::launch_spmv_rhs(n, c->rm.fia, c->rm.fja, c->rm.fval, v.e2, v.e3, v.rh, c->st);
                 solve_dev(c, c->d_Lmm, c->d_Dmm, c->d_fd_cur, v.rh, v.e1);
-                tf::launch_readout(R, c->d_rec_edge, c->d_rec_drc, c->d_narr, c->d_darr, v.e1, c->d_mstage, c->st);
+                tf::launch_readout(R, c->rm.rec_edge, c->d_rec_drc, c->d_narr, c->d_darr, v.e1, c->d_mstage, c->st);
                 HIPCHK(hipMemcpyAsync(v.ring[idx % ring], v.e1, sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
                 HIPCHK(hipStreamEndCapture(c->st, &g));
                 hipGraphExec_t ex;
@@ -2002,16 +2142,16 @@ void run_adjoint(tf_ctx *c, const double *w, double *grad_host) {
             const int slot = slot_for(nd.dt);
             if (accA[k] || accB[k]) {
                 const double *a = accA[k] ? accA[k] : zero, *b = accB[k] ? accB[k] : zero;
-                if (K == 1) tf::launch_spmv_rhs(n, c->d_fia, c->d_fja, c->d_fval, a, b, rh, c->st);
-                else tf::launch_mspmv_rhs(n, K, c->d_fia, c->d_fja, c->d_fval, a, b, c->d_mb, rh, c->st);
+                if (K == 1) tf::launch_spmv_rhs(n, c->rm.fia, c->rm.fja, c->rm.fval, a, b, rh, c->st);
+                else tf::launch_mspmv_rhs(n, K, c->rm.fia, c->rm.fja, c->rm.fval, a, b, c->d_mb, rh, c->spmv_xcd, c->st);
             } else {
                 tf::launch_fill0(rh, len, c->st);
             }
             if (nd.row > 0)
-                tf::launch_adj_seed(R, K, c->d_rec_edge, c->d_rec_drc, c->d_narr, c->d_darr,
+                tf::launch_adj_seed(R, K, c->rm.rec_edge, c->d_rec_drc, c->d_narr, c->d_darr,
                                     d_w + (int64_t)nd.row * K * R * 6, rh, c->st);
             step_solve(c, slot, rh, lam);
-            tf::launch_sens(T, K, c->d_tet_edg, c->d_orient, c->d_vm1, lam, S(k), S(nd.u), S(nd.v), grad, c->st);
+            tf::launch_sens(T, K, c->rm.tet_edg, c->d_orient, c->d_vm1, lam, S(k), S(nd.u), S(nd.v), grad, c->st);
             if (nd.u > 0) {
                 if (!accA[nd.u]) accA[nd.u] = zeroed();
                 tf::launch_axpy(len, 1.0, lam, accA[nd.u], c->st);
@@ -2271,6 +2411,7 @@ int tf_set_source(tf_ctx *c, int isrc, int nseg, const int *seg_vex, int source_
             c->d_src_id = dupload(c->src_id);
             c->d_src_w = dupload(c->src_w);
         }
+        free_native(c);  // the native-order maps follow the new sources, receivers or models
         return TF_OK;
     });
 }
@@ -2348,6 +2489,7 @@ int tf_set_receivers(tf_ctx *c, int nR, const double *xyz, double rec_charac, in
         c->d_darr = dalloc<double>(18 * (size_t)nR);
         tf::launch_recvec(nR, c->d_xyz, c->d_tet_vex, c->d_rec_tet, c->d_rec_xyz, c->d_narr, c->d_darr, c->st);
         HIPCHK(hipStreamSynchronize(c->st));
+        free_native(c);  // the native-order maps follow the new sources, receivers or models
         return TF_OK;
     });
 }
@@ -2381,6 +2523,7 @@ int tf_assemble(tf_ctx *c) {
         tf::launch_gather(t.nnz, c->d_cptr, c->d_ctr, c->d_vm, c->d_vk, c->d_M, c->d_K, c->st);
         tf::launch_gather_vals((int64_t)t.ia.size() > 0 ? (int64_t)(2 * t.nnz - t.nE) : 0, c->d_fmap, c->d_M,
                                c->d_fval, c->st);
+        if (c->native_ok) tf::launch_gather_vals(c->nfnnz, c->d_nfmap, c->d_M, c->d_nfval, c->st);
         HIPCHK(hipStreamSynchronize(c->st));
         HIPCHK(hipGetLastError());
         c->timers[0] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -2439,6 +2582,13 @@ int tf_set_tuning(tf_ctx *c, const char *key, int value) {
         } else if (k == "s_tiles") {
             if (value < 0 || value > 2) return fail(c, TF_ERR_ARG, "s_tiles must be 0, 1 or 2");
             c->s_tiles = value;
+        } else if (k == "native") {
+            if (value != 0 && value != 1) return fail(c, TF_ERR_ARG, "native must be 0 or 1");
+            if (c->native != (value != 0)) c->tape_valid = false;
+            c->native = value != 0;
+        } else if (k == "spmv_xcd") {
+            if (value != 0 && value != 1) return fail(c, TF_ERR_ARG, "spmv_xcd must be 0 or 1");
+            c->spmv_xcd = value;
         } else if (k == "s_tcols") {
             if (value != 16 && value != 32) return fail(c, TF_ERR_ARG, "s_tcols must be 16 or 32");
             c->s_tcols = value;
@@ -2489,6 +2639,7 @@ int tf_adjoint(tf_ctx *c, int nrows, const double *w, double *grad_logsigma) {
         if (nrows != c->nrows) return fail(c, TF_ERR_ARG, "weights must cover every row of the taped run");
         if (!w || !grad_logsigma) return fail(c, TF_ERR_ARG, "null buffer");
         auto t0 = std::chrono::steady_clock::now();
+        RunScope scope(c);  // the taped vectors are in the order tf_run kept them (the "native" key resets the tape)
         run_adjoint(c, w, grad_logsigma);
         c->timers[5] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         return TF_OK;
@@ -2736,6 +2887,7 @@ int tf_set_models(tf_ctx *c, int nmodels, const int *ntet, const int *nrec) {
         c->src_edges.assign(nmodels, {});
         c->src_ws.assign(nmodels, {});
         c->have_src = false;
+        free_native(c);  // the native-order maps follow the new sources, receivers or models
         return TF_OK;
     });
 }
@@ -2754,6 +2906,7 @@ int tf_run_models(tf_ctx *c, int max_steps, int use_graphs) {
         c->row_step.clear(); c->row_t.clear(); c->row_dt.clear();
         c->use_graphs = use_graphs != 0;
         auto t0 = std::chrono::steady_clock::now();
+        RunScope scope(c);
         run_models(c, max_steps);
         c->timers[5] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         return TF_OK;
@@ -2813,6 +2966,7 @@ int tf_run(tf_ctx *c, int max_steps, int max_rows, int *nrows, int *row_step, do
         c->row_step.clear(); c->row_t.clear(); c->row_dt.clear();
         c->chk_step.clear(); c->chk_err.clear(); c->chk_keep.clear();
         auto t0 = std::chrono::steady_clock::now();
+        RunScope scope(c);
         if (c->ptype == 3) {
             double *res = dalloc<double>((size_t)c->topo.nE * c->nsrc);
             run_loop(c, 1, res, max_steps);

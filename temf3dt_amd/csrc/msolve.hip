@@ -21,10 +21,13 @@ __global__ void k_mhist(int64_t len, const double *e2, const double *e3, double 
 // K <= 64).  The row's column indices and values are loaded once across the
 // lanes and broadcast one at a time from a scalar register, so the gathers
 // of the rows of w do not wait on each other and eight of them are in flight.
+// With xcd, every XCD takes one contiguous range of rows, so that the rows of
+// w that neighbouring rows gather (edges of nearby vertices) meet in one L2.
 __global__ void __launch_bounds__(256) k_mspmv_rhs(int n, int K, const int *ia, const int *ja, const double *val,
-                                                   const double *w, double *r) {
+                                                   const double *w, double *r, int xcd) {
     const int lane = threadIdx.x & 63;
-    const int row0 = blockIdx.x * 4 * TF_RPW + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int blk = xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int row0 = blk * 4 * TF_RPW + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int rr = 0; rr < TF_RPW; rr++) {
         const int row = row0 + 4 * rr;
         if (row >= n) return;
@@ -269,11 +272,11 @@ void launch_spull(const FrontDesc *fd, const int *list, const int64_t *pref, int
                            Yin);
 }
 void launch_mspmv_rhs(int n, int K, const int *ia, const int *ja, const double *val, const double *e2,
-                      const double *e3, double *w, double *r, hipStream_t st) {
+                      const double *e3, double *w, double *r, int xcd, hipStream_t st) {
     const int64_t tot = (int64_t)n * K;
     hipLaunchKernelGGL(k_mhist, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, tot, e2, e3, w);
     hipLaunchKernelGGL(k_mspmv_rhs, dim3((unsigned)((n + 4 * TF_RPW - 1) / (4 * TF_RPW))), dim3(256), 0, st, n, K, ia,
-                       ja, val, w, r);
+                       ja, val, w, r, xcd);
 }
 void launch_msource(int ns, int K, const int *edge, const int *src, const double *w, double didt, double cdt,
                     double *r, hipStream_t st) {
