@@ -195,6 +195,11 @@ def uneven_bench(steps, n=48):
     print(f"[bench] {n} models: {int(u['ntet'].sum())} tets ({int(u['ntet'].min())}-{int(u['ntet'].max())}), "
           f"{fe.nE} edges, nnzL {st['nnzL']:.3e}, analyze {t_an:.2f} s", file=sys.stderr, flush=True)
     out = {}
+    fe.run_models(max_steps=3, graphs=True)  # factors every model
+    from temf3dt_amd import autotune
+    tune = autotune.autotune_single(fe, log=lambda x: print(x, file=sys.stderr, flush=True))
+    print(f"[bench] autotune: {tune['best']} ({tune['best_ms']:.3f} ms per solve of all models)", file=sys.stderr,
+          flush=True)
     for graphs in (True, False):
         fe.run_models(max_steps=3, graphs=graphs)  # warm-up
         t = time.time()
@@ -209,6 +214,8 @@ def uneven_bench(steps, n=48):
     seq_wall, seq_steps = 0.0, 0
     for m in ms:
         f1, _ = setup_gpu(m, [0])
+        for k, v in tune["best"].items():
+            f1.set_tuning(k, v)
         f1.run(max_steps=3)
         t = time.time()
         r = f1.run(max_steps=steps)
@@ -222,6 +229,7 @@ def uneven_bench(steps, n=48):
            "value": out["graphs"]["value"], "unit": "source-timesteps/s", "n_gpus": 1, "steps": steps,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"{n} independent models, {int(u['ntet'].sum())} tets in all", "modes": out,
+                      "autotune": tune["best"],
                       "over_sequential": out["graphs"]["value"] / out["sequential"]["value"]}}
     print(json.dumps(res))
 
@@ -258,16 +266,23 @@ def single_source_rooflines(prof, nsolves, nfact, st, n, nnzu, lay):
 
 
 def copy_probe():
-    """Calibration for HBM counters: three device copies of 1 GiB (read 1 GiB,
-    write 1 GiB each); FETCH_SIZE / WRITE_SIZE of these dispatches against the
-    known bytes gives the scale applied to the solver kernels' counters."""
-    import torch
-    a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
+    """Calibration for HBM counters: three device-to-device copies of 1 GiB
+    (read 1 GiB, write 1 GiB each) through the HIP runtime the library links;
+    FETCH_SIZE / WRITE_SIZE of these dispatches against the known bytes give
+    the scale applied to the solver kernels' counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    nbytes = 1 << 30
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
     for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes": a.numel() * 8, "copies": 3}))
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": nbytes, "copies": 3}))
 
 
 def main():
@@ -365,11 +380,19 @@ def main():
     adjoint = None
     if not a.no_plain:
         fp, _ = setup_gpu(m, [0])
+        ptune = None
+        if not a.no_tune:
+            from temf3dt_amd import autotune
+            fp.factor(0, m.source[2] / m.sp_division)
+            ptune = autotune.autotune_single(fp, slot=0, log=lambda x: print(x, file=sys.stderr, flush=True))
+            fp.release(0)
+            print(f"[bench] plain autotune: {ptune['best']} ({ptune['best_ms']:.3f} ms per solve)", file=sys.stderr,
+                  flush=True)
         prow, pwall, ptm, _ = timed_run(fp, a.steps, a.warmup)
         psteps = int(prow["step"][-1])
         pchk = len(fp.checks()["step"])
         plain = {"value": psteps / pwall, "steps": psteps, "wall_s": pwall, "factor_s": ptm["factor"],
-                 "factorisations": int(ptm["nfactor"])}
+                 "factorisations": int(ptm["nfactor"]), "autotune": ptune and ptune["best"]}
         print(f"[bench] plain: {psteps} steps in {pwall:.3f} s ({ptm['factor']:.3f} s factorising)",
               file=sys.stderr, flush=True)
         # kernel breakdown of the plain run (factorisations at the checks)

@@ -182,7 +182,9 @@ int tf_solve_noalloc(tf_ctx *ctx, int slot, const double *b, double *x);
 /* Device-resident solve timing for the autotuner (no reference counterpart):
  * nrhs (1..64) deterministic right-hand sides already in HBM, one warm-up
  * solve, then reps solves timed with HIP events; *ms_per_solve receives the
- * average. */
+ * average.  In a context of independent models (tf_set_models) it times one
+ * solve of every model's current factor (nrhs = 1, slot ignored) after a
+ * tf_run_models. */
 int tf_time_solve(tf_ctx *ctx, int slot, int nrhs, int reps, double *ms_per_solve);
 /* pardiso_direct_3 */
 int tf_release(tf_ctx *ctx, int slot);

@@ -395,6 +395,7 @@ struct tf_ctx {
     int64_t *d_slot_off = nullptr;
     double *d_mcdt = nullptr, *d_mcoef = nullptr, *d_mnorm = nullptr, *d_mstage = nullptr;
     bool use_graphs = true;
+    bool models_factored = false;  // every model's current slot holds a factor
     bool overlap_factor = true;  // tf_run: factor P(2 dt) on st2 ahead of the check
     // rows of the last tf_run_models: all receivers per step, per-model metadata
     std::vector<int> mm_step;                  // [steps]
@@ -441,6 +442,7 @@ void free_models(tf_ctx *c) {
     dfree(c->d_fd_cur); dfree(c->d_fd_oth); dfree(c->d_Lmm); dfree(c->d_Dmm); dfree(c->d_qmodel);
     dfree(c->d_mrange); dfree(c->d_slot_off); dfree(c->d_mcdt); dfree(c->d_mcoef); dfree(c->d_mnorm);
     dfree(c->d_mstage);
+    c->models_factored = false;
 }
 
 void free_tape(tf_ctx *c) {
@@ -1811,6 +1813,7 @@ void factor_models(tf_ctx *c, const std::vector<double> &dt, const std::vector<i
                                 c->d_Lmm, c->st);
     run_factor(c, c->d_Lmm, c->d_Dmm, fd);
     check_flag(c);
+    c->models_factored = true;
     c->timers[7] += 1.0;
 }
 
@@ -2805,15 +2808,19 @@ int tf_solve_noalloc(tf_ctx *c, int slot, const double *b, double *x) {
 
 int tf_time_solve(tf_ctx *c, int slot, int nrhs, int reps, double *ms_per_solve) {
     return guarded(c, [&] {
-        if (slot < 0 || slot > 1 || !c->slot_live[slot]) return fail(c, TF_ERR_STATE, "slot not factored");
+        const bool models = c->nmodels > 0;
+        if (models ? !c->models_factored : (slot < 0 || slot > 1 || !c->slot_live[slot]))
+            return fail(c, TF_ERR_STATE, models ? "models not factored (tf_run_models first)" : "slot not factored");
         if (nrhs < 1 || nrhs > 64 || reps < 1 || !ms_per_solve) return fail(c, TF_ERR_ARG, "bad arguments");
+        if (models && nrhs != 1) return fail(c, TF_ERR_ARG, "independent models solve one right-hand side each");
         const int n = c->topo.nE;
         double *b, *x;
         if (nrhs == 1) { b = c->d_tmpb; x = c->d_tmpx; }
         else { ensure_block(c, nrhs); b = c->d_mb; x = c->d_mx; }
         tf::launch_fill_pattern((int64_t)n * nrhs, b, c->st);
         auto one = [&] {
-            if (nrhs == 1) solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, b, x);
+            if (models) solve_dev(c, c->d_Lmm, c->d_Dmm, c->d_fd_cur, b, x);  // every model's current factor
+            else if (nrhs == 1) solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, b, x);
             else solve_block(c, c->d_L[slot], c->d_D[slot], c->d_fd, nrhs, b, x, nullptr, c->d_LT[slot]);
         };
         one();  // warm-up

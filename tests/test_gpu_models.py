@@ -77,3 +77,26 @@ def test_model_setup_errors(three_models):
     with pytest.raises(fetd.FetdError):
         fe.run()  # tf_run is for a single model
     fe.close()
+
+
+def test_models_single_solve_autotuner(three_models):
+    """The single-solve autotuner times a context of independent models
+    (tf_time_solve over every model's current factor) and its knobs leave the
+    models' rows unchanged beyond summation order."""
+    from temf3dt_amd import autotune
+    fe = _union_ctx(three_models)
+    fe.run_models(max_steps=30, graphs=True)
+    ref = [fe.model_rows(i)["fields"] for i in range(3)]
+    tune = autotune.autotune_single(fe, reps=2, space={"s_units": (64, 8192), "s_chmin": (16, 512),
+                                                      "s_tcols": (16, 32)})
+    print(f"[gpu] models autotune: {tune['best']} {tune['best_ms']:.3f} ms", flush=True)
+    assert tune["best_ms"] > 0 and len(tune["table"]) == 8
+    for k, v in (("s_units", 64), ("s_chmin", 16), ("s_tcols", 16)):  # the most split setting
+        fe.set_tuning(k, v)
+    fe.run_models(max_steps=30, graphs=True)
+    for i in range(3):  # dB/dt and B (E in the 1e-12 S/m air carries solver rounding)
+        a, b = fe.model_rows(i)["fields"][..., 3:9], ref[i][..., 3:9]
+        rel = np.abs(a - b).max() / np.abs(b).max()
+        print(f"[gpu] model {i} with the split tiling vs autotuned: {rel:.3e}", flush=True)
+        assert rel < 1e-10
+    fe.close()

@@ -413,3 +413,36 @@ def test_cpu_cost_sample(oracle_mod_big):
     assert len(g["step"]) == 579
     r = oracle_mod_big.run(max_steps=1)
     assert int(c["factor_nnz"]) == int(r["factor_nnz"])
+
+
+def test_oracle_matches_linux_build_at_air_1e12(tmp_path):
+    """The reference's LINUX build wrote tests/golden/linux_mod_big's .field
+    for the mod_big mesh with air at 1e-12 S/m (the conditioning of the
+    3-D configurations) and 4 printed digits.  It is a different revision
+    of adap_euler: after a dt change its time column advances by another rule
+    (0.2200e-06 against src's 0.21667e-06 at step 64), so the two schedules
+    part there.  Up to that first change (64 rows: the 30-step source ramp,
+    regular steps and a kept check at step 32) the oracle reproduces its
+    steps, times and dBz/dt to the printed digits (relative 5e-4)."""
+    import gzip
+    import shutil
+    d = os.path.join(os.path.dirname(__file__), "golden", "linux_mod_big")
+    mesh = str(tmp_path / "m.mesh")
+    with gzip.open(os.path.join(d, "mod_big_sig_0.001.mesh.gz"), "rb") as f, open(mesh, "wb") as g:
+        shutil.copyfileobj(f, g)
+    ref = np.loadtxt(os.path.join(d, "mod_big_sig_0.001.field"))
+    t = ref[:, 1]
+    first = next(i for i in range(2, len(t)) if (t[i] - t[i - 1]) / (t[i - 1] - t[i - 2]) > 1.5)
+    assert first == 64
+    pyoracle.set_threads(os.cpu_count() or 8)
+    o = pyoracle.Oracle(mesh, os.path.join(d, "mod_big_sig_0.001.cntl"))
+    o.assemble()
+    r = o.run(max_steps=first - 1)
+    o.close()
+    assert len(r["step"]) == first
+    np.testing.assert_array_equal(r["step"], ref[:first, 0])
+    np.testing.assert_allclose(r["t"][1:], t[1:first], rtol=5e-4)
+    a, b = r["fields"][:, 0, 5], ref[:first, 2]
+    sel = b != 0
+    assert np.all(a[~sel] == 0.0)
+    np.testing.assert_allclose(a[sel], b[sel], rtol=5e-4)
