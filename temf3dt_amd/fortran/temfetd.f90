@@ -103,6 +103,20 @@ module temfetd
       type(c_ptr), value :: ctx
       integer(c_int), value :: slot
     end function
+    ! pivots of the last factorisation: the counterpart of the SPD test of
+    ! pardiso_direct_1 (a negative pivot is where mtype 2 would fail)
+    integer(c_int) function tf_factor_pivots(ctx, n_negative, min_abs, max_abs) bind(C, name='tf_factor_pivots')
+      import :: c_ptr, c_int, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: n_negative
+      real(c_double), intent(out) :: min_abs, max_abs
+    end function
+    integer(c_int) function tf_set_tuning(ctx, key, value) bind(C, name='tf_set_tuning')
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: key(*)
+      integer(c_int), value :: value
+    end function
     integer(c_int) function tf_run(ctx, max_steps, max_rows, nrows, row_step, row_t, row_dt, fields) &
         bind(C, name='tf_run')
       import :: c_ptr, c_int, c_double
