@@ -36,6 +36,8 @@ void launch_slice(int nT, const double *xyz, const int *tet_vex, const int *tet_
                   const double *e, double dt, int zero_b, double *B, double *out, hipStream_t st);
 // factor.hip
 void launch_fill0(double *p, int64_t n, hipStream_t st);
+void launch_pinv_w21(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
+                     hipStream_t st);
 void launch_flags_init(FactorFlags *f, hipStream_t st);
 void launch_pivot_scan(const double *Dn, const FrontDesc *fd, int nsn, FactorFlags *f, hipStream_t st);
 void launch_transpose_l21(const double *L, double *LT, const FrontDesc *fd, const int *units, int64_t nunits,
@@ -93,7 +95,7 @@ void launch_scale(int n, const double *s, double *g, hipStream_t st);
 void launch_mtile(int mode, int ks, int nb, int na, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits,
                   int K, const double *in0, const double *in1, const double *Dn, const int *sidx, double *out,
                   double *part, uint32_t gbytes, const int *perm, double *x1, double *x2, const double *LT,
-                  hipStream_t st);
+                  hipStream_t st, double *z = nullptr);
 void launch_spull(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg, const double *b,
                   const int *perm, const int *gptr, const int *gidx, const double *Z, double *Yin, hipStream_t st);
 void launch_mpull(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg, const double *b,
@@ -101,13 +103,13 @@ void launch_mpull(const FrontDesc *fd, const int *list, const int64_t *pref, int
                   hipStream_t st);
 void launch_stile(int mode, int nc, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits,
                   const double *in0, const double *in1, const double *Dn, const int *sidx, double *out, double *part,
-                  const int *perm, double *x1, double *x2, hipStream_t st);
+                  const int *perm, double *x1, double *x2, hipStream_t st, double *z = nullptr);
 void launch_stile_red(int mode, const FrontDesc *fd, const TileUnit *units, int nunits, const double *in0,
                       const double *in1, const double *Dn, double *out, const double *part, const int *perm,
-                      double *x1, double *x2, hipStream_t st);
+                      double *x1, double *x2, hipStream_t st, double *z = nullptr);
 void launch_mtile_red(int mode, int nb, int na, const FrontDesc *fd, const TileUnit *units, int nunits, int K,
                       const double *in0, const double *in1, const double *Dn, double *out, const double *part,
-                      const int *perm, double *x1, double *x2, hipStream_t st);
+                      const int *perm, double *x1, double *x2, hipStream_t st, double *z = nullptr);
 // mfront.hip
 int mfront_max_blocks();
 void launch_mfront_fwd(int ncb, bool leaf, const double *L, const FrontDesc *fd, const int *list, int cnt, int K,
@@ -182,7 +184,8 @@ enum LaunchKind {
     L_BSMALL, L_BT, L_BX,
     L_GINIT, L_GCHILD, L_GXS, L_MT, L_MTRED, L_MPULL,
     L_ST, L_STRED, L_SPULL,  // single right-hand side through the tile kernels
-    L_MFF, L_MFB             // fused per-front block-solve products (small fronts of the lowest levels)
+    L_MFF, L_MFB,            // fused per-front block-solve products (small fronts of the lowest levels)
+    L_PINV                   // factorisation: partitioned inverse of a level's fronts (W = -L21 L11^-1)
 };
 struct Launch {
     int kind, p0, off, cnt, poff;
@@ -198,6 +201,7 @@ int prof_category(int kind) {
     case L_MPULL: return TF_PROF_PULL;
     case L_FUPD: return TF_PROF_FACTOR_UPDATE;
     case L_EXTADD: case L_FFUSED: case L_FDIAG: case L_FTRSM: case L_IMERGE_T: case L_IMERGE_X: case L_IDIAG:
+    case L_PINV:
         return TF_PROF_FACTOR_OTHER;
     default: return TF_PROF_SOLVE1;
     }
@@ -295,7 +299,7 @@ struct tf_ctx {
     int *d_src_edge = nullptr, *d_src_id = nullptr;
     double *d_src_w = nullptr;
     // block-solve work (K right-hand sides)
-    double *d_mV = nullptr, *d_mY = nullptr, *d_mT = nullptr, *d_mxw = nullptr, *d_mb = nullptr, *d_mx = nullptr;
+    double *d_mV = nullptr, *d_mY = nullptr, *d_mT = nullptr, *d_mb = nullptr, *d_mx = nullptr;
     double *d_part = nullptr;
     int mK = 0;
     int64_t vrows = 0;
@@ -313,6 +317,10 @@ struct tf_ctx {
     int mt_wide = 64;  // levels of at most this many fronts use 64-row tiles, the others 32-row tiles
     int mt_ks_n = 2, mt_ks_t = 2;         // k-steps per stage of the N- and T-form tile kernels
     int mt_fuse = 0;  // levels whose fronts all have <= 16 mt_fuse columns use the fused per-front kernels
+    // fronts on the lowest pinv_levels levels keep the partitioned inverse
+    // (FrontDesc::pinv): one tile product per front and direction
+    int pinv_levels = 10;
+    int *d_frows = nullptr;  // per partitioned-inverse front (at voff): its f rows in the [v; x] block vector
     // single right-hand-side solve: 2 through the vector tile kernels, 1
     // through the matrix-core tile kernels with one right-hand-side block, 0
     // through the matrix-vector kernels of solve.hip; launch sizing of its tiles
@@ -434,9 +442,10 @@ void free_solver(tf_ctx *c) {
     dfree(c->d_xw); dfree(c->d_Y); dfree(c->d_T); dfree(c->d_W); dfree(c->d_woff);
     for (int s = 0; s < 2; s++) { dfree(c->d_L[s]); dfree(c->d_D[s]); dfree(c->d_LT[s]); c->slot_live[s] = false; }
     dfree(c->d_tunits);
+    dfree(c->d_frows);
     c->n_tunits = 0;
     dfree(c->d_tmpb); dfree(c->d_tmpx);
-    dfree(c->d_mV); dfree(c->d_mY); dfree(c->d_mT); dfree(c->d_mxw); dfree(c->d_mb); dfree(c->d_mx);
+    dfree(c->d_mV); dfree(c->d_mY); dfree(c->d_mT); dfree(c->d_mb); dfree(c->d_mx);
     dfree(c->d_mZ); dfree(c->d_part);
     c->mK = 0;
     c->analyzed = false;
@@ -557,6 +566,7 @@ void build_plan(tf_ctx *c) {
     p = Plan();
     auto f_of = [&](int k) { return s.nc[k] + (int)(s.sptr[k + 1] - s.sptr[k]); };
     auto m_of = [&](int k) { return (int)(s.sptr[k + 1] - s.sptr[k]); };
+    auto pinv_of = [&](int k) { return s.level[k] < c->pinv_levels; };
     auto add = [&](std::vector<Launch> &dst, int kind, int p0, const std::vector<int> &fr,
                    const std::vector<int64_t> &units) {
         Launch L{kind, p0, (int)p.list.size(), (int)fr.size(), (int)p.pref.size(), 0, 0};
@@ -688,6 +698,15 @@ void build_plan(tf_ctx *c) {
             add(p.factor, L_IMERGE_T, b, act, um);
             add(p.factor, L_IMERGE_X, b, act, um);
         }
+        {  // partitioned inverse of the level's fronts that keep it (after their inversion)
+            std::vector<int> pv;
+            std::vector<int64_t> up;
+            for (int q = s.lev_ptr[l]; q < s.lev_ptr[l + 1]; q++) {
+                const int k = s.lev_sn[q];
+                if (pinv_of(k) && m_of(k) > 0) { pv.push_back(k); up.push_back((m_of(k) + 63) / 64); }
+            }
+            add(p.factor, L_PINV, 0, pv, up);
+        }
     }
     // forward solve, bottom-up
     for (int l = 0; l < s.nlevels; l++) {
@@ -731,8 +750,12 @@ void build_plan(tf_ctx *c) {
             }
             else if (mode == 2)
                 for (int j = 0; j < nc; j += TW) t.push_back({k, j, 0, m});
-            else
+            else if (mode == 3)
                 for (int j = 0; j < nc; j += TW) t.push_back({k, j, j, nc});
+            else if (mode == 5)  // partitioned forward: all f rows, row r reads columns < min(r, nc)
+                for (int r = 0; r < nc + m; r += TW) t.push_back({k, r, 0, std::min(nc, r + TW)});
+            else  // partitioned backward (7): column j reads rows >= j of all f rows
+                for (int j = 0; j < nc; j += TW) t.push_back({k, j, j, nc + m});
         }
         return t;
     };
@@ -796,7 +819,7 @@ void build_plan(tf_ctx *c) {
     // occupancy below what hides the factor loads' latency.
     const int mfmax = tf::mfront_max_blocks();
     auto fusable = [&](int l, const std::vector<int> &fr) {
-        (void)l;
+        if (l < c->pinv_levels) return false;
         const int lim = 16 * std::min(c->mt_fuse, mfmax);
         for (int k : fr)
             if (s.nc[k] > lim) return false;
@@ -819,6 +842,11 @@ void build_plan(tf_ctx *c) {
         }
         std::vector<int64_t> ur;
         for (int k : all) ur.push_back((s.nc[k] + 4 * TF_RPW - 1) / (4 * TF_RPW));
+        if (l < c->pinv_levels) {  // partitioned inverse: one product per front
+            if (l > 0) add(p.mfwd, L_MPULL, 0, all, ur);
+            add_tiles(p.mfwd, l == 0 ? 6 : 5, tiles_of(5, all), nb_of(all));
+            continue;
+        }
         if (l == 0) {  // fronts without children: Yin is b itself, read through perm by the tiles
             add_tiles(p.mfwd, 4, tiles_of(0, all), nb_of(all));
         } else {
@@ -837,6 +865,13 @@ void build_plan(tf_ctx *c) {
                 add_fused(dst, L_MFB, all, l == 0 ? 0 : 1);
                 continue;
             }
+            if (l < c->pinv_levels) {
+                const size_t nb = dst.size();
+                add_tiles(dst, 7, tiles_of(7, all), nb_of(all));
+                if (l == 0)
+                    for (size_t i = nb; i < dst.size(); i++) dst[i].aux = 1;
+                continue;
+            }
             add_tiles(dst, 2, tiles_of(2, all), nb_of(all));
             const size_t nb = dst.size();
             add_tiles(dst, 3, tiles_of(3, all), nb_of(all));
@@ -851,6 +886,15 @@ void build_plan(tf_ctx *c) {
     // pull as a thread per row.
     for (int l = 0; l < s.nlevels; l++) {
         std::vector<int> all(s.lev_sn.begin() + s.lev_ptr[l], s.lev_sn.begin() + s.lev_ptr[l + 1]);
+        if (l < c->pinv_levels) {  // partitioned inverse: one product per front
+            if (l > 0) {
+                std::vector<int64_t> ur;
+                for (int k : all) ur.push_back((s.nc[k] + 63) / 64);
+                add(p.sfwd, L_SPULL, 0, all, ur);
+            }
+            add_tiles(p.sfwd, l == 0 ? 6 : 5, tiles_of(5, all, 64), 4, c->s_units, c->s_chmin, L_ST, L_STRED);
+            continue;
+        }
         if (l == 0) {
             add_tiles(p.sfwd, 4, tiles_of(0, all, 64), 4, c->s_units, c->s_chmin, L_ST, L_STRED);
         } else {
@@ -863,6 +907,13 @@ void build_plan(tf_ctx *c) {
     }
     for (int l = s.nlevels - 1; l >= 0; l--) {
         std::vector<int> all(s.lev_sn.begin() + s.lev_ptr[l], s.lev_sn.begin() + s.lev_ptr[l + 1]);
+        if (l < c->pinv_levels) {
+            const size_t nb = p.sbwd.size();
+            add_tiles(p.sbwd, 7, tiles_of(7, all, 64), 4, c->s_units, c->s_chmin, L_ST, L_STRED);
+            if (l == 0)
+                for (size_t i = nb; i < p.sbwd.size(); i++) p.sbwd[i].aux = 1;
+            continue;
+        }
         add_tiles(p.sbwd, 2, tiles_of(2, all, 64), 4, c->s_units, c->s_chmin, L_ST, L_STRED);
         const size_t nb = p.sbwd.size();
         add_tiles(p.sbwd, 3, tiles_of(3, all, 64), 4, c->s_units, c->s_chmin, L_ST, L_STRED);
@@ -894,17 +945,18 @@ void do_analyze(tf_ctx *c, int leaf) {
         fd[k].c0 = s.c0[k];
         fd[k].ldt = (s.nc[k] + 15) & ~15;
         fd[k].toff = 0;
-        fd[k].pad = 0;
+        fd[k].pinv = s.level[k] < c->pinv_levels ? 1 : 0;
         voff += fd[k].f;
     }
     c->vrows = voff;
-    {  // row-major L21 store: offsets and the copy tiles
+    {  // row-major L21 store (all f rows of a partitioned-inverse front): offsets and the copy tiles
         int64_t toff = 0;
         std::vector<int> tu;
         for (int k = 0; k < s.nsn; k++) {
+            const int rows = fd[k].pinv ? fd[k].f : fd[k].m;
             fd[k].toff = toff;
-            toff += (int64_t)fd[k].m * fd[k].ldt;
-            for (int r0 = 0; r0 < fd[k].m; r0 += 64)
+            toff += (int64_t)rows * fd[k].ldt;
+            for (int r0 = 0; r0 < rows; r0 += 64)
                 for (int c0 = 0; c0 < fd[k].nc; c0 += 64) {
                     tu.push_back(k);
                     tu.push_back(r0);
@@ -914,6 +966,16 @@ void do_analyze(tf_ctx *c, int leaf) {
         c->lt_total = toff + TF_READ_SLACK;
         c->n_tunits = (int64_t)tu.size() / 3;
         c->d_tunits = dupload(tu);
+    }
+    {  // the rows of every partitioned-inverse front in the [v; x] block vector:
+        // own rows c0 + i (v), struct rows n + sidx (x)
+        std::vector<int> fr(std::max<int64_t>(voff, 1), 0);
+        for (int k = 0; k < s.nsn; k++)
+            if (fd[k].pinv) {
+                for (int i = 0; i < fd[k].nc; i++) fr[fd[k].voff + i] = fd[k].c0 + i;
+                for (int i = 0; i < fd[k].m; i++) fr[fd[k].voff + fd[k].nc + i] = s.n + s.sidx[fd[k].soff + i];
+            }
+        c->d_frows = dupload(fr);
     }
     build_plan(c);
     c->d_fd = dupload(fd);
@@ -1090,6 +1152,9 @@ void run_factor(tf_ctx *c, double *L, double *Dn, const FrontDesc *fd) {
         case L_IDIAG:
             tf::launch_inv_diag(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->st);
             break;
+        case L_PINV:
+            tf::launch_pinv_w21(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->st);
+            break;
         }
         prof_close(c, prof_category(l.kind), pm);
     }
@@ -1176,9 +1241,10 @@ void solve_dev(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd
     double *Yin = w.V, *Z = w.V + n;
     for (const Launch &l : p.sfwd) {
         const int pm = prof_mark(c);
-        const bool fromb = l.p0 == 4;
-        const double *in0 = fromb ? b : (l.p0 == 0 ? Yin : w.Y);
-        double *out = l.p0 == 1 ? Z : w.Y;
+        const bool fromb = l.p0 == 4 || l.p0 == 6;
+        const bool pin = l.p0 == 5 || l.p0 == 6;  // partitioned inverse: v = Y_own / d into T, Z rows
+        const double *in0 = fromb ? b : ((l.p0 == 0 || l.p0 == 5) ? Yin : w.Y);
+        double *out = pin ? w.T : (l.p0 == 1 ? Z : w.Y);
         switch (l.kind) {
         case L_SPULL:
             tf::launch_spull(fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, b, sperm(c), c->d_gptr, c->d_gidx, Z, Yin,
@@ -1187,7 +1253,7 @@ void solve_dev(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd
         case L_ST:
             if (c->s_tiles == 2)
                 tf::launch_stile(l.p0, c->s_tcols, L, fd, c->d_units + l.off, l.cnt, in0, nullptr, Dn, nullptr, out,
-                                 w.part, sperm(c), nullptr, nullptr, c->st);
+                                 w.part, sperm(c), nullptr, nullptr, c->st, Z);
             else
                 tf::launch_mtile(l.p0, c->mt_ks_n, 4, 1, L, fd, c->d_units + l.off, l.cnt, 1, in0, nullptr, Dn, nullptr,
                                  out, w.part, fromb ? gbytes : 0u, fromb ? sperm(c) : nullptr, nullptr, nullptr,
@@ -1196,7 +1262,7 @@ void solve_dev(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd
         case L_STRED:
             if (c->s_tiles == 2)
                 tf::launch_stile_red(l.p0, fd, c->d_units + l.off, l.cnt, in0, nullptr, Dn, out, w.part,
-                                     fromb ? sperm(c) : nullptr, nullptr, nullptr, c->st);
+                                     fromb ? sperm(c) : nullptr, nullptr, nullptr, c->st, Z);
             else
                 tf::launch_mtile_red(l.p0, 4, 1, fd, c->d_units + l.off, l.cnt, 1, in0, nullptr, Dn, out, w.part,
                                      fromb ? sperm(c) : nullptr, nullptr, nullptr, c->st);
@@ -1206,8 +1272,18 @@ void solve_dev(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd
     }
     for (const Launch &l : p.sbwd) {
         const int pm = prof_mark(c);
-        const bool bx = l.p0 == 3;
+        const bool bx = l.p0 == 3 || l.p0 == 7;
         double *out = bx ? (l.aux == 1 ? nullptr : w.xw) : w.T;
+        if (l.p0 == 7) {  // partitioned inverse: [v; x(struct)] from T and xw
+            if (l.kind == L_ST)
+                tf::launch_stile(7, c->s_tcols, L, fd, c->d_units + l.off, l.cnt, w.xw, w.T, Dn, c->d_sidx, out,
+                                 w.part, sperm(c), x, x2, c->st);
+            else
+                tf::launch_stile_red(7, fd, c->d_units + l.off, l.cnt, w.xw, w.T, Dn, out, w.part, sperm(c), x, x2,
+                                     c->st);
+            prof_close(c, prof_category(l.kind), pm);
+            continue;
+        }
         switch (l.kind) {
         case L_ST:
             if (c->s_tiles == 2)
@@ -1288,15 +1364,17 @@ void solve_dev(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd
 
 void ensure_block(tf_ctx *c, int K) {
     if (c->mK >= K) return;
-    dfree(c->d_mV); dfree(c->d_mY); dfree(c->d_mT); dfree(c->d_mxw); dfree(c->d_mb); dfree(c->d_mx); dfree(c->d_mZ);
+    dfree(c->d_mV); dfree(c->d_mY); dfree(c->d_mT); dfree(c->d_mb); dfree(c->d_mx); dfree(c->d_mZ);
     const int64_t n = c->sym.n;
     // the tile kernels' unmasked reads of the last rows may run up to 63
     // values past an array (see TF_READ_SLACK)
     c->d_mV = dalloc<double>((size_t)n * K + TF_READ_SLACK);  // Yin: own rows after the descendants' updates
     c->d_mZ = dalloc<double>(std::max<size_t>(c->sym.sidx.size(), 1) * K);  // per-front L21 Y
     c->d_mY = dalloc<double>((size_t)n * K + TF_READ_SLACK);
-    c->d_mT = dalloc<double>((size_t)n * K + TF_READ_SLACK);
-    c->d_mxw = dalloc<double>((size_t)n * K + TF_READ_SLACK);
+    // T (v = Y_own / d on the rows of partitioned-inverse fronts) and the
+    // front-order solution, adjacent: the [v; x] block vector that a
+    // partitioned-inverse front's backward product gathers its rows from
+    c->d_mT = dalloc<double>(2 * (size_t)n * K + TF_READ_SLACK);
     c->d_mb = dalloc<double>((size_t)n * K);
     c->d_mx = dalloc<double>((size_t)n * K);
     c->mK = K;
@@ -1309,8 +1387,10 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
     ensure_block(c, K);
     // the tile kernels address the gathered solution vector with 32-bit byte offsets
     const uint64_t gb = (uint64_t)c->sym.n * K * 8;
-    if (gb >= (1ull << 32)) throw std::runtime_error("block solve: n x K too large for the tile kernels (>= 4 GiB)");
-    const uint32_t gbytes = (uint32_t)gb;
+    if (2 * gb >= (1ull << 32)) throw std::runtime_error("block solve: 2 n x K too large for the tile kernels (>= 4 GiB)");
+    const uint32_t gbytes = (uint32_t)gb, gbytes2 = (uint32_t)(2 * gb);
+    // the front-order solution right after T (rows n .. 2n of the [v; x] block vector at this K)
+    double *const mxw = c->d_mT + (size_t)c->sym.n * K;
     const int na = K <= 16 ? 1 : 4;  // 16-wide right-hand-side blocks per tile
     const Plan &p = c->plan;
     const int *lst = c->d_plist;
@@ -1323,7 +1403,11 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
                              c->d_mZ, c->d_mV, K, c->st);
             break;
         case L_MT:  // own forward (Y from Yin, or from b), struct forward (Z from Y)
-            if (l.p0 == 4)
+            if (l.p0 == 5 || l.p0 == 6)  // partitioned inverse: v rows into T, Z rows
+                tf::launch_mtile(l.p0, c->mt_ks_n, l.nb, na, L, fd, c->d_units + l.off, l.cnt, K,
+                                 l.p0 == 6 ? b : c->d_mV, nullptr, Dn, nullptr, c->d_mT, c->d_part,
+                                 l.p0 == 6 ? gbytes : 0u, sperm(c), nullptr, nullptr, nullptr, c->st, c->d_mZ);
+            else if (l.p0 == 4)
                 tf::launch_mtile(4, c->mt_ks_n, l.nb, na, L, fd, c->d_units + l.off, l.cnt, K, b, nullptr, Dn, nullptr,
                                  c->d_mY, c->d_part, gbytes, sperm(c), nullptr, nullptr, nullptr, c->st);
             else
@@ -1336,7 +1420,10 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
                                   c->d_mZ, c->d_mY, c->st);
             break;
         case L_MTRED:
-            if (l.p0 == 4)
+            if (l.p0 == 5 || l.p0 == 6)
+                tf::launch_mtile_red(l.p0, l.nb, na, fd, c->d_units + l.off, l.cnt, K, l.p0 == 6 ? b : c->d_mV,
+                                     nullptr, Dn, c->d_mT, c->d_part, sperm(c), nullptr, nullptr, c->st, c->d_mZ);
+            else if (l.p0 == 4)
                 tf::launch_mtile_red(4, l.nb, na, fd, c->d_units + l.off, l.cnt, K, b, nullptr, Dn, c->d_mY, c->d_part,
                                      sperm(c), nullptr, nullptr, c->st);
             else
@@ -1350,20 +1437,30 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
     // backward X epilogues write the finished rows in edge order as well
     for (const Launch &l : (LT ? p.mbwd : p.mbwd_nolt)) {
         const int pm = prof_mark(c);
-        const bool bx = l.p0 == 3;
-        double *out = bx ? (l.aux == 1 ? nullptr : c->d_mxw) : c->d_mT;
+        const bool bx = l.p0 == 3 || l.p0 == 7;
+        double *out = bx ? (l.aux == 1 ? nullptr : mxw) : c->d_mT;
+        if (l.p0 == 7) {  // partitioned inverse: rows of [v; x] through the front's row list
+            if (l.kind == L_MT)
+                tf::launch_mtile(7, LT ? c->mt_ks_n : c->mt_ks_t, l.nb, na, L, fd, c->d_units + l.off, l.cnt, K,
+                                 c->d_mT, c->d_mT, Dn, c->d_frows, out, c->d_part, gbytes2, sperm(c), x, x2, LT, c->st);
+            else
+                tf::launch_mtile_red(7, l.nb, na, fd, c->d_units + l.off, l.cnt, K, c->d_mT, c->d_mT, Dn, out,
+                                     c->d_part, sperm(c), x, x2, c->st);
+            prof_close(c, prof_category(l.kind), pm);
+            continue;
+        }
         switch (l.kind) {
         case L_MFB:
-            tf::launch_mfront_bwd(l.p0, L, LT, fd, lst + l.off, l.cnt, K, c->d_mY, Dn, c->d_sidx, c->d_mxw, sperm(c),
+            tf::launch_mfront_bwd(l.p0, L, LT, fd, lst + l.off, l.cnt, K, c->d_mY, Dn, c->d_sidx, mxw, sperm(c),
                                   x, x2, l.aux, c->st);
             break;
         case L_MT:
-            tf::launch_mtile(l.p0, c->mt_ks_t, l.nb, na, L, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : c->d_mxw,
+            tf::launch_mtile(l.p0, c->mt_ks_t, l.nb, na, L, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : mxw,
                              bx ? c->d_mT : c->d_mY, Dn, c->d_sidx, out, c->d_part, gbytes, sperm(c),
                              bx ? x : nullptr, bx ? x2 : nullptr, LT, c->st);
             break;
         case L_MTRED:
-            tf::launch_mtile_red(l.p0, l.nb, na, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : c->d_mxw,
+            tf::launch_mtile_red(l.p0, l.nb, na, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : mxw,
                                  bx ? c->d_mT : c->d_mY, Dn, out, c->d_part, sperm(c), bx ? x : nullptr,
                                  bx ? x2 : nullptr, c->st);
             break;
@@ -2642,7 +2739,19 @@ int tf_set_tuning(tf_ctx *c, const char *key, int value) {
             c->overlap_factor = value != 0;
         } else if (k == "s_tiles") {
             if (value < 0 || value > 2) return fail(c, TF_ERR_ARG, "s_tiles must be 0, 1 or 2");
+            if (value != 2 && c->pinv_levels > 0)
+                return fail(c, TF_ERR_ARG, "s_tiles 0 and 1 read L21 itself: set pinv to 0 first");
             c->s_tiles = value;
+        } else if (k == "pinv") {
+            if (value < 0) return fail(c, TF_ERR_ARG, "pinv must be >= 0");
+            if (value > 0 && c->s_tiles != 2) return fail(c, TF_ERR_ARG, "pinv needs s_tiles 2");
+            if (value != c->pinv_levels) {  // the front layout changes: analyse again (drops the factors)
+                c->pinv_levels = value;
+                if (c->analyzed) {
+                    HIPCHK(hipDeviceSynchronize());
+                    do_analyze(c, c->leaf_size);
+                }
+            }
         } else if (k == "native") {
             if (value != 0 && value != 1) return fail(c, TF_ERR_ARG, "native must be 0 or 1");
             if (c->native != (value != 0)) c->tape_valid = false;

@@ -463,13 +463,65 @@ __global__ void __launch_bounds__(256) k_transpose_l21(const double *L, double *
     const FrontDesc d = fd[u[0]];
     const int r0 = u[1], c0 = u[2];
     const int x = threadIdx.x & 63, y = threadIdx.x >> 6;
-    const double *src = L + d.loff + (int64_t)c0 * d.ld + d.nc + r0;
-    for (int c = y; c < 64; c += 4)  // column c0 + c, struct rows r0 + x
-        t[c][x] = (c0 + c < d.nc && r0 + x < d.m) ? src[(int64_t)c * d.ld + x] : 0.0;
+    // rows of the copy: the m L21 rows, or all f rows of a partitioned-inverse front
+    const int rows = d.pinv ? d.f : d.m, rb = d.pinv ? 0 : d.nc;
+    const double *src = L + d.loff + (int64_t)c0 * d.ld + rb + r0;
+    for (int c = y; c < 64; c += 4)  // column c0 + c, rows r0 + x
+        t[c][x] = (c0 + c < d.nc && r0 + x < rows) ? src[(int64_t)c * d.ld + x] : 0.0;
     __syncthreads();
     double *dst = LT + d.toff + (int64_t)r0 * d.ldt + c0;
-    for (int r = y; r < 64; r += 4)  // struct row r0 + r, columns c0 + x
-        if (r0 + r < d.m && c0 + x < d.ldt) dst[(int64_t)r * d.ldt + x] = t[x][r];
+    for (int r = y; r < 64; r += 4)  // row r0 + r, columns c0 + x
+        if (r0 + r < rows && c0 + x < d.ldt) dst[(int64_t)r * d.ldt + x] = t[x][r];
+}
+
+// Partitioned inverse of a front (fronts with pinv set, after their
+// factorisation and the in-place inversion of L11): the L21 rows become
+// W = -L21 L11^-1 = -(L21 + L21 strict(L11^-1)), so that the solve's forward
+// product of the front is one product of the whole block with the front's
+// right-hand-side rows ([Y_own - Yin; -Z] = [strict(L11^-1); W] Yin) and
+// its backward product one product with [Y_own / d; x(struct)].
+// One workgroup per (front, 64-row slab of L21); column blocks of 64 left to
+// right, in place: block cb reads L21 columns >= 64 cb only, which no earlier
+// block has overwritten.
+__global__ void __launch_bounds__(256) k_pinv_w21(double *L, const FrontDesc *fd, const int *list,
+                                                  const int64_t *pref, int cnt) {
+    __shared__ double Ap[16 * MLD];  // [k][row]: L21 rows of the slab, 16 columns
+    __shared__ double Bp[16 * MLD];  // [k][col]: strict(L11^-1) rows j0 .. j0 + 15, 64 columns
+    const int i = seg_find(pref, cnt, blockIdx.x);
+    const FrontDesc d = fd[list[i]];
+    const int r0 = (int)(blockIdx.x - pref[i]) * 64;
+    double *Lf = L + d.loff;
+    const int nc = d.nc, m = d.m, ld = d.ld;
+    for (int cb = 0; cb < nc; cb += 64) {
+        tf_d4 acc[2][2];
+        acc_zero(acc);
+        // strict(L11^-1)[j][col] is zero for j <= col: columns of the block
+        // take rows j >= cb only
+        for (int j0 = cb; j0 < nc; j0 += 16) {
+            for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+                const int k = e >> 6, row = e & 63;  // consecutive threads: consecutive rows of one column
+                Ap[k * MLD + row] = (j0 + k < nc && r0 + row < m) ? Lf[(int64_t)(j0 + k) * ld + nc + r0 + row] : 0.0;
+                const int col = e >> 4, kk = e & 15;  // 16 consecutive rows of one column
+                Bp[kk * MLD + col] = (j0 + kk < nc && cb + col < nc) ? Lf[(int64_t)(cb + col) * ld + j0 + kk] : 0.0;
+            }
+            __syncthreads();
+            mfma64_acc(Ap, MLD, Bp, MLD, 16, acc);
+            __syncthreads();
+        }
+#pragma unroll
+        for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+            for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int row = r0 + ACC_ROW(bi), col = cb + ACC_COL(bj, r);
+                    if (row < m && col < nc) {
+                        double *p = Lf + (int64_t)col * ld + nc + row;
+                        *p = -(*p + acc[bi][bj][r]);
+                    }
+                }
+        __syncthreads();
+    }
 }
 
 __global__ void k_fill0(double *p, int64_t n) {
@@ -688,6 +740,10 @@ namespace tf {
 void launch_transpose_l21(const double *L, double *LT, const FrontDesc *fd, const int *units, int64_t nunits,
                           hipStream_t st) {
     if (nunits > 0) hipLaunchKernelGGL(k_transpose_l21, dim3((unsigned)nunits), dim3(256), 0, st, L, LT, fd, units);
+}
+void launch_pinv_w21(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
+                     hipStream_t st) {
+    if (nwg > 0) hipLaunchKernelGGL(k_pinv_w21, dim3((unsigned)nwg), dim3(256), 0, st, L, fd, list, pref, cnt);
 }
 void launch_fill0(double *p, int64_t n, hipStream_t st) {
     if (n <= 0) return;

@@ -34,7 +34,17 @@
 // MT_OWN_FWD_B: the own forward product of a front without children, whose
 // Yin is the right-hand side itself: B rows are read from b through perm
 // (no pull launch for the lowest level).
-enum { MT_OWN_FWD = 0, MT_STR_FWD = 1, MT_BWD_T = 2, MT_BWD_X = 3, MT_OWN_FWD_B = 4 };
+// Fronts that keep the partitioned inverse (FrontDesc::pinv; L block
+// [strict(L11^-1); W], W = -L21 L11^-1) take one product per direction:
+//   MT_PIN_FWD    rows 0 .. f of the block times Yin: rows < nc give
+//                 Y_own = Yin + s, stored as Y_own / d; rows >= nc give
+//                 Z = -s (the update of the ancestors' rows)
+//   MT_PIN_FWD_B  the same for a front without children (Yin = b through perm)
+//   MT_PIN_BWD    X_own = v + [strict(L11^-1); W]^T [v; x(struct)], v = Y_own / d,
+//                 the inner rows gathered through the front's row list (own
+//                 rows from v, struct rows from x)
+enum { MT_OWN_FWD = 0, MT_STR_FWD = 1, MT_BWD_T = 2, MT_BWD_X = 3, MT_OWN_FWD_B = 4, MT_PIN_FWD = 5,
+       MT_PIN_FWD_B = 6, MT_PIN_BWD = 7 };
 
 template <int NA, int NB>
 __device__ __forceinline__ void mt_acc_zero(tf_d4 acc[NA][NB]) {
@@ -48,7 +58,8 @@ __device__ __forceinline__ void mt_acc_zero(tf_d4 acc[NA][NB]) {
 // (strict) triangle applies, per mode.
 template <int MODE>
 __device__ __forceinline__ void mt_geom(const FrontDesc &d, int &nout, int &roff) {
-    if (MODE == MT_OWN_FWD || MODE == MT_OWN_FWD_B || MODE == MT_BWD_X) { nout = d.nc; roff = 0; }
+    if (MODE == MT_OWN_FWD || MODE == MT_OWN_FWD_B || MODE == MT_BWD_X || MODE == MT_PIN_BWD) { nout = d.nc; roff = 0; }
+    else if (MODE == MT_PIN_FWD || MODE == MT_PIN_FWD_B) { nout = d.f; roff = 0; }
     else if (MODE == MT_STR_FWD) { nout = d.m; roff = d.nc; }
     else { nout = d.nc; roff = d.nc; }
 }
@@ -60,6 +71,7 @@ __device__ __forceinline__ void mt_geom(const FrontDesc &d, int &nout, int &roff
 struct MtEdgeOut {
     const int *perm;
     double *x1, *x2;
+    double *z;  // partitioned forward: the front's Z rows (at soff)
 };
 
 template <int MODE>
@@ -75,7 +87,15 @@ __device__ __forceinline__ void mt_store(const FrontDesc &d, int o, int q, doubl
     } else if (MODE == MT_BWD_T) {  // T = Y/d - s
         const int64_t oo = (int64_t)(d.c0 + o) * K + q;
         out[oo] = in1[oo] / Dn[d.doff + o] - s;
-    } else {  // X = T + s (front order unless out is null: rows no lower level gathers)
+    } else if (MODE == MT_PIN_FWD || MODE == MT_PIN_FWD_B) {  // v = (Yin + s) / d, or Z = -s
+        if (o < d.nc) {
+            const int64_t oo = (int64_t)(d.c0 + o) * K + q;
+            const double yin = MODE == MT_PIN_FWD ? in0[oo] : in0[(int64_t)xo.perm[d.c0 + o] * K + q];
+            out[oo] = (yin + s) / Dn[d.doff + o];
+        } else {
+            xo.z[(d.soff + o - d.nc) * K + q] = -s;
+        }
+    } else {  // X = T + s (front order unless out is null: rows no lower level gathers); PIN: X = v + s
         const int64_t oo = (int64_t)(d.c0 + o) * K + q;
         const double v = in1[oo] + s;
         if (out) out[oo] = v;
@@ -264,10 +284,15 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
     const uint32_t bbytes = (uint32_t)((int64_t)d.nc * K * 8);  // the front's own right-hand-side rows
     // the strict triangle of X11 needs no mask: the factorisation leaves the
     // diagonal and upper part of L11's storage zero
-    if (MODE == MT_OWN_FWD || MODE == MT_STR_FWD)
+    if (MODE == MT_OWN_FWD || MODE == MT_STR_FWD || MODE == MT_PIN_FWD)
         mt_wave<false, false, KS, NB, NA>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
-    else if (MODE == MT_OWN_FWD_B)
+    else if (MODE == MT_OWN_FWD_B || MODE == MT_PIN_FWD_B)
         mt_wave<false, true, KS, NB, NA>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, in0, gbytes, xo.perm + d.c0, K, acc);
+    else if (MODE == MT_PIN_BWD && LT)  // all f rows from the row-major copy, gathered [v; x] rows (sidx: row lists)
+        mt_wave<false, true, KS, NB, NA>(LT + d.toff, (uint32_t)((int64_t)d.f * d.ldt * 8), d.ldt, 0, tu.t0, tu.k0,
+                                     tu.k1, in0, gbytes, sidx + d.voff, K, acc);
+    else if (MODE == MT_PIN_BWD)
+        mt_wave<true, true, KS, NB, NA>(Lf, lbytes, d.ld, 0, tu.t0, tu.k0, tu.k1, in0, gbytes, sidx + d.voff, K, acc);
     else if (MODE == MT_BWD_T && LT)  // L21 read from its row-major copy: N form over struct rows
         mt_wave<false, true, KS, NB, NA>(LT + d.toff, (uint32_t)((int64_t)d.m * d.ldt * 8), d.ldt, 0, tu.t0, tu.k0,
                                      tu.k1, in0, gbytes, sidx + d.soff, K, acc);
@@ -336,9 +361,11 @@ __global__ void __launch_bounds__(256) k_mtile_red(const FrontDesc *fd, const Ti
 // zeroed before use; per-lane offsets past the L block read zero (buffer
 // range check), so columns past the front read nothing.
 template <int MODE>
-__device__ __forceinline__ double st_b(const FrontDesc &d, int k, const double *in0, const int *sidx, const int *perm) {
-    if (MODE == MT_OWN_FWD_B) return in0[perm[d.c0 + k]];
+__device__ __forceinline__ double st_b(const FrontDesc &d, int k, const double *in0, const double *in1,
+                                       const int *sidx, const int *perm) {
+    if (MODE == MT_OWN_FWD_B || MODE == MT_PIN_FWD_B) return in0[perm[d.c0 + k]];
     if (MODE == MT_BWD_T) return in0[sidx[d.soff + k]];
+    if (MODE == MT_PIN_BWD) return k < d.nc ? in1[d.c0 + k] : in0[sidx[d.soff + k - d.nc]];  // [v; x(struct)]
     return in0[d.c0 + k];
 }
 
@@ -353,8 +380,8 @@ __device__ __forceinline__ void st_out(const FrontDesc &d, const TileUnit &tu, i
 
 template <int MODE>
 __global__ void __launch_bounds__(256) k_stile_n(const double *L, const FrontDesc *fd, const TileUnit *units,
-                                                 int nunits, const double *in0, double *out, double *part,
-                                                 const int *perm) {
+                                                 int nunits, const double *in0, const double *Dn, double *out,
+                                                 double *part, MtEdgeOut xo) {
     constexpr int U = 8;  // column pairs per step
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int u = xcd_block(blockIdx.x, gridDim.x) * 4 + w;
@@ -366,7 +393,6 @@ __global__ void __launch_bounds__(256) k_stile_n(const double *L, const FrontDes
     const int lane = threadIdx.x & 63, cp = lane >> 5;
     const int o = tu.t0 + 2 * (lane & 31);  // rows o, o + 1 (of the block's rows from roff)
     const __amdgpu_buffer_rsrc_t rL = mt_rsrc(L + d.loff, (uint32_t)((int64_t)d.nc * d.ld * 8));
-    const MtEdgeOut xo{perm, nullptr, nullptr};
     double acc0 = 0.0, acc1 = 0.0;
     for (int k = tu.k0; k < tu.k1; k += 2 * U) {
         double x0[U], x1[U], bv[U];
@@ -374,7 +400,7 @@ __global__ void __launch_bounds__(256) k_stile_n(const double *L, const FrontDes
         for (int q = 0; q < U; q++) {
             const int kk = k + 2 * q + cp;
             mt_bld2(rL, (uint32_t)(kk * d.ld + roff + o) * 8u, 0u, x0[q], x1[q]);
-            bv[q] = st_b<MODE>(d, min(kk, tu.k1 - 1), in0, nullptr, perm);
+            bv[q] = st_b<MODE>(d, min(kk, tu.k1 - 1), in0, nullptr, nullptr, xo.perm);
         }
 #pragma unroll
         for (int q = 0; q < U; q++) {
@@ -387,8 +413,8 @@ __global__ void __launch_bounds__(256) k_stile_n(const double *L, const FrontDes
     acc0 += __shfl_xor(acc0, 32, 64);
     acc1 += __shfl_xor(acc1, 32, 64);
     if (cp == 0) {
-        st_out<MODE>(d, tu, o, acc0, nout, in0, nullptr, nullptr, out, part, xo);
-        st_out<MODE>(d, tu, o + 1, acc1, nout, in0, nullptr, nullptr, out, part, xo);
+        st_out<MODE>(d, tu, o, acc0, nout, in0, nullptr, Dn, out, part, xo);
+        st_out<MODE>(d, tu, o + 1, acc1, nout, in0, nullptr, Dn, out, part, xo);
     }
 }
 
@@ -415,8 +441,8 @@ __global__ void __launch_bounds__(256) k_stile_t(const double *L, const FrontDes
         for (int kb = kb0; kb < tu.k1; kb += 128) {
             const int r0 = kb + 2 * lane;
             const bool v0 = r0 >= tu.k0 && r0 < tu.k1, v1 = r0 + 1 >= tu.k0 && r0 + 1 < tu.k1;
-            const double b0 = v0 ? st_b<MODE>(d, r0, in0, sidx, nullptr) : 0.0;
-            const double b1 = v1 ? st_b<MODE>(d, r0 + 1, in0, sidx, nullptr) : 0.0;
+            const double b0 = v0 ? st_b<MODE>(d, r0, in0, in1, sidx, nullptr) : 0.0;
+            const double b1 = v1 ? st_b<MODE>(d, r0 + 1, in0, in1, sidx, nullptr) : 0.0;
             double x0[NC], x1[NC];
 #pragma unroll
             for (int j = 0; j < NC; j++)
@@ -455,10 +481,10 @@ namespace tf {
 void launch_mtile(int mode, int ks, int nb, int na, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits,
                   int K, const double *in0, const double *in1, const double *Dn, const int *sidx, double *out,
                   double *part, uint32_t gbytes, const int *perm, double *x1, double *x2, const double *LT,
-                  hipStream_t st) {
+                  hipStream_t st, double *z) {
     if (nunits <= 0) return;
     const dim3 g((nunits + 3) / 4), b(256);
-    const MtEdgeOut xo{perm, x1, x2};
+    const MtEdgeOut xo{perm, x1, x2, z};
 #define TF_MT(M, S, N)                                                                                     \
     if (na == 1)                                                                                           \
         hipLaunchKernelGGL((k_mtile<M, S, N, 1>), g, b, 0, st, L, fd, units, nunits, K, in0, in1, Dn, sidx, \
@@ -472,6 +498,9 @@ void launch_mtile(int mode, int ks, int nb, int na, const double *L, const Front
     case MT_OWN_FWD_B: TF_MT(MT_OWN_FWD_B, S, N); break;         \
     case MT_STR_FWD: TF_MT(MT_STR_FWD, S, N); break;             \
     case MT_BWD_T: TF_MT(MT_BWD_T, S, N); break;                 \
+    case MT_PIN_FWD: TF_MT(MT_PIN_FWD, S, N); break;             \
+    case MT_PIN_FWD_B: TF_MT(MT_PIN_FWD_B, S, N); break;         \
+    case MT_PIN_BWD: TF_MT(MT_PIN_BWD, S, N); break;             \
     default: TF_MT(MT_BWD_X, S, N); break;                       \
     }
     if (nb == 4) {
@@ -492,19 +521,28 @@ void launch_red_ps(int mode, int nb, const FrontDesc *fd, const TileUnit *units,
     case MT_OWN_FWD_B: hipLaunchKernelGGL((k_mtile_red<MT_OWN_FWD_B, PS>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
     case MT_STR_FWD: hipLaunchKernelGGL((k_mtile_red<MT_STR_FWD, PS>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
     case MT_BWD_T: hipLaunchKernelGGL((k_mtile_red<MT_BWD_T, PS>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
+    case MT_PIN_FWD: hipLaunchKernelGGL((k_mtile_red<MT_PIN_FWD, PS>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
+    case MT_PIN_FWD_B: hipLaunchKernelGGL((k_mtile_red<MT_PIN_FWD_B, PS>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
+    case MT_PIN_BWD: hipLaunchKernelGGL((k_mtile_red<MT_PIN_BWD, PS>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
     default: hipLaunchKernelGGL((k_mtile_red<MT_BWD_X, PS>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
     }
 }
 void launch_stile(int mode, int nc, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits,
                   const double *in0, const double *in1, const double *Dn, const int *sidx, double *out, double *part,
-                  const int *perm, double *x1, double *x2, hipStream_t st) {
+                  const int *perm, double *x1, double *x2, hipStream_t st, double *z) {
     if (nunits <= 0) return;
     const dim3 g((nunits + 3) / 4), b(256);
-    const MtEdgeOut xo{perm, x1, x2};
+    const MtEdgeOut xo{perm, x1, x2, z};
     switch (mode) {
-    case MT_OWN_FWD: hipLaunchKernelGGL(k_stile_n<MT_OWN_FWD>, g, b, 0, st, L, fd, units, nunits, in0, out, part, perm); break;
-    case MT_OWN_FWD_B: hipLaunchKernelGGL(k_stile_n<MT_OWN_FWD_B>, g, b, 0, st, L, fd, units, nunits, in0, out, part, perm); break;
-    case MT_STR_FWD: hipLaunchKernelGGL(k_stile_n<MT_STR_FWD>, g, b, 0, st, L, fd, units, nunits, in0, out, part, perm); break;
+    case MT_OWN_FWD: hipLaunchKernelGGL(k_stile_n<MT_OWN_FWD>, g, b, 0, st, L, fd, units, nunits, in0, Dn, out, part, xo); break;
+    case MT_OWN_FWD_B: hipLaunchKernelGGL(k_stile_n<MT_OWN_FWD_B>, g, b, 0, st, L, fd, units, nunits, in0, Dn, out, part, xo); break;
+    case MT_STR_FWD: hipLaunchKernelGGL(k_stile_n<MT_STR_FWD>, g, b, 0, st, L, fd, units, nunits, in0, Dn, out, part, xo); break;
+    case MT_PIN_FWD: hipLaunchKernelGGL(k_stile_n<MT_PIN_FWD>, g, b, 0, st, L, fd, units, nunits, in0, Dn, out, part, xo); break;
+    case MT_PIN_FWD_B: hipLaunchKernelGGL(k_stile_n<MT_PIN_FWD_B>, g, b, 0, st, L, fd, units, nunits, in0, Dn, out, part, xo); break;
+    case MT_PIN_BWD:
+        if (nc == 32) hipLaunchKernelGGL((k_stile_t<MT_PIN_BWD, 32>), g, b, 0, st, L, fd, units, nunits, in0, in1, Dn, sidx, out, part, xo);
+        else hipLaunchKernelGGL((k_stile_t<MT_PIN_BWD, 16>), g, b, 0, st, L, fd, units, nunits, in0, in1, Dn, sidx, out, part, xo);
+        break;
     case MT_BWD_T:
         if (nc == 32) hipLaunchKernelGGL((k_stile_t<MT_BWD_T, 32>), g, b, 0, st, L, fd, units, nunits, in0, in1, Dn, sidx, out, part, xo);
         else hipLaunchKernelGGL((k_stile_t<MT_BWD_T, 16>), g, b, 0, st, L, fd, units, nunits, in0, in1, Dn, sidx, out, part, xo);
@@ -536,23 +574,26 @@ __global__ void __launch_bounds__(256) k_stile_red(const FrontDesc *fd, const Ti
 }
 void launch_stile_red(int mode, const FrontDesc *fd, const TileUnit *units, int nunits, const double *in0,
                       const double *in1, const double *Dn, double *out, const double *part, const int *perm,
-                      double *x1, double *x2, hipStream_t st) {
+                      double *x1, double *x2, hipStream_t st, double *z) {
     if (nunits <= 0) return;
     const dim3 g((nunits + 3) / 4), b(256);
-    const MtEdgeOut xo{perm, x1, x2};
+    const MtEdgeOut xo{perm, x1, x2, z};
     switch (mode) {
     case MT_OWN_FWD: hipLaunchKernelGGL(k_stile_red<MT_OWN_FWD>, g, b, 0, st, fd, units, nunits, in0, in1, Dn, out, part, xo); break;
     case MT_OWN_FWD_B: hipLaunchKernelGGL(k_stile_red<MT_OWN_FWD_B>, g, b, 0, st, fd, units, nunits, in0, in1, Dn, out, part, xo); break;
     case MT_STR_FWD: hipLaunchKernelGGL(k_stile_red<MT_STR_FWD>, g, b, 0, st, fd, units, nunits, in0, in1, Dn, out, part, xo); break;
     case MT_BWD_T: hipLaunchKernelGGL(k_stile_red<MT_BWD_T>, g, b, 0, st, fd, units, nunits, in0, in1, Dn, out, part, xo); break;
+    case MT_PIN_FWD: hipLaunchKernelGGL(k_stile_red<MT_PIN_FWD>, g, b, 0, st, fd, units, nunits, in0, in1, Dn, out, part, xo); break;
+    case MT_PIN_FWD_B: hipLaunchKernelGGL(k_stile_red<MT_PIN_FWD_B>, g, b, 0, st, fd, units, nunits, in0, in1, Dn, out, part, xo); break;
+    case MT_PIN_BWD: hipLaunchKernelGGL(k_stile_red<MT_PIN_BWD>, g, b, 0, st, fd, units, nunits, in0, in1, Dn, out, part, xo); break;
     default: hipLaunchKernelGGL(k_stile_red<MT_BWD_X>, g, b, 0, st, fd, units, nunits, in0, in1, Dn, out, part, xo); break;
     }
 }
 void launch_mtile_red(int mode, int nb, int na, const FrontDesc *fd, const TileUnit *units, int nunits, int K,
                       const double *in0, const double *in1, const double *Dn, double *out, const double *part,
-                      const int *perm, double *x1, double *x2, hipStream_t st) {
+                      const int *perm, double *x1, double *x2, hipStream_t st, double *z) {
     if (nunits <= 0) return;
-    const MtEdgeOut xo{perm, x1, x2};
+    const MtEdgeOut xo{perm, x1, x2, z};
     if (na == 1)
         launch_red_ps<16>(mode, nb, fd, units, nunits, K, in0, in1, Dn, out, part, xo, st);
     else

@@ -38,7 +38,11 @@ struct FrontDesc {
     int ld;        // column stride of the L block: f rounded up to 16
     int doff;      // offset of the front's pivots in the D array (c0 for one model)
     int ldt;       // row stride of the L21 copy: nc rounded up to 16
-    int pad;
+    // 1: the front keeps the partitioned inverse -- its L block holds
+    // [strict(L11^-1); -L21 L11^-1] and its row-major copy (f x nc, row
+    // stride ldt) all f rows of that block; 0: [strict(L11^-1); L21], copy of
+    // the m L21 rows
+    int pinv;
 };
 
 // Pivot report of one factorisation.  The factor kernels set `bad` for a

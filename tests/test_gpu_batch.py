@@ -89,6 +89,35 @@ def test_fused_front_kernels_match_tile_launches():
         fe.close()
 
 
+@pytest.mark.parametrize("sources", [[0, 1, 2, 3], [1]])
+def test_partitioned_inverse_matches_l21(sources):
+    """The lower levels' partitioned inverse (L21 replaced by -L21 L11^-1 at
+    factorisation; one tile product per front and direction) against the
+    plain L21 factor: block solves with and without the row-major copy (four
+    sources / one source) and the single right-hand-side solve agree to
+    rounding and keep an FP64 backward error."""
+    from temf3dt_amd import workloads
+    m = workloads.multi_source_small_model()
+    fe = gpu_from_model(m, loops=sources)
+    rng = np.random.default_rng(13)
+    B = rng.standard_normal((64, fe.nE))
+    out = {}
+    try:
+        for pv in (0, 10):
+            print(f"[gpu] partitioned inverse: sources {sources}, pinv={pv}", flush=True)
+            fe.set_tuning("pinv", pv)
+            fe.factor(0, 1e-8)
+            out[pv] = (fe.solve(0, B), fe.solve(0, B[:20]), fe.solve(0, B[3]))
+    finally:
+        fe.set_tuning("pinv", 10)
+    for i, name in enumerate(("64 rhs", "20 rhs", "1 rhs")):
+        a, b = out[0][i], out[10][i]
+        rel = np.abs(b - a).max() / np.abs(a).max()
+        print(f"[gpu] pinv vs L21, {name}: {rel:.3e}", flush=True)
+        assert rel < 1e-10
+    fe.close()
+
+
 @pytest.fixture(scope="module")
 def gpu_mod_big_batch(src_mod_big, oracle_mod_big):
     from temf3dt_amd import fetd
@@ -221,7 +250,7 @@ def test_tuning_rejects_bad_values(gpu_mod_big_batch):
     from temf3dt_amd.fetd import FetdError
     fe, _ = gpu_mod_big_batch
     for k, v in [("mt_ks_n", 3), ("mt_chmin", 10), ("mt_units", 1), ("graphs", 2), ("overlap", -1), ("mt_wide", -1),
-                 ("mt_fuse", -1), ("no_such_knob", 1)]:
+                 ("mt_fuse", -1), ("pinv", -1), ("s_tiles", 0), ("no_such_knob", 1)]:
         with pytest.raises(FetdError):
             fe.set_tuning(k, v)
 

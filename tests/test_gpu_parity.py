@@ -117,6 +117,7 @@ def test_single_solve_tile_and_matrix_vector_paths_agree(gpu_mod_big):
     the matrix-vector kernels (0) on one factor: the same solution up to
     summation order, for several launch sizings of the tiles."""
     fe, _, _ = gpu_mod_big
+    fe.set_tuning("pinv", 0)  # L21 kept as it is (the matrix-vector kernels read it)
     fe.factor(0, 1e-7)
     b = np.random.default_rng(17).standard_normal(fe.nE)
     fe.set_tuning("s_tiles", 0)
@@ -134,4 +135,12 @@ def test_single_solve_tile_and_matrix_vector_paths_agree(gpu_mod_big):
     fe.set_tuning("s_tiles", 2)
     fe.set_tuning("s_units", 8192)
     fe.set_tuning("s_chmin", 256)
+    # the partitioned inverse of the lower levels (one tile product per
+    # front and direction): the same solution to rounding
+    fe.set_tuning("pinv", 10)
+    fe.factor(0, 1e-7)
+    x2 = fe.solve(0, b)
+    rel = np.abs(x2 - x0).max() / np.abs(x0).max()
+    print(f"[gpu] single solve, partitioned inverse vs matrix-vector: {rel:.3e}", flush=True)
+    assert rel < 1e-10
     fe.release(0)
