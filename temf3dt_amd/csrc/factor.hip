@@ -485,27 +485,45 @@ __global__ void __launch_bounds__(256) k_transpose_l21(const double *L, double *
 // block has overwritten.
 __global__ void __launch_bounds__(256) k_pinv_w21(double *L, const FrontDesc *fd, const int *list,
                                                   const int64_t *pref, int cnt) {
-    __shared__ double Ap[16 * MLD];  // [k][row]: L21 rows of the slab, 16 columns
-    __shared__ double Bp[16 * MLD];  // [k][col]: strict(L11^-1) rows j0 .. j0 + 15, 64 columns
+    constexpr int KC = 32;  // inner rows per chunk; the next chunk is loaded into registers during the products
+    __shared__ double Ap[KC * MLD];  // [k][row]: L21 rows of the slab, KC columns
+    __shared__ double Bp[KC * MLD];  // [k][col]: strict(L11^-1) rows j0 .. j0 + KC - 1, 64 columns
     const int i = seg_find(pref, cnt, blockIdx.x);
     const FrontDesc d = fd[list[i]];
     const int r0 = (int)(blockIdx.x - pref[i]) * 64;
     double *Lf = L + d.loff;
     const int nc = d.nc, m = d.m, ld = d.ld;
+    constexpr int PER = KC * 64 / 256;  // elements of each panel per thread
+    double ra[PER], rb[PER];
+    auto gload = [&](int cb, int j0) {
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = threadIdx.x + 256 * q;
+            const int k = e >> 6, row = e & 63;  // consecutive threads: consecutive rows of one column
+            ra[q] = (j0 + k < nc && r0 + row < m) ? Lf[(int64_t)(j0 + k) * ld + nc + r0 + row] : 0.0;
+            const int col = e / KC, kk = e % KC;  // KC consecutive rows of one column
+            rb[q] = (j0 + kk < nc && cb + col < nc) ? Lf[(int64_t)(cb + col) * ld + j0 + kk] : 0.0;
+        }
+    };
+    auto lstore = [&]() {
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = threadIdx.x + 256 * q;
+            Ap[(e >> 6) * MLD + (e & 63)] = ra[q];
+            Bp[(e % KC) * MLD + e / KC] = rb[q];
+        }
+    };
     for (int cb = 0; cb < nc; cb += 64) {
         tf_d4 acc[2][2];
         acc_zero(acc);
         // strict(L11^-1)[j][col] is zero for j <= col: columns of the block
         // take rows j >= cb only
-        for (int j0 = cb; j0 < nc; j0 += 16) {
-            for (int e = threadIdx.x; e < 16 * 64; e += 256) {
-                const int k = e >> 6, row = e & 63;  // consecutive threads: consecutive rows of one column
-                Ap[k * MLD + row] = (j0 + k < nc && r0 + row < m) ? Lf[(int64_t)(j0 + k) * ld + nc + r0 + row] : 0.0;
-                const int col = e >> 4, kk = e & 15;  // 16 consecutive rows of one column
-                Bp[kk * MLD + col] = (j0 + kk < nc && cb + col < nc) ? Lf[(int64_t)(cb + col) * ld + j0 + kk] : 0.0;
-            }
+        gload(cb, cb);
+        for (int j0 = cb; j0 < nc; j0 += KC) {
+            lstore();
             __syncthreads();
-            mfma64_acc(Ap, MLD, Bp, MLD, 16, acc);
+            if (j0 + KC < nc) gload(cb, j0 + KC);
+            mfma64_acc(Ap, MLD, Bp, MLD, KC, acc);
             __syncthreads();
         }
 #pragma unroll
