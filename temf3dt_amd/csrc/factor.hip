@@ -542,10 +542,18 @@ __global__ void __launch_bounds__(256) k_pinv_w21(double *L, const FrontDesc *fd
     }
 }
 
+// p[0 .. n) = 0 with 16-byte stores (p itself may be 8-byte aligned only:
+// the leading and trailing odd elements are written separately)
 __global__ void k_fill0(double *p, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) p[i] = 0.0;
+    const int64_t h = ((reinterpret_cast<uintptr_t>(p) & 15) != 0 && n > 0) ? 1 : 0;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    if (t == 0) {
+        if (h) p[0] = 0.0;
+        if ((n - h) & 1) p[n - 1] = 0.0;
+    }
+    double2 *q = reinterpret_cast<double2 *>(p + h);
+    const int64_t n2 = (n - h) / 2;
+    for (int64_t i = t; i < n2; i += stride) q[i] = make_double2(0.0, 0.0);
 }
 
 __global__ void k_flags_init(FactorFlags *f) {
@@ -765,7 +773,7 @@ void launch_pinv_w21(double *L, const FrontDesc *fd, const int *list, const int6
 }
 void launch_fill0(double *p, int64_t n, hipStream_t st) {
     if (n <= 0) return;
-    int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+    int64_t blocks = std::min<int64_t>((n / 2 + 255) / 256 + 1, 16384);
     hipLaunchKernelGGL(k_fill0, dim3((unsigned)blocks), dim3(256), 0, st, p, n);
 }
 void launch_flags_init(FactorFlags *f, hipStream_t st) {
