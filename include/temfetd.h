@@ -109,10 +109,6 @@ int tf_analyze(tf_ctx *ctx, int leaf_size);
  *   "mt_ks_t"   k-steps per stage of the T-form kernels (2 or 4, default 2)
  *   "mt_wide"   levels of the elimination tree with at most this many fronts
  *               use 64-row tiles, the others 32-row tiles (default 64, >= 0)
- *   "mt_fuse"   levels whose fronts all have at most 16 * mt_fuse columns
- *               (mt_fuse <= 12) are solved by fused per-front kernels (pull,
- *               own and struct products in one launch forward, T and X
- *               products backward; default 0: tile launches only)
  *   "graphs"    1 (default): regular steps of tf_run (sources off) replay a
  *               captured hipGraph of right-hand side, solve and history copy;
  *               0: every step is launched kernel by kernel.  Bit-identical.

@@ -1,6 +1,6 @@
 """Autotuner for the solve's tile kernels.
 
-The K-source block solve (mtile.hip, mfront.hip) has six knobs (tf_set_tuning):
+The K-source block solve (mtile.hip) has five knobs (tf_set_tuning):
   mt_ks_n / mt_ks_t  k-steps per register stage of the N- and T-form tile
                      kernels (2: two stages in flight, 4: one wider stage)
   mt_units           work units a tile launch aims for (sets how far long
@@ -8,8 +8,6 @@ The K-source block solve (mtile.hip, mfront.hip) has six knobs (tf_set_tuning):
   mt_chmin           smallest inner chunk of a split tile
   mt_wide            levels of at most this many fronts use 64-row tiles,
                      the others 32-row tiles
-  mt_fuse            levels whose fronts all have at most 16 mt_fuse columns
-                     use the fused per-front kernels (mfront.hip; 0: none)
 The single right-hand-side solve (one source, or independent models with one
 source each) has its own three (autotune_single):
   s_units            work units a tile launch aims for
@@ -37,7 +35,6 @@ DEFAULT_SPACE = {
     "mt_units": (2048, 4096, 8192),
     "mt_chmin": (256, 512),
     "mt_wide": (16, 64, 256),
-    "mt_fuse": (0, 3),
 }
 
 

@@ -110,14 +110,6 @@ void launch_stile_red(int mode, const FrontDesc *fd, const TileUnit *units, int 
 void launch_mtile_red(int mode, int nb, int na, const FrontDesc *fd, const TileUnit *units, int nunits, int K,
                       const double *in0, const double *in1, const double *Dn, double *out, const double *part,
                       const int *perm, double *x1, double *x2, hipStream_t st, double *z = nullptr);
-// mfront.hip
-int mfront_max_blocks();
-void launch_mfront_fwd(int ncb, bool leaf, const double *L, const FrontDesc *fd, const int *list, int cnt, int K,
-                       const double *b, const int *perm, const int *gptr, const int *gidx, double *Z, double *Y,
-                       hipStream_t st);
-void launch_mfront_bwd(int ncb, const double *L, const double *LT, const FrontDesc *fd, const int *list, int cnt,
-                       int K, const double *Y, const double *Dn, const int *sidx, double *xw, const int *perm,
-                       double *x1, double *x2, int own_xw, hipStream_t st);
 void launch_mspmv_rhs(int n, int K, const int *ia, const int *ja, const double *val, const double *e2,
                       const double *e3, double *w, double *r, int xcd, hipStream_t st);
 void launch_msource(int ns, int K, const int *edge, const int *src, const double *w, double didt, double cdt,
@@ -184,7 +176,6 @@ enum LaunchKind {
     L_BSMALL, L_BT, L_BX,
     L_GINIT, L_GCHILD, L_GXS, L_MT, L_MTRED, L_MPULL,
     L_ST, L_STRED, L_SPULL,  // single right-hand side through the tile kernels
-    L_MFF, L_MFB,            // fused per-front block-solve products (small fronts of the lowest levels)
     L_PINV                   // factorisation: partitioned inverse of a level's fronts (W = -L21 L11^-1)
 };
 struct Launch {
@@ -196,7 +187,7 @@ struct Launch {
 
 int prof_category(int kind) {
     switch (kind) {
-    case L_MT: case L_MFF: case L_MFB: return TF_PROF_TILE;
+    case L_MT: return TF_PROF_TILE;
     case L_MTRED: return TF_PROF_TILE_RED;
     case L_MPULL: return TF_PROF_PULL;
     case L_FUPD: return TF_PROF_FACTOR_UPDATE;
@@ -210,7 +201,6 @@ int prof_category(int kind) {
 // Execution plan over the front tree: which fronts each launch touches.
 struct Plan {
     std::vector<Launch> factor, fwd, bwd, mfwd, mbwd, sfwd, sbwd;
-    std::vector<Launch> mbwd_nolt;  // block backward without fused launches (no row-major L21 copy)
     std::vector<int> list;
     std::vector<int64_t> pref;
     std::vector<int64_t> woff;  // per supernode: offset in the inversion scratch
@@ -316,7 +306,6 @@ struct tf_ctx {
     int mt_units = 4096, mt_chmin = 256;  // tile-launch sizing (units per launch, min inner chunk)
     int mt_wide = 64;  // levels of at most this many fronts use 64-row tiles, the others 32-row tiles
     int mt_ks_n = 2, mt_ks_t = 2;         // k-steps per stage of the N- and T-form tile kernels
-    int mt_fuse = 0;  // levels whose fronts all have <= 16 mt_fuse columns use the fused per-front kernels
     // fronts on the lowest pinv_levels levels keep the partitioned inverse
     // (FrontDesc::pinv): one tile product per front and direction
     int pinv_levels = 10;
@@ -812,34 +801,8 @@ void build_plan(tf_ctx *c) {
         int64_t &pmax = (kt == L_ST) ? p.spart_max : p.part_max;
         pmax = std::max<int64_t>(pmax, slots);
     };
-    // levels whose fronts all have at most 16 * min(mt_fuse, mfront_max_blocks())
-    // columns: one fused launch per class of row-block count instead of the
-    // pull and the tile launches.  The fused kernels keep a front's own rows
-    // in registers (8 VGPRs per 16 columns), so wide fronts lower their
-    // occupancy below what hides the factor loads' latency.
-    const int mfmax = tf::mfront_max_blocks();
-    auto fusable = [&](int l, const std::vector<int> &fr) {
-        if (l < c->pinv_levels) return false;
-        const int lim = 16 * std::min(c->mt_fuse, mfmax);
-        for (int k : fr)
-            if (s.nc[k] > lim) return false;
-        return !fr.empty();
-    };
-    auto add_fused = [&](std::vector<Launch> &dst, int kind, const std::vector<int> &fr, int aux) {
-        std::vector<std::vector<int>> by(mfmax + 1);
-        for (int k : fr) by[(s.nc[k] + 15) / 16].push_back(k);
-        for (int nb = 1; nb <= mfmax; nb++)
-            if (!by[nb].empty()) {
-                add(dst, kind, nb, by[nb], {});
-                dst.back().aux = aux;
-            }
-    };
     for (int l = 0; l < s.nlevels; l++) {
         std::vector<int> all(s.lev_sn.begin() + s.lev_ptr[l], s.lev_sn.begin() + s.lev_ptr[l + 1]);
-        if (fusable(l, all)) {  // aux: 1 for the childless fronts of the lowest level (no pull)
-            add_fused(p.mfwd, L_MFF, all, l == 0 ? 1 : 0);
-            continue;
-        }
         std::vector<int64_t> ur;
         for (int k : all) ur.push_back((s.nc[k] + 4 * TF_RPW - 1) / (4 * TF_RPW));
         if (l < c->pinv_levels) {  // partitioned inverse: one product per front
@@ -855,29 +818,21 @@ void build_plan(tf_ctx *c) {
         }
         add_tiles(p.mfwd, 1, tiles_of(1, all), nb_of(all));
     }
-    // the backward sweep with fused launches (they read L21 from its
-    // row-major copy) and without (for solves that have no copy)
-    for (int fused = 0; fused < 2; fused++) {
-        std::vector<Launch> &dst = fused ? p.mbwd : p.mbwd_nolt;
-        for (int l = s.nlevels - 1; l >= 0; l--) {
-            std::vector<int> all(s.lev_sn.begin() + s.lev_ptr[l], s.lev_sn.begin() + s.lev_ptr[l + 1]);
-            if (fused && fusable(l, all)) {  // aux: 1 to keep the front-order copy of x for lower levels
-                add_fused(dst, L_MFB, all, l == 0 ? 0 : 1);
-                continue;
-            }
-            if (l < c->pinv_levels) {
-                const size_t nb = dst.size();
-                add_tiles(dst, 7, tiles_of(7, all), nb_of(all));
-                if (l == 0)
-                    for (size_t i = nb; i < dst.size(); i++) dst[i].aux = 1;
-                continue;
-            }
-            add_tiles(dst, 2, tiles_of(2, all), nb_of(all));
-            const size_t nb = dst.size();
-            add_tiles(dst, 3, tiles_of(3, all), nb_of(all));
-            if (l == 0)  // no lower level gathers these rows: edge order only
-                for (size_t i = nb; i < dst.size(); i++) dst[i].aux = 1;
+    // the backward sweep, top level first
+    for (int l = s.nlevels - 1; l >= 0; l--) {
+        std::vector<int> all(s.lev_sn.begin() + s.lev_ptr[l], s.lev_sn.begin() + s.lev_ptr[l + 1]);
+        if (l < c->pinv_levels) {
+            const size_t nb = p.mbwd.size();
+            add_tiles(p.mbwd, 7, tiles_of(7, all), nb_of(all));
+            if (l == 0)
+                for (size_t i = nb; i < p.mbwd.size(); i++) p.mbwd[i].aux = 1;
+            continue;
         }
+        add_tiles(p.mbwd, 2, tiles_of(2, all), nb_of(all));
+        const size_t nb = p.mbwd.size();
+        add_tiles(p.mbwd, 3, tiles_of(3, all), nb_of(all));
+        if (l == 0)  // no lower level gathers these rows: edge order only
+            for (size_t i = nb; i < p.mbwd.size(); i++) p.mbwd[i].aux = 1;
     }
     // One right-hand side through the same four products, with one 16-wide
     // right-hand-side block per tile: 64-row tiles on every level (a tile's
@@ -1415,10 +1370,6 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
                                  l.p0 == 0 ? c->d_mV : c->d_mY, nullptr, Dn, nullptr, l.p0 == 0 ? c->d_mY : c->d_mZ,
                                  c->d_part, 0u, nullptr, nullptr, nullptr, nullptr, c->st);
             break;
-        case L_MFF:
-            tf::launch_mfront_fwd(l.p0, l.aux == 1, L, fd, lst + l.off, l.cnt, K, b, sperm(c), c->d_gptr, c->d_gidx,
-                                  c->d_mZ, c->d_mY, c->st);
-            break;
         case L_MTRED:
             if (l.p0 == 5 || l.p0 == 6)
                 tf::launch_mtile_red(l.p0, l.nb, na, fd, c->d_units + l.off, l.cnt, K, l.p0 == 6 ? b : c->d_mV,
@@ -1435,7 +1386,7 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
     }
     // backward T (from x at the struct rows, Y, D), backward X (from T); the
     // backward X epilogues write the finished rows in edge order as well
-    for (const Launch &l : (LT ? p.mbwd : p.mbwd_nolt)) {
+    for (const Launch &l : p.mbwd) {
         const int pm = prof_mark(c);
         const bool bx = l.p0 == 3 || l.p0 == 7;
         double *out = bx ? (l.aux == 1 ? nullptr : mxw) : c->d_mT;
@@ -1450,10 +1401,6 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
             continue;
         }
         switch (l.kind) {
-        case L_MFB:
-            tf::launch_mfront_bwd(l.p0, L, LT, fd, lst + l.off, l.cnt, K, c->d_mY, Dn, c->d_sidx, mxw, sperm(c),
-                                  x, x2, l.aux, c->st);
-            break;
         case L_MT:
             tf::launch_mtile(l.p0, c->mt_ks_t, l.nb, na, L, fd, c->d_units + l.off, l.cnt, K, bx ? c->d_mT : mxw,
                              bx ? c->d_mT : c->d_mY, Dn, c->d_sidx, out, c->d_part, gbytes, sperm(c),
@@ -2726,10 +2673,6 @@ int tf_set_tuning(tf_ctx *c, const char *key, int value) {
         } else if (k == "mt_wide") {
             if (value < 0) return fail(c, TF_ERR_ARG, "mt_wide must be >= 0");
             c->mt_wide = value;
-            replan = true;
-        } else if (k == "mt_fuse") {
-            if (value < 0) return fail(c, TF_ERR_ARG, "mt_fuse must be >= 0");
-            c->mt_fuse = value;
             replan = true;
         } else if (k == "graphs") {
             if (value != 0 && value != 1) return fail(c, TF_ERR_ARG, "graphs must be 0 or 1");

@@ -63,32 +63,6 @@ def test_block_solve_row_major_l21_matches_single_rhs():
     fe.close()
 
 
-def test_fused_front_kernels_match_tile_launches():
-    """The fused per-front kernels of the lowest levels (pull, own and struct
-    products forward; T and X products backward, L21 from its row-major copy)
-    agree with the per-level tile launches, on full and partial 16-wide
-    right-hand-side blocks."""
-    from temf3dt_amd import workloads
-    m = workloads.multi_source_small_model()
-    fe = gpu_from_model(m)  # four sources: the row-major L21 copy exists
-    fe.factor(0, 1e-8)
-    rng = np.random.default_rng(5)
-    try:
-        for nrhs in (20, 64):
-            B = rng.standard_normal((nrhs, fe.nE))
-            out = {}
-            for fuse in (0, 12):
-                print(f"[gpu] fused fronts: nrhs={nrhs} mt_fuse={fuse}", flush=True)
-                fe.set_tuning("mt_fuse", fuse)
-                out[fuse] = fe.solve(0, B)
-            rel = np.linalg.norm(out[12] - out[0]) / np.linalg.norm(out[0])
-            print(f"[gpu] fused vs tiles: {rel:.3e}", flush=True)
-            assert rel < 1e-12
-    finally:
-        fe.set_tuning("mt_fuse", 0)
-        fe.close()
-
-
 @pytest.mark.parametrize("sources", [[0, 1, 2, 3], [1]])
 def test_partitioned_inverse_matches_l21(sources):
     """The lower levels' partitioned inverse (L21 replaced by -L21 L11^-1 at
@@ -214,8 +188,6 @@ def test_overlapped_factor_matches_serial(loops):
     {"mt_units": 100000, "mt_chmin": 4096, "mt_ks_n": 2, "mt_ks_t": 4},  # no splits
     {"mt_wide": 0},        # 32-row tiles on every level
     {"mt_wide": 100000, "mt_units": 64, "mt_chmin": 16},  # 64-row tiles on every level, split
-    {"mt_fuse": 3},        # fused per-front kernels on the levels of fronts <= 48 columns
-    {"mt_fuse": 12},       # fused kernels on every level whose fronts fit them
 ])
 def test_block_solve_tuning_knobs(gpu_mod_big_batch, knobs):
     """The tile-kernel knobs change only the schedule and the summation
@@ -233,8 +205,7 @@ def test_block_solve_tuning_knobs(gpu_mod_big_batch, knobs):
         X = fe.solve(0, B)
         ms = fe.time_solve(0, 24, 2)
     finally:
-        for k, v in {"mt_units": 4096, "mt_chmin": 256, "mt_ks_n": 2, "mt_ks_t": 2, "mt_wide": 64,
-                     "mt_fuse": 0}.items():
+        for k, v in {"mt_units": 4096, "mt_chmin": 256, "mt_ks_n": 2, "mt_ks_t": 2, "mt_wide": 64}.items():
             fe.set_tuning(k, v)
     assert ms > 0
     rel = np.linalg.norm(X - X0) / np.linalg.norm(X0)
@@ -250,7 +221,7 @@ def test_tuning_rejects_bad_values(gpu_mod_big_batch):
     from temf3dt_amd.fetd import FetdError
     fe, _ = gpu_mod_big_batch
     for k, v in [("mt_ks_n", 3), ("mt_chmin", 10), ("mt_units", 1), ("graphs", 2), ("overlap", -1), ("mt_wide", -1),
-                 ("mt_fuse", -1), ("pinv", -1), ("s_tiles", 0), ("no_such_knob", 1)]:
+                 ("mt_fuse", 3), ("pinv", -1), ("s_tiles", 0), ("no_such_knob", 1)]:
         with pytest.raises(FetdError):
             fe.set_tuning(k, v)
 
