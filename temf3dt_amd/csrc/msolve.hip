@@ -219,6 +219,55 @@ __global__ void __launch_bounds__(256) k_mpull(const FrontDesc *fd, const int *l
     }
 }
 
+// The same pull with two rows per wave at a time and two right-hand sides per
+// lane (K even): lanes 0-31 take one row and lanes 32-63 the next, each lane
+// the columns 2j, 2j + 1 as one 16-byte load of every Z row it subtracts.
+// Each half walks its own row's Z indices (held across its 32 lanes and
+// fetched with a lane shuffle) in gidx order, so the sums are those of
+// k_mpull bit for bit; a half whose row has fewer entries subtracts zeros.
+__global__ void __launch_bounds__(256) k_mpull2(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
+                                                const double *b, const int *perm, const int *gptr, const int *gidx,
+                                                const double *Z, double *Yin, int K) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const int i = seg_find(pref, cnt, blockIdx.x);
+    const FrontDesc d = fd[list[i]];
+    const int lane = threadIdx.x & 63, h = lane >> 5, j = lane & 31;
+    const int q = min(2 * j, K - 2);  // K even: q, q + 1 < K
+    const int r0 = (int)(blockIdx.x - pref[i]) * 4 * TF_RPW + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int rr = 0; rr < TF_RPW; rr += 2) {
+        const int r = r0 + 4 * (rr + h);
+        if (r0 + 4 * rr >= d.nc) return;
+        const bool live = r < d.nc;
+        const int row = d.c0 + min(r, d.nc - 1);
+        d2 s = *(const d2 *)(b + (int64_t)perm[row] * K + q);
+        const int p0 = gptr[row], p1 = live ? gptr[row + 1] : p0;
+        const int n = p1 - p0;
+        const int nmax = max(n, __shfl_xor(n, 32, 64));
+        for (int pb = 0; pb < nmax; pb += 32) {
+            const int mine = pb + j < n ? gidx[p0 + pb + j] : 0;
+            const int m = min(32, nmax - pb);
+            int t = 0;
+            for (; t + 8 <= m; t += 8) {
+                d2 z[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int g = __shfl(mine, (lane & 32) + t + u, 64);
+                    const d2 v = *(const d2 *)(Z + (int64_t)g * K + q);
+                    z[u] = pb + t + u < n ? v : (d2){0.0, 0.0};
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++) s -= z[u];
+            }
+            for (; t < m; t++) {
+                const int g = __shfl(mine, (lane & 32) + t, 64);
+                const d2 v = *(const d2 *)(Z + (int64_t)g * K + q);
+                s -= pb + t < n ? v : (d2){0.0, 0.0};
+            }
+        }
+        if (live && 2 * j < K) *(d2 *)(Yin + (int64_t)row * K + 2 * j) = s;
+    }
+}
+
 // The same pull for one right-hand side: a thread per own row, 64 rows per
 // workgroup, the row's Z entries subtracted in gidx order (the order of
 // k_mpull, so one column of a block solve and this give the same bits).
@@ -261,7 +310,11 @@ void launch_fill_pattern(int64_t len, double *b, hipStream_t st) {
 void launch_mpull(const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg, const double *b,
                   const int *perm, const int *gptr, const int *gidx, const double *Z, double *Yin, int K,
                   hipStream_t st) {
-    if (nwg > 0)
+    if (nwg <= 0) return;
+    if (K % 2 == 0)
+        hipLaunchKernelGGL(k_mpull2, dim3((unsigned)nwg), dim3(256), 0, st, fd, list, pref, cnt, b, perm, gptr, gidx, Z,
+                           Yin, K);
+    else
         hipLaunchKernelGGL(k_mpull, dim3((unsigned)nwg), dim3(256), 0, st, fd, list, pref, cnt, b, perm, gptr, gidx, Z, Yin,
                            K);
 }

@@ -23,8 +23,17 @@
 //
 // v_mfma_f64_16x16x4f64 lane layout (measured): lane l supplies A[l%16][l/16]
 // and B[l/16][l%16] and receives D[4r + l/16][l%16] in result register r.
-// A carries the matrix (output index = l%16), B the right-hand sides, so
-// result register r of block (a, b) is output 16b + 4r + l/16, rhs 16a + l%16.
+// A carries the matrix (output index = l%16), B the right-hand sides.  The
+// operands are fetched two values per lane (16-byte loads, half the load
+// instructions of 8-byte ones): in the N form a lane reads two consecutive
+// rows of a column, which go to the output blocks 2p and 2p + 1, and with
+// NA even a lane reads two consecutive right-hand sides, which go to the
+// blocks 2c and 2c + 1.  Output block b therefore holds the rows (columns in
+// the T form, which keeps 16 consecutive outputs per block)
+//   N form: 32 (b/2) + 2 i + b%2,   T form: 16 b + i       (i = 0..15)
+// and right-hand-side block a the columns 32 (a/2) + 2 j + a%2 (NA even) or
+// j (NA = 1), so result register r of block (a, b) is output i = 4r + l/16,
+// right-hand side j = l%16 of those blocks.
 //
 // Units whose tile needs several inner chunks write a partial tile
 // (part[slot][out][rhs]); a reduction launch sums the chunks in a fixed
@@ -107,6 +116,45 @@ __device__ __forceinline__ void mt_store(const FrontDesc &d, int o, int q, doubl
     }
 }
 
+// The same epilogue for the right-hand sides q and q + 1 at once (q even, K
+// even: every vector offset is even, so the pair is one 16-byte access).
+typedef double mt_d2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ mt_d2 mt_ld2(const double *p) { return *(const mt_d2 *)p; }
+__device__ __forceinline__ void mt_st2(double *p, mt_d2 v) { *(mt_d2 *)p = v; }
+
+template <int MODE>
+__device__ __forceinline__ void mt_store2(const FrontDesc &d, int o, int q, mt_d2 s, int K, const double *in0,
+                                          const double *in1, const double *Dn, double *out, const MtEdgeOut &xo) {
+    if (MODE == MT_OWN_FWD) {
+        const int64_t oo = (int64_t)(d.c0 + o) * K + q;
+        mt_st2(out + oo, mt_ld2(in0 + oo) + s);
+    } else if (MODE == MT_OWN_FWD_B) {
+        mt_st2(out + (int64_t)(d.c0 + o) * K + q, mt_ld2(in0 + (int64_t)xo.perm[d.c0 + o] * K + q) + s);
+    } else if (MODE == MT_STR_FWD) {
+        mt_st2(out + (d.soff + o) * K + q, s);
+    } else if (MODE == MT_BWD_T) {
+        const int64_t oo = (int64_t)(d.c0 + o) * K + q;
+        mt_st2(out + oo, mt_ld2(in1 + oo) / Dn[d.doff + o] - s);
+    } else if (MODE == MT_PIN_FWD || MODE == MT_PIN_FWD_B) {
+        if (o < d.nc) {
+            const int64_t oo = (int64_t)(d.c0 + o) * K + q;
+            const mt_d2 yin = MODE == MT_PIN_FWD ? mt_ld2(in0 + oo) : mt_ld2(in0 + (int64_t)xo.perm[d.c0 + o] * K + q);
+            mt_st2(out + oo, (yin + s) / Dn[d.doff + o]);
+        } else {
+            mt_st2(xo.z + (d.soff + o - d.nc) * K + q, -s);
+        }
+    } else {
+        const int64_t oo = (int64_t)(d.c0 + o) * K + q;
+        const mt_d2 v = mt_ld2(in1 + oo) + s;
+        if (out) mt_st2(out + oo, v);
+        if (xo.x1) {
+            const int64_t e = (int64_t)xo.perm[d.c0 + o] * K + q;
+            mt_st2(xo.x1 + e, v);
+            if (xo.x2) mt_st2(xo.x2 + e, v);
+        }
+    }
+}
+
 // Byte-offset load from a wave-uniform base: the base stays in scalar
 // registers and each lane adds a 32-bit offset (no 64-bit address math per
 // load).
@@ -148,12 +196,14 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
     // can leave the block stays there: the T-form output column (one offset
     // per output block b), while an N-form row past the block's last column
     // reads at most 62 values beyond it (the factor storage ends with slack).
+    constexpr bool PB = NA % 2 == 0;  // right-hand sides in pairs
+    const int jq = PB ? 2 * l16 : l16;  // first right-hand side of the lane
     uint32_t la[TFORM ? NB : 1];
 #pragma unroll
     for (int b = 0; b < (TFORM ? NB : 1); b++)
         la[b] = TFORM ? (uint32_t)((int64_t)(o0 + 16 * b + l16) * f + roff + KS * lk) * 8u
-                      : (uint32_t)(lk * f + roff + o0 + l16) * 8u;
-    const uint32_t ba = GATH ? (uint32_t)l16 * 8u : (uint32_t)((TFORM ? KS * lk : lk) * K + l16) * 8u;
+                      : (uint32_t)(lk * f + roff + o0 + 2 * l16) * 8u;
+    const uint32_t ba = GATH ? (uint32_t)jq * 8u : (uint32_t)((TFORM ? KS * lk : lk) * K + jq) * 8u;
     const int kb = TFORM ? (k0 - ((roff + k0) & 15)) : k0;
     const int nst = (k1 - kb + SW - 1) / SW;
     // the unmasked run of stages [ja, jb)
@@ -178,19 +228,36 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
             const int i = inner(ib, s);
             const bool iv = i >= k0 && i < k1;
             const int ic = min(max(i, k0), k1 - 1);
+            if (TFORM) {
 #pragma unroll
-            for (int b = 0; b < NB; b++) {
-                const int ob = o0 + 16 * b + l16;
-                const uint32_t off = TFORM ? (uint32_t)((int64_t)ob * f + roff + ic) * 8u
-                                           : (uint32_t)((int64_t)ic * f + roff + ob) * 8u;
-                const double x = mt_bld(rL, off, 0u);
-                xa[s][b] = iv ? x : 0.0;
+                for (int b = 0; b < NB; b++) {
+                    const double x = mt_bld(rL, (uint32_t)((int64_t)(o0 + 16 * b + l16) * f + roff + ic) * 8u, 0u);
+                    xa[s][b] = iv ? x : 0.0;
+                }
+            } else {  // the rows of a pair share the inner index, hence the mask
+#pragma unroll
+                for (int p = 0; p < NB / 2; p++) {
+                    double x0, x1;
+                    mt_bld2(rL, (uint32_t)((int64_t)ic * f + roff + o0 + 32 * p + 2 * l16) * 8u, 0u, x0, x1);
+                    xa[s][2 * p] = iv ? x0 : 0.0;
+                    xa[s][2 * p + 1] = iv ? x1 : 0.0;
+                }
             }
+            const uint32_t row = (uint32_t)(GATH ? bidx[ic] : ic);
+            if (PB) {
 #pragma unroll
-            for (int a = 0; a < NA; a++) {
-                const uint32_t row = (uint32_t)(GATH ? bidx[ic] : ic);
-                const double y = mt_bld(rR, (row * (uint32_t)K + 16u * a + l16) * 8u, 0u);
-                yb[s][a] = iv ? y : 0.0;
+                for (int c = 0; c < NA / 2; c++) {
+                    double y0, y1;
+                    mt_bld2(rR, (row * (uint32_t)K + (uint32_t)(32 * c + jq)) * 8u, 0u, y0, y1);
+                    yb[s][2 * c] = iv ? y0 : 0.0;
+                    yb[s][2 * c + 1] = iv ? y1 : 0.0;
+                }
+            } else {
+#pragma unroll
+                for (int a = 0; a < NA; a++) {
+                    const double y = mt_bld(rR, (row * (uint32_t)K + (uint32_t)(16 * a + l16)) * 8u, 0u);
+                    yb[s][a] = iv ? y : 0.0;
+                }
             }
         }
         mult(xa, yb);
@@ -214,19 +281,20 @@ __device__ __forceinline__ void mt_wave(const double *Lf, uint32_t lbytes, int f
 #pragma unroll
             for (int s = 0; s < KS; s++) {
                 const uint32_t sl = (uint32_t)(TFORM ? (ib + s) : (ib + 4 * s) * f) * 8u;
-                if (!(TFORM && KS % 2 == 0))
+                if (TFORM && KS % 2 != 0)
 #pragma unroll
-                    for (int b = 0; b < NB; b++)
-                        xa[s][b] = TFORM ? mt_bld(rL, la[b], sl) : mt_bld(rL, la[0] + 128u * b, sl);
-                if (GATH) {
-                    const uint32_t vo = ((uint32_t)r[s] * (uint32_t)K) * 8u + ba;
+                    for (int b = 0; b < NB; b++) xa[s][b] = mt_bld(rL, la[b], sl);
+                if (!TFORM)  // rows 2 l16 and 2 l16 + 1 of each 32-row pair of blocks
 #pragma unroll
-                    for (int a = 0; a < NA; a++) yb[s][a] = mt_bld(rR, vo + 128u * a, 0u);
-                } else {
-                    const uint32_t sb = (uint32_t)((TFORM ? ib + s : ib + 4 * s) * K) * 8u;
+                    for (int p = 0; p < NB / 2; p++) mt_bld2(rL, la[0] + 256u * p, sl, xa[s][2 * p], xa[s][2 * p + 1]);
+                const uint32_t vo = GATH ? ((uint32_t)r[s] * (uint32_t)K) * 8u + ba : ba;
+                const uint32_t sb = GATH ? 0u : (uint32_t)((TFORM ? ib + s : ib + 4 * s) * K) * 8u;
+                if (PB)
 #pragma unroll
-                    for (int a = 0; a < NA; a++) yb[s][a] = mt_bld(rR, ba, sb + 128u * a);
-                }
+                    for (int c = 0; c < NA / 2; c++) mt_bld2(rR, vo + 256u * c, sb, yb[s][2 * c], yb[s][2 * c + 1]);
+                else
+#pragma unroll
+                    for (int a = 0; a < NA; a++) yb[s][a] = mt_bld(rR, vo + 128u * a, sb);
             }
         };
         double xa0[KS][NB], yb0[KS][NA], xa1[KS][NB], yb1[KS][NA];
@@ -284,6 +352,9 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
     const uint32_t bbytes = (uint32_t)((int64_t)d.nc * K * 8);  // the front's own right-hand-side rows
     // the strict triangle of X11 needs no mask: the factorisation leaves the
     // diagonal and upper part of L11's storage zero
+    // T form (output index across the columns of L): the backward products
+    // that read L itself rather than its row-major copy
+    const bool tform = (MODE == MT_BWD_T || MODE == MT_PIN_BWD) ? LT == nullptr : MODE == MT_BWD_X;
     if (MODE == MT_OWN_FWD || MODE == MT_STR_FWD || MODE == MT_PIN_FWD)
         mt_wave<false, false, KS, NB, NA>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
     else if (MODE == MT_OWN_FWD_B || MODE == MT_PIN_FWD_B)
@@ -301,17 +372,30 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
     else
         mt_wave<true, false, KS, NB, NA>(Lf, lbytes, d.ld, roff, tu.t0, tu.k0, tu.k1, Bm, bbytes, nullptr, K, acc);
     const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
+    // output (relative to t0) of register r of output block b, and right-hand
+    // side of block a: the operand pairing of mt_wave
+    auto orel = [&](int b, int r) { return tform ? 16 * b + 4 * r + lk : 32 * (b >> 1) + 2 * (4 * r + lk) + (b & 1); };
+    auto qof = [&](int a) { return NA % 2 == 0 ? 32 * (a >> 1) + 2 * l16 + (a & 1) : 16 * a + l16; };
     if (tu.slot < 0) {
 #pragma unroll
         for (int b = 0; b < NB; b++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int o = tu.t0 + 16 * b + 4 * r + lk;
+                const int o = tu.t0 + orel(b, r);
                 if (o < 0 || o >= nout) continue;
+                if (NA % 2 == 0 && (K & 1) == 0) {  // blocks a, a + 1 hold right-hand sides q, q + 1
 #pragma unroll
-                for (int a = 0; a < NA; a++) {
-                    const int q = 16 * a + l16;
-                    if (q < K) mt_store<MODE>(d, o, q, acc[a][b][r], K, in0, in1, Dn, out, xo);
+                    for (int a = 0; a < NA; a += 2) {
+                        const int q = qof(a);
+                        if (q < K)
+                            mt_store2<MODE>(d, o, q, (mt_d2){acc[a][b][r], acc[a + 1][b][r]}, K, in0, in1, Dn, out, xo);
+                    }
+                } else {
+#pragma unroll
+                    for (int a = 0; a < NA; a++) {
+                        const int q = qof(a);
+                        if (q < K) mt_store<MODE>(d, o, q, acc[a][b][r], K, in0, in1, Dn, out, xo);
+                    }
                 }
             }
     } else {
@@ -321,8 +405,13 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
         for (int b = 0; b < NB; b++)
 #pragma unroll
             for (int r = 0; r < 4; r++)
+                if (NA % 2 == 0)
 #pragma unroll
-                for (int a = 0; a < NA; a++) pp[(16 * b + 4 * r + lk) * (16 * NA) + 16 * a + l16] = acc[a][b][r];
+                    for (int a = 0; a < NA; a += 2)
+                        mt_st2(pp + orel(b, r) * (16 * NA) + qof(a), (mt_d2){acc[a][b][r], acc[a + 1][b][r]});
+                else
+#pragma unroll
+                    for (int a = 0; a < NA; a++) pp[orel(b, r) * (16 * NA) + qof(a)] = acc[a][b][r];
     }
 }
 
@@ -330,22 +419,31 @@ __global__ void __launch_bounds__(256, 2) k_mtile(const double *L, const FrontDe
 // Reduction units reuse TileUnit: front, t0, k0 = slot0, k1 = nk.  A partial
 // tile holds 64 outputs x PS right-hand sides (PS = 16 NA); 64 PS / 256
 // workgroups per tile, one element per thread.
-template <int MODE, int PS>
+// With P2 (PS and K even) a thread sums two neighbouring right-hand sides
+// with 16-byte loads (64 PS / 512 workgroups per tile).
+template <int MODE, int PS, bool P2>
 __global__ void __launch_bounds__(256) k_mtile_red(const FrontDesc *fd, const TileUnit *units, int K,
                                                    const double *in0, const double *in1, const double *Dn,
                                                    double *out, const double *part, MtEdgeOut xo, int nb) {
-    constexpr int WPT = 64 * PS / 256;  // workgroups per tile
+    constexpr int EPT = P2 ? 2 : 1;               // elements per thread
+    constexpr int WPT = 64 * PS / (256 * EPT);    // workgroups per tile
     const TileUnit tu = units[blockIdx.x / WPT];
     const FrontDesc d = fd[tu.front];
     int nout, roff;
     mt_geom<MODE>(d, nout, roff);
-    const int e = (blockIdx.x % WPT) * 256 + threadIdx.x;
+    const int e = ((blockIdx.x % WPT) * 256 + threadIdx.x) * EPT;
     const int o = tu.t0 + e / PS, q = e % PS;
     if (e / PS >= 16 * nb || o < 0 || o >= nout || q >= K) return;
     const double *p0 = part + (int64_t)tu.k0 * (64 * PS) + e;
-    double s = 0.0;
-    for (int c = 0; c < tu.k1; c++) s += p0[(int64_t)c * (64 * PS)];
-    mt_store<MODE>(d, o, q, s, K, in0, in1, Dn, out, xo);
+    if (P2) {
+        mt_d2 s = (mt_d2){0.0, 0.0};
+        for (int c = 0; c < tu.k1; c++) s += mt_ld2(p0 + (int64_t)c * (64 * PS));
+        mt_store2<MODE>(d, o, q, s, K, in0, in1, Dn, out, xo);
+    } else {
+        double s = 0.0;
+        for (int c = 0; c < tu.k1; c++) s += p0[(int64_t)c * (64 * PS)];
+        mt_store<MODE>(d, o, q, s, K, in0, in1, Dn, out, xo);
+    }
 }
 
 // ---- one right-hand side on the vector ALUs ------------------------------------------
@@ -511,21 +609,23 @@ void launch_mtile(int mode, int ks, int nb, int na, const double *L, const Front
 #undef TF_MT_MODES
 #undef TF_MT
 }
-template <int PS>
+template <int PS, bool P2>
 void launch_red_ps(int mode, int nb, const FrontDesc *fd, const TileUnit *units, int nunits, int K, const double *in0,
                    const double *in1, const double *Dn, double *out, const double *part, const MtEdgeOut &xo,
                    hipStream_t st) {
-    const dim3 g(nunits * (64 * PS / 256)), b(256);
+    const dim3 g(nunits * (64 * PS / (256 * (P2 ? 2 : 1)))), b(256);
+#define TF_RED(M) hipLaunchKernelGGL((k_mtile_red<M, PS, P2>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb)
     switch (mode) {
-    case MT_OWN_FWD: hipLaunchKernelGGL((k_mtile_red<MT_OWN_FWD, PS>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
-    case MT_OWN_FWD_B: hipLaunchKernelGGL((k_mtile_red<MT_OWN_FWD_B, PS>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
-    case MT_STR_FWD: hipLaunchKernelGGL((k_mtile_red<MT_STR_FWD, PS>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
-    case MT_BWD_T: hipLaunchKernelGGL((k_mtile_red<MT_BWD_T, PS>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
-    case MT_PIN_FWD: hipLaunchKernelGGL((k_mtile_red<MT_PIN_FWD, PS>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
-    case MT_PIN_FWD_B: hipLaunchKernelGGL((k_mtile_red<MT_PIN_FWD_B, PS>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
-    case MT_PIN_BWD: hipLaunchKernelGGL((k_mtile_red<MT_PIN_BWD, PS>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
-    default: hipLaunchKernelGGL((k_mtile_red<MT_BWD_X, PS>), g, b, 0, st, fd, units, K, in0, in1, Dn, out, part, xo, nb); break;
+    case MT_OWN_FWD: TF_RED(MT_OWN_FWD); break;
+    case MT_OWN_FWD_B: TF_RED(MT_OWN_FWD_B); break;
+    case MT_STR_FWD: TF_RED(MT_STR_FWD); break;
+    case MT_BWD_T: TF_RED(MT_BWD_T); break;
+    case MT_PIN_FWD: TF_RED(MT_PIN_FWD); break;
+    case MT_PIN_FWD_B: TF_RED(MT_PIN_FWD_B); break;
+    case MT_PIN_BWD: TF_RED(MT_PIN_BWD); break;
+    default: TF_RED(MT_BWD_X); break;
     }
+#undef TF_RED
 }
 void launch_stile(int mode, int nc, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits,
                   const double *in0, const double *in1, const double *Dn, const int *sidx, double *out, double *part,
@@ -595,8 +695,10 @@ void launch_mtile_red(int mode, int nb, int na, const FrontDesc *fd, const TileU
     if (nunits <= 0) return;
     const MtEdgeOut xo{perm, x1, x2, z};
     if (na == 1)
-        launch_red_ps<16>(mode, nb, fd, units, nunits, K, in0, in1, Dn, out, part, xo, st);
+        launch_red_ps<16, false>(mode, nb, fd, units, nunits, K, in0, in1, Dn, out, part, xo, st);
+    else if (K % 2 == 0)
+        launch_red_ps<64, true>(mode, nb, fd, units, nunits, K, in0, in1, Dn, out, part, xo, st);
     else
-        launch_red_ps<64>(mode, nb, fd, units, nunits, K, in0, in1, Dn, out, part, xo, st);
+        launch_red_ps<64, false>(mode, nb, fd, units, nunits, K, in0, in1, Dn, out, part, xo, st);
 }
 }  // namespace tf
