@@ -65,6 +65,52 @@ __global__ void __launch_bounds__(256) k_mspmv_rhs(int n, int K, const int *ia, 
     }
 }
 
+// The same product with two rows per wave at a time and two right-hand sides
+// per lane (K even): lanes 0-31 take one row and lanes 32-63 the next, each
+// lane the columns 2j, 2j + 1 of every gathered row of w as one 16-byte load
+// (half the load instructions of k_mspmv_rhs).  Each half holds up to 32 of
+// its row's column indices and values and shares them by lane shuffles; the
+// terms are summed in CSR order, as in k_mspmv_rhs.
+__global__ void __launch_bounds__(256) k_mspmv_rhs2(int n, int K, const int *ia, const int *ja, const double *val,
+                                                    const double *w, double *r, int xcd) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    constexpr int D = 4;  // gathers in flight per lane
+    const int lane = threadIdx.x & 63, h = lane >> 5, j = lane & 31;
+    const int q = min(2 * j, K - 2);
+    const int blk = xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int row0 = blk * 4 * TF_RPW + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int rr = 0; rr < TF_RPW; rr += 2) {
+        if (row0 + 4 * rr >= n) return;
+        const int row = row0 + 4 * (rr + h);
+        const bool live = row < n;
+        const int p0 = live ? ia[row] : 0, cnt = live ? ia[row + 1] - p0 : 0;
+        const int cm = max(cnt, __shfl_xor(cnt, 32, 64));
+        d2 acc = (d2){0.0, 0.0};
+        for (int pb = 0; pb < cm; pb += 32) {
+            const int myj = pb + j < cnt ? ja[p0 + pb + j] : 0;
+            const double myv = pb + j < cnt ? val[p0 + pb + j] : 0.0;
+            const int m = min(32, cm - pb);
+            for (int t = 0; t < m; t += D) {
+                d2 x[D];
+                double v[D];
+#pragma unroll
+                for (int u = 0; u < D; u++) {
+                    const int src = (lane & 32) + min(t + u, m - 1);
+                    const int c = __shfl(myj, src, 64);
+                    v[u] = __shfl(myv, src, 64);
+                    x[u] = *(const d2 *)(w + (int64_t)c * K + q);
+                }
+#pragma unroll
+                for (int u = 0; u < D; u++) {  // the shorter row's half skips the terms past its end
+                    const d2 a = acc + v[u] * x[u];
+                    acc = t + u < m && pb + t + u < cnt ? a : acc;
+                }
+            }
+        }
+        if (live && 2 * j < K) *(d2 *)(r + (int64_t)row * K + 2 * j) = acc;
+    }
+}
+
 // r[edge][q] -= (2 dt) (didt w) for every source-wire entry (edge, q, w)
 __global__ void k_msource(int ns, int K, const int *edge, const int *src, const double *w, double didt, double cdt,
                           double *r) {
@@ -328,8 +374,11 @@ void launch_mspmv_rhs(int n, int K, const int *ia, const int *ja, const double *
                       const double *e3, double *w, double *r, int xcd, hipStream_t st) {
     const int64_t tot = (int64_t)n * K;
     hipLaunchKernelGGL(k_mhist, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, tot, e2, e3, w);
-    hipLaunchKernelGGL(k_mspmv_rhs, dim3((unsigned)((n + 4 * TF_RPW - 1) / (4 * TF_RPW))), dim3(256), 0, st, n, K, ia,
-                       ja, val, w, r, xcd);
+    const dim3 g((unsigned)((n + 4 * TF_RPW - 1) / (4 * TF_RPW)));
+    if (K % 2 == 0)
+        hipLaunchKernelGGL(k_mspmv_rhs2, g, dim3(256), 0, st, n, K, ia, ja, val, w, r, xcd);
+    else
+        hipLaunchKernelGGL(k_mspmv_rhs, g, dim3(256), 0, st, n, K, ia, ja, val, w, r, xcd);
 }
 void launch_msource(int ns, int K, const int *edge, const int *src, const double *w, double didt, double cdt,
                     double *r, hipStream_t st) {
