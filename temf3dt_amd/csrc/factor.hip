@@ -286,9 +286,13 @@ __device__ void inv_tile(const double *A, int lda, const double *B, int ldb, int
 // never stored), four inner values per k-step, KS k-steps per stage, the
 // next stage in flight while the current one is multiplied; a last partial
 // stage zeroes the columns past kend.  The column operand is scaled by the
-// pivots and passed as the MFMA's A operand, so that result register r of
-// block (a, b) holds column cb + 16a + 4r + l/16, row rb + 16b + l%16: every
-// store covers four 128-byte row segments.
+// pivots and passed as the MFMA's A operand.  Each operand load fetches two
+// consecutive rows of a panel column per lane (one 16-byte load for two
+// 16-wide blocks), so block 2p holds the rows 32p + 2i and block 2p + 1 the
+// rows 32p + 2i + 1 (i = 0..15) of the tile, on either side: result
+// register r of block (a, b) holds column cb + 32(a/2) + 2(4r + l/16) + a%2,
+// row rb + 32(b/2) + 2(l%16) + b%2, and the blocks b = 2p, 2p + 1 of one
+// register are two consecutive rows, updated with one 16-byte access.
 template <int KS>
 __device__ __forceinline__ void upd_wave(double *L, double *U, const FrontDesc &d, const double *Dn, int kbeg,
                                          int kend, int rb, int cb, int cmax) {
@@ -296,7 +300,7 @@ __device__ __forceinline__ void upd_wave(double *L, double *U, const FrontDesc &
     const double *base = L + d.loff;
     const __amdgpu_buffer_rsrc_t rL = mt_rsrc(base, (uint32_t)((int64_t)d.nc * d.ld * 8));
     const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
-    const uint32_t lr = (uint32_t)(lk * d.ld + rb + l16) * 8u, lc = (uint32_t)(lk * d.ld + cb + l16) * 8u;
+    const uint32_t lr = (uint32_t)(lk * d.ld + rb + 2 * l16) * 8u, lc = (uint32_t)(lk * d.ld + cb + 2 * l16) * 8u;
     const double *Dk = Dn + d.doff;
     tf_d4 acc[4][4];
 #pragma unroll
@@ -318,9 +322,14 @@ __device__ __forceinline__ void upd_wave(double *L, double *U, const FrontDesc &
             const uint32_t sl = (uint32_t)((ib + 4 * s) * d.ld) * 8u;
             const double dk = Dk[ib + 4 * s + lk];
 #pragma unroll
-            for (int b = 0; b < 4; b++) xr[s][b] = mt_bld(rL, lr + 128u * b, sl);
+            for (int p = 0; p < 2; p++) mt_bld2(rL, lr + 256u * p, sl, xr[s][2 * p], xr[s][2 * p + 1]);
 #pragma unroll
-            for (int a = 0; a < 4; a++) yc[s][a] = mt_bld(rL, lc + 128u * a, sl) * dk;
+            for (int p = 0; p < 2; p++) {
+                double y0, y1;
+                mt_bld2(rL, lc + 256u * p, sl, y0, y1);
+                yc[s][2 * p] = y0 * dk;
+                yc[s][2 * p + 1] = y1 * dk;
+            }
         }
     };
     const int nfull = (kend - kbeg) / SW;
@@ -358,37 +367,60 @@ __device__ __forceinline__ void upd_wave(double *L, double *U, const FrontDesc &
             const bool kv = k < kend;
             const int kc = kv ? k : kend - 1;
             const uint32_t sl = (uint32_t)(kc * d.ld) * 8u;
-            const uint32_t l0 = (uint32_t)(rb + l16) * 8u, c0 = (uint32_t)(cb + l16) * 8u;
+            const uint32_t l0 = (uint32_t)(rb + 2 * l16) * 8u, c0 = (uint32_t)(cb + 2 * l16) * 8u;
             const double dk = kv ? Dk[kc] : 0.0;
 #pragma unroll
-            for (int b = 0; b < 4; b++) xr[s][b] = mt_bld(rL, sl + l0 + 128u * b, 0u);
+            for (int p = 0; p < 2; p++) mt_bld2(rL, sl + l0 + 256u * p, 0u, xr[s][2 * p], xr[s][2 * p + 1]);
 #pragma unroll
-            for (int a = 0; a < 4; a++) yc[s][a] = mt_bld(rL, sl + c0 + 128u * a, 0u) * dk;
+            for (int p = 0; p < 2; p++) {
+                double y0, y1;
+                mt_bld2(rL, sl + c0 + 256u * p, 0u, y0, y1);
+                yc[s][2 * p] = y0 * dk;
+                yc[s][2 * p + 1] = y1 * dk;
+            }
         }
         mult(xr, yc);
     }
-    // C -= acc on the stored lower triangle; all loads of a column block first
+    // C -= acc on the stored lower triangle; all loads of a column block
+    // first.  A row pair whose rows are both stored and whose address is
+    // 16-byte aligned (L columns always; update-stack columns when their
+    // offset is even) is read and written as one 16-byte access.
+    typedef double u_d2 __attribute__((ext_vector_type(2)));
 #pragma unroll
     for (int a = 0; a < 4; a++) {
-        double cv[4][4];
+        u_d2 cv[4][2];
         double *cp[4];
+        bool both[4][2];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int c = cb + 16 * a + 4 * r + lk;
+            const int c = cb + 32 * (a >> 1) + 2 * (4 * r + lk) + (a & 1);
             cp[r] = (c < cmax) ? front_col(L, U, d, c) : nullptr;
 #pragma unroll
-            for (int b = 0; b < 4; b++) {
-                const int row = rb + 16 * b + l16;
-                cv[r][b] = (cp[r] && row < d.f && row >= c) ? cp[r][row] : 0.0;
+            for (int p = 0; p < 2; p++) {
+                const int row = rb + 32 * p + 2 * l16;
+                const bool v0 = cp[r] && row < d.f && row >= c, v1 = cp[r] && row + 1 < d.f && row + 1 >= c;
+                both[r][p] = v0 && v1 && (((uintptr_t)(cp[r] + row)) & 15) == 0;
+                if (both[r][p]) {
+                    cv[r][p] = *(const u_d2 *)(cp[r] + row);
+                } else {
+                    cv[r][p].x = v0 ? cp[r][row] : 0.0;
+                    cv[r][p].y = v1 ? cp[r][row + 1] : 0.0;
+                }
             }
         }
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int c = cb + 16 * a + 4 * r + lk;
+            const int c = cb + 32 * (a >> 1) + 2 * (4 * r + lk) + (a & 1);
 #pragma unroll
-            for (int b = 0; b < 4; b++) {
-                const int row = rb + 16 * b + l16;
-                if (cp[r] && row < d.f && row >= c) cp[r][row] = cv[r][b] - acc[a][b][r];
+            for (int p = 0; p < 2; p++) {
+                const int row = rb + 32 * p + 2 * l16;
+                const u_d2 nv = cv[r][p] - (u_d2){acc[a][2 * p][r], acc[a][2 * p + 1][r]};
+                if (both[r][p]) {
+                    *(u_d2 *)(cp[r] + row) = nv;
+                } else {
+                    if (cp[r] && row < d.f && row >= c) cp[r][row] = nv.x;
+                    if (cp[r] && row + 1 < d.f && row + 1 >= c) cp[r][row + 1] = nv.y;
+                }
             }
         }
     }
