@@ -103,12 +103,12 @@ int tf_get_receiver_vectors(const tf_ctx *ctx, double *n_arr, double *d_arr);
 int tf_analyze(tf_ctx *ctx, int leaf_size);
 /* Tuning knobs of the block-solve tile kernels (no reference counterpart;
  * set by the autotuner, temf3dt_amd/autotune.py).  Keys:
- *   "mt_units"  work units a tile launch aims for (default 4096, >= 64)
- *   "mt_chmin"  minimum inner chunk of a split tile (default 256, multiple of 16)
+ *   "mt_units"  work units a tile launch aims for (default 8192, >= 64)
+ *   "mt_chmin"  minimum inner chunk of a split tile (default 512, multiple of 16)
  *   "mt_ks_n"   k-steps per stage of the N-form kernels (2 or 4, default 2)
  *   "mt_ks_t"   k-steps per stage of the T-form kernels (2 or 4, default 2)
  *   "mt_wide"   levels of the elimination tree with at most this many fronts
- *               use 64-row tiles, the others 32-row tiles (default 64, >= 0)
+ *               use 64-row tiles, the others 32-row tiles (default 256, >= 0)
  *   "graphs"    1 (default): regular steps of tf_run (sources off) replay a
  *               captured hipGraph of right-hand side, solve and history copy;
  *               0: every step is launched kernel by kernel.  Bit-identical.

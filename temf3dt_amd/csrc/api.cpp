@@ -303,8 +303,9 @@ struct tf_ctx {
     TileUnit *d_units = nullptr;
     int *d_gptr = nullptr, *d_gidx = nullptr;
     double *d_mZ = nullptr;
-    int mt_units = 4096, mt_chmin = 256;  // tile-launch sizing (units per launch, min inner chunk)
-    int mt_wide = 64;  // levels of at most this many fronts use 64-row tiles, the others 32-row tiles
+    // the defaults are the autotuner's choice on the 1.6M-unknown batched mesh
+    int mt_units = 8192, mt_chmin = 512;  // tile-launch sizing (units per launch, min inner chunk)
+    int mt_wide = 256;  // levels of at most this many fronts use 64-row tiles, the others 32-row tiles
     int mt_ks_n = 2, mt_ks_t = 2;         // k-steps per stage of the N- and T-form tile kernels
     // fronts on the lowest pinv_levels levels keep the partitioned inverse
     // (FrontDesc::pinv): one tile product per front and direction
@@ -314,7 +315,7 @@ struct tf_ctx {
     // through the matrix-core tile kernels with one right-hand-side block, 0
     // through the matrix-vector kernels of solve.hip; launch sizing of its tiles
     int s_tiles = 2;
-    int s_units = 8192, s_chmin = 512;
+    int s_units = 2048, s_chmin = 512;
     int spmv_xcd = 1;  // block SpMV: contiguous row ranges per XCD
     // The time loops (tf_run, tf_run_models, tf_adjoint) keep their vectors
     // in the factor's column order ("native" order: row i is edge perm[i]),
