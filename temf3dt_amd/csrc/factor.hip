@@ -693,16 +693,110 @@ __global__ void __launch_bounds__(64) k_factor_diag(double *L, const FrontDesc *
     wave_diag_factor(L, d, p0, Dn, flag, W + woff[sn]);
 }
 
-__global__ void __launch_bounds__(TF_NT) k_factor_trsm(double *L, const FrontDesc *fd, const int *list,
-                                                       const int64_t *pref, int cnt, int p0, const double *Dn,
-                                                       const double *W, const int64_t *woff) {
-    __shared__ double lds[TF_PB * LDS_LD + TF_PB * MLD];
+// Panel solve of a large front by one wave per 64 rows [rb, rb + 64) below
+// the panel: L21 = A21 X^T D^-1 with X = L11^-1 of the pivot block (unit
+// lower; the pivot kernel leaves its strict part column-major, ld 64, at Xg):
+//   L21[row][c] = (A21[row][c] + sum_{k < c} A21[row][k] X[c][k]) / d_c.
+// Both operands stream from memory into the MFMA lane layout with 16-byte
+// loads, two rows (or two columns of X) per lane, as in the trailing update:
+// result register r of block (a, b) holds column 32(a/2) + 2(4r + l/16) + a%2
+// of the panel, row rb + 32(b/2) + 2(l%16) + b%2.  Rows past the front read
+// whatever follows (buffer range check at the block's end) and are never
+// stored; columns past the panel width pw multiply zeros of Xg.
+template <int KS>
+__device__ __forceinline__ void trsm_wave(double *L, const FrontDesc &d, int p0, int rb, const double *Dn,
+                                          const double *Xg) {
+    constexpr int SW = 4 * KS;
+    const int pw = min(TF_PB, d.nc - p0);
+    double *base = L + d.loff;
+    const __amdgpu_buffer_rsrc_t rL = mt_rsrc(base, (uint32_t)((int64_t)d.nc * d.ld * 8));
+    const __amdgpu_buffer_rsrc_t rX = mt_rsrc(Xg, (uint32_t)(TF_PB * TF_PB * 8));
+    const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
+    const uint32_t lx = (uint32_t)(lk * TF_PB + 2 * l16) * 8u;
+    tf_d4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = (tf_d4){0.0, 0.0, 0.0, 0.0};
+    auto mult = [&](const double xr[KS][4], const double yc[KS][4]) {
+#pragma unroll
+        for (int s = 0; s < KS; s++)
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(yc[s][a], xr[s][b], acc[a][b], 0, 0, 0);
+    };
+    // inner index k of stage value s: ib + 4 s + l/16.  X is zero for k >= pw,
+    // so the panel column is clamped to pw - 1 there (every read stays in
+    // the front's block; the whole offset is per lane)
+    auto load = [&](int ib, double xr[KS][4], double yc[KS][4]) {
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const int k = min(ib + 4 * s + lk, pw - 1);
+            const uint32_t lr = (uint32_t)((p0 + k) * d.ld + rb + 2 * l16) * 8u;
+#pragma unroll
+            for (int p = 0; p < 2; p++) mt_bld2(rL, lr + 256u * p, 0u, xr[s][2 * p], xr[s][2 * p + 1]);
+#pragma unroll
+            for (int p = 0; p < 2; p++)
+                mt_bld2(rX, lx + 256u * p, (uint32_t)((ib + 4 * s) * TF_PB) * 8u, yc[s][2 * p], yc[s][2 * p + 1]);
+        }
+    };
+    const int nst = (pw + SW - 1) / SW;  // X rows k >= pw are zero, so whole stages suffice
+    double xr0[KS][4], yc0[KS][4], xr1[KS][4], yc1[KS][4];
+    int js = 0;
+    if (nst & 1) {
+        load(0, xr0, yc0);
+        mult(xr0, yc0);
+        js = 1;
+    }
+    if (js < nst) {
+        load(js * SW, xr0, yc0);
+        for (; js < nst; js += 2) {
+            load((js + 1) * SW, xr1, yc1);
+            __builtin_amdgcn_sched_barrier(0);
+            mult(xr0, yc0);
+            __builtin_amdgcn_sched_barrier(0);
+            load(min(js + 2, nst - 1) * SW, xr0, yc0);
+            __builtin_amdgcn_sched_barrier(0);
+            mult(xr1, yc1);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    // L21 = (A21 + acc) / d on rows < f, columns < pw (row pairs as one 16-byte access)
+    typedef double t_d2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int c = 32 * (a >> 1) + 2 * (4 * r + lk) + (a & 1);
+            if (c >= pw) continue;
+            const double dc = Dn[d.doff + p0 + c];
+            double *col = base + (int64_t)(p0 + c) * d.ld;
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                const int row = rb + 32 * p + 2 * l16;
+                const t_d2 s = (t_d2){acc[a][2 * p][r], acc[a][2 * p + 1][r]};
+                if (row + 1 < d.f) {
+                    *(t_d2 *)(col + row) = (*(const t_d2 *)(col + row) + s) / dc;
+                } else if (row < d.f) {
+                    col[row] = (col[row] + s.x) / dc;
+                }
+            }
+        }
+}
+
+// one workgroup of two waves per 128 rows below the panel
+__global__ void __launch_bounds__(128) k_factor_trsm(double *L, const FrontDesc *fd, const int *list,
+                                                     const int64_t *pref, int cnt, int p0, const double *Dn,
+                                                     const double *W, const int64_t *woff) {
     const int i = seg_find(pref, cnt, blockIdx.x);
     const int sn = list[i];
     const FrontDesc d = fd[sn];
     const int p1 = min(d.nc, p0 + TF_PB);
-    const int r0 = p1 + (int)(blockIdx.x - pref[i]) * TF_TRSM_ROWS;
-    blk_trsm(L, d, p0, r0, Dn, lds, W + woff[sn]);
+    const int rb = p1 + (int)(blockIdx.x - pref[i]) * TF_TRSM_ROWS + 64 * __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (rb >= d.f) return;
+    trsm_wave<2>(L, d, p0, rb, Dn, W + woff[sn]);
 }
 
 // Updates of a super-panel [P0, P0 + TF_SP) of TF_PB-column panels, one
@@ -846,7 +940,7 @@ void launch_factor_trsm(double *L, const FrontDesc *fd, const int *list, const i
                         int64_t nwg, int p0, const double *Dn, const double *W, const int64_t *woff,
                         hipStream_t st) {
     if (nwg <= 0) return;
-    hipLaunchKernelGGL(k_factor_trsm, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, fd, list, pref, cnt, p0, Dn, W,
+    hipLaunchKernelGGL(k_factor_trsm, dim3((unsigned)nwg), dim3(128), 0, st, L, fd, list, pref, cnt, p0, Dn, W,
                        woff);
 }
 void launch_factor_update(double *L, double *U, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
