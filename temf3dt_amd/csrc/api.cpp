@@ -110,6 +110,10 @@ void launch_stile_red(int mode, const FrontDesc *fd, const TileUnit *units, int 
 void launch_mtile_red(int mode, int nb, int na, const FrontDesc *fd, const TileUnit *units, int nunits, int K,
                       const double *in0, const double *in1, const double *Dn, double *out, const double *part,
                       const int *perm, double *x1, double *x2, hipStream_t st, double *z = nullptr);
+void launch_mspmv(int n, int K, const int *ia, const int *ja, const double *val, const double *e, double *u, int xcd,
+                  hipStream_t st);
+void launch_mspmv_chain(int n, int K, const int *ia, const int *ja, const double *val, const double *e2,
+                        const double *u3, double *u2, double *r, int xcd, hipStream_t st);
 void launch_mspmv_rhs(int n, int K, const int *ia, const int *ja, const double *val, const double *e2,
                       const double *e3, double *w, double *r, int xcd, hipStream_t st);
 void launch_msource(int ns, int K, const int *edge, const int *src, const double *w, double didt, double cdt,
@@ -317,6 +321,9 @@ struct tf_ctx {
     int s_tiles = 2;
     int s_units = 2048, s_chmin = 512;
     int spmv_xcd = 1;  // block SpMV: contiguous row ranges per XCD
+    // block time loop: regular steps form M (4 e2 - e3) as 4 M e2 - M e3 from
+    // the product M e2 kept from the step before (tuning key "rhs_chain")
+    bool rhs_chain = true;
     // The time loops (tf_run, tf_run_models, tf_adjoint) keep their vectors
     // in the factor's column order ("native" order: row i is edge perm[i]),
     // so the solves read b and write x in place instead of through perm, and
@@ -1428,12 +1435,13 @@ struct GraphCache {
 
 struct StepVecs {
     double *e1, *e2, *e3, *rh, *te1, *te2, *te3, *erec, *ev1, *ev2;
+    double *u[3];  // M e of the history buffers vecs[0..2], in that order (see run_loop)
     std::vector<double *> ring;
 };
 
 void ensure_vecs(tf_ctx *c, StepVecs &v) {
     const int64_t len = (int64_t)c->topo.nE * c->nsrc;
-    int need = 10 + c->sp_dblesize + 1;
+    int need = 13 + c->sp_dblesize + 1;
     if ((int)c->vecs.size() != need || c->vec_len != len) {
         for (auto &p : c->vecs) dfree(p);
         c->vecs.clear();
@@ -1443,9 +1451,21 @@ void ensure_vecs(tf_ctx *c, StepVecs &v) {
     v.e1 = c->vecs[0]; v.e2 = c->vecs[1]; v.e3 = c->vecs[2]; v.rh = c->vecs[3];
     v.te1 = c->vecs[4]; v.te2 = c->vecs[5]; v.te3 = c->vecs[6]; v.erec = c->vecs[7];
     v.ev1 = c->vecs[8]; v.ev2 = c->vecs[9];
-    v.ring.assign(c->vecs.begin() + 10, c->vecs.end());
+    for (int i = 0; i < 3; i++) v.u[i] = c->vecs[10 + i];
+    v.ring.assign(c->vecs.begin() + 13, c->vecs.end());
     if (!c->d_norm_part) c->d_norm_part = dalloc<double>(256 * 64 * 2);
     if (!c->d_norm_out) c->d_norm_out = dalloc<double>(2 * 64);
+}
+
+// r -= 2 dt s(t) while the source is on (sys_right: step < sp_division)
+void rhs_source(tf_ctx *c, int step, double t, double dt, bool negate, double *r) {
+    if (step >= c->sp_division) return;
+    const int K = c->nsrc;
+    const tf_ctx::RunMaps &m = c->rm;
+    double didt = source_didt(c->stype, c->ptype, t, c->pulse_width, c->current);
+    if (negate) didt = -didt;
+    if (K == 1) tf::launch_source(c->nseg, m.src_edge, c->d_src_w, didt, 2.0 * dt, r, c->st);
+    else tf::launch_msource(c->nseg, K, m.src_edge, c->d_src_id, c->d_src_w, didt, 2.0 * dt, r, c->st);
 }
 
 // r = Msigma (4 e2 - e3) - 2 dt s(t), K sources side by side
@@ -1455,12 +1475,7 @@ void rhs(tf_ctx *c, int step, double t, const double *e2, const double *e3, doub
     const tf_ctx::RunMaps &m = c->rm;
     if (K == 1) tf::launch_spmv_rhs(c->topo.nE, m.fia, m.fja, m.fval, e2, e3, r, c->st);
     else tf::launch_mspmv_rhs(c->topo.nE, K, m.fia, m.fja, m.fval, e2, e3, c->d_mb, r, c->spmv_xcd, c->st);
-    if (step < c->sp_division) {  // sys_right: source on while step < sp_division
-        double didt = source_didt(c->stype, c->ptype, t, c->pulse_width, c->current);
-        if (negate) didt = -didt;
-        if (K == 1) tf::launch_source(c->nseg, m.src_edge, c->d_src_w, didt, 2.0 * dt, r, c->st);
-        else tf::launch_msource(c->nseg, K, m.src_edge, c->d_src_id, c->d_src_w, didt, 2.0 * dt, r, c->st);
-    }
+    rhs_source(c, step, t, dt, negate, r);
     prof_close(c, TF_PROF_STEP, pm);
 }
 
@@ -1607,6 +1622,44 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
     const int ring = dbl + 1;
     int cur = 0;
     for (double *p : {v.e1, v.e2, v.e3}) HIPCHK(hipMemsetAsync(p, 0, sizeof(double) * n * c->nsrc, c->st));
+    // Regular steps of a block run take M (4 e2 - e3) = 4 M e2 - M e3 with M e3
+    // kept from the step before (the history buffers rotate, so M e of each of
+    // the three buffers lives in the product buffer paired with it; anything
+    // written into a history buffer other than by rotation marks its product
+    // stale, and a stale M e3 is formed again before the step).  Taped runs
+    // keep the direct form, which the adjoint's recomputation repeats.
+    const bool tape_run = c->ckpt_every > 0 && inout == 0;
+    const bool chain = c->rhs_chain && c->nsrc > 1 && c->nsrc % 2 == 0 && !tape_run;
+    double *const ebuf[3] = {v.e1, v.e2, v.e3};
+    bool ufresh[3] = {false, false, false};
+    auto uslot = [&](const double *p) {
+        for (int i = 0; i < 3; i++)
+            if (ebuf[i] == p) return i;
+        throw std::logic_error("history buffer not in the product table");
+    };
+    auto touch = [&](const double *p) { ufresh[uslot(p)] = false; };
+    // regular step's right-hand side into r (the chain's M e3 formed already)
+    auto rhs_regular = [&](double *r) {
+        if (!chain) {
+            rhs(c, step, time_cur, v.e2, v.e3, step_cur, negate, r);
+            return;
+        }
+        const int pm = prof_mark(c);
+        const int i2 = uslot(v.e2), i3 = uslot(v.e3);
+        tf::launch_mspmv_chain(n, c->nsrc, c->rm.fia, c->rm.fja, c->rm.fval, v.e2, v.u[i3], v.u[i2], r,
+                               c->spmv_xcd, c->st);
+        rhs_source(c, step, time_cur, step_cur, negate, r);
+        prof_close(c, TF_PROF_STEP, pm);
+    };
+    auto chain_ready = [&] {  // M e3 for the next regular step, formed eagerly when stale
+        if (!chain) return;
+        const int i3 = uslot(v.e3);
+        if (ufresh[i3]) return;
+        const int pm = prof_mark(c);
+        tf::launch_mspmv(n, c->nsrc, c->rm.fia, c->rm.fja, c->rm.fval, v.e3, v.u[i3], c->spmv_xcd, c->st);
+        prof_close(c, TF_PROF_STEP, pm);
+        ufresh[i3] = true;
+    };
     if (outp) prepare_slices(c);
     // factors from before this run may be of other materials: none is reused
     c->slot_live[0] = c->slot_live[1] = false;
@@ -1690,7 +1743,7 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
             c->ckpts.push_back(k);
         }
         if (step == 1) {
-            if (inout == 2) { dcopy(c, v.e2, res); dcopy(c, v.e3, res); }
+            if (inout == 2) { dcopy(c, v.e2, res); dcopy(c, v.e3, res); touch(v.e2); touch(v.e3); }
             if (outp) output_row(c, 0, time_cur - step_cur, step_cur, v.e2);
             timed_factor(cur, step_cur);
             start_factor(1 - cur, step_cur * dbl);
@@ -1747,10 +1800,12 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
             last_changed = !keep;
             if (keep) {
                 dcopy(c, v.e1, v.erec);
+                touch(v.e1);
                 if (outp) output_row(c, step, time_cur, step_cur, v.e1);
                 if (tape) { row_of(f1id); id3 = id2; id2 = f1id; }
                 std::swap(v.e3, v.e2);  // e3 <- e2
                 dcopy(c, v.e2, v.e1);   // e2 <- e1
+                touch(v.e2);
                 not_changed++;
                 idx = 0;
             } else {
@@ -1758,6 +1813,8 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
                 cur = other;
                 dcopy(c, v.e3, ea);
                 dcopy(c, v.e2, v.ev1);
+                touch(v.e3);
+                touch(v.e2);
                 time_cur = ttime;
                 step_cur = tstep;
                 if (outp) output_row(c, step, time_cur, step_cur, v.e2);
@@ -1772,12 +1829,14 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
             // regular step, sources off: the right-hand side and the solve
             // (with its ring copy) replay a graph captured for this factor slot and
             // arrangement of the history vectors (at most 2 x 3 x ring of them)
+            // (the product buffers follow from the history buffers, so the key covers them)
             const std::array<const void *, 6> key = {c->d_L[cur], v.e1, v.e2, v.e3, v.rh, v.ring[idx % ring]};
+            chain_ready();
             auto it = graphs.m.find(key);
             if (it == graphs.m.end()) {
                 hipGraph_t g;
                 HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-                rhs(c, step, time_cur, v.e2, v.e3, step_cur, negate, v.rh);
+                rhs_regular(v.rh);
                 step_solve(c, cur, v.rh, v.e1, v.ring[idx % ring]);
                 HIPCHK(hipStreamEndCapture(c->st, &g));
                 hipGraphExec_t ex;
@@ -1787,9 +1846,12 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
             }
             HIPCHK(hipGraphLaunch(it->second, c->st));
         } else {
-            rhs(c, step, time_cur, v.e2, v.e3, step_cur, negate, v.rh);
+            chain_ready();
+            rhs_regular(v.rh);
             step_solve(c, cur, v.rh, v.e1, v.ring[idx % ring]);
         }
+        if (chain) ufresh[uslot(v.e2)] = true;  // M e2 formed by this step
+        touch(v.e1);
         if (outp) output_row(c, step, time_cur, step_cur, v.e1);
         if (tape) {
             const int k = node(id2, id3, step, step_cur, time_cur);
@@ -2675,6 +2737,9 @@ int tf_set_tuning(tf_ctx *c, const char *key, int value) {
             if (value < 0) return fail(c, TF_ERR_ARG, "mt_wide must be >= 0");
             c->mt_wide = value;
             replan = true;
+        } else if (k == "rhs_chain") {
+            if (value != 0 && value != 1) return fail(c, TF_ERR_ARG, "rhs_chain must be 0 or 1");
+            c->rhs_chain = value != 0;
         } else if (k == "graphs") {
             if (value != 0 && value != 1) return fail(c, TF_ERR_ARG, "graphs must be 0 or 1");
             c->use_graphs = value != 0;

@@ -111,6 +111,57 @@ __global__ void __launch_bounds__(256) k_mspmv_rhs2(int n, int K, const int *ia,
     }
 }
 
+// The same product of the newest history state alone, with the BDF2
+// combination taken from the product kept for the state before it (the SpMV
+// is linear: M (4 e2 - e3) = 4 M e2 - M e3): u2 = M e2 and r = 4 u2 - u3, as
+// 16-byte pairs (K even).  u3 was written by this kernel one step earlier
+// (or by k_mspmv_rhs2 from e3 after the history was reset).
+__global__ void __launch_bounds__(256) k_mspmv_chain(int n, int K, const int *ia, const int *ja, const double *val,
+                                                     const double *e2, const double *u3, double *u2, double *r,
+                                                     int xcd) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    constexpr int D = 4;
+    const int lane = threadIdx.x & 63, h = lane >> 5, j = lane & 31;
+    const int q = min(2 * j, K - 2);
+    const int blk = xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int row0 = blk * 4 * TF_RPW + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int rr = 0; rr < TF_RPW; rr += 2) {
+        if (row0 + 4 * rr >= n) return;
+        const int row = row0 + 4 * (rr + h);
+        const bool live = row < n;
+        const int p0 = live ? ia[row] : 0, cnt = live ? ia[row + 1] - p0 : 0;
+        const int cm = max(cnt, __shfl_xor(cnt, 32, 64));
+        // the older product's row, fetched ahead of the gathers
+        const d2 old = live ? *(const d2 *)(u3 + (int64_t)row * K + q) : (d2){0.0, 0.0};
+        d2 acc = (d2){0.0, 0.0};
+        for (int pb = 0; pb < cm; pb += 32) {
+            const int myj = pb + j < cnt ? ja[p0 + pb + j] : 0;
+            const double myv = pb + j < cnt ? val[p0 + pb + j] : 0.0;
+            const int m = min(32, cm - pb);
+            for (int t = 0; t < m; t += D) {
+                d2 x[D];
+                double v[D];
+#pragma unroll
+                for (int u = 0; u < D; u++) {
+                    const int src = (lane & 32) + min(t + u, m - 1);
+                    const int c = __shfl(myj, src, 64);
+                    v[u] = __shfl(myv, src, 64);
+                    x[u] = *(const d2 *)(e2 + (int64_t)c * K + q);
+                }
+#pragma unroll
+                for (int u = 0; u < D; u++) {
+                    const d2 a = acc + v[u] * x[u];
+                    acc = t + u < m && pb + t + u < cnt ? a : acc;
+                }
+            }
+        }
+        if (live && 2 * j < K) {
+            *(d2 *)(u2 + (int64_t)row * K + 2 * j) = acc;
+            *(d2 *)(r + (int64_t)row * K + 2 * j) = 4.0 * acc - old;
+        }
+    }
+}
+
 // r[edge][q] -= (2 dt) (didt w) for every source-wire entry (edge, q, w)
 __global__ void k_msource(int ns, int K, const int *edge, const int *src, const double *w, double didt, double cdt,
                           double *r) {
@@ -379,6 +430,18 @@ void launch_mspmv_rhs(int n, int K, const int *ia, const int *ja, const double *
         hipLaunchKernelGGL(k_mspmv_rhs2, g, dim3(256), 0, st, n, K, ia, ja, val, w, r, xcd);
     else
         hipLaunchKernelGGL(k_mspmv_rhs, g, dim3(256), 0, st, n, K, ia, ja, val, w, r, xcd);
+}
+// u = M e (no history combination), K even
+void launch_mspmv(int n, int K, const int *ia, const int *ja, const double *val, const double *e, double *u, int xcd,
+                  hipStream_t st) {
+    hipLaunchKernelGGL(k_mspmv_rhs2, dim3((unsigned)((n + 4 * TF_RPW - 1) / (4 * TF_RPW))), dim3(256), 0, st, n, K,
+                       ia, ja, val, e, u, xcd);
+}
+// u2 = M e2 and r = 4 u2 - u3, K even
+void launch_mspmv_chain(int n, int K, const int *ia, const int *ja, const double *val, const double *e2,
+                        const double *u3, double *u2, double *r, int xcd, hipStream_t st) {
+    hipLaunchKernelGGL(k_mspmv_chain, dim3((unsigned)((n + 4 * TF_RPW - 1) / (4 * TF_RPW))), dim3(256), 0, st, n, K,
+                       ia, ja, val, e2, u3, u2, r, xcd);
 }
 void launch_msource(int ns, int K, const int *edge, const int *src, const double *w, double didt, double cdt,
                     double *r, hipStream_t st) {

@@ -182,6 +182,37 @@ def test_overlapped_factor_matches_serial(loops):
         np.testing.assert_array_equal(a[k], b[k])
 
 
+def test_chained_rhs_matches_direct():
+    """Regular steps of a block run form M (4 e2 - e3) as 4 M e2 - M e3 from
+    the product kept from the step before (rhs_chain): the run through three
+    doubling checks (after each of which the kept products go stale and are
+    formed again) agrees with the direct product to rounding,
+    with the same steps, times and decisions."""
+    from temf3dt_amd import workloads
+    m = workloads.multi_source_small_model()
+    out = {}
+    for ch in (1, 0):
+        print(f"[gpu] chained right-hand side: rhs_chain={ch}", flush=True)
+        fe = gpu_from_model(m)
+        fe.set_tuning("rhs_chain", ch)
+        out[ch] = (fe.run(), fe.checks())
+        fe.close()
+    (a, ca), (b, cb) = out[1], out[0]
+    print(f"[gpu] check decisions {list(ca['keep'])}", flush=True)
+    assert len(ca["keep"]) >= 3
+    np.testing.assert_array_equal(ca["keep"], cb["keep"])
+    for k in ("step", "t", "dt"):
+        np.testing.assert_array_equal(a[k], b[k])
+    fa, fb = a["fields"], b["fields"]
+    scale = np.maximum(np.abs(fb).max(axis=(0, 1), keepdims=True), 1e-300)
+    err = (np.abs(fa - fb) / scale).max(axis=(0, 1))  # per component (E, dB/dt)
+    print(f"[gpu] chained vs direct: max difference per component {err} of its largest value", flush=True)
+    # dB/dt tightly; E at receivers in the air carries its gradient part,
+    # fixed only by the air conductivity, which amplifies rounding
+    assert err[3:].max() < 1e-8, err
+    assert err.max() < 1e-6, err
+
+
 @pytest.mark.parametrize("knobs", [
     {"mt_units": 64, "mt_chmin": 16},                        # every long tile split into partials
     {"mt_units": 64, "mt_chmin": 16, "mt_ks_n": 4, "mt_ks_t": 2},
@@ -221,7 +252,7 @@ def test_tuning_rejects_bad_values(gpu_mod_big_batch):
     from temf3dt_amd.fetd import FetdError
     fe, _ = gpu_mod_big_batch
     for k, v in [("mt_ks_n", 3), ("mt_chmin", 10), ("mt_units", 1), ("graphs", 2), ("overlap", -1), ("mt_wide", -1),
-                 ("mt_fuse", 3), ("pinv", -1), ("s_tiles", 0), ("no_such_knob", 1)]:
+                 ("mt_fuse", 3), ("rhs_chain", 2), ("pinv", -1), ("s_tiles", 0), ("no_such_knob", 1)]:
         with pytest.raises(FetdError):
             fe.set_tuning(k, v)
 
