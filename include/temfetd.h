@@ -116,6 +116,10 @@ int tf_analyze(tf_ctx *ctx, int leaf_size);
  *               second stream while the steps before the check run, and
  *               tf_adjoint its sweep's next dt while the current one is in
  *               use; 0: at the point of use.  Bit-identical.
+ *   "rhs_chain" 1 (default): regular steps of a multi-source tf_run (even
+ *               source count, no checkpointing) form Msigma (4 e2 - e3) as
+ *               4 Msigma e2 - Msigma e3, the second product kept from the step
+ *               before; 0: the history combination first.  Equal to rounding.
  *   "indefinite" 0 (default): a factorisation with a negative pivot fails
  *               with TF_ERR_NUMERIC (P is SPD in exact arithmetic; the
  *               reference's SPD factorisation would fail there and be redone
