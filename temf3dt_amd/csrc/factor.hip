@@ -146,7 +146,6 @@ __device__ void blk_update_tile(double *L, double *U, const FrontDesc &d, int p0
     __syncthreads();
 }
 
-#define TF_ULD (TF_UT + 16)  // LDS row stride of the inversion's staged operands
 
 // Invert the unit-lower pw x pw diagonal block of panel p0 into LDS
 // (column-major, ld 65, strict lower part; the diagonal is implicit).
@@ -198,83 +197,9 @@ __device__ void blk_invert_diag(double *L, const FrontDesc &d, int p0, double *l
 //   inv([L_II 0; L_JI L_JJ]) = [X_II 0; -X_JJ L_JI X_II  X_JJ]:
 //   step 0: T    = L_JI X_II    (scratch, |J| x b, column-major, ld ldT)
 //   step 1: X_JI = -X_JJ T      (in place over L_JI)
-// Each step is a grid of 128 x 128 matrix-core tiles, so the top merges of
-// a front with thousands of columns fill the whole chip in a handful of
-// launches.
-
-// 128 x 128 tile (rows rb.., columns cb..) of op(A) op(B) over k in [kb, ke)
-// into acc (same fragment layout as the trailing update).  A(i, k) is read
-// at A[k*lda + i] and B(k, j) at B[j*ldb + k]; UA/UB make that operand unit
-// lower triangular (1 on the diagonal, 0 above it, stored values below).
-// B is staged with its k index fastest (coalesced) and an XOR swizzle on the
-// column index, so that neither the staging writes nor the fragment reads
-// pile onto a few LDS banks.
-template <bool UA, bool UB>
-__device__ void inv_tile(const double *A, int lda, const double *B, int ldb, int M, int N, int kb, int ke, int rb,
-                         int cb, double *lds, tf_d4 (&acc)[4][4]) {
-    const int KH = 32;
-    double *Ap = lds;                // [k][TF_ULD]: A(rb + i, k)
-    double *Bp = lds + KH * TF_ULD;  // [k][TF_ULD]: B(k, cb + (j ^ (k & 15)))
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int wr = w >> 1, wc = w & 1;
-    const int l16 = lane & 15, lk = lane >> 4;
-#pragma unroll
-    for (int bi = 0; bi < 4; bi++)
-#pragma unroll
-        for (int bj = 0; bj < 4; bj++) acc[bi][bj] = (tf_d4){0.0, 0.0, 0.0, 0.0};
-    for (int k0 = kb; k0 < ke; k0 += KH) {
-        constexpr int PER = 32 * TF_UT / TF_NT;  // 16
-#pragma unroll
-        for (int half = 0; half < PER; half += 8) {
-            double va[8], vb[8];
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const int idx = (half + q) * TF_NT + threadIdx.x;
-                {   // A: row index fastest
-                    const int k = k0 + idx / TF_UT, i = rb + idx % TF_UT;
-                    double v = 0.0;
-                    if (k < ke && i < M) {
-                        if (!UA) v = A[(int64_t)k * lda + i];
-                        else v = (i > k) ? A[(int64_t)k * lda + i] : (i == k ? 1.0 : 0.0);
-                    }
-                    va[q] = v;
-                }
-                {   // B: k index fastest
-                    const int k = k0 + (idx & (KH - 1)), j = cb + idx / KH;
-                    double v = 0.0;
-                    if (k < ke && j < N) {
-                        if (!UB) v = B[(int64_t)j * ldb + k];
-                        else v = (k > j) ? B[(int64_t)j * ldb + k] : (k == j ? 1.0 : 0.0);
-                    }
-                    vb[q] = v;
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const int idx = (half + q) * TF_NT + threadIdx.x;
-                Ap[(idx / TF_UT) * TF_ULD + idx % TF_UT] = va[q];
-                const int kk = idx & (KH - 1), jj = idx / KH;
-                Bp[kk * TF_ULD + (jj ^ (kk & 15))] = vb[q];
-            }
-        }
-        __syncthreads();
-        const int kn = min(KH, ((ke - k0) + 3) & ~3);
-        for (int kq = 0; kq < kn; kq += 4) {
-            const int kk = kq + lk;
-            double x[4], y[4];
-#pragma unroll
-            for (int bj = 0; bj < 4; bj++) x[bj] = Bp[kk * TF_ULD + ((64 * wc + 16 * bj + l16) ^ (kk & 15))];
-#pragma unroll
-            for (int bi = 0; bi < 4; bi++) y[bi] = Ap[kk * TF_ULD + 64 * wr + 16 * bi + l16];
-#pragma unroll
-            for (int bi = 0; bi < 4; bi++)
-#pragma unroll
-                for (int bj = 0; bj < 4; bj++)
-                    acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[bj], y[bi], acc[bi][bj], 0, 0, 0);
-        }
-        __syncthreads();
-    }
-}
+// Each step is a grid of 128 x 128 output regions, a 64 x 64 matrix-core
+// tile per wave, so the top merges of a front with thousands of columns fill
+// the whole chip in a handful of launches.
 
 // Wave-level trailing update on matrix cores: one wave updates the 64 x 64
 // tile C(rb.., cb..) -= L_rows D L_cols^T over the panel columns
@@ -841,47 +766,144 @@ __global__ void __launch_bounds__(TF_NT) k_inv_diag(double *L, const FrontDesc *
     blk_invert_diag(L, d, (int)(blockIdx.x - pref[i]) * TF_PB, lds);
 }
 
-// One merge step at block size b (see inv_tile); work unit u of a front is
-// (pair, ti, tj) with ntb x ntb tiles of 128 per pair, ntb = ceil(b / 128).
+// One wave's 64 x 64 tile of a merge step, straight from memory into the
+// MFMA lane layout (no LDS, no barriers):
+//   acc(c, r) = sum_{k0 <= k < k1} Cop(k, c) Rop(r, k),
+//   Cop(k, c) = C[c * ldc + k]  (a lane reads two consecutive k of column c),
+//   Rop(r, k) = R[k * ldr + r]  (a lane reads two consecutive rows r of column k),
+// with UC / UR making that operand unit lower triangular (stored values below
+// the diagonal, 1 on it, 0 above).  The inner index of k-step s of a stage is
+// ib + 2 l/16 + s for both operands.  Result register q of block (a, b) holds
+// column cb + 16a + 4q + l/16 and row rb + 32(b/2) + 2(l%16) + b%2, so the
+// blocks b = 2p, 2p + 1 of one register are two consecutive rows.  Reads past
+// k1 are clamped to k1 - 1 and zeroed; rows past the operand's valid range
+// read neighbouring storage (the results are not stored).
+template <bool UC, bool UR>
+__device__ __forceinline__ void inv_wave(const double *R, int ldr, uint32_t rbytes, const double *C, int ldc,
+                                         uint32_t cbytes, int rb, int cb, int k0, int k1, tf_d4 (&acc)[4][4]) {
+    constexpr int SW = 8;  // inner values per stage (two k-steps)
+    const __amdgpu_buffer_rsrc_t rR = mt_rsrc(R, rbytes), rC = mt_rsrc(C, cbytes);
+    const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = (tf_d4){0.0, 0.0, 0.0, 0.0};
+    if (k0 >= k1) return;
+    auto load = [&](int ib, double cv[2][4], double rv[2][4]) {
+        const int kp = ib + 2 * lk;  // the lane's two inner values kp, kp + 1
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const int c = cb + 16 * a + l16;
+            double x0, x1;
+            mt_bld2(rC, (uint32_t)((int64_t)c * ldc + kp) * 8u, 0u, x0, x1);  // zeroed past k1 below
+            const double v[2] = {x0, x1};
+#pragma unroll
+            for (int s2 = 0; s2 < 2; s2++) {
+                const int k = kp + s2;
+                double y = k < k1 ? v[s2] : 0.0;
+                if (UC) y = k > c ? y : (k == c ? 1.0 : 0.0);
+                cv[s2][a] = y;
+            }
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < 2; s2++) {
+            const int k = min(kp + s2, k1 - 1);
+            const bool kv = kp + s2 < k1;
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                const int r0 = rb + 32 * p + 2 * l16;
+                double y0, y1;
+                mt_bld2(rR, (uint32_t)((int64_t)k * ldr + r0) * 8u, 0u, y0, y1);
+                if (UR) {
+                    y0 = r0 > k ? y0 : (r0 == k ? 1.0 : 0.0);
+                    y1 = r0 + 1 > k ? y1 : (r0 + 1 == k ? 1.0 : 0.0);
+                }
+                rv[s2][2 * p] = kv ? y0 : 0.0;
+                rv[s2][2 * p + 1] = kv ? y1 : 0.0;
+            }
+        }
+    };
+    auto mult = [&](const double cv[2][4], const double rv[2][4]) {
+#pragma unroll
+        for (int s2 = 0; s2 < 2; s2++)
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(cv[s2][a], rv[s2][b], acc[a][b], 0, 0, 0);
+    };
+    const int nst = (k1 - k0 + SW - 1) / SW;
+    double c0v[2][4], r0v[2][4], c1v[2][4], r1v[2][4];
+    int js = 0;
+    if (nst & 1) {
+        load(k0, c0v, r0v);
+        mult(c0v, r0v);
+        js = 1;
+    }
+    if (js < nst) {
+        load(k0 + js * SW, c0v, r0v);
+        for (; js < nst; js += 2) {
+            load(k0 + (js + 1) * SW, c1v, r1v);
+            __builtin_amdgcn_sched_barrier(0);
+            mult(c0v, r0v);
+            __builtin_amdgcn_sched_barrier(0);
+            load(k0 + min(js + 2, nst - 1) * SW, c0v, r0v);
+            __builtin_amdgcn_sched_barrier(0);
+            mult(c1v, r1v);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
+// One merge step at block size b; work unit u of a front is (pair, ti, tj)
+// with ntb x ntb output regions of 128 x 128 per pair, ntb = ceil(b / 128),
+// one 64 x 64 quadrant per wave:
+//   step 0: T    = L_JI X_II   (k >= c: X_II(k, c) = 0 above its diagonal)
+//   step 1: X_JI = -X_JJ T     (k <= r: X_JJ(r, k) = 0 above its diagonal)
 template <int STEP>
-__global__ void __launch_bounds__(TF_NT) k_inv_merge(double *L, const FrontDesc *fd, const int *list,
-                                                     const int64_t *pref, int cnt, int b, double *W,
-                                                     const int64_t *woff) {
-    __shared__ double lds[2 * 32 * TF_ULD];
+__global__ void __launch_bounds__(TF_NT, 2) k_inv_merge(double *L, const FrontDesc *fd, const int *list,
+                                                        const int64_t *pref, int cnt, int b, double *W,
+                                                        const int64_t *woff) {
     const int i = seg_find(pref, cnt, blockIdx.x);
     const int sn = list[i];
     const FrontDesc d = fd[sn];
     const int64_t u = (int64_t)blockIdx.x - pref[i];
     const int ntb = (b + TF_UT - 1) / TF_UT;
     const int pair = (int)(u / (ntb * ntb)), t = (int)(u % (ntb * ntb));
-    const int ti = t / ntb, tj = t % ntb;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int I0 = 2 * b * pair, J0 = I0 + b, Jn = min(b, d.nc - J0);
-    const int rb = ti * TF_UT, cb = tj * TF_UT;
-    if (rb >= Jn) return;  // whole workgroup: the last pair is short
+    const int rb = (t / ntb) * TF_UT + 64 * (w >> 1), cb = (t % ntb) * TF_UT + 64 * (w & 1);
+    if (rb >= Jn || cb >= b) return;  // this wave's quadrant lies past the pair
     const int ldT = inv_ldt(d.nc, b);
     double *T = W + woff[sn] + (int64_t)pair * ldT * b;
     double *base = L + d.loff;
+    const int64_t lend = (int64_t)d.nc * d.ld;  // the front's L block
     tf_d4 acc[4][4];
-    if (STEP == 0)  // T = L_JI X_II; X_II(k, c) = 0 for k < c
-        inv_tile<false, true>(base + (int64_t)I0 * d.ld + J0, d.ld, base + (int64_t)I0 * d.ld + I0, d.ld, Jn, b,
-                              cb, b, rb, cb, lds, acc);
-    else  // X_JI = -X_JJ T; X_JJ(i, k) = 0 for k > i
-        inv_tile<true, false>(base + (int64_t)J0 * d.ld + J0, d.ld, T, ldT, Jn, b, 0, min(rb + TF_UT, Jn), rb,
-                              cb, lds, acc);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int wr = w >> 1, wc = w & 1;
-    const int l16 = lane & 15, lk = lane >> 4;
+    if (STEP == 0) {
+        const int64_t ro = (int64_t)I0 * d.ld + J0, co = (int64_t)I0 * d.ld + I0;
+        inv_wave<true, false>(base + ro, d.ld, (uint32_t)((lend - ro) * 8), base + co, d.ld,
+                              (uint32_t)((lend - co) * 8), rb, cb, cb, b, acc);
+    } else {
+        const int64_t ro = (int64_t)J0 * d.ld + J0;
+        inv_wave<false, true>(base + ro, d.ld, (uint32_t)((lend - ro) * 8), T, ldT, (uint32_t)((int64_t)ldT * b * 8),
+                              rb, cb, 0, min(rb + 64, Jn), acc);
+    }
+    typedef double i_d2 __attribute__((ext_vector_type(2)));
+    const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
 #pragma unroll
-    for (int bj = 0; bj < 4; bj++)
+    for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int c = cb + 64 * wc + 16 * bj + 4 * r + lk;
+        for (int q = 0; q < 4; q++) {
+            const int c = cb + 16 * a + 4 * q + lk;
             if (c >= b) continue;
             double *col = (STEP == 0) ? T + (int64_t)c * ldT : base + (int64_t)(I0 + c) * d.ld + J0;
 #pragma unroll
-            for (int bi = 0; bi < 4; bi++) {
-                const int row = rb + 64 * wr + 16 * bi + l16;
-                if (row < Jn) col[row] = (STEP == 0) ? acc[bi][bj][r] : -acc[bi][bj][r];
+            for (int p = 0; p < 2; p++) {
+                const int r0 = rb + 32 * p + 2 * l16;
+                const i_d2 v = (STEP == 0) ? (i_d2){acc[a][2 * p][q], acc[a][2 * p + 1][q]}
+                                           : (i_d2){-acc[a][2 * p][q], -acc[a][2 * p + 1][q]};
+                if (r0 + 1 < Jn) *(i_d2 *)(col + r0) = v;
+                else if (r0 < Jn) col[r0] = v.x;
             }
         }
 }
