@@ -1202,12 +1202,17 @@ void solve_dev(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd
     const int64_t n = c->sym.n;
     const uint32_t gbytes = (uint32_t)(n * 8);
     double *Yin = w.V, *Z = w.V + n;
+    // b already in the factor's column order (time loops): the lowest level
+    // reads its rows contiguously, as the products of a pulled Yin do
+    const bool ident = sperm(c) == c->d_ident;
     for (const Launch &l : p.sfwd) {
         const int pm = prof_mark(c);
         const bool fromb = l.p0 == 4 || l.p0 == 6;
+        const int md = fromb && ident ? (l.p0 == 6 ? 5 : 0) : l.p0;
         const bool pin = l.p0 == 5 || l.p0 == 6;  // partitioned inverse: v = Y_own / d into T, Z rows
         const double *in0 = fromb ? b : ((l.p0 == 0 || l.p0 == 5) ? Yin : w.Y);
         double *out = pin ? w.T : (l.p0 == 1 ? Z : w.Y);
+        const bool gath = md == 4 || md == 6;
         switch (l.kind) {
         case L_SPULL:
             tf::launch_spull(fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, b, sperm(c), c->d_gptr, c->d_gidx, Z, Yin,
@@ -1215,20 +1220,20 @@ void solve_dev(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd
             break;
         case L_ST:
             if (c->s_tiles == 2)
-                tf::launch_stile(l.p0, c->s_tcols, L, fd, c->d_units + l.off, l.cnt, in0, nullptr, Dn, nullptr, out,
+                tf::launch_stile(md, c->s_tcols, L, fd, c->d_units + l.off, l.cnt, in0, nullptr, Dn, nullptr, out,
                                  w.part, sperm(c), nullptr, nullptr, c->st, Z);
             else
-                tf::launch_mtile(l.p0, c->mt_ks_n, 4, 1, L, fd, c->d_units + l.off, l.cnt, 1, in0, nullptr, Dn, nullptr,
-                                 out, w.part, fromb ? gbytes : 0u, fromb ? sperm(c) : nullptr, nullptr, nullptr,
-                                 nullptr, c->st);
+                tf::launch_mtile(md, c->mt_ks_n, 4, 1, L, fd, c->d_units + l.off, l.cnt, 1, in0, nullptr, Dn, nullptr,
+                                 out, w.part, gath ? gbytes : 0u, gath ? sperm(c) : nullptr, nullptr, nullptr,
+                                 nullptr, c->st, Z);
             break;
         case L_STRED:
             if (c->s_tiles == 2)
-                tf::launch_stile_red(l.p0, fd, c->d_units + l.off, l.cnt, in0, nullptr, Dn, out, w.part,
-                                     fromb ? sperm(c) : nullptr, nullptr, nullptr, c->st, Z);
+                tf::launch_stile_red(md, fd, c->d_units + l.off, l.cnt, in0, nullptr, Dn, out, w.part,
+                                     gath ? sperm(c) : nullptr, nullptr, nullptr, c->st, Z);
             else
-                tf::launch_mtile_red(l.p0, 4, 1, fd, c->d_units + l.off, l.cnt, 1, in0, nullptr, Dn, out, w.part,
-                                     fromb ? sperm(c) : nullptr, nullptr, nullptr, c->st);
+                tf::launch_mtile_red(md, 4, 1, fd, c->d_units + l.off, l.cnt, 1, in0, nullptr, Dn, out, w.part,
+                                     gath ? sperm(c) : nullptr, nullptr, nullptr, c->st, Z);
             break;
         }
         prof_close(c, prof_category(l.kind), pm);
@@ -1356,6 +1361,9 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
     double *const mxw = c->d_mT + (size_t)c->sym.n * K;
     const int na = K <= 16 ? 1 : 4;  // 16-wide right-hand-side blocks per tile
     const Plan &p = c->plan;
+    // b already in the factor's column order (time loops): the lowest level
+    // reads its rows contiguously (modes 5 and 0 on b instead of 6 and 4)
+    const bool ident = sperm(c) == c->d_ident;
     const int *lst = c->d_plist;
     const int64_t *pr = c->d_ppref;
     for (const Launch &l : p.mfwd) {
@@ -1367,12 +1375,14 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
             break;
         case L_MT:  // own forward (Y from Yin, or from b), struct forward (Z from Y)
             if (l.p0 == 5 || l.p0 == 6)  // partitioned inverse: v rows into T, Z rows
-                tf::launch_mtile(l.p0, c->mt_ks_n, l.nb, na, L, fd, c->d_units + l.off, l.cnt, K,
-                                 l.p0 == 6 ? b : c->d_mV, nullptr, Dn, nullptr, c->d_mT, c->d_part,
-                                 l.p0 == 6 ? gbytes : 0u, sperm(c), nullptr, nullptr, nullptr, c->st, c->d_mZ);
+                tf::launch_mtile(l.p0 == 6 && !ident ? 6 : 5, c->mt_ks_n, l.nb, na, L, fd, c->d_units + l.off, l.cnt,
+                                 K, l.p0 == 6 ? b : c->d_mV, nullptr, Dn, nullptr, c->d_mT, c->d_part,
+                                 l.p0 == 6 && !ident ? gbytes : 0u, sperm(c), nullptr, nullptr, nullptr, c->st,
+                                 c->d_mZ);
             else if (l.p0 == 4)
-                tf::launch_mtile(4, c->mt_ks_n, l.nb, na, L, fd, c->d_units + l.off, l.cnt, K, b, nullptr, Dn, nullptr,
-                                 c->d_mY, c->d_part, gbytes, sperm(c), nullptr, nullptr, nullptr, c->st);
+                tf::launch_mtile(ident ? 0 : 4, c->mt_ks_n, l.nb, na, L, fd, c->d_units + l.off, l.cnt, K, b, nullptr,
+                                 Dn, nullptr, c->d_mY, c->d_part, ident ? 0u : gbytes, sperm(c), nullptr, nullptr,
+                                 nullptr, c->st);
             else
                 tf::launch_mtile(l.p0, c->mt_ks_n, l.nb, na, L, fd, c->d_units + l.off, l.cnt, K,
                                  l.p0 == 0 ? c->d_mV : c->d_mY, nullptr, Dn, nullptr, l.p0 == 0 ? c->d_mY : c->d_mZ,
@@ -1380,11 +1390,12 @@ void solve_block(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *
             break;
         case L_MTRED:
             if (l.p0 == 5 || l.p0 == 6)
-                tf::launch_mtile_red(l.p0, l.nb, na, fd, c->d_units + l.off, l.cnt, K, l.p0 == 6 ? b : c->d_mV,
-                                     nullptr, Dn, c->d_mT, c->d_part, sperm(c), nullptr, nullptr, c->st, c->d_mZ);
+                tf::launch_mtile_red(l.p0 == 6 && !ident ? 6 : 5, l.nb, na, fd, c->d_units + l.off, l.cnt, K,
+                                     l.p0 == 6 ? b : c->d_mV, nullptr, Dn, c->d_mT, c->d_part, sperm(c), nullptr,
+                                     nullptr, c->st, c->d_mZ);
             else if (l.p0 == 4)
-                tf::launch_mtile_red(4, l.nb, na, fd, c->d_units + l.off, l.cnt, K, b, nullptr, Dn, c->d_mY, c->d_part,
-                                     sperm(c), nullptr, nullptr, c->st);
+                tf::launch_mtile_red(ident ? 0 : 4, l.nb, na, fd, c->d_units + l.off, l.cnt, K, b, nullptr, Dn, c->d_mY,
+                                     c->d_part, sperm(c), nullptr, nullptr, c->st);
             else
                 tf::launch_mtile_red(l.p0, l.nb, na, fd, c->d_units + l.off, l.cnt, K, l.p0 == 0 ? c->d_mV : c->d_mY, nullptr,
                                      Dn, l.p0 == 0 ? c->d_mY : c->d_mZ, c->d_part, nullptr, nullptr, nullptr, c->st);
