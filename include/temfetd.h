@@ -116,6 +116,10 @@ int tf_analyze(tf_ctx *ctx, int leaf_size);
  *               second stream while the steps before the check run, and
  *               tf_adjoint its sweep's next dt while the current one is in
  *               use; 0: at the point of use.  Bit-identical.
+ *   "s_lt"      1 (default): single-source contexts also keep the row-major
+ *               copy of L21 that multi-source contexts make after each
+ *               factorisation, and the single solve's backward products read
+ *               it row-wise; 0: they read L column-wise.  Equal to rounding.
  *   "rhs_chain" 1 (default): regular steps of a multi-source tf_run (even
  *               source count, no checkpointing) form Msigma (4 e2 - e3) as
  *               4 Msigma e2 - Msigma e3, the second product kept from the step

@@ -104,6 +104,9 @@ void launch_mpull(const FrontDesc *fd, const int *list, const int64_t *pref, int
 void launch_stile(int mode, int nc, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits,
                   const double *in0, const double *in1, const double *Dn, const int *sidx, double *out, double *part,
                   const int *perm, double *x1, double *x2, hipStream_t st, double *z = nullptr);
+void launch_stile_lt(int mode, const double *LT, const FrontDesc *fd, const TileUnit *units, int nunits,
+                     const double *in0, const double *in1, const double *Dn, const int *sidx, double *out,
+                     double *part, const int *perm, double *x1, double *x2, hipStream_t st);
 void launch_stile_red(int mode, const FrontDesc *fd, const TileUnit *units, int nunits, const double *in0,
                       const double *in1, const double *Dn, double *out, const double *part, const int *perm,
                       double *x1, double *x2, hipStream_t st, double *z = nullptr);
@@ -349,6 +352,9 @@ struct tf_ctx {
     } rm{};
     const int *solve_perm = nullptr;
     int s_tcols = 32;  // output columns per pass of the vector T-form tiles (16 or 32)
+    // single-source contexts keep the row-major copy of L21 as well, and the
+    // single solve's backward products read it as N-form tiles (tuning key s_lt)
+    bool s_lt = true;
     double *d_U = nullptr, *d_V = nullptr, *d_xw = nullptr, *d_Y = nullptr, *d_T = nullptr, *d_W = nullptr;
     int64_t *d_woff = nullptr;
     double *d_L[2] = {nullptr, nullptr}, *d_D[2] = {nullptr, nullptr};
@@ -1136,7 +1142,7 @@ void copy_l21(tf_ctx *c, int slot) {
 void prepare_slot(tf_ctx *c, int slot) {
     if (!c->d_L[slot]) c->d_L[slot] = dalloc<double>(c->sym.ltotal + TF_READ_SLACK);
     if (!c->d_D[slot]) c->d_D[slot] = dalloc<double>(c->sym.n);
-    if (c->nsrc > 1 && !c->d_LT[slot]) c->d_LT[slot] = dalloc<double>(c->lt_total);
+    if ((c->nsrc > 1 || c->s_lt) && !c->d_LT[slot]) c->d_LT[slot] = dalloc<double>(c->lt_total);
     tf::launch_fill0(c->d_L[slot], c->sym.ltotal, c->st);
     tf::launch_fill0(c->d_U, c->sym.utotal, c->st);
     tf::launch_flags_init(c->d_flag, c->st);
@@ -1189,8 +1195,10 @@ void solve_dev_gemv(tf_ctx *c, const double *L, const double *Dn, const FrontDes
 // struct forward, backward T and X) run as tile launches per level with one
 // right-hand side; the forward pull forms each level's Yin from b and the
 // descendants' Z, the backward X epilogue writes x in edge order.
+// LT: the factor's row-major L21 copy (backward products as N-form tiles), or
+// nullptr (T-form tiles over L itself)
 void solve_dev(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd, const double *b, double *x,
-               const SolveWork &w, double *x2 = nullptr) {
+               const SolveWork &w, double *x2 = nullptr, const double *LT = nullptr) {
     if (!c->s_tiles) {
         solve_dev_gemv(c, L, Dn, fd, b, x, w);
         if (x2) HIPCHK(hipMemcpyAsync(x2, x, sizeof(double) * c->sym.n, hipMemcpyDeviceToDevice, c->st));
@@ -1242,8 +1250,12 @@ void solve_dev(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd
         const int pm = prof_mark(c);
         const bool bx = l.p0 == 3 || l.p0 == 7;
         double *out = bx ? (l.aux == 1 ? nullptr : w.xw) : w.T;
+        const bool nt = LT && c->s_lt && c->s_tiles == 2;  // N form over the row-major copy
         if (l.p0 == 7) {  // partitioned inverse: [v; x(struct)] from T and xw
-            if (l.kind == L_ST)
+            if (l.kind == L_ST && nt)
+                tf::launch_stile_lt(7, LT, fd, c->d_units + l.off, l.cnt, w.xw, w.T, Dn, c->d_sidx, out, w.part,
+                                    sperm(c), x, x2, c->st);
+            else if (l.kind == L_ST)
                 tf::launch_stile(7, c->s_tcols, L, fd, c->d_units + l.off, l.cnt, w.xw, w.T, Dn, c->d_sidx, out,
                                  w.part, sperm(c), x, x2, c->st);
             else
@@ -1254,7 +1266,10 @@ void solve_dev(tf_ctx *c, const double *L, const double *Dn, const FrontDesc *fd
         }
         switch (l.kind) {
         case L_ST:
-            if (c->s_tiles == 2)
+            if (nt && l.p0 == 2)
+                tf::launch_stile_lt(2, LT, fd, c->d_units + l.off, l.cnt, w.xw, w.Y, Dn, c->d_sidx, out, w.part,
+                                    sperm(c), nullptr, nullptr, c->st);
+            else if (c->s_tiles == 2)
                 tf::launch_stile(l.p0, c->s_tcols, L, fd, c->d_units + l.off, l.cnt, bx ? w.T : w.xw, bx ? w.T : w.Y,
                                  Dn, c->d_sidx, out, w.part, sperm(c), bx ? x : nullptr, bx ? x2 : nullptr, c->st);
             else
@@ -1497,7 +1512,7 @@ void dcopy(tf_ctx *c, double *dst, const double *src) {
 // x2, if given, receives a copy of the solution
 void step_solve(tf_ctx *c, int slot, const double *b, double *x, double *x2 = nullptr) {
     if (c->nsrc == 1) {
-        solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, b, x, ctx_work(c), x2);
+        solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, b, x, ctx_work(c), x2, c->d_LT[slot]);
     } else {
         solve_block(c, c->d_L[slot], c->d_D[slot], c->d_fd, c->nsrc, b, x, x2, c->d_LT[slot]);
     }
@@ -2748,6 +2763,9 @@ int tf_set_tuning(tf_ctx *c, const char *key, int value) {
             if (value < 0) return fail(c, TF_ERR_ARG, "mt_wide must be >= 0");
             c->mt_wide = value;
             replan = true;
+        } else if (k == "s_lt") {
+            if (value != 0 && value != 1) return fail(c, TF_ERR_ARG, "s_lt must be 0 or 1");
+            c->s_lt = value != 0;
         } else if (k == "rhs_chain") {
             if (value != 0 && value != 1) return fail(c, TF_ERR_ARG, "rhs_chain must be 0 or 1");
             c->rhs_chain = value != 0;
@@ -2934,7 +2952,7 @@ int tf_solve(tf_ctx *c, int slot, int nrhs, const double *b, double *x) {
         const int n = c->topo.nE;
         if (nrhs == 1) {
             HIPCHK(hipMemcpyAsync(c->d_tmpb, b, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
-            solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, c->d_tmpb, c->d_tmpx);
+            solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, c->d_tmpb, c->d_tmpx, ctx_work(c), nullptr, c->d_LT[slot]);
             HIPCHK(hipMemcpyAsync(x, c->d_tmpx, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
         } else {
             // blocks of up to 64 right-hand sides through the block solve
@@ -2985,7 +3003,7 @@ int tf_solve_noalloc(tf_ctx *c, int slot, const double *b, double *x) {
         const SolveWork w{p, p + nv, p + nv + nn, p + nv + 2 * nn, p + nv + 3 * nn};
         const bool prof = c->prof_on;  // no event pool growth inside this call
         c->prof_on = false;
-        solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, b, x, w);
+        solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, b, x, w, nullptr, c->d_LT[slot]);
         c->prof_on = prof;
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->st));
@@ -3007,7 +3025,7 @@ int tf_time_solve(tf_ctx *c, int slot, int nrhs, int reps, double *ms_per_solve)
         tf::launch_fill_pattern((int64_t)n * nrhs, b, c->st);
         auto one = [&] {
             if (models) solve_dev(c, c->d_Lmm, c->d_Dmm, c->d_fd_cur, b, x);  // every model's current factor
-            else if (nrhs == 1) solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, b, x);
+            else if (nrhs == 1) solve_dev(c, c->d_L[slot], c->d_D[slot], c->d_fd, b, x, ctx_work(c), nullptr, c->d_LT[slot]);
             else solve_block(c, c->d_L[slot], c->d_D[slot], c->d_fd, nrhs, b, x, nullptr, c->d_LT[slot]);
         };
         one();  // warm-up

@@ -516,6 +516,56 @@ __global__ void __launch_bounds__(256) k_stile_n(const double *L, const FrontDes
     }
 }
 
+// The backward products (T: L21^T x(struct), partitioned: [strict(L11^-1); W]^T
+// [v; x(struct)]) of one right-hand side as N-form tiles over the row-major
+// copy of the front's block (LT: the m rows of L21, or all f rows of a
+// partitioned-inverse front, row stride ldt): a tile of 64 output columns,
+// a lane reading two neighbouring columns of one row per load, lanes 0-31
+// and 32-63 taking alternate rows (folded at the end).  The gathered
+// right-hand-side values are fetched one per lane for 64 rows at a time and
+// handed to the lanes that use them by shuffles.  Same units, partial sums
+// and epilogue as the T-form tiles (k_stile_t).
+template <int MODE>
+__global__ void __launch_bounds__(256) k_stile_nt(const double *LT, const FrontDesc *fd, const TileUnit *units,
+                                                  int nunits, const double *in0, const double *in1, const double *Dn,
+                                                  const int *sidx, double *out, double *part, MtEdgeOut xo) {
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int u = xcd_block(blockIdx.x, gridDim.x) * 4 + w;
+    if (u >= nunits) return;
+    const TileUnit tu = units[u];
+    const FrontDesc d = fd[tu.front];
+    int nout, roff;
+    mt_geom<MODE>(d, nout, roff);
+    const int lane = threadIdx.x & 63, cp = lane >> 5;
+    const int o = tu.t0 + 2 * (lane & 31);  // columns o, o + 1 of the block
+    const int rows = MODE == MT_PIN_BWD ? d.f : d.m;
+    const __amdgpu_buffer_rsrc_t rL = mt_rsrc(LT + d.toff, (uint32_t)((int64_t)rows * d.ldt * 8));
+    double acc0 = 0.0, acc1 = 0.0;
+    for (int k = tu.k0; k < tu.k1; k += 64) {
+        const int kl = k + lane;  // this lane's row of the 64: its right-hand-side value (zero past k1)
+        const double bl = kl < tu.k1 ? st_b<MODE>(d, kl, in0, in1, sidx, nullptr) : 0.0;
+#pragma unroll
+        for (int g = 0; g < 4; g++) {  // 16 rows per group, a row pair per step
+            double x0[8], x1[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                mt_bld2(rL, (uint32_t)((k + 16 * g + 2 * q + cp) * d.ldt + o) * 8u, 0u, x0[q], x1[q]);
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const double b = __shfl(bl, 16 * g + 2 * q + cp, 64);  // rows past k1 (or past the block) have b = 0
+                acc0 += x0[q] * b;
+                acc1 += x1[q] * b;
+            }
+        }
+    }
+    acc0 += __shfl_xor(acc0, 32, 64);
+    acc1 += __shfl_xor(acc1, 32, 64);
+    if (cp == 0) {
+        st_out<MODE>(d, tu, o, acc0, nout, in0, in1, Dn, out, part, xo);
+        st_out<MODE>(d, tu, o + 1, acc1, nout, in0, in1, Dn, out, part, xo);
+    }
+}
+
 template <int MODE, int NC>
 __global__ void __launch_bounds__(256) k_stile_t(const double *L, const FrontDesc *fd, const TileUnit *units,
                                                  int nunits, const double *in0, const double *in1, const double *Dn,
@@ -626,6 +676,18 @@ void launch_red_ps(int mode, int nb, const FrontDesc *fd, const TileUnit *units,
     default: TF_RED(MT_BWD_X); break;
     }
 #undef TF_RED
+}
+// backward products of one right-hand side over the row-major copy LT (modes 2 and 7)
+void launch_stile_lt(int mode, const double *LT, const FrontDesc *fd, const TileUnit *units, int nunits,
+                     const double *in0, const double *in1, const double *Dn, const int *sidx, double *out,
+                     double *part, const int *perm, double *x1, double *x2, hipStream_t st) {
+    if (nunits <= 0) return;
+    const dim3 g((nunits + 3) / 4), b(256);
+    const MtEdgeOut xo{perm, x1, x2, nullptr};
+    if (mode == MT_PIN_BWD)
+        hipLaunchKernelGGL(k_stile_nt<MT_PIN_BWD>, g, b, 0, st, LT, fd, units, nunits, in0, in1, Dn, sidx, out, part, xo);
+    else
+        hipLaunchKernelGGL(k_stile_nt<MT_BWD_T>, g, b, 0, st, LT, fd, units, nunits, in0, in1, Dn, sidx, out, part, xo);
 }
 void launch_stile(int mode, int nc, const double *L, const FrontDesc *fd, const TileUnit *units, int nunits,
                   const double *in0, const double *in1, const double *Dn, const int *sidx, double *out, double *part,
