@@ -480,7 +480,7 @@ template <int MODE>
 __global__ void __launch_bounds__(256) k_stile_n(const double *L, const FrontDesc *fd, const TileUnit *units,
                                                  int nunits, const double *in0, const double *Dn, double *out,
                                                  double *part, MtEdgeOut xo) {
-    constexpr int U = 8;  // column pairs per step
+    constexpr int U = 8;  // column pairs per group (four groups per 64 columns)
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int u = xcd_block(blockIdx.x, gridDim.x) * 4 + w;
     if (u >= nunits) return;
@@ -492,20 +492,25 @@ __global__ void __launch_bounds__(256) k_stile_n(const double *L, const FrontDes
     const int o = tu.t0 + 2 * (lane & 31);  // rows o, o + 1 (of the block's rows from roff)
     const __amdgpu_buffer_rsrc_t rL = mt_rsrc(L + d.loff, (uint32_t)((int64_t)d.nc * d.ld * 8));
     double acc0 = 0.0, acc1 = 0.0;
-    for (int k = tu.k0; k < tu.k1; k += 2 * U) {
-        double x0[U], x1[U], bv[U];
+    for (int k = tu.k0; k < tu.k1; k += 64) {
+        // the right-hand-side values of the next 64 columns, one per lane,
+        // handed to the lanes that use them by shuffles
+        const int kl = k + lane;
+        const double bl = kl < tu.k1 ? st_b<MODE>(d, kl, in0, nullptr, nullptr, xo.perm) : 0.0;
 #pragma unroll
-        for (int q = 0; q < U; q++) {
-            const int kk = k + 2 * q + cp;
-            mt_bld2(rL, (uint32_t)(kk * d.ld + roff + o) * 8u, 0u, x0[q], x1[q]);
-            bv[q] = st_b<MODE>(d, min(kk, tu.k1 - 1), in0, nullptr, nullptr, xo.perm);
-        }
+        for (int g = 0; g < 4; g++) {  // U column pairs per group
+            if (k + 2 * U * g >= tu.k1) break;
+            double x0[U], x1[U];
 #pragma unroll
-        for (int q = 0; q < U; q++) {
-            const bool kv = k + 2 * q + cp < tu.k1;
-            const double b = kv ? bv[q] : 0.0;
-            acc0 += (kv ? x0[q] : 0.0) * b;
-            acc1 += (kv ? x1[q] : 0.0) * b;
+            for (int q = 0; q < U; q++)
+                mt_bld2(rL, (uint32_t)((k + 2 * U * g + 2 * q + cp) * d.ld + roff + o) * 8u, 0u, x0[q], x1[q]);
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                const bool kv = k + 2 * U * g + 2 * q + cp < tu.k1;
+                const double b = __shfl(bl, 2 * U * g + 2 * q + cp, 64);
+                acc0 += (kv ? x0[q] : 0.0) * b;
+                acc1 += (kv ? x1[q] : 0.0) * b;
+            }
         }
     }
     acc0 += __shfl_xor(acc0, 32, 64);
