@@ -124,6 +124,11 @@ int tf_analyze(tf_ctx *ctx, int leaf_size);
  *               source count, no checkpointing) form Msigma (4 e2 - e3) as
  *               4 Msigma e2 - Msigma e3, the second product kept from the step
  *               before; 0: the history combination first.  Equal to rounding.
+ *   "f_band"    the factorisation's trailing-update tiles are walked in
+ *               bands of this many 64-row tile rows, column by column inside
+ *               a band, so that the tiles running at once share their operand
+ *               panels in L2 (default 16; 0: row by row; 0 .. 1024).
+ *               Bit-identical.
  *   "indefinite" 0 (default): a factorisation with a negative pivot fails
  *               with TF_ERR_NUMERIC (P is SPD in exact arithmetic; the
  *               reference's SPD factorisation would fail there and be redone

@@ -54,7 +54,7 @@ void launch_factor_diag(double *L, const FrontDesc *fd, const int *list, int cnt
 void launch_factor_trsm(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
                         int p0, const double *Dn, const double *W, const int64_t *woff, hipStream_t st);
 void launch_factor_update(double *L, double *U, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
-                          int64_t nwg, int p0, int mode, const double *Dn, hipStream_t st);
+                          int64_t nwg, int p0, int mode, const double *Dn, int band, hipStream_t st);
 // factor.hip (inversion)
 void launch_inv_diag(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
                      hipStream_t st);
@@ -355,6 +355,9 @@ struct tf_ctx {
     // single-source contexts keep the row-major copy of L21 as well, and the
     // single solve's backward products read it as N-form tiles (tuning key s_lt)
     bool s_lt = true;
+    // trailing-update tiles walked in bands of this many tile rows (0: row by
+    // row; tuning key "f_band")
+    int f_band = 16;
     double *d_U = nullptr, *d_V = nullptr, *d_xw = nullptr, *d_Y = nullptr, *d_T = nullptr, *d_W = nullptr;
     int64_t *d_woff = nullptr;
     double *d_L[2] = {nullptr, nullptr}, *d_D[2] = {nullptr, nullptr};
@@ -1110,7 +1113,8 @@ void run_factor(tf_ctx *c, double *L, double *Dn, const FrontDesc *fd) {
             tf::launch_factor_trsm(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, Dn, c->d_W, c->d_woff, c->st);
             break;
         case L_FUPD:
-            tf::launch_factor_update(L, c->d_U, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, l.aux, Dn, c->st);
+            tf::launch_factor_update(L, c->d_U, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, l.aux, Dn, c->f_band,
+                                     c->st);
             break;
         case L_IMERGE_T:
             tf::launch_inv_merge(0, L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, c->d_W, c->d_woff, c->st);
@@ -2766,6 +2770,9 @@ int tf_set_tuning(tf_ctx *c, const char *key, int value) {
         } else if (k == "s_lt") {
             if (value != 0 && value != 1) return fail(c, TF_ERR_ARG, "s_lt must be 0 or 1");
             c->s_lt = value != 0;
+        } else if (k == "f_band") {
+            if (value < 0 || value > 1024) return fail(c, TF_ERR_ARG, "f_band must be in 0..1024");
+            c->f_band = value;
         } else if (k == "rhs_chain") {
             if (value != 0 && value != 1) return fail(c, TF_ERR_ARG, "rhs_chain must be 0 or 1");
             c->rhs_chain = value != 0;

@@ -232,60 +232,61 @@ __device__ __forceinline__ void upd_wave(double *L, double *U, const FrontDesc &
     for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int b = 0; b < 4; b++) acc[a][b] = (tf_d4){0.0, 0.0, 0.0, 0.0};
-    auto mult = [&](const double xr[KS][4], const double yc[KS][4]) {
+    // The column operand is scaled by D when its stage is multiplied, not
+    // when it is loaded, so that a stage's loads stay in flight while the
+    // stage before it runs on the matrix cores.
+    auto mult = [&](const double xr[KS][4], double yc[KS][4], const double dk[KS]) {
 #pragma unroll
-        for (int s = 0; s < KS; s++)
+        for (int s = 0; s < KS; s++) {
+#pragma unroll
+            for (int a = 0; a < 4; a++) yc[s][a] *= dk[s];
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
                 for (int b = 0; b < 4; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(yc[s][a], xr[s][b], acc[a][b], 0, 0, 0);
+        }
     };
-    auto load = [&](int ib, double xr[KS][4], double yc[KS][4]) {
+    auto load = [&](int ib, double xr[KS][4], double yc[KS][4], double dk[KS]) {
 #pragma unroll
         for (int s = 0; s < KS; s++) {
             const uint32_t sl = (uint32_t)((ib + 4 * s) * d.ld) * 8u;
-            const double dk = Dk[ib + 4 * s + lk];
+            dk[s] = Dk[ib + 4 * s + lk];
 #pragma unroll
             for (int p = 0; p < 2; p++) mt_bld2(rL, lr + 256u * p, sl, xr[s][2 * p], xr[s][2 * p + 1]);
 #pragma unroll
-            for (int p = 0; p < 2; p++) {
-                double y0, y1;
-                mt_bld2(rL, lc + 256u * p, sl, y0, y1);
-                yc[s][2 * p] = y0 * dk;
-                yc[s][2 * p + 1] = y1 * dk;
-            }
+            for (int p = 0; p < 2; p++) mt_bld2(rL, lc + 256u * p, sl, yc[s][2 * p], yc[s][2 * p + 1]);
         }
     };
     const int nfull = (kend - kbeg) / SW;
     int ib = kbeg;
     if (nfull > 0) {
-        double xr0[KS][4], yc0[KS][4], xr1[KS][4], yc1[KS][4];
+        double xr0[KS][4], yc0[KS][4], xr1[KS][4], yc1[KS][4], dk0[KS], dk1[KS];
         int j = 0;
         if (nfull & 1) {  // odd count: the first stage on its own, the rest in pairs
-            load(ib, xr0, yc0);
-            mult(xr0, yc0);
+            load(ib, xr0, yc0, dk0);
+            mult(xr0, yc0, dk0);
             ib += SW;
             j = 1;
         }
         if (j < nfull) {
             const int last = kbeg + (nfull - 1) * SW;
-            load(ib, xr0, yc0);
+            load(ib, xr0, yc0, dk0);
             for (; j < nfull; j += 2) {  // ib + SW <= last
-                load(ib + SW, xr1, yc1);
+                load(ib + SW, xr1, yc1, dk1);
                 __builtin_amdgcn_sched_barrier(0);
-                mult(xr0, yc0);
+                mult(xr0, yc0, dk0);
                 __builtin_amdgcn_sched_barrier(0);
-                load(min(ib + 2 * SW, last), xr0, yc0);
+                load(min(ib + 2 * SW, last), xr0, yc0, dk0);
                 __builtin_amdgcn_sched_barrier(0);
-                mult(xr1, yc1);
+                mult(xr1, yc1, dk1);
                 __builtin_amdgcn_sched_barrier(0);
                 ib += 2 * SW;
             }
         }
     }
     if (ib < kend) {  // partial stage: columns past kend read as zero
-        double xr[KS][4], yc[KS][4];
+        double xr[KS][4], yc[KS][4], dk[KS];
 #pragma unroll
         for (int s = 0; s < KS; s++) {
             const int k = ib + 4 * s + lk;
@@ -293,18 +294,13 @@ __device__ __forceinline__ void upd_wave(double *L, double *U, const FrontDesc &
             const int kc = kv ? k : kend - 1;
             const uint32_t sl = (uint32_t)(kc * d.ld) * 8u;
             const uint32_t l0 = (uint32_t)(rb + 2 * l16) * 8u, c0 = (uint32_t)(cb + 2 * l16) * 8u;
-            const double dk = kv ? Dk[kc] : 0.0;
+            dk[s] = kv ? Dk[kc] : 0.0;
 #pragma unroll
             for (int p = 0; p < 2; p++) mt_bld2(rL, sl + l0 + 256u * p, 0u, xr[s][2 * p], xr[s][2 * p + 1]);
 #pragma unroll
-            for (int p = 0; p < 2; p++) {
-                double y0, y1;
-                mt_bld2(rL, sl + c0 + 256u * p, 0u, y0, y1);
-                yc[s][2 * p] = y0 * dk;
-                yc[s][2 * p + 1] = y1 * dk;
-            }
+            for (int p = 0; p < 2; p++) mt_bld2(rL, sl + c0 + 256u * p, 0u, yc[s][2 * p], yc[s][2 * p + 1]);
         }
-        mult(xr, yc);
+        mult(xr, yc, dk);
     }
     // C -= acc on the stored lower triangle; all loads of a column block
     // first.  A row pair whose rows are both stored and whose address is
@@ -732,9 +728,35 @@ __global__ void __launch_bounds__(128) k_factor_trsm(double *L, const FrontDesc 
 //          super-panel with all of its columns at once.
 // The trailing block is read and written once per super-panel instead of
 // once per panel.  pref[] counts tiles per front.
+// Tile (ti, tj), tj <= ti, of a lower triangle of nt tile rows for index u
+// in band order: bands of g tile rows, each walked column by column (the g
+// tiles of a column in a row, then the triangle's part of the band).  The
+// tiles that run at once on one XCD then share a few row panels and column
+// panels in its L2 instead of streaming every column panel once per tile
+// row.  g = 0 walks the triangle row by row.
+__device__ __forceinline__ void band_index(int64_t u, int nt, int g, int &ti, int &tj) {
+    tri_index(u, ti, tj);
+    if (g <= 0) return;
+    const int r0 = ti - ti % g, h = min(g, nt - r0);
+    int64_t v = u - (int64_t)r0 * (r0 + 1) / 2;
+    if (v < (int64_t)r0 * h) {
+        tj = (int)(v / h);
+        ti = r0 + (int)(v % h);
+        return;
+    }
+    v -= (int64_t)r0 * h;
+    int c = 0;
+    while (v >= h - c) {
+        v -= h - c;
+        c++;
+    }
+    tj = r0 + c;
+    ti = tj + (int)v;
+}
+
 __global__ void __launch_bounds__(TF_NT, 2) k_factor_update(double *L, double *U, const FrontDesc *fd,
                                                             const int *list, const int64_t *pref, int cnt,
-                                                            int p0, int mode, const double *Dn) {
+                                                            int p0, int mode, const double *Dn, int band) {
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t ug = (int64_t)xcd_block(blockIdx.x, gridDim.x) * 4 + w;
     if (ug >= pref[cnt]) return;
@@ -752,7 +774,7 @@ __global__ void __launch_bounds__(TF_NT, 2) k_factor_update(double *L, double *U
     } else {
         const int pe = min(d.nc, p0 + TF_SP);
         int ti, tj;
-        tri_index(u, ti, tj);
+        band_index(u, (d.f - pe + 63) / 64, band, ti, tj);
         upd_wave<2>(L, U, d, Dn, p0, pe, pe + 64 * ti, pe + 64 * tj, d.f);
     }
 }
@@ -789,55 +811,64 @@ __device__ __forceinline__ void inv_wave(const double *R, int ldr, uint32_t rbyt
 #pragma unroll
         for (int b = 0; b < 4; b++) acc[a][b] = (tf_d4){0.0, 0.0, 0.0, 0.0};
     if (k0 >= k1) return;
+    // A stage's values are loaded raw and masked when the stage is
+    // multiplied, so that its loads stay in flight while the stage before it
+    // runs on the matrix cores.
     auto load = [&](int ib, double cv[2][4], double rv[2][4]) {
         const int kp = ib + 2 * lk;  // the lane's two inner values kp, kp + 1
 #pragma unroll
         for (int a = 0; a < 4; a++) {
             const int c = cb + 16 * a + l16;
-            double x0, x1;
-            mt_bld2(rC, (uint32_t)((int64_t)c * ldc + kp) * 8u, 0u, x0, x1);  // zeroed past k1 below
-            const double v[2] = {x0, x1};
-#pragma unroll
-            for (int s2 = 0; s2 < 2; s2++) {
-                const int k = kp + s2;
-                double y = k < k1 ? v[s2] : 0.0;
-                if (UC) y = k > c ? y : (k == c ? 1.0 : 0.0);
-                cv[s2][a] = y;
-            }
+            mt_bld2(rC, (uint32_t)((int64_t)c * ldc + kp) * 8u, 0u, cv[0][a], cv[1][a]);  // zeroed past k1
         }
 #pragma unroll
         for (int s2 = 0; s2 < 2; s2++) {
             const int k = min(kp + s2, k1 - 1);
-            const bool kv = kp + s2 < k1;
 #pragma unroll
             for (int p = 0; p < 2; p++) {
                 const int r0 = rb + 32 * p + 2 * l16;
-                double y0, y1;
-                mt_bld2(rR, (uint32_t)((int64_t)k * ldr + r0) * 8u, 0u, y0, y1);
+                mt_bld2(rR, (uint32_t)((int64_t)k * ldr + r0) * 8u, 0u, rv[s2][2 * p], rv[s2][2 * p + 1]);
+            }
+        }
+    };
+    auto mult = [&](int ib, double cv[2][4], double rv[2][4]) {
+        const int kp = ib + 2 * lk;
+#pragma unroll
+        for (int s2 = 0; s2 < 2; s2++) {
+            const int k = kp + s2;
+            const bool kv = k < k1;
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const int c = cb + 16 * a + l16;
+                double y = kv ? cv[s2][a] : 0.0;
+                if (UC) y = k > c ? y : (k == c ? 1.0 : 0.0);
+                cv[s2][a] = y;
+            }
+            const int kc = min(k, k1 - 1);
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                const int r0 = rb + 32 * p + 2 * l16;
+                double y0 = rv[s2][2 * p], y1 = rv[s2][2 * p + 1];
                 if (UR) {
-                    y0 = r0 > k ? y0 : (r0 == k ? 1.0 : 0.0);
-                    y1 = r0 + 1 > k ? y1 : (r0 + 1 == k ? 1.0 : 0.0);
+                    y0 = r0 > kc ? y0 : (r0 == kc ? 1.0 : 0.0);
+                    y1 = r0 + 1 > kc ? y1 : (r0 + 1 == kc ? 1.0 : 0.0);
                 }
                 rv[s2][2 * p] = kv ? y0 : 0.0;
                 rv[s2][2 * p + 1] = kv ? y1 : 0.0;
             }
-        }
-    };
-    auto mult = [&](const double cv[2][4], const double rv[2][4]) {
-#pragma unroll
-        for (int s2 = 0; s2 < 2; s2++)
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
                 for (int b = 0; b < 4; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(cv[s2][a], rv[s2][b], acc[a][b], 0, 0, 0);
+        }
     };
     const int nst = (k1 - k0 + SW - 1) / SW;
     double c0v[2][4], r0v[2][4], c1v[2][4], r1v[2][4];
     int js = 0;
     if (nst & 1) {
         load(k0, c0v, r0v);
-        mult(c0v, r0v);
+        mult(k0, c0v, r0v);
         js = 1;
     }
     if (js < nst) {
@@ -845,11 +876,11 @@ __device__ __forceinline__ void inv_wave(const double *R, int ldr, uint32_t rbyt
         for (; js < nst; js += 2) {
             load(k0 + (js + 1) * SW, c1v, r1v);
             __builtin_amdgcn_sched_barrier(0);
-            mult(c0v, r0v);
+            mult(k0 + js * SW, c0v, r0v);
             __builtin_amdgcn_sched_barrier(0);
             load(k0 + min(js + 2, nst - 1) * SW, c0v, r0v);
             __builtin_amdgcn_sched_barrier(0);
-            mult(c1v, r1v);
+            mult(k0 + (js + 1) * SW, c1v, r1v);
             __builtin_amdgcn_sched_barrier(0);
         }
     }
@@ -966,10 +997,10 @@ void launch_factor_trsm(double *L, const FrontDesc *fd, const int *list, const i
                        woff);
 }
 void launch_factor_update(double *L, double *U, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt,
-                          int64_t ntiles, int p0, int mode, const double *Dn, hipStream_t st) {
+                          int64_t ntiles, int p0, int mode, const double *Dn, int band, hipStream_t st) {
     if (ntiles > 0)
         hipLaunchKernelGGL(k_factor_update, dim3((unsigned)((ntiles + 3) / 4)), dim3(TF_NT), 0, st, L, U, fd, list,
-                           pref, cnt, p0, mode, Dn);
+                           pref, cnt, p0, mode, Dn, band);
 }
 
 void launch_inv_diag(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,

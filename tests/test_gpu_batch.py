@@ -248,11 +248,32 @@ def test_block_solve_tuning_knobs(gpu_mod_big_batch, knobs):
     assert berr < 1e-12
 
 
+@pytest.mark.parametrize("band", [0, 1, 3])
+def test_factor_update_order_is_bit_identical(gpu_mod_big_batch, band):
+    """The trailing update's tile order (row by row, or bands of tile rows
+    walked column by column) only changes which tiles run together: every
+    tile sums the same terms in the same order, so the factor and the solve
+    are bit-identical to those of the default order."""
+    fe, _ = gpu_mod_big_batch
+    rng = np.random.default_rng(11)
+    B = rng.standard_normal((3, fe.nE))
+    X0 = fe.solve(0, B)
+    print(f"[gpu] factor update tile order: f_band={band}", flush=True)
+    fe.set_tuning("f_band", band)
+    try:
+        fe.factor(0, 1e-8)
+        X = fe.solve(0, B)
+    finally:
+        fe.set_tuning("f_band", 16)
+        fe.factor(0, 1e-8)
+    np.testing.assert_array_equal(X, X0)
+
+
 def test_tuning_rejects_bad_values(gpu_mod_big_batch):
     from temf3dt_amd.fetd import FetdError
     fe, _ = gpu_mod_big_batch
     for k, v in [("mt_ks_n", 3), ("mt_chmin", 10), ("mt_units", 1), ("graphs", 2), ("overlap", -1), ("mt_wide", -1),
-                 ("mt_fuse", 3), ("rhs_chain", 2), ("pinv", -1), ("s_tiles", 0), ("no_such_knob", 1)]:
+                 ("mt_fuse", 3), ("rhs_chain", 2), ("f_band", -1), ("f_band", 1025), ("pinv", -1), ("s_tiles", 0), ("no_such_knob", 1)]:
         with pytest.raises(FetdError):
             fe.set_tuning(k, v)
 
