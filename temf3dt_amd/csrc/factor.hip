@@ -720,14 +720,6 @@ __global__ void __launch_bounds__(128) k_factor_trsm(double *L, const FrontDesc 
     trsm_wave<2>(L, d, p0, rb, Dn, W + woff[sn]);
 }
 
-// Updates of a super-panel [P0, P0 + TF_SP) of TF_PB-column panels, one
-// 64 x 64 tile per wave:
-//   mode 0 (after panel p0): the rest of the super-panel's own columns,
-//          rows from p1 down, with panel p0 only;
-//   mode 1 (p0 = P0, after its last panel): everything right of the
-//          super-panel with all of its columns at once.
-// The trailing block is read and written once per super-panel instead of
-// once per panel.  pref[] counts tiles per front.
 // Tile (ti, tj), tj <= ti, of a lower triangle of nt tile rows for index u
 // in band order: bands of g tile rows, each walked column by column (the g
 // tiles of a column in a row, then the triangle's part of the band).  The
@@ -754,6 +746,14 @@ __device__ __forceinline__ void band_index(int64_t u, int nt, int g, int &ti, in
     ti = tj + (int)v;
 }
 
+// Updates of a super-panel [P0, P0 + TF_SP) of TF_PB-column panels, one
+// 64 x 64 tile per wave:
+//   mode 0 (after panel p0): the rest of the super-panel's own columns,
+//          rows from p1 down, with panel p0 only;
+//   mode 1 (p0 = P0, after its last panel): everything right of the
+//          super-panel with all of its columns at once.
+// The trailing block is read and written once per super-panel instead of
+// once per panel.  pref[] counts tiles per front.
 __global__ void __launch_bounds__(TF_NT, 2) k_factor_update(double *L, double *U, const FrontDesc *fd,
                                                             const int *list, const int64_t *pref, int cnt,
                                                             int p0, int mode, const double *Dn, int band) {

@@ -9,7 +9,7 @@
 #define TF_NT 256      // threads per workgroup
 #define TF_TRSM_ROWS 128
 #define TF_UT 128      // tile edge of the matrix-core trailing update
-#define TF_SP 256      // super-panel: panel columns whose trailing update is applied at once
+#define TF_SP 512      // super-panel: panel columns whose trailing update is applied at once
 
 // Values allocated past the end of the factor storage and of the block-solve
 // vectors: the tile kernels' unmasked stage loads may read up to 63 values
