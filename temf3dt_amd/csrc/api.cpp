@@ -47,8 +47,6 @@ void launch_scatter_p(int64_t nnz, const int64_t *amap, const double *M, const d
 void launch_scatter_vals(int64_t nnz, const int64_t *amap, const double *va, double *L, hipStream_t st);
 void launch_extend_add(double *L, double *U, const FrontDesc *fd, const int *parent, const int *list,
                        const int64_t *pref, int cnt, int64_t nwg, const int *rel, hipStream_t st);
-void launch_factor_fused(double *L, double *U, const FrontDesc *fd, const int *list, int cnt, double *Dn, int *flag,
-                         hipStream_t st);
 void launch_factor_diag(double *L, const FrontDesc *fd, const int *list, int cnt, int p0, double *Dn, int *flag,
                         double *W, const int64_t *woff, hipStream_t st);
 void launch_factor_trsm(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
@@ -178,7 +176,7 @@ void dfree(T *&p) {
 }
 
 enum LaunchKind {
-    L_EXTADD, L_FFUSED, L_FDIAG, L_FTRSM, L_FUPD, L_IMERGE_T, L_IMERGE_X, L_IDIAG,
+    L_EXTADD, L_FDIAG, L_FTRSM, L_FUPD, L_IMERGE_T, L_IMERGE_X, L_IDIAG,
     L_SSMALL, L_SOWN, L_SSTRUCT,
     L_BSMALL, L_BT, L_BX,
     L_GINIT, L_GCHILD, L_GXS, L_MT, L_MTRED, L_MPULL,
@@ -198,7 +196,7 @@ int prof_category(int kind) {
     case L_MTRED: return TF_PROF_TILE_RED;
     case L_MPULL: return TF_PROF_PULL;
     case L_FUPD: return TF_PROF_FACTOR_UPDATE;
-    case L_EXTADD: case L_FFUSED: case L_FDIAG: case L_FTRSM: case L_IMERGE_T: case L_IMERGE_X: case L_IDIAG:
+    case L_EXTADD: case L_FDIAG: case L_FTRSM: case L_IMERGE_T: case L_IMERGE_X: case L_IDIAG:
     case L_PINV:
         return TF_PROF_FACTOR_OTHER;
     default: return TF_PROF_SOLVE1;
@@ -608,7 +606,8 @@ void build_plan(tf_ctx *c) {
             add(dst, L_GCHILD, 0, chl, uc);
         }
     };
-    // small fronts (one workgroup each) vs large fronts (grid launches)
+    // the single-RHS matrix-vector solve (s_tiles 0) takes small fronts one
+    // workgroup each and large fronts in grid launches
     std::vector<std::vector<int>> small(s.nlevels), grid(s.nlevels);
     for (int l = 0; l < s.nlevels; l++)
         for (int q = s.lev_ptr[l]; q < s.lev_ptr[l + 1]; q++) {
@@ -617,8 +616,12 @@ void build_plan(tf_ctx *c) {
             else grid[l].push_back(k);
         }
     p.woff.assign(s.nsn, 0);
-    // factorisation
+    // factorisation: every front of a level in the same grid launches (the
+    // fronts of at most 64 columns too: one workgroup per such front, with
+    // LDS loops and a barrier per pivot, made the factorisation of the
+    // batched mesh 10 ms slower)
     for (int l = 0; l < s.nlevels; l++) {
+        const std::vector<int> lev(s.lev_sn.begin() + s.lev_ptr[l], s.lev_sn.begin() + s.lev_ptr[l + 1]);
         int maxch = 0;
         for (int q = s.lev_ptr[l]; q < s.lev_ptr[l + 1]; q++) {
             int k = s.lev_sn[q];
@@ -636,10 +639,9 @@ void build_plan(tf_ctx *c) {
             }
             add(p.factor, L_EXTADD, 0, fr, un);
         }
-        add(p.factor, L_FFUSED, 0, small[l], {});
         int kmax = 0;
         int64_t wlev = 0;
-        for (int k : grid[l]) {
+        for (int k : lev) {
             kmax = std::max(kmax, (s.nc[k] + TF_PB - 1) / TF_PB);
             // scratch: the pivot block's inverse during the panels (64 x 64),
             // then the largest merge level's T blocks of the inversion
@@ -660,7 +662,7 @@ void build_plan(tf_ctx *c) {
             int p0 = pk * TF_PB, P0 = p0 - p0 % TF_SP;
             std::vector<int> act, a0, a1;
             std::vector<int64_t> ut, u0, u1;
-            for (int k : grid[l])
+            for (int k : lev)
                 if (s.nc[k] > p0) {
                     int p1 = std::min(s.nc[k], p0 + TF_PB), pe = std::min(s.nc[k], P0 + TF_SP);
                     int rows = f_of(k) - p1;
@@ -684,18 +686,18 @@ void build_plan(tf_ctx *c) {
                 p.factor.back().aux = 1;
             }
         }
-        // in-place inversion of every large front's L11 by doubling: all 64 x 64
+        // in-place inversion of every front's L11 by doubling: all 64 x 64
         // diagonal blocks, then merges of block pairs of size b = 64, 128, ...
         {
             std::vector<int64_t> ud;
-            for (int k : grid[l]) ud.push_back((s.nc[k] + TF_PB - 1) / TF_PB);
-            add(p.factor, L_IDIAG, 0, grid[l], ud);
+            for (int k : lev) ud.push_back((s.nc[k] + TF_PB - 1) / TF_PB);
+            add(p.factor, L_IDIAG, 0, lev, ud);
         }
         for (int b = TF_PB;; b *= 2) {
             std::vector<int> act;
             std::vector<int64_t> um;
             const int64_t ntb = (b + TF_UT - 1) / TF_UT;
-            for (int k : grid[l])
+            for (int k : lev)
                 if (s.nc[k] > b) {
                     act.push_back(k);
                     um.push_back((int64_t)((s.nc[k] - b + 2 * b - 1) / (2 * b)) * ntb * ntb);
@@ -1102,9 +1104,6 @@ void run_factor(tf_ctx *c, double *L, double *Dn, const FrontDesc *fd) {
         case L_EXTADD:
             tf::launch_extend_add(L, c->d_U, fd, c->d_parent, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->d_rel,
                                   c->st);
-            break;
-        case L_FFUSED:
-            tf::launch_factor_fused(L, c->d_U, fd, lst + l.off, l.cnt, Dn, &c->d_flag->bad, c->st);
             break;
         case L_FDIAG:
             tf::launch_factor_diag(L, fd, lst + l.off, l.cnt, l.p0, Dn, &c->d_flag->bad, c->d_W, c->d_woff, c->st);
@@ -2826,12 +2825,16 @@ int tf_set_tuning(tf_ctx *c, const char *key, int value) {
         }
         if (replan && c->analyzed) {  // the tile units and partial buffer follow the plan
             HIPCHK(hipStreamSynchronize(c->st));
+            HIPCHK(hipStreamSynchronize(c->st2));
             build_plan(c);
             dfree(c->d_plist); dfree(c->d_ppref); dfree(c->d_units); dfree(c->d_part);
             c->d_plist = dupload(c->plan.list);
             c->d_ppref = dupload(c->plan.pref);
             c->d_units = dupload(c->plan.units);
             c->d_part = dalloc<double>(part_doubles(c->plan));
+            dfree(c->d_W); dfree(c->d_woff);  // the panel-inverse scratch follows the small/grid split
+            c->d_W = dalloc<double>((size_t)std::max<int64_t>(c->plan.wsize, 1));
+            c->d_woff = dupload(c->plan.woff);
         }
         return TF_OK;
     });

@@ -7,145 +7,15 @@
 // raise a flag that the host turns into an error.
 //
 // Work is organised by levels of the supernode tree (all fronts of a level
-// are independent).  Levels with many fronts run one workgroup per front
-// ("fused"); levels with few, large fronts split every 64-column panel into
-// grid launches (diagonal-block factor, panel solve, trailing updates
-// grouped by super-panels), then invert L11 by recursive doubling.
+// are independent).  Every 64-column panel of a level's fronts is one set of
+// grid launches (pivot block, panel solve, trailing updates grouped by
+// super-panels); then L11 is inverted by recursive doubling.
 #include "tf_device.h"
 
 #define LDS_LD 65
 
 
 // ---- workgroup-level building blocks ------------------------------------
-
-// Factor the pw x pw diagonal block of panel p0 in LDS: A = L D L^T with
-// unit-lower L; writes L (strict lower), d on the diagonal and into D.
-__device__ void blk_diag_factor(double *L, const FrontDesc &d, int p0, double *Dn, int *flag,
-                                double *lds) {
-    const int pw = min(TF_PB, d.nc - p0);
-    double *base = L + d.loff;
-    for (int idx = threadIdx.x; idx < pw * pw; idx += blockDim.x) {
-        int i = idx % pw, j = idx / pw;
-        lds[j * LDS_LD + i] = (i >= j) ? base[(int64_t)(p0 + j) * d.ld + p0 + i] : 0.0;
-    }
-    __syncthreads();
-    for (int j = 0; j < pw; j++) {
-        double dj = lds[j * LDS_LD + j];
-        if (!(fabs(dj) > 1e-300) || !isfinite(dj)) dj = (dj < 0.0) ? -1e-300 : 1e-300;
-        const int nt = pw - j - 1;
-        for (int idx = threadIdx.x; idx < nt * nt; idx += blockDim.x) {
-            int i = j + 1 + idx % nt, k = j + 1 + idx / nt;
-            if (i >= k) lds[k * LDS_LD + i] -= (lds[j * LDS_LD + i] / dj) * lds[j * LDS_LD + k];
-        }
-        __syncthreads();
-        for (int i = j + 1 + threadIdx.x; i < pw; i += blockDim.x) lds[j * LDS_LD + i] /= dj;
-        __syncthreads();
-    }
-    for (int idx = threadIdx.x; idx < pw * pw; idx += blockDim.x) {
-        int i = idx % pw, j = idx / pw;
-        if (i >= j) base[(int64_t)(p0 + j) * d.ld + p0 + i] = lds[j * LDS_LD + i];
-    }
-    for (int j = threadIdx.x; j < pw; j += blockDim.x) {
-        double dj = lds[j * LDS_LD + j];
-        Dn[d.doff + p0 + j] = dj;
-        if (!(fabs(dj) > 1e-300) || !isfinite(dj)) atomicOr(flag, 1);
-    }
-    __syncthreads();
-}
-
-// Panel solve for rows [r0, r0+TF_TRSM_ROWS): L21 = A21 L11^-T D^-1,
-// computed as (A21 Xpp^T) D^-1 with Xpp = L11^-1 inverted in LDS, 64 rows
-// at a time with 4x4 register tiles.
-__device__ void blk_invert_to_lds(const double *L, const FrontDesc &d, int p0, double *lds);
-
-__device__ void blk_trsm(double *L, const FrontDesc &d, int p0, int r0, const double *Dn, double *lds,
-                         const double *Xg = nullptr) {
-    const int pw = min(TF_PB, d.nc - p0);
-    double *base = L + d.loff;
-    double *X = lds;                     // Xpp [c*65 + r]; unit diagonal set below
-    double *At = lds + TF_PB * LDS_LD;   // [k][MLD rows]
-    if (Xg) {  // the inverse the pivot kernel left (column-major, ld 64)
-        for (int idx = threadIdx.x; idx < TF_PB * TF_PB; idx += blockDim.x)
-            X[(idx >> 6) * LDS_LD + (idx & 63)] = Xg[idx];
-        __syncthreads();
-    } else {
-        blk_invert_to_lds(L, d, p0, X);
-    }
-    for (int j = threadIdx.x; j < TF_PB; j += blockDim.x) X[j * LDS_LD + j] = (j < pw) ? 1.0 : 0.0;
-    __syncthreads();
-    const int kend = (pw + 3) & ~3;
-    for (int sb = 0; sb < TF_TRSM_ROWS; sb += TF_TB) {
-        const int R0 = r0 + sb;
-        if (R0 >= d.f) break;
-#pragma unroll 8
-        for (int idx = threadIdx.x; idx < TF_PB * TF_TB; idx += TF_NT) {
-            int k = idx / TF_TB, r = idx % TF_TB;
-            At[k * MLD + r] = (k < pw && R0 + r < d.f) ? base[(int64_t)(p0 + k) * d.ld + R0 + r] : 0.0;
-        }
-        __syncthreads();
-        tf_d4 acc[2][2];
-        acc_zero(acc);
-        // out[row][c] = sum_k A21[row][k] Xpp[c][k];  Xpp[c][k] = X[k*65 + c]
-        mfma64_acc(At, MLD, X, LDS_LD, kend, acc);
-#pragma unroll
-        for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int c = ACC_COL(bj, r);
-                if (c >= pw) continue;
-                const double dc = Dn[d.doff + p0 + c];
-#pragma unroll
-                for (int bi = 0; bi < 2; bi++) {
-                    const int row = R0 + ACC_ROW(bi);
-                    if (row < d.f) base[(int64_t)(p0 + c) * d.ld + row] = acc[bi][bj][r] / dc;
-                }
-            }
-        __syncthreads();
-    }
-}
-
-// Trailing update of tile (ti, tj), ti >= tj, of the rows/columns after the
-// panel: C -= L_i D L_j^T on the lower triangle.
-__device__ void blk_update_tile(double *L, double *U, const FrontDesc &d, int p0, const double *Dn,
-                                int ti, int tj, double *lds) {
-    const int pw = min(TF_PB, d.nc - p0);
-    const int p1 = p0 + pw;
-    const int rb = p1 + ti * TF_TB, cb = p1 + tj * TF_TB;
-    const double *base = L + d.loff;
-    double *A = lds;               // [k][MLD]: panel rows of the tile rows
-    double *B = lds + TF_PB * MLD; // [k][MLD]: panel rows of the tile columns, times d_k
-#pragma unroll 8
-    for (int idx = threadIdx.x; idx < TF_PB * TF_TB; idx += TF_NT) {
-        int k = idx / TF_TB, i = idx % TF_TB;
-        double va = 0.0, vb = 0.0;
-        if (k < pw) {
-            const double *col = base + (int64_t)(p0 + k) * d.ld;
-            if (rb + i < d.f) va = col[rb + i];
-            if (cb + i < d.f) vb = col[cb + i] * Dn[d.doff + p0 + k];
-        }
-        A[k * MLD + i] = va;
-        B[k * MLD + i] = vb;
-    }
-    __syncthreads();
-    tf_d4 acc[2][2];
-    acc_zero(acc);
-    mfma64_acc(A, MLD, B, MLD, (pw + 3) & ~3, acc);
-#pragma unroll
-    for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int c = cb + ACC_COL(bj, r);
-            if (c >= d.f) continue;
-            double *col = front_col(L, U, d, c);
-#pragma unroll
-            for (int bi = 0; bi < 2; bi++) {
-                const int row = rb + ACC_ROW(bi);
-                if (row < d.f && row >= c) col[row] -= acc[bi][bj][r];
-            }
-        }
-    __syncthreads();
-}
-
 
 // Invert the unit-lower pw x pw diagonal block of panel p0 into LDS
 // (column-major, ld 65, strict lower part; the diagonal is implicit).
@@ -571,38 +441,39 @@ __global__ void k_scatter_vals(int64_t nnz, const int64_t *amap, const double *v
 
 // Extend-add of child update blocks into their parents' fronts; one
 // workgroup per (child, column); at most one child per parent per launch.
+// Extend-add of the children's update stacks into their parents, one wave
+// per child column b: rows a >= b of the column go to the parent's column
+// rel[b], rows rel[a].  pref[] counts columns per child; four loads of each
+// kind are issued before their adds.
 __global__ void __launch_bounds__(TF_NT) k_extend_add(double *L, double *U, const FrontDesc *fd,
                                                       const int *parent, const int *list,
                                                       const int64_t *pref, int cnt, const int *rel) {
-    const int i = seg_find(pref, cnt, blockIdx.x);
+    const int64_t g = (int64_t)blockIdx.x * (TF_NT / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (g >= pref[cnt]) return;
+    const int lane = threadIdx.x & 63;
+    const int i = seg_find(pref, cnt, g);
     const int c = list[i];
-    const int b = (int)(blockIdx.x - pref[i]);
+    const int b = (int)(g - pref[i]);
     const FrontDesc dc = fd[c];
     const FrontDesc dp = fd[parent[c]];
     const double *uc = U + dc.uoff + (int64_t)b * dc.m;
     const int *rc = rel + dc.soff;
     double *dst = front_col(L, U, dp, rc[b]);
-    for (int a = b + threadIdx.x; a < dc.m; a += blockDim.x) dst[rc[a]] += uc[a];
-}
-
-// One workgroup per small front (nc <= 64): factor, update, invert L11.
-__global__ void __launch_bounds__(TF_NT) k_factor_fused(double *L, double *U, const FrontDesc *fd,
-                                                        const int *list, double *Dn, int *flag) {
-    __shared__ double lds[2 * TF_PB * MLD + 64];
-    const FrontDesc d = fd[list[blockIdx.x]];
-    for (int p0 = 0; p0 < d.nc; p0 += TF_PB) {
-        blk_diag_factor(L, d, p0, Dn, flag, lds);
-        const int p1 = min(d.nc, p0 + TF_PB);
-        for (int r0 = p1; r0 < d.f; r0 += TF_TRSM_ROWS) blk_trsm(L, d, p0, r0, Dn, lds);
-        const int nt = (d.f - p1 + TF_TB - 1) / TF_TB;
-        const int ntile = nt * (nt + 1) / 2;
-        for (int t = 0; t < ntile; t++) {
-            int ti, tj;
-            tri_index(t, ti, tj);
-            blk_update_tile(L, U, d, p0, Dn, ti, tj, lds);
+    int a = b + lane;
+    for (; a + 192 < dc.m; a += 256) {
+        int r[4];
+        double u[4], v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            r[q] = rc[a + 64 * q];
+            u[q] = uc[a + 64 * q];
         }
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = dst[r[q]];
+#pragma unroll
+        for (int q = 0; q < 4; q++) dst[r[q]] = v[q] + u[q];
     }
-    blk_invert_diag(L, d, 0, lds);
+    for (; a < dc.m; a += 64) dst[rc[a]] += uc[a];
 }
 
 // One wave per front: pivot block of panel p0 and its inverse (for the panel
@@ -976,13 +847,9 @@ void launch_scatter_vals(int64_t nnz, const int64_t *amap, const double *va, dou
 void launch_extend_add(double *L, double *U, const FrontDesc *fd, const int *parent, const int *list,
                        const int64_t *pref, int cnt, int64_t nwg, const int *rel, hipStream_t st) {
     if (nwg <= 0) return;
-    hipLaunchKernelGGL(k_extend_add, dim3((unsigned)nwg), dim3(TF_NT), 0, st, L, U, fd, parent, list,
-                       pref, cnt, rel);
-}
-void launch_factor_fused(double *L, double *U, const FrontDesc *fd, const int *list, int cnt, double *Dn,
-                         int *flag, hipStream_t st) {
-    if (cnt <= 0) return;
-    hipLaunchKernelGGL(k_factor_fused, dim3(cnt), dim3(TF_NT), 0, st, L, U, fd, list, Dn, flag);
+    // nwg counts child columns: one wave each
+    hipLaunchKernelGGL(k_extend_add, dim3((unsigned)((nwg + TF_NT / 64 - 1) / (TF_NT / 64))), dim3(TF_NT), 0, st, L,
+                       U, fd, parent, list, pref, cnt, rel);
 }
 void launch_factor_diag(double *L, const FrontDesc *fd, const int *list, int cnt, int p0, double *Dn, int *flag,
                         double *W, const int64_t *woff, hipStream_t st) {
