@@ -37,7 +37,7 @@ void launch_slice(int nT, const double *xyz, const int *tet_vex, const int *tet_
 // factor.hip
 void launch_fill0(double *p, int64_t n, hipStream_t st);
 void launch_pinv_w21(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
-                     hipStream_t st);
+                     int cb, hipStream_t st);
 void launch_flags_init(FactorFlags *f, hipStream_t st);
 void launch_pivot_scan(const double *Dn, const FrontDesc *fd, int nsn, FactorFlags *f, hipStream_t st);
 void launch_transpose_l21(const double *L, double *LT, const FrontDesc *fd, const int *units, int64_t nunits,
@@ -706,14 +706,17 @@ void build_plan(tf_ctx *c) {
             add(p.factor, L_IMERGE_T, b, act, um);
             add(p.factor, L_IMERGE_X, b, act, um);
         }
-        {  // partitioned inverse of the level's fronts that keep it (after their inversion)
+        // partitioned inverse of the level's fronts that keep it (after their
+        // inversion): one launch per 64-column block, left to right
+        for (int cb = 0;; cb += 64) {
             std::vector<int> pv;
             std::vector<int64_t> up;
             for (int q = s.lev_ptr[l]; q < s.lev_ptr[l + 1]; q++) {
                 const int k = s.lev_sn[q];
-                if (pinv_of(k) && m_of(k) > 0) { pv.push_back(k); up.push_back((m_of(k) + 63) / 64); }
+                if (pinv_of(k) && m_of(k) > 0 && s.nc[k] > cb) { pv.push_back(k); up.push_back((m_of(k) + 63) / 64); }
             }
-            add(p.factor, L_PINV, 0, pv, up);
+            if (pv.empty()) break;
+            add(p.factor, L_PINV, cb, pv, up);
         }
     }
     // forward solve, bottom-up
@@ -1125,7 +1128,7 @@ void run_factor(tf_ctx *c, double *L, double *Dn, const FrontDesc *fd) {
             tf::launch_inv_diag(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->st);
             break;
         case L_PINV:
-            tf::launch_pinv_w21(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, c->st);
+            tf::launch_pinv_w21(L, fd, lst + l.off, pr + l.poff, l.cnt, l.nwg, l.p0, c->st);
             break;
         }
         prof_close(c, prof_category(l.kind), pm);

@@ -297,74 +297,6 @@ __global__ void __launch_bounds__(256) k_transpose_l21(const double *L, double *
         if (r0 + r < rows && c0 + x < d.ldt) dst[(int64_t)r * d.ldt + x] = t[x][r];
 }
 
-// Partitioned inverse of a front (fronts with pinv set, after their
-// factorisation and the in-place inversion of L11): the L21 rows become
-// W = -L21 L11^-1 = -(L21 + L21 strict(L11^-1)), so that the solve's forward
-// product of the front is one product of the whole block with the front's
-// right-hand-side rows ([Y_own - Yin; -Z] = [strict(L11^-1); W] Yin) and
-// its backward product one product with [Y_own / d; x(struct)].
-// One workgroup per (front, 64-row slab of L21); column blocks of 64 left to
-// right, in place: block cb reads L21 columns >= 64 cb only, which no earlier
-// block has overwritten.
-__global__ void __launch_bounds__(256) k_pinv_w21(double *L, const FrontDesc *fd, const int *list,
-                                                  const int64_t *pref, int cnt) {
-    constexpr int KC = 32;  // inner rows per chunk; the next chunk is loaded into registers during the products
-    __shared__ double Ap[KC * MLD];  // [k][row]: L21 rows of the slab, KC columns
-    __shared__ double Bp[KC * MLD];  // [k][col]: strict(L11^-1) rows j0 .. j0 + KC - 1, 64 columns
-    const int i = seg_find(pref, cnt, blockIdx.x);
-    const FrontDesc d = fd[list[i]];
-    const int r0 = (int)(blockIdx.x - pref[i]) * 64;
-    double *Lf = L + d.loff;
-    const int nc = d.nc, m = d.m, ld = d.ld;
-    constexpr int PER = KC * 64 / 256;  // elements of each panel per thread
-    double ra[PER], rb[PER];
-    auto gload = [&](int cb, int j0) {
-#pragma unroll
-        for (int q = 0; q < PER; q++) {
-            const int e = threadIdx.x + 256 * q;
-            const int k = e >> 6, row = e & 63;  // consecutive threads: consecutive rows of one column
-            ra[q] = (j0 + k < nc && r0 + row < m) ? Lf[(int64_t)(j0 + k) * ld + nc + r0 + row] : 0.0;
-            const int col = e / KC, kk = e % KC;  // KC consecutive rows of one column
-            rb[q] = (j0 + kk < nc && cb + col < nc) ? Lf[(int64_t)(cb + col) * ld + j0 + kk] : 0.0;
-        }
-    };
-    auto lstore = [&]() {
-#pragma unroll
-        for (int q = 0; q < PER; q++) {
-            const int e = threadIdx.x + 256 * q;
-            Ap[(e >> 6) * MLD + (e & 63)] = ra[q];
-            Bp[(e % KC) * MLD + e / KC] = rb[q];
-        }
-    };
-    for (int cb = 0; cb < nc; cb += 64) {
-        tf_d4 acc[2][2];
-        acc_zero(acc);
-        // strict(L11^-1)[j][col] is zero for j <= col: columns of the block
-        // take rows j >= cb only
-        gload(cb, cb);
-        for (int j0 = cb; j0 < nc; j0 += KC) {
-            lstore();
-            __syncthreads();
-            if (j0 + KC < nc) gload(cb, j0 + KC);
-            mfma64_acc(Ap, MLD, Bp, MLD, KC, acc);
-            __syncthreads();
-        }
-#pragma unroll
-        for (int bi = 0; bi < 2; bi++)
-#pragma unroll
-            for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int row = r0 + ACC_ROW(bi), col = cb + ACC_COL(bj, r);
-                    if (row < m && col < nc) {
-                        double *p = Lf + (int64_t)col * ld + nc + row;
-                        *p = -(*p + acc[bi][bj][r]);
-                    }
-                }
-        __syncthreads();
-    }
-}
-
 // p[0 .. n) = 0 with 16-byte stores (p itself may be 8-byte aligned only:
 // the leading and trailing odd elements are written separately)
 __global__ void k_fill0(double *p, int64_t n) {
@@ -810,6 +742,54 @@ __global__ void __launch_bounds__(TF_NT, 2) k_inv_merge(double *L, const FrontDe
         }
 }
 
+// Partitioned inverse of a front (fronts with pinv set, after their
+// factorisation and the in-place inversion of L11): the L21 rows become
+// W21 = -L21 X11, X11 = L11^-1 (unit lower; its strict part stored in L11's
+// place, whose diagonal and upper part read as zero), so that the solve's
+// forward product of the front is one product of the whole block with the
+// front's right-hand-side rows ([Y_own - Yin; -Z] = [strict(X11); W21] Yin)
+// and its backward product one product with [Y_own / d; x(struct)].
+// One launch per 64-column block cb, left to right; one wave per 64-row
+// slab of L21: the block is the product of the slab's columns j >= cb with
+// X11's rows j >= cb (X11[j][c] = 0 for j < c; its unit diagonal takes L21
+// itself into the product) and overwrites L21's columns [cb, cb + 64),
+// which no later launch reads.
+__global__ void __launch_bounds__(TF_NT, 2) k_pinv_w21(double *L, const FrontDesc *fd, const int *list,
+                                                       const int64_t *pref, int cnt, int cb) {
+    const int64_t g = (int64_t)blockIdx.x * (TF_NT / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (g >= pref[cnt]) return;
+    const int i = seg_find(pref, cnt, g);
+    const FrontDesc d = fd[list[i]];
+    const int r0 = (int)(g - pref[i]) * 64;
+    double *base = L + d.loff;
+    const int nc = d.nc, m = d.m, ld = d.ld;
+    const int64_t lend = (int64_t)nc * ld;  // the front's L block
+    tf_d4 acc[4][4];
+    inv_wave<true, false>(base + nc, ld, (uint32_t)((lend - nc) * 8), base, ld, (uint32_t)(lend * 8), r0, cb, cb,
+                          nc, acc);
+    typedef double i_d2 __attribute__((ext_vector_type(2)));
+    const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int c = cb + 16 * a + 4 * q + lk;
+            if (c >= nc) continue;
+            double *col = base + (int64_t)c * ld + nc;
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                const int r = r0 + 32 * p + 2 * l16;
+                const i_d2 v = (i_d2){-acc[a][2 * p][q], -acc[a][2 * p + 1][q]};
+                if (r + 1 < m && ((reinterpret_cast<uintptr_t>(col + r) & 15) == 0)) {
+                    *(i_d2 *)(col + r) = v;
+                } else {
+                    if (r < m) col[r] = v.x;
+                    if (r + 1 < m) col[r + 1] = v.y;
+                }
+            }
+        }
+}
+
 // ---- host launchers --------------------------------------------------------
 namespace tf {
 
@@ -818,8 +798,11 @@ void launch_transpose_l21(const double *L, double *LT, const FrontDesc *fd, cons
     if (nunits > 0) hipLaunchKernelGGL(k_transpose_l21, dim3((unsigned)nunits), dim3(256), 0, st, L, LT, fd, units);
 }
 void launch_pinv_w21(double *L, const FrontDesc *fd, const int *list, const int64_t *pref, int cnt, int64_t nwg,
-                     hipStream_t st) {
-    if (nwg > 0) hipLaunchKernelGGL(k_pinv_w21, dim3((unsigned)nwg), dim3(256), 0, st, L, fd, list, pref, cnt);
+                     int cb, hipStream_t st) {
+    // nwg counts 64-row slabs: one wave each
+    if (nwg > 0)
+        hipLaunchKernelGGL(k_pinv_w21, dim3((unsigned)((nwg + TF_NT / 64 - 1) / (TF_NT / 64))), dim3(TF_NT), 0, st,
+                           L, fd, list, pref, cnt, cb);
 }
 void launch_fill0(double *p, int64_t n, hipStream_t st) {
     if (n <= 0) return;

@@ -122,32 +122,3 @@ __device__ __forceinline__ void mt_bld2(__amdgpu_buffer_rsrc_t r, uint32_t voff,
     x1 = __builtin_bit_cast(double, (mt_u2){v.z, v.w});
 }
 
-#define MLD 80  // LDS leading dimension of 64-wide operand panels (bank-conflict padding)
-
-// 64 x 64 tile product on FP64 matrix cores, four waves of 32 x 32:
-//   acc(row, col) += sum_k Rop[k*ldr + row] * Cop[k*ldc + col],  k < kend
-// (kend a multiple of 4, operands zero-padded).  v_mfma_f64_16x16x4f64
-// lane layout (measured): lane l supplies A[l%16][l/16], B[l/16][l%16] and
-// receives D[4r + l/16][l%16] in result register r.  The column operand is
-// passed as A so that a result register's fast index is the row:
-//   acc[bi][bj][r] -> row 32*wr + 16*bi + (l & 15), col 32*wc + 16*bj + 4r + (l >> 4).
-__device__ __forceinline__ void mfma64_acc(const double *Rop, int ldr, const double *Cop, int ldc, int kend,
-                                           tf_d4 acc[2][2]) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1;
-    const int l16 = lane & 15, lk = lane >> 4;
-    for (int k0 = 0; k0 < kend; k0 += 4) {
-        const int kk = k0 + lk;
-        const double x0 = Cop[kk * ldc + 32 * wc + l16], x1 = Cop[kk * ldc + 32 * wc + 16 + l16];
-        const double y0 = Rop[kk * ldr + 32 * wr + l16], y1 = Rop[kk * ldr + 32 * wr + 16 + l16];
-        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, y0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, y0, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, y1, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, y1, acc[1][1], 0, 0, 0);
-    }
-}
-__device__ __forceinline__ void acc_zero(tf_d4 acc[2][2]) {
-    for (int i = 0; i < 2; i++)
-        for (int j = 0; j < 2; j++) acc[i][j] = (tf_d4){0.0, 0.0, 0.0, 0.0};
-}
-#define ACC_ROW(bi) (32 * ((threadIdx.x >> 6) >> 1) + 16 * (bi) + (threadIdx.x & 15))
-#define ACC_COL(bj, r) (32 * ((threadIdx.x >> 6) & 1) + 16 * (bj) + 4 * (r) + ((threadIdx.x & 63) >> 4))
