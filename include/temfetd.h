@@ -124,6 +124,11 @@ int tf_analyze(tf_ctx *ctx, int leaf_size);
  *               source count, no checkpointing) form Msigma (4 e2 - e3) as
  *               4 Msigma e2 - Msigma e3, the second product kept from the step
  *               before; 0: the history combination first.  Equal to rounding.
+ *   "keep_factors" a taped tf_run keeps up to this many of the factors
+ *               it would overwrite (besides the two slots) while a tenth of
+ *               the device memory stays free, and tf_adjoint uses them
+ *               instead of factoring P(dt) again (default 8; 0: none).
+ *               Bit-identical.
  *   "f_band"    the factorisation's trailing-update tiles are walked in
  *               bands of this many 64-row tile rows, column by column inside
  *               a band, so that the tiles running at once share their operand

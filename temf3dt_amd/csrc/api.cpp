@@ -405,6 +405,16 @@ struct tf_ctx {
     int *d_tet_edg = nullptr;
     double *d_vm1 = nullptr;
     double slot_dt[2] = {-1.0, -1.0};  // dt a slot was factored for (-1: none / caller matrix)
+    // factors of a taped run kept for its adjoint sweep (tuning key
+    // "keep_factors": at most this many besides the two slots; 0: none)
+    struct KeptFactor {
+        double dt;
+        double *L, *D, *LT;
+    };
+    std::vector<KeptFactor> kept;
+    int keep_factors = 8;
+    bool slot_taped[2] = {false, false};  // the slot was factored during the current taped run
+    bool taping = false;                   // a taped tf_run is in progress
     // independent models in one context (tf_set_models): block-diagonal
     // system, one tree of fronts per model, per-model controller state
     std::vector<FrontDesc> fd_host;       // front descriptors, offsets of slot 0
@@ -438,6 +448,7 @@ struct tf_ctx {
 namespace {
 
 void free_models(tf_ctx *c);
+void drop_kept(tf_ctx *c);
 
 void free_solver(tf_ctx *c) {
     free_models(c);
@@ -445,6 +456,7 @@ void free_solver(tf_ctx *c) {
     dfree(c->d_sidx); dfree(c->d_plist); dfree(c->d_amap); dfree(c->d_ppref); dfree(c->d_units); dfree(c->d_gptr); dfree(c->d_gidx); dfree(c->d_U); dfree(c->d_V);
     dfree(c->d_xw); dfree(c->d_Y); dfree(c->d_T); dfree(c->d_W); dfree(c->d_woff);
     for (int s = 0; s < 2; s++) { dfree(c->d_L[s]); dfree(c->d_D[s]); dfree(c->d_LT[s]); c->slot_live[s] = false; }
+    drop_kept(c);
     dfree(c->d_tunits);
     dfree(c->d_frows);
     c->n_tunits = 0;
@@ -469,7 +481,13 @@ void free_models(tf_ctx *c) {
     c->models_factored = false;
 }
 
+void drop_kept(tf_ctx *c) {
+    for (auto &k : c->kept) { dfree(k.L); dfree(k.D); dfree(k.LT); }
+    c->kept.clear();
+}
+
 void free_tape(tf_ctx *c) {
+    drop_kept(c);
     for (auto &k : c->ckpts) { dfree(k.s2); dfree(k.s3); }
     c->ckpts.clear();
     c->tape.clear();
@@ -1175,7 +1193,26 @@ void check_flag(tf_ctx *c) {
                            "PARDISO mtype -2; set tuning key \"indefinite\" to 1 to accept the unpivoted factor)");
 }
 
+// A taped run's factor about to be overwritten is kept for the adjoint
+// sweep instead, when the cap and the device's free memory allow (a tenth
+// of the device stays free): its buffers move to the kept list, no copy, and
+// the slot is given new ones.
+void keep_slot(tf_ctx *c, int slot) {
+    if ((int)c->kept.size() >= c->keep_factors || !c->d_L[slot] || !c->d_D[slot]) return;
+    for (const auto &k : c->kept)
+        if (k.dt == c->slot_dt[slot]) return;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return;
+    const size_t need = sizeof(double) * ((size_t)c->sym.ltotal + TF_READ_SLACK + (size_t)c->sym.n +
+                                          (c->d_LT[slot] ? (size_t)c->lt_total : 0));
+    if (fr < need + tot / 10) return;
+    c->kept.push_back(tf_ctx::KeptFactor{c->slot_dt[slot], c->d_L[slot], c->d_D[slot], c->d_LT[slot]});
+    c->d_L[slot] = c->d_D[slot] = c->d_LT[slot] = nullptr;
+}
+
 void factor_dt(tf_ctx *c, int slot, double dt) {
+    if (c->taping && c->slot_taped[slot]) keep_slot(c, slot);
+    c->slot_taped[slot] = c->taping;
     prepare_slot(c, slot);
     tf::launch_scatter_p(c->topo.nnz, c->d_amap, c->d_M, c->d_K, 2.0 * dt, c->d_L[slot], c->st);
     run_factor(c, c->d_L[slot], c->d_D[slot], c->d_fd);
@@ -1695,8 +1732,15 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
     if (outp) prepare_slices(c);
     // factors from before this run may be of other materials: none is reused
     c->slot_live[0] = c->slot_live[1] = false;
+    c->slot_taped[0] = c->slot_taped[1] = false;
+    drop_kept(c);
     // adjoint tape: node ids of the history states e2, e3 and of the ring
     const bool tape = c->ckpt_every > 0 && inout == 0;
+    c->taping = tape && c->keep_factors > 0;
+    struct TapingOff {
+        tf_ctx *c;
+        ~TapingOff() { c->taping = false; }
+    } taping_off{c};
     int id2 = 0, id3 = 0;
     std::vector<int> ring_id(ring, 0);
     if (tape) {
@@ -2271,6 +2315,11 @@ void run_adjoint(tf_ctx *c, const double *w, double *grad_host) {
         hipStream_t s;
         ~StreamDrain() { (void)hipStreamSynchronize(s); }
     } drain{c->st2};
+    auto kept_index = [&](double dt) {
+        for (size_t q = 0; q < c->kept.size(); q++)
+            if (c->kept[q].dt == dt) return (int)q;
+        return -1;
+    };
     auto slot_for = [&](double dt) {
         if (pos < dts.size() && dts[pos] == dt) pos++;
         int s = -1;
@@ -2280,6 +2329,27 @@ void run_adjoint(tf_ctx *c, const double *w, double *grad_host) {
             HIPCHK(hipStreamWaitEvent(c->st, a2, 0));
             check_flag(c);
             pend = -1;
+        }
+        const int ki = kept_index(dt);
+        if (s < 0 && ki >= 0) {  // kept by the taped run: swapped into a slot, the slot's factor kept instead
+            if (pend >= 0) {
+                HIPCHK(hipStreamSynchronize(c->st2));
+                check_flag(c);
+                pend = -1;
+            }
+            s = 1 - last_used;
+            tf_ctx::KeptFactor &k = c->kept[ki];
+            std::swap(k.L, c->d_L[s]);
+            std::swap(k.D, c->d_D[s]);
+            std::swap(k.LT, c->d_LT[s]);
+            if (c->slot_live[s] && k.L && k.D && kept_index(c->slot_dt[s]) < 0) {
+                k.dt = c->slot_dt[s];
+            } else {
+                dfree(k.L); dfree(k.D); dfree(k.LT);
+                c->kept.erase(c->kept.begin() + ki);
+            }
+            c->slot_live[s] = true;
+            c->slot_dt[s] = dt;
         }
         if (s < 0) {
             if (pend >= 0) {  // not the dt it was factored for
@@ -2298,7 +2368,7 @@ void run_adjoint(tf_ctx *c, const double *w, double *grad_host) {
             size_t j = pos;
             while (j < dts.size() && dts[j] == dt) j++;
             const int o = 1 - s;
-            if (j < dts.size() && !(c->slot_live[o] && c->slot_dt[o] == dts[j])) {
+            if (j < dts.size() && !(c->slot_live[o] && c->slot_dt[o] == dts[j]) && kept_index(dts[j]) < 0) {
                 HIPCHK(hipEventRecord(a0, c->st));
                 HIPCHK(hipStreamWaitEvent(c->st2, a0, 0));
                 std::swap(c->st, c->st2);
@@ -2775,6 +2845,10 @@ int tf_set_tuning(tf_ctx *c, const char *key, int value) {
         } else if (k == "f_band") {
             if (value < 0 || value > 1024) return fail(c, TF_ERR_ARG, "f_band must be in 0..1024");
             c->f_band = value;
+        } else if (k == "keep_factors") {
+            if (value < 0) return fail(c, TF_ERR_ARG, "keep_factors must be >= 0");
+            c->keep_factors = value;
+            if ((int)c->kept.size() > value) drop_kept(c);
         } else if (k == "rhs_chain") {
             if (value != 0 && value != 1) return fail(c, TF_ERR_ARG, "rhs_chain must be 0 or 1");
             c->rhs_chain = value != 0;

@@ -104,6 +104,29 @@ def test_adjoint_factoring_ahead_is_bit_identical(adj_setup):
     np.testing.assert_array_equal(g1, g0)
 
 
+def test_adjoint_with_kept_factors_is_bit_identical(adj_setup):
+    """The taped run keeps the factors it would overwrite, and the sweep
+    swaps them into its slots instead of factoring P(dt) again: the same
+    gradient, bit for bit, with fewer factorisations once the run used more
+    dt than the two slots hold."""
+    s = adj_setup
+    fe = s["fe"]
+    rows, g1 = _adjoint(s, 7)
+    n1 = fe.timers()["nfactor"]
+    fe.set_tuning("keep_factors", 0)
+    try:
+        _, g0 = _adjoint(s, 7)
+        n0 = fe.timers()["nfactor"]
+    finally:
+        fe.set_tuning("keep_factors", 8)
+    ndt = len(np.unique(rows["dt"]))
+    print(f"[gpu] kept factors: {ndt} dt in the run, factorisations {n1:.0f} (kept) vs {n0:.0f} (none kept)",
+          flush=True)
+    np.testing.assert_array_equal(g1, g0)
+    if ndt > 2:
+        assert n1 < n0
+
+
 def test_adjoint_matches_oracle_finite_differences(adj_setup, tmp_path):
     """Central differences through the oracle (the reference algorithm with
     MKL PARDISO on the CPU) on the same replayed schedule."""
