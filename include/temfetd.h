@@ -124,6 +124,11 @@ int tf_analyze(tf_ctx *ctx, int leaf_size);
  *               source count, no checkpointing) form Msigma (4 e2 - e3) as
  *               4 Msigma e2 - Msigma e3, the second product kept from the step
  *               before; 0: the history combination first.  Equal to rounding.
+ *   "keep_states" 1 (default): a taped tf_run keeps the solution of every
+ *               node while the kept states take at most a quarter of the
+ *               device memory and a fifth of it stays free, and tf_adjoint
+ *               reads them instead of recomputing them from the checkpoints;
+ *               0: every state is recomputed.  Bit-identical.
  *   "keep_factors" a taped tf_run keeps up to this many of the factors
  *               it would overwrite (besides the two slots) while a tenth of
  *               the device memory stays free, and tf_adjoint uses them

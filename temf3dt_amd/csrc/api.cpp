@@ -415,6 +415,15 @@ struct tf_ctx {
     int keep_factors = 8;
     bool slot_taped[2] = {false, false};  // the slot was factored during the current taped run
     bool taping = false;                   // a taped tf_run is in progress
+    // solutions of a taped run's nodes kept for its adjoint sweep, which then
+    // skips recomputing them (tuning key "keep_states": while the kept
+    // states take at most a quarter of the device memory and a fifth of it
+    // stays free; 0: none), in chunks of TF_STATE_CHUNK states
+    std::vector<double *> tstate;         // by node; nullptr: recomputed from the checkpoints
+    std::vector<double *> tstate_chunks;
+    int tstate_used = 0;                  // states taken from the last chunk
+    size_t tstate_bytes = 0;
+    bool keep_states = true;
     // independent models in one context (tf_set_models): block-diagonal
     // system, one tree of fronts per model, per-model controller state
     std::vector<FrontDesc> fd_host;       // front descriptors, offsets of slot 0
@@ -486,8 +495,17 @@ void drop_kept(tf_ctx *c) {
     c->kept.clear();
 }
 
+void drop_states(tf_ctx *c) {
+    for (auto &p : c->tstate_chunks) dfree(p);
+    c->tstate_chunks.clear();
+    c->tstate.clear();
+    c->tstate_used = 0;
+    c->tstate_bytes = 0;
+}
+
 void free_tape(tf_ctx *c) {
     drop_kept(c);
+    drop_states(c);
     for (auto &k : c->ckpts) { dfree(k.s2); dfree(k.s3); }
     c->ckpts.clear();
     c->tape.clear();
@@ -1552,6 +1570,26 @@ void dcopy(tf_ctx *c, double *dst, const double *src) {
     HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * c->topo.nE * c->nsrc, hipMemcpyDeviceToDevice, c->st));
 }
 
+#define TF_STATE_CHUNK 32
+// Keeps node k's solution x for the adjoint sweep when the budget allows.
+void keep_state(tf_ctx *c, int k, const double *x) {
+    if (!c->keep_states) return;
+    const size_t len = (size_t)c->topo.nE * c->nsrc;
+    if (c->tstate_chunks.empty() || c->tstate_used == TF_STATE_CHUNK) {
+        const size_t bytes = sizeof(double) * len * TF_STATE_CHUNK;
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return;
+        if (c->tstate_bytes + bytes > tot / 4 || fr < bytes + tot / 5) return;
+        c->tstate_chunks.push_back(dalloc<double>(len * TF_STATE_CHUNK));
+        c->tstate_used = 0;
+        c->tstate_bytes += bytes;
+    }
+    if ((int)c->tstate.size() <= k) c->tstate.resize(k + 1, nullptr);
+    double *p = c->tstate_chunks.back() + len * (size_t)c->tstate_used++;
+    dcopy(c, p, x);
+    c->tstate[k] = p;
+}
+
 // x2, if given, receives a copy of the solution
 void step_solve(tf_ctx *c, int slot, const double *b, double *x, double *x2 = nullptr) {
     if (c->nsrc == 1) {
@@ -1746,6 +1784,7 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
     if (tape) {
         for (auto &k : c->ckpts) { dfree(k.s2); dfree(k.s3); }
         c->ckpts.clear();
+        drop_states(c);
         c->tape.assign(1, tf_ctx::TNode{-1, -1, 0, 0, 0.0, 0.0});  // node 0: the zero initial state
         c->row_node.assign(1, 0);
     }
@@ -1835,6 +1874,7 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
                 step_solve(c, cur, v.rh, v.te1);
                 if (tape) {
                     fid[0] = node(fid[1], fid[2], step, tstep, ttime);
+                    keep_state(c, fid[0], v.te1);
                     if (i == 1) f1id = fid[0];
                     fid[2] = fid[1];
                     fid[1] = fid[0];
@@ -1931,6 +1971,7 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
         if (outp) output_row(c, step, time_cur, step_cur, v.e1);
         if (tape) {
             const int k = node(id2, id3, step, step_cur, time_cur);
+            keep_state(c, k, v.e1);
             row_of(k);
             id3 = id2;
             id2 = k;
@@ -2298,11 +2339,12 @@ void run_adjoint(tf_ctx *c, const double *w, double *grad_host) {
     // order (segments from the last; recompute forward, adjoint backward) is
     // known from the tape, so the next dt to come is factored on the second
     // stream while the current one is in use, as in tf_run.
+    auto stored = [&](int k) { return k < (int)c->tstate.size() && c->tstate[k] != nullptr; };
     std::vector<double> dts;
     for (int g = (int)c->ckpts.size() - 1; g >= 0; g--) {
         const int first = c->ckpts[g].first, end = (g + 1 < (int)c->ckpts.size()) ? c->ckpts[g + 1].first : nn;
         for (int k = first; k < end; k++)
-            if (live[k]) dts.push_back(c->tape[k].dt);
+            if (live[k] && !stored(k)) dts.push_back(c->tape[k].dt);
         for (int k = end - 1; k >= first; k--)
             if (live[k]) dts.push_back(c->tape[k].dt);
     }
@@ -2383,8 +2425,9 @@ void run_adjoint(tf_ctx *c, const double *w, double *grad_host) {
     };
     auto S = [&](int k) -> double * {
         if (k == 0) return zero;
-        if (!state[k]) throw std::runtime_error("adjoint: history state missing from the tape");
-        return state[k];
+        if (state[k]) return state[k];
+        if (stored(k)) return c->tstate[k];
+        throw std::runtime_error("adjoint: history state missing from the tape");
     };
     const int nseg = (int)c->ckpts.size();
     for (int g = nseg - 1; g >= 0; g--) {
@@ -2392,9 +2435,9 @@ void run_adjoint(tf_ctx *c, const double *w, double *grad_host) {
         const int first = ck.first, end = (g + 1 < nseg) ? c->ckpts[g + 1].first : nn;
         if (ck.id2 > 0) state[ck.id2] = ck.s2;
         if (ck.id3 > 0) state[ck.id3] = ck.s3;
-        // recompute the segment's live states
+        // recompute the segment's live states that the run did not keep
         for (int k = first; k < end; k++) {
-            if (!live[k]) continue;
+            if (!live[k] || stored(k)) continue;
             const tf_ctx::TNode &nd = c->tape[k];
             const int slot = slot_for(nd.dt);
             rhs(c, nd.step, nd.t, S(nd.u), S(nd.v), nd.dt, false, rh);
@@ -2845,6 +2888,9 @@ int tf_set_tuning(tf_ctx *c, const char *key, int value) {
         } else if (k == "f_band") {
             if (value < 0 || value > 1024) return fail(c, TF_ERR_ARG, "f_band must be in 0..1024");
             c->f_band = value;
+        } else if (k == "keep_states") {
+            if (value != 0 && value != 1) return fail(c, TF_ERR_ARG, "keep_states must be 0 or 1");
+            c->keep_states = value != 0;
         } else if (k == "keep_factors") {
             if (value < 0) return fail(c, TF_ERR_ARG, "keep_factors must be >= 0");
             c->keep_factors = value;

@@ -127,6 +127,21 @@ def test_adjoint_with_kept_factors_is_bit_identical(adj_setup):
         assert n1 < n0
 
 
+def test_adjoint_with_kept_states_is_bit_identical(adj_setup):
+    """The taped run keeps its nodes' solutions and the sweep reads them
+    instead of recomputing each checkpoint segment: the same gradient, bit
+    for bit, as the sweep that recomputes every state."""
+    s = adj_setup
+    fe = s["fe"]
+    _, g1 = _adjoint(s, 7)
+    fe.set_tuning("keep_states", 0)
+    try:
+        _, g0 = _adjoint(s, 7)
+    finally:
+        fe.set_tuning("keep_states", 1)
+    np.testing.assert_array_equal(g1, g0)
+
+
 def test_adjoint_matches_oracle_finite_differences(adj_setup, tmp_path):
     """Central differences through the oracle (the reference algorithm with
     MKL PARDISO on the CPU) on the same replayed schedule."""

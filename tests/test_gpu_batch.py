@@ -273,7 +273,7 @@ def test_tuning_rejects_bad_values(gpu_mod_big_batch):
     from temf3dt_amd.fetd import FetdError
     fe, _ = gpu_mod_big_batch
     for k, v in [("mt_ks_n", 3), ("mt_chmin", 10), ("mt_units", 1), ("graphs", 2), ("overlap", -1), ("mt_wide", -1),
-                 ("mt_fuse", 3), ("rhs_chain", 2), ("f_band", -1), ("f_band", 1025), ("keep_factors", -1), ("pinv", -1), ("s_tiles", 0), ("no_such_knob", 1)]:
+                 ("mt_fuse", 3), ("rhs_chain", 2), ("f_band", -1), ("f_band", 1025), ("keep_factors", -1), ("keep_states", 2), ("pinv", -1), ("s_tiles", 0), ("no_such_knob", 1)]:
         with pytest.raises(FetdError):
             fe.set_tuning(k, v)
 
