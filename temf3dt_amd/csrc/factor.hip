@@ -388,7 +388,7 @@ __global__ void __launch_bounds__(TF_NT) k_extend_add(double *L, double *U, cons
     const int b = (int)(g - pref[i]);
     const FrontDesc dc = fd[c];
     const FrontDesc dp = fd[parent[c]];
-    const double *uc = U + dc.uoff + (int64_t)b * dc.m;
+    const double *uc = U + dc.uoff + ucol(dc.m, b) - b;  // uc[a], a >= b: row a of column b
     const int *rc = rel + dc.soff;
     double *dst = front_col(L, U, dp, rc[b]);
     int a = b + lane;

@@ -512,7 +512,7 @@ void analyze(Symbolic &s, const Topology &t, int leaf_size) {
     for (int k = 0; k < s.nsn; k++) {
         int64_t nc = s.nc[k], m = s.sptr[k + 1] - s.sptr[k], f = nc + m;
         s.loff[k + 1] = s.loff[k] + front_ld(f) * nc;
-        s.uoff[k + 1] = s.uoff[k] + m * m;
+        s.uoff[k + 1] = s.uoff[k] + m * (m + 1) / 2;  // packed lower triangle
         s.nnzL += nc * (nc + 1) / 2 + m * nc;
         for (int64_t j = 0; j < nc; j++) {
             double c = (double)(f - j - 1);

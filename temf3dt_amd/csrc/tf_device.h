@@ -27,7 +27,8 @@ __host__ __device__ inline int inv_ldt(int nc, int b) { return ((nc - b < b ? nc
 // One multifrontal front (= supernode).  Its columns are the contiguous new
 // indices [c0, c0+nc); its rows are those columns followed by the m struct
 // rows.  L block: f x nc column-major (ld f) at L + loff; update block
-// (Schur complement passed to the parent): m x m column-major at U + uoff.
+// (Schur complement passed to the parent): its lower triangle at U + uoff,
+// column j (struct-relative) holding rows j .. m-1 from offset ucol(m, j).
 struct FrontDesc {
     int64_t loff;  // multiple of 16 doubles (128-byte aligned columns)
     int64_t uoff;
@@ -66,11 +67,16 @@ struct TileUnit {
     int front, t0, k0, k1, slot, pad;
 };
 
+// Offset of column j of an m x m packed lower triangle (rows j .. m-1).
+__host__ __device__ __forceinline__ int64_t ucol(int64_t m, int64_t j) { return j * m - j * (j - 1) / 2; }
+
 // Column c of a front addressed uniformly: columns < nc live in the L block,
-// the rest in the update block.  Element (r, c), r >= c, is colptr(c)[r].
+// the rest in the update block.  Element (r, c), r >= c, is colptr(c)[r];
+// the update block stores no element above its diagonal.
 __device__ __forceinline__ double *front_col(double *L, double *U, const FrontDesc &d, int c) {
-    return (c < d.nc) ? (L + d.loff + (int64_t)c * d.ld)
-                      : (U + d.uoff + (int64_t)(c - d.nc) * d.m - d.nc);
+    if (c < d.nc) return L + d.loff + (int64_t)c * d.ld;
+    const int64_t j = c - d.nc;
+    return U + d.uoff + ucol(d.m, j) - j - d.nc;
 }
 
 // Work segment of a grid launch: fronts[i] is a supernode id and

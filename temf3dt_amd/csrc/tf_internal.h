@@ -42,7 +42,7 @@ struct Symbolic {
     std::vector<int64_t> sptr;      // struct rows beyond own columns
     std::vector<int> sidx;          // new indices, sorted
     std::vector<int64_t> loff;      // L block offset: f x nc column-major, ld = f
-    std::vector<int64_t> uoff;      // U block offset: m x m column-major
+    std::vector<int64_t> uoff;      // U block offset: m x m lower triangle, packed by columns
     std::vector<int64_t> rptr;      // relative indices of struct in parent front
     std::vector<int> rel;
     std::vector<int> children_ptr, children;  // CSR of children per supernode
