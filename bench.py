@@ -285,6 +285,29 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes": nbytes, "copies": 3}))
 
 
+def stream_probe():
+    """What HBM sustains on this device for plain streams (context for the
+    fraction-of-8-TB/s figures): a read-only FP64 sum and a copy over 8 GiB
+    buffers, timed with HIP events over 10 repetitions after a warm-up."""
+    import torch
+    n = (8 << 30) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    out = {}
+    for name, fn, nbytes in (("read_sum", lambda: x.sum(), 8 * n), ("copy", lambda: y.copy_(x), 16 * n)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 10
+        out[name] = {"ms": ms, "GB/s": nbytes / ms / 1e6}
+    print(json.dumps({"stream_probe_bytes": 8 * n, **out}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -300,9 +323,13 @@ def main():
     ap.add_argument("--ckpt", type=int, default=50, help="adjoint checkpoint interval (steps)")
     ap.add_argument("--leaf", type=int, default=0, help="nested-dissection leaf size (0: the library default)")
     ap.add_argument("--copy-probe", action="store_true", help="1 GiB device copies only (counter calibration)")
+    ap.add_argument("--stream-probe", action="store_true", help="sustained HBM read and copy rates only")
     ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
                     help="tf_set_tuning key for the batched run (repeatable), applied before the autotuner")
     a = ap.parse_args()
+    if a.stream_probe:
+        stream_probe()
+        return
     if a.copy_probe:
         copy_probe()
         return
