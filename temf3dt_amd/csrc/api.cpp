@@ -734,9 +734,11 @@ void build_plan(tf_ctx *c) {
             std::vector<int64_t> um;
             const int64_t ntb = (b + TF_UT - 1) / TF_UT;
             for (int k : lev)
-                if (s.nc[k] > b) {
+                if (s.nc[k] > b) {  // ntb x ntb regions per pair; the last pair only those of its J rows
+                    const int64_t npair = (s.nc[k] - b + 2 * b - 1) / (2 * b);
+                    const int64_t jlast = std::min<int64_t>(b, s.nc[k] - (2 * b * (npair - 1) + b));
                     act.push_back(k);
-                    um.push_back((int64_t)((s.nc[k] - b + 2 * b - 1) / (2 * b)) * ntb * ntb);
+                    um.push_back((npair - 1) * ntb * ntb + (jlast + TF_UT - 1) / TF_UT * ntb);
                 }
             if (act.empty()) break;
             add(p.factor, L_IMERGE_T, b, act, um);

@@ -690,18 +690,25 @@ __device__ __forceinline__ void inv_wave(const double *R, int ldr, uint32_t rbyt
 }
 
 // One merge step at block size b; work unit u of a front is (pair, ti, tj)
-// with ntb x ntb output regions of 128 x 128 per pair, ntb = ceil(b / 128),
-// one 64 x 64 quadrant per wave:
+// with ntb x ntb output regions of 128 x 128 per pair, ntb = ceil(b / 128)
+// (the last pair only the row regions of its J rows), one 64 x 64 quadrant
+// per wave:
 //   step 0: T    = L_JI X_II   (k >= c: X_II(k, c) = 0 above its diagonal)
 //   step 1: X_JI = -X_JJ T     (k <= r: X_JJ(r, k) = 0 above its diagonal)
 template <int STEP>
 __global__ void __launch_bounds__(TF_NT, 2) k_inv_merge(double *L, const FrontDesc *fd, const int *list,
                                                         const int64_t *pref, int cnt, int b, double *W,
                                                         const int64_t *woff) {
-    const int i = seg_find(pref, cnt, blockIdx.x);
+    // Step 0 runs consecutive work units on one XCD (each XCD then takes whole
+    // row regions, whose tiles share L_JI in its L2 and mix long and short
+    // inner ranges, k >= c); step 1 keeps the round-robin order, since its
+    // inner range grows with the row region and whole row regions per XCD
+    // would leave the last XCDs the longest tiles.
+    const int64_t gb = STEP == 0 ? (int64_t)xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+    const int i = seg_find(pref, cnt, gb);
     const int sn = list[i];
     const FrontDesc d = fd[sn];
-    const int64_t u = (int64_t)blockIdx.x - pref[i];
+    const int64_t u = gb - pref[i];
     const int ntb = (b + TF_UT - 1) / TF_UT;
     const int pair = (int)(u / (ntb * ntb)), t = (int)(u % (ntb * ntb));
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
