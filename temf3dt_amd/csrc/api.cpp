@@ -1226,8 +1226,22 @@ void keep_slot(tf_ctx *c, int slot) {
     const size_t need = sizeof(double) * ((size_t)c->sym.ltotal + TF_READ_SLACK + (size_t)c->sym.n +
                                           (c->d_LT[slot] ? (size_t)c->lt_total : 0));
     if (fr < need + tot / 10) return;
+    // the slot's new buffers first, without throwing: if they cannot be had,
+    // nothing is kept and the slot is overwritten as usual
+    void *L = nullptr, *D = nullptr, *LT = nullptr;
+    const bool ok = hipMalloc(&L, sizeof(double) * ((size_t)c->sym.ltotal + TF_READ_SLACK)) == hipSuccess &&
+                    hipMalloc(&D, sizeof(double) * (size_t)c->sym.n) == hipSuccess &&
+                    (!c->d_LT[slot] || hipMalloc(&LT, sizeof(double) * (size_t)c->lt_total) == hipSuccess);
+    if (!ok) {
+        (void)hipGetLastError();
+        if (L) (void)hipFree(L);
+        if (D) (void)hipFree(D);
+        return;
+    }
     c->kept.push_back(tf_ctx::KeptFactor{c->slot_dt[slot], c->d_L[slot], c->d_D[slot], c->d_LT[slot]});
-    c->d_L[slot] = c->d_D[slot] = c->d_LT[slot] = nullptr;
+    c->d_L[slot] = (double *)L;
+    c->d_D[slot] = (double *)D;
+    c->d_LT[slot] = (double *)LT;
 }
 
 void factor_dt(tf_ctx *c, int slot, double dt) {
@@ -1582,7 +1596,12 @@ void keep_state(tf_ctx *c, int k, const double *x) {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) return;
         if (c->tstate_bytes + bytes > tot / 4 || fr < bytes + tot / 5) return;
-        c->tstate_chunks.push_back(dalloc<double>(len * TF_STATE_CHUNK));
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {  // not kept: the sweep recomputes this state
+            (void)hipGetLastError();
+            return;
+        }
+        c->tstate_chunks.push_back((double *)p);
         c->tstate_used = 0;
         c->tstate_bytes += bytes;
     }
