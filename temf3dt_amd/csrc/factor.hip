@@ -511,7 +511,7 @@ __device__ __forceinline__ void trsm_wave(double *L, const FrontDesc &d, int p0,
 }
 
 // one workgroup of two waves per 128 rows below the panel
-__global__ void __launch_bounds__(128) k_factor_trsm(double *L, const FrontDesc *fd, const int *list,
+__global__ void __launch_bounds__(128, 2) k_factor_trsm(double *L, const FrontDesc *fd, const int *list,
                                                      const int64_t *pref, int cnt, int p0, const double *Dn,
                                                      const double *W, const int64_t *woff) {
     const int i = seg_find(pref, cnt, blockIdx.x);
