@@ -226,11 +226,16 @@ int tf_run(tf_ctx *ctx, int max_steps, int max_rows, int *nrows, int *row_step, 
  *   J = sum over rows, sources, receivers, components of w * fields
  * (w laid out like tf_run's fields: [nrows][src][rec][9], nrows = the rows of
  * the taped run), grad_logsigma[nT] = dJ/d(log sigma_t) per tet: a
- * backward-in-time sweep that recomputes each checkpoint segment, solves the
- * adjoint systems with the same LDL^T factors (P is symmetric) and reduces
- * -lambda^T M1_t (3 y - 4 u + v) per element.  The controller's keep/change
- * decisions are those of the taped run (differentiate with them frozen; see
- * tf_set_replay).  Step-on and impulse sources (pulse types 1, 2). */
+ * backward-in-time sweep that takes each node's forward state (kept by the
+ * taped run, or recomputed from its checkpoint segment), solves the adjoint
+ * systems with the same LDL^T factors (P is symmetric; kept by the run or
+ * factored again) and reduces -lambda^T M1_t (3 y - 4 u + v) per element.
+ * What the run keeps (tuning keys "keep_states", "keep_factors") stays on
+ * the device until the next tf_run, tf_set_checkpointing(ctx, 0) or
+ * tf_destroy, so tf_adjoint can be called again with other weights.  The
+ * controller's keep/change decisions are those of the taped run
+ * (differentiate with them frozen; see tf_set_replay).  Step-on and impulse
+ * sources (pulse types 1, 2). */
 int tf_set_checkpointing(tf_ctx *ctx, int every);
 int tf_adjoint(tf_ctx *ctx, int nrows, const double *w, double *grad_logsigma);
 /* Independent models in one context (no reference counterpart: the

@@ -1792,7 +1792,8 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
     // factors from before this run may be of other materials: none is reused
     c->slot_live[0] = c->slot_live[1] = false;
     c->slot_taped[0] = c->slot_taped[1] = false;
-    drop_kept(c);
+    drop_kept(c);    // what the last taped run kept is of no use once this run starts
+    drop_states(c);
     // adjoint tape: node ids of the history states e2, e3 and of the ring
     const bool tape = c->ckpt_every > 0 && inout == 0;
     c->taping = tape && c->keep_factors > 0;
@@ -1805,7 +1806,6 @@ void run_loop(tf_ctx *c, int inout, double *res, int max_steps) {
     if (tape) {
         for (auto &k : c->ckpts) { dfree(k.s2); dfree(k.s3); }
         c->ckpts.clear();
-        drop_states(c);
         c->tape.assign(1, tf_ctx::TNode{-1, -1, 0, 0, 0.0, 0.0});  // node 0: the zero initial state
         c->row_node.assign(1, 0);
     }
