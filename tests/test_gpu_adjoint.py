@@ -113,18 +113,21 @@ def test_adjoint_with_kept_factors_is_bit_identical(adj_setup):
     fe = s["fe"]
     rows, g1 = _adjoint(s, 7)
     n1 = fe.timers()["nfactor"]
-    fe.set_tuning("keep_factors", 0)
-    try:
-        _, g0 = _adjoint(s, 7)
-        n0 = fe.timers()["nfactor"]
-    finally:
-        fe.set_tuning("keep_factors", 8)
+    counts = {}
+    for cap in (1, 0):  # at most one kept (the sweep mixes kept and refactored dt), none kept
+        fe.set_tuning("keep_factors", cap)
+        try:
+            _, g0 = _adjoint(s, 7)
+            counts[cap] = fe.timers()["nfactor"]
+        finally:
+            fe.set_tuning("keep_factors", 8)
+        np.testing.assert_array_equal(g1, g0)
     ndt = len(np.unique(rows["dt"]))
-    print(f"[gpu] kept factors: {ndt} dt in the run, factorisations {n1:.0f} (kept) vs {n0:.0f} (none kept)",
-          flush=True)
-    np.testing.assert_array_equal(g1, g0)
+    print(f"[gpu] kept factors: {ndt} dt in the run, factorisations {n1:.0f} (up to 8 kept), "
+          f"{counts[1]:.0f} (1 kept), {counts[0]:.0f} (none kept)", flush=True)
+    assert n1 <= counts[1] <= counts[0]
     if ndt > 2:
-        assert n1 < n0
+        assert n1 < counts[0]
 
 
 def test_adjoint_with_kept_states_is_bit_identical(adj_setup):
