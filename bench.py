@@ -1,6 +1,6 @@
 """Benchmark: FP64 source-timesteps/s of the FETD forward path on one GPU.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--model batched3d|plain3d|small]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--model batched3d|batched3d_refined|plain3d|small]
 
 A "step" is one BDF2 time step of the reference's adaptive loop
 (src/FETD_FWD.f90:1281-1443) for every source: right-hand side (SpMV with
@@ -16,7 +16,9 @@ Default workload (batched3d): 64 transmitters (20 m loops on an 8 x 8 grid,
 100 km cube), ~1.35M tets / ~1.6M edges, one receiver per transmitter.
 `value` is the batched rate (64 sources per block solve, one factorisation
 per dt); the plain single-source path is timed on the same mesh over the
-same steps and reported beside it.
+same steps and reported beside it.  batched3d_refined runs the same
+transmitters and earth on a mesh bisected to 1.25 m cells at every wire
+(1.69M tets).
 
 Prints one JSON line (with roofline and cpu_baseline objects).
 """
@@ -35,8 +37,9 @@ FP64_PEAK_TFLOPS = 78.6
 
 def build_model(name):
     from temf3dt_amd import workloads
-    return {"batched3d": workloads.batched_3d_model, "plain3d": workloads.plain_3d_model,
-            "small": workloads.small_model, "multismall": workloads.multi_source_small_model}[name]()
+    return {"batched3d": workloads.batched_3d_model, "batched3d_refined": workloads.batched_3d_refined_model,
+            "plain3d": workloads.plain_3d_model, "small": workloads.small_model,
+            "multismall": workloads.multi_source_small_model}[name]()
 
 
 def setup_gpu(m, loops, leaf=0):
@@ -234,7 +237,7 @@ def uneven_bench(steps, n=48):
     print(json.dumps(res))
 
 
-def single_source_rooflines(prof, nsolves, nfact, st, n, nnzu, lay):
+def single_source_rooflines(prof, nsolves, nfact, st, n, nnzu, lay, counted=True):
     """Roofline entries of a single-source context's profiled run: the step
     (HBM) and the numeric factorisation (FP64 MFMA).
     Step: bytes_1 = 16 nnzL_off (L read in both sweeps) + 8 n (D) + 8 sum of
@@ -243,7 +246,8 @@ def single_source_rooflines(prof, nsolves, nfact, st, n, nnzu, lay):
     summed duration of its SpMV/right-hand-side and solve kernels per solve.
     Factorisation: F_fact = sum_j (c_j^2 + 2 c_j) flops over the summed
     duration of all factorisation kernels (update, panel, inversion,
-    extend-add, scatter) per factorisation."""
+    extend-add, scatter) per factorisation.  The counter-measured traffic
+    (profiles/) is that of the default workload: counted=False for others."""
     out = {}
     nnzl_off = st["nnzL"] - n
     bytes1 = 16.0 * nnzl_off + 8.0 * n + 8.0 * lay["struct_rows"] + 12.0 * nnzu + 4.0 * (n + 1) + 48.0 * n
@@ -251,7 +255,7 @@ def single_source_rooflines(prof, nsolves, nfact, st, n, nnzu, lay):
     ach = bytes1 / (step_ms * 1e-3) / 1e9
     out["plain_step"] = {
         "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-        "traffic": measured_traffic("hbm_traffic_plain_step.txt", "traffic_bytes_per_step"),
+        "traffic": measured_traffic("hbm_traffic_plain_step.txt", "traffic_bytes_per_step") if counted else None,
         "kernel": (f"single right-hand-side step (k_spmv_rhs + forward/backward solve kernels): bytes_1 = "
                    f"{bytes1:.4e} B per step, {step_ms:.3f} ms per step ({nsolves} solves; HIP events)")}
     fact_ms = (prof["factor_update"][1] + prof["factor_other"][1]) / max(nfact, 1)
@@ -259,7 +263,8 @@ def single_source_rooflines(prof, nsolves, nfact, st, n, nnzu, lay):
     out["factorisation"] = {
         "bound": "mfma", "achieved": fach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
         "frac": fach / FP64_PEAK_TFLOPS,
-        "traffic": measured_traffic("hbm_traffic_factor.txt", "traffic_bytes_per_factorisation"),
+        "traffic": (measured_traffic("hbm_traffic_factor.txt", "traffic_bytes_per_factorisation")
+                    if counted else None),
         "kernel": (f"multifrontal LDL^T (k_factor_* + inversion + extend-add): F_fact = {st['flops']:.4e} flops "
                    f"per factorisation, {fact_ms:.1f} ms each ({nfact} factorisations; HIP events)")}
     return out
@@ -402,7 +407,8 @@ def main():
                        f"{tile_ms:.1f} ms (HIP events; breakdown run with the factorisations at the checks)"),
             "step_breakdown_ms": {k: round(v[1], 2) for k, v in prof.items()}}
     if nsrc == 1:
-        rooflines.update(single_source_rooflines(prof, nsolves, int(tm_prof["nfactor"]), st, n, nnzu, lay))
+        rooflines.update(single_source_rooflines(prof, nsolves, int(tm_prof["nfactor"]), st, n, nnzu, lay,
+                                                 a.model == "batched3d"))
     plain = None
     adjoint = None
     if not a.no_plain:
@@ -428,7 +434,8 @@ def main():
         fp.set_tuning("overlap", 1)
         print("[bench] plain kernel time by category (launches, ms): " +
               ", ".join(f"{k} {v[0]} {v[1]:.1f}" for k, v in pprof.items()), file=sys.stderr, flush=True)
-        rooflines.update(single_source_rooflines(pprof, psteps + 2 * pchk, int(pptm["nfactor"]), st, n, nnzu, lay))
+        rooflines.update(single_source_rooflines(pprof, psteps + 2 * pchk, int(pptm["nfactor"]), st, n, nnzu, lay,
+                                                 a.model == "batched3d"))
         if not a.no_adjoint:
             adjoint = adjoint_ratio(fp, a.steps, pwall, a.ckpt)
             print(f"[bench] adjoint: taped forward {adjoint['taped_forward_s']:.3f} s + backward "

@@ -83,6 +83,18 @@ def build(xs, ys, zs, loops, receivers, blocks=(), host_sigma=0.01, air_sigma=1e
     """Tensor grid with an air half-space (z<0), an earth half-space and
     rectangular conductive blocks [(x0,x1,y0,y1,z0,z1,sigma), ...]."""
     xyz, tets, hexid = meshgen.tensor_mesh(xs, ys, zs)
+    phy, domains = _materials(xs, ys, zs, hexid, blocks, host_sigma, air_sigma)
+    # loop vertex chains in grid-node ids
+    chains = []
+    for (x0, x1, y0, y1, z) in loops:
+        segs = meshgen.square_loop(xs, ys, zs, x0, x1, y0, y1, z)
+        chains.append(segs[:, 0].copy())  # path order
+    return _loops_first(xyz, tets, phy, chains, domains, receivers, kw)
+
+
+def _materials(xs, ys, zs, hexid, blocks, host_sigma, air_sigma):
+    """Physical group of every tet from the centre of its grid hexahedron
+    (hexid [T,3]): air above z = 0, earth below, blocks by containment."""
     cx = 0.5 * (xs[hexid[:, 0]] + xs[hexid[:, 0] + 1])
     cy = 0.5 * (ys[hexid[:, 1]] + ys[hexid[:, 1] + 1])
     cz = 0.5 * (zs[hexid[:, 2]] + zs[hexid[:, 2] + 1])
@@ -92,12 +104,77 @@ def build(xs, ys, zs, loops, receivers, blocks=(), host_sigma=0.01, air_sigma=1e
         m = (cx > x0) & (cx < x1) & (cy > y0) & (cy < y1) & (cz > z0) & (cz < z1)
         phy[m] = 13 + bi
         domains.append((13 + bi, s, 1.0, 1.0))
-    # loop vertex chains in grid-node ids
+    return phy, domains
+
+
+def near_wire(loops, r):
+    """Predicate (x, y, z) -> True within distance r of the wire of one of
+    the square loops [(x0, x1, y0, y1, z)]; loops are looked up in 16 m
+    buckets of the horizontal plane."""
+    B = 16.0
+    buckets = {}
+    for lp in loops:
+        x0, x1, y0, y1, _ = lp
+        for bx in range(math.floor((x0 - r) / B), math.floor((x1 + r) / B) + 1):
+            for by in range(math.floor((y0 - r) / B), math.floor((y1 + r) / B) + 1):
+                buckets.setdefault((bx, by), []).append(lp)
+    r2 = r * r
+
+    def near(x, y, z):
+        for x0, x1, y0, y1, zl in buckets.get((math.floor(x / B), math.floor(y / B)), ()):
+            dz = z - zl
+            dx = max(x0 - x, x - x1, 0.0)
+            dy = max(y0 - y, y - y1, 0.0)
+            if dx > 0.0 or dy > 0.0:
+                dh2 = dx * dx + dy * dy
+            else:  # inside the square: distance to its nearest side
+                dh = min(x - x0, x1 - x, y - y0, y1 - y)
+                dh2 = dh * dh
+            if dh2 + dz * dz <= r2:
+                return True
+        return False
+    return near
+
+
+def build_refined(xs, ys, zs, loops, receivers, blocks=(), host_sigma=0.01, air_sigma=1e-12, h_wire=2.2,
+                  r_wire=0.5, margin=3, **kw):
+    """build()'s model with the mesh refined at the transmitters: the Kuhn
+    tets of the hexahedra within `margin` grid cells of the loops' extent
+    are bisected (newest-vertex bisection, which keeps the mesh conforming)
+    until no tet with a vertex or its centroid within r_wire of a wire has an
+    edge longer than h_wire; the refinement grades back to the grid's cells
+    through a few tets.  Materials are those of build() (by the grid
+    hexahedron containing each tet); every loop follows its cut grid edges
+    as a chain of shorter wire segments."""
+    from . import refine
+    near = near_wire(loops, r_wire)
+
+    def want(pts, le):
+        return le > h_wire and any(near(*q) for q in pts)
+
+    def span(a, v0, v1):
+        return (max(int(np.searchsorted(a, v0)) - margin, 0), min(int(np.searchsorted(a, v1)) + margin, len(a) - 1))
+
+    box = (span(xs, min(lp[0] for lp in loops), max(lp[1] for lp in loops)),
+           span(ys, min(lp[2] for lp in loops), max(lp[3] for lp in loops)),
+           span(zs, min(lp[4] for lp in loops), max(lp[4] for lp in loops)))
+    xyz, tets, r, _ = refine.refined_tensor_mesh(xs, ys, zs, box, want)
+    cen = xyz[tets].mean(axis=1)
+    hexid = np.stack([np.searchsorted(a, cen[:, d], side="right") - 1 for d, a in enumerate((xs, ys, zs))], axis=1)
+    phy, domains = _materials(xs, ys, zs, hexid, blocks, host_sigma, air_sigma)
     chains = []
     for (x0, x1, y0, y1, z) in loops:
-        segs = meshgen.square_loop(xs, ys, zs, x0, x1, y0, y1, z)
-        chains.append(segs[:, 0].copy())  # path order
-    # renumber: loop vertices first, in path order
+        path = meshgen.square_loop(xs, ys, zs, x0, x1, y0, y1, z)[:, 0] - 1
+        ch = []
+        for q in range(len(path)):
+            ch += r.expand_edge(int(path[q]), int(path[(q + 1) % len(path)]))[:-1]
+        chains.append(np.array(ch, dtype=np.int64) + 1)
+    return _loops_first(xyz, tets + 1, phy, chains, domains, receivers, kw)
+
+
+def _loops_first(xyz, tets, phy, chains, domains, receivers, kw):
+    """The model with its vertices renumbered so that every loop's vertices
+    come first, in path order (tets and chains 1-based)."""
     N = len(xyz)
     newid = np.zeros(N + 1, np.int64)
     nxt = 1
@@ -198,6 +275,37 @@ def batched_3d_model(nside=8):
     return build(xs, ys, zs, loops=loops, receivers=rec, blocks=[(50, 250, -100, 100, 100, 200, 1.0)],
                  host_sigma=0.01, air_sigma=1e-12, source=(1, 1, 1.0e-7, 1.0), sp_division=10,
                  time_maximum=1.0e-3, sp_double=100, sp_dblesize=2, time_points=gates, rec_charac=20.0)
+
+
+def batched_3d_refined_model(nside=8, r_wire=0.5):
+    """batched_3d_model's transmitters and earth on a locally refined mesh:
+    the Kuhn tets of its 10 m grid that touch a wire (within r_wire) are
+    bisected down to 1.25 m cubes (longest edge 2.17 m), grading back to
+    10 m within a few cells; each 20 m loop becomes 64 wire segments of
+    1.25 m.  1,690,764 tets at nside 8."""
+    cs = [-25.0 * (nside - 1) + 50.0 * i for i in range(nside)]
+    lo, hi = cs[0] - 20.0, cs[-1] + 20.0
+    xs = axis(-50000, 50000, lo, hi, 10.0, 1.75)
+    ys = axis(-50000, 50000, lo, hi, 10.0, 1.75)
+    zs = axis(-50000, 50000, -20, 200, 10.0, 1.75)
+    loops = [(cx - 10, cx + 10, cy - 10, cy + 10, 0.0) for cy in cs for cx in cs]
+    rec = [[cx, cy, -1.0] for cy in cs for cx in cs]
+    gates = list(np.logspace(-6, -3, 40))
+    return build_refined(xs, ys, zs, loops=loops, receivers=rec, blocks=[(50, 250, -100, 100, 100, 200, 1.0)],
+                         host_sigma=0.01, air_sigma=1e-12, h_wire=2.2, r_wire=r_wire, source=(1, 1, 1.0e-7, 1.0),
+                         sp_division=10, time_maximum=1.0e-3, sp_double=100, sp_dblesize=2, time_points=gates,
+                         rec_charac=20.0)
+
+
+def refined_small_model():
+    """small_model's earth and 100 m loop with the tets at the wire bisected
+    to 12.5 m cubes (tests)."""
+    xs = axis(-2000, 2000, -100, 100, 50.0, 2.0)
+    ys = axis(-2000, 2000, -100, 100, 50.0, 2.0)
+    zs = axis(-2000, 2000, -50, 100, 50.0, 2.0)
+    return build_refined(xs, ys, zs, loops=[(-50, 50, -50, 50, 0.0)], receivers=[[0.0, 0.0, -3.0], [10.0, 5.0, 7.0]],
+                         host_sigma=0.01, air_sigma=1e-8, h_wire=22.0, r_wire=2.0, source=(1, 1, 1.0e-6, 1.0),
+                         sp_division=10, time_maximum=1e-3, sp_double=12, sp_dblesize=2, rec_charac=100.0)
 
 
 def uneven_model(target_tets, host_sigma, block, seed_name="", time_maximum=1e-2):

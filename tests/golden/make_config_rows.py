@@ -15,12 +15,16 @@ refactorisation -- and the rows, times and check decisions are stored:
                              replays it), 110 steps
   uneven48/oracle_rows.npz   three of the 48 uneven models (seed 3): the
                              smallest, the median and the largest, 110 steps
+  batched3d_refined/oracle_rows.npz
+                             transmitters 0 and 45 of the 64-transmitter
+                             model on its locally refined mesh (1,690,764
+                             tets, 1.25 m cells at the wires), as batched3d
 
 The models come from temf3dt_amd/workloads.py (deterministic), so the GPU
 tests rebuild the same meshes.  Sizes and the mesh fingerprint (tet count,
 edge count, coordinate sum) are stored to catch a drifting generator.
 
-Usage: python tests/golden/make_config_rows.py [plain3d] [batched3d] [uneven48]
+Usage: python tests/golden/make_config_rows.py [plain3d] [batched3d] [uneven48] [batched3d_refined]
 (about 15 minutes on 8 cores; 62 GB of memory is enough).
 """
 import os
@@ -75,9 +79,9 @@ def plain3d():
          chk_step=r["chk_step"], chk_err=r["chk_err"], chk_keep=r["chk_keep"], factor_nnz=r["factor_nnz"])
 
 
-def batched3d():
-    m = workloads.batched_3d_model()
-    print(f"batched3d: {m.ntet} tets, transmitters {BATCH_TX}", flush=True)
+def batched3d(name="batched3d", make=workloads.batched_3d_model):
+    m = make()
+    print(f"{name}: {m.ntet} tets, transmitters {BATCH_TX}", flush=True)
     out = {"fp": fingerprint(m), "tx": np.array(BATCH_TX)}
     keep = None
     for i, tx in enumerate(BATCH_TX):
@@ -90,7 +94,11 @@ def batched3d():
         assert np.array_equal(r["t"], out["t"])
         out[f"fields_{i}"] = r["fields"]
         out["nE"] = nE
-    save("batched3d", **out)
+    save(name, **out)
+
+
+def batched3d_refined():
+    batched3d("batched3d_refined", workloads.batched_3d_refined_model)
 
 
 def uneven48():
@@ -111,9 +119,9 @@ def uneven48():
 
 def main():
     pyoracle.set_threads(os.cpu_count() or 8)
-    which = sys.argv[1:] or ["plain3d", "batched3d", "uneven48"]
+    which = sys.argv[1:] or ["plain3d", "batched3d", "uneven48", "batched3d_refined"]
     for w in which:
-        {"plain3d": plain3d, "batched3d": batched3d, "uneven48": uneven48}[w]()
+        {"plain3d": plain3d, "batched3d": batched3d, "uneven48": uneven48, "batched3d_refined": batched3d_refined}[w]()
 
 
 if __name__ == "__main__":

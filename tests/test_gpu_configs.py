@@ -9,6 +9,8 @@ the first doubling check at step 100 with its refactorisation.
   batched 3-D      1,346,700 tets, 64 transmitters in one block solve;
                    transmitters 0 and 45 against single-source oracle runs on
                    the same (replayed) schedule
+  batched 3-D,     the same transmitters and earth on the mesh bisected to
+  refined          1.25 m cells at every wire (1,690,764 tets), likewise
   uneven models    the 48 seeded models in one context; the smallest, the
                    median and the largest against their oracle runs
   adjoint          the plain 3-D mesh, 160 steps with checkpoints every 50
@@ -76,12 +78,14 @@ def test_cfg_plain3d_matches_oracle():
     _compare_dbdt(rows["fields"], g["fields"], "plain 3-D, 41 receivers")  # [row][receiver][9]
 
 
-def test_cfg_batched3d_transmitters_match_oracle():
+@pytest.mark.parametrize("name", ["batched3d", "batched3d_refined"])
+def test_cfg_batched3d_transmitters_match_oracle(name):
     from temf3dt_amd import workloads
-    m = workloads.batched_3d_model()
-    g = _golden("batched3d")
+    print(f"[gpu] {name}: building the mesh", flush=True)
+    m = {"batched3d": workloads.batched_3d_model, "batched3d_refined": workloads.batched_3d_refined_model}[name]()
+    g = _golden(name)
     np.testing.assert_allclose(_fingerprint(m), g["fp"], rtol=1e-12)
-    print(f"[gpu] batched 3-D: {m.ntet} tets, {len(m.loops)} transmitters, {STEPS} steps", flush=True)
+    print(f"[gpu] {name}: {m.ntet} tets, {len(m.loops)} transmitters, {STEPS} steps", flush=True)
     fe = gpu_from_model(m)
     assert fe.nE == int(g["nE"])
     fe.set_replay(list(g["chk_keep"]))
@@ -93,8 +97,8 @@ def test_cfg_batched3d_transmitters_match_oracle():
     np.testing.assert_array_equal(rows["t"], g["t"])
     for i, tx in enumerate(g["tx"]):
         ref = g[f"fields_{i}"]  # [row][receiver][9], transmitter tx alone
-        _compare_dbdt(rows["fields"][:, tx], ref, f"batched 3-D transmitter {tx}, all 64 receivers")
-        _compare_dbdt(rows["fields"][:, tx, tx], ref[:, tx], f"batched 3-D transmitter {tx}, coincident receiver")
+        _compare_dbdt(rows["fields"][:, tx], ref, f"{name} transmitter {tx}, all 64 receivers")
+        _compare_dbdt(rows["fields"][:, tx, tx], ref[:, tx], f"{name} transmitter {tx}, coincident receiver")
 
 
 def test_cfg_uneven48_models_match_oracle():
