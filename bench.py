@@ -11,14 +11,14 @@ The timed region is K steps of the loop from t = 0, including every
 factorisation that falls inside it; mesh, assembly and symbolic analysis
 are set up before it (inputs resident in HBM).
 
-Default workload (batched3d): 64 transmitters (20 m loops on an 8 x 8 grid,
-50 m apart) over the 3-D earth (0.01 S/m host, 1 S/m block, 1e-12 S/m air,
-100 km cube), ~1.35M tets / ~1.6M edges, one receiver per transmitter.
-`value` is the batched rate (64 sources per block solve, one factorisation
-per dt); the plain single-source path is timed on the same mesh over the
-same steps and reported beside it.  batched3d_refined runs the same
-transmitters and earth on a mesh bisected to 1.25 m cells at every wire
-(1.69M tets).
+Default workload (batched3d_refined): 64 transmitters (20 m loops on an
+8 x 8 grid, 50 m apart) over the 3-D earth (0.01 S/m host, 1 S/m block,
+1e-12 S/m air, 100 km cube), one receiver per transmitter, on a 10 m grid
+whose tets are bisected to 1.25 m cells at every wire: 1,690,764 tets /
+2,009,893 edges.  batched3d is the same on the unrefined 10 m grid
+(1,346,700 tets).  `value` is the batched rate (64 sources per block solve,
+one factorisation per dt); the plain single-source path is timed on the
+same mesh over the same steps and reported beside it.
 
 Prints one JSON line (with roofline and cpu_baseline objects).
 """
@@ -237,7 +237,15 @@ def uneven_bench(steps, n=48):
     print(json.dumps(res))
 
 
-def single_source_rooflines(prof, nsolves, nfact, st, n, nnzu, lay, counted=True):
+# counter-measured HBM traffic (profiles/) per workload: tile launch, plain step, factorisation
+TRAFFIC_FILES = {
+    "batched3d": ("hbm_traffic_batched3d.txt", "hbm_traffic_plain_step.txt", "hbm_traffic_factor.txt"),
+    "batched3d_refined": ("hbm_traffic_batched3d_refined.txt", "hbm_traffic_plain_step_refined.txt",
+                          "hbm_traffic_factor_refined.txt"),
+}
+
+
+def single_source_rooflines(prof, nsolves, nfact, st, n, nnzu, lay, model):
     """Roofline entries of a single-source context's profiled run: the step
     (HBM) and the numeric factorisation (FP64 MFMA).
     Step: bytes_1 = 16 nnzL_off (L read in both sweeps) + 8 n (D) + 8 sum of
@@ -246,8 +254,9 @@ def single_source_rooflines(prof, nsolves, nfact, st, n, nnzu, lay, counted=True
     summed duration of its SpMV/right-hand-side and solve kernels per solve.
     Factorisation: F_fact = sum_j (c_j^2 + 2 c_j) flops over the summed
     duration of all factorisation kernels (update, panel, inversion,
-    extend-add, scatter) per factorisation.  The counter-measured traffic
-    (profiles/) is that of the default workload: counted=False for others."""
+    extend-add, scatter) per factorisation.  Traffic: the counter-measured
+    bytes of this workload (TRAFFIC_FILES), None for workloads without."""
+    files = TRAFFIC_FILES.get(model)
     out = {}
     nnzl_off = st["nnzL"] - n
     bytes1 = 16.0 * nnzl_off + 8.0 * n + 8.0 * lay["struct_rows"] + 12.0 * nnzu + 4.0 * (n + 1) + 48.0 * n
@@ -255,7 +264,7 @@ def single_source_rooflines(prof, nsolves, nfact, st, n, nnzu, lay, counted=True
     ach = bytes1 / (step_ms * 1e-3) / 1e9
     out["plain_step"] = {
         "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-        "traffic": measured_traffic("hbm_traffic_plain_step.txt", "traffic_bytes_per_step") if counted else None,
+        "traffic": measured_traffic(files[1], "traffic_bytes_per_step") if files else None,
         "kernel": (f"single right-hand-side step (k_spmv_rhs + forward/backward solve kernels): bytes_1 = "
                    f"{bytes1:.4e} B per step, {step_ms:.3f} ms per step ({nsolves} solves; HIP events)")}
     fact_ms = (prof["factor_update"][1] + prof["factor_other"][1]) / max(nfact, 1)
@@ -263,8 +272,7 @@ def single_source_rooflines(prof, nsolves, nfact, st, n, nnzu, lay, counted=True
     out["factorisation"] = {
         "bound": "mfma", "achieved": fach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
         "frac": fach / FP64_PEAK_TFLOPS,
-        "traffic": (measured_traffic("hbm_traffic_factor.txt", "traffic_bytes_per_factorisation")
-                    if counted else None),
+        "traffic": measured_traffic(files[2], "traffic_bytes_per_factorisation") if files else None,
         "kernel": (f"multifrontal LDL^T (k_factor_* + inversion + extend-add): F_fact = {st['flops']:.4e} flops "
                    f"per factorisation, {fact_ms:.1f} ms each ({nfact} factorisations; HIP events)")}
     return out
@@ -318,7 +326,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--model", default="batched3d")
+    ap.add_argument("--model", default="batched3d_refined")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sources", type=int, default=0,
                     help="use only the first N transmitters (0: all); 1 makes the main run the plain path")
@@ -396,19 +404,19 @@ def main():
         flops_solve = 4.0 * nsrc * nnzl_off
         tile_launches, tile_ms = prof["tile"]
         achieved = flops_solve * nsolves / (tile_ms * 1e-3) / 1e12
-        traffic = measured_traffic("hbm_traffic_batched3d.txt") if a.model == "batched3d" else None
+        files = TRAFFIC_FILES.get(a.model)
+        traffic = measured_traffic(files[0]) if files else None
         rooflines["block_solve_tiles"] = {
             "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
             "traffic_note": ("HBM bytes per k_mtile launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on a "
-                             "1 GiB copy), profiles/hbm_traffic_batched3d.txt") if traffic else None,
+                             f"1 GiB copy), profiles/{files[0]}") if traffic else None,
             "kernel": (f"k_mtile (block-solve tile products, {nsrc} right-hand sides): {nsolves} solves x "
                        f"4*K*nnzL_off = {flops_solve:.3e} flops each, {tile_launches} launches, "
                        f"{tile_ms:.1f} ms (HIP events; breakdown run with the factorisations at the checks)"),
             "step_breakdown_ms": {k: round(v[1], 2) for k, v in prof.items()}}
     if nsrc == 1:
-        rooflines.update(single_source_rooflines(prof, nsolves, int(tm_prof["nfactor"]), st, n, nnzu, lay,
-                                                 a.model == "batched3d"))
+        rooflines.update(single_source_rooflines(prof, nsolves, int(tm_prof["nfactor"]), st, n, nnzu, lay, a.model))
     plain = None
     adjoint = None
     if not a.no_plain:
@@ -434,8 +442,8 @@ def main():
         fp.set_tuning("overlap", 1)
         print("[bench] plain kernel time by category (launches, ms): " +
               ", ".join(f"{k} {v[0]} {v[1]:.1f}" for k, v in pprof.items()), file=sys.stderr, flush=True)
-        rooflines.update(single_source_rooflines(pprof, psteps + 2 * pchk, int(pptm["nfactor"]), st, n, nnzu, lay,
-                                                 a.model == "batched3d"))
+        rooflines.update(single_source_rooflines(pprof, psteps + 2 * pchk, int(pptm["nfactor"]), st, n, nnzu,
+                                                 lay, a.model))
         if not a.no_adjoint:
             adjoint = adjoint_ratio(fp, a.steps, pwall, a.ckpt)
             print(f"[bench] adjoint: taped forward {adjoint['taped_forward_s']:.3f} s + backward "
